@@ -1,0 +1,183 @@
+/* woo_amd — MI355X (gfx950) DEM step engine, C ABI.
+ *
+ * This is the drop-in boundary: each entry point is what one of Woo's
+ * Engine::run() implementations (or a ParticleContainer / DemData accessor)
+ * calls in place of its OpenMP C++ body. Plain pointers and sizes only; every
+ * function returns 0 on success and a negative WDG_E_* code on failure, with
+ * a message in wdg_last_error() (thread-local). No exception crosses the ABI.
+ *
+ * Reference interfaces replaced (woodem/woo tree):
+ *   wdg_collide        InsertionSortCollider::run        src/dem/InsertionSortCollider.cpp:386-613
+ *   wdg_contact_loop   ContactLoop::run                  src/dem/ContactLoop.cpp:93-268
+ *                      (Cg2_Sphere_Sphere_L6Geom        src/dem/Sphere.cpp:92-117,
+ *                       Cg2_Wall_Sphere_L6Geom          src/dem/Wall.cpp:50-84,
+ *                       Cp2_FrictMat_FrictPhys          src/dem/FrictMat.cpp:20-44,
+ *                       Cp2_HertzMat_HertzPhys          src/dem/Hertz.cpp:34-77,
+ *                       Law2_L6Geom_FrictPhys_IdealElPl src/dem/IdealElPl.cpp:15-125,
+ *                       Law2_L6Geom_HertzPhys_DMT       src/dem/Hertz.cpp:85-210)
+ *   wdg_leapfrog       Leapfrog::run                     src/dem/Leapfrog.cpp:104-255
+ *   wdg_force_reset    ForceResetter::run                src/dem/Leapfrog.cpp:15-26
+ *   wdg_crit_dt        DynDt::critDt_stiffness           src/dem/DynDt.cpp:61-71
+ *   wdg_upload_particles / wdg_upload / wdg_download
+ *                      ParticleContainer::pyAppend, DemData attributes
+ *                                                        src/dem/ParticleContainer.cpp:176-201, src/dem/Particle.hpp:281-292
+ *   wdg_contact_count / wdg_contact_list
+ *                      ContactContainer size / linView   src/dem/ContactContainer.hpp:16-141
+ *   wdg_observables    DemFuncs::coordNumber/porosity    src/dem/Funcs.cpp:84-114
+ *   wdg_run            Scene::doOneStep (engine loop)    src/core/Scene.cpp:366-467
+ */
+#ifndef WOO_AMD_H
+#define WOO_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WDG_OK 0
+#define WDG_E_ARG (-1)      /* invalid argument */
+#define WDG_E_HIP (-2)      /* HIP runtime error */
+#define WDG_E_STATE (-3)    /* call not valid in the current state */
+#define WDG_E_NUMERIC (-4)  /* NaN force or similar, raised by a kernel */
+#define WDG_E_GEOM (-5)     /* periodic double contact, too many periods */
+#define WDG_E_CAPACITY (-6) /* a size limit was exceeded */
+
+typedef struct wdg_ctx wdg_ctx;
+
+/* particle shapes */
+enum { WDG_SPHERE = 0, WDG_WALL = 1 };
+/* material kinds */
+enum { WDG_FRICTMAT = 0, WDG_HERTZMAT = 1 };
+/* contact models */
+enum { WDG_PHYS_FRICT = 0, WDG_PHYS_HERTZ = 1 };
+enum { WDG_LAW_IDEALELPL = 0, WDG_LAW_HERTZDMT = 1 };
+/* node fields for wdg_upload / wdg_download */
+enum {
+  WDG_F_POS = 0, WDG_F_VEL = 1, WDG_F_ANGVEL = 2, WDG_F_ORI = 3, WDG_F_FORCE = 4, WDG_F_TORQUE = 5,
+  WDG_F_MASS = 6, WDG_F_INERTIA = 7
+};
+/* engine codes for wdg_run */
+enum { WDG_E_FORCERESETTER = 1, WDG_E_COLLIDER = 2, WDG_E_CONTACTLOOP = 3, WDG_E_LEAPFROG = 4, WDG_E_DYNDT = 5 };
+
+/* DemField + Scene.cell (diagonal hSize only; gradV is zero) */
+typedef struct {
+  double gravity[3];
+  double distFactor;   /* DemField.distFactor; negative means 1 */
+  uint32_t loneMask;   /* DemField.loneMask */
+  int32_t periodic;    /* Scene.periodic */
+  double cell[3];      /* Scene.cell.hSize diagonal */
+} wdg_field;
+
+/* FrictMat / HertzMat */
+typedef struct {
+  int32_t kind;
+  double density, young, ktDivKn, tanPhi, surfEnergy, alpha;
+} wdg_material;
+
+/* bulk particle description, one node per particle; arrays of length n (x3 / x4 where noted) */
+typedef struct {
+  int64_t n;
+  const int32_t* shape;       /* WDG_SPHERE | WDG_WALL */
+  const double* radius;       /* Sphere.radius (ignored for walls) */
+  const int32_t* wallAxis;    /* Wall.axis (ignored for spheres); may be NULL if no walls */
+  const int32_t* wallSense;   /* Wall.sense */
+  const int32_t* material;    /* index into wdg_set_materials */
+  const uint32_t* mask;       /* Particle.mask */
+  const uint32_t* flags;      /* DemData.flags: blocked DOFs bits 0-5, gravitySkip 512, dampingSkip 2048 */
+  const uint8_t* integrated;  /* 1 if the node is in DemField.nodes */
+  const double* pos;          /* x3 */
+  const double* vel;          /* x3, may be NULL (zero) */
+  const double* angVel;       /* x3, may be NULL */
+  const double* ori;          /* x4 (w,x,y,z), may be NULL (identity) */
+  const double* force;        /* x3, may be NULL */
+  const double* torque;       /* x3, may be NULL */
+  const double* mass;         /* x1 */
+  const double* inertia;      /* x3 */
+} wdg_particles;
+
+typedef struct {
+  double verletDist;   /* InsertionSortCollider.verletDist: <0 relative to the smallest sphere radius */
+  int32_t forceInit;   /* InsertionSortCollider.forceInitSort */
+} wdg_collide_params;
+
+typedef struct {
+  int64_t step;        /* Scene.step (trsfRenorm uses it) */
+  double dt;           /* Scene.dt */
+  int32_t physKind;    /* WDG_PHYS_* */
+  int32_t lawKind;     /* WDG_LAW_* */
+  int32_t applyForces; /* ContactLoop.applyForces */
+  int32_t trsfRenorm;  /* Cg2_Any_Any_L6Geom__Base.trsfRenorm */
+  double poisson, en;  /* Cp2_HertzMat_HertzPhys.poisson, .en */
+  int32_t noSlip, noBreak, noFrict; /* Law2_L6Geom_FrictPhys_IdealElPl */
+  int32_t noAttraction;             /* Law2_L6Geom_HertzPhys_DMT */
+} wdg_contact_params;
+
+typedef struct {
+  double dt;           /* Scene.dt */
+  double damping;      /* Leapfrog.damping */
+  int32_t reset;       /* Leapfrog.reset */
+} wdg_leapfrog_params;
+
+/* counters, read back by wdg_stats */
+typedef struct {
+  int64_t nParticles;
+  int64_t nCandidates;   /* contact slots (potential pairs after the last refresh, plus carried real contacts) */
+  int64_t nReal;         /* real contacts after the last contact loop (computed on request) */
+  int64_t nRefresh;      /* broadphase rebuilds so far */
+  int64_t nSteps;        /* collider calls so far */
+  double verletDist;     /* resolved absolute Verlet distance */
+  int32_t brute;         /* last refresh used the all-pairs path */
+  int32_t gridCells[3];  /* cell grid of the last refresh */
+} wdg_counters;
+
+/* one Scene::doOneStep engine list, for wdg_run */
+typedef struct {
+  int32_t nEngines;
+  int32_t engines[8];  /* WDG_E_* codes in Scene.engines order */
+  wdg_collide_params collide;
+  wdg_contact_params contact;
+  wdg_leapfrog_params leapfrog;
+  int32_t dynDtPeriod; /* DynDt.stepPeriod */
+  double maxRelInc;    /* DynDt.maxRelInc */
+  double dtSafety;     /* Scene.dtSafety */
+} wdg_program;
+
+const char* wdg_last_error(void);
+int wdg_ctx_create(int device, wdg_ctx** out);
+int wdg_ctx_destroy(wdg_ctx* ctx);
+int wdg_set_field(wdg_ctx* ctx, const wdg_field* f);
+int wdg_set_materials(wdg_ctx* ctx, const wdg_material* mats, int32_t n);
+int wdg_upload_particles(wdg_ctx* ctx, const wdg_particles* p);
+int wdg_upload(wdg_ctx* ctx, int32_t field, const double* host);
+int wdg_download(wdg_ctx* ctx, int32_t field, double* host);
+
+int wdg_collide(wdg_ctx* ctx, const wdg_collide_params* p);
+int wdg_contact_loop(wdg_ctx* ctx, const wdg_contact_params* p);
+int wdg_leapfrog(wdg_ctx* ctx, const wdg_leapfrog_params* p);
+int wdg_force_reset(wdg_ctx* ctx);
+int wdg_crit_dt(wdg_ctx* ctx, double* critDt);
+
+/* which=0: potential+real (the ContactContainer view), which=1: real only.
+ * ids n*2 (pA,pB), cellDist n*3, data n*16 doubles:
+ *   trsf col0 (3), trsf col1 (3), contact point (3), local force (3), uN, kn, kt, real(1/0).
+ * Sorted by (min id, max id). Any output pointer may be NULL. */
+int wdg_contact_count(wdg_ctx* ctx, int32_t which, int64_t* n);
+int wdg_contact_list(wdg_ctx* ctx, int32_t which, int32_t* ids, int32_t* cellDist, double* data);
+
+/* coordNumber (skipFree=0 and 1) and solid fraction in a box */
+int wdg_observables(wdg_ctx* ctx, const double boxMin[3], const double boxMax[3], double out[3]);
+int wdg_stats(wdg_ctx* ctx, wdg_counters* out);
+
+/* run nSteps of Scene::doOneStep with a fixed engine list; *step and *dt are the
+ * scene's step counter and timestep, updated in place (DynDt may change dt). */
+int wdg_run(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t* step, double* dt);
+
+int wdg_sync(wdg_ctx* ctx);
+/* HIP stream the engine launches on (hipStream_t), for callers that time it with events */
+void* wdg_stream(wdg_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

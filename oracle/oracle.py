@@ -1,0 +1,188 @@
+"""ctypes front end of the CPU oracle (test infrastructure only)."""
+import ctypes as C
+import math
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+liboracle_path = os.path.join(_HERE, "_build", "liboracle.so")
+
+ENG = dict(ForceResetter=1, Collider=2, ContactLoop=3, Leapfrog=4, DynDt=5)
+FIELD = dict(pos=0, vel=1, angVel=2, ori=3, force=4, torque=5, mass=6, inertia=7)
+_FIELD_W = dict(pos=3, vel=3, angVel=3, ori=4, force=3, torque=3, mass=1, inertia=3)
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("dtSafety", C.c_double), ("trsfRenorm", C.c_int), ("deterministic", C.c_int),
+        ("nThreads", C.c_int),
+        ("gravity", C.c_double * 3), ("distFactor", C.c_double), ("loneMask", C.c_uint32),
+        ("periodic", C.c_int), ("cell", C.c_double * 3),
+        ("verletDist", C.c_double), ("colliderMode", C.c_int),
+        ("physKind", C.c_int), ("lawKind", C.c_int), ("applyForces", C.c_int),
+        ("poisson", C.c_double), ("en", C.c_double),
+        ("noSlip", C.c_int), ("noBreak", C.c_int), ("noFrict", C.c_int), ("noAttraction", C.c_int),
+        ("damping", C.c_double), ("reset", C.c_int),
+        ("dynDtPeriod", C.c_int), ("maxRelInc", C.c_double),
+    ]
+
+
+def build_oracle():
+    """Compile the oracle with its Makefile (gcc, no GPU needed)."""
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(liboracle_path):
+        build_oracle()
+    L = C.CDLL(liboracle_path)
+    P, D, I = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)
+    L.orc_default_params.argtypes = [C.POINTER(OrcParams)]
+    L.orc_create.argtypes = [C.POINTER(OrcParams)]; L.orc_create.restype = P
+    L.orc_destroy.argtypes = [P]
+    L.orc_set_params.argtypes = [P, C.POINTER(OrcParams)]
+    L.orc_add_material.argtypes = [P, C.c_int] + [C.c_double] * 6; L.orc_add_material.restype = C.c_int
+    L.orc_add_sphere.argtypes = [P, D, C.c_double, C.c_int, C.c_int, C.c_uint32]; L.orc_add_sphere.restype = C.c_int
+    L.orc_add_wall.argtypes = [P, D, C.c_int, C.c_int, C.c_int, C.c_uint32]; L.orc_add_wall.restype = C.c_int
+    L.orc_num_particles.argtypes = [P]; L.orc_num_particles.restype = C.c_int
+    L.orc_get.argtypes = [P, C.c_int, D]; L.orc_get.restype = C.c_int
+    L.orc_set.argtypes = [P, C.c_int, D]; L.orc_set.restype = C.c_int
+    L.orc_run.argtypes = [P, I, C.c_int, C.c_int]; L.orc_run.restype = C.c_int
+    L.orc_step.argtypes = [P]; L.orc_step.restype = C.c_int64
+    L.orc_dt.argtypes = [P]; L.orc_dt.restype = C.c_double
+    L.orc_verlet_dist.argtypes = [P]; L.orc_verlet_dist.restype = C.c_double
+    L.orc_num_refresh.argtypes = [P]; L.orc_num_refresh.restype = C.c_int
+    L.orc_num_contacts.argtypes = [P, C.c_int]; L.orc_num_contacts.restype = C.c_int64
+    L.orc_get_contacts.argtypes = [P, C.c_int, I, I, D]; L.orc_get_contacts.restype = C.c_int64
+    L.orc_coord_number.argtypes = [P, C.c_int]; L.orc_coord_number.restype = C.c_double
+    L.orc_solid_fraction.argtypes = [P, D, D]; L.orc_solid_fraction.restype = C.c_double
+    L.orc_kinetic_energy.argtypes = [P]; L.orc_kinetic_energy.restype = C.c_double
+    L.orc_pwave_dt.argtypes = [P]; L.orc_pwave_dt.restype = C.c_double
+    L.orc_crit_dt.argtypes = [P]; L.orc_crit_dt.restype = C.c_double
+    L.orc_brute_aabb_pairs.argtypes = [P, I, C.c_int64]; L.orc_brute_aabb_pairs.restype = C.c_int64
+    L.orc_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class Oracle:
+    """One oracle scene. ``params`` are OrcParams field overrides."""
+
+    def __init__(self, **params):
+        L = _load()
+        self.p = OrcParams()
+        L.orc_default_params(C.byref(self.p))
+        self.p.en = math.nan
+        self.set_params(**params, _apply=False)
+        self.h = L.orc_create(C.byref(self.p))
+        self.L = L
+
+    def set_params(self, _apply=True, **kw):
+        for k, v in kw.items():
+            if k in ("gravity", "cell"):
+                getattr(self.p, k)[:] = list(v)
+            else:
+                setattr(self.p, k, v)
+        if _apply:
+            self.L.orc_set_params(self.h, C.byref(self.p))
+
+    def __del__(self):
+        try:
+            self.L.orc_destroy(self.h)
+        except Exception:
+            pass
+
+    def material(self, kind=0, density=1e3, young=1e7, ktDivKn=.2, tanPhi=.5, surfEnergy=0., alpha=0.):
+        return self.L.orc_add_material(self.h, kind, density, young, ktDivKn, tanPhi, surfEnergy, alpha)
+
+    def sphere(self, pos, radius, mat=0, fixed=False, mask=0b0101):
+        a = np.ascontiguousarray(pos, dtype=np.float64)
+        return self.L.orc_add_sphere(self.h, _dp(a), float(radius), mat, int(bool(fixed)), mask)
+
+    def wall(self, pos, axis, sense=0, mat=0, mask=0b0011):
+        a = np.ascontiguousarray(pos, dtype=np.float64)
+        return self.L.orc_add_wall(self.h, _dp(a), axis, sense, mat, mask)
+
+    @property
+    def n(self):
+        return self.L.orc_num_particles(self.h)
+
+    def get(self, field):
+        w = _FIELD_W[field]
+        out = np.empty((self.n, w) if w > 1 else (self.n,), dtype=np.float64)
+        self.L.orc_get(self.h, FIELD[field], _dp(out))
+        return out
+
+    def set(self, field, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        self.L.orc_set(self.h, FIELD[field], _dp(a))
+
+    def run(self, engines, steps=1):
+        e = np.array([ENG[x] if isinstance(x, str) else x for x in engines], dtype=np.int32)
+        rc = self.L.orc_run(self.h, _ip(e), len(e), steps)
+        if rc != 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
+
+    @property
+    def step(self):
+        return self.L.orc_step(self.h)
+
+    @property
+    def dt(self):
+        return self.L.orc_dt(self.h)
+
+    @property
+    def verletDist(self):
+        return self.L.orc_verlet_dist(self.h)
+
+    @property
+    def nRefresh(self):
+        return self.L.orc_num_refresh(self.h)
+
+    def contacts(self, real_only=False, data=True):
+        """Return (ids[n,2], cellDist[n,3], data[n,16]) sorted by (min id, max id)."""
+        w = 1 if real_only else 0
+        n = self.L.orc_num_contacts(self.h, w)
+        ids = np.empty((n, 2), dtype=np.int32)
+        cd = np.empty((n, 3), dtype=np.int32)
+        d = np.empty((n, 16), dtype=np.float64) if data else None
+        self.L.orc_get_contacts(self.h, w, _ip(ids), _ip(cd), _dp(d) if data else None)
+        return ids, cd, d
+
+    def brute_aabb_pairs(self):
+        n = self.L.orc_brute_aabb_pairs(self.h, None, 0)
+        ids = np.empty((n, 2), dtype=np.int32)
+        self.L.orc_brute_aabb_pairs(self.h, _ip(ids), n)
+        return ids
+
+    def coord_number(self, skipFree=False):
+        return self.L.orc_coord_number(self.h, int(skipFree))
+
+    def solid_fraction(self, lo, hi):
+        a, b = np.asarray(lo, dtype=np.float64), np.asarray(hi, dtype=np.float64)
+        return self.L.orc_solid_fraction(self.h, _dp(a), _dp(b))
+
+    def kinetic_energy(self):
+        return self.L.orc_kinetic_energy(self.h)
+
+    def pwave_dt(self):
+        return self.L.orc_pwave_dt(self.h)
+
+    def crit_dt(self):
+        return self.L.orc_crit_dt(self.h)

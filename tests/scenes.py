@@ -1,0 +1,217 @@
+"""Seeded test scenes, built identically on the CPU oracle and on the HIP engine.
+
+A scene is a plain dict of numpy arrays (the woo.utils.sphere / woo.utils.wall
+inputs); mass and inertia follow Sphere::lumpMassInertia (Sphere.cpp:26-33).
+"""
+import math
+
+import numpy as np
+
+MOVABLE_MASK = 0b0101  # DemField.defaultMovableMask
+BOUNDARY_MASK = 0b0011  # DemField.defaultBoundaryMask
+DOF_ALL = 63
+
+
+def sphere_mass_inertia(r, rho):
+    r = np.asarray(r, dtype=np.float64)
+    m = (4 / 3.) * math.pi * (r * r * r) * rho
+    i = 0. + 1. * (2. / 5.) * m * (r * r)
+    return m, i
+
+
+def _rss_positions(rng, n, lo, hi, radii, max_tries=2000000):
+    """random sequential placement without overlap (cell-binned rejection)"""
+    pos = np.empty((n, 3))
+    rmax = float(np.max(radii))
+    h = 2 * rmax
+    cells = {}
+    k = 0
+    tries = 0
+    while k < n:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError("random placement did not converge")
+        r = radii[k]
+        c = rng.uniform(lo + r, hi - r, size=3)
+        key = tuple((c // h).astype(int))
+        ok = True
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for j in cells.get((key[0] + dx, key[1] + dy, key[2] + dz), ()):
+                        if np.sum((pos[j] - c) ** 2) < (radii[j] + r) ** 2:
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if not ok:
+                break
+        if ok:
+            pos[k] = c
+            cells.setdefault(key, []).append(k)
+            k += 1
+    return pos
+
+
+def deposition(n=2000, seed=1, box=0.2, r=5e-3, rho=3000., young=1e7, ktDivKn=.2, tanPhi=.5, dense=False,
+               walls_first=False, vel_sigma=0.):
+    """Gravity deposition in a box of 6 walls (FrictMat). dense=True packs spheres on a
+    jittered lattice with slight overlaps so that contacts exist from the first step."""
+    rng = np.random.default_rng(seed)
+    radii = np.full(n, r)
+    if dense:
+        a = 1.98 * r
+        m = int(math.ceil(n ** (1 / 3.)))
+        g = np.stack(np.meshgrid(*[np.arange(m)] * 3, indexing="ij"), -1).reshape(-1, 3)[:n].astype(float)
+        pos = (g + 0.5) * a + rng.uniform(-0.01 * r, 0.01 * r, size=(n, 3))
+        box = m * a
+    else:
+        pos = _rss_positions(rng, n, 0., box, radii)
+    vel = rng.normal(0, vel_sigma, size=(n, 3)) if vel_sigma > 0 else np.zeros((n, 3))
+    mass, inertia = sphere_mass_inertia(radii, rho)
+    walls = []
+    for sense, coord in ((1, 0.), (-1, box)):
+        for ax in (0, 1, 2):
+            p = np.zeros(3)
+            p[ax] = coord
+            walls.append(dict(pos=p, axis=ax, sense=sense))
+    pwave = r / math.sqrt(young / rho)
+    return dict(
+        materials=[dict(kind=0, density=rho, young=young, ktDivKn=ktDivKn, tanPhi=tanPhi, surfEnergy=0., alpha=0.)],
+        pos=pos, radius=radii, vel=vel, mass=mass, inertia=np.repeat(inertia[:, None], 3, 1),
+        walls=walls, walls_first=walls_first, gravity=(0., 0., -9.81), periodic=False, cell=(1., 1., 1.),
+        dt=0.5 * pwave, physKind=0, lawKind=0, damping=0.2, box=box)
+
+
+def periodic_hertz(n=1000, seed=2, rmin=2e-3, rmax=4e-3, rho=3000., young=1e8, tanPhi=.4, poisson=.25, en=.6,
+                   spacing=1.0, vel_sigma=0.1):
+    """Periodic Hertz box with polydisperse spheres on a jittered cubic lattice;
+    spacing is the lattice pitch relative to 2*rmax (below 1 gives initial contacts)."""
+    rng = np.random.default_rng(seed)
+    radii = rng.uniform(rmin, rmax, size=n)
+    m = int(math.ceil(n ** (1 / 3.)))
+    a = spacing * 2 * rmax
+    g = np.stack(np.meshgrid(*[np.arange(m)] * 3, indexing="ij"), -1).reshape(-1, 3)[:n].astype(float)
+    pos = (g + 0.5) * a + rng.uniform(-0.02 * rmin, 0.02 * rmin, size=(n, 3))
+    L = m * a
+    # unwrapped coordinates: shift some particles by whole periods
+    pos += L * rng.integers(-1, 2, size=(n, 3)) * (rng.uniform(size=(n, 1)) < 0.1)
+    vel = rng.normal(0, vel_sigma, size=(n, 3))
+    mass, inertia = sphere_mass_inertia(radii, rho)
+    pwave = rmin / math.sqrt(young / rho)
+    return dict(
+        materials=[dict(kind=1, density=rho, young=young, ktDivKn=.2, tanPhi=tanPhi, surfEnergy=0., alpha=0.)],
+        pos=pos, radius=radii, vel=vel, mass=mass, inertia=np.repeat(inertia[:, None], 3, 1),
+        walls=[], walls_first=False, gravity=(0., 0., 0.), periodic=True, cell=(L, L, L),
+        dt=0.3 * pwave, physKind=1, lawKind=1, damping=0., poisson=poisson, en=en, box=L)
+
+
+def _particle_order(sc):
+    """list of ('s', i) / ('w', j) in id order"""
+    n = len(sc["radius"])
+    sph = [("s", i) for i in range(n)]
+    wal = [("w", j) for j in range(len(sc["walls"]))]
+    return wal + sph if sc["walls_first"] else sph + wal
+
+
+def build_oracle(sc, **params):
+    from oracle import Oracle
+
+    p = dict(dt=sc["dt"], gravity=sc["gravity"], periodic=int(sc["periodic"]), cell=sc["cell"],
+             physKind=sc["physKind"], lawKind=sc["lawKind"], damping=sc["damping"],
+             poisson=sc.get("poisson", .2), en=sc.get("en", float("nan")), deterministic=1)
+    p.update(params)
+    o = Oracle(**p)
+    for m in sc["materials"]:
+        o.material(**m)
+    for kind, i in _particle_order(sc):
+        if kind == "s":
+            o.sphere(sc["pos"][i], sc["radius"][i], 0, False, MOVABLE_MASK)
+        else:
+            w = sc["walls"][i]
+            o.wall(w["pos"], w["axis"], w["sense"], 0, BOUNDARY_MASK)
+    order = _particle_order(sc)
+    o.set("vel", _gather(sc, order, "vel", 3))
+    o.set("mass", _gather(sc, order, "mass", 1))
+    o.set("inertia", _gather(sc, order, "inertia", 3))
+    return o
+
+
+def _gather(sc, order, key, w):
+    out = np.zeros((len(order), w)) if w > 1 else np.zeros(len(order))
+    for k, (kind, i) in enumerate(order):
+        if kind == "s":
+            out[k] = sc[key][i]
+    return out
+
+
+def gpu_arrays(sc):
+    order = _particle_order(sc)
+    n = len(order)
+    shape = np.array([0 if k == "s" else 1 for k, _ in order], dtype=np.int32)
+    pos = np.zeros((n, 3))
+    radius = np.zeros(n)
+    axis = np.zeros(n, dtype=np.int32)
+    sense = np.zeros(n, dtype=np.int32)
+    mask = np.zeros(n, dtype=np.uint32)
+    flags = np.zeros(n, dtype=np.uint32)
+    integ = np.zeros(n, dtype=np.uint8)
+    for k, (kind, i) in enumerate(order):
+        if kind == "s":
+            pos[k] = sc["pos"][i]
+            radius[k] = sc["radius"][i]
+            mask[k] = MOVABLE_MASK
+            integ[k] = 1
+        else:
+            w = sc["walls"][i]
+            pos[k] = w["pos"]
+            axis[k] = w["axis"]
+            sense[k] = w["sense"]
+            mask[k] = BOUNDARY_MASK
+            flags[k] = DOF_ALL
+    return dict(shape=shape, pos=pos, radius=radius, wallAxis=axis, wallSense=sense, material=np.zeros(n, np.int32),
+                mask=mask, flags=flags, integrated=integ, vel=_gather(sc, order, "vel", 3),
+                mass=_gather(sc, order, "mass", 1), inertia=_gather(sc, order, "inertia", 3))
+
+
+def build_gpu(sc, device=0):
+    from woo_amd._native import Device
+
+    d = Device(device)
+    d.set_field(gravity=sc["gravity"], periodic=sc["periodic"], cell=sc["cell"])
+    d.set_materials(sc["materials"])
+    d.upload_particles(**gpu_arrays(sc))
+    return d
+
+
+def contact_kw(sc):
+    return dict(physKind=sc["physKind"], lawKind=sc["lawKind"], poisson=sc.get("poisson", .2),
+                en=sc.get("en", float("nan")))
+
+
+class GpuRunner:
+    """Drives the HIP engine through the C ABI one engine at a time (Scene::doOneStep)."""
+
+    def __init__(self, sc, verletDist=-.05, device=0):
+        self.sc = sc
+        self.d = build_gpu(sc, device)
+        self.step = 0
+        self.dt = sc["dt"]
+        self.vd = verletDist
+
+    def run(self, engines, steps=1):
+        for _ in range(steps):
+            for e in engines:
+                if e == "Collider":
+                    self.d.collide(self.vd)
+                elif e == "ContactLoop":
+                    self.d.contact_loop(self.step, self.dt, **contact_kw(self.sc))
+                elif e == "Leapfrog":
+                    self.d.leapfrog(self.dt, self.sc["damping"], True)
+                elif e == "ForceResetter":
+                    self.d.force_reset()
+                else:
+                    raise ValueError(e)
+            self.step += 1

@@ -1,0 +1,255 @@
+"""Parity of the HIP engine (through the C ABI) with the CPU oracle.
+
+Bars (integer/index work exact, FP64 physics per the engine's spec):
+  * contact-pair sets (potential and real) identical, cellDist identical;
+  * per-particle forces: max_i |F_i^gpu - F_i^cpu| / max(|F_i^cpu|, S_i) <= 1e-12, where
+    S_i is the sum of contact-force magnitudes on particle i (guards equilibrium cancellation);
+  * 100-step trajectories: max_i |x_i^gpu - x_i^cpu| / r_i <= 1e-9.
+Both sides evaluate the same expressions in the same order without FMA, so the
+observed differences are expected to be zero; the tests print them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests.scenes import GpuRunner, build_oracle, deposition, periodic_hertz
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-12
+TRAJ_TOL = 1e-9
+
+
+def _sets(ids):
+    return set(map(tuple, ids.tolist()))
+
+
+def compare_contact_sets(o, g, real_only):
+    io, co, do = o.contacts(real_only=real_only)
+    ig, cg, dg = g.d.contacts(real_only=real_only)
+    assert io.shape == ig.shape, (len(_sets(io) - _sets(ig)), len(_sets(ig) - _sets(io)))
+    np.testing.assert_array_equal(io, ig)
+    np.testing.assert_array_equal(co, cg)
+    return do, dg
+
+
+def force_metric(o, g):
+    Fo, Fg = o.get("force"), g.d.get("force")
+    To, Tg = o.get("torque"), g.d.get("torque")
+    ids, _, data = o.contacts(real_only=True)
+    S = np.zeros(len(Fo))
+    if len(ids):
+        mag = np.linalg.norm(data[:, 9:12], axis=1)
+        np.add.at(S, ids[:, 0], mag)
+        np.add.at(S, ids[:, 1], mag)
+    den = np.maximum(np.linalg.norm(Fo, axis=1), S)
+    den[den == 0] = 1.
+    ef = np.max(np.linalg.norm(Fg - Fo, axis=1) / den) if len(Fo) else 0.
+    # torque scale: S_i times the largest radius
+    rmax = 5e-3
+    dent = np.maximum(np.linalg.norm(To, axis=1), S * rmax)
+    dent[dent == 0] = 1.
+    et = np.max(np.linalg.norm(Tg - To, axis=1) / dent) if len(To) else 0.
+    return ef, et, int(np.sum(np.any(Fg != Fo, axis=1)))
+
+
+def traj_metric(o, g, sc, nw_first):
+    Po, Pg = o.get("pos"), g.d.get("pos")
+    r = np.concatenate([np.ones(len(sc["walls"])), sc["radius"]]) if nw_first else np.concatenate(
+        [sc["radius"], np.ones(len(sc["walls"]))])
+    return np.max(np.linalg.norm(Pg - Po, axis=1) / r), int(np.sum(np.any(Pg != Po, axis=1)))
+
+
+ENG = ["Collider", "ContactLoop", "Leapfrog"]
+
+
+@pytest.mark.parametrize("n,dense,seed", [(400, True, 1), (3000, False, 2), (1200, True, 3)])
+def test_broadphase_pairs_aperiodic(n, dense, seed):
+    sc = deposition(n=n, dense=dense, seed=seed)
+    o = build_oracle(sc)
+    g = GpuRunner(sc)
+    o.run(["Collider"])
+    g.run(["Collider"])
+    compare_contact_sets(o, g, real_only=False)
+    # the potential set is exactly the brute-force enlarged-AABB overlap set
+    assert _sets(o.brute_aabb_pairs()) == _sets(g.d.contacts(data=False)[0])
+
+
+@pytest.mark.parametrize("n,spacing,seed", [(300, 1.0, 4), (2000, 1.01, 5)])
+def test_broadphase_pairs_periodic(n, spacing, seed):
+    sc = periodic_hertz(n=n, spacing=spacing, seed=seed)
+    o = build_oracle(sc)
+    g = GpuRunner(sc)
+    o.run(["Collider"])
+    g.run(["Collider"])
+    compare_contact_sets(o, g, real_only=False)
+    st = g.d.stats()
+    print("grid", st["gridCells"], "brute", st["brute"], "candidates", st["nCandidates"])
+
+
+def test_walls_first_ids():
+    sc = deposition(n=600, dense=True, seed=6, walls_first=True)
+    o = build_oracle(sc)
+    g = GpuRunner(sc)
+    o.run(["Collider", "ContactLoop"])
+    g.run(["Collider", "ContactLoop"])
+    compare_contact_sets(o, g, real_only=True)
+    ef, et, ndiff = force_metric(o, g)
+    print("walls-first force err", ef, et, ndiff)
+    assert ef <= FORCE_TOL and et <= FORCE_TOL
+
+
+def test_first_step_forces_frict():
+    sc = deposition(n=1000, dense=True, seed=7, vel_sigma=0.02)
+    o = build_oracle(sc)
+    g = GpuRunner(sc)
+    o.run(["Collider", "ContactLoop"])
+    g.run(["Collider", "ContactLoop"])
+    do, dg = compare_contact_sets(o, g, real_only=True)
+    assert len(do) > 500
+    np.testing.assert_allclose(dg, do, rtol=1e-13, atol=0)
+    ef, et, ndiff = force_metric(o, g)
+    print("frict step-1 force err", ef, et, "particles differing:", ndiff)
+    assert ef <= FORCE_TOL and et <= FORCE_TOL
+
+
+def test_first_step_forces_hertz_periodic():
+    sc = periodic_hertz(n=1000, spacing=0.88, seed=8)
+    o = build_oracle(sc)
+    g = GpuRunner(sc)
+    o.run(["Collider", "ContactLoop"])
+    g.run(["Collider", "ContactLoop"])
+    do, dg = compare_contact_sets(o, g, real_only=True)
+    assert len(do) > 250
+    np.testing.assert_allclose(dg, do, rtol=1e-13, atol=0)
+    ef, et, ndiff = force_metric(o, g)
+    print("hertz step-1 force err", ef, et, "particles differing:", ndiff)
+    assert ef <= FORCE_TOL and et <= FORCE_TOL
+
+
+def _run_both(sc, engines, steps, check_every=10):
+    o = build_oracle(sc)
+    g = GpuRunner(sc)
+    worst = 0.
+    for k in range(0, steps, check_every):
+        o.run(engines, check_every)
+        g.run(engines, check_every)
+        compare_contact_sets(o, g, real_only=True)
+        e, _ = traj_metric(o, g, sc, sc["walls_first"])
+        worst = max(worst, e)
+    return o, g, worst
+
+
+def test_trajectory_100_frict_deposition():
+    sc = deposition(n=800, dense=True, seed=9, vel_sigma=0.05)
+    o, g, worst = _run_both(sc, ENG, 100)
+    e, nd = traj_metric(o, g, sc, False)
+    print("frict 100-step position err", worst, "particles differing:", nd, "refresh", o.nRefresh,
+          g.d.stats()["nRefresh"])
+    assert worst <= TRAJ_TOL
+    np.testing.assert_allclose(g.d.get("vel"), o.get("vel"), rtol=0, atol=1e-9 * 0.05 + 1e-15)
+
+
+def test_trajectory_100_hertz_periodic():
+    sc = periodic_hertz(n=1500, spacing=0.92, seed=10)
+    o, g, worst = _run_both(sc, ENG, 100)
+    print("hertz 100-step position err", worst, "refresh", o.nRefresh, g.d.stats()["nRefresh"])
+    assert worst <= TRAJ_TOL
+
+
+def test_minimal_engine_order_with_forceresetter():
+    # DemField.minimalEngines order (utils.py:122-129): Leapfrog first, forces gathered lazily
+    sc = deposition(n=700, dense=True, seed=11, vel_sigma=0.05)
+    eng = ["Leapfrog", "Collider", "ContactLoop"]
+    o, g, worst = _run_both(sc, eng, 60)
+    assert worst <= TRAJ_TOL
+    eng2 = ["ForceResetter", "Collider", "ContactLoop", "Leapfrog"]
+    sc2 = deposition(n=500, dense=True, seed=12, vel_sigma=0.05)
+    o2 = build_oracle(sc2, reset=0)
+    g2 = GpuRunner(sc2)
+    o2.run(eng2, 40)
+    for _ in range(40):
+        g2.d.force_reset()
+        g2.d.collide(-.05)
+        g2.d.contact_loop(g2.step, g2.dt)
+        g2.d.leapfrog(g2.dt, sc2["damping"], reset=False)
+        g2.step += 1
+    e, _ = traj_metric(o2, g2, sc2, False)
+    assert e <= TRAJ_TOL
+
+
+def test_crit_dt_matches():
+    for sc in (deposition(n=600, dense=True, seed=13), periodic_hertz(n=800, spacing=0.97, seed=14)):
+        o = build_oracle(sc)
+        g = GpuRunner(sc)
+        o.run(ENG, 5)
+        g.run(ENG, 5)
+        co, cg = o.crit_dt(), g.d.crit_dt()
+        print("critDt", co, cg)
+        assert math.isfinite(co) and cg == pytest.approx(co, rel=1e-13)
+
+
+def test_observables_match():
+    sc = deposition(n=900, dense=True, seed=15)
+    o = build_oracle(sc)
+    g = GpuRunner(sc)
+    o.run(["Collider", "ContactLoop"])
+    g.run(["Collider", "ContactLoop"])
+    lo, hi = (0, 0, 0), (sc["box"],) * 3
+    ob = g.d.observables(lo, hi)
+    assert ob["coordNumber"] == pytest.approx(o.coord_number(False), rel=1e-14)
+    assert ob["coordNumberSkipFree"] == pytest.approx(o.coord_number(True), rel=1e-14)
+    assert ob["solidFraction"] == pytest.approx(o.solid_fraction(lo, hi), rel=1e-12)
+
+
+def test_program_loop_matches_stepwise():
+    sc = deposition(n=500, dense=True, seed=16, vel_sigma=0.05)
+    g1 = GpuRunner(sc)
+    g1.run(ENG, 30)
+    g2 = GpuRunner(sc)
+    step, dt = g2.d.run(ENG, 30, 0, sc["dt"], contact=dict(physKind=0, lawKind=0),
+                        leapfrog=dict(damping=sc["damping"]), dynDtPeriod=0)
+    assert step == 30
+    np.testing.assert_array_equal(g1.d.get("pos"), g2.d.get("pos"))
+
+
+def test_dyndt_program_matches_oracle():
+    sc = deposition(n=500, dense=True, seed=17, vel_sigma=0.05)
+    eng = ["Leapfrog", "Collider", "ContactLoop", "DynDt"]
+    o = build_oracle(sc, dynDtPeriod=10, maxRelInc=1e-2, dtSafety=.9)
+    o.run(eng, 35)
+    g = GpuRunner(sc)
+    step, dt = g.d.run(eng, 35, 0, sc["dt"], contact=dict(physKind=0, lawKind=0), leapfrog=dict(damping=.2),
+                       dynDtPeriod=10, maxRelInc=1e-2, dtSafety=.9)
+    print("dt oracle", o.dt, "gpu", dt)
+    assert dt == pytest.approx(o.dt, rel=1e-13)
+    e, _ = traj_metric(o, g, sc, False)
+    assert e <= TRAJ_TOL
+
+
+def test_periodic_double_image_raises():
+    # PBC: a pair touching through two images is an error (test_pbc.py:151-160)
+    sc = periodic_hertz(n=2, spacing=1.0, seed=18)
+    sc["pos"] = np.array([[.3, .5, .5], [.8, .9, .5]])
+    sc["radius"] = np.array([.4, .2])
+    sc["vel"] = np.zeros((2, 3))
+    sc["cell"] = (1., 1., 1.)
+    g = GpuRunner(sc, verletDist=0.)
+    with pytest.raises(RuntimeError):
+        g.run(["Collider"])
+
+
+def test_periodic_celldist_cases():
+    # cellDist follows the shift of the second particle (test_pbc.py:162-172)
+    for shift2 in [(0, 0, 0), (1, 2, 3), (6, 8, -4)]:
+        sc = periodic_hertz(n=2, spacing=1.0, seed=19)
+        sc["pos"] = np.array([[.5, .5, .5], [.8, .6, .5]]) + np.array([[0, 0, 0], shift2])
+        sc["radius"] = np.array([.2, .2])
+        sc["vel"] = np.zeros((2, 3))
+        sc["cell"] = (1., 1., 1.)
+        g = GpuRunner(sc, verletDist=0.)
+        g.run(["Collider"])
+        ids, cd, _ = g.d.contacts()
+        assert ids.tolist() == [[0, 1]]
+        assert cd.tolist() == [[-s for s in shift2]]

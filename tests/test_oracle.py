@@ -1,0 +1,300 @@
+"""Pins of the CPU oracle against closed forms and the reference's own test cases.
+
+Each check is chosen so that a plausible mistake in the restatement (a dropped
+term, a wrong sign, a swapped operand) fails it; none of them retypes the
+oracle's formula.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import Oracle
+from tests.scenes import deposition, periodic_hertz, build_oracle
+
+ENG = ["Collider", "ContactLoop", "Leapfrog"]
+
+
+def two_spheres(v0, en=0., r=.05, rho=1e3, E=1e8, nu=.2, dtfrac=.04, gap=1e-5):
+    """Hertz head-on impact (py/tests/test_hertz.py:13-47 setup, relative approach speed v0)."""
+    o = Oracle(physKind=1, lawKind=1, poisson=nu, en=en, damping=0., gravity=(0, 0, 0))
+    m = o.material(kind=1, density=rho, young=E, ktDivKn=.2, tanPhi=.5)
+    o.sphere((0, 0, 0), r, m)
+    o.sphere((0, 2 * r + gap, 0), r, m)
+    o.set("vel", np.array([[0, v0 / 2, 0], [0, -v0 / 2, 0]]))
+    o.set_params(dt=dtfrac * o.pwave_dt())
+    return o
+
+
+def collide_until_separated(o, max_steps=200000):
+    n_in, started = 0, False
+    for _ in range(max_steps):
+        o.run(ENG, 1)
+        real = len(o.contacts(real_only=True, data=False)[0])
+        if real:
+            started = True
+            n_in += 1
+        elif started:
+            return n_in
+    raise AssertionError("collision did not end")
+
+
+@pytest.mark.parametrize("v0", [1., 1e-1, 1e-2, 1e-3])
+def test_hertz_elastic_collision_time(v0):
+    # TestHertz.testElasticCollisionTime: t = 2.214 (rho/E*)^(2/5) d v0^(-1/5) within 1 %
+    o = two_spheres(v0)
+    E, nu, r, rho = 1e8, .2, .05, 1e3
+    Eeff = (E * E) / (E * (1 - nu ** 2) + E * (1 - nu ** 2))
+    tau = 2.214 * (rho / Eeff) ** (2 / 5.) * ((2 * r) / v0 ** (1 / 5.))
+    t = collide_until_separated(o) * o.dt
+    assert t == pytest.approx(tau, rel=0.01)
+    # elastic (en outside (0,1)): the velocities are exchanged
+    v = o.get("vel")
+    assert (v[1, 1] - v[0, 1]) / v0 == pytest.approx(1., rel=1e-4)
+    # head-on: no tangential force arises (only quaternion-rotation roundoff of the normal force)
+    assert abs(v[0, 0]) + abs(v[0, 2]) + abs(v[1, 0]) + abs(v[1, 2]) < 1e-14 * v0
+
+
+@pytest.mark.parametrize("en", [.3, .6, .9])
+def test_hertz_restitution(en):
+    # Antypov & Elliott (2011) eq. (10) chooses the viscosity so that the restitution
+    # coefficient equals en for the unclipped model (noAttraction=False)
+    o = two_spheres(0.5, en=en)
+    o.set_params(noAttraction=0)
+    collide_until_separated(o)
+    v = o.get("vel")
+    e = (v[1, 1] - v[0, 1]) / 0.5
+    assert e == pytest.approx(en, rel=2e-3)
+    # with the attraction clip (the default) less energy is dissipated at the end of the impact
+    o = two_spheres(0.5, en=en)
+    collide_until_separated(o)
+    v = o.get("vel")
+    e2 = (v[1, 1] - v[0, 1]) / 0.5
+    assert en < e2 < en + 0.15 * (1 - en)
+
+
+def test_linear_contact_period_pins_kn():
+    # elastic bounce on a wall (linear spring, no damping): contact lasts half a period pi*sqrt(m/kn);
+    # for a wall contact lens=(r,r), contA=pi r^2, so kn = E*pi*r/2 (FrictMat.cpp:37)
+    for E in (1e6, 4e6):
+        r, rho = .01, 2000.
+        o = Oracle(damping=0., gravity=(0, 0, 0))
+        m = o.material(density=rho, young=E, ktDivKn=.2, tanPhi=.5)
+        o.wall((0, 0, 0), 2, 1, m)
+        o.sphere((0, 0, r + 1e-6), r, m)
+        o.set("vel", np.array([[0, 0, 0], [0, 0, -0.1]]))
+        o.set_params(dt=.01 * o.pwave_dt())
+        steps = collide_until_separated(o)
+        mass = o.get("mass")[1]
+        kn = E * math.pi * r / 2
+        assert steps * o.dt == pytest.approx(math.pi * math.sqrt(mass / kn), rel=2e-3)
+        assert o.get("vel")[1, 2] == pytest.approx(0.1, rel=1e-6)
+
+
+def test_sliding_sphere_rolls_at_five_sevenths():
+    # a sphere sliding on a wall ends up rolling at 5/7 of its initial speed (solid sphere),
+    # independent of the friction coefficient; checks IdealElPl slip, torque branch and rotation
+    for tanPhi in (.3, .6):
+        r, rho, E = .01, 2500., 1e7
+        o = Oracle(damping=0., gravity=(0, 0, -9.81))
+        m = o.material(density=rho, young=E, ktDivKn=.2, tanPhi=tanPhi)
+        o.wall((0, 0, 0), 2, 1, m)
+        mass = (4 / 3.) * math.pi * r ** 3 * rho
+        kn = E * math.pi * r / 2
+        o.sphere((0, 0, r - mass * 9.81 / kn), r, m)
+        v0 = 0.2
+        o.set("vel", np.array([[0, 0, 0], [v0, 0, 0]]))
+        o.set_params(dt=.2 * o.pwave_dt())
+        t_slip = 2 / 7. * v0 / (tanPhi * 9.81)
+        nsteps = int(3 * t_slip / o.dt)
+        o.run(ENG, nsteps)
+        v = o.get("vel")[1]
+        w = o.get("angVel")[1]
+        assert v[0] == pytest.approx(5 / 7. * v0, rel=0.01)
+        # rolling without slip on a floor: contact-point velocity v_x - w_y r = 0
+        assert w[1] * r == pytest.approx(v[0], rel=0.01)
+
+
+def test_coulomb_cap_is_exact_while_sliding():
+    r, rho, E, tanPhi = .01, 2500., 1e7, .4
+    o = Oracle(damping=0., gravity=(0, 0, -9.81))
+    m = o.material(density=rho, young=E, ktDivKn=.2, tanPhi=tanPhi)
+    o.wall((0, 0, 0), 2, 1, m)
+    mass = (4 / 3.) * math.pi * r ** 3 * rho
+    o.sphere((0, 0, r - mass * 9.81 / (E * math.pi * r / 2)), r, m)
+    o.set("vel", np.array([[0, 0, 0], [0.5, 0, 0]]))
+    o.set_params(dt=.2 * o.pwave_dt())
+    o.run(ENG, 50)
+    ids, cd, d = o.contacts(real_only=True)
+    assert ids.tolist() == [[0, 1]]
+    Fn, Ft = d[0, 9], np.hypot(d[0, 10], d[0, 11])
+    assert Ft == pytest.approx(abs(Fn) * tanPhi, rel=1e-14)
+
+
+def test_free_fall_leapfrog_closed_form():
+    # gravity enters at the force reset, so step 0 integrates with zero force
+    g, dt, n, v0 = -9.81, 1e-4, 137, 0.3
+    for damping in (0., .2):
+        o = Oracle(damping=damping, gravity=(0, 0, g), dt=dt)
+        m = o.material()
+        o.sphere((0, 0, 1.), .01, m)
+        o.set("vel", np.array([[0, 0, v0]]))
+        o.run(["Leapfrog"], 1)
+        o.set("vel", np.array([[0, 0, 0.]]))  # from rest, force = m g now
+        z0 = o.get("pos")[0, 2]
+        o.run(["Leapfrog"], n)
+        # non-viscous damping opposes the motion: a *= (1 - damping) while falling
+        geff = g * (1 - damping)
+        vz = o.get("vel")[0, 2]
+        z = o.get("pos")[0, 2]
+        assert vz == pytest.approx(n * geff * dt, rel=1e-12)
+        assert z - z0 == pytest.approx(geff * dt * dt * n * (n + 1) / 2, rel=1e-10)
+
+
+def test_spherical_rotation_constant_angvel():
+    dt, n = 1e-3, 250
+    w = np.array([0.3, -1.2, 2.0])
+    o = Oracle(damping=0., gravity=(0, 0, 0), dt=dt)
+    m = o.material()
+    o.sphere((0, 0, 0), .01, m)
+    o.set("angVel", w[None, :])
+    o.run(["Leapfrog"], n)
+    q = o.get("ori")[0]
+    th = np.linalg.norm(w) * dt * n
+    ax = w / np.linalg.norm(w)
+    qe = np.concatenate([[math.cos(th / 2)], math.sin(th / 2) * ax])
+    assert np.max(np.abs(q - qe)) < 1e-12
+
+
+def test_dyndt_single_wall_contact():
+    # one vertical wall contact: kt=kn/5 so the critical step is sqrt(2 m / kn) (DynDt.cpp:40-58)
+    r, rho, E = .01, 2000., 1e7
+    o = Oracle(damping=0., gravity=(0, 0, 0))
+    mt = o.material(density=rho, young=E, ktDivKn=.2, tanPhi=.5)
+    o.wall((0, 0, 0), 2, 1, mt)
+    o.sphere((0, 0, r * (1 - 1e-4)), r, mt)
+    o.set_params(dt=.1 * o.pwave_dt())
+    o.run(["Collider", "ContactLoop"], 1)
+    mass = o.get("mass")[1]
+    kn = E * math.pi * r / 2
+    assert o.crit_dt() == pytest.approx(math.sqrt(2 * mass / kn), rel=1e-13)
+
+
+def test_lattice_observables():
+    # simple cubic lattice of touching spheres in a periodic cell: Z = 6, solid fraction -> pi/6
+    m, r = 6, .01
+    a = 2 * r * (1 - 1e-7)
+    L = m * a
+    o = Oracle(periodic=1, cell=(L, L, L), physKind=0, lawKind=0, damping=0., gravity=(0, 0, 0), verletDist=0.)
+    mt = o.material(density=1000., young=1e7)
+    for i in range(m):
+        for j in range(m):
+            for k in range(m):
+                o.sphere(((i + .5) * a, (j + .5) * a, (k + .5) * a), r, mt)
+    o.set_params(dt=.1 * o.pwave_dt())
+    o.run(["Collider", "ContactLoop"], 1)
+    assert o.coord_number(False) == 6.
+    assert o.coord_number(True) == 6.
+    assert len(o.contacts(real_only=True, data=False)[0]) == 3 * m ** 3
+    assert o.solid_fraction((0, 0, 0), (L, L, L)) == pytest.approx(math.pi / 6, rel=1e-6)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_sap_incremental_equals_full_rebuild_and_bruteforce(seed):
+    sc = deposition(n=1200, dense=True, seed=seed, vel_sigma=0.3)
+    a = build_oracle(sc, colliderMode=0)
+    b = build_oracle(sc, colliderMode=1)
+    for k in range(6):
+        a.run(ENG, 15)
+        b.run(ENG, 15)
+        ia, _, _ = a.contacts(data=False)
+        ib, _, _ = b.contacts(data=False)
+        np.testing.assert_array_equal(ia, ib)
+        np.testing.assert_array_equal(a.get("pos"), b.get("pos"))
+    assert a.nRefresh > 1
+    # right after a refresh the potential set is the brute-force enlarged-AABB set plus real contacts
+    real = set(map(tuple, a.contacts(real_only=True, data=False)[0].tolist()))
+    brute = set(map(tuple, a.brute_aabb_pairs().tolist()))
+    pot = set(map(tuple, a.contacts(data=False)[0].tolist()))
+    assert pot == brute | real
+
+
+def test_periodic_celldist_normal_contact():
+    # TestPBCCollisions.testNormalContact (test_pbc.py:162-172)
+    for shift2 in [(0, 0, 0), (1, 2, 3), (6, 8, -4)]:
+        o = Oracle(periodic=1, cell=(1, 1, 1), verletDist=0., gravity=(0, 0, 0))
+        mt = o.material()
+        o.sphere((.5, .5, .5), .2, mt)
+        o.sphere(np.array([.8, .6, .5]) + shift2, .2, mt)
+        o.run(["Collider"], 1)
+        ids, cd, _ = o.contacts(data=False)
+        assert ids.tolist() == [[0, 1]]
+        assert cd.tolist() == [[-s for s in shift2]]
+
+
+def test_periodic_over_half_contact():
+    # TestPBCCollisions.testOverHalfContact (test_pbc.py:135-149)
+    for move in [(0, 0, 0), (13, 14, 15)]:
+        o = Oracle(periodic=1, cell=(1, 1, 1), verletDist=0., gravity=(0, 0, 0))
+        mt = o.material()
+        o.sphere(np.array([.3, .5, .5]) + move, .4, mt)
+        o.sphere((.7, .9, .5), .1, mt)
+        o.run(["Collider"], 1)
+        ids, cd, _ = o.contacts(data=False)
+        assert ids.tolist() == [[0, 1]]
+        assert cd.tolist() == [list(move)]
+
+
+def test_periodic_double_contact_raises():
+    # TestPBCCollisions.testDoubleContact (test_pbc.py:151-160)
+    o = Oracle(periodic=1, cell=(1, 1, 1), verletDist=0., gravity=(0, 0, 0))
+    mt = o.material()
+    o.sphere((.3, .5, .5), .4, mt)
+    o.sphere((.8, .9, .5), .2, mt)
+    with pytest.raises(RuntimeError):
+        o.run(["Collider"], 1)
+
+
+def test_elastic_energy_balance_hertz_periodic():
+    # no damping, en outside (0,1): kinetic energy is recovered after collisions end (1e-3)
+    sc = periodic_hertz(n=64, spacing=1.3, seed=3, vel_sigma=0.2, en=float("nan"))
+    o = build_oracle(sc)
+    e0 = o.kinetic_energy()
+    for _ in range(400):
+        o.run(ENG, 50)
+        if len(o.contacts(real_only=True, data=False)[0]) == 0:
+            break
+    assert len(o.contacts(real_only=True, data=False)[0]) == 0
+    assert o.kinetic_energy() == pytest.approx(e0, rel=1e-3)
+
+
+def test_deterministic_and_openmp_modes_agree():
+    sc = deposition(n=900, dense=True, seed=4, vel_sigma=0.05)
+    a = build_oracle(sc, deterministic=1)
+    b = build_oracle(sc, deterministic=0, nThreads=4)
+    a.run(ENG, 20)
+    b.run(ENG, 20)
+    np.testing.assert_allclose(b.get("pos"), a.get("pos"), rtol=0, atol=1e-12)
+    ia, _, _ = a.contacts(real_only=True, data=False)
+    ib, _, _ = b.contacts(real_only=True, data=False)
+    np.testing.assert_array_equal(ia, ib)
+
+
+def test_column_contact_force_sign():
+    # TestContact.testForceSign (test_core.py:436-453): a resting column touches at (4,5),
+    # and the contact pushes 4 down and 5 up
+    r = .01
+    o = Oracle(damping=.2, gravity=(0, 0, -9.81))
+    mt = o.material(density=1000., young=1e6)
+    for i in range(10):
+        o.sphere((0, 0, r + i * 2 * r * (1 - 1e-3)), r, mt)
+    o.wall((0, 0, 0), 2, 1, mt)
+    o.set_params(dt=.2 * o.pwave_dt())
+    o.run(ENG, 5)
+    o.run(["Collider", "ContactLoop"], 1)
+    ids, _, d = o.contacts(real_only=True)
+    pairs = ids.tolist()
+    assert [4, 5] in pairs
+    k = pairs.index([4, 5])
+    # global normal points from pA to pB (+z); local normal force is compressive (negative)
+    assert d[k, 2] > 0.999 and d[k, 9] < 0

@@ -1,0 +1,285 @@
+"""ctypes binding of the C ABI declared in include/woo_amd.h.
+
+Loading fails loudly when libwoo_amd.so is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwoo_amd.so")
+
+SPHERE, WALL = 0, 1
+FRICTMAT, HERTZMAT = 0, 1
+PHYS_FRICT, PHYS_HERTZ = 0, 1
+LAW_IDEALELPL, LAW_HERTZDMT = 0, 1
+FIELD = dict(pos=0, vel=1, angVel=2, ori=3, force=4, torque=5, mass=6, inertia=7)
+FIELD_W = dict(pos=3, vel=3, angVel=3, ori=4, force=3, torque=3, mass=1, inertia=3)
+ENG = dict(ForceResetter=1, Collider=2, ContactLoop=3, Leapfrog=4, DynDt=5)
+
+# every entry point include/woo_amd.h declares
+EXPORTS = [
+    "wdg_last_error", "wdg_ctx_create", "wdg_ctx_destroy", "wdg_set_field", "wdg_set_materials",
+    "wdg_upload_particles", "wdg_upload", "wdg_download", "wdg_collide", "wdg_contact_loop", "wdg_leapfrog",
+    "wdg_force_reset", "wdg_crit_dt", "wdg_contact_count", "wdg_contact_list", "wdg_observables", "wdg_stats",
+    "wdg_run", "wdg_sync", "wdg_stream",
+]
+
+
+class Field(C.Structure):
+    _fields_ = [("gravity", C.c_double * 3), ("distFactor", C.c_double), ("loneMask", C.c_uint32),
+                ("periodic", C.c_int32), ("cell", C.c_double * 3)]
+
+
+class Material(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("density", C.c_double), ("young", C.c_double), ("ktDivKn", C.c_double),
+                ("tanPhi", C.c_double), ("surfEnergy", C.c_double), ("alpha", C.c_double)]
+
+
+_P = C.POINTER
+
+
+class Particles(C.Structure):
+    _fields_ = [("n", C.c_int64), ("shape", _P(C.c_int32)), ("radius", _P(C.c_double)),
+                ("wallAxis", _P(C.c_int32)), ("wallSense", _P(C.c_int32)), ("material", _P(C.c_int32)),
+                ("mask", _P(C.c_uint32)), ("flags", _P(C.c_uint32)), ("integrated", _P(C.c_uint8)),
+                ("pos", _P(C.c_double)), ("vel", _P(C.c_double)), ("angVel", _P(C.c_double)),
+                ("ori", _P(C.c_double)), ("force", _P(C.c_double)), ("torque", _P(C.c_double)),
+                ("mass", _P(C.c_double)), ("inertia", _P(C.c_double))]
+
+
+class CollideParams(C.Structure):
+    _fields_ = [("verletDist", C.c_double), ("forceInit", C.c_int32)]
+
+
+class ContactParams(C.Structure):
+    _fields_ = [("step", C.c_int64), ("dt", C.c_double), ("physKind", C.c_int32), ("lawKind", C.c_int32),
+                ("applyForces", C.c_int32), ("trsfRenorm", C.c_int32), ("poisson", C.c_double), ("en", C.c_double),
+                ("noSlip", C.c_int32), ("noBreak", C.c_int32), ("noFrict", C.c_int32), ("noAttraction", C.c_int32)]
+
+
+class LeapfrogParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("damping", C.c_double), ("reset", C.c_int32)]
+
+
+class Counters(C.Structure):
+    _fields_ = [("nParticles", C.c_int64), ("nCandidates", C.c_int64), ("nReal", C.c_int64),
+                ("nRefresh", C.c_int64), ("nSteps", C.c_int64), ("verletDist", C.c_double),
+                ("brute", C.c_int32), ("gridCells", C.c_int32 * 3)]
+
+
+class Program(C.Structure):
+    _fields_ = [("nEngines", C.c_int32), ("engines", C.c_int32 * 8), ("collide", CollideParams),
+                ("contact", ContactParams), ("leapfrog", LeapfrogParams), ("dynDtPeriod", C.c_int32),
+                ("maxRelInc", C.c_double), ("dtSafety", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libwoo_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m woo_amd.build` (there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    V, I = C.c_void_p, C.c_int
+    L.wdg_last_error.restype = C.c_char_p
+    L.wdg_ctx_create.argtypes = [I, _P(V)]
+    L.wdg_ctx_destroy.argtypes = [V]
+    L.wdg_set_field.argtypes = [V, _P(Field)]
+    L.wdg_set_materials.argtypes = [V, _P(Material), C.c_int32]
+    L.wdg_upload_particles.argtypes = [V, _P(Particles)]
+    L.wdg_upload.argtypes = [V, C.c_int32, _P(C.c_double)]
+    L.wdg_download.argtypes = [V, C.c_int32, _P(C.c_double)]
+    L.wdg_collide.argtypes = [V, _P(CollideParams)]
+    L.wdg_contact_loop.argtypes = [V, _P(ContactParams)]
+    L.wdg_leapfrog.argtypes = [V, _P(LeapfrogParams)]
+    L.wdg_force_reset.argtypes = [V]
+    L.wdg_crit_dt.argtypes = [V, _P(C.c_double)]
+    L.wdg_contact_count.argtypes = [V, C.c_int32, _P(C.c_int64)]
+    L.wdg_contact_list.argtypes = [V, C.c_int32, _P(C.c_int32), _P(C.c_int32), _P(C.c_double)]
+    L.wdg_observables.argtypes = [V, _P(C.c_double), _P(C.c_double), _P(C.c_double)]
+    L.wdg_stats.argtypes = [V, _P(Counters)]
+    L.wdg_run.argtypes = [V, _P(Program), C.c_int64, _P(C.c_int64), _P(C.c_double)]
+    L.wdg_sync.argtypes = [V]
+    L.wdg_stream.argtypes = [V]
+    L.wdg_stream.restype = V
+    _lib = L
+    return L
+
+
+class WdgError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        raise WdgError(f"woo_amd error {rc}: {lib().wdg_last_error().decode()}")
+
+
+def dptr(a):
+    return a.ctypes.data_as(_P(C.c_double)) if a is not None else None
+
+
+def iptr(a, t=C.c_int32):
+    return a.ctypes.data_as(_P(t)) if a is not None else None
+
+
+class Device:
+    """One engine context on a HIP device (owns all device buffers of a DemField)."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        self.h = C.c_void_p()
+        check(self.L.wdg_ctx_create(device, C.byref(self.h)))
+        self.n = 0
+
+    def close(self):
+        if self.h:
+            self.L.wdg_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_field(self, gravity=(0, 0, 0), distFactor=-1., loneMask=2, periodic=False, cell=(1, 1, 1)):
+        f = Field()
+        f.gravity[:] = list(gravity)
+        f.distFactor = distFactor
+        f.loneMask = loneMask
+        f.periodic = int(bool(periodic))
+        f.cell[:] = list(cell)
+        check(self.L.wdg_set_field(self.h, C.byref(f)))
+
+    def set_materials(self, mats):
+        arr = (Material * max(1, len(mats)))()
+        for i, m in enumerate(mats):
+            arr[i] = Material(**m)
+        check(self.L.wdg_set_materials(self.h, arr, len(mats)))
+
+    def upload_particles(self, shape, pos, mass, inertia, radius=None, wallAxis=None, wallSense=None, material=None,
+                         mask=None, flags=None, integrated=None, vel=None, angVel=None, ori=None, force=None,
+                         torque=None):
+        n = len(shape)
+        keep = []
+
+        def arr(x, dt, w=None):
+            if x is None:
+                return None
+            a = np.ascontiguousarray(x, dtype=dt)
+            if w is not None:
+                a = a.reshape(n, w) if w > 1 else a.reshape(n)
+            keep.append(a)
+            return a
+
+        p = Particles()
+        p.n = n
+        p.shape = iptr(arr(shape, np.int32))
+        p.radius = dptr(arr(radius, np.float64))
+        p.wallAxis = iptr(arr(wallAxis, np.int32))
+        p.wallSense = iptr(arr(wallSense, np.int32))
+        p.material = iptr(arr(material, np.int32))
+        p.mask = iptr(arr(mask, np.uint32), C.c_uint32)
+        p.flags = iptr(arr(flags, np.uint32), C.c_uint32)
+        p.integrated = iptr(arr(integrated, np.uint8), C.c_uint8)
+        p.pos = dptr(arr(pos, np.float64, 3))
+        p.vel = dptr(arr(vel, np.float64, 3))
+        p.angVel = dptr(arr(angVel, np.float64, 3))
+        p.ori = dptr(arr(ori, np.float64, 4))
+        p.force = dptr(arr(force, np.float64, 3))
+        p.torque = dptr(arr(torque, np.float64, 3))
+        p.mass = dptr(arr(mass, np.float64))
+        p.inertia = dptr(arr(inertia, np.float64, 3))
+        check(self.L.wdg_upload_particles(self.h, C.byref(p)))
+        self.n = n
+
+    def get(self, field):
+        w = FIELD_W[field]
+        out = np.empty((self.n, w) if w > 1 else (self.n,), dtype=np.float64)
+        check(self.L.wdg_download(self.h, FIELD[field], dptr(out)))
+        return out
+
+    def set(self, field, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        check(self.L.wdg_upload(self.h, FIELD[field], dptr(a)))
+
+    def collide(self, verletDist=-.05, forceInit=False):
+        p = CollideParams(verletDist=verletDist, forceInit=int(forceInit))
+        check(self.L.wdg_collide(self.h, C.byref(p)))
+
+    def contact_loop(self, step, dt, physKind=PHYS_FRICT, lawKind=LAW_IDEALELPL, applyForces=True, trsfRenorm=100,
+                     poisson=.2, en=float("nan"), noSlip=False, noBreak=False, noFrict=False, noAttraction=True):
+        p = ContactParams(step=step, dt=dt, physKind=physKind, lawKind=lawKind, applyForces=int(applyForces),
+                          trsfRenorm=trsfRenorm, poisson=poisson, en=en, noSlip=int(noSlip), noBreak=int(noBreak),
+                          noFrict=int(noFrict), noAttraction=int(noAttraction))
+        check(self.L.wdg_contact_loop(self.h, C.byref(p)))
+
+    def leapfrog(self, dt, damping=.2, reset=True):
+        p = LeapfrogParams(dt=dt, damping=damping, reset=int(reset))
+        check(self.L.wdg_leapfrog(self.h, C.byref(p)))
+
+    def force_reset(self):
+        check(self.L.wdg_force_reset(self.h))
+
+    def crit_dt(self):
+        d = C.c_double()
+        check(self.L.wdg_crit_dt(self.h, C.byref(d)))
+        return d.value
+
+    def contacts(self, real_only=False, data=True):
+        w = 1 if real_only else 0
+        n = C.c_int64()
+        check(self.L.wdg_contact_count(self.h, w, C.byref(n)))
+        n = n.value
+        ids = np.empty((n, 2), dtype=np.int32)
+        cd = np.empty((n, 3), dtype=np.int32)
+        d = np.empty((n, 16), dtype=np.float64) if data else None
+        check(self.L.wdg_contact_list(self.h, w, iptr(ids), iptr(cd), dptr(d)))
+        return ids, cd, d
+
+    def observables(self, lo, hi):
+        lo = np.asarray(lo, dtype=np.float64)
+        hi = np.asarray(hi, dtype=np.float64)
+        out = np.empty(3, dtype=np.float64)
+        check(self.L.wdg_observables(self.h, dptr(lo), dptr(hi), dptr(out)))
+        return dict(coordNumber=out[0], coordNumberSkipFree=out[1], solidFraction=out[2])
+
+    def stats(self):
+        s = Counters()
+        check(self.L.wdg_stats(self.h, C.byref(s)))
+        return {k: (list(getattr(s, k)) if k == "gridCells" else getattr(s, k)) for k, _ in Counters._fields_}
+
+    def run(self, engines, nSteps, step, dt, collide=None, contact=None, leapfrog=None, dynDtPeriod=100,
+            maxRelInc=1e-4, dtSafety=.9):
+        prog = Program()
+        prog.nEngines = len(engines)
+        for i, e in enumerate(engines):
+            prog.engines[i] = ENG[e] if isinstance(e, str) else e
+        prog.collide = CollideParams(**(collide or dict(verletDist=-.05, forceInit=0)))
+        cp = dict(step=0, dt=dt, physKind=PHYS_FRICT, lawKind=LAW_IDEALELPL, applyForces=1, trsfRenorm=100,
+                  poisson=.2, en=float("nan"), noSlip=0, noBreak=0, noFrict=0, noAttraction=1)
+        cp.update(contact or {})
+        prog.contact = ContactParams(**cp)
+        lp = dict(dt=dt, damping=.2, reset=1)
+        lp.update(leapfrog or {})
+        prog.leapfrog = LeapfrogParams(**lp)
+        prog.dynDtPeriod = dynDtPeriod
+        prog.maxRelInc = maxRelInc
+        prog.dtSafety = dtSafety
+        st = C.c_int64(step)
+        d = C.c_double(dt)
+        check(self.L.wdg_run(self.h, C.byref(prog), nSteps, C.byref(st), C.byref(d)))
+        return st.value, d.value
+
+    def sync(self):
+        check(self.L.wdg_sync(self.h))
+
+    def stream(self):
+        return self.L.wdg_stream(self.h)

@@ -1,0 +1,449 @@
+// Per-particle kernels: deterministic force/torque gather over the sorted
+// contact slots (no float atomics), fused into the leapfrog integrator
+// (Leapfrog.cpp:104-297, spherical integrator, gradV zero), ForceResetter
+// (Leapfrog.cpp:15-26), DynDt stiffness (DynDt.cpp:16-71) and observables
+// (Funcs.cpp:84-114).
+//
+// Gather order for particle i: slots where i is the larger id, in increasing
+// order of the smaller id, then slots where i is the smaller id, in increasing
+// order of the larger id, i.e. increasing order of the other particle's id.
+#include <cstring>
+
+#include "wdg_internal.hpp"
+#include "wdg_math.hpp"
+#include "wdg_physics.hpp"
+
+namespace wdg {
+
+struct GatherArgs {
+  int64_t n;
+  const uint32_t *A, *B, *info, *real;
+  const uint64_t* dist;
+  const double *P0, *P1, *P2, *G0, *G1, *G2;
+  const uint32_t *offA, *offB, *idxB;
+  double L[3];
+};
+
+template <bool PERIODIC>
+__device__ __forceinline__ void gather_one(const GatherArgs& g, uint32_t i, uint32_t slot, d3 pos, d3& F, d3& T) {
+  if (!g.real[slot]) return;
+  const uint32_t inf = g.info[slot];
+  const uint32_t pA = (inf & CI_SWAP) ? g.B[slot] : g.A[slot];
+  const bool isPA = (pA == i);
+  const double sign = isPA ? 1. : -1.;
+  d3 Fc = mul3(mk3(g.G0[slot], g.G1[slot], g.G2[slot]), sign);
+  d3 shift = mk3(0., 0., 0.);
+  if (PERIODIC && !isPA) {
+    uint64_t cd = g.dist[slot];
+    shift = mk3(g.L[0] * unpack_cd(cd, 0), g.L[1] * unpack_cd(cd, 1), g.L[2] * unpack_cd(cd, 2));
+  }
+  d3 xc = sub3(mk3(g.P0[slot], g.P1[slot], g.P2[slot]), add3(pos, shift));
+  F = add3(F, Fc);
+  T = add3(T, cross3(xc, Fc));
+}
+
+template <bool PERIODIC>
+__device__ __forceinline__ void gather_all(const GatherArgs& g, uint32_t i, d3 pos, d3& F, d3& T) {
+  const uint32_t b0 = g.offB[i], b1 = g.offB[i + 1];
+  for (uint32_t k = b0; k < b1; k++) gather_one<PERIODIC>(g, i, g.idxB[k], pos, F, T);
+  const uint32_t a0 = g.offA[i], a1 = g.offA[i + 1];
+  for (uint32_t s = a0; s < a1; s++) gather_one<PERIODIC>(g, i, s, pos, F, T);
+}
+
+template <bool PERIODIC>
+__global__ __launch_bounds__(256) void k_gather(GatherArgs g, double* fx, double* fy, double* fz, double* tx,
+                                                 double* ty, double* tz, const double* __restrict__ px,
+                                                 const double* __restrict__ py, const double* __restrict__ pz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  if (g.offB[i] == g.offB[i + 1] && g.offA[i] == g.offA[i + 1]) return;
+  d3 F = mk3(fx[i], fy[i], fz[i]), T = mk3(tx[i], ty[i], tz[i]);
+  gather_all<PERIODIC>(g, (uint32_t)i, mk3(px[i], py[i], pz[i]), F, T);
+  fx[i] = F.x; fy[i] = F.y; fz[i] = F.z;
+  tx[i] = T.x; ty[i] = T.y; tz[i] = T.z;
+}
+
+struct LeapArgs {
+  int64_t n;
+  double *px, *py, *pz, *vx, *vy, *vz, *wx, *wy, *wz, *qw, *qx, *qy, *qz, *fx, *fy, *fz, *tx, *ty, *tz;
+  const double *mass, *ix, *iy, *iz;
+  const double *lx, *ly, *lz;
+  const uint32_t* flags;
+  double dt, damping;
+  int reset;
+  double g[3];
+  int hasGravity;
+  int gather;
+  double maxD2;
+  int checkDirty;
+  uint32_t* dflags;
+};
+
+template <bool PERIODIC>
+__global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool dirty = false;
+  if (i < a.n) {
+    const uint32_t fl = a.flags[i];
+    const bool integrated = (fl & F_INTEGRATED) != 0u;
+    d3 pos = mk3(a.px[i], a.py[i], a.pz[i]);
+    d3 F = mk3(a.fx[i], a.fy[i], a.fz[i]), T = mk3(a.tx[i], a.ty[i], a.tz[i]);
+    if (a.gather) gather_all<PERIODIC>(g, (uint32_t)i, pos, F, T);
+    if (!integrated) {
+      if (a.gather) {
+        a.fx[i] = F.x; a.fy[i] = F.y; a.fz[i] = F.z;
+        a.tx[i] = T.x; a.ty[i] = T.y; a.tz[i] = T.z;
+      }
+    } else {
+      const double dt = a.dt, damping = a.damping;
+      const bool damp = (damping != 0. && !(fl & F_DAMPING_SKIP));
+      const uint32_t bl = fl & F_DOF_ALL;
+      const double m = a.mass[i];
+      d3 vel = mk3(a.vx[i], a.vy[i], a.vz[i]);
+      d3 av = mk3(a.wx[i], a.wy[i], a.wz[i]);
+      if (bl != F_DOF_ALL) {
+        double f[3] = {F.x, F.y, F.z}, t[3] = {T.x, T.y, T.z};
+        double v[3] = {vel.x, vel.y, vel.z}, w[3] = {av.x, av.y, av.z};
+        double I[3] = {a.ix[i], a.iy[i], a.iz[i]};
+        double la[3], aa[3];
+        for (int k = 0; k < 3; k++) {
+          if (bl == 0) la[k] = f[k] / m;
+          else la[k] = (bl & (1u << k)) ? 0. : 0. + f[k] / m;
+        }
+        if (damp)
+          for (int k = 0; k < 3; k++) la[k] *= 1 - damping * sgn(f[k] * (v[k] + 0.5 * dt * la[k]));
+        for (int k = 0; k < 3; k++) v[k] += dt * la[k];
+        if (!(I[0] == 0 && I[1] == 0 && I[2] == 0)) {
+          for (int k = 0; k < 3; k++) {
+            if (bl == 0) aa[k] = t[k] / I[k];
+            else aa[k] = (bl & (8u << k)) ? 0. : 0. + t[k] / I[k];
+          }
+          if (damp)
+            for (int k = 0; k < 3; k++) aa[k] *= 1 - damping * sgn(t[k] * (w[k] + 0.5 * dt * aa[k]));
+          for (int k = 0; k < 3; k++) w[k] += dt * aa[k];
+        }
+        vel = mk3(v[0], v[1], v[2]);
+        av = mk3(w[0], w[1], w[2]);
+      }
+      pos = mk3(pos.x + vel.x * dt, pos.y + vel.y * dt, pos.z + vel.z * dt);
+      // leapfrogSphericalRotate (Leapfrog.cpp:280-297)
+      q4 ori = q4{a.qw[i], a.qx[i], a.qy[i], a.qz[i]};
+      if (!(av.x == 0 && av.y == 0 && av.z == 0)) {
+        double angle = sqrt(sqn3(av));
+        d3 axis = div3(av, angle);
+        double ha = 0.5 * (angle * dt);
+        double sh = sin(ha);
+        q4 q = q4{cos(ha), sh * axis.x, sh * axis.y, sh * axis.z};
+        ori = quat_mul(q, ori);
+      }
+      double qn = sqrt(ori.w * ori.w + ori.x * ori.x + ori.y * ori.y + ori.z * ori.z);
+      a.qw[i] = ori.w / qn; a.qx[i] = ori.x / qn; a.qy[i] = ori.y / qn; a.qz[i] = ori.z / qn;
+      a.px[i] = pos.x; a.py[i] = pos.y; a.pz[i] = pos.z;
+      a.vx[i] = vel.x; a.vy[i] = vel.y; a.vz[i] = vel.z;
+      a.wx[i] = av.x; a.wy[i] = av.y; a.wz[i] = av.z;
+      if (a.reset) {
+        const bool grav = a.hasGravity && !(fl & F_GRAVITY_SKIP);
+        F = grav ? mk3(m * a.g[0], m * a.g[1], m * a.g[2]) : mk3(0., 0., 0.);
+        T = mk3(0., 0., 0.);
+      }
+      a.fx[i] = F.x; a.fy[i] = F.y; a.fz[i] = F.z;
+      a.tx[i] = T.x; a.ty[i] = T.y; a.tz[i] = T.z;
+      if (a.checkDirty) {
+        d3 d = sub3(mk3(a.lx[i], a.ly[i], a.lz[i]), pos);
+        double d2 = 0;
+        d2 = fmax(d2, sqn3(d));
+        dirty = d2 + 0. > a.maxD2;
+      }
+    }
+  }
+  if (__ballot(dirty) != 0ull && (threadIdx.x & 63) == 0) atomicOr(&a.dflags[0], 1u);
+}
+
+__global__ void k_force_reset(int64_t n, const uint32_t* __restrict__ flags, const double* __restrict__ mass,
+                              double g0, double g1, double g2, int hasGravity, double* fx, double* fy, double* fz,
+                              double* tx, double* ty, double* tz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t fl = flags[i];
+  if (!(fl & F_INTEGRATED)) return;
+  const double m = mass[i];
+  const bool grav = hasGravity && !(fl & F_GRAVITY_SKIP);
+  fx[i] = grav ? m * g0 : 0.;
+  fy[i] = grav ? m * g1 : 0.;
+  fz[i] = grav ? m * g2 : 0.;
+  tx[i] = 0.; ty[i] = 0.; tz[i] = 0.;
+}
+
+static GatherArgs gather_args(Ctx& c) {
+  Contacts& C = c.con();
+  GatherArgs g{};
+  g.n = c.n;
+  g.A = C.A.p; g.B = C.B.p; g.info = C.info.p; g.real = C.real.p; g.dist = C.dist.p;
+  g.P0 = C.hP[0].p; g.P1 = C.hP[1].p; g.P2 = C.hP[2].p;
+  g.G0 = C.F[0].p; g.G1 = C.F[1].p; g.G2 = C.F[2].p;
+  g.offA = C.offA.p; g.offB = C.offB.p; g.idxB = C.idxB.p;
+  for (int k = 0; k < 3; k++) g.L[k] = c.field.cell[k];
+  return g;
+}
+
+void flush_forces(Ctx& c) {
+  if (!c.forcesPending) return;
+  c.forcesPending = false;
+  if (c.con().n == 0 || c.n == 0) return;
+  GatherArgs g = gather_args(c);
+  dim3 grid(nblocks(c.n, 256)), block(256);
+  if (c.field.periodic)
+    hipLaunchKernelGGL(k_gather<true>, grid, block, 0, c.stream, g, c.force[0].p, c.force[1].p, c.force[2].p,
+                       c.torque[0].p, c.torque[1].p, c.torque[2].p, c.pos[0].p, c.pos[1].p, c.pos[2].p);
+  else
+    hipLaunchKernelGGL(k_gather<false>, grid, block, 0, c.stream, g, c.force[0].p, c.force[1].p, c.force[2].p,
+                       c.torque[0].p, c.torque[1].p, c.torque[2].p, c.pos[0].p, c.pos[1].p, c.pos[2].p);
+  WDG_LAUNCH_CHECK();
+}
+
+void leapfrog(Ctx& c, const wdg_leapfrog_params* p) {
+  if (c.n == 0) return;
+  LeapArgs a{};
+  a.n = c.n;
+  a.px = c.pos[0].p; a.py = c.pos[1].p; a.pz = c.pos[2].p;
+  a.vx = c.vel[0].p; a.vy = c.vel[1].p; a.vz = c.vel[2].p;
+  a.wx = c.angVel[0].p; a.wy = c.angVel[1].p; a.wz = c.angVel[2].p;
+  a.qw = c.ori[0].p; a.qx = c.ori[1].p; a.qy = c.ori[2].p; a.qz = c.ori[3].p;
+  a.fx = c.force[0].p; a.fy = c.force[1].p; a.fz = c.force[2].p;
+  a.tx = c.torque[0].p; a.ty = c.torque[1].p; a.tz = c.torque[2].p;
+  a.mass = c.mass.p; a.ix = c.inertia[0].p; a.iy = c.inertia[1].p; a.iz = c.inertia[2].p;
+  a.lx = c.lastPos[0].p; a.ly = c.lastPos[1].p; a.lz = c.lastPos[2].p;
+  a.flags = c.flags.p;
+  a.dt = p->dt; a.damping = p->damping; a.reset = p->reset;
+  for (int k = 0; k < 3; k++) a.g[k] = c.field.gravity[k];
+  a.hasGravity = !(c.field.gravity[0] == 0 && c.field.gravity[1] == 0 && c.field.gravity[2] == 0);
+  a.gather = (c.forcesPending && c.con().n > 0) ? 1 : 0;
+  a.checkDirty = c.boundsValid && c.verletDist > 0 ? 1 : 0;
+  a.maxD2 = c.verletDist * c.verletDist;
+  a.dflags = c.dflags.p;
+  if (a.checkDirty) {
+    // the flag must describe the positions after this step only
+    WDG_HIP(hipMemsetAsync(c.dflags.p, 0, sizeof(uint32_t), c.stream));
+  }
+  GatherArgs g = gather_args(c);
+  dim3 grid(nblocks(c.n, 256)), block(256);
+  if (c.field.periodic) hipLaunchKernelGGL(k_leapfrog<true>, grid, block, 0, c.stream, a, g);
+  else hipLaunchKernelGGL(k_leapfrog<false>, grid, block, 0, c.stream, a, g);
+  WDG_LAUNCH_CHECK();
+  c.forcesPending = false;
+}
+
+void force_reset(Ctx& c) {
+  if (c.n == 0) return;
+  if (c.forcesPending) flush_forces(c);
+  hipLaunchKernelGGL(k_force_reset, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, c.n, c.flags.p, c.mass.p,
+                     c.field.gravity[0], c.field.gravity[1], c.field.gravity[2],
+                     (int)!(c.field.gravity[0] == 0 && c.field.gravity[1] == 0 && c.field.gravity[2] == 0),
+                     c.force[0].p, c.force[1].p, c.force[2].p, c.torque[0].p, c.torque[1].p, c.torque[2].p);
+  WDG_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ DynDt
+struct DynArgs {
+  int64_t n;
+  const uint32_t *A, *B, *info, *real;
+  const double *N0, *N1, *N2, *Y0, *Y1, *Y2, *uN;
+  const uint32_t *offA, *offB, *idxB;
+  const double *rad, *mass, *ix, *iy, *iz;
+  const uint32_t* flags;
+  const int32_t* mat;
+  const wdg_material* mats;
+  int hertz;
+  double poisson, en, viscAlpha;
+  unsigned long long* out;
+};
+
+__device__ __forceinline__ void dyn_one(const DynArgs& a, uint32_t i, uint32_t s, d3& kt, d3& kr) {
+  if (!a.real[s]) return;
+  const uint32_t inf = a.info[s];
+  const bool swap = (inf & CI_SWAP) != 0u;
+  const uint32_t pA = swap ? a.B[s] : a.A[s], pB = swap ? a.A[s] : a.B[s];
+  const int ix = (pA == i) ? 0 : 1;
+  double r1, r2;
+  if ((inf & CI_TYPE_MASK) == CT_SS) { r1 = a.rad[pA]; r2 = a.rad[pB]; }
+  else { r1 = -a.rad[pB]; r2 = a.rad[pB]; }
+  double l0, l1, A;
+  lens_area(r1, r2, l0, l1, A);
+  const wdg_material mA = a.mats[a.mat[pA]], mB = a.mats[a.mat[pB]];
+  PhysVals ph;
+  double kn, ktt;
+  if (a.hertz) {
+    phys_hertz(mA, mB, l0, l1, a.mass[pA], a.mass[pB], a.poisson, a.en, a.viscAlpha, ph);
+    double kn0 = ph.K * sqrt(ph.R);
+    double uN = a.uN[s];
+    kn = (3 / 2.) * kn0 * sqrt(-uN);
+    ktt = ph.kt0 * sqrt(uN < 0 ? -uN : 0);
+  } else {
+    phys_frict(mA, mB, l0, l1, A, ph);
+    kn = ph.kn;
+    ktt = ph.kt;
+  }
+  d3 T0 = mk3(a.N0[s], a.N1[s], a.N2[s]), T1 = mk3(a.Y0[s], a.Y1[s], a.Y2[s]);
+  q4 q = quat_from_cols(T0, T1, cross3(T0, T1));
+  d3 n = quat_rot(q, mk3(1., 0., 0.));
+  d3 n2 = mk3(n.x * n.x, n.y * n.y, n.z * n.z);
+  kt = mk3(kt.x + (n2.x * (kn - ktt) + ktt), kt.y + (n2.y * (kn - ktt) + ktt), kt.z + (n2.z * (kn - ktt) + ktt));
+  double l = ix == 0 ? l0 : l1;
+  double lk = (l * l) * ktt;
+  kr = mk3(kr.x + lk * (n2.y + n2.z), kr.y + lk * (n2.z + n2.x), kr.z + lk * (n2.x + n2.y));
+}
+
+__global__ __launch_bounds__(256) void k_dyndt(DynArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double ret = INFINITY;
+  if (i < a.n) {
+    const uint32_t fl = a.flags[i];
+    if ((fl & F_INTEGRATED) && (fl & F_DOF_ALL) != F_DOF_ALL) {
+      d3 kt = mk3(0., 0., 0.), kr = mk3(0., 0., 0.);
+      for (uint32_t k = a.offB[i]; k < a.offB[i + 1]; k++) dyn_one(a, (uint32_t)i, a.idxB[k], kt, kr);
+      for (uint32_t s = a.offA[i]; s < a.offA[i + 1]; s++) dyn_one(a, (uint32_t)i, s, kt, kr);
+      const double m = a.mass[i];
+      const double I[3] = {a.ix[i], a.iy[i], a.iz[i]};
+      const double ktv[3] = {kt.x, kt.y, kt.z}, krv[3] = {kr.x, kr.y, kr.z};
+      for (int k = 0; k < 3; k++)
+        if (ktv[k] != 0 && m > 0. && !(fl & (1u << k))) ret = fmin(ret, m / fabs(ktv[k]));
+      for (int k = 0; k < 3; k++)
+        if (krv[k] != 0 && I[k] > 0. && !(fl & (8u << k))) ret = fmin(ret, I[k] / fabs(krv[k]));
+      ret = 2 * ret;
+    }
+  }
+  // non-negative doubles order like their bit patterns
+  unsigned long long b = (unsigned long long)__double_as_longlong(ret);
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(b, off);
+    b = o < b ? o : b;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMin(a.out, b);
+}
+
+double crit_dt(Ctx& c) {
+  if (c.n == 0) return INFINITY;
+  Contacts& C = c.con();
+  if (C.n == 0) return INFINITY;
+  DynArgs a{};
+  a.n = c.n;
+  a.A = C.A.p; a.B = C.B.p; a.info = C.info.p; a.real = C.real.p;
+  a.N0 = C.hN[0].p; a.N1 = C.hN[1].p; a.N2 = C.hN[2].p;
+  a.Y0 = C.hY[0].p; a.Y1 = C.hY[1].p; a.Y2 = C.hY[2].p; a.uN = C.huN.p;
+  a.offA = C.offA.p; a.offB = C.offB.p; a.idxB = C.idxB.p;
+  a.rad = c.radius.p; a.mass = c.mass.p; a.ix = c.inertia[0].p; a.iy = c.inertia[1].p; a.iz = c.inertia[2].p;
+  a.flags = c.flags.p; a.mat = c.mat.p; a.mats = c.dMats.p;
+  a.hertz = c.lastPhysKind == WDG_PHYS_HERTZ;
+  a.poisson = c.lastPoisson; a.en = c.lastEn; a.viscAlpha = c.lastViscAlpha;
+  unsigned long long* out = reinterpret_cast<unsigned long long*>(c.red.p);
+  unsigned long long init = (unsigned long long)0x7FF0000000000000ull;  // +inf
+  WDG_HIP(hipMemcpyAsync(out, &init, sizeof(init), hipMemcpyHostToDevice, c.stream));
+  a.out = out;
+  hipLaunchKernelGGL(k_dyndt, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, a);
+  WDG_LAUNCH_CHECK();
+  unsigned long long hb;
+  WDG_HIP(hipMemcpyAsync(&hb, out, sizeof(hb), hipMemcpyDeviceToHost, c.stream));
+  WDG_HIP(hipStreamSynchronize(c.stream));
+  double r;
+  std::memcpy(&r, &hb, 8);
+  return std::sqrt(r);
+}
+
+// ------------------------------------------------------------------ observables
+__global__ void k_count_contacts(int64_t n, const uint32_t* __restrict__ real, const uint32_t* __restrict__ offA,
+                                 const uint32_t* __restrict__ offB, const uint32_t* __restrict__ idxB,
+                                 uint32_t* cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t k = 0;
+  for (uint32_t j = offB[i]; j < offB[i + 1]; j++) k += real[idxB[j]];
+  for (uint32_t s = offA[i]; s < offA[i + 1]; s++) k += real[s];
+  cnt[i] = k;
+}
+
+// block partial sums in fixed order (deterministic): [C2, N0, N1, N, Vs]
+__global__ __launch_bounds__(256) void k_obs_partial(int64_t n, const uint32_t* __restrict__ cnt,
+                                                     const uint32_t* __restrict__ flags, const double* __restrict__ rad,
+                                                     const double* __restrict__ px, const double* __restrict__ py,
+                                                     const double* __restrict__ pz, double lo0, double lo1, double lo2,
+                                                     double hi0, double hi1, double hi2, double* part) {
+  __shared__ double s[5][256];
+  const int tid = threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
+  double v[5] = {0, 0, 0, 0, 0};
+  if (i < n) {
+    uint32_t k = cnt[i];
+    v[0] = k;
+    v[1] = k == 0;
+    v[2] = k == 1;
+    v[3] = 1;
+    if (flag_shape(flags[i]) == WDG_SPHERE) {
+      double x = px[i], y = py[i], z = pz[i];
+      bool in = x >= lo0 && x <= hi0 && y >= lo1 && y <= hi1 && z >= lo2 && z <= hi2;
+      double r = rad[i];
+      if (in) v[4] = (4 / 3.) * M_PI * (r * r * r);
+    }
+  }
+  for (int q = 0; q < 5; q++) s[q][tid] = v[q];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st)
+      for (int q = 0; q < 5; q++) s[q][tid] += s[q][tid + st];
+    __syncthreads();
+  }
+  if (tid == 0)
+    for (int q = 0; q < 5; q++) part[(int64_t)q * gridDim.x + blockIdx.x] = s[q][0];
+}
+
+void observables(Ctx& c, const double lo[3], const double hi[3], double out[3]) {
+  if (c.forcesPending) flush_forces(c);
+  Contacts& C = c.con();
+  c.scount.ensure(c.n + 1);
+  if (C.n > 0 && c.n > 0) {
+    hipLaunchKernelGGL(k_count_contacts, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, c.n, C.real.p, C.offA.p,
+                       C.offB.p, C.idxB.p, c.scount.p);
+    WDG_LAUNCH_CHECK();
+  } else {
+    WDG_HIP(hipMemsetAsync(c.scount.p, 0, (c.n + 1) * sizeof(uint32_t), c.stream));
+  }
+  unsigned nb = nblocks(c.n, 256);
+  std::vector<double> part((size_t)5 * nb);
+  DBuf<double> dpart;
+  dpart.ensure(part.size());
+  hipLaunchKernelGGL(k_obs_partial, dim3(nb), dim3(256), 0, c.stream, c.n, c.scount.p, c.flags.p, c.radius.p,
+                     c.pos[0].p, c.pos[1].p, c.pos[2].p, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], dpart.p);
+  WDG_LAUNCH_CHECK();
+  WDG_HIP(hipMemcpyAsync(part.data(), dpart.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  WDG_HIP(hipStreamSynchronize(c.stream));
+  dpart.release();
+  double S[5] = {0, 0, 0, 0, 0};
+  for (int q = 0; q < 5; q++)
+    for (unsigned b = 0; b < nb; b++) S[q] += part[(size_t)q * nb + b];
+  const double C2 = S[0], N0 = S[1], N1 = S[2], N = S[3], Vs = S[4];
+  out[0] = C2 / N;
+  out[1] = (C2 - N1) / (N - N0 - N1);
+  double V = ((hi[0] - lo[0]) * (hi[1] - lo[1])) * (hi[2] - lo[2]);
+  out[2] = Vs / V;
+}
+
+__global__ void k_sum_real(int64_t nc, const uint32_t* __restrict__ real, unsigned long long* out) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long v = s < nc ? real[s] : 0u;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
+}
+
+int64_t count_real(Ctx& c) {
+  Contacts& C = c.con();
+  if (C.n == 0) return 0;
+  unsigned long long* out = reinterpret_cast<unsigned long long*>(c.red.p);
+  WDG_HIP(hipMemsetAsync(out, 0, sizeof(unsigned long long), c.stream));
+  hipLaunchKernelGGL(k_sum_real, dim3(nblocks(C.n, 256)), dim3(256), 0, c.stream, C.n, C.real.p, out);
+  WDG_LAUNCH_CHECK();
+  unsigned long long h;
+  WDG_HIP(hipMemcpyAsync(&h, out, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+  WDG_HIP(hipStreamSynchronize(c.stream));
+  return (int64_t)h;
+}
+
+}  // namespace wdg
