@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""DEM step benchmark on one MI355X: 1,000,000 polydisperse Hertz spheres.
+
+Workload: radius ~ U[2,4] mm, periodic cube filled to solid fraction 0.55,
+velocities N(0, 0.1 m/s), HertzMat(young=1e8, tanPhi=0.4, density=3000),
+Cp2_HertzMat_HertzPhys(poisson=0.25, en=0.6), Law2_L6Geom_HertzPhys_DMT, no gravity,
+dt = 0.3 pWaveDt, engines [InsertionSortCollider, ContactLoop, Leapfrog(damping=0)],
+plain FP64. One step = one pass of those engines over all particles.
+
+Prints one JSON line: whole-job particle-steps/s over K timed steps (inputs resident
+in HBM), the roofline of the dominant kernel (HIP-event timed on the engine's stream),
+and the CPU oracle timed on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+METRIC = "particle-steps/s, 1M Hertz spheres on 1 MI355X; % HBM roofline; speedup vs FP64"
+RHO, YOUNG, TANPHI, POISSON, EN = 3000., 1e8, .4, .25, .6
+ENGINES = ["Collider", "ContactLoop", "Leapfrog"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class HipEvents:
+    """hipEvent timing on an arbitrary stream (the engine's own)."""
+
+    def __init__(self):
+        self.L = C.CDLL("libamdhip64.so")
+        self.L.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.L.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.L.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.L.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.L.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def new(self):
+        e = C.c_void_p()
+        assert self.L.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.L.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b):
+        self.L.hipEventSynchronize(b)
+        t = C.c_float()
+        assert self.L.hipEventElapsedTime(C.byref(t), a, b) == 0
+        return t.value
+
+
+def make_workload(n, dev, seed=2):
+    from woo_amd.pack import periodic_pack
+    from woo_amd._native import SPHERE
+
+    t0 = time.perf_counter()
+    pos, radius, L, info = periodic_pack(n, 2e-3, 4e-3, 0.55, seed=seed, dev=dev)
+    log(f"packing: n={n} L={L:.4f} m relax iterations={info['iterations']} max overlap={info['maxOverlap']:.3e} "
+        f"({time.perf_counter() - t0:.1f} s)")
+    rng = np.random.default_rng(3)
+    vel = rng.normal(0., 0.1, size=(n, 3))
+    mass = (4 / 3.) * math.pi * (radius * radius * radius) * RHO
+    inertia = np.repeat((0. + 1. * (2. / 5.) * mass * (radius * radius))[:, None], 3, 1)
+    dt = 0.3 * float(np.min(radius)) / math.sqrt(YOUNG / RHO)
+    mat = dict(kind=1, density=RHO, young=YOUNG, ktDivKn=.2, tanPhi=TANPHI, surfEnergy=0., alpha=0.)
+    dev.set_field(periodic=True, cell=(L, L, L))
+    dev.set_materials([mat])
+    dev.upload_particles(shape=np.full(n, SPHERE, np.int32), pos=pos, radius=radius, mass=mass, inertia=inertia,
+                         vel=vel)
+    return dict(pos=pos, radius=radius, vel=vel, mass=mass, inertia=inertia, L=L, dt=dt, mat=mat)
+
+
+def contact_params():
+    from woo_amd._native import PHYS_HERTZ, LAW_HERTZDMT
+
+    return dict(physKind=PHYS_HERTZ, lawKind=LAW_HERTZDMT, poisson=POISSON, en=EN)
+
+
+def algorithmic_bytes(n, slots, real):
+    """Compulsory HBM bytes per launch (FP64, periodic), per kernel.
+
+    k_contact: per slot info+real+ids+cellDist (24 B); per particle pos, vel, angVel,
+      radius, mass, material (92 B); per real contact history read (88 B), history write
+      (104 B) and global force write (24 B).
+    k_leapfrog (gather fused): per particle flags, CSR offsets, pos/vel/angVel/ori/force/
+      torque read+write, mass, inertia, refresh position (372 B); per slot transposed index
+      and two real flags (12 B); per real contact, both sides: info, id, cellDist, contact
+      point and global force (120 B).
+    """
+    return {
+        "k_contact": slots * 24 + n * 92 + real * 216,
+        "k_leapfrog": n * 372 + slots * 12 + real * 120,
+    }
+
+
+def time_kernels(dev, wl, step, dt, k):
+    """Average device time of the contact and the (fused gather) leapfrog launches."""
+    ev = HipEvents()
+    s = dev.stream()
+    cp = contact_params()
+    e = [ev.new() for _ in range(4)]
+    tc = tl = 0.
+    for i in range(k):
+        dev.collide(-.05)
+        ev.record(e[0], s)
+        dev.contact_loop(step, dt, **cp)
+        ev.record(e[1], s)
+        ev.record(e[2], s)
+        dev.leapfrog(dt, 0., True)
+        ev.record(e[3], s)
+        tc += ev.elapsed_ms(e[0], e[1])
+        tl += ev.elapsed_ms(e[2], e[3])
+        step += 1
+    return tc / k * 1e-3, tl / k * 1e-3, step
+
+
+def cpu_baseline(wl, n_sample_steps_max=20, budget_s=15.):
+    from oracle import Oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    threads = min(threads, 16)
+    n = len(wl["radius"])
+    o = Oracle(dt=wl["dt"], gravity=(0, 0, 0), periodic=1, cell=(wl["L"],) * 3, physKind=1, lawKind=1,
+               damping=0., poisson=POISSON, en=EN, deterministic=0, nThreads=threads)
+    m = wl["mat"]
+    o.material(**m)
+    o.spheres(wl["pos"], wl["radius"], 0)
+    o.set("vel", wl["vel"])
+    o.set("mass", wl["mass"])
+    o.set("inertia", wl["inertia"])
+    t0 = time.perf_counter()
+    o.run(ENGINES, 1)  # initial sort + contact creation, excluded like Woo's t1
+    t_first = time.perf_counter() - t0
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < n_sample_steps_max and (time.perf_counter() - t0) < budget_s:
+        o.run(ENGINES, 1)
+        steps += 1
+    el = time.perf_counter() - t0
+    return dict(value=n * steps / el, unit="particle-steps/s", cores=threads, kind="port",
+                sample=f"{steps} steps of the same {n}-sphere workload after 1 untimed first step "
+                       f"({t_first:.1f} s), OpenMP oracle (per-node mutex force accumulation), "
+                       f"{el:.1f} s wall")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--kernel-steps", type=int, default=50)
+    ap.add_argument("--cpu-baseline", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from woo_amd._native import Device
+
+    dev = Device(0)
+    wl = make_workload(args.n, dev)
+    dt = wl["dt"]
+    step = 0
+    cp = contact_params()
+    log(f"dt={dt:.4e} s, warmup {args.warmup} steps")
+    step, _ = dev.run(ENGINES, args.warmup, step, dt, contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
+    dev.sync()
+    st0 = dev.stats()
+    log(f"after warmup: {st0}")
+    t0 = time.perf_counter()
+    step, _ = dev.run(ENGINES, args.steps, step, dt, contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
+    dev.sync()
+    el = time.perf_counter() - t0
+    st1 = dev.stats()
+    value = args.n * args.steps / el
+    log(f"timed {args.steps} steps in {el:.3f} s: {value:.4e} particle-steps/s; refreshes {st1['nRefresh'] - st0['nRefresh']}")
+
+    # per-kernel device time on the engine's stream
+    tc, tl, step = time_kernels(dev, wl, step, dt, args.kernel_steps)
+    st2 = dev.stats()
+    B = algorithmic_bytes(args.n, st2["nCandidates"], st2["nReal"])
+    times = {"k_contact": tc, "k_leapfrog": tl}
+    dom = max(times, key=times.get)
+    achieved = B[dom] / times[dom]
+    log(f"kernel times: contact {tc * 1e6:.1f} us ({B['k_contact'] / tc / 1e9:.0f} GB/s), "
+        f"leapfrog+gather {tl * 1e6:.1f} us ({B['k_leapfrog'] / tl / 1e9:.0f} GB/s); slots {st2['nCandidates']}, "
+        f"real {st2['nReal']}")
+    roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
+                traffic=None, kernel=dom, bytes_per_launch=B[dom], launch_us=times[dom] * 1e6,
+                other_kernels={k: dict(bytes_per_launch=B[k], launch_us=times[k] * 1e6,
+                                       achieved_GBs=B[k] / times[k] / 1e9) for k in times if k != dom})
+    base = None
+    if args.cpu_baseline:
+        log("timing the CPU oracle on the same workload ...")
+        base = cpu_baseline(wl)
+        log(f"cpu baseline: {base}")
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: seeded polydisperse periodic packing at solid fraction 0.55, Gaussian velocities",
+        "config": {"workload": "1M polydisperse Hertz spheres, periodic cube, phi=0.55, FP64 (Collider, ContactLoop, "
+                               "Leapfrog)", "n": args.n, "dt": dt, "cell": wl["L"],
+                   "contact_slots": st2["nCandidates"], "real_contacts": st2["nReal"],
+                   "refreshes_in_timed_region": st1["nRefresh"] - st0["nRefresh"]},
+        "roofline": roof,
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+    dev.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -173,6 +173,12 @@ int wdg_stats(wdg_ctx* ctx, wdg_counters* out);
  * scene's step counter and timestep, updated in place (DynDt may change dt). */
 int wdg_run(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t* step, double* dt);
 
+/* Initial-condition generation (the role of SpherePack::makeCloud / pack.randomPeriPack,
+ * src/supp/sphere-pack/SpherePack.cpp:140-247, py/pack.py:572): pushes overlapping spheres
+ * apart (radii inflated by 1+inflate while relaxing) until the largest overlap is <= tol
+ * or maxIter sweeps ran. Drops all contact state afterwards. */
+int wdg_relax(wdg_ctx* ctx, int32_t maxIter, double inflate, double tol, double* maxOverlap, int32_t* iters);
+
 int wdg_sync(wdg_ctx* ctx);
 /* HIP stream the engine launches on (hipStream_t), for callers that time it with events */
 void* wdg_stream(wdg_ctx* ctx);

@@ -1085,6 +1085,13 @@ int orc_add_sphere(orc_scene* s, const double pos[3], double radius, int mat, in
   return (int)s->par.size() - 1;
 }
 
+int orc_add_spheres(orc_scene* s, int64_t n, const double* pos, const double* radius, int mat, uint32_t mask) {
+  int first = (int)s->par.size();
+  s->par.reserve(s->par.size() + n);
+  for (int64_t i = 0; i < n; i++) orc_add_sphere(s, pos + 3 * i, radius[i], mat, 0, mask);
+  return first;
+}
+
 // woo.utils.wall (utils.py:245): fixed, zero mass
 int orc_add_wall(orc_scene* s, const double pos[3], int axis, int sense, int mat, uint32_t mask) {
   Particle p;

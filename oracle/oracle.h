@@ -84,6 +84,8 @@ int orc_add_material(orc_scene* s, int kind, double density, double young, doubl
 /* returns particle id */
 int orc_add_sphere(orc_scene* s, const double pos[3], double radius, int mat, int fixed, uint32_t mask);
 int orc_add_wall(orc_scene* s, const double pos[3], int axis, int sense, int mat, uint32_t mask);
+/* bulk: n free spheres (pos n*3, radius n), one material and mask; returns the first id */
+int orc_add_spheres(orc_scene* s, int64_t n, const double* pos, const double* radius, int mat, uint32_t mask);
 
 int orc_num_particles(const orc_scene* s);
 /* bulk node data (one node per particle), n*3 or n*4 or n doubles */

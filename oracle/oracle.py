@@ -52,6 +52,7 @@ def _load():
     L.orc_add_material.argtypes = [P, C.c_int] + [C.c_double] * 6; L.orc_add_material.restype = C.c_int
     L.orc_add_sphere.argtypes = [P, D, C.c_double, C.c_int, C.c_int, C.c_uint32]; L.orc_add_sphere.restype = C.c_int
     L.orc_add_wall.argtypes = [P, D, C.c_int, C.c_int, C.c_int, C.c_uint32]; L.orc_add_wall.restype = C.c_int
+    L.orc_add_spheres.argtypes = [P, C.c_int64, D, D, C.c_int, C.c_uint32]; L.orc_add_spheres.restype = C.c_int
     L.orc_num_particles.argtypes = [P]; L.orc_num_particles.restype = C.c_int
     L.orc_get.argtypes = [P, C.c_int, D]; L.orc_get.restype = C.c_int
     L.orc_set.argtypes = [P, C.c_int, D]; L.orc_set.restype = C.c_int
@@ -114,6 +115,11 @@ class Oracle:
     def sphere(self, pos, radius, mat=0, fixed=False, mask=0b0101):
         a = np.ascontiguousarray(pos, dtype=np.float64)
         return self.L.orc_add_sphere(self.h, _dp(a), float(radius), mat, int(bool(fixed)), mask)
+
+    def spheres(self, pos, radius, mat=0, mask=0b0101):
+        p = np.ascontiguousarray(pos, dtype=np.float64)
+        r = np.ascontiguousarray(radius, dtype=np.float64)
+        return self.L.orc_add_spheres(self.h, len(r), _dp(p), _dp(r), mat, mask)
 
     def wall(self, pos, axis, sense=0, mat=0, mask=0b0011):
         a = np.ascontiguousarray(pos, dtype=np.float64)

@@ -23,7 +23,7 @@ EXPORTS = [
     "wdg_last_error", "wdg_ctx_create", "wdg_ctx_destroy", "wdg_set_field", "wdg_set_materials",
     "wdg_upload_particles", "wdg_upload", "wdg_download", "wdg_collide", "wdg_contact_loop", "wdg_leapfrog",
     "wdg_force_reset", "wdg_crit_dt", "wdg_contact_count", "wdg_contact_list", "wdg_observables", "wdg_stats",
-    "wdg_run", "wdg_sync", "wdg_stream",
+    "wdg_run", "wdg_relax", "wdg_sync", "wdg_stream",
 ]
 
 
@@ -105,6 +105,7 @@ def lib():
     L.wdg_observables.argtypes = [V, _P(C.c_double), _P(C.c_double), _P(C.c_double)]
     L.wdg_stats.argtypes = [V, _P(Counters)]
     L.wdg_run.argtypes = [V, _P(Program), C.c_int64, _P(C.c_int64), _P(C.c_double)]
+    L.wdg_relax.argtypes = [V, C.c_int32, C.c_double, C.c_double, _P(C.c_double), _P(C.c_int32)]
     L.wdg_sync.argtypes = [V]
     L.wdg_stream.argtypes = [V]
     L.wdg_stream.restype = V
@@ -277,6 +278,12 @@ class Device:
         d = C.c_double(dt)
         check(self.L.wdg_run(self.h, C.byref(prog), nSteps, C.byref(st), C.byref(d)))
         return st.value, d.value
+
+    def relax(self, maxIter=2000, inflate=0.01, tol=0.):
+        ov = C.c_double()
+        it = C.c_int32()
+        check(self.L.wdg_relax(self.h, maxIter, inflate, tol, C.byref(ov), C.byref(it)))
+        return ov.value, it.value
 
     def sync(self):
         check(self.L.wdg_sync(self.h))

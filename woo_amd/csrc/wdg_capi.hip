@@ -490,6 +490,18 @@ int wdg_run(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t* step
   });
 }
 
+int wdg_relax(wdg_ctx* ctx, int32_t maxIter, double inflate, double tol, double* maxOverlap, int32_t* iters) {
+  if (!ctx || !maxOverlap || !iters || maxIter < 0 || !(inflate >= 0)) {
+    g_err = "bad arguments";
+    return WDG_E_ARG;
+  }
+  return guard(ctx, [&] {
+    int it = 0;
+    relax(ctx->c, maxIter, inflate, tol, maxOverlap, &it);
+    *iters = it;
+  });
+}
+
 int wdg_sync(wdg_ctx* ctx) {
   if (!ctx) { g_err = "NULL argument"; return WDG_E_ARG; }
   return guard(ctx, [&] {

@@ -192,6 +192,7 @@ double crit_dt(Ctx& c);
 void observables(Ctx& c, const double lo[3], const double hi[3], double out[3]);
 int64_t count_real(Ctx& c);
 void check_errors(Ctx& c);
+void relax(Ctx& c, int maxIter, double inflate, double tol, double* maxOverlap, int* iters);
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 inline int bits_for(uint64_t maxval) {
