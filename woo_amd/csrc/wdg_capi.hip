@@ -1,6 +1,9 @@
 // extern "C" entry points of the DEM step engine (see include/woo_amd.h).
+// The ABI speaks user ids (Particle.id) and user-ordered arrays; the device keeps
+// particles in storage order (walls, then spheres sorted by cell), mapped by uid/sto.
 #include <algorithm>
 #include <cstring>
+#include <numeric>
 
 #include "wdg_internal.hpp"
 #include "wdg_math.hpp"
@@ -10,57 +13,18 @@ struct wdg_ctx {
   wdg::Ctx c;
 };
 
-namespace {
-thread_local std::string g_err;
-
-template <class F>
-int guard(wdg_ctx* ctx, F&& f) {
-  try {
-    if (ctx) WDG_HIP(hipSetDevice(ctx->c.device));
-    f();
-    return WDG_OK;
-  } catch (wdg::Error& e) {
-    g_err = e.what();
-    return e.code;
-  } catch (std::exception& e) {
-    g_err = e.what();
-    return WDG_E_STATE;
-  }
-}
-
-// upload n*w doubles (AoS, row-major) into w SoA arrays
-void upload_soa(wdg::Ctx& c, wdg::DBuf<double>* dst, int w, const double* host, int64_t n, double fill[4]) {
-  std::vector<double> col((size_t)n);
-  for (int k = 0; k < w; k++) {
-    dst[k].ensure((size_t)n + 1);
-    for (int64_t i = 0; i < n; i++) col[i] = host ? host[i * w + k] : fill[k];
-    if (n) WDG_HIP(hipMemcpy(dst[k].p, col.data(), n * sizeof(double), hipMemcpyHostToDevice));
-  }
-}
-void download_soa(wdg::Ctx& c, wdg::DBuf<double>* src, int w, double* host, int64_t n) {
-  std::vector<double> col((size_t)n);
-  WDG_HIP(hipStreamSynchronize(c.stream));
-  for (int k = 0; k < w; k++) {
-    if (n) WDG_HIP(hipMemcpy(col.data(), src[k].p, n * sizeof(double), hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < n; i++) host[i * w + k] = col[i];
-  }
-}
-wdg::DBuf<double>* field_bufs(wdg::Ctx& c, int field, int& w) {
-  switch (field) {
-    case WDG_F_POS: w = 3; return c.pos;
-    case WDG_F_VEL: w = 3; return c.vel;
-    case WDG_F_ANGVEL: w = 3; return c.angVel;
-    case WDG_F_ORI: w = 4; return c.ori;
-    case WDG_F_FORCE: w = 3; return c.force;
-    case WDG_F_TORQUE: w = 3; return c.torque;
-    case WDG_F_MASS: w = 1; return &c.mass;
-    case WDG_F_INERTIA: w = 3; return c.inertia;
-  }
-  throw wdg::Error(WDG_E_ARG, "unknown field " + std::to_string(field));
-}
-}  // namespace
-
 namespace wdg {
+
+// out[u] = in[sto[u]]
+__global__ void k_gather_uid(int64_t n, const uint32_t* __restrict__ sto, const double* __restrict__ in, double* out) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u < n) out[u] = in[sto[u]];
+}
+// out[sto[u]] = in[u]
+__global__ void k_scatter_uid(int64_t n, const uint32_t* __restrict__ sto, const double* __restrict__ in, double* out) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u < n) out[sto[u]] = in[u];
+}
 
 // one record of 16 doubles per slot, for inspection and parity tests
 struct ExportArgs {
@@ -116,7 +80,150 @@ __global__ void k_export(ExportArgs a) {
 
 }  // namespace wdg
 
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guard(wdg_ctx* ctx, F&& f) {
+  try {
+    if (ctx) WDG_HIP(hipSetDevice(ctx->c.device));
+    f();
+    return WDG_OK;
+  } catch (wdg::Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return WDG_E_STATE;
+  }
+}
+
+wdg::DBuf<double>* field_bufs(wdg::Ctx& c, int field, int& w) {
+  switch (field) {
+    case WDG_F_POS: w = 3; return c.pos;
+    case WDG_F_VEL: w = 3; return c.vel;
+    case WDG_F_ANGVEL: w = 3; return c.angVel;
+    case WDG_F_ORI: w = 4; return c.ori;
+    case WDG_F_FORCE: w = 3; return c.force;
+    case WDG_F_TORQUE: w = 3; return c.torque;
+    case WDG_F_MASS: w = 1; return &c.mass;
+    case WDG_F_INERTIA: w = 3; return c.inertia;
+  }
+  throw wdg::Error(WDG_E_ARG, "unknown field " + std::to_string(field));
+}
+
+// user-ordered AoS host array -> storage-ordered SoA device arrays
+void put_field(wdg::Ctx& c, wdg::DBuf<double>* dst, int w, const double* host) {
+  using namespace wdg;
+  const int64_t n = c.n;
+  if (!n) return;
+  c.xfer.ensure(n);
+  std::vector<double> col((size_t)n);
+  for (int k = 0; k < w; k++) {
+    for (int64_t u = 0; u < n; u++) col[u] = host[u * w + k];
+    WDG_HIP(hipMemcpyAsync(c.xfer.p, col.data(), n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_scatter_uid, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, n, (const uint32_t*)c.sto.p,
+                       (const double*)c.xfer.p, dst[k].p);
+    WDG_LAUNCH_CHECK();
+    WDG_HIP(hipStreamSynchronize(c.stream));
+  }
+}
+void get_field(wdg::Ctx& c, wdg::DBuf<double>* src, int w, double* host) {
+  using namespace wdg;
+  const int64_t n = c.n;
+  if (!n) return;
+  c.xfer.ensure(n);
+  std::vector<double> col((size_t)n);
+  for (int k = 0; k < w; k++) {
+    hipLaunchKernelGGL(k_gather_uid, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, n, (const uint32_t*)c.sto.p,
+                       (const double*)src[k].p, c.xfer.p);
+    WDG_LAUNCH_CHECK();
+    WDG_HIP(hipMemcpyAsync(col.data(), c.xfer.p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    WDG_HIP(hipStreamSynchronize(c.stream));
+    for (int64_t u = 0; u < n; u++) host[u * w + k] = col[u];
+  }
+}
+
+struct Record {
+  uint32_t a, b;  // user ids of pA, pB
+  int cd[3];
+  int64_t slot;   // -1: potential only
+};
+}  // namespace
+
 using namespace wdg;
+
+static std::vector<Record> collect_contacts(Ctx& c, int which, std::vector<double>* rec) {
+  Contacts& C = c.con();
+  const int64_t nc = C.n;
+  std::vector<uint32_t> A((size_t)nc), B((size_t)nc), info((size_t)nc), real((size_t)nc), uid((size_t)c.n);
+  std::vector<uint64_t> dist;
+  WDG_HIP(hipStreamSynchronize(c.stream));
+  if (c.n) WDG_HIP(hipMemcpy(uid.data(), c.uid.p, c.n * 4, hipMemcpyDeviceToHost));
+  if (nc) {
+    WDG_HIP(hipMemcpy(A.data(), C.A.p, nc * 4, hipMemcpyDeviceToHost));
+    WDG_HIP(hipMemcpy(B.data(), C.B.p, nc * 4, hipMemcpyDeviceToHost));
+    WDG_HIP(hipMemcpy(info.data(), C.info.p, nc * 4, hipMemcpyDeviceToHost));
+    WDG_HIP(hipMemcpy(real.data(), C.real.p, nc * 4, hipMemcpyDeviceToHost));
+    if (c.field.periodic) {
+      dist.resize((size_t)nc);
+      WDG_HIP(hipMemcpy(dist.data(), C.dist.p, nc * 8, hipMemcpyDeviceToHost));
+    }
+    if (rec) {
+      DBuf<double> out;
+      out.ensure((size_t)nc * 16);
+      ExportArgs a{};
+      a.nc = nc; a.A = C.A.p; a.B = C.B.p; a.info = C.info.p; a.real = C.real.p;
+      a.N0 = C.hN[0].p; a.N1 = C.hN[1].p; a.N2 = C.hN[2].p;
+      a.Y0 = C.hY[0].p; a.Y1 = C.hY[1].p; a.Y2 = C.hY[2].p;
+      a.P0 = C.hP[0].p; a.P1 = C.hP[1].p; a.P2 = C.hP[2].p;
+      a.F0 = C.hF[0].p; a.F1 = C.hF[1].p; a.F2 = C.hF[2].p; a.uN = C.huN.p;
+      a.rad = c.radius.p; a.mass = c.mass.p; a.mat = c.mat.p; a.mats = c.dMats.p;
+      a.hertz = c.lastPhysKind == WDG_PHYS_HERTZ;
+      a.poisson = c.lastPoisson; a.en = c.lastEn; a.viscAlpha = c.lastViscAlpha;
+      a.out = out.p;
+      hipLaunchKernelGGL(k_export, dim3(nblocks(nc, 256)), dim3(256), 0, c.stream, a);
+      WDG_LAUNCH_CHECK();
+      rec->resize((size_t)nc * 16);
+      WDG_HIP(hipMemcpyAsync(rec->data(), out.p, rec->size() * 8, hipMemcpyDeviceToHost, c.stream));
+      WDG_HIP(hipStreamSynchronize(c.stream));
+      out.release();
+    }
+  }
+  std::vector<Record> R;
+  auto key = [](uint32_t a, uint32_t b) { return std::make_pair(std::min(a, b), std::max(a, b)); };
+  std::vector<std::pair<std::pair<uint32_t, uint32_t>, size_t>> order;
+  for (int64_t s = 0; s < nc; s++) {
+    if (!real[s]) continue;
+    bool swap = (info[s] & CI_SWAP) != 0;
+    Record r;
+    r.a = uid[swap ? B[s] : A[s]];
+    r.b = uid[swap ? A[s] : B[s]];
+    for (int m = 0; m < 3; m++) r.cd[m] = c.field.periodic ? unpack_cd(dist[s], m) : 0;
+    r.slot = s;
+    R.push_back(r);
+  }
+  if (which == 0) {
+    // potential set: enlarged AABBs of the last refresh, plus the real contacts
+    std::vector<uint32_t> pa, pb;
+    std::vector<uint64_t> pcd;
+    potential_pairs(c, pa, pb, pcd);
+    std::vector<std::pair<uint32_t, uint32_t>> have;
+    for (auto& r : R) have.push_back(key(r.a, r.b));
+    std::sort(have.begin(), have.end());
+    for (size_t i = 0; i < pa.size(); i++) {
+      if (std::binary_search(have.begin(), have.end(), key(pa[i], pb[i]))) continue;
+      Record r;
+      r.a = pa[i];
+      r.b = pb[i];
+      for (int m = 0; m < 3; m++) r.cd[m] = c.field.periodic ? unpack_cd(pcd[i], m) : 0;
+      r.slot = -1;
+      R.push_back(r);
+    }
+  }
+  std::sort(R.begin(), R.end(), [&](const Record& x, const Record& y) { return key(x.a, x.b) < key(y.a, y.b); });
+  return R;
+}
 
 extern "C" {
 
@@ -157,14 +264,16 @@ int wdg_ctx_destroy(wdg_ctx* ctx) {
     DBuf<double>* arrs[] = {c.pos, c.vel, c.angVel, c.force, c.torque, c.inertia, c.lastPos};
     for (auto* a : arrs) for (int k = 0; k < 3; k++) a[k].release();
     for (int k = 0; k < 4; k++) c.ori[k].release();
+    for (int k = 0; k < NPD; k++) c.spareD[k].release();
+    for (int k = 0; k < NPU; k++) c.spareU[k].release();
     c.mass.release(); c.radius.release(); c.flags.release(); c.mask.release(); c.mat.release();
-    c.dMats.release(); c.dWalls.release();
+    c.uid.release(); c.sto.release(); c.dMats.release();
     c.cellKey.release(); c.sortIdx.release(); c.cellKeyTmp.release(); c.sortIdxTmp.release();
-    c.cellStart.release(); c.cellEnd.release(); c.sx.release(); c.sy.release(); c.sz.release(); c.shs.release();
-    c.smask.release(); c.scount.release(); c.soff.release(); c.pairKey.release(); c.pairKeyTmp.release();
-    c.pairDist.release(); c.pairInfo.release(); c.pairVal.release(); c.pairValTmp.release(); c.runFlag.release();
-    c.runOff.release(); c.keyB.release(); c.keyBTmp.release(); c.valB.release(); c.valBTmp.release();
-    c.sortHist.release(); c.scanAux.release(); c.red.release(); c.dflags.release();
+    c.perm.release(); c.invPerm.release(); c.cellStart.release(); c.cellEnd.release();
+    c.scount.release(); c.soff.release(); c.pairKey.release(); c.pairKeyTmp.release(); c.pairDist.release();
+    c.pairInfo.release(); c.pairVal.release(); c.pairValTmp.release(); c.runFlag.release(); c.runOff.release();
+    c.adjKey.release(); c.adjKeyTmp.release(); c.adjValTmp.release(); c.adjPart.release();
+    c.sortHist.release(); c.scanAux.release(); c.red.release(); c.xfer.release(); c.dflags.release();
     if (c.hflags) (void)hipHostFree(c.hflags);
     if (c.stream) (void)hipStreamDestroy(c.stream);
   });
@@ -207,72 +316,84 @@ int wdg_upload_particles(wdg_ctx* ctx, const wdg_particles* p) {
     const int64_t n = p->n;
     if (n >= (int64_t)1 << 31) throw Error(WDG_E_CAPACITY, "too many particles");
     WDG_HIP(hipStreamSynchronize(c.stream));
-    std::vector<uint32_t> flags((size_t)n), mask((size_t)n);
-    std::vector<int32_t> mat((size_t)n), walls, sph;
-    std::vector<double> rad((size_t)n, 0.), mass((size_t)n);
+    // storage order: walls (user-id order), then spheres (user-id order until the first refresh)
+    std::vector<uint32_t> order;
+    for (int pass = 0; pass < 2; pass++)
+      for (int64_t i = 0; i < n; i++) {
+        int shape = p->shape[i];
+        if (shape != WDG_SPHERE && shape != WDG_WALL) throw Error(WDG_E_ARG, "unsupported shape " + std::to_string(shape));
+        if ((pass == 0) == (shape == WDG_WALL)) order.push_back((uint32_t)i);
+      }
+    std::vector<uint32_t> flags((size_t)n), mask((size_t)n), uid((size_t)n), sto((size_t)n);
+    std::vector<int32_t> mat((size_t)n);
+    std::vector<double> col[NPD];
+    for (auto& v : col) v.resize((size_t)n);
     c.rmin = INFINITY;
     c.rmax = 0;
-    for (int64_t i = 0; i < n; i++) {
-      int shape = p->shape[i];
-      if (shape != WDG_SPHERE && shape != WDG_WALL) throw Error(WDG_E_ARG, "unsupported shape " + std::to_string(shape));
+    int64_t nW = 0;
+    for (int64_t s = 0; s < n; s++) {
+      const int64_t i = order[s];
+      const int shape = p->shape[i];
       int m = p->material ? p->material[i] : 0;
       if (m < 0 || m >= (int)c.mats.size()) throw Error(WDG_E_ARG, "material index out of range");
       uint32_t f = (p->flags ? p->flags[i] : 0u) & 0xFFFFu;
       if (p->integrated ? p->integrated[i] : 1) f |= F_INTEGRATED;
       f |= (uint32_t)shape << F_SHAPE_SHIFT;
+      double r = 0.;
       if (shape == WDG_WALL) {
         int ax = p->wallAxis ? p->wallAxis[i] : 0, se = p->wallSense ? p->wallSense[i] : 0;
         if (ax < 0 || ax > 2 || se < -1 || se > 1) throw Error(WDG_E_ARG, "bad wall axis/sense");
         f |= (uint32_t)ax << F_AXIS_SHIFT;
         f |= (uint32_t)(se + 1) << F_SENSE_SHIFT;
-        walls.push_back((int32_t)i);
+        nW++;
       } else {
-        double r = p->radius ? p->radius[i] : 0.;
+        r = p->radius ? p->radius[i] : 0.;
         if (!(r > 0)) throw Error(WDG_E_ARG, "Sphere #" + std::to_string(i) + ": radius must be positive");
-        rad[i] = r;
         c.rmin = std::min(c.rmin, r);
         c.rmax = std::max(c.rmax, r);
-        sph.push_back((int32_t)i);
       }
-      flags[i] = f;
-      mask[i] = p->mask ? p->mask[i] : (shape == WDG_WALL ? 3u : 5u);
-      mat[i] = m;
-      mass[i] = p->mass[i];
+      flags[s] = f;
+      mask[s] = p->mask ? p->mask[i] : (shape == WDG_WALL ? 3u : 5u);
+      mat[s] = m;
+      uid[s] = (uint32_t)i;
+      sto[i] = (uint32_t)s;
+      // the permuted array list of the refresh: pos3 vel3 angVel3 ori4 force3 torque3 inertia3 mass radius
+      for (int k = 0; k < 3; k++) {
+        col[k][s] = p->pos[3 * i + k];
+        col[3 + k][s] = p->vel ? p->vel[3 * i + k] : 0.;
+        col[6 + k][s] = p->angVel ? p->angVel[3 * i + k] : 0.;
+        col[13 + k][s] = p->force ? p->force[3 * i + k] : 0.;
+        col[16 + k][s] = p->torque ? p->torque[3 * i + k] : 0.;
+        col[19 + k][s] = p->inertia[3 * i + k];
+      }
+      for (int k = 0; k < 4; k++) col[9 + k][s] = p->ori ? p->ori[4 * i + k] : (k == 0 ? 1. : 0.);
+      col[22][s] = p->mass[i];
+      col[23][s] = r;
     }
     c.n = n;
-    c.nSph = (int64_t)sph.size();
-    c.wallIds = walls;
-    std::vector<int32_t> ids(walls);
-    ids.insert(ids.end(), sph.begin(), sph.end());
-    c.dWalls.ensure(ids.size() + 1);
-    if (!ids.empty()) WDG_HIP(hipMemcpy(c.dWalls.p, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    double zero[4] = {0, 0, 0, 0}, qid[4] = {1, 0, 0, 0};
-    upload_soa(c, c.pos, 3, p->pos, n, zero);
-    upload_soa(c, c.vel, 3, p->vel, n, zero);
-    upload_soa(c, c.angVel, 3, p->angVel, n, zero);
-    upload_soa(c, c.ori, 4, p->ori, n, qid);
-    upload_soa(c, c.force, 3, p->force, n, zero);
-    upload_soa(c, c.torque, 3, p->torque, n, zero);
-    upload_soa(c, c.inertia, 3, p->inertia, n, zero);
-    upload_soa(c, &c.mass, 1, mass.data(), n, zero);
-    upload_soa(c, &c.radius, 1, rad.data(), n, zero);
-    upload_soa(c, c.lastPos, 3, p->pos, n, zero);
-    c.flags.ensure(n + 1); c.mask.ensure(n + 1); c.mat.ensure(n + 1);
+    c.nW = nW;
+    DBuf<double>* d[NPD] = {&c.pos[0], &c.pos[1], &c.pos[2], &c.vel[0], &c.vel[1], &c.vel[2], &c.angVel[0],
+                            &c.angVel[1], &c.angVel[2], &c.ori[0], &c.ori[1], &c.ori[2], &c.ori[3], &c.force[0],
+                            &c.force[1], &c.force[2], &c.torque[0], &c.torque[1], &c.torque[2], &c.inertia[0],
+                            &c.inertia[1], &c.inertia[2], &c.mass, &c.radius};
+    for (int k = 0; k < NPD; k++) {
+      d[k]->ensure(n + 1);
+      if (n) WDG_HIP(hipMemcpy(d[k]->p, col[k].data(), n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    for (int k = 0; k < 3; k++) {
+      c.lastPos[k].ensure(n + 1);
+      if (n) WDG_HIP(hipMemcpy(c.lastPos[k].p, col[k].data(), n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    c.flags.ensure(n + 1); c.mask.ensure(n + 1); c.mat.ensure(n + 1); c.uid.ensure(n + 1); c.sto.ensure(n + 1);
     if (n) {
-      WDG_HIP(hipMemcpy(c.flags.p, flags.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
-      WDG_HIP(hipMemcpy(c.mask.p, mask.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
-      WDG_HIP(hipMemcpy(c.mat.p, mat.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+      WDG_HIP(hipMemcpy(c.flags.p, flags.data(), n * 4, hipMemcpyHostToDevice));
+      WDG_HIP(hipMemcpy(c.mask.p, mask.data(), n * 4, hipMemcpyHostToDevice));
+      WDG_HIP(hipMemcpy(c.mat.p, mat.data(), n * 4, hipMemcpyHostToDevice));
+      WDG_HIP(hipMemcpy(c.uid.p, uid.data(), n * 4, hipMemcpyHostToDevice));
+      WDG_HIP(hipMemcpy(c.sto.p, sto.data(), n * 4, hipMemcpyHostToDevice));
     }
-    // contacts are dropped: the next collider call is a full run
-    for (auto& s : c.C) {
-      s.n = 0;
-      s.ensure(0, n, c.field.periodic != 0);
-      WDG_HIP(hipMemset(s.offA.p, 0, (n + 1) * sizeof(uint32_t)));
-      WDG_HIP(hipMemset(s.offB.p, 0, (n + 1) * sizeof(uint32_t)));
-    }
-    c.boundsValid = false;
-    c.dirtyKnown = false;
-    c.forcesPending = false;
+    drop_contacts(c);
+    WDG_HIP(hipStreamSynchronize(c.stream));
     c.verletDist = NAN;
     c.dynStepLast = c.dynStepPrev = -1;
   });
@@ -285,13 +406,7 @@ int wdg_upload(wdg_ctx* ctx, int32_t field, const double* host) {
     int w;
     DBuf<double>* b = field_bufs(c, field, w);
     if ((field == WDG_F_FORCE || field == WDG_F_TORQUE) && c.forcesPending) flush_forces(c);
-    WDG_HIP(hipStreamSynchronize(c.stream));
-    double zero[4] = {0, 0, 0, 0};
-    if (field == WDG_F_MASS) {
-      WDG_HIP(hipMemcpy(c.mass.p, host, c.n * sizeof(double), hipMemcpyHostToDevice));
-    } else {
-      upload_soa(c, b, w, host, c.n, zero);
-    }
+    put_field(c, b, w, host);
     if (field == WDG_F_POS) c.dirtyKnown = false;
   });
 }
@@ -303,7 +418,7 @@ int wdg_download(wdg_ctx* ctx, int32_t field, double* host) {
     int w;
     DBuf<double>* b = field_bufs(c, field, w);
     if ((field == WDG_F_FORCE || field == WDG_F_TORQUE) && c.forcesPending) flush_forces(c);
-    download_soa(c, b, w, host, c.n);
+    get_field(c, b, w, host);
   });
 }
 
@@ -335,74 +450,23 @@ int wdg_crit_dt(wdg_ctx* ctx, double* critDt) {
 int wdg_contact_count(wdg_ctx* ctx, int32_t which, int64_t* n) {
   if (!ctx || !n) { g_err = "NULL argument"; return WDG_E_ARG; }
   return guard(ctx, [&] {
-    Ctx& c = ctx->c;
-    Contacts& C = c.con();
-    if (which == 1) { *n = count_real(c); return; }
-    std::vector<uint32_t> info((size_t)C.n), real((size_t)C.n);
-    WDG_HIP(hipStreamSynchronize(c.stream));
-    if (C.n) {
-      WDG_HIP(hipMemcpy(info.data(), C.info.p, C.n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-      WDG_HIP(hipMemcpy(real.data(), C.real.p, C.n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    }
-    int64_t k = 0;
-    for (int64_t s = 0; s < C.n; s++) k += (real[s] || (info[s] & CI_AABB)) ? 1 : 0;
-    *n = k;
+    if (which == 1) { *n = count_real(ctx->c); return; }
+    *n = (int64_t)collect_contacts(ctx->c, which, nullptr).size();
   });
 }
 
 int wdg_contact_list(wdg_ctx* ctx, int32_t which, int32_t* ids, int32_t* cellDist, double* data) {
   if (!ctx) { g_err = "NULL argument"; return WDG_E_ARG; }
   return guard(ctx, [&] {
-    Ctx& c = ctx->c;
-    Contacts& C = c.con();
-    const int64_t nc = C.n;
-    std::vector<uint32_t> A((size_t)nc), B((size_t)nc), info((size_t)nc), real((size_t)nc);
-    std::vector<uint64_t> dist;
     std::vector<double> rec;
-    WDG_HIP(hipStreamSynchronize(c.stream));
-    if (nc) {
-      WDG_HIP(hipMemcpy(A.data(), C.A.p, nc * 4, hipMemcpyDeviceToHost));
-      WDG_HIP(hipMemcpy(B.data(), C.B.p, nc * 4, hipMemcpyDeviceToHost));
-      WDG_HIP(hipMemcpy(info.data(), C.info.p, nc * 4, hipMemcpyDeviceToHost));
-      WDG_HIP(hipMemcpy(real.data(), C.real.p, nc * 4, hipMemcpyDeviceToHost));
-      if (c.field.periodic) {
-        dist.resize((size_t)nc);
-        WDG_HIP(hipMemcpy(dist.data(), C.dist.p, nc * 8, hipMemcpyDeviceToHost));
-      }
+    std::vector<Record> R = collect_contacts(ctx->c, which, data ? &rec : nullptr);
+    for (size_t k = 0; k < R.size(); k++) {
+      if (ids) { ids[2 * k] = (int32_t)R[k].a; ids[2 * k + 1] = (int32_t)R[k].b; }
+      if (cellDist) for (int m = 0; m < 3; m++) cellDist[3 * k + m] = R[k].cd[m];
       if (data) {
-        DBuf<double> out;
-        out.ensure((size_t)nc * 16);
-        ExportArgs a{};
-        a.nc = nc; a.A = C.A.p; a.B = C.B.p; a.info = C.info.p; a.real = C.real.p;
-        a.N0 = C.hN[0].p; a.N1 = C.hN[1].p; a.N2 = C.hN[2].p;
-        a.Y0 = C.hY[0].p; a.Y1 = C.hY[1].p; a.Y2 = C.hY[2].p;
-        a.P0 = C.hP[0].p; a.P1 = C.hP[1].p; a.P2 = C.hP[2].p;
-        a.F0 = C.hF[0].p; a.F1 = C.hF[1].p; a.F2 = C.hF[2].p; a.uN = C.huN.p;
-        a.rad = c.radius.p; a.mass = c.mass.p; a.mat = c.mat.p; a.mats = c.dMats.p;
-        a.hertz = c.lastPhysKind == WDG_PHYS_HERTZ;
-        a.poisson = c.lastPoisson; a.en = c.lastEn; a.viscAlpha = c.lastViscAlpha;
-        a.out = out.p;
-        hipLaunchKernelGGL(k_export, dim3(nblocks(nc, 256)), dim3(256), 0, c.stream, a);
-        WDG_LAUNCH_CHECK();
-        rec.resize((size_t)nc * 16);
-        WDG_HIP(hipMemcpyAsync(rec.data(), out.p, rec.size() * 8, hipMemcpyDeviceToHost, c.stream));
-        WDG_HIP(hipStreamSynchronize(c.stream));
-        out.release();
+        if (R[k].slot >= 0) std::memcpy(data + 16 * k, rec.data() + 16 * R[k].slot, 16 * sizeof(double));
+        else { for (int m = 0; m < 15; m++) data[16 * k + m] = NAN; data[16 * k + 15] = 0.; }
       }
-    }
-    int64_t k = 0;
-    for (int64_t s = 0; s < nc; s++) {
-      bool take = which == 1 ? real[s] != 0 : (real[s] || (info[s] & CI_AABB));
-      if (!take) continue;
-      bool swap = (info[s] & CI_SWAP) != 0;
-      if (ids) {
-        ids[2 * k] = (int32_t)(swap ? B[s] : A[s]);
-        ids[2 * k + 1] = (int32_t)(swap ? A[s] : B[s]);
-      }
-      if (cellDist)
-        for (int m = 0; m < 3; m++) cellDist[3 * k + m] = c.field.periodic ? unpack_cd(dist[s], m) : 0;
-      if (data) std::memcpy(data + 16 * k, rec.data() + 16 * s, 16 * sizeof(double));
-      k++;
     }
   });
 }
@@ -423,9 +487,17 @@ int wdg_stats(wdg_ctx* ctx, wdg_counters* out) {
     out->nRefresh = c.nRefresh;
     out->nSteps = c.nCollide;
     out->verletDist = c.verletDist;
-    out->brute = c.lastBrute;
-    for (int k = 0; k < 3; k++) out->gridCells[k] = c.gridCells[k];
+    out->brute = c.grid.brute;
+    for (int k = 0; k < 3; k++) out->gridCells[k] = c.grid.brute ? 0 : c.grid.nc[k];
   });
+}
+
+// pown: square-and-multiply (Math.hpp:166-175)
+static double pown(double x, unsigned n) {
+  unsigned d = n >> 1, r = n & 1;
+  double x2d = d == 0 ? 1. : pown(x * x, d);
+  double xr = r == 0 ? 1. : x;
+  return x2d * xr;
 }
 
 // Scene::doOneStep (Scene.cpp:366-467) for a fixed engine list
@@ -465,18 +537,7 @@ int wdg_run(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t* step
               double cr = crit_dt(c);
               if (std::isinf(cr)) break;
               int64_t ns = *step - c.dynStepPrev;
-              double base = 1. + prog->maxRelInc, pw = 1.;
-              // pown: square-and-multiply (Math.hpp:166-175)
-              struct P {
-                static double pown(double x, unsigned n) {
-                  unsigned d = n >> 1, r = n & 1;
-                  double x2d = d == 0 ? 1. : pown(x * x, d);
-                  double xr = r == 0 ? 1. : x;
-                  return x2d * xr;
-                }
-              };
-              pw = P::pown(base, (unsigned)ns);
-              double maxNewDt = *dt * pw;
+              double maxNewDt = *dt * pown(1. + prog->maxRelInc, (unsigned)ns);
               double nd = std::min(cr * prog->dtSafety, maxNewDt);
               if (nd != *dt) nextDt = nd;
             }

@@ -1,6 +1,7 @@
-// Broadphase: Verlet-skin refresh test, cell-list pair generation over
-// radix-sorted cell keys, and the merge that carries contact history across
-// refreshes. Replaces InsertionSortCollider::run (InsertionSortCollider.cpp:386-613).
+// Broadphase: Verlet-skin refresh test, spatial re-sort of the particle
+// storage, cell-list pair generation over radix-sorted cell keys, and the merge
+// that carries contact history across refreshes. Replaces
+// InsertionSortCollider::run (InsertionSortCollider.cpp:386-613).
 //
 // Pair-set semantics kept from the reference:
 //  * AABB of a sphere = centre -/+ (r*distFactor') -/+ verletDist, of a wall =
@@ -8,12 +9,19 @@
 //    Collision.cpp:125-162);
 //  * refresh when any particle moved more than verletDist since the last one
 //    (Collision.cpp:80-123);
-//  * after a refresh the contact set is {pairs whose enlarged AABBs overlap and
-//    that mayCollide} U {real contacts}; a broken contact whose AABBs did not
-//    overlap is dropped (InsertionSortCollider.hpp:213-218).
-// Contact slots are sorted by (min id, max id); history of real contacts is
-// carried to the new slots by one stable sort of [new pairs ; old real slots],
-// so equal keys become adjacent and the compaction joins them.
+//  * potential set after a refresh = {pairs whose enlarged AABBs overlap and that
+//    mayCollide} U {real contacts}; a broken contact whose AABBs did not overlap
+//    is dropped (InsertionSortCollider.hpp:213-218).
+// The step kernels only need the pairs that can touch before the next refresh:
+// the slots hold the AABB pairs whose centres are within r1+r2+2*verletDist
+// (each particle moves at most verletDist until then), plus carried real
+// contacts. The full potential set is regenerated on request (potential_pairs).
+//
+// Refresh: spheres are re-sorted by cell (storage order), the particle arrays
+// permuted, pairs generated per sphere over the 27 neighbour cells, then one
+// stable radix sort of [new pairs ; old real slots] makes equal keys adjacent
+// so that the compaction carries history; finally the per-particle adjacency is
+// sorted by (particle, other user id), the canonical summation order.
 #include <cstring>
 
 #include "wdg_internal.hpp"
@@ -31,7 +39,7 @@ struct Grid {
 
 struct AabbParams {
   double vd;      // verletDist (absolute)
-  double dfBox;   // (distFactor>0 ? distFactor : 1)
+  double absDf;   // |distFactor|
   uint32_t loneMask;
   int periodic;
   double L[3];
@@ -102,20 +110,19 @@ __device__ __forceinline__ unsigned long long ord_bits(double x) {
   unsigned long long b = (unsigned long long)__double_as_longlong(x);
   return (b >> 63) ? ~b : (b | (1ull << 63));
 }
-__host__ inline double ord_to_double(unsigned long long b) {
+static inline double ord_to_double(unsigned long long b) {
   unsigned long long r = (b >> 63) ? (b & ~(1ull << 63)) : ~b;
   double d;
   std::memcpy(&d, &r, 8);
   return d;
 }
 
-__global__ void k_bbox(int64_t nSph, const int32_t* __restrict__ sphIds, const double* __restrict__ px,
-                       const double* __restrict__ py, const double* __restrict__ pz, unsigned long long* out) {
-  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_bbox(int64_t s0, int64_t n, const double* __restrict__ px, const double* __restrict__ py,
+                       const double* __restrict__ pz, unsigned long long* out) {
+  int64_t s = s0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long mn[3] = {~0ull, ~0ull, ~0ull}, mx[3] = {0, 0, 0};
-  if (s < nSph) {
-    int i = sphIds[s];
-    double p[3] = {px[i], py[i], pz[i]};
+  if (s < n) {
+    double p[3] = {px[s], py[s], pz[s]};
     for (int k = 0; k < 3; k++) mn[k] = mx[k] = ord_bits(p[k]);
   }
   for (int k = 0; k < 3; k++) {
@@ -134,14 +141,7 @@ __global__ void k_bbox(int64_t nSph, const int32_t* __restrict__ sphIds, const d
   }
 }
 
-// cell key per sphere; refresh positions are recorded for every particle (lastPos)
-__global__ void k_keys(int64_t nSph, const int32_t* __restrict__ sphIds, const double* __restrict__ px,
-                       const double* __restrict__ py, const double* __restrict__ pz, Grid g, uint32_t* key,
-                       uint32_t* idx) {
-  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nSph) return;
-  int i = sphIds[s];
-  double p[3] = {px[i], py[i], pz[i]};
+__device__ __forceinline__ uint32_t cell_key(const double p[3], const Grid& g) {
   int c[3];
   for (int k = 0; k < 3; k++) {
     double x = p[k] - g.org[k];
@@ -149,8 +149,51 @@ __global__ void k_keys(int64_t nSph, const int32_t* __restrict__ sphIds, const d
     int ci = (int)floor(x / g.h[k]);
     c[k] = min(max(ci, 0), g.nc[k] - 1);
   }
-  key[s] = (uint32_t)(((int64_t)c[2] * g.nc[1] + c[1]) * g.nc[0] + c[0]);
-  idx[s] = (uint32_t)i;
+  return (uint32_t)(((int64_t)c[2] * g.nc[1] + c[1]) * g.nc[0] + c[0]);
+}
+
+// cell key of every sphere (storage [s0,n)); value = current storage index
+__global__ void k_keys(int64_t s0, int64_t n, const double* __restrict__ px, const double* __restrict__ py,
+                       const double* __restrict__ pz, Grid g, uint32_t* key, uint32_t* idx) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t s = s0 + t;
+  if (s >= n) return;
+  double p[3] = {px[s], py[s], pz[s]};
+  key[t] = cell_key(p, g);
+  idx[t] = (uint32_t)s;
+}
+
+// perm[new storage] = old storage; walls keep their place
+__global__ void k_build_perm(int64_t nW, int64_t n, const uint32_t* __restrict__ sorted, uint32_t* perm,
+                             uint32_t* invPerm) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t o = i < nW ? (uint32_t)i : sorted[i - nW];
+  perm[i] = o;
+  invPerm[o] = (uint32_t)i;
+}
+
+struct PermArgs {
+  int64_t n;
+  const uint32_t* perm;
+  const double* sd[NPD];
+  double* dd[NPD];
+  const uint32_t* su[NPU];
+  uint32_t* du[NPU];
+};
+__global__ __launch_bounds__(256) void k_permute(PermArgs a) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const uint32_t o = a.perm[i];
+#pragma unroll
+  for (int k = 0; k < NPD; k++) a.dd[k][i] = a.sd[k][o];
+#pragma unroll
+  for (int k = 0; k < NPU; k++) a.du[k][i] = a.su[k][o];
+}
+
+__global__ void k_sto(int64_t n, const uint32_t* __restrict__ uid, uint32_t* sto) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) sto[uid[i]] = (uint32_t)i;
 }
 
 __global__ void k_last_pos(int64_t n, const double* __restrict__ px, const double* __restrict__ py,
@@ -162,86 +205,86 @@ __global__ void k_last_pos(int64_t n, const double* __restrict__ px, const doubl
   lz[i] = pz[i];
 }
 
-__global__ void k_gather_sorted(int64_t nSph, const uint32_t* __restrict__ idx, const double* __restrict__ px,
-                                const double* __restrict__ py, const double* __restrict__ pz,
-                                const double* __restrict__ rad, const uint32_t* __restrict__ mask, double dfBox,
-                                double* sx, double* sy, double* sz, double* shs, uint32_t* smask) {
-  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nSph) return;
-  uint32_t i = idx[s];
-  sx[s] = px[i];
-  sy[s] = py[i];
-  sz[s] = pz[i];
-  shs[s] = dfBox * rad[i];
-  smask[s] = mask[i];
+__global__ void k_cell_ranges(int64_t nSph, int64_t s0, const uint32_t* __restrict__ key, uint32_t* cstart,
+                              uint32_t* cend) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nSph) return;
+  uint32_t k = key[t];
+  if (t == 0 || key[t - 1] != k) cstart[k] = (uint32_t)(s0 + t);
+  if (t == nSph - 1 || key[t + 1] != k) cend[k] = (uint32_t)(s0 + t) + 1;
 }
 
-__global__ void k_cell_ranges(int64_t nSph, const uint32_t* __restrict__ key, uint32_t* cstart, uint32_t* cend) {
-  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nSph) return;
-  uint32_t k = key[s];
-  if (s == 0 || key[s - 1] != k) cstart[k] = (uint32_t)s;
-  if (s == nSph - 1 || key[s + 1] != k) cend[k] = (uint32_t)s + 1;
-}
+enum { MODE_NEAR = 0, MODE_AABB = 1 };
 
 struct PairArgs {
-  int64_t nSph;
-  const uint32_t* key;
-  const uint32_t* idx;
-  const double *sx, *sy, *sz, *shs;
-  const uint32_t* smask;
+  int64_t nW, n;
+  const uint32_t* key;  // cell key of storage nW+t
+  const double *px, *py, *pz;  // refresh positions (lastPos), storage order
+  const double* rad;
+  const uint32_t *mask, *uid, *flags;
   const uint32_t *cstart, *cend;
   Grid g;
   AabbParams P;
   int brute;
-  // walls
-  int nWalls;
-  const int32_t* walls;
-  const double *px, *py, *pz;
-  const uint32_t* flags;
-  const uint32_t* mask;
   int idBits;
   // outputs
   uint32_t* count;
   const uint32_t* off;
-  uint64_t* pkey;
+  uint64_t* pkey;    // MODE_NEAR: (lo<<b)|hi storage ; MODE_AABB: (uidA<<32)|uidB
   uint32_t* pinfo;
   uint64_t* pdist;
   uint32_t* dflags;
 };
 
-template <bool WRITE>
+template <bool WRITE, int MODE>
 __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
-  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.nSph) return;
-  const uint32_t i = a.idx[s];
-  const double pi[3] = {a.sx[s], a.sy[s], a.sz[s]};
-  const double hi = a.shs[s];
-  const uint32_t mi = a.smask[s];
+  const int64_t s = a.nW + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  const double ps[3] = {a.px[s], a.py[s], a.pz[s]};
+  const double rs = a.rad[s];
+  const double hs = a.P.absDf * rs;
+  const uint32_t ms = a.mask[s], us = a.uid[s];
   uint32_t cnt = 0;
-  uint32_t o = WRITE ? a.off[s] : 0;
-  auto consider = [&](int64_t t) {
-    uint32_t j = a.idx[t];
-    if (j <= i) return;
-    if (!may_collide(mi, a.smask[t], a.P.loneMask)) return;
-    const double pj[3] = {a.sx[t], a.sy[t], a.sz[t]};
+  uint32_t o = WRITE ? a.off[s - a.nW] : 0;
+  auto consider = [&](int64_t j) {
+    if (j <= s) return;
+    if (!may_collide(ms, a.mask[j], a.P.loneMask)) return;
+    const double pj[3] = {a.px[j], a.py[j], a.pz[j]};
+    const double rj = a.rad[j];
+    const uint32_t uj = a.uid[j];
+    // canonical roles: the smaller user id is pA
+    const bool sA = us < uj;
+    const double* pa = sA ? ps : pj;
+    const double* pb = sA ? pj : ps;
+    const double ra = sA ? rs : rj, rb = sA ? rj : rs;
     int cd[3], nimg;
-    if (!sph_overlap(pi, hi, pj, a.shs[t], a.P, cd, nimg)) return;
+    if (!sph_overlap(pa, a.P.absDf * ra, pb, a.P.absDf * rb, a.P, cd, nimg)) return;
     if (a.P.periodic && nimg > 1) {
       atomicOr(&a.dflags[1], ERR_DOUBLE_IMAGE);
       return;
     }
+    if (MODE == MODE_NEAR) {
+      // centres within reach before the next refresh
+      d3 d = mk3(pb[0] + a.P.L[0] * cd[0] * a.P.periodic - pa[0], pb[1] + a.P.L[1] * cd[1] * a.P.periodic - pa[1],
+                 pb[2] + a.P.L[2] * cd[2] * a.P.periodic - pa[2]);
+      double reach = a.P.absDf * (ra + rb) + 2 * a.P.vd;
+      if (sqn3(d) > reach * reach * (1 + 1e-9)) return;
+    }
     if (WRITE) {
-      a.pkey[o + cnt] = ((uint64_t)i << a.idBits) | (uint64_t)j;
-      a.pinfo[o + cnt] = (uint32_t)CT_SS | CI_AABB;
+      if (MODE == MODE_NEAR) {
+        a.pkey[o + cnt] = ((uint64_t)s << a.idBits) | (uint64_t)j;
+        a.pinfo[o + cnt] = (uint32_t)CT_SS | CI_NEAR | (sA ? 0u : CI_SWAP);
+      } else {
+        a.pkey[o + cnt] = sA ? (((uint64_t)us << 32) | uj) : (((uint64_t)uj << 32) | us);
+      }
       if (a.P.periodic) a.pdist[o + cnt] = pack_cd(cd[0], cd[1], cd[2]);
     }
     cnt++;
   };
   if (a.brute) {
-    for (int64_t t = 0; t < a.nSph; t++) consider(t);
+    for (int64_t j = s + 1; j < a.n; j++) consider(j);
   } else {
-    uint32_t k = a.key[s];
+    uint32_t k = a.key[s - a.nW];
     int c[3];
     c[0] = (int)(k % (uint32_t)a.g.nc[0]);
     c[1] = (int)((k / (uint32_t)a.g.nc[0]) % (uint32_t)a.g.nc[1]);
@@ -263,39 +306,48 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
           for (uint32_t t = t0; t < t1; t++) consider(t);
         }
   }
-  // walls: every sphere is tested against every wall (walls are few)
-  for (int w = 0; w < a.nWalls; w++) {
-    uint32_t wid = (uint32_t)a.walls[w];
-    if (!may_collide(a.mask[wid], mi, a.P.loneMask)) continue;
-    uint32_t fl = a.flags[wid];
+  // walls (storage [0,nW)) against every sphere; the wall is pA
+  for (int64_t w = 0; w < a.nW; w++) {
+    if (!may_collide(a.mask[w], ms, a.P.loneMask)) continue;
+    uint32_t fl = a.flags[w];
     int ax = flag_axis(fl);
-    double wp = ax == 0 ? a.px[wid] : (ax == 1 ? a.py[wid] : a.pz[wid]);
+    double wp = ax == 0 ? a.px[w] : (ax == 1 ? a.py[w] : a.pz[w]);
     double wlo = wp, whi = wp;
     if (a.P.vd > 0) {
       whi = whi + a.P.vd;
       wlo = wlo - a.P.vd;
     }
     double slo, shi;
-    sph_bounds(pi[ax], hi, a.P.vd, slo, shi);
+    sph_bounds(ps[ax], hs, a.P.vd, slo, shi);
     if (!(wlo <= shi + 0. && whi >= slo + 0.)) continue;
     if (WRITE) {
-      uint32_t lo = wid < i ? wid : i, hi2 = wid < i ? i : wid;
-      a.pkey[o + cnt] = ((uint64_t)lo << a.idBits) | (uint64_t)hi2;
-      a.pinfo[o + cnt] = (uint32_t)CT_WS | CI_AABB | (wid > i ? CI_SWAP : 0u);
+      if (MODE == MODE_NEAR) {
+        a.pkey[o + cnt] = ((uint64_t)w << a.idBits) | (uint64_t)s;
+        a.pinfo[o + cnt] = (uint32_t)CT_WS | CI_NEAR;
+      } else {
+        a.pkey[o + cnt] = ((uint64_t)a.uid[w] << 32) | us;
+      }
       if (a.P.periodic) a.pdist[o + cnt] = pack_cd(0, 0, 0);
     }
     cnt++;
   }
-  if (!WRITE) a.count[s] = cnt;
+  if (!WRITE) a.count[s - a.nW] = cnt;
 }
 
-// old real slots appended behind the new pairs; non-real ones get the sentinel key
+// old real slots appended behind the new pairs, re-keyed in the new storage order;
+// non-real ones get the sentinel key
 __global__ void k_append_old(int64_t nOld, int64_t nPairs, const uint32_t* __restrict__ A,
-                             const uint32_t* __restrict__ B, const uint32_t* __restrict__ real, int idBits,
-                             uint64_t sentinel, uint64_t* pkey) {
+                             const uint32_t* __restrict__ B, const uint32_t* __restrict__ real,
+                             const uint32_t* __restrict__ invPerm, int idBits, uint64_t sentinel, uint64_t* pkey) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nOld) return;
-  pkey[nPairs + c] = real[c] ? (((uint64_t)A[c] << idBits) | (uint64_t)B[c]) : sentinel;
+  uint64_t k = sentinel;
+  if (real[c]) {
+    uint32_t a = invPerm[A[c]], b = invPerm[B[c]];
+    uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    k = ((uint64_t)lo << idBits) | (uint64_t)hi;
+  }
+  pkey[nPairs + c] = k;
 }
 
 __global__ void k_iota(int64_t n, uint32_t* v) {
@@ -319,10 +371,11 @@ struct CompactArgs {
   const uint32_t* off;
   const uint32_t* pinfo;
   const uint64_t* pdist;
+  const uint32_t* invPerm;
   int idBits;
   int periodic;
   // old set
-  const uint32_t* oInfo;
+  const uint32_t *oA, *oB, *oInfo;
   const uint64_t* oDist;
   const double *oN0, *oN1, *oN2, *oY0, *oY1, *oY2, *oP0, *oP1, *oP2, *oF0, *oF1, *oF2, *ouN;
   // new set
@@ -343,15 +396,20 @@ __global__ void k_compact(CompactArgs a) {
   if (e1 >= 0) {
     if (e1 < a.nPairs) bp = e1; else old = e1 - a.nPairs;
   }
-  a.A[o] = (uint32_t)(k >> a.idBits);
-  a.B[o] = (uint32_t)(k & ((1ull << a.idBits) - 1));
+  const uint32_t lo = (uint32_t)(k >> a.idBits), hi = (uint32_t)(k & ((1ull << a.idBits) - 1));
+  a.A[o] = lo;
+  a.B[o] = hi;
   uint32_t inf;
   uint64_t dist = 0;
   if (bp >= 0) {
     inf = a.pinfo[bp];
     if (a.periodic) dist = a.pdist[bp];
   } else {
-    inf = a.oInfo[old] & ~CI_AABB;
+    // roles are fixed by user ids: find where the old pA went
+    const uint32_t oi = a.oInfo[old];
+    const uint32_t oldPA = (oi & CI_SWAP) ? a.oB[old] : a.oA[old];
+    const uint32_t newPA = a.invPerm[oldPA];
+    inf = (oi & CI_TYPE_MASK) | (newPA == hi ? CI_SWAP : 0u);
   }
   if (old >= 0 && a.periodic) dist = a.oDist[old];
   a.info[o] = inf;
@@ -368,18 +426,32 @@ __global__ void k_compact(CompactArgs a) {
   }
 }
 
-// CSR offsets from a sorted id column: off[id] = first slot with that id
+// two adjacency entries per slot, keyed (particle storage, other user id)
+__global__ void k_adj_keys(int64_t nc, const uint32_t* __restrict__ A, const uint32_t* __restrict__ B,
+                           const uint32_t* __restrict__ info, const uint32_t* __restrict__ uid, int idBits,
+                           uint64_t* key, uint32_t* val) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const uint32_t lo = A[c], hi = B[c];
+  const bool swap = (info[c] & CI_SWAP) != 0u;
+  key[2 * c] = ((uint64_t)lo << idBits) | (uint64_t)uid[hi];
+  val[2 * c] = (uint32_t)c | (swap ? 0u : ADJ_PA);
+  key[2 * c + 1] = ((uint64_t)hi << idBits) | (uint64_t)uid[lo];
+  val[2 * c + 1] = (uint32_t)c | (swap ? ADJ_PA : 0u);
+}
+
+__global__ void k_key_hi(int64_t n, const uint64_t* __restrict__ key, int idBits, uint32_t* part) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) part[i] = (uint32_t)(key[i] >> idBits);
+}
+
+// CSR offsets from a sorted id column: off[id] = first index with that id
 __global__ void k_offsets(int64_t nc, int64_t npar, const uint32_t* __restrict__ ids, uint32_t* off) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t > nc) return;
   int64_t prev = t == 0 ? -1 : (int64_t)ids[t - 1];
   int64_t cur = t == nc ? npar : (int64_t)ids[t];
   for (int64_t id = prev + 1; id <= cur; id++) off[id] = (uint32_t)t;
-}
-
-__global__ void k_copy_u32(int64_t n, const uint32_t* __restrict__ in, uint32_t* out) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = in[i];
 }
 
 // ------------------------------------------------------------------ host side
@@ -391,22 +463,47 @@ static void check_device_errors(Ctx& c, uint32_t err) {
   if (err & ERR_CD_RANGE) throw Error(WDG_E_GEOM, "periodic cell distance out of range");
 }
 
+void check_errors(Ctx& c) {
+  uint32_t err = read_u32(c, c.dflags.p + 1);
+  if (err) {
+    WDG_HIP(hipMemsetAsync(c.dflags.p + 1, 0, sizeof(uint32_t), c.stream));
+    check_device_errors(c, err);
+  }
+}
+
+void drop_contacts(Ctx& c) {
+  for (auto& s : c.C) {
+    s.n = 0;
+    s.ensure(0, c.n, c.field.periodic != 0);
+    WDG_HIP(hipMemsetAsync(s.adjOff.p, 0, (c.n + 1) * sizeof(uint32_t), c.stream));
+  }
+  c.boundsValid = false;
+  c.dirtyKnown = false;
+  c.forcesPending = false;
+  c.grid.valid = false;
+}
+
+static PairArgs pair_args(Ctx& c, const Grid& g, const AabbParams& P, int idBits) {
+  PairArgs pa{};
+  pa.nW = c.nW; pa.n = c.n; pa.key = c.cellKey.p;
+  pa.px = c.lastPos[0].p; pa.py = c.lastPos[1].p; pa.pz = c.lastPos[2].p; pa.rad = c.radius.p;
+  pa.mask = c.mask.p; pa.uid = c.uid.p; pa.flags = c.flags.p;
+  pa.cstart = c.cellStart.p; pa.cend = c.cellEnd.p; pa.g = g; pa.P = P; pa.brute = c.grid.brute;
+  pa.idBits = idBits; pa.count = c.scount.p; pa.off = c.soff.p; pa.dflags = c.dflags.p;
+  return pa;
+}
+
 static void refresh(Ctx& c) {
-  const int64_t n = c.n, nSph = c.nSph;
+  const int64_t n = c.n, nW = c.nW, nSph = n - nW;
   const double vd = c.verletDist;
-  const double dfBox = c.field.distFactor > 0 ? c.field.distFactor : 1.;
   const bool periodic = c.field.periodic != 0;
   c.nRefresh++;
-
-  // refresh positions
-  hipLaunchKernelGGL(k_last_pos, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, n, c.pos[0].p, c.pos[1].p, c.pos[2].p,
-                     c.lastPos[0].p, c.lastPos[1].p, c.lastPos[2].p);
-  WDG_LAUNCH_CHECK();
 
   // grid: cell edge >= largest enlarged-AABB extent
   Grid g{};
   g.periodic = periodic;
-  double ext = 2. * (dfBox * c.rmax + vd);
+  double absDf = std::fabs(c.field.distFactor);
+  double ext = 2. * (absDf * c.rmax + vd);
   ext = ext * (1 + 1e-9) + 1e-300;
   bool brute = nSph < 512;
   if (periodic) {
@@ -419,12 +516,11 @@ static void refresh(Ctx& c) {
       g.org[k] = 0;
     }
   } else if (nSph > 0) {
-    c.dflags.ensure(64);
     unsigned long long init[6] = {~0ull, ~0ull, ~0ull, 0, 0, 0};
     unsigned long long* dbb = reinterpret_cast<unsigned long long*>(c.red.p);
     WDG_HIP(hipMemcpyAsync(dbb, init, sizeof(init), hipMemcpyHostToDevice, c.stream));
-    hipLaunchKernelGGL(k_bbox, dim3(nblocks(nSph, 256)), dim3(256), 0, c.stream, nSph, c.dWalls.p + c.wallIds.size(),
-                       c.pos[0].p, c.pos[1].p, c.pos[2].p, dbb);
+    hipLaunchKernelGGL(k_bbox, dim3(nblocks(nSph, 256)), dim3(256), 0, c.stream, nW, n, c.pos[0].p, c.pos[1].p,
+                       c.pos[2].p, dbb);
     WDG_LAUNCH_CHECK();
     unsigned long long hb[6];
     WDG_HIP(hipMemcpyAsync(hb, dbb, sizeof(hb), hipMemcpyDeviceToHost, c.stream));
@@ -448,58 +544,84 @@ static void refresh(Ctx& c) {
     for (int k = 0; k < 3; k++) g.org[k] = lo[k];
   }
   int64_t ncells = (int64_t)g.nc[0] * g.nc[1] * g.nc[2];
-  c.lastBrute = brute;
-  for (int k = 0; k < 3; k++) c.gridCells[k] = brute ? 0 : g.nc[k];
 
   AabbParams P{};
   P.vd = vd;
-  P.dfBox = dfBox;
+  P.absDf = absDf;
   P.loneMask = c.field.loneMask;
   P.periodic = periodic;
   for (int k = 0; k < 3; k++) P.L[k] = c.field.cell[k];
 
-  // sorted sphere order and cell ranges
+  // 1. spatial order of the spheres and the permutation of all particle arrays
   c.cellKey.ensure(nSph + 1); c.sortIdx.ensure(nSph + 1);
   c.cellKeyTmp.ensure(nSph + 1); c.sortIdxTmp.ensure(nSph + 1);
-  c.sx.ensure(nSph + 1); c.sy.ensure(nSph + 1); c.sz.ensure(nSph + 1); c.shs.ensure(nSph + 1);
-  c.smask.ensure(nSph + 1); c.scount.ensure(nSph + 1); c.soff.ensure(nSph + 1);
-  const int32_t* sphIds = c.dWalls.p + c.wallIds.size();
+  c.perm.ensure(n + 1); c.invPerm.ensure(n + 1);
+  c.scount.ensure(nSph + 1); c.soff.ensure(nSph + 1);
   if (nSph > 0) {
     Grid gk = g;
     if (brute) { for (int k = 0; k < 3; k++) { gk.nc[k] = 1; gk.h[k] = 1e300; gk.org[k] = 0; } gk.periodic = 0; }
-    hipLaunchKernelGGL(k_keys, dim3(nblocks(nSph, 256)), dim3(256), 0, c.stream, nSph, sphIds, c.pos[0].p, c.pos[1].p,
+    hipLaunchKernelGGL(k_keys, dim3(nblocks(nSph, 256)), dim3(256), 0, c.stream, nW, n, c.pos[0].p, c.pos[1].p,
                        c.pos[2].p, gk, c.cellKey.p, c.sortIdx.p);
     WDG_LAUNCH_CHECK();
-    if (!brute) {
+    if (!brute)
       radix_sort_pairs<uint32_t>(c, c.cellKey.p, c.sortIdx.p, c.cellKeyTmp.p, c.sortIdxTmp.p, nSph,
                                  bits_for((uint64_t)(ncells > 1 ? ncells - 1 : 1)));
-      c.cellStart.ensure(ncells);
-      c.cellEnd.ensure(ncells);
-      WDG_HIP(hipMemsetAsync(c.cellStart.p, 0, ncells * sizeof(uint32_t), c.stream));
-      WDG_HIP(hipMemsetAsync(c.cellEnd.p, 0, ncells * sizeof(uint32_t), c.stream));
-      hipLaunchKernelGGL(k_cell_ranges, dim3(nblocks(nSph, 256)), dim3(256), 0, c.stream, nSph, c.cellKey.p,
-                         c.cellStart.p, c.cellEnd.p);
-      WDG_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_build_perm, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, nW, n, c.sortIdx.p, c.perm.p,
+                     c.invPerm.p);
+  WDG_LAUNCH_CHECK();
+  {
+    PermArgs pa{};
+    pa.n = n;
+    pa.perm = c.perm.p;
+    DBuf<double>* d[NPD] = {&c.pos[0], &c.pos[1], &c.pos[2], &c.vel[0], &c.vel[1], &c.vel[2], &c.angVel[0],
+                            &c.angVel[1], &c.angVel[2], &c.ori[0], &c.ori[1], &c.ori[2], &c.ori[3], &c.force[0],
+                            &c.force[1], &c.force[2], &c.torque[0], &c.torque[1], &c.torque[2], &c.inertia[0],
+                            &c.inertia[1], &c.inertia[2], &c.mass, &c.radius};
+    DBuf<uint32_t>* u[NPU] = {&c.flags, &c.mask, &c.uid, reinterpret_cast<DBuf<uint32_t>*>(&c.mat)};
+    for (int k = 0; k < NPD; k++) {
+      c.spareD[k].ensure(n + 1);
+      pa.sd[k] = d[k]->p;
+      pa.dd[k] = c.spareD[k].p;
     }
-    hipLaunchKernelGGL(k_gather_sorted, dim3(nblocks(nSph, 256)), dim3(256), 0, c.stream, nSph, c.sortIdx.p,
-                       c.pos[0].p, c.pos[1].p, c.pos[2].p, c.radius.p, c.mask.p, dfBox, c.sx.p, c.sy.p, c.sz.p,
-                       c.shs.p, c.smask.p);
+    for (int k = 0; k < NPU; k++) {
+      c.spareU[k].ensure(n + 1);
+      pa.su[k] = u[k]->p;
+      pa.du[k] = c.spareU[k].p;
+    }
+    hipLaunchKernelGGL(k_permute, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, pa);
+    WDG_LAUNCH_CHECK();
+    for (int k = 0; k < NPD; k++) d[k]->swap(c.spareD[k]);
+    for (int k = 0; k < NPU; k++) u[k]->swap(c.spareU[k]);
+  }
+  hipLaunchKernelGGL(k_sto, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, n, c.uid.p, c.sto.p);
+  WDG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_last_pos, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, n, c.pos[0].p, c.pos[1].p, c.pos[2].p,
+                     c.lastPos[0].p, c.lastPos[1].p, c.lastPos[2].p);
+  WDG_LAUNCH_CHECK();
+  if (nSph > 0 && !brute) {
+    c.cellStart.ensure(ncells);
+    c.cellEnd.ensure(ncells);
+    WDG_HIP(hipMemsetAsync(c.cellStart.p, 0, ncells * sizeof(uint32_t), c.stream));
+    WDG_HIP(hipMemsetAsync(c.cellEnd.p, 0, ncells * sizeof(uint32_t), c.stream));
+    hipLaunchKernelGGL(k_cell_ranges, dim3(nblocks(nSph, 256)), dim3(256), 0, c.stream, nSph, nW, c.cellKey.p,
+                       c.cellStart.p, c.cellEnd.p);
     WDG_LAUNCH_CHECK();
   }
+  for (int k = 0; k < 3; k++) {
+    c.grid.org[k] = g.org[k]; c.grid.h[k] = g.h[k]; c.grid.L[k] = g.L[k]; c.grid.nc[k] = g.nc[k];
+  }
+  c.grid.periodic = periodic;
+  c.grid.brute = brute;
+  c.grid.valid = true;
 
+  // 2. pairs that can touch before the next refresh
   const int idBits = bits_for((uint64_t)(n > 1 ? n - 1 : 1));
   const uint64_t sentinel = (2 * idBits >= 64) ? ~0ull : ((1ull << (2 * idBits)) - 1);
-
-  PairArgs pa{};
-  pa.nSph = nSph; pa.key = c.cellKey.p; pa.idx = c.sortIdx.p;
-  pa.sx = c.sx.p; pa.sy = c.sy.p; pa.sz = c.sz.p; pa.shs = c.shs.p; pa.smask = c.smask.p;
-  pa.cstart = c.cellStart.p; pa.cend = c.cellEnd.p; pa.g = g; pa.P = P; pa.brute = brute;
-  pa.nWalls = (int)c.wallIds.size(); pa.walls = c.dWalls.p;
-  pa.px = c.pos[0].p; pa.py = c.pos[1].p; pa.pz = c.pos[2].p; pa.flags = c.flags.p; pa.mask = c.mask.p;
-  pa.idBits = idBits; pa.count = c.scount.p; pa.off = c.soff.p; pa.dflags = c.dflags.p;
+  PairArgs pa = pair_args(c, g, P, idBits);
   int64_t nPairs = 0;
   if (nSph > 0) {
-    hipLaunchKernelGGL(k_pairs<false>, dim3(nblocks(nSph, 256)), dim3(256), 0, c.stream, pa);
+    hipLaunchKernelGGL((k_pairs<false, MODE_NEAR>), dim3(nblocks(nSph, 256)), dim3(256), 0, c.stream, pa);
     WDG_LAUNCH_CHECK();
     scan_exclusive(c, c.scount.p, c.soff.p, nSph, c.dflags.p + 4);
     nPairs = read_u32(c, c.dflags.p + 4);
@@ -512,12 +634,13 @@ static void refresh(Ctx& c) {
   if (periodic) c.pairDist.ensure(nPairs + 1);
   if (nPairs > 0) {
     pa.pkey = c.pairKey.p; pa.pinfo = c.pairInfo.p; pa.pdist = c.pairDist.p;
-    hipLaunchKernelGGL(k_pairs<true>, dim3(nblocks(nSph, 256)), dim3(256), 0, c.stream, pa);
+    hipLaunchKernelGGL((k_pairs<true, MODE_NEAR>), dim3(nblocks(nSph, 256)), dim3(256), 0, c.stream, pa);
     WDG_LAUNCH_CHECK();
   }
+  // 3. merge with the old real contacts (history)
   if (nOld > 0) {
     hipLaunchKernelGGL(k_append_old, dim3(nblocks(nOld, 256)), dim3(256), 0, c.stream, nOld, nPairs, O.A.p, O.B.p,
-                       O.real.p, idBits, sentinel, c.pairKey.p);
+                       O.real.p, c.invPerm.p, idBits, sentinel, c.pairKey.p);
     WDG_LAUNCH_CHECK();
   }
   int64_t nNew = 0;
@@ -537,8 +660,9 @@ static void refresh(Ctx& c) {
   if (nNew > 0) {
     CompactArgs ca{};
     ca.n = tot; ca.nPairs = nPairs; ca.key = c.pairKey.p; ca.val = c.pairVal.p; ca.flag = c.runFlag.p;
-    ca.off = c.runOff.p; ca.pinfo = c.pairInfo.p; ca.pdist = c.pairDist.p; ca.idBits = idBits; ca.periodic = periodic;
-    ca.oInfo = O.info.p; ca.oDist = O.dist.p;
+    ca.off = c.runOff.p; ca.pinfo = c.pairInfo.p; ca.pdist = c.pairDist.p; ca.invPerm = c.invPerm.p;
+    ca.idBits = idBits; ca.periodic = periodic;
+    ca.oA = O.A.p; ca.oB = O.B.p; ca.oInfo = O.info.p; ca.oDist = O.dist.p;
     ca.oN0 = O.hN[0].p; ca.oN1 = O.hN[1].p; ca.oN2 = O.hN[2].p;
     ca.oY0 = O.hY[0].p; ca.oY1 = O.hY[1].p; ca.oY2 = O.hY[2].p;
     ca.oP0 = O.hP[0].p; ca.oP1 = O.hP[1].p; ca.oP2 = O.hP[2].p;
@@ -551,34 +675,26 @@ static void refresh(Ctx& c) {
     hipLaunchKernelGGL(k_compact, dim3(nblocks(tot, 256)), dim3(256), 0, c.stream, ca);
     WDG_LAUNCH_CHECK();
   }
-  // CSR over min id, and the (max id, min id)-ordered transpose for the gather
-  hipLaunchKernelGGL(k_offsets, dim3(nblocks(nNew + 1, 256)), dim3(256), 0, c.stream, nNew, n, N.A.p, N.offA.p);
-  WDG_LAUNCH_CHECK();
-  if (nNew > 0) {
-    c.keyB.ensure(nNew); c.keyBTmp.ensure(nNew); c.valBTmp.ensure(nNew);
-    hipLaunchKernelGGL(k_copy_u32, dim3(nblocks(nNew, 256)), dim3(256), 0, c.stream, nNew, N.B.p, c.keyB.p);
+  // 4. per-particle adjacency in canonical order (particle storage, other user id)
+  const int64_t nAdj = 2 * nNew;
+  if (nAdj > 0) {
+    c.adjKey.ensure(nAdj); c.adjKeyTmp.ensure(nAdj); c.adjValTmp.ensure(nAdj); c.adjPart.ensure(nAdj);
+    hipLaunchKernelGGL(k_adj_keys, dim3(nblocks(nNew, 256)), dim3(256), 0, c.stream, nNew, N.A.p, N.B.p, N.info.p,
+                       c.uid.p, idBits, c.adjKey.p, N.adj.p);
     WDG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_iota, dim3(nblocks(nNew, 256)), dim3(256), 0, c.stream, nNew, N.idxB.p);
+    radix_sort_pairs<uint64_t>(c, c.adjKey.p, N.adj.p, c.adjKeyTmp.p, c.adjValTmp.p, nAdj, 2 * idBits);
+    hipLaunchKernelGGL(k_key_hi, dim3(nblocks(nAdj, 256)), dim3(256), 0, c.stream, nAdj, c.adjKey.p, idBits,
+                       c.adjPart.p);
     WDG_LAUNCH_CHECK();
-    radix_sort_pairs<uint32_t>(c, c.keyB.p, N.idxB.p, c.keyBTmp.p, c.valBTmp.p, nNew, idBits);
   }
-  hipLaunchKernelGGL(k_offsets, dim3(nblocks(nNew + 1, 256)), dim3(256), 0, c.stream, nNew, n,
-                     nNew > 0 ? c.keyB.p : N.A.p, N.offB.p);
+  hipLaunchKernelGGL(k_offsets, dim3(nblocks(nAdj + 1, 256)), dim3(256), 0, c.stream, nAdj, n,
+                     nAdj > 0 ? c.adjPart.p : N.A.p, N.adjOff.p);
   WDG_LAUNCH_CHECK();
   c.cur = 1 - c.cur;
   c.boundsValid = true;
-  // clean after the refresh; device error bits are checked now
   WDG_HIP(hipMemsetAsync(c.dflags.p, 0, sizeof(uint32_t), c.stream));
   uint32_t err = read_u32(c, c.dflags.p + 1);
   c.dirtyKnown = true;
-  if (err) {
-    WDG_HIP(hipMemsetAsync(c.dflags.p + 1, 0, sizeof(uint32_t), c.stream));
-    check_device_errors(c, err);
-  }
-}
-
-void check_errors(Ctx& c) {
-  uint32_t err = read_u32(c, c.dflags.p + 1);
   if (err) {
     WDG_HIP(hipMemsetAsync(c.dflags.p + 1, 0, sizeof(uint32_t), c.stream));
     check_device_errors(c, err);
@@ -616,6 +732,50 @@ void collide(Ctx& c, const wdg_collide_params* p) {
     if (c.hflags[0]) full = true;
   }
   if (full) refresh(c);
+}
+
+// potential set of the last refresh: enlarged-AABB pairs from the refresh positions
+void potential_pairs(Ctx& c, std::vector<uint32_t>& ida, std::vector<uint32_t>& idb, std::vector<uint64_t>& cd) {
+  ida.clear(); idb.clear(); cd.clear();
+  const int64_t nSph = c.n - c.nW;
+  if (!c.grid.valid || nSph <= 0) return;
+  Grid g{};
+  for (int k = 0; k < 3; k++) {
+    g.org[k] = c.grid.org[k]; g.h[k] = c.grid.h[k]; g.L[k] = c.grid.L[k]; g.nc[k] = c.grid.nc[k];
+  }
+  g.periodic = c.grid.periodic;
+  AabbParams P{};
+  P.vd = c.verletDist;
+  P.absDf = std::fabs(c.field.distFactor);
+  P.loneMask = c.field.loneMask;
+  P.periodic = c.field.periodic;
+  for (int k = 0; k < 3; k++) P.L[k] = c.field.cell[k];
+  PairArgs pa = pair_args(c, g, P, 32);
+  hipLaunchKernelGGL((k_pairs<false, MODE_AABB>), dim3(nblocks(nSph, 256)), dim3(256), 0, c.stream, pa);
+  WDG_LAUNCH_CHECK();
+  scan_exclusive(c, c.scount.p, c.soff.p, nSph, c.dflags.p + 4);
+  int64_t np = read_u32(c, c.dflags.p + 4);
+  if (np == 0) return;
+  DBuf<uint64_t> k, d;
+  k.ensure(np);
+  d.ensure(np);
+  pa.pkey = k.p; pa.pdist = d.p;
+  hipLaunchKernelGGL((k_pairs<true, MODE_AABB>), dim3(nblocks(nSph, 256)), dim3(256), 0, c.stream, pa);
+  WDG_LAUNCH_CHECK();
+  std::vector<uint64_t> hk((size_t)np), hd;
+  WDG_HIP(hipMemcpyAsync(hk.data(), k.p, np * 8, hipMemcpyDeviceToHost, c.stream));
+  if (c.field.periodic) {
+    hd.resize((size_t)np);
+    WDG_HIP(hipMemcpyAsync(hd.data(), d.p, np * 8, hipMemcpyDeviceToHost, c.stream));
+  }
+  WDG_HIP(hipStreamSynchronize(c.stream));
+  k.release();
+  d.release();
+  for (int64_t i = 0; i < np; i++) {
+    ida.push_back((uint32_t)(hk[i] >> 32));
+    idb.push_back((uint32_t)(hk[i] & 0xFFFFFFFFu));
+    cd.push_back(c.field.periodic ? hd[i] : 0);
+  }
 }
 
 }  // namespace wdg

@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void k_contact(ContactArgs a) {
   if (c >= a.nc) return;
   const uint32_t info = a.info[c];
   const bool real = a.real[c] != 0u;
-  if (!real && !(info & CI_AABB)) return;
+  if (!real && !(info & CI_NEAR)) return;
   const uint32_t iA = a.A[c], iB = a.B[c];
   const bool swap = (info & CI_SWAP) != 0u;
   const uint32_t pA = swap ? iB : iA, pB = swap ? iA : iB;

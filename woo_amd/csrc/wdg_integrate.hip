@@ -1,12 +1,12 @@
-// Per-particle kernels: deterministic force/torque gather over the sorted
-// contact slots (no float atomics), fused into the leapfrog integrator
+// Per-particle kernels: deterministic force/torque gather over the contact
+// slots (no float atomics), fused into the leapfrog integrator
 // (Leapfrog.cpp:104-297, spherical integrator, gradV zero), ForceResetter
 // (Leapfrog.cpp:15-26), DynDt stiffness (DynDt.cpp:16-71) and observables
 // (Funcs.cpp:84-114).
 //
-// Gather order for particle i: slots where i is the larger id, in increasing
-// order of the smaller id, then slots where i is the smaller id, in increasing
-// order of the larger id, i.e. increasing order of the other particle's id.
+// Each particle walks its adjacency list, which holds its slots in increasing
+// order of the other particle's user id: the order of Particle::contacts (a
+// std::map keyed by the other id), so sums are reproducible bit for bit.
 #include <cstring>
 
 #include "wdg_internal.hpp"
@@ -17,37 +17,32 @@ namespace wdg {
 
 struct GatherArgs {
   int64_t n;
-  const uint32_t *A, *B, *info, *real;
+  const uint32_t *adj, *adjOff, *real;
   const uint64_t* dist;
   const double *P0, *P1, *P2, *G0, *G1, *G2;
-  const uint32_t *offA, *offB, *idxB;
   double L[3];
 };
 
 template <bool PERIODIC>
-__device__ __forceinline__ void gather_one(const GatherArgs& g, uint32_t i, uint32_t slot, d3 pos, d3& F, d3& T) {
-  if (!g.real[slot]) return;
-  const uint32_t inf = g.info[slot];
-  const uint32_t pA = (inf & CI_SWAP) ? g.B[slot] : g.A[slot];
-  const bool isPA = (pA == i);
-  const double sign = isPA ? 1. : -1.;
-  d3 Fc = mul3(mk3(g.G0[slot], g.G1[slot], g.G2[slot]), sign);
-  d3 shift = mk3(0., 0., 0.);
-  if (PERIODIC && !isPA) {
-    uint64_t cd = g.dist[slot];
-    shift = mk3(g.L[0] * unpack_cd(cd, 0), g.L[1] * unpack_cd(cd, 1), g.L[2] * unpack_cd(cd, 2));
+__device__ __forceinline__ void gather_all(const GatherArgs& g, int64_t i, d3 pos, d3& F, d3& T) {
+  const uint32_t k0 = g.adjOff[i], k1 = g.adjOff[i + 1];
+  for (uint32_t k = k0; k < k1; k++) {
+    const uint32_t e = g.adj[k];
+    const uint32_t slot = e & ~ADJ_PA;
+    if (!g.real[slot]) continue;
+    const bool isPA = (e & ADJ_PA) != 0u;
+    const double sign = isPA ? 1. : -1.;
+    d3 Fc = mul3(mk3(g.G0[slot], g.G1[slot], g.G2[slot]), sign);
+    d3 shift = mk3(0., 0., 0.);
+    if (PERIODIC && !isPA) {
+      uint64_t cd = g.dist[slot];
+      shift = mk3(g.L[0] * unpack_cd(cd, 0), g.L[1] * unpack_cd(cd, 1), g.L[2] * unpack_cd(cd, 2));
+    }
+    // Contact::getForceTorqueBranch: xc = contact point - (node pos + periodic shift of pB)
+    d3 xc = sub3(mk3(g.P0[slot], g.P1[slot], g.P2[slot]), add3(pos, shift));
+    F = add3(F, Fc);
+    T = add3(T, cross3(xc, Fc));
   }
-  d3 xc = sub3(mk3(g.P0[slot], g.P1[slot], g.P2[slot]), add3(pos, shift));
-  F = add3(F, Fc);
-  T = add3(T, cross3(xc, Fc));
-}
-
-template <bool PERIODIC>
-__device__ __forceinline__ void gather_all(const GatherArgs& g, uint32_t i, d3 pos, d3& F, d3& T) {
-  const uint32_t b0 = g.offB[i], b1 = g.offB[i + 1];
-  for (uint32_t k = b0; k < b1; k++) gather_one<PERIODIC>(g, i, g.idxB[k], pos, F, T);
-  const uint32_t a0 = g.offA[i], a1 = g.offA[i + 1];
-  for (uint32_t s = a0; s < a1; s++) gather_one<PERIODIC>(g, i, s, pos, F, T);
 }
 
 template <bool PERIODIC>
@@ -56,9 +51,9 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs g, double* fx, double
                                                  const double* __restrict__ py, const double* __restrict__ pz) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.n) return;
-  if (g.offB[i] == g.offB[i + 1] && g.offA[i] == g.offA[i + 1]) return;
+  if (g.adjOff[i] == g.adjOff[i + 1]) return;
   d3 F = mk3(fx[i], fy[i], fz[i]), T = mk3(tx[i], ty[i], tz[i]);
-  gather_all<PERIODIC>(g, (uint32_t)i, mk3(px[i], py[i], pz[i]), F, T);
+  gather_all<PERIODIC>(g, i, mk3(px[i], py[i], pz[i]), F, T);
   fx[i] = F.x; fy[i] = F.y; fz[i] = F.z;
   tx[i] = T.x; ty[i] = T.y; tz[i] = T.z;
 }
@@ -88,7 +83,7 @@ __global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
     const bool integrated = (fl & F_INTEGRATED) != 0u;
     d3 pos = mk3(a.px[i], a.py[i], a.pz[i]);
     d3 F = mk3(a.fx[i], a.fy[i], a.fz[i]), T = mk3(a.tx[i], a.ty[i], a.tz[i]);
-    if (a.gather) gather_all<PERIODIC>(g, (uint32_t)i, pos, F, T);
+    if (a.gather) gather_all<PERIODIC>(g, i, pos, F, T);
     if (!integrated) {
       if (a.gather) {
         a.fx[i] = F.x; a.fy[i] = F.y; a.fz[i] = F.z;
@@ -178,10 +173,9 @@ static GatherArgs gather_args(Ctx& c) {
   Contacts& C = c.con();
   GatherArgs g{};
   g.n = c.n;
-  g.A = C.A.p; g.B = C.B.p; g.info = C.info.p; g.real = C.real.p; g.dist = C.dist.p;
+  g.adj = C.adj.p; g.adjOff = C.adjOff.p; g.real = C.real.p; g.dist = C.dist.p;
   g.P0 = C.hP[0].p; g.P1 = C.hP[1].p; g.P2 = C.hP[2].p;
   g.G0 = C.F[0].p; g.G1 = C.F[1].p; g.G2 = C.F[2].p;
-  g.offA = C.offA.p; g.offB = C.offB.p; g.idxB = C.idxB.p;
   for (int k = 0; k < 3; k++) g.L[k] = c.field.cell[k];
   return g;
 }
@@ -221,10 +215,8 @@ void leapfrog(Ctx& c, const wdg_leapfrog_params* p) {
   a.checkDirty = c.boundsValid && c.verletDist > 0 ? 1 : 0;
   a.maxD2 = c.verletDist * c.verletDist;
   a.dflags = c.dflags.p;
-  if (a.checkDirty) {
-    // the flag must describe the positions after this step only
-    WDG_HIP(hipMemsetAsync(c.dflags.p, 0, sizeof(uint32_t), c.stream));
-  }
+  // the flag must describe the positions after this step only
+  if (a.checkDirty) WDG_HIP(hipMemsetAsync(c.dflags.p, 0, sizeof(uint32_t), c.stream));
   GatherArgs g = gather_args(c);
   dim3 grid(nblocks(c.n, 256)), block(256);
   if (c.field.periodic) hipLaunchKernelGGL(k_leapfrog<true>, grid, block, 0, c.stream, a, g);
@@ -246,9 +238,8 @@ void force_reset(Ctx& c) {
 // ------------------------------------------------------------------ DynDt
 struct DynArgs {
   int64_t n;
-  const uint32_t *A, *B, *info, *real;
+  const uint32_t *A, *B, *info, *real, *adj, *adjOff;
   const double *N0, *N1, *N2, *Y0, *Y1, *Y2, *uN;
-  const uint32_t *offA, *offB, *idxB;
   const double *rad, *mass, *ix, *iy, *iz;
   const uint32_t* flags;
   const int32_t* mat;
@@ -258,12 +249,13 @@ struct DynArgs {
   unsigned long long* out;
 };
 
-__device__ __forceinline__ void dyn_one(const DynArgs& a, uint32_t i, uint32_t s, d3& kt, d3& kr) {
+__device__ __forceinline__ void dyn_one(const DynArgs& a, uint32_t e, d3& kt, d3& kr) {
+  const uint32_t s = e & ~ADJ_PA;
   if (!a.real[s]) return;
   const uint32_t inf = a.info[s];
   const bool swap = (inf & CI_SWAP) != 0u;
   const uint32_t pA = swap ? a.B[s] : a.A[s], pB = swap ? a.A[s] : a.B[s];
-  const int ix = (pA == i) ? 0 : 1;
+  const int ix = (e & ADJ_PA) ? 0 : 1;
   double r1, r2;
   if ((inf & CI_TYPE_MASK) == CT_SS) { r1 = a.rad[pA]; r2 = a.rad[pB]; }
   else { r1 = -a.rad[pB]; r2 = a.rad[pB]; }
@@ -300,8 +292,7 @@ __global__ __launch_bounds__(256) void k_dyndt(DynArgs a) {
     const uint32_t fl = a.flags[i];
     if ((fl & F_INTEGRATED) && (fl & F_DOF_ALL) != F_DOF_ALL) {
       d3 kt = mk3(0., 0., 0.), kr = mk3(0., 0., 0.);
-      for (uint32_t k = a.offB[i]; k < a.offB[i + 1]; k++) dyn_one(a, (uint32_t)i, a.idxB[k], kt, kr);
-      for (uint32_t s = a.offA[i]; s < a.offA[i + 1]; s++) dyn_one(a, (uint32_t)i, s, kt, kr);
+      for (uint32_t k = a.adjOff[i]; k < a.adjOff[i + 1]; k++) dyn_one(a, a.adj[k], kt, kr);
       const double m = a.mass[i];
       const double I[3] = {a.ix[i], a.iy[i], a.iz[i]};
       const double ktv[3] = {kt.x, kt.y, kt.z}, krv[3] = {kr.x, kr.y, kr.z};
@@ -327,10 +318,9 @@ double crit_dt(Ctx& c) {
   if (C.n == 0) return INFINITY;
   DynArgs a{};
   a.n = c.n;
-  a.A = C.A.p; a.B = C.B.p; a.info = C.info.p; a.real = C.real.p;
+  a.A = C.A.p; a.B = C.B.p; a.info = C.info.p; a.real = C.real.p; a.adj = C.adj.p; a.adjOff = C.adjOff.p;
   a.N0 = C.hN[0].p; a.N1 = C.hN[1].p; a.N2 = C.hN[2].p;
   a.Y0 = C.hY[0].p; a.Y1 = C.hY[1].p; a.Y2 = C.hY[2].p; a.uN = C.huN.p;
-  a.offA = C.offA.p; a.offB = C.offB.p; a.idxB = C.idxB.p;
   a.rad = c.radius.p; a.mass = c.mass.p; a.ix = c.inertia[0].p; a.iy = c.inertia[1].p; a.iz = c.inertia[2].p;
   a.flags = c.flags.p; a.mat = c.mat.p; a.mats = c.dMats.p;
   a.hertz = c.lastPhysKind == WDG_PHYS_HERTZ;
@@ -350,14 +340,12 @@ double crit_dt(Ctx& c) {
 }
 
 // ------------------------------------------------------------------ observables
-__global__ void k_count_contacts(int64_t n, const uint32_t* __restrict__ real, const uint32_t* __restrict__ offA,
-                                 const uint32_t* __restrict__ offB, const uint32_t* __restrict__ idxB,
-                                 uint32_t* cnt) {
+__global__ void k_count_contacts(int64_t n, const uint32_t* __restrict__ real, const uint32_t* __restrict__ adj,
+                                 const uint32_t* __restrict__ adjOff, uint32_t* cnt) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t k = 0;
-  for (uint32_t j = offB[i]; j < offB[i + 1]; j++) k += real[idxB[j]];
-  for (uint32_t s = offA[i]; s < offA[i + 1]; s++) k += real[s];
+  for (uint32_t j = adjOff[i]; j < adjOff[i + 1]; j++) k += real[adj[j] & ~ADJ_PA];
   cnt[i] = k;
 }
 
@@ -400,8 +388,8 @@ void observables(Ctx& c, const double lo[3], const double hi[3], double out[3]) 
   Contacts& C = c.con();
   c.scount.ensure(c.n + 1);
   if (C.n > 0 && c.n > 0) {
-    hipLaunchKernelGGL(k_count_contacts, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, c.n, C.real.p, C.offA.p,
-                       C.offB.p, C.idxB.p, c.scount.p);
+    hipLaunchKernelGGL(k_count_contacts, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, c.n, C.real.p, C.adj.p,
+                       C.adjOff.p, c.scount.p);
     WDG_LAUNCH_CHECK();
   } else {
     WDG_HIP(hipMemsetAsync(c.scount.p, 0, (c.n + 1) * sizeof(uint32_t), c.stream));

@@ -1,4 +1,9 @@
 // Internal device state of the DEM step engine (not part of the C ABI).
+//
+// Particles are stored in "storage order": walls first (user-id order), then
+// spheres sorted by broadphase cell, re-sorted at every broadphase refresh so
+// that neighbours in space are neighbours in memory. uid[] / sto[] map storage
+// index <-> user id (Particle.id); the C ABI speaks user ids only.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -6,6 +11,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/woo_amd.h"
@@ -44,10 +50,11 @@ __host__ __device__ inline int flag_sense(uint32_t f) { return (int)((f >> F_SEN
 // ------------------------------------------------------------------ contact info word
 enum : uint32_t {
   CI_TYPE_MASK = 3u,   // 0 sphere-sphere, 1 wall-sphere
-  CI_SWAP = 4u,        // geometric first particle (pA) is the larger id
-  CI_AABB = 8u,        // enlarged AABBs overlapped at the last refresh
+  CI_SWAP = 4u,        // geometric first particle (pA) is B (the larger storage index)
+  CI_NEAR = 8u,        // close enough at the last refresh to touch before the next one
 };
 enum { CT_SS = 0, CT_WS = 1 };
+constexpr uint32_t ADJ_PA = 0x80000000u;  // adjacency entry: this particle is pA of the slot
 
 // periodic image offsets packed 3 x 21 bits (two's complement)
 __host__ __device__ inline uint64_t pack_cd(int a, int b, int c) {
@@ -59,7 +66,6 @@ __host__ __device__ inline int unpack_cd(uint64_t v, int k) {
   if (f & (1ll << 20)) f -= (1ll << 21);
   return (int)f;
 }
-constexpr int CD_LIMIT = (1 << 20) - 1;
 
 // device error bits
 enum : uint32_t { ERR_NAN_FORCE = 1u, ERR_DOUBLE_IMAGE = 2u, ERR_CD_RANGE = 4u };
@@ -77,29 +83,21 @@ struct DBuf {
     WDG_HIP(hipMalloc(&p, c * sizeof(T)));
     cap = c;
   }
-  // grow keeping the first `keep` elements
-  void grow(size_t n, size_t keep, hipStream_t s) {
-    if (n <= cap && p) return;
-    size_t c = n < 64 ? 64 : n + n / 2;
-    T* q = nullptr;
-    WDG_HIP(hipMalloc(&q, c * sizeof(T)));
-    if (p && keep) WDG_HIP(hipMemcpyAsync(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice, s));
-    WDG_HIP(hipStreamSynchronize(s));
-    if (p) WDG_HIP(hipFree(p));
-    p = q;
-    cap = c;
-  }
   void release() {
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
   }
+  void swap(DBuf& o) {
+    std::swap(p, o.p);
+    std::swap(cap, o.cap);
+  }
 };
 
-// one contact set: slots sorted by (min id, max id); SoA
+// one contact set: slots sorted by (lo, hi) storage index; SoA
 struct Contacts {
   int64_t n = 0;
-  DBuf<uint32_t> A, B;        // min id, max id
+  DBuf<uint32_t> A, B;        // lo / hi storage index
   DBuf<uint32_t> info;        // CI_* bits
   DBuf<uint64_t> dist;        // packed cellDist of pB relative to pA (periodic only)
   DBuf<uint32_t> real;        // 1 if geom+phys exist (history valid)
@@ -108,23 +106,27 @@ struct Contacts {
   DBuf<double> hF[3];         // CPhys.force, local frame
   DBuf<double> huN;           // L6Geom.uN
   DBuf<double> F[3];          // global force on pA, written by the contact kernel for the gather
-  DBuf<uint32_t> offA, offB;  // CSR over min id (slots) and over max id (into idxB), size n+1
-  DBuf<uint32_t> idxB;        // slots sorted by (max id, min id)
+  // per-particle adjacency: entries slot|ADJ_PA grouped by storage index, each group in
+  // increasing order of the other particle's user id (the canonical summation order)
+  DBuf<uint32_t> adj, adjOff;
   void ensure(int64_t nc, int64_t npar, bool periodic) {
     size_t c = (size_t)(nc > 0 ? nc : 1);
     A.ensure(c); B.ensure(c); info.ensure(c); real.ensure(c);
     if (periodic) dist.ensure(c);
     for (int k = 0; k < 3; k++) { hN[k].ensure(c); hY[k].ensure(c); hP[k].ensure(c); hF[k].ensure(c); F[k].ensure(c); }
     huN.ensure(c);
-    idxB.ensure(c);
-    offA.ensure((size_t)npar + 1); offB.ensure((size_t)npar + 1);
+    adj.ensure(2 * c);
+    adjOff.ensure((size_t)npar + 1);
   }
   void release() {
     A.release(); B.release(); info.release(); dist.release(); real.release();
     for (int k = 0; k < 3; k++) { hN[k].release(); hY[k].release(); hP[k].release(); hF[k].release(); F[k].release(); }
-    huN.release(); offA.release(); offB.release(); idxB.release();
+    huN.release(); adj.release(); adjOff.release();
   }
 };
+
+constexpr int NPD = 24;  // permuted double arrays per particle
+constexpr int NPU = 4;   // permuted u32 arrays per particle
 
 struct Ctx {
   int device = 0;
@@ -133,16 +135,16 @@ struct Ctx {
   std::vector<wdg_material> mats;
   DBuf<wdg_material> dMats;
 
-  // particles (SoA, one node each)
+  // particles, storage order
   int64_t n = 0;
-  int64_t nSph = 0;
-  std::vector<int32_t> wallIds;
-  DBuf<int32_t> dWalls;
+  int64_t nW = 0;       // walls occupy storage [0, nW)
   double rmin = INFINITY, rmax = 0;
   DBuf<double> pos[3], vel[3], angVel[3], ori[4], force[3], torque[3], inertia[3], lastPos[3];
   DBuf<double> mass, radius;
-  DBuf<uint32_t> flags, mask;
+  DBuf<uint32_t> flags, mask, uid, sto;
   DBuf<int32_t> mat;
+  DBuf<double> spareD[NPD];
+  DBuf<uint32_t> spareU[NPU];
 
   // contacts, double-buffered across refreshes
   Contacts C[2];
@@ -150,18 +152,28 @@ struct Ctx {
   Contacts& con() { return C[cur]; }
 
   // broadphase scratch
-  DBuf<uint32_t> cellKey, sortIdx, cellKeyTmp, sortIdxTmp;
+  DBuf<uint32_t> cellKey, sortIdx, cellKeyTmp, sortIdxTmp, perm, invPerm;
   DBuf<uint32_t> cellStart, cellEnd;
-  DBuf<double> sx, sy, sz, shs;  // sorted refresh positions and AABB half sizes
-  DBuf<uint32_t> smask, scount, soff;
+  DBuf<uint32_t> scount, soff;
   DBuf<uint64_t> pairKey, pairKeyTmp, pairDist;
   DBuf<uint32_t> pairInfo, pairVal, pairValTmp;
   DBuf<uint32_t> runFlag, runOff;
-  DBuf<uint32_t> keyB, keyBTmp, valB, valBTmp;
+  DBuf<uint64_t> adjKey, adjKeyTmp;
+  DBuf<uint32_t> adjValTmp, adjPart;
   DBuf<uint32_t> sortHist, scanAux;
-  DBuf<double> red;  // reductions
+  DBuf<double> red;       // reductions
+  DBuf<double> xfer;      // user-id-ordered staging for upload/download
   DBuf<uint32_t> dflags;  // [0] dirty, [1] error bits, [2..] scratch counters
   uint32_t* hflags = nullptr;  // pinned host mirror
+
+  // grid of the last refresh (for the on-demand potential-set report)
+  struct GridState {
+    double org[3], h[3], L[3];
+    int nc[3];
+    int periodic;
+    bool brute;
+    bool valid = false;
+  } grid;
 
   // engine state
   bool boundsValid = false;
@@ -169,8 +181,6 @@ struct Ctx {
   bool forcesPending = false;  // contact forces not yet gathered into force/torque
   double verletDist = NAN;
   int64_t nRefresh = 0, nCollide = 0;
-  bool lastBrute = false;
-  int gridCells[3] = {0, 0, 0};
   int64_t dynStepLast = -1, dynStepPrev = -1;
   // contact model of the last contact loop (DynDt and contact export need it)
   int lastPhysKind = WDG_PHYS_FRICT;
@@ -184,6 +194,8 @@ void radix_sort_pairs(Ctx& c, K* keys, uint32_t* vals, K* keysTmp, uint32_t* val
 uint32_t read_u32(Ctx& c, const uint32_t* d);
 
 void collide(Ctx& c, const wdg_collide_params* p);
+// potential pairs (enlarged AABBs of the last refresh overlapping) as user-id pairs (pA,pB) + cellDist
+void potential_pairs(Ctx& c, std::vector<uint32_t>& ida, std::vector<uint32_t>& idb, std::vector<uint64_t>& cd);
 void contact_loop(Ctx& c, const wdg_contact_params* p);
 void leapfrog(Ctx& c, const wdg_leapfrog_params* p);
 void force_reset(Ctx& c);
@@ -193,6 +205,7 @@ void observables(Ctx& c, const double lo[3], const double hi[3], double out[3]);
 int64_t count_real(Ctx& c);
 void check_errors(Ctx& c);
 void relax(Ctx& c, int maxIter, double inflate, double tol, double* maxOverlap, int* iters);
+void drop_contacts(Ctx& c);
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 inline int bits_for(uint64_t maxval) {

@@ -23,10 +23,10 @@ struct RelaxArgs {
   const double *px, *py, *pz, *rad;
   double L[3];
   int periodic;
-  double grow;  // 1 + inflate
   unsigned long long* maxOv;
 };
 
+// push of pB away from pA along the centre line (pA gets the opposite)
 __global__ __launch_bounds__(256) void k_relax_pairs(RelaxArgs a) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double ov = 0.;
@@ -34,7 +34,8 @@ __global__ __launch_bounds__(256) void k_relax_pairs(RelaxArgs a) {
     a.active[c] = 0u;
     const uint32_t inf = a.info[c];
     if ((inf & CI_TYPE_MASK) == CT_SS) {
-      const uint32_t i = a.A[c], j = a.B[c];
+      const bool swap = (inf & CI_SWAP) != 0u;
+      const uint32_t i = swap ? a.B[c] : a.A[c], j = swap ? a.A[c] : a.B[c];
       d3 s = mk3(0., 0., 0.);
       if (a.periodic) {
         uint64_t cd = a.dist[c];
@@ -42,8 +43,7 @@ __global__ __launch_bounds__(256) void k_relax_pairs(RelaxArgs a) {
       }
       d3 d = sub3(add3(mk3(a.px[j], a.py[j], a.pz[j]), s), mk3(a.px[i], a.py[i], a.pz[i]));
       double dist = sqrt(sqn3(d));
-      double rr = (a.rad[i] + a.rad[j]) * a.grow;
-      ov = rr - dist;
+      ov = (a.rad[i] + a.rad[j]) - dist;
       if (ov > 0 && dist > 0) {
         d3 p = mul3(d, 0.5 * ov / dist);
         a.G0[c] = p.x;
@@ -64,20 +64,21 @@ __global__ __launch_bounds__(256) void k_relax_pairs(RelaxArgs a) {
 }
 
 __global__ __launch_bounds__(256) void k_relax_move(int64_t n, const uint32_t* __restrict__ flags,
-                                                     const uint32_t* __restrict__ offA, const uint32_t* __restrict__ offB,
-                                                     const uint32_t* __restrict__ idxB, const uint32_t* __restrict__ active,
-                                                     const double* __restrict__ G0, const double* __restrict__ G1,
-                                                     const double* __restrict__ G2, double* px, double* py, double* pz) {
+                                                     const uint32_t* __restrict__ adj,
+                                                     const uint32_t* __restrict__ adjOff,
+                                                     const uint32_t* __restrict__ active, const double* __restrict__ G0,
+                                                     const double* __restrict__ G1, const double* __restrict__ G2,
+                                                     double* px, double* py, double* pz) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (flag_shape(flags[i]) != WDG_SPHERE || !(flags[i] & F_INTEGRATED)) return;
   d3 m = mk3(0., 0., 0.);
-  for (uint32_t k = offB[i]; k < offB[i + 1]; k++) {
-    uint32_t s = idxB[k];
-    if (active[s]) m = add3(m, mk3(G0[s], G1[s], G2[s]));
+  for (uint32_t k = adjOff[i]; k < adjOff[i + 1]; k++) {
+    const uint32_t e = adj[k], s = e & ~ADJ_PA;
+    if (!active[s]) continue;
+    d3 p = mk3(G0[s], G1[s], G2[s]);
+    m = (e & ADJ_PA) ? sub3(m, p) : add3(m, p);
   }
-  for (uint32_t s = offA[i]; s < offA[i + 1]; s++)
-    if (active[s]) m = sub3(m, mk3(G0[s], G1[s], G2[s]));
   px[i] += m.x;
   py[i] += m.y;
   pz[i] += m.z;
@@ -91,13 +92,28 @@ __global__ void k_scale(int64_t n, const double* __restrict__ in, double s, doub
 void relax(Ctx& c, int maxIter, double inflate, double tol, double* maxOverlap, int* iters) {
   if (c.n == 0) { *maxOverlap = 0; *iters = 0; return; }
   if (c.forcesPending) flush_forces(c);
-  // relax with inflated radii (so that true radii end up overlap-free); originals restored bit for bit
-  DBuf<double> rsave;
-  rsave.ensure(c.n);
-  WDG_HIP(hipMemcpyAsync(rsave.p, c.radius.p, c.n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-  hipLaunchKernelGGL(k_scale, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, c.n, (const double*)rsave.p,
+  // relax with inflated radii (so that true radii end up overlap-free); the radii are
+  // restored bit for bit from a copy (they are permuted along with the particles)
+  DBuf<double> rtrue;
+  rtrue.ensure(c.n);
+  WDG_HIP(hipMemcpyAsync(rtrue.p, c.radius.p, c.n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  hipLaunchKernelGGL(k_scale, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, c.n, (const double*)rtrue.p,
                      1 + inflate, c.radius.p);
   WDG_LAUNCH_CHECK();
+  // the particle arrays are permuted at every refresh, so the true radii are kept in user-id order
+  DBuf<double> ruid;
+  ruid.ensure(c.n);
+  {
+    std::vector<double> h((size_t)c.n);
+    std::vector<uint32_t> u((size_t)c.n);
+    WDG_HIP(hipMemcpyAsync(h.data(), rtrue.p, c.n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    WDG_HIP(hipMemcpyAsync(u.data(), c.uid.p, c.n * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+    WDG_HIP(hipStreamSynchronize(c.stream));
+    std::vector<double> byUid((size_t)c.n);
+    for (int64_t s = 0; s < c.n; s++) byUid[u[s]] = h[s];
+    WDG_HIP(hipMemcpy(ruid.p, byUid.data(), c.n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  rtrue.release();
   const double rminSave = c.rmin, rmaxSave = c.rmax;
   c.rmin *= 1 + inflate;
   c.rmax *= 1 + inflate;
@@ -107,7 +123,6 @@ void relax(Ctx& c, int maxIter, double inflate, double tol, double* maxOverlap, 
   cp.forceInit = 1;
   const double saveVd = c.verletDist;
   c.verletDist = NAN;
-  inflate = 0;
   unsigned long long* dmax = reinterpret_cast<unsigned long long*>(c.red.p) + 8;
   double ov = INFINITY;
   int it = 0;
@@ -123,13 +138,11 @@ void relax(Ctx& c, int maxIter, double inflate, double tol, double* maxOverlap, 
       a.px = c.pos[0].p; a.py = c.pos[1].p; a.pz = c.pos[2].p; a.rad = c.radius.p;
       for (int k = 0; k < 3; k++) a.L[k] = c.field.cell[k];
       a.periodic = c.field.periodic;
-      a.grow = 1 + inflate;
       a.maxOv = dmax;
       hipLaunchKernelGGL(k_relax_pairs, dim3(nblocks(C.n, 256)), dim3(256), 0, c.stream, a);
       WDG_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_relax_move, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, c.n, c.flags.p, C.offA.p,
-                         C.offB.p, C.idxB.p, C.real.p, C.F[0].p, C.F[1].p, C.F[2].p, c.pos[0].p, c.pos[1].p,
-                         c.pos[2].p);
+      hipLaunchKernelGGL(k_relax_move, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, c.n, c.flags.p, C.adj.p,
+                         C.adjOff.p, C.real.p, C.F[0].p, C.F[1].p, C.F[2].p, c.pos[0].p, c.pos[1].p, c.pos[2].p);
       WDG_LAUNCH_CHECK();
       c.dirtyKnown = false;
     }
@@ -141,20 +154,21 @@ void relax(Ctx& c, int maxIter, double inflate, double tol, double* maxOverlap, 
       if (ov <= tol) { it++; break; }
     }
   }
-  // drop the relaxation contacts; the simulation starts without history
-  WDG_HIP(hipMemcpyAsync(c.radius.p, rsave.p, c.n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  // restore the true radii in the current storage order: radius[s] = ruid[uid[s]]
+  {
+    std::vector<double> byUid((size_t)c.n), h((size_t)c.n);
+    std::vector<uint32_t> u((size_t)c.n);
+    WDG_HIP(hipMemcpyAsync(byUid.data(), ruid.p, c.n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    WDG_HIP(hipMemcpyAsync(u.data(), c.uid.p, c.n * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+    WDG_HIP(hipStreamSynchronize(c.stream));
+    for (int64_t s = 0; s < c.n; s++) h[s] = byUid[u[s]];
+    WDG_HIP(hipMemcpy(c.radius.p, h.data(), c.n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  ruid.release();
   c.rmin = rminSave;
   c.rmax = rmaxSave;
-  for (auto& s : c.C) {
-    s.n = 0;
-    WDG_HIP(hipMemsetAsync(s.offA.p, 0, (c.n + 1) * sizeof(uint32_t), c.stream));
-    WDG_HIP(hipMemsetAsync(s.offB.p, 0, (c.n + 1) * sizeof(uint32_t), c.stream));
-  }
+  drop_contacts(c);
   WDG_HIP(hipStreamSynchronize(c.stream));
-  rsave.release();
-  c.boundsValid = false;
-  c.dirtyKnown = false;
-  c.forcesPending = false;
   c.verletDist = saveVd;
   c.nRefresh = 0;
   c.nCollide = 0;

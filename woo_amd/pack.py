@@ -41,28 +41,18 @@ def periodic_pack(n, rmin, rmax, phi, seed=2, inflate=0.005, max_iter=4000, devi
 
 
 def check_overlaps(pos, radius, L):
-    """Largest relative overlap of a periodic packing (CPU, cell-binned), for tests."""
-    n = len(radius)
-    h = 2 * float(np.max(radius))
-    nc = max(1, int(L // h))
-    cell = np.floor(np.mod(pos, L) / (L / nc)).astype(np.int64) % nc
-    key = (cell[:, 2] * nc + cell[:, 1]) * nc + cell[:, 0]
-    order = np.argsort(key, kind="stable")
-    worst = 0.
-    buckets = {}
-    for i in order:
-        buckets.setdefault(int(key[i]), []).append(i)
-    for i in range(n):
-        cx, cy, cz = cell[i]
-        for dz in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dx in (-1, 0, 1):
-                    k = (((cz + dz) % nc) * nc + (cy + dy) % nc) * nc + (cx + dx) % nc
-                    for j in buckets.get(int(k), ()):
-                        if j <= i:
-                            continue
-                        d = pos[j] - pos[i]
-                        d -= L * np.round(d / L)
-                        ov = radius[i] + radius[j] - math.sqrt(float(d @ d))
-                        worst = max(worst, ov / min(radius[i], radius[j]))
-    return worst
+    """Largest overlap of a periodic packing relative to the smaller radius (CPU, for tests);
+    negative means the closest pair is separated."""
+    from scipy.spatial import cKDTree
+
+    x = np.mod(pos, L)
+    x[x >= L] = 0.
+    tree = cKDTree(x, boxsize=L)
+    pairs = tree.query_pairs(2 * float(np.max(radius)), output_type="ndarray")
+    if len(pairs) == 0:
+        return -np.inf
+    i, j = pairs[:, 0], pairs[:, 1]
+    d = x[j] - x[i]
+    d -= L * np.round(d / L)
+    ov = radius[i] + radius[j] - np.linalg.norm(d, axis=1)
+    return float(np.max(ov / np.minimum(radius[i], radius[j])))
