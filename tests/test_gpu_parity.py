@@ -179,6 +179,41 @@ def test_minimal_engine_order_with_forceresetter():
     assert e <= TRAJ_TOL
 
 
+def test_force_state_changes_mid_run():
+    # the integrator skips loading forces that hold the reset value; every change of what
+    # that value is (gravity, mass) or of the forces themselves must be seen by the next step
+    sc = deposition(n=600, dense=True, seed=21, vel_sigma=0.05)
+    o = build_oracle(sc)
+    g = GpuRunner(sc)
+    o.run(ENG, 15)
+    g.run(ENG, 15)
+    grav = (0.5, -0.25, -4.0)
+    o.set_params(gravity=grav)
+    g.d.set_field(gravity=grav, periodic=sc["periodic"], cell=sc["cell"])
+    o.run(ENG, 15)
+    g.run(ENG, 15)
+    e, _ = traj_metric(o, g, sc, sc["walls_first"])
+    assert e <= TRAJ_TOL
+    rng = np.random.default_rng(5)
+    m = o.get("mass")
+    F = rng.normal(0., 1., size=(len(m), 3)) * (m * 9.81)[:, None]
+    o.set("force", F)
+    g.d.set("force", F)
+    o.run(ENG, 10)
+    g.run(ENG, 10)
+    e, _ = traj_metric(o, g, sc, sc["walls_first"])
+    assert e <= TRAJ_TOL
+    o.set("mass", m * 1.25)
+    g.d.set("mass", m * 1.25)
+    o.run(ENG, 10)
+    g.run(ENG, 10)
+    compare_contact_sets(o, g, real_only=True)
+    e, n_diff = traj_metric(o, g, sc, sc["walls_first"])
+    print("positions differing", n_diff, "worst", e)
+    assert e <= TRAJ_TOL
+    np.testing.assert_array_equal(o.get("force"), g.d.get("force"))
+
+
 def test_crit_dt_matches():
     for sc in (deposition(n=600, dense=True, seed=13), periodic_hertz(n=800, spacing=0.97, seed=14)):
         o = build_oracle(sc)

@@ -29,8 +29,9 @@ __global__ void k_scatter_uid(int64_t n, const uint32_t* __restrict__ sto, const
 // one record of 16 doubles per slot, for inspection and parity tests
 struct ExportArgs {
   int64_t nc;
-  const uint32_t *A, *B, *info, *real;
-  const double *N0, *N1, *N2, *Y0, *Y1, *Y2, *P0, *P1, *P2, *F0, *F1, *F2, *uN;
+  const uint32_t *A, *B, *info;
+  RealView rv;
+  const double *N0, *N1, *N2, *Y0, *Y1, *Y2, *F0, *F1, *F2, *uN, *gp;
   const double *rad, *mass;
   const int32_t* mat;
   const wdg_material* mats;
@@ -43,12 +44,12 @@ __global__ void k_export(ExportArgs a) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.nc) return;
   double* o = a.out + 16 * s;
-  const bool real = a.real[s] != 0u;
-  if (!real) {
+  if (!rv_real(a.rv, (uint32_t)s)) {
     for (int k = 0; k < 15; k++) o[k] = NAN;
     o[15] = 0.;
     return;
   }
+  const uint32_t r = rv_index(a.rv, (uint32_t)s);
   const uint32_t inf = a.info[s];
   const bool swap = (inf & CI_SWAP) != 0u;
   const uint32_t pA = swap ? a.B[s] : a.A[s], pB = swap ? a.A[s] : a.B[s];
@@ -60,7 +61,7 @@ __global__ void k_export(ExportArgs a) {
   const wdg_material mA = a.mats[a.mat[pA]], mB = a.mats[a.mat[pB]];
   PhysVals ph;
   double kn, kt;
-  const double uN = a.uN[s];
+  const double uN = a.uN[r];
   if (a.hertz) {
     phys_hertz(mA, mB, l0, l1, a.mass[pA], a.mass[pB], a.poisson, a.en, a.viscAlpha, ph);
     double kn0 = ph.K * sqrt(ph.R);
@@ -71,10 +72,11 @@ __global__ void k_export(ExportArgs a) {
     kn = ph.kn;
     kt = ph.kt;
   }
-  o[0] = a.N0[s]; o[1] = a.N1[s]; o[2] = a.N2[s];
-  o[3] = a.Y0[s]; o[4] = a.Y1[s]; o[5] = a.Y2[s];
-  o[6] = a.P0[s]; o[7] = a.P1[s]; o[8] = a.P2[s];
-  o[9] = a.F0[s]; o[10] = a.F1[s]; o[11] = a.F2[s];
+  const double* g = a.gp + (int64_t)GP * r;
+  o[0] = a.N0[r]; o[1] = a.N1[r]; o[2] = a.N2[r];
+  o[3] = a.Y0[r]; o[4] = a.Y1[r]; o[5] = a.Y2[r];
+  o[6] = g[3]; o[7] = g[4]; o[8] = g[5];
+  o[9] = a.F0[r]; o[10] = a.F1[r]; o[11] = a.F2[r];
   o[12] = uN; o[13] = kn; o[14] = kt; o[15] = 1.;
 }
 
@@ -156,7 +158,9 @@ using namespace wdg;
 static std::vector<Record> collect_contacts(Ctx& c, int which, std::vector<double>* rec) {
   Contacts& C = c.con();
   const int64_t nc = C.n;
-  std::vector<uint32_t> A((size_t)nc), B((size_t)nc), info((size_t)nc), real((size_t)nc), uid((size_t)c.n);
+  std::vector<uint32_t> A((size_t)nc), B((size_t)nc), info((size_t)nc), uid((size_t)c.n);
+  std::vector<unsigned long long> bits((size_t)(nc + 63) / 64);
+  Reals& RS = c.reals();
   std::vector<uint64_t> dist;
   WDG_HIP(hipStreamSynchronize(c.stream));
   if (c.n) WDG_HIP(hipMemcpy(uid.data(), c.uid.p, c.n * 4, hipMemcpyDeviceToHost));
@@ -164,7 +168,7 @@ static std::vector<Record> collect_contacts(Ctx& c, int which, std::vector<doubl
     WDG_HIP(hipMemcpy(A.data(), C.A.p, nc * 4, hipMemcpyDeviceToHost));
     WDG_HIP(hipMemcpy(B.data(), C.B.p, nc * 4, hipMemcpyDeviceToHost));
     WDG_HIP(hipMemcpy(info.data(), C.info.p, nc * 4, hipMemcpyDeviceToHost));
-    WDG_HIP(hipMemcpy(real.data(), C.real.p, nc * 4, hipMemcpyDeviceToHost));
+    WDG_HIP(hipMemcpy(bits.data(), RS.bits.p, bits.size() * 8, hipMemcpyDeviceToHost));
     if (c.field.periodic) {
       dist.resize((size_t)nc);
       WDG_HIP(hipMemcpy(dist.data(), C.dist.p, nc * 8, hipMemcpyDeviceToHost));
@@ -173,11 +177,10 @@ static std::vector<Record> collect_contacts(Ctx& c, int which, std::vector<doubl
       DBuf<double> out;
       out.ensure((size_t)nc * 16);
       ExportArgs a{};
-      a.nc = nc; a.A = C.A.p; a.B = C.B.p; a.info = C.info.p; a.real = C.real.p;
-      a.N0 = C.hN[0].p; a.N1 = C.hN[1].p; a.N2 = C.hN[2].p;
-      a.Y0 = C.hY[0].p; a.Y1 = C.hY[1].p; a.Y2 = C.hY[2].p;
-      a.P0 = C.hP[0].p; a.P1 = C.hP[1].p; a.P2 = C.hP[2].p;
-      a.F0 = C.hF[0].p; a.F1 = C.hF[1].p; a.F2 = C.hF[2].p; a.uN = C.huN.p;
+      a.nc = nc; a.A = C.A.p; a.B = C.B.p; a.info = C.info.p; a.rv = c.rview();
+      a.N0 = RS.hN[0].p; a.N1 = RS.hN[1].p; a.N2 = RS.hN[2].p;
+      a.Y0 = RS.hY[0].p; a.Y1 = RS.hY[1].p; a.Y2 = RS.hY[2].p;
+      a.F0 = RS.hF[0].p; a.F1 = RS.hF[1].p; a.F2 = RS.hF[2].p; a.uN = RS.huN.p; a.gp = RS.gp.p;
       a.rad = c.radius.p; a.mass = c.mass.p; a.mat = c.mat.p; a.mats = c.dMats.p;
       a.hertz = c.lastPhysKind == WDG_PHYS_HERTZ;
       a.poisson = c.lastPoisson; a.en = c.lastEn; a.viscAlpha = c.lastViscAlpha;
@@ -194,7 +197,7 @@ static std::vector<Record> collect_contacts(Ctx& c, int which, std::vector<doubl
   auto key = [](uint32_t a, uint32_t b) { return std::make_pair(std::min(a, b), std::max(a, b)); };
   std::vector<std::pair<std::pair<uint32_t, uint32_t>, size_t>> order;
   for (int64_t s = 0; s < nc; s++) {
-    if (!real[s]) continue;
+    if (!((bits[(size_t)s >> 6] >> (s & 63)) & 1ull)) continue;
     bool swap = (info[s] & CI_SWAP) != 0;
     Record r;
     r.a = uid[swap ? B[s] : A[s]];
@@ -261,6 +264,8 @@ int wdg_ctx_destroy(wdg_ctx* ctx) {
     Ctx& c = ctx->c;
     (void)hipStreamSynchronize(c.stream);
     for (auto& s : c.C) s.release();
+    for (auto& r : c.R) r.release();
+    c.act.release();
     DBuf<double>* arrs[] = {c.pos, c.vel, c.angVel, c.force, c.torque, c.inertia, c.lastPos};
     for (auto* a : arrs) for (int k = 0; k < 3; k++) a[k].release();
     for (int k = 0; k < 4; k++) c.ori[k].release();
@@ -291,6 +296,9 @@ int wdg_set_field(wdg_ctx* ctx, const wdg_field* f) {
     bool changed = f->periodic != c.field.periodic;
     for (int k = 0; k < 3; k++) changed = changed || f->cell[k] != c.field.cell[k];
     changed = changed || f->distFactor != c.field.distFactor || f->loneMask != c.field.loneMask;
+    // the reset value of the forces depends on gravity
+    for (int k = 0; k < 3; k++)
+      if (f->gravity[k] != c.field.gravity[k]) c.forceClean = false;
     c.field = *f;
     if (changed) c.boundsValid = false;
   });
@@ -394,6 +402,7 @@ int wdg_upload_particles(wdg_ctx* ctx, const wdg_particles* p) {
     }
     drop_contacts(c);
     WDG_HIP(hipStreamSynchronize(c.stream));
+    c.forceClean = false;
     c.verletDist = NAN;
     c.dynStepLast = c.dynStepPrev = -1;
   });
@@ -408,6 +417,8 @@ int wdg_upload(wdg_ctx* ctx, int32_t field, const double* host) {
     if ((field == WDG_F_FORCE || field == WDG_F_TORQUE) && c.forcesPending) flush_forces(c);
     put_field(c, b, w, host);
     if (field == WDG_F_POS) c.dirtyKnown = false;
+    // new forces, or a new mass changes the gravity part of the reset value
+    if (field == WDG_F_FORCE || field == WDG_F_TORQUE || field == WDG_F_MASS) c.forceClean = false;
   });
 }
 

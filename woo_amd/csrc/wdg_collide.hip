@@ -337,12 +337,12 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
 // old real slots appended behind the new pairs, re-keyed in the new storage order;
 // non-real ones get the sentinel key
 __global__ void k_append_old(int64_t nOld, int64_t nPairs, const uint32_t* __restrict__ A,
-                             const uint32_t* __restrict__ B, const uint32_t* __restrict__ real,
+                             const uint32_t* __restrict__ B, RealView rv,
                              const uint32_t* __restrict__ invPerm, int idBits, uint64_t sentinel, uint64_t* pkey) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nOld) return;
   uint64_t k = sentinel;
-  if (real[c]) {
+  if (rv_real(rv, (uint32_t)c)) {
     uint32_t a = invPerm[A[c]], b = invPerm[B[c]];
     uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
     k = ((uint64_t)lo << idBits) | (uint64_t)hi;
@@ -377,11 +377,9 @@ struct CompactArgs {
   // old set
   const uint32_t *oA, *oB, *oInfo;
   const uint64_t* oDist;
-  const double *oN0, *oN1, *oN2, *oY0, *oY1, *oY2, *oP0, *oP1, *oP2, *oF0, *oF1, *oF2, *ouN;
   // new set
-  uint32_t *A, *B, *info, *real;
+  uint32_t *A, *B, *info, *carry;
   uint64_t* dist;
-  double *N0, *N1, *N2, *Y0, *Y1, *Y2, *P0, *P1, *P2, *F0, *F1, *F2, *uN;
 };
 
 __global__ void k_compact(CompactArgs a) {
@@ -414,16 +412,31 @@ __global__ void k_compact(CompactArgs a) {
   if (old >= 0 && a.periodic) dist = a.oDist[old];
   a.info[o] = inf;
   if (a.periodic) a.dist[o] = dist;
-  if (old >= 0) {
-    a.real[o] = 1u;
-    a.N0[o] = a.oN0[old]; a.N1[o] = a.oN1[old]; a.N2[o] = a.oN2[old];
-    a.Y0[o] = a.oY0[old]; a.Y1[o] = a.oY1[old]; a.Y2[o] = a.oY2[old];
-    a.P0[o] = a.oP0[old]; a.P1[o] = a.oP1[old]; a.P2[o] = a.oP2[old];
-    a.F0[o] = a.oF0[old]; a.F1[o] = a.oF1[old]; a.F2[o] = a.oF2[old];
-    a.uN[o] = a.ouN[old];
-  } else {
-    a.real[o] = 0u;
-  }
+  a.carry[o] = old >= 0 ? (uint32_t)old + 1u : 0u;
+}
+
+// contact state of the carried real contacts: old rank -> new rank
+struct CarryArgs {
+  int64_t n;
+  const uint32_t* carry;
+  RealView ov, nv;
+  const double* o[10];
+  double* d[10];
+  const double* ogp;
+  double* gp;
+};
+__global__ void k_carry_state(CarryArgs a) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  const uint32_t cy = a.carry[s];
+  if (!cy) return;
+  const uint32_t ko = rv_index(a.ov, cy - 1u), kn = rv_index(a.nv, (uint32_t)s);
+#pragma unroll
+  for (int q = 0; q < 10; q++) a.d[q][kn] = a.o[q][ko];
+  const double2* src = reinterpret_cast<const double2*>(a.ogp + (int64_t)GP * ko);
+  double2* dst = reinterpret_cast<double2*>(a.gp + (int64_t)GP * kn);
+#pragma unroll
+  for (int q = 0; q < GP / 2; q++) dst[q] = src[q];
 }
 
 // two adjacency entries per slot, keyed (particle storage, other user id)
@@ -476,6 +489,10 @@ void drop_contacts(Ctx& c) {
     s.n = 0;
     s.ensure(0, c.n, c.field.periodic != 0);
     WDG_HIP(hipMemsetAsync(s.adjOff.p, 0, (c.n + 1) * sizeof(uint32_t), c.stream));
+  }
+  for (auto& r : c.R) {
+    r.ensure(0);
+    WDG_HIP(hipMemsetAsync(r.tpre.p, 0, sizeof(uint32_t), c.stream));
   }
   c.boundsValid = false;
   c.dirtyKnown = false;
@@ -640,7 +657,7 @@ static void refresh(Ctx& c) {
   // 3. merge with the old real contacts (history)
   if (nOld > 0) {
     hipLaunchKernelGGL(k_append_old, dim3(nblocks(nOld, 256)), dim3(256), 0, c.stream, nOld, nPairs, O.A.p, O.B.p,
-                       O.real.p, c.invPerm.p, idBits, sentinel, c.pairKey.p);
+                       c.rview(), c.invPerm.p, idBits, sentinel, c.pairKey.p);
     WDG_LAUNCH_CHECK();
   }
   int64_t nNew = 0;
@@ -663,16 +680,24 @@ static void refresh(Ctx& c) {
     ca.off = c.runOff.p; ca.pinfo = c.pairInfo.p; ca.pdist = c.pairDist.p; ca.invPerm = c.invPerm.p;
     ca.idBits = idBits; ca.periodic = periodic;
     ca.oA = O.A.p; ca.oB = O.B.p; ca.oInfo = O.info.p; ca.oDist = O.dist.p;
-    ca.oN0 = O.hN[0].p; ca.oN1 = O.hN[1].p; ca.oN2 = O.hN[2].p;
-    ca.oY0 = O.hY[0].p; ca.oY1 = O.hY[1].p; ca.oY2 = O.hY[2].p;
-    ca.oP0 = O.hP[0].p; ca.oP1 = O.hP[1].p; ca.oP2 = O.hP[2].p;
-    ca.oF0 = O.hF[0].p; ca.oF1 = O.hF[1].p; ca.oF2 = O.hF[2].p; ca.ouN = O.huN.p;
-    ca.A = N.A.p; ca.B = N.B.p; ca.info = N.info.p; ca.real = N.real.p; ca.dist = N.dist.p;
-    ca.N0 = N.hN[0].p; ca.N1 = N.hN[1].p; ca.N2 = N.hN[2].p;
-    ca.Y0 = N.hY[0].p; ca.Y1 = N.hY[1].p; ca.Y2 = N.hY[2].p;
-    ca.P0 = N.hP[0].p; ca.P1 = N.hP[1].p; ca.P2 = N.hP[2].p;
-    ca.F0 = N.hF[0].p; ca.F1 = N.hF[1].p; ca.F2 = N.hF[2].p; ca.uN = N.huN.p;
+    ca.A = N.A.p; ca.B = N.B.p; ca.info = N.info.p; ca.carry = N.carry.p; ca.dist = N.dist.p;
     hipLaunchKernelGGL(k_compact, dim3(nblocks(tot, 256)), dim3(256), 0, c.stream, ca);
+    WDG_LAUNCH_CHECK();
+  }
+  // real set of the new list: the carried contacts, with their state moved to the new ranks
+  const int rNew = 1 - c.rcur;
+  reals_from_flags(c, N.carry.p, nNew, rNew);
+  if (nNew > 0) {
+    Reals &RO = c.R[c.rcur], &RN = c.R[rNew];
+    CarryArgs ya{};
+    ya.n = nNew; ya.carry = N.carry.p; ya.ov = c.rview(c.rcur); ya.nv = c.rview(rNew);
+    DBuf<double>* so[10] = {&RO.hN[0], &RO.hN[1], &RO.hN[2], &RO.hY[0], &RO.hY[1],
+                            &RO.hY[2], &RO.hF[0], &RO.hF[1], &RO.hF[2], &RO.huN};
+    DBuf<double>* sn[10] = {&RN.hN[0], &RN.hN[1], &RN.hN[2], &RN.hY[0], &RN.hY[1],
+                            &RN.hY[2], &RN.hF[0], &RN.hF[1], &RN.hF[2], &RN.huN};
+    for (int q = 0; q < 10; q++) { ya.o[q] = so[q]->p; ya.d[q] = sn[q]->p; }
+    ya.ogp = RO.gp.p; ya.gp = RN.gp.p;
+    hipLaunchKernelGGL(k_carry_state, dim3(nblocks(nNew, 256)), dim3(256), 0, c.stream, ya);
     WDG_LAUNCH_CHECK();
   }
   // 4. per-particle adjacency in canonical order (particle storage, other user id)
@@ -691,10 +716,12 @@ static void refresh(Ctx& c) {
                      nAdj > 0 ? c.adjPart.p : N.A.p, N.adjOff.p);
   WDG_LAUNCH_CHECK();
   c.cur = 1 - c.cur;
+  c.rcur = rNew;
   c.boundsValid = true;
   WDG_HIP(hipMemsetAsync(c.dflags.p, 0, sizeof(uint32_t), c.stream));
   uint32_t err = read_u32(c, c.dflags.p + 1);
   c.dirtyKnown = true;
+  c.dirtyIdx = 0;
   if (err) {
     WDG_HIP(hipMemsetAsync(c.dflags.p + 1, 0, sizeof(uint32_t), c.stream));
     check_device_errors(c, err);
@@ -721,15 +748,17 @@ void collide(Ctx& c, const wdg_collide_params* p) {
                          c.dflags.p);
       WDG_LAUNCH_CHECK();
       c.dirtyKnown = true;
+      c.dirtyIdx = 0;
     }
-    WDG_HIP(hipMemcpyAsync(c.hflags, c.dflags.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+    WDG_HIP(hipMemcpyAsync(c.hflags, c.dflags.p, (DIRTY_PP + 2) * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           c.stream));
     WDG_HIP(hipStreamSynchronize(c.stream));
     if (c.hflags[1]) {
       uint32_t err = c.hflags[1];
       WDG_HIP(hipMemsetAsync(c.dflags.p + 1, 0, sizeof(uint32_t), c.stream));
       check_device_errors(c, err);
     }
-    if (c.hflags[0]) full = true;
+    if (c.hflags[c.dirtyIdx]) full = true;
   }
   if (full) refresh(c);
 }

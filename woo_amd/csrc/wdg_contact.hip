@@ -1,11 +1,21 @@
-// Fused contact kernel: geometry (Cg2) + physics (Cp2) + constitutive law (Law2)
-// + rotation of the local force to global coordinates, one thread per contact
-// slot. Replaces the per-contact body of ContactLoop::run (ContactLoop.cpp:134-231).
+// Contact loop: geometry (Cg2) + physics (Cp2) + constitutive law (Law2) +
+// rotation of the local force to global coordinates. Replaces the per-contact
+// body of ContactLoop::run (ContactLoop.cpp:134-231).
 //
-// Per real contact the persistent state is 13 doubles (L6Geom.trsf columns 0
-// and 1, contact point, local force, uN) in SoA arrays aligned with the slot
-// list, so every access is coalesced; column 2 of trsf is always col0 x col1
-// and the Cp2 parameters are functions of radii, materials and masses.
+// Two passes over the slot list of the last refresh:
+//  1. k_cfilter, one workgroup per group of slots, one thread per slot: the
+//     geometric test and the law's break test decide which slots are real after
+//     this step (a contact that fails the geometry test while not real is not
+//     created; a real one whose law breaks it is removed). One ballot per
+//     wavefront writes the real bits of 64 slots; the workgroup turns the
+//     wavefront counts into tile prefixes and the list of real slots of the group.
+//  2. k_cforce, one thread per real contact (dense wavefronts): full
+//     L6Geom update, Cp2 and Law2, with the previous state read at the slot's
+//     old index and the new state written at its new index. Both follow slot
+//     order, so state reads and writes stream through HBM.
+// The state of a real contact is L6Geom.trsf columns 0 and 1 (column 2 is
+// always col0 x col1), local force, uN, and the 64-byte gather record (global
+// force on pA, contact point, cellDist) that the force gather reads.
 #include "wdg_internal.hpp"
 #include "wdg_math.hpp"
 #include "wdg_physics.hpp"
@@ -16,9 +26,12 @@ struct ContactArgs {
   int64_t nc;
   const uint32_t *A, *B, *info;
   const uint64_t* dist;
-  uint32_t* real;
-  double *N0, *N1, *N2, *Y0, *Y1, *Y2, *P0, *P1, *P2, *F0, *F1, *F2, *uN;
-  double *G0, *G1, *G2;  // global force on pA
+  RealView ov, nv;  // real contacts before / after this step
+  const double *oN0, *oN1, *oN2, *oY0, *oY1, *oY2, *oF1, *oF2, *ogp;
+  double *N0, *N1, *N2, *Y0, *Y1, *Y2, *F0, *F1, *F2, *uN, *gp;
+  unsigned long long* nbits;
+  uint32_t *ntpre, *ngcnt;
+  uint32_t* act;
   // particles
   const double *px, *py, *pz, *vx, *vy, *vz, *wx, *wy, *wz, *rad, *mass;
   const uint32_t* flags;
@@ -34,184 +47,316 @@ struct ContactArgs {
   uint32_t* dflags;
 };
 
+template <bool PERIODIC>
+__device__ __forceinline__ d3 slot_shift(const ContactArgs& a, int64_t c, uint64_t& cd) {
+  cd = 0;
+  if (!PERIODIC) return mk3(0., 0., 0.);
+  cd = a.dist[c];
+  return mk3(a.L[0] * unpack_cd(cd, 0), a.L[1] * unpack_cd(cd, 1), a.L[2] * unpack_cd(cd, 2));
+}
+
+// Law2 break test: IdealElPl.cpp:26-33 (unless noBreak), Hertz.cpp:105-108
+template <int LAW>
+__device__ __forceinline__ bool law_keeps(double uN, int noBreak) {
+  if (LAW == WDG_LAW_IDEALELPL) return !(uN > 0 && !noBreak);
+  return !(uN > 0);
+}
+
+// Group kernel tail, one workgroup of GROUP_BS threads per group of GROUP slots, called
+// once per pass of GROUP_BS slots: real bits of the pass, tile prefixes, and (when act
+// is given) the slot of every state index. running = real slots of the group so far.
+__device__ __forceinline__ void group_pass(int64_t c, bool keep, int64_t base, uint32_t& running,
+                                           unsigned long long* bits, uint32_t* tpre, uint32_t* act) {
+  __shared__ uint32_t wc[GROUP_BS / 64];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const unsigned long long b = __ballot(keep);
+  if (lane == 0) {
+    bits[c >> 6] = b;
+    wc[wv] = (uint32_t)__popcll(b);
+  }
+  __syncthreads();
+  uint32_t wpre = 0, tot = 0;
+  for (int w = 0; w < GROUP_BS / 64; w++) {
+    const uint32_t x = wc[w];
+    wpre += w < wv ? x : 0u;
+    tot += x;
+  }
+  if (lane == 0 && (wv & 3) == 0) tpre[c >> 8] = running + wpre;
+  if (keep && act) {
+    const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    act[state_index((uint32_t)(base / GROUP), running + wpre + (uint32_t)__popcll(b & lt))] = (uint32_t)c;
+  }
+  running += tot;
+  __syncthreads();
+}
+
 template <bool PERIODIC, int LAW>
-__global__ __launch_bounds__(256) void k_contact(ContactArgs a) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= a.nc) return;
-  const uint32_t info = a.info[c];
-  const bool real = a.real[c] != 0u;
-  if (!real && !(info & CI_NEAR)) return;
-  const uint32_t iA = a.A[c], iB = a.B[c];
-  const bool swap = (info & CI_SWAP) != 0u;
-  const uint32_t pA = swap ? iB : iA, pB = swap ? iA : iB;
-  const int type = (int)(info & CI_TYPE_MASK);
-
-  d3 shift2 = mk3(0., 0., 0.);
-  if (PERIODIC) {
-    uint64_t cd = a.dist[c];
-    shift2 = mk3(a.L[0] * unpack_cd(cd, 0), a.L[1] * unpack_cd(cd, 1), a.L[2] * unpack_cd(cd, 2));
-  }
-  const d3 posA = mk3(a.px[pA], a.py[pA], a.pz[pA]);
-  const d3 posB = mk3(a.px[pB], a.py[pB], a.pz[pB]);
-
-  d3 normal, contPt, pos1, pos2;
-  double uN, r1, r2;
-  if (type == CT_SS) {
-    // Cg2_Sphere_Sphere_L6Geom::go (Sphere.cpp:92-117)
-    r1 = a.rad[pA];
-    r2 = a.rad[pB];
-    pos1 = posA;
-    pos2 = add3(posB, shift2);
-    d3 relPos = sub3(pos2, posA);
-    double unDistSq = sqn3(relPos) - p2(a.absDf * (r1 + r2));
-    if (unDistSq > 0 && !real) return;
-    double dist = sqrt(sqn3(relPos));
-    uN = dist - (r1 + r2);
-    normal = div3(relPos, dist);
-    contPt = add3(posA, mul3(normal, r1 + 0.5 * uN));
-  } else {
-    // Cg2_Wall_Sphere_L6Geom::go (Wall.cpp:50-84), pA is the wall
-    const uint32_t fl = a.flags[pA];
-    const int ax = flag_axis(fl), sense = flag_sense(fl);
-    const double radius = a.rad[pB];
-    pos1 = posA;
-    pos2 = add3(posB, shift2);
-    double dist = comp3(pos2, ax) - comp3(posA, ax);
-    if (!real && fabs(dist) > radius) return;
-    contPt = pos2;
-    setcomp3(contPt, ax, comp3(posA, ax));
-    double nv;
-    if (sense == 0) nv = !real ? (dist > 0 ? 1. : -1.) : comp3(mk3(a.N0[c], a.N1[c], a.N2[c]), ax);
-    else nv = (sense == 1 ? 1. : -1.);
-    normal = mk3(0., 0., 0.);
-    setcomp3(normal, ax, nv);
-    uN = nv * dist - radius;
-    r1 = -radius;
-    r2 = radius;
-  }
-  const d3 vel1 = mk3(a.vx[pA], a.vy[pA], a.vz[pA]), vel2 = mk3(a.vx[pB], a.vy[pB], a.vz[pB]);
-  const d3 av1 = mk3(a.wx[pA], a.wy[pA], a.wz[pA]), av2 = mk3(a.wx[pB], a.wy[pB], a.wz[pB]);
-
-  // Cg2_Any_Any_L6Geom__Base::handleSpheresLikeContact (L6Geom.cpp:49-216)
-  d3 T0, T1, T2, gvel;
-  if (!real) {
-    // setInitialLocalCoords (L6Geom.cpp:25-36)
-    d3 locX = normal;
-    d3 e = fabs(locX.y) < fabs(locX.z) ? mk3(0., 1., 0.) : mk3(0., 0., 1.);
-    d3 locY = cross3(locX, e);
-    locY = sub3(locY, mul3(locX, dot3(locY, locX)));
-    locY = normalized3(locY);
-    T0 = locX;
-    T1 = locY;
-    T2 = cross3(locX, locY);
-    gvel = mk3(0., 0., 0.);
-  } else {
-    const double dt = a.dt;
-    const d3 prevNormal = mk3(a.N0[c], a.N1[c], a.N2[c]);
-    const d3 prevY = mk3(a.Y0[c], a.Y1[c], a.Y2[c]);
-    const d3 prevContPt = mk3(a.P0[c], a.P1[c], a.P2[c]);
-    const d3& currNormal = normal;
-    d3 normRotVec = cross3(prevNormal, currNormal);
-    d3 midNormal = mul3(add3(prevNormal, currNormal), .5);
-    if (sqrt(sqn3(midNormal)) == 0) midNormal = currNormal;
-    else midNormal = normalized3(midNormal);
-    double twist = dot3(midNormal, add3(av1, av2));
-    d3 normTwistVec = mul3(mul3(mul3(midNormal, dt), .5), twist);
-    d3 rt = add3(normRotVec, normTwistVec);
-    d3 M0 = midNormal;
-    d3 M1 = sub3(prevY, div3(cross3(prevY, rt), 2.));
-    d3 M2 = cross3(M0, M1);
-    T0 = currNormal;
-    T1 = sub3(prevY, cross3(M1, rt));
-    T2 = cross3(T0, T1);
-    if (a.renorm) {
-      T0 = normalized3(T0);
-      T1 = sub3(T1, mul3(T0, dot3(T1, T0)));
-      T1 = normalized3(T1);
-      T2 = cross3(T0, T1);
+__device__ __forceinline__ bool filter_slot(const ContactArgs& a, int64_t c) {
+  bool keep = false;
+  if (c < a.nc) {
+    const uint32_t info = a.info[c];
+    const bool real = rv_real(a.ov, (uint32_t)c);
+    if (real || (info & CI_NEAR)) {
+      const bool swap = (info & CI_SWAP) != 0u;
+      const uint32_t pA = swap ? a.B[c] : a.A[c], pB = swap ? a.A[c] : a.B[c];
+      uint64_t cd;
+      const d3 shift2 = slot_shift<PERIODIC>(a, c, cd);
+      const d3 posA = mk3(a.px[pA], a.py[pA], a.pz[pA]);
+      const d3 pos2 = add3(mk3(a.px[pB], a.py[pB], a.pz[pB]), shift2);
+      bool pass;
+      double uN;
+      if ((info & CI_TYPE_MASK) == CT_SS) {
+        // Cg2_Sphere_Sphere_L6Geom::go (Sphere.cpp:92-105)
+        const double r1 = a.rad[pA], r2 = a.rad[pB];
+        const d3 relPos = sub3(pos2, posA);
+        const double unDistSq = sqn3(relPos) - p2(a.absDf * (r1 + r2));
+        pass = !(unDistSq > 0 && !real);
+        uN = sqrt(sqn3(relPos)) - (r1 + r2);
+      } else {
+        // Cg2_Wall_Sphere_L6Geom::go (Wall.cpp:50-71)
+        const uint32_t fl = a.flags[pA];
+        const int ax = flag_axis(fl), sense = flag_sense(fl);
+        const double radius = a.rad[pB];
+        const double dist = comp3(pos2, ax) - comp3(posA, ax);
+        pass = !(!real && fabs(dist) > radius);
+        double nvv;
+        if (sense == 0) {
+          if (!real) nvv = dist > 0 ? 1. : -1.;
+          else {
+            const uint32_t ko = rv_index(a.ov, (uint32_t)c);
+            nvv = ax == 0 ? a.oN0[ko] : (ax == 1 ? a.oN1[ko] : a.oN2[ko]);
+          }
+        } else {
+          nvv = (sense == 1 ? 1. : -1.);
+        }
+        uN = nvv * dist - radius;
+      }
+      keep = pass && law_keeps<LAW>(uN, a.noBreak);
     }
-    d3 midContPt = mul3(add3(prevContPt, contPt), .5);
-    d3 midPos1 = sub3(pos1, mul3(vel1, dt / 2.));
-    d3 midPos2 = sub3(pos2, mul3(vel2, dt / 2.));
-    d3 c1x = sub3(midContPt, midPos1);
-    d3 c2x = sub3(midContPt, midPos2);
-    d3 relVel = sub3(add3(vel2, cross3(av2, c2x)), add3(vel1, cross3(av1, c1x)));
-    gvel = mk3(dot3(M0, relVel), dot3(M1, relVel), dot3(M2, relVel));
   }
+  return keep;
+}
 
-  // Cp2 (recomputed; constant over the life of the contact)
-  double l0, l1, A;
-  lens_area(r1, r2, l0, l1, A);
-  const wdg_material mA = a.mats[a.mat[pA]], mB = a.mats[a.mat[pB]];
-  double Fn, Ft0 = real ? a.F1[c] : 0., Ft1 = real ? a.F2[c] : 0.;
-  if (LAW == WDG_LAW_IDEALELPL) {
-    // Law2_L6Geom_FrictPhys_IdealElPl::go (IdealElPl.cpp:15-125)
-    PhysVals ph;
-    phys_frict(mA, mB, l0, l1, A, ph);
-    if (uN > 0 && !a.noBreak) {
-      if (real) a.real[c] = 0u;
-      return;
-    }
-    Fn = ph.kn * uN;
-    if (a.noFrict || ph.tanPhi == 0.) {
-      Ft0 = 0.;
-      Ft1 = 0.;
+template <bool PERIODIC, int LAW>
+__global__ __launch_bounds__(GROUP_BS) void k_cfilter(ContactArgs a) {
+  const uint32_t g = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t base = (int64_t)g * GROUP;
+  // all slots of the thread first (their loads overlap), then the bookkeeping passes
+  constexpr int NP = GROUP / GROUP_BS;
+  bool keep[NP];
+#pragma unroll
+  for (int pass = 0; pass < NP; pass++) keep[pass] = filter_slot<PERIODIC, LAW>(a, base + pass * GROUP_BS + threadIdx.x);
+  uint32_t running = 0;
+#pragma unroll
+  for (int pass = 0; pass < NP; pass++)
+    group_pass(base + pass * GROUP_BS + threadIdx.x, keep[pass], base, running, a.nbits, a.ntpre, a.act);
+  if (threadIdx.x == 0) a.ngcnt[g] = running;
+}
+
+// CF_BLOCKS workgroups per group of slots
+constexpr int CF_BS = 256;
+constexpr int CF_BLOCKS = 4;
+
+template <bool PERIODIC, int LAW>
+__global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
+  const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t g = lb / CF_BLOCKS;
+  const uint32_t cnt = a.ngcnt[g];
+  for (uint32_t loc = (lb % CF_BLOCKS) * CF_BS + threadIdx.x; loc < cnt; loc += CF_BS * CF_BLOCKS) {
+    const int64_t k = state_index((uint32_t)g, loc);
+    const uint32_t c = a.act[k];
+    const uint32_t info = a.info[c];
+    const bool real = rv_real(a.ov, c);
+    const uint32_t ko = real ? rv_index(a.ov, c) : 0u;
+    const bool swap = (info & CI_SWAP) != 0u;
+    const uint32_t pA = swap ? a.B[c] : a.A[c], pB = swap ? a.A[c] : a.B[c];
+    const int type = (int)(info & CI_TYPE_MASK);
+    uint64_t cd;
+    const d3 shift2 = slot_shift<PERIODIC>(a, c, cd);
+    const d3 posA = mk3(a.px[pA], a.py[pA], a.pz[pA]);
+    const d3 posB = mk3(a.px[pB], a.py[pB], a.pz[pB]);
+    d3 prevNormal = mk3(0., 0., 0.);
+    if (real) prevNormal = mk3(a.oN0[ko], a.oN1[ko], a.oN2[ko]);
+
+    d3 normal, contPt, pos1, pos2;
+    double uN, r1, r2;
+    if (type == CT_SS) {
+      // Cg2_Sphere_Sphere_L6Geom::go (Sphere.cpp:92-117)
+      r1 = a.rad[pA];
+      r2 = a.rad[pB];
+      pos1 = posA;
+      pos2 = add3(posB, shift2);
+      const d3 relPos = sub3(pos2, posA);
+      const double dist = sqrt(sqn3(relPos));
+      uN = dist - (r1 + r2);
+      normal = div3(relPos, dist);
+      contPt = add3(posA, mul3(normal, r1 + 0.5 * uN));
     } else {
-      double dk = a.dt * ph.kt;
+      // Cg2_Wall_Sphere_L6Geom::go (Wall.cpp:50-84), pA is the wall
+      const uint32_t fl = a.flags[pA];
+      const int ax = flag_axis(fl), sense = flag_sense(fl);
+      const double radius = a.rad[pB];
+      pos1 = posA;
+      pos2 = add3(posB, shift2);
+      const double dist = comp3(pos2, ax) - comp3(posA, ax);
+      contPt = pos2;
+      setcomp3(contPt, ax, comp3(posA, ax));
+      double nvv;
+      if (sense == 0) nvv = !real ? (dist > 0 ? 1. : -1.) : comp3(prevNormal, ax);
+      else nvv = (sense == 1 ? 1. : -1.);
+      normal = mk3(0., 0., 0.);
+      setcomp3(normal, ax, nvv);
+      uN = nvv * dist - radius;
+      r1 = -radius;
+      r2 = radius;
+    }
+    const d3 vel1 = mk3(a.vx[pA], a.vy[pA], a.vz[pA]), vel2 = mk3(a.vx[pB], a.vy[pB], a.vz[pB]);
+    const d3 av1 = mk3(a.wx[pA], a.wy[pA], a.wz[pA]), av2 = mk3(a.wx[pB], a.wy[pB], a.wz[pB]);
+
+    // Cg2_Any_Any_L6Geom__Base::handleSpheresLikeContact (L6Geom.cpp:49-216)
+    d3 T0, T1, T2, gvel;
+    if (!real) {
+      // setInitialLocalCoords (L6Geom.cpp:25-36)
+      const d3 locX = normal;
+      const d3 e = fabs(locX.y) < fabs(locX.z) ? mk3(0., 1., 0.) : mk3(0., 0., 1.);
+      d3 locY = cross3(locX, e);
+      locY = sub3(locY, mul3(locX, dot3(locY, locX)));
+      locY = normalized3(locY);
+      T0 = locX;
+      T1 = locY;
+      T2 = cross3(locX, locY);
+      gvel = mk3(0., 0., 0.);
+    } else {
+      const double dt = a.dt;
+      const d3 prevY = mk3(a.oY0[ko], a.oY1[ko], a.oY2[ko]);
+      const double* og = a.ogp + (int64_t)GP * ko;
+      const d3 prevContPt = mk3(og[3], og[4], og[5]);
+      const d3& currNormal = normal;
+      const d3 normRotVec = cross3(prevNormal, currNormal);
+      d3 midNormal = mul3(add3(prevNormal, currNormal), .5);
+      if (sqrt(sqn3(midNormal)) == 0) midNormal = currNormal;
+      else midNormal = normalized3(midNormal);
+      const double twist = dot3(midNormal, add3(av1, av2));
+      const d3 normTwistVec = mul3(mul3(mul3(midNormal, dt), .5), twist);
+      const d3 rt = add3(normRotVec, normTwistVec);
+      const d3 M0 = midNormal;
+      const d3 M1 = sub3(prevY, div3(cross3(prevY, rt), 2.));
+      const d3 M2 = cross3(M0, M1);
+      T0 = currNormal;
+      T1 = sub3(prevY, cross3(M1, rt));
+      T2 = cross3(T0, T1);
+      if (a.renorm) {
+        T0 = normalized3(T0);
+        T1 = sub3(T1, mul3(T0, dot3(T1, T0)));
+        T1 = normalized3(T1);
+        T2 = cross3(T0, T1);
+      }
+      const d3 midContPt = mul3(add3(prevContPt, contPt), .5);
+      const d3 midPos1 = sub3(pos1, mul3(vel1, dt / 2.));
+      const d3 midPos2 = sub3(pos2, mul3(vel2, dt / 2.));
+      const d3 c1x = sub3(midContPt, midPos1);
+      const d3 c2x = sub3(midContPt, midPos2);
+      const d3 relVel = sub3(add3(vel2, cross3(av2, c2x)), add3(vel1, cross3(av1, c1x)));
+      gvel = mk3(dot3(M0, relVel), dot3(M1, relVel), dot3(M2, relVel));
+    }
+
+    // Cp2 (recomputed; constant over the life of the contact)
+    double l0, l1, A;
+    lens_area(r1, r2, l0, l1, A);
+    const wdg_material mA = a.mats[a.mat[pA]], mB = a.mats[a.mat[pB]];
+    double Fn, Ft0 = real ? a.oF1[ko] : 0., Ft1 = real ? a.oF2[ko] : 0.;
+    if (LAW == WDG_LAW_IDEALELPL) {
+      // Law2_L6Geom_FrictPhys_IdealElPl::go (IdealElPl.cpp:15-125); breaking decided by k_cfilter
+      PhysVals ph;
+      phys_frict(mA, mB, l0, l1, A, ph);
+      Fn = ph.kn * uN;
+      if (a.noFrict || ph.tanPhi == 0.) {
+        Ft0 = 0.;
+        Ft1 = 0.;
+      } else {
+        const double dk = a.dt * ph.kt;
+        Ft0 = Ft0 + dk * gvel.y;
+        Ft1 = Ft1 + dk * gvel.z;
+        const double maxFt = fabs(Fn) * ph.tanPhi;
+        if (Ft0 * Ft0 + Ft1 * Ft1 > maxFt * maxFt && !a.noSlip) {
+          const double FtNorm = sqrt(Ft0 * Ft0 + Ft1 * Ft1);
+          double ratio = maxFt / FtNorm;
+          if (FtNorm == 0.) ratio = 0.;
+          Ft0 = Ft0 * ratio;
+          Ft1 = Ft1 * ratio;
+        }
+        if (isnan(Fn) || isnan(Ft0) || isnan(Ft1)) atomicOr(&a.dflags[1], ERR_NAN_FORCE);
+      }
+    } else {
+      // Law2_L6Geom_HertzPhys_DMT::go (Hertz.cpp:85-210), DMT branch; breaking decided by k_cfilter
+      PhysVals ph;
+      phys_hertz(mA, mB, l0, l1, a.mass[pA], a.mass[pB], a.poisson, a.en, a.viscAlpha, ph);
+      const double velN = gvel.x;
+      const double kn0 = ph.K * sqrt(ph.R);
+      const double s = sqrt(-uN);
+      const double Fne = -kn0 * ((s * s) * s);
+      const double Fna = 4 * M_PI * ph.R * ph.gamma;
+      const double eta = (ph.alpha_sqrtMK > 0 ? ph.alpha_sqrtMK * sqrt(sqrt(fmax(-uN, 0.))) : 0.);
+      double Fnv = eta * velN;
+      if (ph.alpha == 0. && a.noAttraction && Fne + Fnv > 0) Fnv = -Fne;
+      Fn = Fne + Fna + Fnv;
+      const double kt = ph.kt0 * sqrt(uN < 0 ? -uN : 0);
+      const double dk = a.dt * kt;
       Ft0 = Ft0 + dk * gvel.y;
       Ft1 = Ft1 + dk * gvel.z;
-      double maxFt = fabs(Fn) * ph.tanPhi;
-      if (Ft0 * Ft0 + Ft1 * Ft1 > maxFt * maxFt && !a.noSlip) {
-        double FtNorm = sqrt(Ft0 * Ft0 + Ft1 * Ft1);
-        double ratio = maxFt / FtNorm;
-        if (FtNorm == 0.) ratio = 0.;
+      const double maxFt = fabs(fmin(0., Fn) * ph.tanPhi);
+      if (Ft0 * Ft0 + Ft1 * Ft1 > p2(maxFt)) {
+        const double FtNorm = sqrt(Ft0 * Ft0 + Ft1 * Ft1);
+        const double ratio = maxFt / FtNorm;
         Ft0 = Ft0 * ratio;
         Ft1 = Ft1 * ratio;
+      } else {
+        Ft0 = Ft0 + eta * gvel.y;
+        Ft1 = Ft1 + eta * gvel.z;
       }
-      if (isnan(Fn) || isnan(Ft0) || isnan(Ft1)) atomicOr(&a.dflags[1], ERR_NAN_FORCE);
     }
-  } else {
-    // Law2_L6Geom_HertzPhys_DMT::go (Hertz.cpp:85-210), DMT branch
-    PhysVals ph;
-    phys_hertz(mA, mB, l0, l1, a.mass[pA], a.mass[pB], a.poisson, a.en, a.viscAlpha, ph);
-    const double velN = gvel.x;
-    double kn0 = ph.K * sqrt(ph.R);
-    if (uN > 0) {
-      if (real) a.real[c] = 0u;
-      return;
-    }
-    double s = sqrt(-uN);
-    double Fne = -kn0 * ((s * s) * s);
-    double Fna = 4 * M_PI * ph.R * ph.gamma;
-    double eta = (ph.alpha_sqrtMK > 0 ? ph.alpha_sqrtMK * sqrt(sqrt(fmax(-uN, 0.))) : 0.);
-    double Fnv = eta * velN;
-    if (ph.alpha == 0. && a.noAttraction && Fne + Fnv > 0) Fnv = -Fne;
-    Fn = Fne + Fna + Fnv;
-    double kt = ph.kt0 * sqrt(uN < 0 ? -uN : 0);
-    double dk = a.dt * kt;
-    Ft0 = Ft0 + dk * gvel.y;
-    Ft1 = Ft1 + dk * gvel.z;
-    double maxFt = fabs(fmin(0., Fn) * ph.tanPhi);
-    if (Ft0 * Ft0 + Ft1 * Ft1 > p2(maxFt)) {
-      double FtNorm = sqrt(Ft0 * Ft0 + Ft1 * Ft1);
-      double ratio = maxFt / FtNorm;
-      Ft0 = Ft0 * ratio;
-      Ft1 = Ft1 * ratio;
-    } else {
-      Ft0 = Ft0 + eta * gvel.y;
-      Ft1 = Ft1 + eta * gvel.z;
-    }
+    a.N0[k] = T0.x; a.N1[k] = T0.y; a.N2[k] = T0.z;
+    a.Y0[k] = T1.x; a.Y1[k] = T1.y; a.Y2[k] = T1.z;
+    a.F0[k] = Fn; a.F1[k] = Ft0; a.F2[k] = Ft1;
+    a.uN[k] = uN;
+    // Contact::getForceTorqueBranch: F = node.ori * force, node.ori = Quaternion(trsf)
+    const q4 q = quat_from_cols(T0, T1, T2);
+    const d3 G = quat_rot(q, mk3(Fn, Ft0, Ft1));
+    double2* g2 = reinterpret_cast<double2*>(a.gp + (int64_t)GP * k);
+    g2[0] = make_double2(G.x, G.y);
+    g2[1] = make_double2(G.z, contPt.x);
+    g2[2] = make_double2(contPt.y, contPt.z);
+    g2[3] = make_double2(__longlong_as_double((long long)cd), 0.);
   }
-  if (!real) a.real[c] = 1u;
-  a.N0[c] = T0.x; a.N1[c] = T0.y; a.N2[c] = T0.z;
-  a.Y0[c] = T1.x; a.Y1[c] = T1.y; a.Y2[c] = T1.z;
-  a.P0[c] = contPt.x; a.P1[c] = contPt.y; a.P2[c] = contPt.z;
-  a.F0[c] = Fn; a.F1[c] = Ft0; a.F2[c] = Ft1;
-  a.uN[c] = uN;
-  // Contact::getForceTorqueBranch: F = node.ori * force, node.ori = Quaternion(trsf)
-  q4 q = quat_from_cols(T0, T1, T2);
-  d3 G = quat_rot(q, mk3(Fn, Ft0, Ft1));
-  a.G0[c] = G.x; a.G1[c] = G.y; a.G2[c] = G.z;
+}
+
+static ContactArgs contact_args(Ctx& c, int from, int to) {
+  Contacts& C = c.con();
+  Reals& O = c.R[from];
+  Reals& N = c.R[to];
+  ContactArgs a{};
+  a.nc = C.n;
+  a.A = C.A.p; a.B = C.B.p; a.info = C.info.p; a.dist = C.dist.p;
+  a.ov = c.rview(from);
+  a.nv = c.rview(to);
+  a.oN0 = O.hN[0].p; a.oN1 = O.hN[1].p; a.oN2 = O.hN[2].p;
+  a.oY0 = O.hY[0].p; a.oY1 = O.hY[1].p; a.oY2 = O.hY[2].p;
+  a.oF1 = O.hF[1].p; a.oF2 = O.hF[2].p; a.ogp = O.gp.p;
+  a.N0 = N.hN[0].p; a.N1 = N.hN[1].p; a.N2 = N.hN[2].p;
+  a.Y0 = N.hY[0].p; a.Y1 = N.hY[1].p; a.Y2 = N.hY[2].p;
+  a.F0 = N.hF[0].p; a.F1 = N.hF[1].p; a.F2 = N.hF[2].p; a.uN = N.huN.p; a.gp = N.gp.p;
+  a.nbits = N.bits.p;
+  a.ntpre = N.tpre.p;
+  a.ngcnt = N.gcnt.p;
+  a.act = c.act.p;
+  a.px = c.pos[0].p; a.py = c.pos[1].p; a.pz = c.pos[2].p;
+  a.vx = c.vel[0].p; a.vy = c.vel[1].p; a.vz = c.vel[2].p;
+  a.wx = c.angVel[0].p; a.wy = c.angVel[1].p; a.wz = c.angVel[2].p;
+  a.rad = c.radius.p; a.mass = c.mass.p; a.flags = c.flags.p; a.mat = c.mat.p; a.mats = c.dMats.p;
+  for (int k = 0; k < 3; k++) a.L[k] = c.field.cell[k];
+  a.absDf = std::fabs(c.field.distFactor);
+  a.dflags = c.dflags.p;
+  return a;
 }
 
 void contact_loop(Ctx& c, const wdg_contact_params* p) {
@@ -226,48 +371,74 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
       if (!(m.alpha == 0. || m.surfEnergy == 0.))
         throw Error(WDG_E_ARG, "Schwarz adhesion (HertzMat.alpha and surfEnergy both non-zero) is not supported");
   if (c.forcesPending) flush_forces(c);
+  const double viscAlpha = (p->en > 0 && p->en < 1.)
+                               ? -std::sqrt(5) * std::log(p->en) /
+                                     (std::sqrt(std::log(p->en) * std::log(p->en) + M_PI * M_PI))
+                               : 0.;
+  c.lastPhysKind = p->physKind;
+  c.lastPoisson = p->poisson;
+  c.lastEn = p->en;
+  c.lastViscAlpha = viscAlpha;
   if (C.n == 0 || c.n == 0) {
     c.forcesPending = false;
     return;
   }
-  ContactArgs a{};
-  a.nc = C.n;
-  a.A = C.A.p; a.B = C.B.p; a.info = C.info.p; a.dist = C.dist.p; a.real = C.real.p;
-  a.N0 = C.hN[0].p; a.N1 = C.hN[1].p; a.N2 = C.hN[2].p;
-  a.Y0 = C.hY[0].p; a.Y1 = C.hY[1].p; a.Y2 = C.hY[2].p;
-  a.P0 = C.hP[0].p; a.P1 = C.hP[1].p; a.P2 = C.hP[2].p;
-  a.F0 = C.hF[0].p; a.F1 = C.hF[1].p; a.F2 = C.hF[2].p; a.uN = C.huN.p;
-  a.G0 = C.F[0].p; a.G1 = C.F[1].p; a.G2 = C.F[2].p;
-  a.px = c.pos[0].p; a.py = c.pos[1].p; a.pz = c.pos[2].p;
-  a.vx = c.vel[0].p; a.vy = c.vel[1].p; a.vz = c.vel[2].p;
-  a.wx = c.angVel[0].p; a.wy = c.angVel[1].p; a.wz = c.angVel[2].p;
-  a.rad = c.radius.p; a.mass = c.mass.p; a.flags = c.flags.p; a.mat = c.mat.p; a.mats = c.dMats.p;
-  for (int k = 0; k < 3; k++) a.L[k] = c.field.cell[k];
-  a.absDf = std::fabs(c.field.distFactor);
+  const int from = c.rcur, to = 1 - c.rcur;
+  c.R[to].ensure(C.n);
+  const int64_t ng = ngroups(C.n);
+  c.act.ensure(ng * GROUP);
+  ContactArgs a = contact_args(c, from, to);
   a.dt = p->dt;
   a.renorm = (p->trsfRenorm > 0 && (p->step % p->trsfRenorm) == 0) ? 1 : 0;
   a.poisson = p->poisson;
   a.en = p->en;
-  a.viscAlpha = (p->en > 0 && p->en < 1.)
-                    ? -std::sqrt(5) * std::log(p->en) / (std::sqrt(std::log(p->en) * std::log(p->en) + M_PI * M_PI))
-                    : 0.;
-  c.lastPhysKind = p->physKind;
-  c.lastPoisson = p->poisson;
-  c.lastEn = p->en;
-  c.lastViscAlpha = a.viscAlpha;
+  a.viscAlpha = viscAlpha;
   a.noSlip = p->noSlip; a.noBreak = p->noBreak; a.noFrict = p->noFrict; a.noAttraction = p->noAttraction;
-  a.dflags = c.dflags.p;
   const bool per = c.field.periodic != 0;
-  dim3 grid(nblocks(C.n, 256)), block(256);
-  if (p->lawKind == WDG_LAW_HERTZDMT) {
-    if (per) hipLaunchKernelGGL((k_contact<true, WDG_LAW_HERTZDMT>), grid, block, 0, c.stream, a);
-    else hipLaunchKernelGGL((k_contact<false, WDG_LAW_HERTZDMT>), grid, block, 0, c.stream, a);
+  const bool hertz = p->lawKind == WDG_LAW_HERTZDMT;
+  const dim3 gfil((unsigned)ng), bfil(GROUP_BS);
+  if (hertz) {
+    if (per) hipLaunchKernelGGL((k_cfilter<true, WDG_LAW_HERTZDMT>), gfil, bfil, 0, c.stream, a);
+    else hipLaunchKernelGGL((k_cfilter<false, WDG_LAW_HERTZDMT>), gfil, bfil, 0, c.stream, a);
   } else {
-    if (per) hipLaunchKernelGGL((k_contact<true, WDG_LAW_IDEALELPL>), grid, block, 0, c.stream, a);
-    else hipLaunchKernelGGL((k_contact<false, WDG_LAW_IDEALELPL>), grid, block, 0, c.stream, a);
+    if (per) hipLaunchKernelGGL((k_cfilter<true, WDG_LAW_IDEALELPL>), gfil, bfil, 0, c.stream, a);
+    else hipLaunchKernelGGL((k_cfilter<false, WDG_LAW_IDEALELPL>), gfil, bfil, 0, c.stream, a);
   }
   WDG_LAUNCH_CHECK();
+  const dim3 gforce((unsigned)(ng * CF_BLOCKS)), bforce(CF_BS);
+  if (hertz) {
+    if (per) hipLaunchKernelGGL((k_cforce<true, WDG_LAW_HERTZDMT>), gforce, bforce, 0, c.stream, a);
+    else hipLaunchKernelGGL((k_cforce<false, WDG_LAW_HERTZDMT>), gforce, bforce, 0, c.stream, a);
+  } else {
+    if (per) hipLaunchKernelGGL((k_cforce<true, WDG_LAW_IDEALELPL>), gforce, bforce, 0, c.stream, a);
+    else hipLaunchKernelGGL((k_cforce<false, WDG_LAW_IDEALELPL>), gforce, bforce, 0, c.stream, a);
+  }
+  WDG_LAUNCH_CHECK();
+  c.rcur = to;
   c.forcesPending = p->applyForces != 0;
+}
+
+// ------------------------------------------------------------------ real set from per-slot flags
+__global__ __launch_bounds__(GROUP_BS) void k_flag_bits(int64_t nc, const uint32_t* __restrict__ flag,
+                                                        unsigned long long* bits, uint32_t* tpre, uint32_t* gcnt) {
+  const int64_t base = (int64_t)blockIdx.x * GROUP;
+  uint32_t running = 0;
+  for (int pass = 0; pass < GROUP / GROUP_BS; pass++) {
+    const int64_t c = base + pass * GROUP_BS + threadIdx.x;
+    group_pass(c, c < nc && flag[c] != 0u, base, running, bits, tpre, nullptr);
+  }
+  if (threadIdx.x == 0) gcnt[blockIdx.x] = running;
+}
+
+void reals_from_flags(Ctx& c, const uint32_t* flag, int64_t nslots, int dst) {
+  Reals& N = c.R[dst];
+  N.ensure(nslots);
+  const int64_t ng = ngroups(nslots);
+  if (ng > 0) {
+    hipLaunchKernelGGL(k_flag_bits, dim3((unsigned)ng), dim3(GROUP_BS), 0, c.stream, nslots, flag, N.bits.p,
+                       N.tpre.p, N.gcnt.p);
+    WDG_LAUNCH_CHECK();
+  }
 }
 
 }  // namespace wdg

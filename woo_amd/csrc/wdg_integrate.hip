@@ -17,9 +17,9 @@ namespace wdg {
 
 struct GatherArgs {
   int64_t n;
-  const uint32_t *adj, *adjOff, *real;
-  const uint64_t* dist;
-  const double *P0, *P1, *P2, *G0, *G1, *G2;
+  const uint32_t *adj, *adjOff;
+  RealView rv;
+  const double* gp;
   double L[3];
 };
 
@@ -29,17 +29,19 @@ __device__ __forceinline__ void gather_all(const GatherArgs& g, int64_t i, d3 po
   for (uint32_t k = k0; k < k1; k++) {
     const uint32_t e = g.adj[k];
     const uint32_t slot = e & ~ADJ_PA;
-    if (!g.real[slot]) continue;
+    if (!rv_real(g.rv, slot)) continue;
+    const double2* r = reinterpret_cast<const double2*>(g.gp + (int64_t)GP * rv_index(g.rv, slot));
+    const double2 r0 = r[0], r1 = r[1], r2 = r[2];
     const bool isPA = (e & ADJ_PA) != 0u;
     const double sign = isPA ? 1. : -1.;
-    d3 Fc = mul3(mk3(g.G0[slot], g.G1[slot], g.G2[slot]), sign);
+    const d3 Fc = mul3(mk3(r0.x, r0.y, r1.x), sign);
     d3 shift = mk3(0., 0., 0.);
     if (PERIODIC && !isPA) {
-      uint64_t cd = g.dist[slot];
+      const uint64_t cd = (uint64_t)__double_as_longlong(r[3].x);
       shift = mk3(g.L[0] * unpack_cd(cd, 0), g.L[1] * unpack_cd(cd, 1), g.L[2] * unpack_cd(cd, 2));
     }
     // Contact::getForceTorqueBranch: xc = contact point - (node pos + periodic shift of pB)
-    d3 xc = sub3(mk3(g.P0[slot], g.P1[slot], g.P2[slot]), add3(pos, shift));
+    const d3 xc = sub3(mk3(r1.y, r2.x, r2.y), add3(pos, shift));
     F = add3(F, Fc);
     T = add3(T, cross3(xc, Fc));
   }
@@ -71,18 +73,30 @@ struct LeapArgs {
   int gather;
   double maxD2;
   int checkDirty;
-  uint32_t* dflags;
+  int clean;  // force/torque of integrated particles hold the reset value
+  uint32_t* dirty;      // set when a particle left its Verlet skin
+  uint32_t* dirtyNext;  // cleared for the next launch
 };
 
 template <bool PERIODIC>
 __global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   bool dirty = false;
   if (i < a.n) {
     const uint32_t fl = a.flags[i];
     const bool integrated = (fl & F_INTEGRATED) != 0u;
+    const bool grav = a.hasGravity && !(fl & F_GRAVITY_SKIP);
     d3 pos = mk3(a.px[i], a.py[i], a.pz[i]);
-    d3 F = mk3(a.fx[i], a.fy[i], a.fz[i]), T = mk3(a.tx[i], a.ty[i], a.tz[i]);
+    d3 F, T;
+    if (a.clean && integrated) {
+      // the value the last reset stored (ForceResetter / Leapfrog.reset)
+      const double m = a.mass[i];
+      F = grav ? mk3(m * a.g[0], m * a.g[1], m * a.g[2]) : mk3(0., 0., 0.);
+      T = mk3(0., 0., 0.);
+    } else {
+      F = mk3(a.fx[i], a.fy[i], a.fz[i]);
+      T = mk3(a.tx[i], a.ty[i], a.tz[i]);
+    }
     if (a.gather) gather_all<PERIODIC>(g, i, pos, F, T);
     if (!integrated) {
       if (a.gather) {
@@ -121,37 +135,43 @@ __global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
         av = mk3(w[0], w[1], w[2]);
       }
       pos = mk3(pos.x + vel.x * dt, pos.y + vel.y * dt, pos.z + vel.z * dt);
-      // leapfrogSphericalRotate (Leapfrog.cpp:280-297)
-      q4 ori = q4{a.qw[i], a.qx[i], a.qy[i], a.qz[i]};
-      if (!(av.x == 0 && av.y == 0 && av.z == 0)) {
-        double angle = sqrt(sqn3(av));
-        d3 axis = div3(av, angle);
-        double ha = 0.5 * (angle * dt);
-        double sh = sin(ha);
-        q4 q = q4{cos(ha), sh * axis.x, sh * axis.y, sh * axis.z};
-        ori = quat_mul(q, ori);
-      }
-      double qn = sqrt(ori.w * ori.w + ori.x * ori.x + ori.y * ori.y + ori.z * ori.z);
-      a.qw[i] = ori.w / qn; a.qx[i] = ori.x / qn; a.qy[i] = ori.y / qn; a.qz[i] = ori.z / qn;
       a.px[i] = pos.x; a.py[i] = pos.y; a.pz[i] = pos.z;
       a.vx[i] = vel.x; a.vy[i] = vel.y; a.vz[i] = vel.z;
       a.wx[i] = av.x; a.wy[i] = av.y; a.wz[i] = av.z;
+      // Leapfrog.cpp:240-244; when the arrays already hold the reset value nothing is stored
       if (a.reset) {
-        const bool grav = a.hasGravity && !(fl & F_GRAVITY_SKIP);
-        F = grav ? mk3(m * a.g[0], m * a.g[1], m * a.g[2]) : mk3(0., 0., 0.);
-        T = mk3(0., 0., 0.);
+        if (!a.clean) {
+          F = grav ? mk3(m * a.g[0], m * a.g[1], m * a.g[2]) : mk3(0., 0., 0.);
+          a.fx[i] = F.x; a.fy[i] = F.y; a.fz[i] = F.z;
+          a.tx[i] = 0.; a.ty[i] = 0.; a.tz[i] = 0.;
+        }
+      } else {
+        a.fx[i] = F.x; a.fy[i] = F.y; a.fz[i] = F.z;
+        a.tx[i] = T.x; a.ty[i] = T.y; a.tz[i] = T.z;
       }
-      a.fx[i] = F.x; a.fy[i] = F.y; a.fz[i] = F.z;
-      a.tx[i] = T.x; a.ty[i] = T.y; a.tz[i] = T.z;
       if (a.checkDirty) {
         d3 d = sub3(mk3(a.lx[i], a.ly[i], a.lz[i]), pos);
         double d2 = 0;
         d2 = fmax(d2, sqn3(d));
         dirty = d2 + 0. > a.maxD2;
       }
+      // leapfrogSphericalRotate (Leapfrog.cpp:280-297)
+      q4 ori = q4{a.qw[i], a.qx[i], a.qy[i], a.qz[i]};
+      if (!(av.x == 0 && av.y == 0 && av.z == 0)) {
+        double angle = sqrt(sqn3(av));
+        d3 axis = div3(av, angle);
+        double ha = 0.5 * (angle * dt);
+        double sh, ch;
+        sincos(ha, &sh, &ch);
+        q4 q = q4{ch, sh * axis.x, sh * axis.y, sh * axis.z};
+        ori = quat_mul(q, ori);
+      }
+      double qn = sqrt(ori.w * ori.w + ori.x * ori.x + ori.y * ori.y + ori.z * ori.z);
+      a.qw[i] = ori.w / qn; a.qx[i] = ori.x / qn; a.qy[i] = ori.y / qn; a.qz[i] = ori.z / qn;
     }
   }
-  if (__ballot(dirty) != 0ull && (threadIdx.x & 63) == 0) atomicOr(&a.dflags[0], 1u);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *a.dirtyNext = 0u;
+  if (__ballot(dirty) != 0ull && (threadIdx.x & 63) == 0) atomicOr(a.dirty, 1u);
 }
 
 __global__ void k_force_reset(int64_t n, const uint32_t* __restrict__ flags, const double* __restrict__ mass,
@@ -173,9 +193,7 @@ static GatherArgs gather_args(Ctx& c) {
   Contacts& C = c.con();
   GatherArgs g{};
   g.n = c.n;
-  g.adj = C.adj.p; g.adjOff = C.adjOff.p; g.real = C.real.p; g.dist = C.dist.p;
-  g.P0 = C.hP[0].p; g.P1 = C.hP[1].p; g.P2 = C.hP[2].p;
-  g.G0 = C.F[0].p; g.G1 = C.F[1].p; g.G2 = C.F[2].p;
+  g.adj = C.adj.p; g.adjOff = C.adjOff.p; g.rv = c.rview(); g.gp = c.reals().gp.p;
   for (int k = 0; k < 3; k++) g.L[k] = c.field.cell[k];
   return g;
 }
@@ -183,6 +201,7 @@ static GatherArgs gather_args(Ctx& c) {
 void flush_forces(Ctx& c) {
   if (!c.forcesPending) return;
   c.forcesPending = false;
+  c.forceClean = false;
   if (c.con().n == 0 || c.n == 0) return;
   GatherArgs g = gather_args(c);
   dim3 grid(nblocks(c.n, 256)), block(256);
@@ -214,15 +233,21 @@ void leapfrog(Ctx& c, const wdg_leapfrog_params* p) {
   a.gather = (c.forcesPending && c.con().n > 0) ? 1 : 0;
   a.checkDirty = c.boundsValid && c.verletDist > 0 ? 1 : 0;
   a.maxD2 = c.verletDist * c.verletDist;
-  a.dflags = c.dflags.p;
-  // the flag must describe the positions after this step only
-  if (a.checkDirty) WDG_HIP(hipMemsetAsync(c.dflags.p, 0, sizeof(uint32_t), c.stream));
+  a.clean = c.forceClean ? 1 : 0;
+  // the flag word of this launch was cleared by the previous launch (or at creation)
+  const int s = c.lfParity;
+  a.dirty = c.dflags.p + DIRTY_PP + s;
+  a.dirtyNext = c.dflags.p + DIRTY_PP + (1 - s);
   GatherArgs g = gather_args(c);
   dim3 grid(nblocks(c.n, 256)), block(256);
   if (c.field.periodic) hipLaunchKernelGGL(k_leapfrog<true>, grid, block, 0, c.stream, a, g);
   else hipLaunchKernelGGL(k_leapfrog<false>, grid, block, 0, c.stream, a, g);
   WDG_LAUNCH_CHECK();
+  c.lfParity = 1 - s;
+  c.dirtyIdx = DIRTY_PP + s;
   c.forcesPending = false;
+  // after a resetting step the integrated particles hold the reset value
+  c.forceClean = p->reset != 0;
 }
 
 void force_reset(Ctx& c) {
@@ -233,12 +258,14 @@ void force_reset(Ctx& c) {
                      (int)!(c.field.gravity[0] == 0 && c.field.gravity[1] == 0 && c.field.gravity[2] == 0),
                      c.force[0].p, c.force[1].p, c.force[2].p, c.torque[0].p, c.torque[1].p, c.torque[2].p);
   WDG_LAUNCH_CHECK();
+  c.forceClean = true;
 }
 
 // ------------------------------------------------------------------ DynDt
 struct DynArgs {
   int64_t n;
-  const uint32_t *A, *B, *info, *real, *adj, *adjOff;
+  const uint32_t *A, *B, *info, *adj, *adjOff;
+  RealView rv;
   const double *N0, *N1, *N2, *Y0, *Y1, *Y2, *uN;
   const double *rad, *mass, *ix, *iy, *iz;
   const uint32_t* flags;
@@ -251,7 +278,8 @@ struct DynArgs {
 
 __device__ __forceinline__ void dyn_one(const DynArgs& a, uint32_t e, d3& kt, d3& kr) {
   const uint32_t s = e & ~ADJ_PA;
-  if (!a.real[s]) return;
+  if (!rv_real(a.rv, s)) return;
+  const uint32_t r = rv_index(a.rv, s);
   const uint32_t inf = a.info[s];
   const bool swap = (inf & CI_SWAP) != 0u;
   const uint32_t pA = swap ? a.B[s] : a.A[s], pB = swap ? a.A[s] : a.B[s];
@@ -267,7 +295,7 @@ __device__ __forceinline__ void dyn_one(const DynArgs& a, uint32_t e, d3& kt, d3
   if (a.hertz) {
     phys_hertz(mA, mB, l0, l1, a.mass[pA], a.mass[pB], a.poisson, a.en, a.viscAlpha, ph);
     double kn0 = ph.K * sqrt(ph.R);
-    double uN = a.uN[s];
+    double uN = a.uN[r];
     kn = (3 / 2.) * kn0 * sqrt(-uN);
     ktt = ph.kt0 * sqrt(uN < 0 ? -uN : 0);
   } else {
@@ -275,7 +303,7 @@ __device__ __forceinline__ void dyn_one(const DynArgs& a, uint32_t e, d3& kt, d3
     kn = ph.kn;
     ktt = ph.kt;
   }
-  d3 T0 = mk3(a.N0[s], a.N1[s], a.N2[s]), T1 = mk3(a.Y0[s], a.Y1[s], a.Y2[s]);
+  d3 T0 = mk3(a.N0[r], a.N1[r], a.N2[r]), T1 = mk3(a.Y0[r], a.Y1[r], a.Y2[r]);
   q4 q = quat_from_cols(T0, T1, cross3(T0, T1));
   d3 n = quat_rot(q, mk3(1., 0., 0.));
   d3 n2 = mk3(n.x * n.x, n.y * n.y, n.z * n.z);
@@ -318,9 +346,10 @@ double crit_dt(Ctx& c) {
   if (C.n == 0) return INFINITY;
   DynArgs a{};
   a.n = c.n;
-  a.A = C.A.p; a.B = C.B.p; a.info = C.info.p; a.real = C.real.p; a.adj = C.adj.p; a.adjOff = C.adjOff.p;
-  a.N0 = C.hN[0].p; a.N1 = C.hN[1].p; a.N2 = C.hN[2].p;
-  a.Y0 = C.hY[0].p; a.Y1 = C.hY[1].p; a.Y2 = C.hY[2].p; a.uN = C.huN.p;
+  Reals& R = c.reals();
+  a.A = C.A.p; a.B = C.B.p; a.info = C.info.p; a.rv = c.rview(); a.adj = C.adj.p; a.adjOff = C.adjOff.p;
+  a.N0 = R.hN[0].p; a.N1 = R.hN[1].p; a.N2 = R.hN[2].p;
+  a.Y0 = R.hY[0].p; a.Y1 = R.hY[1].p; a.Y2 = R.hY[2].p; a.uN = R.huN.p;
   a.rad = c.radius.p; a.mass = c.mass.p; a.ix = c.inertia[0].p; a.iy = c.inertia[1].p; a.iz = c.inertia[2].p;
   a.flags = c.flags.p; a.mat = c.mat.p; a.mats = c.dMats.p;
   a.hertz = c.lastPhysKind == WDG_PHYS_HERTZ;
@@ -340,12 +369,12 @@ double crit_dt(Ctx& c) {
 }
 
 // ------------------------------------------------------------------ observables
-__global__ void k_count_contacts(int64_t n, const uint32_t* __restrict__ real, const uint32_t* __restrict__ adj,
+__global__ void k_count_contacts(int64_t n, RealView rv, const uint32_t* __restrict__ adj,
                                  const uint32_t* __restrict__ adjOff, uint32_t* cnt) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t k = 0;
-  for (uint32_t j = adjOff[i]; j < adjOff[i + 1]; j++) k += real[adj[j] & ~ADJ_PA];
+  for (uint32_t j = adjOff[i]; j < adjOff[i + 1]; j++) k += rv_real(rv, adj[j] & ~ADJ_PA) ? 1u : 0u;
   cnt[i] = k;
 }
 
@@ -388,7 +417,7 @@ void observables(Ctx& c, const double lo[3], const double hi[3], double out[3]) 
   Contacts& C = c.con();
   c.scount.ensure(c.n + 1);
   if (C.n > 0 && c.n > 0) {
-    hipLaunchKernelGGL(k_count_contacts, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, c.n, C.real.p, C.adj.p,
+    hipLaunchKernelGGL(k_count_contacts, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, c.n, c.rview(), C.adj.p,
                        C.adjOff.p, c.scount.p);
     WDG_LAUNCH_CHECK();
   } else {
@@ -414,24 +443,15 @@ void observables(Ctx& c, const double lo[3], const double hi[3], double out[3]) 
   out[2] = Vs / V;
 }
 
-__global__ void k_sum_real(int64_t nc, const uint32_t* __restrict__ real, unsigned long long* out) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long v = s < nc ? real[s] : 0u;
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
-}
-
 int64_t count_real(Ctx& c) {
   Contacts& C = c.con();
   if (C.n == 0) return 0;
-  unsigned long long* out = reinterpret_cast<unsigned long long*>(c.red.p);
-  WDG_HIP(hipMemsetAsync(out, 0, sizeof(unsigned long long), c.stream));
-  hipLaunchKernelGGL(k_sum_real, dim3(nblocks(C.n, 256)), dim3(256), 0, c.stream, C.n, C.real.p, out);
-  WDG_LAUNCH_CHECK();
-  unsigned long long h;
-  WDG_HIP(hipMemcpyAsync(&h, out, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+  std::vector<uint32_t> g((size_t)ngroups(C.n));
+  WDG_HIP(hipMemcpyAsync(g.data(), c.reals().gcnt.p, g.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
   WDG_HIP(hipStreamSynchronize(c.stream));
-  return (int64_t)h;
+  int64_t s = 0;
+  for (uint32_t x : g) s += x;
+  return s;
 }
 
 }  // namespace wdg

@@ -67,6 +67,10 @@ __host__ __device__ inline int unpack_cd(uint64_t v, int k) {
   return (int)f;
 }
 
+// dflags words: [0] dirty flag of a refresh / k_dirty, [1] error bits, [3..5] scan totals,
+// [DIRTY_PP + 0/1] dirty flags set by alternate leapfrog launches (each launch clears the other)
+constexpr int DIRTY_PP = 12;
+
 // device error bits
 enum : uint32_t { ERR_NAN_FORCE = 1u, ERR_DOUBLE_IMAGE = 2u, ERR_CD_RANGE = 4u };
 
@@ -94,36 +98,85 @@ struct DBuf {
   }
 };
 
-// one contact set: slots sorted by (lo, hi) storage index; SoA
+// slot list of one broadphase refresh: candidate pairs sorted by (lo, hi) storage index, SoA
 struct Contacts {
   int64_t n = 0;
   DBuf<uint32_t> A, B;        // lo / hi storage index
   DBuf<uint32_t> info;        // CI_* bits
   DBuf<uint64_t> dist;        // packed cellDist of pB relative to pA (periodic only)
-  DBuf<uint32_t> real;        // 1 if geom+phys exist (history valid)
-  DBuf<double> hN[3], hY[3];  // L6Geom.trsf columns 0 and 1
-  DBuf<double> hP[3];         // contact point (L6Geom.node.pos)
-  DBuf<double> hF[3];         // CPhys.force, local frame
-  DBuf<double> huN;           // L6Geom.uN
-  DBuf<double> F[3];          // global force on pA, written by the contact kernel for the gather
+  DBuf<uint32_t> carry;       // at a refresh: 1 + old slot of a carried real contact, else 0
   // per-particle adjacency: entries slot|ADJ_PA grouped by storage index, each group in
   // increasing order of the other particle's user id (the canonical summation order)
   DBuf<uint32_t> adj, adjOff;
   void ensure(int64_t nc, int64_t npar, bool periodic) {
     size_t c = (size_t)(nc > 0 ? nc : 1);
-    A.ensure(c); B.ensure(c); info.ensure(c); real.ensure(c);
+    A.ensure(c); B.ensure(c); info.ensure(c); carry.ensure(c);
     if (periodic) dist.ensure(c);
-    for (int k = 0; k < 3; k++) { hN[k].ensure(c); hY[k].ensure(c); hP[k].ensure(c); hF[k].ensure(c); F[k].ensure(c); }
-    huN.ensure(c);
     adj.ensure(2 * c);
     adjOff.ensure((size_t)npar + 1);
   }
   void release() {
-    A.release(); B.release(); info.release(); dist.release(); real.release();
-    for (int k = 0; k < 3; k++) { hN[k].release(); hY[k].release(); hP[k].release(); hF[k].release(); F[k].release(); }
-    huN.release(); adj.release(); adjOff.release();
+    A.release(); B.release(); info.release(); dist.release(); carry.release(); adj.release(); adjOff.release();
   }
 };
+
+// Real contacts of the current slot list (the Contact objects that have geom and phys).
+// Slot s is real when bit s is set. Slots form groups of GROUP; the state of the real
+// contacts of a group is packed densely in the group's range, in slot order: the local
+// index of s is the number of real slots of the group before s. Reads and writes of the
+// state therefore stream, and no global prefix sum is needed to find an index.
+constexpr int TILE = 256;     // slots per prefix-count tile (4 words)
+constexpr int GROUP = 4096;   // slots per group (16 tiles)
+constexpr int GROUP_BS = 1024;  // threads of a group kernel
+inline int64_t ntiles(int64_t nslots) { return (nslots + TILE - 1) / TILE; }
+inline int64_t ngroups(int64_t nslots) { return (nslots + GROUP - 1) / GROUP; }
+constexpr int GP = 8;         // doubles per gather record
+struct Reals {
+  DBuf<unsigned long long> bits;  // 64 words per group
+  DBuf<uint32_t> tpre;            // per tile: real slots of its group before the tile
+  DBuf<uint32_t> gcnt;            // per group: real slots
+  DBuf<double> hN[3], hY[3];      // L6Geom.trsf columns 0 and 1
+  DBuf<double> hF[3];             // CPhys.force, local frame
+  DBuf<double> huN;               // L6Geom.uN
+  // gather record per contact: global force on pA (3), contact point (3), cellDist bits, unused
+  DBuf<double> gp;
+  // room for every slot of a list of nslots being real
+  void ensure(int64_t nslots) {
+    const size_t g = (size_t)ngroups(nslots);
+    const size_t c = g > 0 ? g * GROUP : 1;
+    bits.ensure(g * (GROUP / 64) + 1);
+    tpre.ensure(g * (GROUP / TILE) + 1);
+    gcnt.ensure(g + 1);
+    for (int k = 0; k < 3; k++) { hN[k].ensure(c); hY[k].ensure(c); hF[k].ensure(c); }
+    huN.ensure(c);
+    gp.ensure(GP * c);
+  }
+  void release() {
+    bits.release(); tpre.release(); gcnt.release();
+    for (int k = 0; k < 3; k++) { hN[k].release(); hY[k].release(); hF[k].release(); }
+    huN.release(); gp.release();
+  }
+};
+struct RealView {
+  const unsigned long long* bits;
+  const uint32_t* tpre;
+};
+// The used part of a group's range starts at a rotated offset (golden-ratio skew), so
+// that the densely used regions of consecutive groups spread over all HBM channels
+// instead of all starting at the same power-of-two alignment.
+constexpr uint32_t GROUP_SKEW = 2531;
+__host__ __device__ inline uint32_t state_index(uint32_t g, uint32_t local) {
+  return g * (uint32_t)GROUP + ((local + g * GROUP_SKEW) & (uint32_t)(GROUP - 1));
+}
+__device__ __forceinline__ bool rv_real(const RealView& v, uint32_t s) { return (v.bits[s >> 6] >> (s & 63)) & 1ull; }
+// state index of real slot s
+__device__ __forceinline__ uint32_t rv_index(const RealView& v, uint32_t s) {
+  const uint32_t w = s >> 6;
+  uint32_t r = v.tpre[s >> 8];
+  for (uint32_t q = (s >> 8) << 2; q < w; q++) r += (uint32_t)__popcll(v.bits[q]);
+  r += (uint32_t)__popcll(v.bits[w] & ((1ull << (s & 63)) - 1ull));
+  return state_index(s / (uint32_t)GROUP, r);
+}
 
 constexpr int NPD = 24;  // permuted double arrays per particle
 constexpr int NPU = 4;   // permuted u32 arrays per particle
@@ -146,10 +199,16 @@ struct Ctx {
   DBuf<double> spareD[NPD];
   DBuf<uint32_t> spareU[NPU];
 
-  // contacts, double-buffered across refreshes
+  // slot lists, double-buffered across refreshes
   Contacts C[2];
   int cur = 0;
   Contacts& con() { return C[cur]; }
+  // real contacts of con(), double-buffered across contact loops and refreshes
+  Reals R[2];
+  int rcur = 0;
+  Reals& reals() { return R[rcur]; }
+  RealView rview(int k) const { return RealView{R[k].bits.p, R[k].tpre.p}; }
+  RealView rview() const { return rview(rcur); }
 
   // broadphase scratch
   DBuf<uint32_t> cellKey, sortIdx, cellKeyTmp, sortIdxTmp, perm, invPerm;
@@ -161,6 +220,7 @@ struct Ctx {
   DBuf<uint64_t> adjKey, adjKeyTmp;
   DBuf<uint32_t> adjValTmp, adjPart;
   DBuf<uint32_t> sortHist, scanAux;
+  DBuf<uint32_t> act;     // slot of each state index (contact loop), per-slot flags (relaxation)
   DBuf<double> red;       // reductions
   DBuf<double> xfer;      // user-id-ordered staging for upload/download
   DBuf<uint32_t> dflags;  // [0] dirty, [1] error bits, [2..] scratch counters
@@ -177,8 +237,13 @@ struct Ctx {
 
   // engine state
   bool boundsValid = false;
-  bool dirtyKnown = false;     // dflags[0] reflects the current positions
+  bool dirtyKnown = false;     // dflags[dirtyIdx] reflects the current positions
+  int dirtyIdx = 0;            // 0 (refresh, k_dirty) or DIRTY_PP + parity (leapfrog)
+  int lfParity = 0;            // flag word the next leapfrog launch sets
   bool forcesPending = false;  // contact forces not yet gathered into force/torque
+  // force/torque of every integrated particle hold the reset value (m*gravity unless
+  // gravity-skipped, zero torque), so the integrator computes them instead of loading
+  bool forceClean = false;
   double verletDist = NAN;
   int64_t nRefresh = 0, nCollide = 0;
   int64_t dynStepLast = -1, dynStepPrev = -1;
@@ -203,11 +268,24 @@ void flush_forces(Ctx& c);
 double crit_dt(Ctx& c);
 void observables(Ctx& c, const double lo[3], const double hi[3], double out[3]);
 int64_t count_real(Ctx& c);
+// bits + tile prefix of R[dst] from a per-slot flag array (nonzero = real)
+void reals_from_flags(Ctx& c, const uint32_t* flag, int64_t nslots, int dst);
 void check_errors(Ctx& c);
 void relax(Ctx& c, int maxIter, double inflate, double tol, double* maxOverlap, int* iters);
 void drop_contacts(Ctx& c);
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Workgroups are handed to the 8 XCDs round-robin (workgroup b runs on XCD b % 8), and
+// each XCD has its own L2. This maps the hardware workgroup index to a logical one so
+// that each XCD receives one contiguous run of logical workgroups: neighbouring work
+// (neighbouring particles and slots) shares an L2, and work that is uneven along the
+// logical index is spread evenly over the XCDs.
+constexpr uint32_t NXCD = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  const uint32_t x = b % NXCD, i = b / NXCD, q = nb / NXCD, r = nb % NXCD;
+  return x * q + (x < r ? x : r) + i;
+}
 inline int bits_for(uint64_t maxval) {
   int b = 1;
   while (b < 64 && (maxval >> b) != 0) b++;

@@ -59,20 +59,33 @@ __device__ __forceinline__ q4 quat_from_cols(d3 c0, d3 c1, d3 c2) {
     q.y = (m[0][2] - m[2][0]) * t;
     q.z = (m[1][0] - m[0][1]) * t;
   } else {
+    // largest diagonal element i, then j = i+1, k = i+2 (mod 3); the three cases are
+    // written out so that nothing is indexed at run time
     int i = 0;
     if (m[1][1] > m[0][0]) i = 1;
-    if (m[2][2] > m[i][i]) i = 2;
-    int j = (i + 1) % 3, k = (j + 1) % 3;
-    t = sqrt(m[i][i] - m[j][j] - m[k][k] + 1.0);
-    double v[3];
-    v[i] = 0.5 * t;
-    t = 0.5 / t;
-    q.w = (m[k][j] - m[j][k]) * t;
-    v[j] = (m[j][i] + m[i][j]) * t;
-    v[k] = (m[k][i] + m[i][k]) * t;
-    q.x = v[0];
-    q.y = v[1];
-    q.z = v[2];
+    if (m[2][2] > (i == 0 ? m[0][0] : m[1][1])) i = 2;
+    if (i == 0) {
+      t = sqrt(m[0][0] - m[1][1] - m[2][2] + 1.0);
+      q.x = 0.5 * t;
+      t = 0.5 / t;
+      q.w = (m[2][1] - m[1][2]) * t;
+      q.y = (m[1][0] + m[0][1]) * t;
+      q.z = (m[2][0] + m[0][2]) * t;
+    } else if (i == 1) {
+      t = sqrt(m[1][1] - m[2][2] - m[0][0] + 1.0);
+      q.y = 0.5 * t;
+      t = 0.5 / t;
+      q.w = (m[0][2] - m[2][0]) * t;
+      q.z = (m[2][1] + m[1][2]) * t;
+      q.x = (m[0][1] + m[1][0]) * t;
+    } else {
+      t = sqrt(m[2][2] - m[0][0] - m[1][1] + 1.0);
+      q.z = 0.5 * t;
+      t = 0.5 / t;
+      q.w = (m[1][0] - m[0][1]) * t;
+      q.x = (m[0][2] + m[2][0]) * t;
+      q.y = (m[1][2] + m[2][1]) * t;
+    }
   }
   return q;
 }

@@ -124,6 +124,7 @@ void relax(Ctx& c, int maxIter, double inflate, double tol, double* maxOverlap, 
   const double saveVd = c.verletDist;
   c.verletDist = NAN;
   unsigned long long* dmax = reinterpret_cast<unsigned long long*>(c.red.p) + 8;
+  DBuf<double> push[3];  // per-slot push of pB
   double ov = INFINITY;
   int it = 0;
   for (; it < maxIter; it++) {
@@ -132,9 +133,11 @@ void relax(Ctx& c, int maxIter, double inflate, double tol, double* maxOverlap, 
     Contacts& C = c.con();
     WDG_HIP(hipMemsetAsync(dmax, 0, sizeof(unsigned long long), c.stream));
     if (C.n > 0) {
+      c.act.ensure(C.n);
+      for (int k = 0; k < 3; k++) push[k].ensure(C.n);
       RelaxArgs a{};
-      a.nc = C.n; a.A = C.A.p; a.B = C.B.p; a.info = C.info.p; a.dist = C.dist.p; a.active = C.real.p;
-      a.G0 = C.F[0].p; a.G1 = C.F[1].p; a.G2 = C.F[2].p;
+      a.nc = C.n; a.A = C.A.p; a.B = C.B.p; a.info = C.info.p; a.dist = C.dist.p; a.active = c.act.p;
+      a.G0 = push[0].p; a.G1 = push[1].p; a.G2 = push[2].p;
       a.px = c.pos[0].p; a.py = c.pos[1].p; a.pz = c.pos[2].p; a.rad = c.radius.p;
       for (int k = 0; k < 3; k++) a.L[k] = c.field.cell[k];
       a.periodic = c.field.periodic;
@@ -142,7 +145,7 @@ void relax(Ctx& c, int maxIter, double inflate, double tol, double* maxOverlap, 
       hipLaunchKernelGGL(k_relax_pairs, dim3(nblocks(C.n, 256)), dim3(256), 0, c.stream, a);
       WDG_LAUNCH_CHECK();
       hipLaunchKernelGGL(k_relax_move, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, c.n, c.flags.p, C.adj.p,
-                         C.adjOff.p, C.real.p, C.F[0].p, C.F[1].p, C.F[2].p, c.pos[0].p, c.pos[1].p, c.pos[2].p);
+                         C.adjOff.p, c.act.p, push[0].p, push[1].p, push[2].p, c.pos[0].p, c.pos[1].p, c.pos[2].p);
       WDG_LAUNCH_CHECK();
       c.dirtyKnown = false;
     }
@@ -165,6 +168,7 @@ void relax(Ctx& c, int maxIter, double inflate, double tol, double* maxOverlap, 
     WDG_HIP(hipMemcpy(c.radius.p, h.data(), c.n * sizeof(double), hipMemcpyHostToDevice));
   }
   ruid.release();
+  for (int k = 0; k < 3; k++) push[k].release();
   c.rmin = rminSave;
   c.rmax = rmaxSave;
   drop_contacts(c);

@@ -60,6 +60,46 @@ __global__ void k_scan_add(uint32_t* out, int64_t n, const uint32_t* __restrict_
   if (idx < n) out[idx] += blockOff[idx / SCAN_TILE];
 }
 
+// single-workgroup scan for short arrays (one launch); in may equal out
+constexpr int S1_BS = 1024;
+constexpr int S1_ITEMS = 8;
+constexpr int64_t S1_MAX = 8192;
+__global__ __launch_bounds__(S1_BS) void k_scan_one(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* dTotal) {
+  __shared__ uint32_t wsum[S1_BS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  uint32_t carry = 0;
+  for (int64_t base = 0; base < n; base += S1_BS * S1_ITEMS) {
+    uint32_t v[S1_ITEMS], sum = 0;
+    for (int i = 0; i < S1_ITEMS; i++) {
+      const int64_t idx = base + (int64_t)tid * S1_ITEMS + i;
+      const uint32_t x = idx < n ? in[idx] : 0u;
+      v[i] = sum;
+      sum += x;
+    }
+    uint32_t inc = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t t = __shfl_up(inc, off);
+      if (lane >= off) inc += t;
+    }
+    if (lane == 63) wsum[wv] = inc;
+    __syncthreads();
+    uint32_t wpre = 0, tot = 0;
+    for (int w = 0; w < S1_BS / 64; w++) {
+      const uint32_t s = wsum[w];
+      if (w < wv) wpre += s;
+      tot += s;
+    }
+    const uint32_t excl = carry + wpre + (inc - sum);
+    for (int i = 0; i < S1_ITEMS; i++) {
+      const int64_t idx = base + (int64_t)tid * S1_ITEMS + i;
+      if (idx < n) out[idx] = v[i] + excl;
+    }
+    carry += tot;
+    __syncthreads();
+  }
+  if (tid == 0) *dTotal = carry;
+}
+
 static void scan_rec(Ctx& c, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* aux, uint32_t* dTotal) {
   int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
   if (nb <= 1) {
@@ -78,6 +118,11 @@ static void scan_rec(Ctx& c, const uint32_t* in, uint32_t* out, int64_t n, uint3
 void scan_exclusive(Ctx& c, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* dTotal) {
   if (n <= 0) {
     WDG_HIP(hipMemsetAsync(dTotal, 0, sizeof(uint32_t), c.stream));
+    return;
+  }
+  if (n <= S1_MAX) {
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(S1_BS), 0, c.stream, in, out, n, dTotal);
+    WDG_LAUNCH_CHECK();
     return;
   }
   int64_t need = 0, m = n;
