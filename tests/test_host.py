@@ -1,0 +1,111 @@
+"""The Woo scripting surface without a GPU: object model, defaults taken from the
+reference's attribute declarations, constructors, labels, mass lumping and the refusal
+to run without the device engine (there is no CPU fallback)."""
+import math
+
+import numpy as np
+import pytest
+
+
+@pytest.fixture(scope="module")
+def woo():
+    import woo_amd.woo as w
+
+    return w
+
+
+def test_defaults_follow_reference(woo):
+    from woo_amd.woo.dem import (Cp2_HertzMat_HertzPhys, DemField, DynDt, FrictMat, HertzMat, InsertionSortCollider,
+                                 Law2_L6Geom_FrictPhys_IdealElPl, Law2_L6Geom_HertzPhys_DMT, Leapfrog)
+
+    assert InsertionSortCollider().verletDist == -.05        # Collision.hpp:94
+    lf = Leapfrog()
+    assert lf.damping == .2 and lf.reset is False            # Leapfrog.hpp:50-51
+    assert DynDt().maxRelInc == 1e-4                         # DynDt.hpp:21
+    cp = Cp2_HertzMat_HertzPhys()
+    assert cp.poisson == .2 and math.isnan(cp.en)            # Hertz.hpp:58-59
+    assert Law2_L6Geom_HertzPhys_DMT().noAttraction is True  # Hertz.hpp:96
+    iep = Law2_L6Geom_FrictPhys_IdealElPl()
+    assert not (iep.noSlip or iep.noBreak or iep.noFrict)    # IdealElPl.hpp:22-24
+    f = FrictMat()
+    assert (f.density, f.young, f.tanPhi, f.ktDivKn) == (1000., 1e9, .5, .2)
+    h = HertzMat(surfEnergy=.1)
+    assert h.surfEnergy == .1 and h.alpha == 0.
+    d = DemField()
+    assert d.distFactor == -1 and d.loneMask == 2            # Particle.hpp:359-361
+    assert (DemField.defaultMovableMask, DemField.defaultBoundaryMask) == (5, 3)
+    S = woo.core.Scene()
+    assert S.dtSafety == .9 and math.isnan(S.dt) and S.step == 0
+
+
+def test_particles_and_containers(woo):
+    from woo_amd.woo.dem import DemField, FrictMat, Sphere, Wall
+
+    S = woo.core.Scene(fields=[DemField(gravity=(0, 0, -9.81))])
+    m = FrictMat(density=3000, young=1e7)
+    s = woo.utils.sphere((0, 0, .1), .01, mat=m, vel=(1, 0, 0))
+    w = woo.utils.wall(.5, axis=2, sense=-1, mat=m)
+    ids = S.dem.par.add([s, w])
+    assert ids == [0, 1] and len(S.dem.par) == 2 and S.dem.par[-1].id == 1
+    # Sphere::lumpMassInertia
+    r, rho = .01, 3000.
+    mass = (4 / 3.) * math.pi * (r * r * r) * rho
+    assert s.mass == mass
+    np.testing.assert_array_equal(s.inertia, [0. + 1. * (2. / 5.) * mass * (r * r)] * 3)
+    assert s.blocked == "" and w.blocked == "xyzXYZ" and w.mass == 0
+    np.testing.assert_array_equal(w.pos, [0, 0, .5])
+    assert s.mask == 5 and w.mask == 3
+    # DemData.guessMoving decides DemField.nodes
+    assert [n.pos[2] for n in S.dem.nodes] == [.1]
+    with pytest.raises(Exception):
+        S.dem.par.add(s)  # already added
+    s.blocked = "xZ"
+    assert s.blocked == "xZ" and s.shape.nodes[0].dem.blocked == "xZ"
+    with pytest.raises(Exception):
+        s.blocked = "q"
+    assert isinstance(Sphere.make((0, 0, 0), 1.), type(s)) and isinstance(w.shape, Wall)
+    box = woo.utils.wallBox(((0, 0, 0), (1, 2, 3)))
+    assert [(b.shape.axis, b.shape.sense, float(b.pos[b.shape.axis])) for b in box] == [
+        (0, 1, 0.), (1, 1, 0.), (2, 1, 0.), (0, -1, 1.), (1, -1, 2.), (2, -1, 3.)]
+
+
+def test_engines_functors_labels(woo):
+    from woo_amd.woo.dem import Cp2_HertzMat_HertzPhys, Law2_L6Geom_HertzPhys_DMT
+
+    S = woo.core.Scene()
+    S.engines = woo.utils.defaultEngines(damping=.1, cp2=Cp2_HertzMat_HertzPhys(poisson=.25, en=.6, label='cp2'),
+                                         law=Law2_L6Geom_HertzPhys_DMT(), dynDtPeriod=10)
+    names = [type(e).__name__ for e in S.engines]
+    assert names == ["Leapfrog", "InsertionSortCollider", "ContactLoop", "DynDt"]  # utils.py:120-129
+    assert S.lab.leapfrog.damping == .1 and S.lab.leapfrog.reset
+    assert S.lab.cp2.poisson == .25 and S.lab.dynDt.stepPeriod == 10
+    cl = S.lab.contactLoop
+    assert [type(f).__name__ for f in cl.geoDisp] == ["Cg2_Sphere_Sphere_L6Geom", "Cg2_Wall_Sphere_L6Geom"]
+    assert cl.geoDisp[0].trsfRenorm == 100
+    pr = woo.core.PyRunner(7, "pass")
+    assert pr.stepPeriod == 7 and pr.command == "pass"
+    with pytest.raises(ValueError):
+        woo.utils.defaultEngines(gravity=(0, 0, -9.81))
+
+
+def test_install_alias(woo):
+    import sys
+
+    woo.install()
+    import woo as w2  # noqa: F811
+
+    assert sys.modules["woo.dem"] is woo.dem and w2.utils is woo.utils
+
+
+def test_run_needs_the_device_engine(woo):
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from woo_amd.woo.dem import DemField
+
+    S = woo.core.Scene(fields=[DemField()])
+    S.dem.par.add(woo.utils.sphere((0, 0, 0), .01))
+    S.engines = woo.utils.defaultEngines()
+    with pytest.raises(RuntimeError):
+        S.run(1)

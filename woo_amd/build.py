@@ -46,6 +46,43 @@ def _compile(src, force):
     return obj
 
 
+HOST = os.path.join(HERE, "host")
+HOST_SOURCES = ["woo_host.cpp", "woo_py.cpp"]
+HOST_HEADERS = ["woo_host.hpp"]
+
+
+def host_module_path():
+    import sysconfig
+
+    return os.path.join(HERE, "_woo" + sysconfig.get_config_var("EXT_SUFFIX"))
+
+
+def build_host(force=False, verbose=True):
+    """The host classes (Scene, DemField, engines, functors) as the Python module
+    woo_amd._woo: plain C++ (g++) against pybind11, linked to libwoo_amd.so."""
+    import sysconfig
+
+    import pybind11
+
+    out = host_module_path()
+    deps = [os.path.join(HOST, f) for f in HOST_SOURCES + HOST_HEADERS] + [os.path.join(ROOT, "include", "woo_amd.h"),
+                                                                         LIB]
+    if not force and os.path.exists(out) and os.path.getmtime(out) >= _newest(deps):
+        return out
+    cxx = os.environ.get("CXX", "g++")
+    cmd = [cxx, "-O2", "-std=c++17", "-shared", "-fPIC", "-fvisibility=hidden", "-I", pybind11.get_include(),
+           "-I", sysconfig.get_paths()["include"], "-I", os.path.join(ROOT, "include")]
+    cmd += [os.path.join(HOST, f) for f in HOST_SOURCES]
+    cmd += ["-L", HERE, "-lwoo_amd", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host module build failed:\n{r.stderr[-6000:]}")
+    os.replace(out + ".tmp", out)
+    if verbose:
+        print(f"built {out}")
+    return out
+
+
 def build(force=False, verbose=True):
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
@@ -58,6 +95,7 @@ def build(force=False, verbose=True):
         os.replace(LIB + ".tmp", LIB)
     if verbose:
         print(f"built {LIB}")
+    build_host(force=force, verbose=verbose)
     return LIB
 
 

@@ -1,0 +1,407 @@
+// Host side of the DEM step engine: lazy host/device coherence and the step driver
+// (Scene::doOneStep, src/core/Scene.cpp:366-467) over the C ABI.
+#include "woo_host.hpp"
+
+#include <algorithm>
+#include <map>
+
+namespace woo {
+
+void check(int rc) {
+  if (rc != WDG_OK) throw std::runtime_error(std::string(wdg_last_error()));
+}
+
+// 'xyzXYZ' <-> blocked-DOF bits (DemData.blocked, Particle.hpp:210-213)
+static const char DOF_CHARS[6] = {'x', 'y', 'z', 'X', 'Y', 'Z'};
+std::string DemData::blocked() const {
+  std::string s;
+  for (int k = 0; k < 6; k++)
+    if (flags & (1u << k)) s += DOF_CHARS[k];
+  return s;
+}
+void DemData::setBlocked(const std::string& s) {
+  uint32_t b = 0;
+  for (char ch : s) {
+    const char* p = std::find(DOF_CHARS, DOF_CHARS + 6, ch);
+    if (p == DOF_CHARS + 6) throw std::invalid_argument(std::string("invalid blocked DOF '") + ch + "' (use xyzXYZ)");
+    b |= 1u << (p - DOF_CHARS);
+  }
+  flags = (flags & ~(uint32_t)DOF_ALL) | b;
+}
+
+DeviceState::DeviceState(int device) { check(wdg_ctx_create(device, &ctx)); }
+DeviceState::~DeviceState() {
+  if (ctx) wdg_ctx_destroy(ctx);
+}
+
+// ------------------------------------------------------------------ DemField
+int64_t DemField::add(const std::shared_ptr<Particle>& p, int nodes) {
+  if (!p || !p->shape || !p->material || p->shape->nodes.size() != 1 || !p->shape->nodes[0])
+    throw std::invalid_argument("particle needs a material and a shape with one node");
+  if (p->id >= 0) throw std::invalid_argument("particle was already added (id " + std::to_string(p->id) + ")");
+  if (!dynamic_cast<Sphere*>(p->shape.get()) && !dynamic_cast<Wall*>(p->shape.get()))
+    throw std::invalid_argument(std::string("shape ") + p->shape->className() + " is not supported by the device engine");
+  pull();
+  p->id = (int64_t)particles.size();
+  p->node().field = this;
+  // ParticleContainer::pyAppend (ParticleContainer.cpp:176-201): nodes=1 always, 0 never,
+  // -1 by DemData::guessMoving
+  p->node().integrated = nodes == 1 || (nodes == -1 && p->node().dem.guessMoving());
+  particles.push_back(p);
+  structureChanged = true;
+  contactsValid = false;
+  return p->id;
+}
+
+static const int FIELDS[8] = {WDG_F_POS, WDG_F_VEL, WDG_F_ANGVEL, WDG_F_ORI,
+                              WDG_F_FORCE, WDG_F_TORQUE, WDG_F_MASS, WDG_F_INERTIA};
+
+static double* node_field(Node& n, int f) {
+  switch (f) {
+    case WDG_F_POS: return n.pos.data();
+    case WDG_F_VEL: return n.dem.vel.data();
+    case WDG_F_ANGVEL: return n.dem.angVel.data();
+    case WDG_F_ORI: return n.ori.data();
+    case WDG_F_FORCE: return n.dem.force.data();
+    case WDG_F_TORQUE: return n.dem.torque.data();
+    case WDG_F_MASS: return &n.dem.mass;
+    case WDG_F_INERTIA: return n.dem.inertia.data();
+  }
+  return nullptr;
+}
+static int field_width(int f) { return f == WDG_F_ORI ? 4 : (f == WDG_F_MASS ? 1 : 3); }
+
+void DemField::pull() {
+  if (!deviceNewer || !dev) return;
+  deviceNewer = false;
+  const size_t n = particles.size();
+  std::vector<double> buf;
+  for (int f : FIELDS) {
+    const int w = field_width(f);
+    buf.resize(n * w);
+    check(wdg_download(dev->ctx, f, buf.data()));
+    for (size_t i = 0; i < n; i++) std::copy(&buf[i * w], &buf[i * w] + w, node_field(particles[i]->node(), f));
+  }
+}
+
+void DemField::push(Scene& S) {
+  if (!dev) {
+    dev = std::make_shared<DeviceState>(S.device);
+    structureChanged = true;
+  }
+  wdg_field fd{};
+  for (int k = 0; k < 3; k++) fd.gravity[k] = gravity[k];
+  fd.distFactor = distFactor;
+  fd.loneMask = loneMask;
+  fd.periodic = S.periodic ? 1 : 0;
+  for (int k = 0; k < 3; k++) fd.cell[k] = S.cell->size[k];
+  check(wdg_set_field(dev->ctx, &fd));
+  const size_t n = particles.size();
+  if (structureChanged) {
+    // materials in first-use order
+    std::map<const Material*, int> midx;
+    std::vector<wdg_material> mats;
+    std::vector<int32_t> shape(n), axis(n), sense(n), mat(n);
+    std::vector<uint32_t> mask(n), flags(n);
+    std::vector<uint8_t> integ(n);
+    std::vector<double> radius(n), bufs[8];
+    for (size_t i = 0; i < n; i++) {
+      const Particle& p = *particles[i];
+      auto it = midx.find(p.material.get());
+      if (it == midx.end()) {
+        const auto* fm = dynamic_cast<const FrictMat*>(p.material.get());
+        if (!fm) throw std::runtime_error(std::string("material ") + p.material->className() +
+                                          " is not supported (FrictMat or HertzMat)");
+        const auto* hm = dynamic_cast<const HertzMat*>(fm);
+        wdg_material m{};
+        m.kind = hm ? WDG_HERTZMAT : WDG_FRICTMAT;
+        m.density = fm->density;
+        m.young = fm->young;
+        m.ktDivKn = fm->ktDivKn;
+        m.tanPhi = fm->tanPhi;
+        m.surfEnergy = hm ? hm->surfEnergy : 0.;
+        m.alpha = hm ? hm->alpha : 0.;
+        it = midx.emplace(p.material.get(), (int)mats.size()).first;
+        mats.push_back(m);
+      }
+      mat[i] = it->second;
+      if (const auto* s = dynamic_cast<const Sphere*>(p.shape.get())) {
+        shape[i] = WDG_SPHERE;
+        radius[i] = s->radius;
+      } else {
+        const auto* w = dynamic_cast<const Wall*>(p.shape.get());
+        shape[i] = WDG_WALL;
+        axis[i] = w->axis;
+        sense[i] = w->sense;
+      }
+      mask[i] = p.mask;
+      flags[i] = p.node().dem.flags;
+      integ[i] = p.node().integrated ? 1 : 0;
+    }
+    for (int fi = 0; fi < 8; fi++) {
+      const int w = field_width(FIELDS[fi]);
+      bufs[fi].resize(n * w);
+      for (size_t i = 0; i < n; i++) {
+        const double* src = node_field(particles[i]->node(), FIELDS[fi]);
+        std::copy(src, src + w, &bufs[fi][i * w]);
+      }
+    }
+    check(wdg_set_materials(dev->ctx, mats.data(), (int32_t)mats.size()));
+    wdg_particles pd{};
+    pd.n = (int64_t)n;
+    pd.shape = shape.data(); pd.radius = radius.data(); pd.wallAxis = axis.data(); pd.wallSense = sense.data();
+    pd.material = mat.data(); pd.mask = mask.data(); pd.flags = flags.data(); pd.integrated = integ.data();
+    pd.pos = bufs[0].data(); pd.vel = bufs[1].data(); pd.angVel = bufs[2].data(); pd.ori = bufs[3].data();
+    pd.force = bufs[4].data(); pd.torque = bufs[5].data(); pd.mass = bufs[6].data(); pd.inertia = bufs[7].data();
+    check(wdg_upload_particles(dev->ctx, &pd));
+    structureChanged = hostDirty = false;
+  } else if (hostDirty) {
+    std::vector<double> buf;
+    for (int f : FIELDS) {
+      const int w = field_width(f);
+      buf.resize(n * w);
+      for (size_t i = 0; i < n; i++) {
+        const double* src = node_field(particles[i]->node(), f);
+        std::copy(src, src + w, &buf[i * w]);
+      }
+      check(wdg_upload(dev->ctx, f, buf.data()));
+    }
+    hostDirty = false;
+  }
+}
+
+const std::vector<ContactView>& DemField::contacts() {
+  if (contactsValid) return contactCache;
+  contactCache.clear();
+  if (dev && !structureChanged) {
+    int64_t nc = 0;
+    check(wdg_contact_count(dev->ctx, 0, &nc));
+    std::vector<int32_t> ids(2 * nc), cd(3 * nc);
+    std::vector<double> data(16 * nc);
+    if (nc) check(wdg_contact_list(dev->ctx, 0, ids.data(), cd.data(), data.data()));
+    contactCache.resize(nc);
+    for (int64_t i = 0; i < nc; i++) {
+      ContactView& c = contactCache[i];
+      const double* d = &data[16 * i];
+      c.id1 = ids[2 * i];
+      c.id2 = ids[2 * i + 1];
+      for (int k = 0; k < 3; k++) c.cellDist[k] = cd[3 * i + k];
+      c.real = d[15] != 0.;
+      if (c.real) {
+        // L6Geom.trsf rows are the local axes: row 0 = col0 of the stored columns, etc.
+        const double c0[3] = {d[0], d[1], d[2]}, c1[3] = {d[3], d[4], d[5]};
+        const double c2[3] = {c0[1] * c1[2] - c0[2] * c1[1], c0[2] * c1[0] - c0[0] * c1[2],
+                              c0[0] * c1[1] - c0[1] * c1[0]};
+        for (int k = 0; k < 3; k++) {
+          c.trsf[k] = c0[k];
+          c.trsf[3 + k] = c1[k];
+          c.trsf[6 + k] = c2[k];
+        }
+        c.contPt = {d[6], d[7], d[8]};
+        c.force = {d[9], d[10], d[11]};
+        c.uN = d[12];
+        c.kn = d[13];
+        c.kt = d[14];
+      }
+    }
+  }
+  contactsValid = true;
+  return contactCache;
+}
+
+int64_t DemField::countReal() {
+  int64_t n = 0;
+  for (const auto& c : contacts()) n += c.real ? 1 : 0;
+  return n;
+}
+
+double DemField::pWaveDt() const {
+  double dt = Inf;
+  for (const auto& p : particles) {
+    const auto* s = dynamic_cast<const Sphere*>(p->shape.get());
+    const auto* m = dynamic_cast<const ElastMat*>(p->material.get());
+    if (!s || !m) continue;
+    dt = std::min(dt, s->radius / std::sqrt(m->young / m->density));
+  }
+  return dt;
+}
+
+std::array<double, 3> DemField::observables(const Vec3& lo, const Vec3& hi) {
+  if (!dev || structureChanged) throw std::runtime_error("no simulation state on the device yet (run a step first)");
+  std::array<double, 3> out{};
+  check(wdg_observables(dev->ctx, lo.data(), hi.data(), out.data()));
+  return out;
+}
+
+// ------------------------------------------------------------------ engines
+double DynDt::critDt(Scene& S) {
+  auto f = S.dem();
+  if (!f || !f->dev || f->structureChanged) return Inf;
+  double d = Inf;
+  check(wdg_crit_dt(f->dev->ctx, &d));
+  return d;
+}
+
+// ------------------------------------------------------------------ Scene
+std::shared_ptr<DemField> Scene::dem() {
+  for (auto& f : fields)
+    if (f) return f;
+  return nullptr;
+}
+
+// Scene::doOneStep (Scene.cpp:380-398): dt = dtSafety * min(critDt of engines and fields)
+void Scene::autoDt(DemField& f) {
+  double d = Inf;
+  for (auto& e : engines)
+    if (e && !e->dead) d = std::min(d, e->critDt(*this));
+  d = std::min(d, f.pWaveDt());
+  if (std::isinf(d)) throw std::runtime_error("Failed to obtain meaningful dt from engines and fields automatically.");
+  dt = dtSafety * d;
+}
+
+template <class T>
+static T* find_functor(const std::vector<std::shared_ptr<T>>& v, const char* name) {
+  for (auto& f : v)
+    if (f && std::string(f->className()) == name) return f.get();
+  return nullptr;
+}
+
+wdg_program Scene::program(bool& hostEngines) {
+  wdg_program p{};
+  hostEngines = false;
+  auto f = dem();
+  bool anyWall = false, anySphere = false, allHertz = true;
+  for (auto& q : f->particles) {
+    if (dynamic_cast<Wall*>(q->shape.get())) anyWall = true;
+    else anySphere = true;
+    if (!dynamic_cast<HertzMat*>(q->material.get())) allHertz = false;
+  }
+  for (auto& e : engines) {
+    if (!e || e->dead) continue;
+    if (p.nEngines >= 8) throw std::runtime_error("at most 8 engines are supported");
+    if (auto* c = dynamic_cast<InsertionSortCollider*>(e.get())) {
+      if (anySphere && !find_functor(c->boundDispatcher, "Bo1_Sphere_Aabb"))
+        throw std::runtime_error("InsertionSortCollider: spheres need Bo1_Sphere_Aabb");
+      if (anyWall && !find_functor(c->boundDispatcher, "Bo1_Wall_Aabb"))
+        throw std::runtime_error("InsertionSortCollider: walls need Bo1_Wall_Aabb");
+      p.collide.verletDist = c->verletDist;
+      p.collide.forceInit = c->forceInitSort ? 1 : 0;
+      p.engines[p.nEngines++] = WDG_E_COLLIDER;
+    } else if (auto* cl = dynamic_cast<ContactLoop*>(e.get())) {
+      if (anySphere && !find_functor(cl->geoDisp, "Cg2_Sphere_Sphere_L6Geom"))
+        throw std::runtime_error("ContactLoop: spheres need Cg2_Sphere_Sphere_L6Geom");
+      if (anyWall && !find_functor(cl->geoDisp, "Cg2_Wall_Sphere_L6Geom"))
+        throw std::runtime_error("ContactLoop: walls need Cg2_Wall_Sphere_L6Geom");
+      p.contact.trsfRenorm = cl->geoDisp.empty() ? 100 : cl->geoDisp[0]->trsfRenorm;
+      p.contact.applyForces = cl->applyForces ? 1 : 0;
+      // CPhys dispatch by material type: HertzMat pairs get HertzPhys when its functor is
+      // present, FrictMat pairs FrictPhys; the law follows the phys type
+      auto* hz = static_cast<Cp2_HertzMat_HertzPhys*>(find_functor(cl->phyDisp, "Cp2_HertzMat_HertzPhys"));
+      auto* fr = find_functor(cl->phyDisp, "Cp2_FrictMat_FrictPhys");
+      auto* dmt = static_cast<Law2_L6Geom_HertzPhys_DMT*>(find_functor(cl->lawDisp, "Law2_L6Geom_HertzPhys_DMT"));
+      auto* iep =
+          static_cast<Law2_L6Geom_FrictPhys_IdealElPl*>(find_functor(cl->lawDisp, "Law2_L6Geom_FrictPhys_IdealElPl"));
+      if (hz && allHertz && dmt) {
+        p.contact.physKind = WDG_PHYS_HERTZ;
+        p.contact.lawKind = WDG_LAW_HERTZDMT;
+        p.contact.poisson = hz->poisson;
+        p.contact.en = hz->en;
+        p.contact.noAttraction = dmt->noAttraction ? 1 : 0;
+      } else if (fr && iep) {
+        p.contact.physKind = WDG_PHYS_FRICT;
+        p.contact.lawKind = WDG_LAW_IDEALELPL;
+        p.contact.poisson = .2;
+        p.contact.en = NaN;
+        p.contact.noSlip = iep->noSlip;
+        p.contact.noBreak = iep->noBreak;
+        p.contact.noFrict = iep->noFrict;
+      } else {
+        throw std::runtime_error("ContactLoop: unsupported functor combination (supported: Cp2_FrictMat_FrictPhys + "
+                                 "Law2_L6Geom_FrictPhys_IdealElPl, Cp2_HertzMat_HertzPhys + Law2_L6Geom_HertzPhys_DMT "
+                                 "with HertzMat particles)");
+      }
+      p.engines[p.nEngines++] = WDG_E_CONTACTLOOP;
+    } else if (auto* lf = dynamic_cast<Leapfrog*>(e.get())) {
+      p.leapfrog.damping = lf->damping;
+      p.leapfrog.reset = lf->reset ? 1 : 0;
+      p.engines[p.nEngines++] = WDG_E_LEAPFROG;
+    } else if (dynamic_cast<ForceResetter*>(e.get())) {
+      p.engines[p.nEngines++] = WDG_E_FORCERESETTER;
+    } else if (auto* dd = dynamic_cast<DynDt*>(e.get())) {
+      p.dynDtPeriod = (int32_t)dd->stepPeriod;
+      p.maxRelInc = dd->maxRelInc;
+      p.engines[p.nEngines++] = WDG_E_DYNDT;
+    } else if (dynamic_cast<PyRunner*>(e.get())) {
+      hostEngines = true;
+    } else {
+      throw std::runtime_error(std::string("engine ") + e->className() + " is not supported by the device engine");
+    }
+  }
+  p.dtSafety = dtSafety;
+  return p;
+}
+
+void Scene::run(int64_t steps) {
+  auto f = dem();
+  if (!f) throw std::runtime_error("Scene has no DemField");
+  if (steps < 0) throw std::invalid_argument("the number of steps must be given (run(steps))");
+  stopRequested = false;
+  f->scene = this;
+  f->push(*this);
+  if (std::isnan(dt) || std::isinf(dt) || dt < 0) autoDt(*f);
+  bool hostEngines = false;
+  const wdg_program prog = program(hostEngines);
+  wdg_ctx* ctx = f->dev->ctx;
+  bool hasDyn = false;
+  for (int i = 0; i < prog.nEngines; i++) hasDyn = hasDyn || prog.engines[i] == WDG_E_DYNDT;
+  f->contactsValid = false;
+  if (!hostEngines && !hasDyn) {
+    // one call for all steps; time accumulates step by step as in doOneStep
+    int64_t st = step;
+    double d = dt;
+    f->deviceNewer = true;
+    check(wdg_run(ctx, &prog, steps, &st, &d));
+    for (int64_t s = 0; s < steps; s++) time += dt;
+    step = st;
+    return;
+  }
+  for (int64_t s = 0; s < steps && !stopRequested; s++) {
+    if (!hostEngines) {
+      const double d0 = dt;
+      f->deviceNewer = true;
+      check(wdg_run(ctx, &prog, 1, &step, &dt));  // applies DynDt's nextDt at the end of the step
+      time += d0;
+      continue;
+    }
+    // engine by engine: device engines as one-engine programs, host engines in between
+    nextDt = NaN;
+    int di = 0;
+    for (auto& e : engines) {
+      if (!e || e->dead) continue;
+      if (auto* pr = dynamic_cast<PyRunner*>(e.get())) {
+        if (pr->isActivated(step) && pr->command) {
+          pr->command();
+          f->push(*this);
+          f->contactsValid = false;
+        }
+        continue;
+      }
+      wdg_program one = prog;
+      one.nEngines = 1;
+      one.engines[0] = prog.engines[di++];
+      int64_t st = step;
+      double d = dt;
+      f->deviceNewer = true;
+      check(wdg_run(ctx, &one, 1, &st, &d));
+      if (one.engines[0] == WDG_E_DYNDT && d != dt) nextDt = d;
+    }
+    step++;
+    time += dt;
+    if (!std::isnan(nextDt)) {
+      dt = nextDt;
+      nextDt = NaN;
+    }
+  }
+}
+
+}  // namespace woo

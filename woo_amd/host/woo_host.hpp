@@ -1,0 +1,282 @@
+// Host side of the DEM step engine: Woo's scene objects as C++ classes.
+//
+// Scene, DemField, Particle/Node/DemData, the materials, the engines and the functors
+// keep the reference's class and attribute names (src/core/Scene.hpp,
+// src/dem/Particle.hpp:189-392, src/dem/FrictMat.hpp, src/dem/Hertz.hpp,
+// src/dem/Collision.hpp, src/dem/ContactLoop.hpp, src/dem/Leapfrog.hpp,
+// src/dem/DynDt.hpp). Engines run on the device through the C ABI
+// (include/woo_amd.h); functors are descriptors that select the fused kernel variant
+// and its parameters. Particle state lives on the host between runs and is moved
+// lazily: host edits are uploaded before the next step, device results are
+// downloaded on the first read after a run.
+#pragma once
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <functional>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/woo_amd.h"
+
+namespace woo {
+
+using Vec3 = std::array<double, 3>;
+using Quat = std::array<double, 4>;  // w, x, y, z
+constexpr double NaN = std::numeric_limits<double>::quiet_NaN();
+constexpr double Inf = std::numeric_limits<double>::infinity();
+
+struct Object {
+  virtual ~Object() = default;
+  std::string label;
+  virtual const char* className() const = 0;
+};
+
+// ------------------------------------------------------------------ materials (FrictMat.hpp, Hertz.hpp)
+struct Material : Object {
+  double density = 1000;
+  const char* className() const override { return "Material"; }
+};
+struct ElastMat : Material {
+  double young = 1e9;
+  const char* className() const override { return "ElastMat"; }
+};
+struct FrictMat : ElastMat {
+  double tanPhi = .5, ktDivKn = .2;
+  const char* className() const override { return "FrictMat"; }
+};
+struct HertzMat : FrictMat {
+  double surfEnergy = 0, alpha = 0;
+  const char* className() const override { return "HertzMat"; }
+};
+
+// ------------------------------------------------------------------ shapes (Sphere.hpp, Wall.hpp)
+struct Node;
+struct Shape : Object {
+  std::vector<std::shared_ptr<Node>> nodes;  // Shape.nodes: one node for spheres and walls
+  const char* className() const override { return "Shape"; }
+};
+struct Sphere : Shape {
+  double radius = NaN;
+  const char* className() const override { return "Sphere"; }
+};
+struct Wall : Shape {
+  int axis = 0, sense = 0;
+  const char* className() const override { return "Wall"; }
+};
+
+struct DemField;
+
+// ------------------------------------------------------------------ DemData, Node, Particle (Particle.hpp)
+enum : uint32_t {
+  DOF_X = 1, DOF_Y = 2, DOF_Z = 4, DOF_RX = 8, DOF_RY = 16, DOF_RZ = 32, DOF_ALL = 63,
+  FLAG_GRAVITY_SKIP = 512, FLAG_DAMPING_SKIP = 2048
+};
+struct DemData {
+  Vec3 vel{}, angVel{}, force{}, torque{}, inertia{};
+  double mass = 0;
+  uint32_t flags = 0;
+  // DemData::guessMoving (Particle.cpp:638-640)
+  bool guessMoving() const {
+    const bool blockedAll = (flags & DOF_ALL) == DOF_ALL;
+    const bool v = vel[0] != 0 || vel[1] != 0 || vel[2] != 0, w = angVel[0] != 0 || angVel[1] != 0 || angVel[2] != 0;
+    return (mass != 0. && !blockedAll) || v || w;
+  }
+  std::string blocked() const;
+  void setBlocked(const std::string& s);
+};
+struct Node : Object {
+  Vec3 pos{};
+  Quat ori{1, 0, 0, 0};
+  DemData dem;
+  DemField* field = nullptr;  // set when its particle is added
+  bool integrated = false;    // the node is in DemField.nodes
+  const char* className() const override { return "Node"; }
+};
+struct Particle : Object {
+  int64_t id = -1;
+  std::shared_ptr<Shape> shape;
+  std::shared_ptr<Material> material;
+  uint32_t mask = 0b0101;
+  // the particle's single node (uninodal particles only, like Particle.pos in the reference)
+  Node& node() const {
+    if (!shape || shape->nodes.size() != 1 || !shape->nodes[0])
+      throw std::runtime_error("particle has no shape with exactly one node");
+    return *shape->nodes[0];
+  }
+  const char* className() const override { return "Particle"; }
+};
+
+// ------------------------------------------------------------------ functors (descriptors)
+struct Functor : Object {
+  const char* className() const override { return "Functor"; }
+};
+struct BoundFunctor : Functor {};
+struct Bo1_Sphere_Aabb : BoundFunctor {
+  const char* className() const override { return "Bo1_Sphere_Aabb"; }
+};
+struct Bo1_Wall_Aabb : BoundFunctor {
+  const char* className() const override { return "Bo1_Wall_Aabb"; }
+};
+struct CGeomFunctor : Functor {
+  int trsfRenorm = 100;  // Cg2_Any_Any_L6Geom__Base.trsfRenorm (L6Geom.hpp:37)
+};
+struct Cg2_Sphere_Sphere_L6Geom : CGeomFunctor {
+  const char* className() const override { return "Cg2_Sphere_Sphere_L6Geom"; }
+};
+struct Cg2_Wall_Sphere_L6Geom : CGeomFunctor {
+  const char* className() const override { return "Cg2_Wall_Sphere_L6Geom"; }
+};
+struct CPhysFunctor : Functor {};
+struct Cp2_FrictMat_FrictPhys : CPhysFunctor {
+  const char* className() const override { return "Cp2_FrictMat_FrictPhys"; }
+};
+struct Cp2_HertzMat_HertzPhys : CPhysFunctor {
+  double poisson = .2, en = NaN;  // Hertz.hpp:58-59
+  const char* className() const override { return "Cp2_HertzMat_HertzPhys"; }
+};
+struct LawFunctor : Functor {};
+struct Law2_L6Geom_FrictPhys_IdealElPl : LawFunctor {
+  bool noSlip = false, noBreak = false, noFrict = false;  // IdealElPl.hpp:22-24
+  const char* className() const override { return "Law2_L6Geom_FrictPhys_IdealElPl"; }
+};
+struct Law2_L6Geom_HertzPhys_DMT : LawFunctor {
+  bool noAttraction = true;  // Hertz.hpp:96
+  const char* className() const override { return "Law2_L6Geom_HertzPhys_DMT"; }
+};
+
+// ------------------------------------------------------------------ engines
+struct Scene;
+struct Engine : Object {
+  bool dead = false;
+  const char* className() const override { return "Engine"; }
+  virtual double critDt(Scene&) { return Inf; }
+};
+struct PeriodicEngine : Engine {
+  int64_t stepPeriod = 1;  // PeriodicEngine.stepPeriod (only steps are supported)
+  int64_t stepLast = -1, nDone = 0;
+  // PeriodicEngine::isActivated (Engine.cpp:170-196) with stepModulo and initRun (both true)
+  bool isActivated(int64_t stepNow) {
+    if (stepLast < 0) stepLast = stepNow;
+    if (stepPeriod > 0 && (stepNow - stepLast >= stepPeriod || stepNow % stepPeriod == 0)) {
+      stepLast = stepNow;
+      nDone++;
+      return true;
+    }
+    return false;
+  }
+};
+struct InsertionSortCollider : Engine {
+  std::vector<std::shared_ptr<BoundFunctor>> boundDispatcher;
+  double verletDist = -.05;  // Collision.hpp:94
+  bool forceInitSort = false;
+  const char* className() const override { return "InsertionSortCollider"; }
+};
+struct ContactLoop : Engine {
+  std::vector<std::shared_ptr<CGeomFunctor>> geoDisp;
+  std::vector<std::shared_ptr<CPhysFunctor>> phyDisp;
+  std::vector<std::shared_ptr<LawFunctor>> lawDisp;
+  bool applyForces = true;  // ContactLoop.hpp:118
+  const char* className() const override { return "ContactLoop"; }
+};
+struct Leapfrog : Engine {
+  double damping = .2;  // Leapfrog.hpp:50-51
+  bool reset = false;
+  const char* className() const override { return "Leapfrog"; }
+};
+struct ForceResetter : Engine {
+  const char* className() const override { return "ForceResetter"; }
+};
+struct DynDt : PeriodicEngine {
+  double maxRelInc = 1e-4;  // DynDt.hpp:21
+  DynDt() { stepPeriod = 100; }
+  const char* className() const override { return "DynDt"; }
+  double critDt(Scene& S) override;
+};
+// runs host code between device steps (the role of woo.core.PyRunner)
+struct PyRunner : PeriodicEngine {
+  std::function<void()> command;
+  std::string source;  // for display
+  const char* className() const override { return "PyRunner"; }
+};
+
+// ------------------------------------------------------------------ contacts (read-only view)
+struct ContactView {
+  int64_t id1 = -1, id2 = -1;  // pA, pB user ids
+  bool real = false;
+  std::array<int, 3> cellDist{};
+  // L6Geom: trsf rows, contact point, uN; FrictPhys: local force, kn, kt
+  std::array<double, 9> trsf{};
+  Vec3 contPt{}, force{};
+  double uN = NaN, kn = NaN, kt = NaN;
+};
+
+// ------------------------------------------------------------------ fields
+struct Cell : Object {
+  Vec3 size{1, 1, 1};  // diagonal of hSize (gradV is zero)
+  const char* className() const override { return "Cell"; }
+};
+
+struct DeviceState;
+struct DemField : Object {
+  Vec3 gravity{0, 0, 0};
+  double distFactor = -1;
+  uint32_t loneMask = 0b0010;
+  std::vector<std::shared_ptr<Particle>> particles;
+  Scene* scene = nullptr;
+  std::shared_ptr<DeviceState> dev;
+  // host/device coherence
+  bool structureChanged = true;  // particles, shapes, materials, masks or flags changed: full upload
+  bool hostDirty = false;        // kinematic state edited on the host: upload fields
+  bool deviceNewer = false;      // device ran steps since the last download
+  bool contactsValid = false;
+  std::vector<ContactView> contactCache;
+  const char* className() const override { return "DemField"; }
+
+  int64_t add(const std::shared_ptr<Particle>& p, int nodes);
+  void pull();                   // device -> host, if the device is newer
+  void push(Scene& S);           // host -> device, if the host changed
+  void touch() { pull(); hostDirty = true; contactsValid = false; }
+  void restructure() { pull(); structureChanged = true; contactsValid = false; }
+  const std::vector<ContactView>& contacts();
+  int64_t countReal();
+  double pWaveDt() const;        // DemFuncs::pWaveDt (Funcs.cpp:151-175)
+  std::array<double, 3> observables(const Vec3& lo, const Vec3& hi);
+};
+
+struct Scene : Object {
+  std::vector<std::shared_ptr<DemField>> fields;
+  std::vector<std::shared_ptr<Engine>> engines;
+  double dt = NaN, nextDt = NaN, dtSafety = .9, time = 0;  // Scene.hpp:191-193
+  int64_t step = 0;
+  bool periodic = false;
+  std::shared_ptr<Cell> cell = std::make_shared<Cell>();
+  bool stopRequested = false;
+  int device = 0;
+  const char* className() const override { return "Scene"; }
+
+  std::shared_ptr<DemField> dem();
+  void run(int64_t steps);
+  void one() { run(1); }
+  void stop() { stopRequested = true; }
+
+ private:
+  void autoDt(DemField& f);
+  wdg_program program(bool& hostEngines);
+};
+
+// owns the device context of a DemField
+struct DeviceState {
+  wdg_ctx* ctx = nullptr;
+  explicit DeviceState(int device);
+  ~DeviceState();
+  DeviceState(const DeviceState&) = delete;
+  DeviceState& operator=(const DeviceState&) = delete;
+};
+
+void check(int rc);  // throws std::runtime_error with wdg_last_error() on failure
+
+}  // namespace woo

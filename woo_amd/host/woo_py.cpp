@@ -1,0 +1,492 @@
+// Python bindings of the host classes (module woo_amd._woo). Class, attribute and
+// method names follow the reference's Python surface (py/_cxxInternal.cpp and the
+// WOO_CLASS_BASE_DOC_ATTRS blocks of src/core/Scene.hpp, src/dem/Particle.hpp,
+// src/dem/ParticleContainer.hpp:182-184, src/dem/ContactContainer.hpp): objects take
+// their attributes as keyword arguments, vectors are returned as numpy arrays.
+#include <pybind11/eval.h>
+#include <pybind11/functional.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "woo_host.hpp"
+
+namespace py = pybind11;
+using namespace woo;
+
+namespace {
+
+py::array_t<double> arr(const double* p, size_t n) {
+  py::array_t<double> a(n);
+  std::copy(p, p + n, a.mutable_data());
+  return a;
+}
+Vec3 vec3(py::handle h) {
+  auto v = h.cast<std::vector<double>>();
+  if (v.size() != 3) throw py::value_error("expected 3 components");
+  return {v[0], v[1], v[2]};
+}
+Quat quat(py::handle h) {
+  auto v = h.cast<std::vector<double>>();
+  if (v.size() != 4) throw py::value_error("expected 4 components (w, x, y, z)");
+  return {v[0], v[1], v[2], v[3]};
+}
+
+// keyword arguments become attributes, as in the reference's Object constructors
+template <class T>
+std::shared_ptr<T> make_kw(const py::kwargs& kw) {
+  auto o = std::make_shared<T>();
+  py::object po = py::cast(o);
+  for (auto item : kw) py::setattr(po, item.first, item.second);
+  return o;
+}
+
+// state access through the node's field keeps host and device coherent
+void before_read(Node& n) {
+  if (n.field) n.field->pull();
+}
+void before_write(Node& n, bool structural) {
+  if (!n.field) return;
+  if (structural) n.field->restructure();
+  else n.field->touch();
+}
+
+#define NODE_VEC(cls, name, expr, structural)                                                   \
+  def_property(                                                                                 \
+      name, [](cls& self) { Node& n = node_of(self); before_read(n); return arr((expr).data(), (expr).size()); }, \
+      [](cls& self, py::handle v) { Node& n = node_of(self); before_write(n, structural); auto x = vec3(v); std::copy(x.begin(), x.end(), (expr).begin()); })
+
+struct DemDataRef {
+  std::shared_ptr<Node> node;
+};
+Node& node_of(DemDataRef& r) { return *r.node; }
+Node& node_of(Node& n) { return n; }
+Node& node_of(Particle& p) { return p.node(); }
+
+struct ParRef {
+  std::shared_ptr<DemField> f;
+};
+struct ConRef {
+  std::shared_ptr<DemField> f;
+};
+struct ContactRef {
+  ContactView c;
+};
+struct L6GeomRef {
+  ContactView c;
+};
+struct PhysRef {
+  ContactView c;
+};
+
+py::object contact_or_none(const std::vector<ContactView>& v, int64_t a, int64_t b) {
+  for (const auto& c : v)
+    if ((c.id1 == a && c.id2 == b) || (c.id1 == b && c.id2 == a)) return py::cast(ContactRef{c});
+  return py::none();
+}
+
+// S.lab: labelled engines and functors (LabelMapper)
+py::object labels(Scene& S) {
+  py::object ns = py::module_::import("types").attr("SimpleNamespace")();
+  auto put = [&](const std::shared_ptr<Object>& o) {
+    if (o && !o->label.empty()) py::setattr(ns, o->label.c_str(), py::cast(o));
+  };
+  for (auto& e : S.engines) {
+    put(e);
+    if (auto* c = dynamic_cast<InsertionSortCollider*>(e.get()))
+      for (auto& f : c->boundDispatcher) put(f);
+    if (auto* c = dynamic_cast<ContactLoop*>(e.get())) {
+      for (auto& f : c->geoDisp) put(f);
+      for (auto& f : c->phyDisp) put(f);
+      for (auto& f : c->lawDisp) put(f);
+    }
+  }
+  return ns;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_woo, m) {
+  m.doc() = "Host classes of the MI355X DEM step engine (Woo's Scene/DemField/engine surface)";
+
+  py::class_<Object, std::shared_ptr<Object>>(m, "Object", py::dynamic_attr())
+      .def_readwrite("label", &Object::label)
+      .def("__repr__", [](Object& o) {
+        return "<" + std::string(o.className()) + (o.label.empty() ? "" : " @" + o.label) + ">";
+      });
+
+  // ---------------------------------------------------------------- materials
+  py::class_<Material, Object, std::shared_ptr<Material>>(m, "Material", py::dynamic_attr())
+      .def(py::init(&make_kw<Material>))
+      .def_readwrite("density", &Material::density);
+  py::class_<ElastMat, Material, std::shared_ptr<ElastMat>>(m, "ElastMat", py::dynamic_attr())
+      .def(py::init(&make_kw<ElastMat>))
+      .def_readwrite("young", &ElastMat::young);
+  py::class_<FrictMat, ElastMat, std::shared_ptr<FrictMat>>(m, "FrictMat", py::dynamic_attr())
+      .def(py::init(&make_kw<FrictMat>))
+      .def_readwrite("tanPhi", &FrictMat::tanPhi)
+      .def_readwrite("ktDivKn", &FrictMat::ktDivKn);
+  py::class_<HertzMat, FrictMat, std::shared_ptr<HertzMat>>(m, "HertzMat", py::dynamic_attr())
+      .def(py::init(&make_kw<HertzMat>))
+      .def_readwrite("surfEnergy", &HertzMat::surfEnergy)
+      .def_readwrite("alpha", &HertzMat::alpha);
+
+  // ---------------------------------------------------------------- nodes and DemData
+  py::class_<DemDataRef>(m, "DemData")
+      .NODE_VEC(DemDataRef, "vel", n.dem.vel, false)
+      .NODE_VEC(DemDataRef, "angVel", n.dem.angVel, false)
+      .NODE_VEC(DemDataRef, "force", n.dem.force, false)
+      .NODE_VEC(DemDataRef, "torque", n.dem.torque, false)
+      .NODE_VEC(DemDataRef, "inertia", n.dem.inertia, false)
+      .def_property(
+          "mass", [](DemDataRef& r) { before_read(*r.node); return r.node->dem.mass; },
+          [](DemDataRef& r, double v) { before_write(*r.node, false); r.node->dem.mass = v; })
+      .def_property(
+          "blocked", [](DemDataRef& r) { return r.node->dem.blocked(); },
+          [](DemDataRef& r, const std::string& s) { before_write(*r.node, true); r.node->dem.setBlocked(s); })
+      .def_property(
+          "flags", [](DemDataRef& r) { return r.node->dem.flags; },
+          [](DemDataRef& r, uint32_t f) { before_write(*r.node, true); r.node->dem.flags = f; })
+      .def_property(
+          "gravitySkip", [](DemDataRef& r) { return (r.node->dem.flags & FLAG_GRAVITY_SKIP) != 0; },
+          [](DemDataRef& r, bool b) {
+            before_write(*r.node, true);
+            r.node->dem.flags = b ? (r.node->dem.flags | FLAG_GRAVITY_SKIP) : (r.node->dem.flags & ~FLAG_GRAVITY_SKIP);
+          })
+      .def_property(
+          "dampingSkip", [](DemDataRef& r) { return (r.node->dem.flags & FLAG_DAMPING_SKIP) != 0; },
+          [](DemDataRef& r, bool b) {
+            before_write(*r.node, true);
+            r.node->dem.flags = b ? (r.node->dem.flags | FLAG_DAMPING_SKIP) : (r.node->dem.flags & ~FLAG_DAMPING_SKIP);
+          })
+      .def("guessMoving", [](DemDataRef& r) { before_read(*r.node); return r.node->dem.guessMoving(); });
+
+  py::class_<Node, Object, std::shared_ptr<Node>>(m, "Node", py::dynamic_attr())
+      .def(py::init(&make_kw<Node>))
+      .NODE_VEC(Node, "pos", n.pos, false)
+      .def_property(
+          "ori", [](Node& n) { before_read(n); return arr(n.ori.data(), 4); },
+          [](Node& n, py::handle v) { before_write(n, false); n.ori = quat(v); })
+      .def_property_readonly("dem", [](std::shared_ptr<Node> n) { return DemDataRef{n}; });
+
+  // ---------------------------------------------------------------- shapes
+  py::class_<Shape, Object, std::shared_ptr<Shape>>(m, "Shape", py::dynamic_attr())
+      .def_readwrite("nodes", &Shape::nodes);
+  py::class_<Sphere, Shape, std::shared_ptr<Sphere>>(m, "Sphere", py::dynamic_attr())
+      .def(py::init(&make_kw<Sphere>))
+      .def_property(
+          "radius", [](Sphere& s) { return s.radius; },
+          [](Sphere& s, double r) {
+            if (!s.nodes.empty() && s.nodes[0] && s.nodes[0]->field) s.nodes[0]->field->restructure();
+            s.radius = r;
+          });
+  py::class_<Wall, Shape, std::shared_ptr<Wall>>(m, "Wall", py::dynamic_attr())
+      .def(py::init(&make_kw<Wall>))
+      .def_readwrite("axis", &Wall::axis)
+      .def_readwrite("sense", &Wall::sense);
+
+  // ---------------------------------------------------------------- particles
+  py::class_<Particle, Object, std::shared_ptr<Particle>>(m, "Particle", py::dynamic_attr())
+      .def(py::init(&make_kw<Particle>))
+      .def_readonly("id", &Particle::id)
+      .def_readwrite("shape", &Particle::shape)
+      .def_property(
+          "material", [](Particle& p) { return p.material; },
+          [](Particle& p, std::shared_ptr<Material> m) {
+            if (p.id >= 0) p.node().field->restructure();
+            p.material = m;
+          })
+      .def_property(
+          "mat", [](Particle& p) { return p.material; },
+          [](Particle& p, std::shared_ptr<Material> m) {
+            if (p.id >= 0) p.node().field->restructure();
+            p.material = m;
+          })
+      .def_property(
+          "mask", [](Particle& p) { return p.mask; },
+          [](Particle& p, uint32_t v) {
+            if (p.id >= 0) p.node().field->restructure();
+            p.mask = v;
+          })
+      // Particle shorthands for the single node (Particle.hpp:100-120)
+      .NODE_VEC(Particle, "pos", n.pos, false)
+      .NODE_VEC(Particle, "vel", n.dem.vel, false)
+      .NODE_VEC(Particle, "angVel", n.dem.angVel, false)
+      .NODE_VEC(Particle, "f", n.dem.force, false)
+      .NODE_VEC(Particle, "t", n.dem.torque, false)
+      .NODE_VEC(Particle, "inertia", n.dem.inertia, false)
+      .def_property(
+          "ori", [](Particle& p) { before_read(p.node()); return arr(p.node().ori.data(), 4); },
+          [](Particle& p, py::handle v) { before_write(p.node(), false); p.node().ori = quat(v); })
+      .def_property(
+          "mass", [](Particle& p) { before_read(p.node()); return p.node().dem.mass; },
+          [](Particle& p, double v) { before_write(p.node(), false); p.node().dem.mass = v; })
+      .def_property(
+          "blocked", [](Particle& p) { return p.node().dem.blocked(); },
+          [](Particle& p, const std::string& s) { before_write(p.node(), true); p.node().dem.setBlocked(s); })
+      .def_property_readonly("Ek", [](Particle& p) {
+        Node& n = p.node();
+        before_read(n);
+        double e = 0;
+        for (int k = 0; k < 3; k++) e += .5 * n.dem.mass * n.dem.vel[k] * n.dem.vel[k];
+        for (int k = 0; k < 3; k++) e += .5 * n.dem.inertia[k] * n.dem.angVel[k] * n.dem.angVel[k];
+        return e;
+      });
+
+  // ---------------------------------------------------------------- contacts
+  py::class_<L6GeomRef>(m, "L6Geom")
+      .def_property_readonly("uN", [](L6GeomRef& g) { return g.c.uN; })
+      .def_property_readonly("trsf", [](L6GeomRef& g) {
+        py::array_t<double> a({3, 3});
+        std::copy(g.c.trsf.begin(), g.c.trsf.end(), a.mutable_data());
+        return a;
+      })
+      .def_property_readonly("contPt", [](L6GeomRef& g) { return arr(g.c.contPt.data(), 3); })
+      .def_property_readonly("node", [](L6GeomRef& g) {
+        auto n = std::make_shared<Node>();
+        n->pos = g.c.contPt;
+        return n;
+      });
+  py::class_<PhysRef>(m, "FrictPhys")
+      .def_property_readonly("force", [](PhysRef& p) { return arr(p.c.force.data(), 3); })
+      .def_property_readonly("kn", [](PhysRef& p) { return p.c.kn; })
+      .def_property_readonly("kt", [](PhysRef& p) { return p.c.kt; });
+  py::class_<ContactRef>(m, "Contact")
+      .def_property_readonly("id1", [](ContactRef& c) { return c.c.id1; })
+      .def_property_readonly("id2", [](ContactRef& c) { return c.c.id2; })
+      .def_property_readonly("real", [](ContactRef& c) { return c.c.real; })
+      .def_property_readonly("cellDist", [](ContactRef& c) { return c.c.cellDist; })
+      .def_property_readonly("geom", [](ContactRef& c) -> py::object {
+        return c.c.real ? py::cast(L6GeomRef{c.c}) : py::none();
+      })
+      .def_property_readonly("phys", [](ContactRef& c) -> py::object {
+        return c.c.real ? py::cast(PhysRef{c.c}) : py::none();
+      })
+      .def("__repr__", [](ContactRef& c) {
+        return "<Contact ##" + std::to_string(c.c.id1) + "+" + std::to_string(c.c.id2) + (c.c.real ? "" : " (potential)") + ">";
+      });
+
+  py::class_<ConRef>(m, "ContactContainer")
+      .def("__len__", [](ConRef& r) { return r.f->contacts().size(); })
+      .def("countReal", [](ConRef& r) { return r.f->countReal(); })
+      .def("__getitem__", [](ConRef& r, int64_t i) {
+        const auto& v = r.f->contacts();
+        if (i < 0) i += (int64_t)v.size();
+        if (i < 0 || i >= (int64_t)v.size()) throw py::index_error("contact index out of range");
+        return ContactRef{v[(size_t)i]};
+      })
+      .def("__getitem__", [](ConRef& r, std::pair<int64_t, int64_t> ij) {
+        py::object c = contact_or_none(r.f->contacts(), ij.first, ij.second);
+        if (c.is_none()) throw py::index_error("no contact between the particles");
+        return c;
+      })
+      .def("exists", [](ConRef& r, int64_t a, int64_t b) { return !contact_or_none(r.f->contacts(), a, b).is_none(); })
+      .def("existsReal", [](ConRef& r, int64_t a, int64_t b) {
+        py::object c = contact_or_none(r.f->contacts(), a, b);
+        return !c.is_none() && c.cast<ContactRef>().c.real;
+      })
+      .def("__iter__", [](ConRef& r) {
+        py::list l;
+        for (const auto& c : r.f->contacts()) l.append(ContactRef{c});
+        return py::iter(l);
+      });
+
+  py::class_<ParRef>(m, "ParticleContainer")
+      .def("add", [](ParRef& r, std::shared_ptr<Particle> p, int nodes) { return r.f->add(p, nodes); },
+           py::arg("par"), py::arg("nodes") = -1)
+      .def("add",
+           [](ParRef& r, std::vector<std::shared_ptr<Particle>> pp, int nodes) {
+             py::list ids;
+             for (auto& p : pp) ids.append(r.f->add(p, nodes));
+             return ids;
+           },
+           py::arg("pars"), py::arg("nodes") = -1)
+      .def("__len__", [](ParRef& r) { return r.f->particles.size(); })
+      .def("__getitem__", [](ParRef& r, int64_t i) {
+        const int64_t n = (int64_t)r.f->particles.size();
+        if (i < 0) i += n;
+        if (i < 0 || i >= n) throw py::index_error("particle id out of range");
+        return r.f->particles[(size_t)i];
+      })
+      .def("__iter__", [](ParRef& r) { return py::iter(py::cast(r.f->particles)); });
+
+  // ---------------------------------------------------------------- fields
+  py::class_<Cell, Object, std::shared_ptr<Cell>>(m, "Cell", py::dynamic_attr())
+      .def(py::init(&make_kw<Cell>))
+      .def_property(
+          "size", [](Cell& c) { return arr(c.size.data(), 3); }, [](Cell& c, py::handle v) { c.size = vec3(v); })
+      .def("setBox", [](Cell& c, py::handle v) { c.size = vec3(v); });
+
+  py::class_<DemField, Object, std::shared_ptr<DemField>>(m, "DemField", py::dynamic_attr())
+      .def(py::init(&make_kw<DemField>))
+      .def_property(
+          "gravity", [](DemField& f) { return arr(f.gravity.data(), 3); },
+          [](DemField& f, py::handle v) { f.gravity = vec3(v); })
+      .def_readwrite("distFactor", &DemField::distFactor)
+      .def_property(
+          "loneMask", [](DemField& f) { return f.loneMask; },
+          [](DemField& f, uint32_t v) { f.restructure(); f.loneMask = v; })
+      .def_property_readonly("par", [](std::shared_ptr<DemField> f) { return ParRef{f}; })
+      .def_property_readonly("con", [](std::shared_ptr<DemField> f) { return ConRef{f}; })
+      .def_property_readonly("nodes", [](DemField& f) {
+        py::list l;
+        for (auto& p : f.particles)
+          if (p->node().integrated) l.append(p->shape->nodes[0]);
+        return l;
+      })
+      .def("critDt", &DemField::pWaveDt)
+      .def("observables", [](DemField& f, py::handle lo, py::handle hi) { return f.observables(vec3(lo), vec3(hi)); })
+      .def_property_readonly_static("defaultMovableMask", [](py::object) { return 0b0101; })
+      .def_property_readonly_static("defaultBoundaryMask", [](py::object) { return 0b0011; })
+      .def_property_readonly_static("defaultLoneMask", [](py::object) { return 0b0010; });
+
+  // ---------------------------------------------------------------- functors
+  py::class_<Functor, Object, std::shared_ptr<Functor>>(m, "Functor", py::dynamic_attr());
+  py::class_<BoundFunctor, Functor, std::shared_ptr<BoundFunctor>>(m, "BoundFunctor", py::dynamic_attr());
+  py::class_<Bo1_Sphere_Aabb, BoundFunctor, std::shared_ptr<Bo1_Sphere_Aabb>>(m, "Bo1_Sphere_Aabb", py::dynamic_attr())
+      .def(py::init(&make_kw<Bo1_Sphere_Aabb>));
+  py::class_<Bo1_Wall_Aabb, BoundFunctor, std::shared_ptr<Bo1_Wall_Aabb>>(m, "Bo1_Wall_Aabb", py::dynamic_attr())
+      .def(py::init(&make_kw<Bo1_Wall_Aabb>));
+  py::class_<CGeomFunctor, Functor, std::shared_ptr<CGeomFunctor>>(m, "CGeomFunctor", py::dynamic_attr())
+      .def_readwrite("trsfRenorm", &CGeomFunctor::trsfRenorm);
+  py::class_<Cg2_Sphere_Sphere_L6Geom, CGeomFunctor, std::shared_ptr<Cg2_Sphere_Sphere_L6Geom>>(
+      m, "Cg2_Sphere_Sphere_L6Geom", py::dynamic_attr())
+      .def(py::init(&make_kw<Cg2_Sphere_Sphere_L6Geom>));
+  py::class_<Cg2_Wall_Sphere_L6Geom, CGeomFunctor, std::shared_ptr<Cg2_Wall_Sphere_L6Geom>>(
+      m, "Cg2_Wall_Sphere_L6Geom", py::dynamic_attr())
+      .def(py::init(&make_kw<Cg2_Wall_Sphere_L6Geom>));
+  py::class_<CPhysFunctor, Functor, std::shared_ptr<CPhysFunctor>>(m, "CPhysFunctor", py::dynamic_attr());
+  py::class_<Cp2_FrictMat_FrictPhys, CPhysFunctor, std::shared_ptr<Cp2_FrictMat_FrictPhys>>(
+      m, "Cp2_FrictMat_FrictPhys", py::dynamic_attr())
+      .def(py::init(&make_kw<Cp2_FrictMat_FrictPhys>));
+  py::class_<Cp2_HertzMat_HertzPhys, CPhysFunctor, std::shared_ptr<Cp2_HertzMat_HertzPhys>>(
+      m, "Cp2_HertzMat_HertzPhys", py::dynamic_attr())
+      .def(py::init(&make_kw<Cp2_HertzMat_HertzPhys>))
+      .def_readwrite("poisson", &Cp2_HertzMat_HertzPhys::poisson)
+      .def_readwrite("en", &Cp2_HertzMat_HertzPhys::en);
+  py::class_<LawFunctor, Functor, std::shared_ptr<LawFunctor>>(m, "LawFunctor", py::dynamic_attr());
+  py::class_<Law2_L6Geom_FrictPhys_IdealElPl, LawFunctor, std::shared_ptr<Law2_L6Geom_FrictPhys_IdealElPl>>(
+      m, "Law2_L6Geom_FrictPhys_IdealElPl", py::dynamic_attr())
+      .def(py::init(&make_kw<Law2_L6Geom_FrictPhys_IdealElPl>))
+      .def_readwrite("noSlip", &Law2_L6Geom_FrictPhys_IdealElPl::noSlip)
+      .def_readwrite("noBreak", &Law2_L6Geom_FrictPhys_IdealElPl::noBreak)
+      .def_readwrite("noFrict", &Law2_L6Geom_FrictPhys_IdealElPl::noFrict);
+  py::class_<Law2_L6Geom_HertzPhys_DMT, LawFunctor, std::shared_ptr<Law2_L6Geom_HertzPhys_DMT>>(
+      m, "Law2_L6Geom_HertzPhys_DMT", py::dynamic_attr())
+      .def(py::init(&make_kw<Law2_L6Geom_HertzPhys_DMT>))
+      .def_readwrite("noAttraction", &Law2_L6Geom_HertzPhys_DMT::noAttraction);
+
+  // ---------------------------------------------------------------- engines
+  py::class_<Engine, Object, std::shared_ptr<Engine>>(m, "Engine", py::dynamic_attr())
+      .def_readwrite("dead", &Engine::dead)
+      .def("critDt", [](Engine& e, Scene& S) { return e.critDt(S); });
+  py::class_<PeriodicEngine, Engine, std::shared_ptr<PeriodicEngine>>(m, "PeriodicEngine", py::dynamic_attr())
+      .def_readwrite("stepPeriod", &PeriodicEngine::stepPeriod)
+      .def_readwrite("stepLast", &PeriodicEngine::stepLast)
+      .def_readonly("nDone", &PeriodicEngine::nDone);
+  // InsertionSortCollider([Bo1_*], verletDist=...) (utils.py:118)
+  py::class_<InsertionSortCollider, Engine, std::shared_ptr<InsertionSortCollider>>(m, "InsertionSortCollider",
+                                                                                    py::dynamic_attr())
+      .def(py::init([](py::args a, py::kwargs kw) {
+        auto o = make_kw<InsertionSortCollider>(kw);
+        if (a.size() > 1) throw py::type_error("InsertionSortCollider takes one positional argument (bound functors)");
+        if (a.size() == 1) o->boundDispatcher = a[0].cast<std::vector<std::shared_ptr<BoundFunctor>>>();
+        return o;
+      }))
+      .def_readwrite("boundDispatcher", &InsertionSortCollider::boundDispatcher)
+      .def_readwrite("verletDist", &InsertionSortCollider::verletDist)
+      .def_readwrite("forceInitSort", &InsertionSortCollider::forceInitSort);
+  // ContactLoop([Cg2_*], [Cp2_*], [Law2_*], applyForces=...) (ContactLoop.cpp:52-66)
+  py::class_<ContactLoop, Engine, std::shared_ptr<ContactLoop>>(m, "ContactLoop", py::dynamic_attr())
+      .def(py::init([](py::args a, py::kwargs kw) {
+        auto o = make_kw<ContactLoop>(kw);
+        if (a.size() != 0 && a.size() != 3) throw py::type_error("ContactLoop takes 0 or 3 positional arguments");
+        if (a.size() == 3) {
+          o->geoDisp = a[0].cast<std::vector<std::shared_ptr<CGeomFunctor>>>();
+          o->phyDisp = a[1].cast<std::vector<std::shared_ptr<CPhysFunctor>>>();
+          o->lawDisp = a[2].cast<std::vector<std::shared_ptr<LawFunctor>>>();
+        }
+        return o;
+      }))
+      .def_readwrite("geoDisp", &ContactLoop::geoDisp)
+      .def_readwrite("phyDisp", &ContactLoop::phyDisp)
+      .def_readwrite("lawDisp", &ContactLoop::lawDisp)
+      .def_readwrite("applyForces", &ContactLoop::applyForces);
+  py::class_<Leapfrog, Engine, std::shared_ptr<Leapfrog>>(m, "Leapfrog", py::dynamic_attr())
+      .def(py::init(&make_kw<Leapfrog>))
+      .def_readwrite("damping", &Leapfrog::damping)
+      .def_readwrite("reset", &Leapfrog::reset);
+  py::class_<ForceResetter, Engine, std::shared_ptr<ForceResetter>>(m, "ForceResetter", py::dynamic_attr())
+      .def(py::init(&make_kw<ForceResetter>));
+  py::class_<DynDt, PeriodicEngine, std::shared_ptr<DynDt>>(m, "DynDt", py::dynamic_attr())
+      .def(py::init(&make_kw<DynDt>))
+      .def_readwrite("maxRelInc", &DynDt::maxRelInc);
+  // PyRunner(stepPeriod, command) or PyRunner(command=..., stepPeriod=...) (Engine.cpp:198-220)
+  py::class_<PyRunner, PeriodicEngine, std::shared_ptr<PyRunner>>(m, "PyRunner", py::dynamic_attr())
+      .def(py::init([](py::args a, py::kwargs kw) {
+        auto o = std::make_shared<PyRunner>();
+        py::object po = py::cast(o);
+        for (auto item : kw) py::setattr(po, item.first, item.second);
+        for (auto h : a) {
+          if (py::isinstance<py::int_>(h)) o->stepPeriod = h.cast<int64_t>();
+          else py::setattr(po, "command", h);
+        }
+        return o;
+      }))
+      .def_property(
+          "command", [](PyRunner& r) { return r.source; },
+          [](py::object self, py::object cmd) {
+            PyRunner& r = self.cast<PyRunner&>();
+            if (py::isinstance<py::str>(cmd)) r.source = cmd.cast<std::string>();
+            else r.source = py::str(py::repr(cmd)).cast<std::string>();
+            // the Scene binds the callable before running (it needs S in the namespace)
+            self.attr("_command") = cmd;
+          });
+
+  // ---------------------------------------------------------------- Scene
+  py::class_<Scene, Object, std::shared_ptr<Scene>>(m, "Scene", py::dynamic_attr())
+      .def(py::init(&make_kw<Scene>))
+      .def_readwrite("fields", &Scene::fields)
+      .def_readwrite("engines", &Scene::engines)
+      .def_readwrite("dt", &Scene::dt)
+      .def_readwrite("nextDt", &Scene::nextDt)
+      .def_readwrite("dtSafety", &Scene::dtSafety)
+      .def_readwrite("step", &Scene::step)
+      .def_readwrite("time", &Scene::time)
+      .def_readwrite("periodic", &Scene::periodic)
+      .def_readwrite("cell", &Scene::cell)
+      .def_readwrite("device", &Scene::device)
+      .def_property_readonly("dem", [](Scene& S) {
+        auto f = S.dem();
+        if (!f) throw py::attribute_error("Scene has no DemField");
+        return f;
+      })
+      .def_property_readonly("lab", &labels)
+      .def("run",
+           [](py::object self, int64_t steps, bool /*wait*/) {
+             Scene& S = self.cast<Scene&>();
+             // PyRunner commands run with S (and woo) in their namespace
+             for (auto& e : S.engines) {
+               if (auto pr = std::dynamic_pointer_cast<PyRunner>(e)) {
+                 py::object pe = py::cast(pr);
+                 if (!py::hasattr(pe, "_command")) continue;
+                 py::object cmd = pe.attr("_command");
+                 if (py::isinstance<py::str>(cmd)) {
+                   py::dict g;
+                   g["S"] = self;
+                   g["woo"] = py::module_::import("woo_amd.woo");
+                   g["__builtins__"] = py::module_::import("builtins");
+                   std::string src = cmd.cast<std::string>();
+                   pr->command = [g, src]() { py::exec(src, g); };
+                 } else {
+                   pr->command = [cmd]() { cmd(); };
+                 }
+               }
+             }
+             S.run(steps);
+           },
+           py::arg("steps"), py::arg("wait") = true)
+      .def("one", [](py::object self) { self.attr("run")(1); })
+      .def("wait", [](Scene&) {})
+      .def("stop", &Scene::stop);
+}
