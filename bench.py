@@ -156,6 +156,51 @@ def cpu_baseline(wl, n_sample_steps_max=20, budget_s=15.):
                        f"{el:.1f} s wall")
 
 
+def observables(dev, wl):
+    """Kinetic energy, coordination number, solid fraction of the central eighth of the
+    cell, and the stress tensor."""
+    L = wl["L"]
+    ob = dev.observables((.25 * L,) * 3, (.75 * L,) * 3)
+    return dict(kinetic_energy=dev.kinetic_energy(), coord_number=ob["coordNumber"],
+                solid_fraction=ob["solidFraction"], stress=dev.stress(-1.))
+
+
+def fp32_form(wl, args, value64, obs64):
+    """The FP32 contact form on the same initial state and step count: throughput and its
+    speedup over FP64, and each observable's relative error against the FP64 run (which
+    is bit-identical to the FP64 CPU oracle)."""
+    from woo_amd._native import Device, SPHERE
+
+    d = Device(0)
+    n = len(wl["radius"])
+    d.set_field(periodic=True, cell=(wl["L"],) * 3)
+    d.set_materials([wl["mat"]])
+    d.upload_particles(shape=np.full(n, SPHERE, np.int32), pos=wl["pos"], radius=wl["radius"], mass=wl["mass"],
+                       inertia=wl["inertia"], vel=wl["vel"])
+    d.set_precision(32)
+    cp = contact_params()
+    step, _ = d.run(ENGINES, args.warmup, 0, wl["dt"], contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
+    d.sync()
+    t0 = time.perf_counter()
+    step, _ = d.run(ENGINES, args.steps, step, wl["dt"], contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
+    d.sync()
+    el = time.perf_counter() - t0
+    v32 = n * args.steps / el
+    o32 = observables(d, wl)
+    d.close()
+    err = {}
+    for k in ("kinetic_energy", "coord_number", "solid_fraction"):
+        err[k] = dict(fp64=obs64[k], fp32=o32[k], rel_err=abs(o32[k] - obs64[k]) / abs(obs64[k]))
+    p64, p32 = np.trace(obs64["stress"]) / 3, np.trace(o32["stress"]) / 3
+    err["pressure"] = dict(fp64=p64, fp32=p32, rel_err=abs(p32 - p64) / abs(p64))
+    err["stress_max_abs_diff_over_max"] = float(np.max(np.abs(o32["stress"] - obs64["stress"])) /
+                                                np.max(np.abs(obs64["stress"])))
+    return dict(value=v32, unit="particle-steps/s", ms_per_step=el / args.steps * 1e3,
+                speedup_vs_fp64=v32 / value64, steps=args.steps, warmup=args.warmup,
+                dtype="f32 contact geometry/law, f64 positions and sums",
+                observable_errors_vs_fp64=err)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +209,7 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--kernel-steps", type=int, default=50)
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--fp32", type=int, default=1, help="also time the FP32 contact form")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
@@ -189,6 +235,12 @@ def main():
     st1 = dev.stats()
     value = args.n * args.steps / el
     log(f"timed {args.steps} steps in {el:.3f} s: {value:.4e} particle-steps/s; refreshes {st1['nRefresh'] - st0['nRefresh']}")
+
+    obs64 = observables(dev, wl)
+    fp32 = None
+    if args.fp32:
+        fp32 = fp32_form(wl, args, value, obs64)
+        log(f"fp32 form: {fp32}")
 
     # per-kernel device time on the engine's stream
     tc, tl, step = time_kernels(dev, wl, step, dt, args.kernel_steps)
@@ -222,6 +274,7 @@ def main():
                    "refreshes_in_timed_region": st1["nRefresh"] - st0["nRefresh"]},
         "roofline": roof,
         "cpu_baseline": base,
+        "fp32_form": fp32,
     }
     print(json.dumps(out), flush=True)
     dev.close()

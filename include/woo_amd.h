@@ -24,6 +24,8 @@
  *   wdg_contact_count / wdg_contact_list
  *                      ContactContainer size / linView   src/dem/ContactContainer.hpp:16-141
  *   wdg_observables    DemFuncs::coordNumber/porosity    src/dem/Funcs.cpp:84-114
+ *   wdg_stress         DemFuncs::stressStiffness (stress) src/dem/Funcs.cpp:183-229
+ *   wdg_kinetic_energy DemData kinetic energy             src/dem/Particle.cpp:247-265
  *   wdg_run            Scene::doOneStep (engine loop)    src/core/Scene.cpp:366-467
  */
 #ifndef WOO_AMD_H
@@ -167,6 +169,16 @@ int wdg_contact_list(wdg_ctx* ctx, int32_t which, int32_t* ids, int32_t* cellDis
 
 /* coordNumber (skipFree=0 and 1) and solid fraction in a box */
 int wdg_observables(wdg_ctx* ctx, const double boxMin[3], const double boxMax[3], double out[3]);
+/* Contact form: 64 = plain FP64 (the default); 32 = contact geometry, physics and law in
+ * FP32 while positions, the position/velocity update and the force sums stay FP64.
+ * No counterpart in the reference (a build-only form). Changing it restarts the contact
+ * history. */
+int wdg_set_precision(wdg_ctx* ctx, int32_t bits);
+/* Cauchy stress tensor (row-major 3x3) from the real contacts; volume <= 0 takes the
+ * periodic cell's volume (an error for aperiodic scenes) */
+int wdg_stress(wdg_ctx* ctx, double volume, double out[9]);
+/* sum over integrated nodes of m v.v/2 + w.(I w)/2 */
+int wdg_kinetic_energy(wdg_ctx* ctx, double* ek);
 int wdg_stats(wdg_ctx* ctx, wdg_counters* out);
 
 /* run nSteps of Scene::doOneStep with a fixed engine list; *step and *dt are the

@@ -1250,6 +1250,39 @@ double orc_kinetic_energy(const orc_scene* s) {
   return E;
 }
 
+// DemFuncs::stressStiffness, stress part (Funcs.cpp:183-229): over the real contacts in
+// ContactContainer order, d0 = |posA - shift - posB| with the node positions,
+// n = ori*e_x, F = ori*force, sigma_ij += d0 (fN n_i n_j + (fT_i n_j + fT_j n_i)/2),
+// divided by the volume (the periodic cell's when volume <= 0)
+int orc_stress(const orc_scene* s, double volume, double out[9]) {
+  double S[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (const auto& C : s->con) {
+    if (!C.isReal()) continue;
+    const auto& A = s->par[C.pA];
+    const auto& B = s->par[C.pB];
+    V3 posA = ld3(A.pos), posB = ld3(B.pos);
+    if (s->prm.periodic)
+      posA = sub(posA, mk(s->prm.cell[0] * C.cellDist[0], s->prm.cell[1] * C.cellDist[1], s->prm.cell[2] * C.cellDist[2]));
+    const double d0 = std::sqrt(sqn(sub(posA, posB)));
+    const V3 n = quatRot(C.nodeOri, mk(1, 0, 0));
+    const V3 F = quatRot(C.nodeOri, C.force);
+    const double fN = dot(F, n);
+    const V3 fT = sub(F, mul(n, fN));
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) S[i][j] += d0 * (fN * n[i] * n[j] + .5 * (fT[i] * n[j] + fT[j] * n[i]));
+  }
+  if (volume <= 0) {
+    if (!s->prm.periodic) {
+      g_err = "a positive volume must be given for aperiodic scenes";
+      return -1;
+    }
+    volume = s->prm.cell[0] * s->prm.cell[1] * s->prm.cell[2];
+  }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) out[3 * i + j] = S[i][j] / volume;
+  return 0;
+}
+
 // DemFuncs::pWaveDt (Funcs.cpp:151-175)
 double orc_pwave_dt(const orc_scene* s) {
   double dt = Inf;

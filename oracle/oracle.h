@@ -110,6 +110,7 @@ int64_t orc_get_contacts(const orc_scene* s, int which, int32_t* ids, int32_t* c
 double orc_coord_number(const orc_scene* s, int skipFree);
 double orc_solid_fraction(const orc_scene* s, const double boxMin[3], const double boxMax[3]);
 double orc_kinetic_energy(const orc_scene* s);
+int orc_stress(const orc_scene* s, double volume, double out[9]);
 double orc_pwave_dt(const orc_scene* s);
 /* DynDt::critDt_stiffness on the current contact state (no side effects) */
 double orc_crit_dt(const orc_scene* s);

@@ -66,6 +66,7 @@ def _load():
     L.orc_coord_number.argtypes = [P, C.c_int]; L.orc_coord_number.restype = C.c_double
     L.orc_solid_fraction.argtypes = [P, D, D]; L.orc_solid_fraction.restype = C.c_double
     L.orc_kinetic_energy.argtypes = [P]; L.orc_kinetic_energy.restype = C.c_double
+    L.orc_stress.argtypes = [P, C.c_double, D]; L.orc_stress.restype = C.c_int
     L.orc_pwave_dt.argtypes = [P]; L.orc_pwave_dt.restype = C.c_double
     L.orc_crit_dt.argtypes = [P]; L.orc_crit_dt.restype = C.c_double
     L.orc_brute_aabb_pairs.argtypes = [P, I, C.c_int64]; L.orc_brute_aabb_pairs.restype = C.c_int64
@@ -186,6 +187,13 @@ class Oracle:
 
     def kinetic_energy(self):
         return self.L.orc_kinetic_energy(self.h)
+
+    def stress(self, volume=-1.):
+        """Cauchy stress tensor (3x3) from the real contacts (DemFuncs::stressStiffness)."""
+        out = np.zeros(9)
+        if self.L.orc_stress(self.h, volume, _dp(out)) != 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
+        return out.reshape(3, 3)
 
     def pwave_dt(self):
         return self.L.orc_pwave_dt(self.h)

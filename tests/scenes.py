@@ -108,6 +108,48 @@ def periodic_hertz(n=1000, seed=2, rmin=2e-3, rmax=4e-3, rho=3000., young=1e8, t
         dt=0.3 * pwave, physKind=1, lawKind=1, damping=0., poisson=poisson, en=en, box=L)
 
 
+def periodic_bidisperse(n_side=18, r_small=1e-3, r_big=6e-3, n_big=3, seed=4, rho=3000., young=1e8, tanPhi=.4,
+                        poisson=.25, en=.6, vel_sigma=0.05):
+    """Periodic Hertz box: small spheres on a jittered lattice with a few large spheres put
+    in their place. A large sphere has well over a hundred near neighbours (the rare case of
+    the broadphase, whose per-sphere staging then overflows)."""
+    rng = np.random.default_rng(seed)
+    a = 2.02 * r_small
+    L = n_side * a
+    g = np.stack(np.meshgrid(*[np.arange(n_side)] * 3, indexing="ij"), -1).reshape(-1, 3).astype(float)
+    small = (g + 0.5) * a + rng.uniform(-0.01 * r_small, 0.01 * r_small, size=g.shape)
+    big = []
+    while len(big) < n_big:
+        c = rng.uniform(0, L, size=3)
+        if all(np.linalg.norm((c - b) - L * np.round((c - b) / L)) > 2 * r_big + 4 * a for b in big):
+            big.append(c)
+    keep = np.ones(len(small), bool)
+    shell = []
+    n_shell = 140
+    k = np.arange(n_shell) + 0.5
+    # Fibonacci points: small spheres on the large sphere's surface, slightly overlapping it
+    u = np.stack([np.cos(np.pi * (1 + 5 ** 0.5) * k) * np.sqrt(1 - (1 - 2 * k / n_shell) ** 2),
+                  np.sin(np.pi * (1 + 5 ** 0.5) * k) * np.sqrt(1 - (1 - 2 * k / n_shell) ** 2),
+                  1 - 2 * k / n_shell], -1)
+    for b in big:
+        d = small - b
+        d -= L * np.round(d / L)
+        keep &= np.linalg.norm(d, axis=1) > r_big + 3.2 * r_small
+        shell.append(b + (r_big + 0.995 * r_small) * u)
+    pos = np.concatenate([small[keep], np.concatenate(shell), np.array(big)])
+    radii = np.concatenate([np.full(int(keep.sum()) + n_shell * n_big, r_small), np.full(n_big, r_big)])
+    perm = rng.permutation(len(radii))
+    pos, radii = pos[perm], radii[perm]
+    vel = rng.normal(0, vel_sigma, size=pos.shape)
+    mass, inertia = sphere_mass_inertia(radii, rho)
+    pwave = r_small / math.sqrt(young / rho)
+    return dict(
+        materials=[dict(kind=1, density=rho, young=young, ktDivKn=.2, tanPhi=tanPhi, surfEnergy=0., alpha=0.)],
+        pos=pos, radius=radii, vel=vel, mass=mass, inertia=np.repeat(inertia[:, None], 3, 1),
+        walls=[], walls_first=False, gravity=(0., 0., 0.), periodic=True, cell=(L, L, L),
+        dt=0.3 * pwave, physKind=1, lawKind=1, damping=0., poisson=poisson, en=en, box=L)
+
+
 def _particle_order(sc):
     """list of ('s', i) / ('w', j) in id order"""
     n = len(sc["radius"])

@@ -13,7 +13,7 @@ import math
 import numpy as np
 import pytest
 
-from tests.scenes import GpuRunner, build_oracle, deposition, periodic_hertz
+from tests.scenes import GpuRunner, build_oracle, deposition, periodic_bidisperse, periodic_hertz
 
 pytestmark = pytest.mark.gpu
 
@@ -158,6 +158,21 @@ def test_trajectory_100_hertz_periodic():
     assert worst <= TRAJ_TOL
 
 
+def test_trajectory_high_degree_spheres():
+    # large spheres among small ones: more neighbours per sphere than the broadphase stages,
+    # so pair generation runs again in place and the adjacency takes the general path
+    sc = periodic_bidisperse()
+    big = np.flatnonzero(sc["radius"] > 2e-3)
+    o, g, worst = _run_both(sc, ENG, 40)
+    ids = o.contacts(real_only=False, data=False)[0]
+    deg = max(int(np.sum(np.any(ids == b, axis=1))) for b in big)
+    real = o.contacts(real_only=True, data=False)[0]
+    print("spheres", len(sc["radius"]), "candidate pairs of the busiest large sphere", deg, "real contacts",
+          len(real), "worst", worst)
+    assert deg > 100
+    assert worst <= TRAJ_TOL
+
+
 def test_minimal_engine_order_with_forceresetter():
     # DemField.minimalEngines order (utils.py:122-129): Leapfrog first, forces gathered lazily
     sc = deposition(n=700, dense=True, seed=11, vel_sigma=0.05)
@@ -212,6 +227,25 @@ def test_force_state_changes_mid_run():
     print("positions differing", n_diff, "worst", e)
     assert e <= TRAJ_TOL
     np.testing.assert_array_equal(o.get("force"), g.d.get("force"))
+
+
+def test_stress_and_kinetic_energy_match_oracle():
+    # DemFuncs::stressStiffness and the kinetic energy on the same state; the GPU sums in
+    # slot / block order, so agreement is to summation roundoff
+    for sc, vol in ((periodic_hertz(n=1500, spacing=0.92, seed=41), -1.),
+                    (deposition(n=800, dense=True, seed=42, vel_sigma=0.05), None)):
+        o = build_oracle(sc)
+        g = GpuRunner(sc)
+        o.run(ENG, 30)
+        g.run(ENG, 30)
+        v = vol if vol is not None else float(sc["box"]) ** 3
+        so, sg = o.stress(v), g.d.stress(v)
+        scale = np.max(np.abs(so))
+        print("stress", so.diagonal(), "max diff / max", np.max(np.abs(sg - so)) / scale)
+        assert scale > 0
+        np.testing.assert_allclose(sg, so, rtol=0, atol=1e-11 * scale)
+        eo, eg = o.kinetic_energy(), g.d.kinetic_energy()
+        assert eg == pytest.approx(eo, rel=1e-12)
 
 
 def test_crit_dt_matches():

@@ -199,6 +199,57 @@ def test_lattice_observables():
     assert o.solid_fraction((0, 0, 0), (L, L, L)) == pytest.approx(math.pi / 6, rel=1e-6)
 
 
+def test_stress_compressed_lattice_closed_form():
+    # simple cubic lattice, pitch a < 2r, one linear contact per sphere and axis:
+    # sigma = diag(kn uN / a^2) with kn = E pi r / 2 and uN = a - 2r (compression negative)
+    m, r, E = 5, .01, 1e7
+    a = 2 * r * (1 - 1e-4)
+    L = m * a
+    o = Oracle(periodic=1, cell=(L, L, L), physKind=0, lawKind=0, damping=0., gravity=(0, 0, 0), verletDist=0.)
+    mt = o.material(density=1000., young=E)
+    for i in range(m):
+        for j in range(m):
+            for k in range(m):
+                o.sphere(((i + .5) * a, (j + .5) * a, (k + .5) * a), r, mt)
+    o.set_params(dt=.1 * o.pwave_dt())
+    o.run(["Collider", "ContactLoop"], 1)
+    kn = E * math.pi * r / 2
+    sig = o.stress()
+    expect = kn * (a - 2 * r) / a ** 2
+    np.testing.assert_allclose(np.diag(sig), [expect] * 3, rtol=1e-9)
+    assert np.max(np.abs(sig - np.diag(np.diag(sig)))) < 1e-9 * abs(expect)
+    with pytest.raises(RuntimeError):
+        Oracle().stress()  # aperiodic needs a volume
+
+
+def test_stress_tangential_term_symmetric():
+    # two spheres pressed along x and sheared along y: the tangential force enters as
+    # (fT_i n_j + fT_j n_i)/2; compare with the tensor rebuilt from the contact's frame
+    r, L = .01, .1
+    o = Oracle(periodic=1, cell=(L, L, L), physKind=0, lawKind=0, damping=0., gravity=(0, 0, 0), verletDist=0.)
+    mt = o.material(density=1000., young=1e7, tanPhi=.8)
+    o.sphere((.05, .05, .05), r, mt)
+    o.sphere((.05 + 2 * r * (1 - 1e-3), .05, .05), r, mt)
+    o.set("vel", np.array([[0, -.01, 0], [0, .01, 0]]))
+    o.set_params(dt=.05 * o.pwave_dt())
+    o.run(ENG, 20)
+    ids, _, d = o.contacts(real_only=True)
+    assert len(ids) == 1
+    t0, t1 = d[0, 0:3], d[0, 3:6]
+    t2 = np.cross(t0, t1)
+    F = d[0, 9] * t0 + d[0, 10] * t1 + d[0, 11] * t2
+    n = t0
+    fN = F @ n
+    fT = F - n * fN
+    p = o.get("pos")
+    d0 = np.linalg.norm(p[1] - p[0])
+    expect = d0 * (fN * np.outer(n, n) + .5 * (np.outer(fT, n) + np.outer(n, fT))) / L ** 3
+    sig = o.stress()
+    assert abs(fT[1]) > 1e-3 * abs(fN)
+    np.testing.assert_allclose(sig, expect, rtol=1e-9, atol=1e-12 * np.max(np.abs(expect)))
+    np.testing.assert_allclose(sig, sig.T, rtol=0, atol=1e-15 * np.max(np.abs(sig)))
+
+
 @pytest.mark.parametrize("seed", [1, 2])
 def test_sap_incremental_equals_full_rebuild_and_bruteforce(seed):
     sc = deposition(n=1200, dense=True, seed=seed, vel_sigma=0.3)

@@ -23,7 +23,8 @@ EXPORTS = [
     "wdg_last_error", "wdg_ctx_create", "wdg_ctx_destroy", "wdg_set_field", "wdg_set_materials",
     "wdg_upload_particles", "wdg_upload", "wdg_download", "wdg_collide", "wdg_contact_loop", "wdg_leapfrog",
     "wdg_force_reset", "wdg_crit_dt", "wdg_contact_count", "wdg_contact_list", "wdg_observables", "wdg_stats",
-    "wdg_run", "wdg_relax", "wdg_sync", "wdg_stream",
+    "wdg_run", "wdg_relax", "wdg_sync", "wdg_stream", "wdg_stress", "wdg_kinetic_energy",
+    "wdg_set_precision",
 ]
 
 
@@ -104,6 +105,9 @@ def lib():
     L.wdg_contact_list.argtypes = [V, C.c_int32, _P(C.c_int32), _P(C.c_int32), _P(C.c_double)]
     L.wdg_observables.argtypes = [V, _P(C.c_double), _P(C.c_double), _P(C.c_double)]
     L.wdg_stats.argtypes = [V, _P(Counters)]
+    L.wdg_stress.argtypes = [V, C.c_double, _P(C.c_double)]
+    L.wdg_kinetic_energy.argtypes = [V, _P(C.c_double)]
+    L.wdg_set_precision.argtypes = [V, C.c_int32]
     L.wdg_run.argtypes = [V, _P(Program), C.c_int64, _P(C.c_int64), _P(C.c_double)]
     L.wdg_relax.argtypes = [V, C.c_int32, C.c_double, C.c_double, _P(C.c_double), _P(C.c_int32)]
     L.wdg_sync.argtypes = [V]
@@ -251,6 +255,21 @@ class Device:
         out = np.empty(3, dtype=np.float64)
         check(self.L.wdg_observables(self.h, dptr(lo), dptr(hi), dptr(out)))
         return dict(coordNumber=out[0], coordNumberSkipFree=out[1], solidFraction=out[2])
+
+    def stress(self, volume=-1.):
+        """Cauchy stress (3x3) from the real contacts; volume <= 0: the periodic cell's."""
+        out = np.empty(9, dtype=np.float64)
+        check(self.L.wdg_stress(self.h, volume, dptr(out)))
+        return out.reshape(3, 3)
+
+    def set_precision(self, bits):
+        """64: plain FP64 engine; 32: contact geometry and law in FP32 (restarts contacts)."""
+        check(self.L.wdg_set_precision(self.h, int(bits)))
+
+    def kinetic_energy(self):
+        e = C.c_double()
+        check(self.L.wdg_kinetic_energy(self.h, C.byref(e)))
+        return e.value
 
     def stats(self):
         s = Counters()

@@ -25,7 +25,7 @@ FLAGS = [
 
 SOURCES = ["wdg_sort.hip", "wdg_collide.hip", "wdg_contact.hip", "wdg_integrate.hip", "wdg_pack.hip",
            "wdg_capi.hip"]
-HEADERS = ["wdg_internal.hpp", "wdg_math.hpp", "wdg_physics.hpp"]
+HEADERS = ["wdg_internal.hpp", "wdg_math.hpp", "wdg_math32.hpp", "wdg_physics.hpp"]
 
 
 def _newest(paths):

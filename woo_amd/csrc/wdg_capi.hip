@@ -33,6 +33,7 @@ struct ExportArgs {
   RealView rv;
   const double *N0, *N1, *N2, *Y0, *Y1, *Y2, *F0, *F1, *F2, *uN, *gp;
   const double *rad, *mass;
+  const double *px, *py, *pz;
   const int32_t* mat;
   const wdg_material* mats;
   int hertz;
@@ -40,6 +41,7 @@ struct ExportArgs {
   double* out;
 };
 
+template <int PREC>
 __global__ void k_export(ExportArgs a) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.nc) return;
@@ -61,7 +63,7 @@ __global__ void k_export(ExportArgs a) {
   const wdg_material mA = a.mats[a.mat[pA]], mB = a.mats[a.mat[pB]];
   PhysVals ph;
   double kn, kt;
-  const double uN = a.uN[r];
+  const double uN = ldr<PREC>(a.uN, r);
   if (a.hertz) {
     phys_hertz(mA, mB, l0, l1, a.mass[pA], a.mass[pB], a.poisson, a.en, a.viscAlpha, ph);
     double kn0 = ph.K * sqrt(ph.R);
@@ -72,11 +74,17 @@ __global__ void k_export(ExportArgs a) {
     kn = ph.kn;
     kt = ph.kt;
   }
-  const double* g = a.gp + (int64_t)GP * r;
-  o[0] = a.N0[r]; o[1] = a.N1[r]; o[2] = a.N2[r];
-  o[3] = a.Y0[r]; o[4] = a.Y1[r]; o[5] = a.Y2[r];
-  o[6] = g[3]; o[7] = g[4]; o[8] = g[5];
-  o[9] = a.F0[r]; o[10] = a.F1[r]; o[11] = a.F2[r];
+  o[0] = ldr<PREC>(a.N0, r); o[1] = ldr<PREC>(a.N1, r); o[2] = ldr<PREC>(a.N2, r);
+  o[3] = ldr<PREC>(a.Y0, r); o[4] = ldr<PREC>(a.Y1, r); o[5] = ldr<PREC>(a.Y2, r);
+  if (PREC == 32) {
+    // the record holds the contact point relative to pA
+    const float* g = reinterpret_cast<const float*>(a.gp) + (int64_t)GP32 * r;
+    o[6] = a.px[pA] + (double)g[3]; o[7] = a.py[pA] + (double)g[4]; o[8] = a.pz[pA] + (double)g[5];
+  } else {
+    const double* g = a.gp + (int64_t)GP * r;
+    o[6] = g[3]; o[7] = g[4]; o[8] = g[5];
+  }
+  o[9] = ldr<PREC>(a.F0, r); o[10] = ldr<PREC>(a.F1, r); o[11] = ldr<PREC>(a.F2, r);
   o[12] = uN; o[13] = kn; o[14] = kt; o[15] = 1.;
 }
 
@@ -182,10 +190,12 @@ static std::vector<Record> collect_contacts(Ctx& c, int which, std::vector<doubl
       a.Y0 = RS.hY[0].p; a.Y1 = RS.hY[1].p; a.Y2 = RS.hY[2].p;
       a.F0 = RS.hF[0].p; a.F1 = RS.hF[1].p; a.F2 = RS.hF[2].p; a.uN = RS.huN.p; a.gp = RS.gp.p;
       a.rad = c.radius.p; a.mass = c.mass.p; a.mat = c.mat.p; a.mats = c.dMats.p;
+      a.px = c.pos[0].p; a.py = c.pos[1].p; a.pz = c.pos[2].p;
       a.hertz = c.lastPhysKind == WDG_PHYS_HERTZ;
       a.poisson = c.lastPoisson; a.en = c.lastEn; a.viscAlpha = c.lastViscAlpha;
       a.out = out.p;
-      hipLaunchKernelGGL(k_export, dim3(nblocks(nc, 256)), dim3(256), 0, c.stream, a);
+      hipLaunchKernelGGL(c.precision == 32 ? k_export<32> : k_export<64>, dim3(nblocks(nc, 256)), dim3(256), 0,
+                         c.stream, a);
       WDG_LAUNCH_CHECK();
       rec->resize((size_t)nc * 16);
       WDG_HIP(hipMemcpyAsync(rec->data(), out.p, rec->size() * 8, hipMemcpyDeviceToHost, c.stream));
@@ -275,9 +285,8 @@ int wdg_ctx_destroy(wdg_ctx* ctx) {
     c.uid.release(); c.sto.release(); c.dMats.release();
     c.cellKey.release(); c.sortIdx.release(); c.cellKeyTmp.release(); c.sortIdxTmp.release();
     c.perm.release(); c.invPerm.release(); c.cellStart.release(); c.cellEnd.release();
-    c.scount.release(); c.soff.release(); c.pairKey.release(); c.pairKeyTmp.release(); c.pairDist.release();
-    c.pairInfo.release(); c.pairVal.release(); c.pairValTmp.release(); c.runFlag.release(); c.runOff.release();
-    c.adjKey.release(); c.adjKeyTmp.release(); c.adjValTmp.release(); c.adjPart.release();
+    c.scount.release(); c.soff.release(); c.stPartner.release(); c.stInfo.release(); c.stDist.release();
+    c.adjEnt.release(); c.adjDeg.release(); c.stNb.release(); c.sdeg.release(); c.posf.release();
     c.sortHist.release(); c.scanAux.release(); c.red.release(); c.xfer.release(); c.dflags.release();
     if (c.hflags) (void)hipHostFree(c.hflags);
     if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -485,6 +494,28 @@ int wdg_contact_list(wdg_ctx* ctx, int32_t which, int32_t* ids, int32_t* cellDis
 int wdg_observables(wdg_ctx* ctx, const double boxMin[3], const double boxMax[3], double out[3]) {
   if (!ctx || !boxMin || !boxMax || !out) { g_err = "NULL argument"; return WDG_E_ARG; }
   return guard(ctx, [&] { observables(ctx->c, boxMin, boxMax, out); });
+}
+
+int wdg_set_precision(wdg_ctx* ctx, int32_t bits) {
+  if (!ctx || (bits != 32 && bits != 64)) { g_err = "precision must be 32 or 64"; return WDG_E_ARG; }
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    if (c.precision == bits) return;
+    // the contact state is laid out per precision: start the history afresh
+    if (c.forcesPending) flush_forces(c);
+    drop_contacts(c);
+    c.precision = bits;
+  });
+}
+
+int wdg_stress(wdg_ctx* ctx, double volume, double out[9]) {
+  if (!ctx || !out) { g_err = "NULL argument"; return WDG_E_ARG; }
+  return guard(ctx, [&] { stress(ctx->c, volume, out); });
+}
+
+int wdg_kinetic_energy(wdg_ctx* ctx, double* ek) {
+  if (!ctx || !ek) { g_err = "NULL argument"; return WDG_E_ARG; }
+  return guard(ctx, [&] { *ek = kinetic_energy(ctx->c); });
 }
 
 int wdg_stats(wdg_ctx* ctx, wdg_counters* out) {

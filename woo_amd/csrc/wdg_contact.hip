@@ -18,6 +18,7 @@
 // force on pA, contact point, cellDist) that the force gather reads.
 #include "wdg_internal.hpp"
 #include "wdg_math.hpp"
+#include "wdg_math32.hpp"
 #include "wdg_physics.hpp"
 
 namespace wdg {
@@ -330,6 +331,240 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
   }
 }
 
+// ================================================================== FP32 contact form
+// Contact geometry, physics and law in float; positions, the position/velocity update and
+// the per-particle force sums stay FP64. The geometry starts from the FP64 relative position
+// of pB's image (no cancellation in float); the contact point is kept relative to pA, and
+// the gather record carries both branch vectors (from pA, from pB's image), so the gather
+// reads no positions. The contact point's update uses its previous value relative to pA:
+// midContPt - midPos1 = (prev + current)/2 when pA moved by vel1*dt since the last step.
+// Float state per real contact: trsf columns 0-1 (6), local force (3), uN (1) in the
+// state arrays (their first halves), and a 12-float record: global force on pA (3), branch
+// from pA (3), branch from pB (3), unused (3).
+__device__ __forceinline__ f3 tof3(d3 v) { return mkf3((float)v.x, (float)v.y, (float)v.z); }
+
+template <bool PERIODIC, int LAW>
+__device__ __forceinline__ bool filter_slot32(const ContactArgs& a, int64_t c) {
+  bool keep = false;
+  if (c < a.nc) {
+    const uint32_t info = a.info[c];
+    const bool real = rv_real(a.ov, (uint32_t)c);
+    if (real || (info & CI_NEAR)) {
+      const bool swap = (info & CI_SWAP) != 0u;
+      const uint32_t pA = swap ? a.B[c] : a.A[c], pB = swap ? a.A[c] : a.B[c];
+      uint64_t cd;
+      const d3 shift2 = slot_shift<PERIODIC>(a, c, cd);
+      const f3 r = tof3(sub3(add3(mk3(a.px[pB], a.py[pB], a.pz[pB]), shift2), mk3(a.px[pA], a.py[pA], a.pz[pA])));
+      bool pass;
+      float uN;
+      if ((info & CI_TYPE_MASK) == CT_SS) {
+        const float r1 = (float)a.rad[pA], r2 = (float)a.rad[pB];
+        const float reach = (float)a.absDf * (r1 + r2);
+        pass = !(sqn3(r) - reach * reach > 0.f && !real);
+        uN = sqrtf(sqn3(r)) - (r1 + r2);
+      } else {
+        const uint32_t fl = a.flags[pA];
+        const int ax = flag_axis(fl), sense = flag_sense(fl);
+        const float radius = (float)a.rad[pB];
+        const float dist = comp3(r, ax);
+        pass = !(!real && fabsf(dist) > radius);
+        float nvv;
+        if (sense == 0) {
+          if (!real) nvv = dist > 0.f ? 1.f : -1.f;
+          else {
+            const uint32_t ko = rv_index(a.ov, (uint32_t)c);
+            const float* on = reinterpret_cast<const float*>(ax == 0 ? a.oN0 : (ax == 1 ? a.oN1 : a.oN2));
+            nvv = on[ko];
+          }
+        } else {
+          nvv = sense == 1 ? 1.f : -1.f;
+        }
+        uN = nvv * dist - radius;
+      }
+      keep = pass && law_keeps<LAW>((double)uN, a.noBreak);
+    }
+  }
+  return keep;
+}
+
+template <bool PERIODIC, int LAW>
+__global__ __launch_bounds__(GROUP_BS) void k_cfilter32(ContactArgs a) {
+  const uint32_t g = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t base = (int64_t)g * GROUP;
+  constexpr int NP = GROUP / GROUP_BS;
+  bool keep[NP];
+#pragma unroll
+  for (int pass = 0; pass < NP; pass++) keep[pass] = filter_slot32<PERIODIC, LAW>(a, base + pass * GROUP_BS + threadIdx.x);
+  uint32_t running = 0;
+#pragma unroll
+  for (int pass = 0; pass < NP; pass++)
+    group_pass(base + pass * GROUP_BS + threadIdx.x, keep[pass], base, running, a.nbits, a.ntpre, a.act);
+  if (threadIdx.x == 0) a.ngcnt[g] = running;
+}
+
+template <bool PERIODIC, int LAW>
+__global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
+  const float *oN0 = reinterpret_cast<const float*>(a.oN0), *oN1 = reinterpret_cast<const float*>(a.oN1),
+              *oN2 = reinterpret_cast<const float*>(a.oN2), *oY0 = reinterpret_cast<const float*>(a.oY0),
+              *oY1 = reinterpret_cast<const float*>(a.oY1), *oY2 = reinterpret_cast<const float*>(a.oY2),
+              *oF1 = reinterpret_cast<const float*>(a.oF1), *oF2 = reinterpret_cast<const float*>(a.oF2),
+              *ogp = reinterpret_cast<const float*>(a.ogp);
+  float *N0 = reinterpret_cast<float*>(a.N0), *N1 = reinterpret_cast<float*>(a.N1),
+        *N2 = reinterpret_cast<float*>(a.N2), *Y0 = reinterpret_cast<float*>(a.Y0), *Y1 = reinterpret_cast<float*>(a.Y1),
+        *Y2 = reinterpret_cast<float*>(a.Y2), *F0 = reinterpret_cast<float*>(a.F0), *F1 = reinterpret_cast<float*>(a.F1),
+        *F2 = reinterpret_cast<float*>(a.F2), *UN = reinterpret_cast<float*>(a.uN), *gp = reinterpret_cast<float*>(a.gp);
+  const float dt = (float)a.dt;
+  const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t g = lb / CF_BLOCKS;
+  const uint32_t cnt = a.ngcnt[g];
+  for (uint32_t loc = (lb % CF_BLOCKS) * CF_BS + threadIdx.x; loc < cnt; loc += CF_BS * CF_BLOCKS) {
+    const int64_t k = state_index((uint32_t)g, loc);
+    const uint32_t c = a.act[k];
+    const uint32_t info = a.info[c];
+    const bool real = rv_real(a.ov, c);
+    const uint32_t ko = real ? rv_index(a.ov, c) : 0u;
+    const bool swap = (info & CI_SWAP) != 0u;
+    const uint32_t pA = swap ? a.B[c] : a.A[c], pB = swap ? a.A[c] : a.B[c];
+    uint64_t cd;
+    const d3 shift2 = slot_shift<PERIODIC>(a, c, cd);
+    const f3 r = tof3(sub3(add3(mk3(a.px[pB], a.py[pB], a.pz[pB]), shift2), mk3(a.px[pA], a.py[pA], a.pz[pA])));
+    f3 prevNormal = mkf3(0.f, 0.f, 0.f);
+    if (real) prevNormal = mkf3(oN0[ko], oN1[ko], oN2[ko]);
+    f3 normal, cA;
+    float uN, r1, r2;
+    if ((info & CI_TYPE_MASK) == CT_SS) {
+      r1 = (float)a.rad[pA];
+      r2 = (float)a.rad[pB];
+      const float dist = sqrtf(sqn3(r));
+      uN = dist - (r1 + r2);
+      normal = div3(r, dist);
+      cA = mul3(normal, r1 + 0.5f * uN);
+    } else {
+      const uint32_t fl = a.flags[pA];
+      const int ax = flag_axis(fl), sense = flag_sense(fl);
+      const float radius = (float)a.rad[pB];
+      const float dist = comp3(r, ax);
+      // the sphere centre projected on the wall plane, relative to the wall node
+      cA = r;
+      setcomp3(cA, ax, 0.f);
+      float nvv;
+      if (sense == 0) nvv = !real ? (dist > 0.f ? 1.f : -1.f) : comp3(prevNormal, ax);
+      else nvv = sense == 1 ? 1.f : -1.f;
+      normal = mkf3(0.f, 0.f, 0.f);
+      setcomp3(normal, ax, nvv);
+      uN = nvv * dist - radius;
+      r1 = -radius;
+      r2 = radius;
+    }
+    const f3 cB = sub3(cA, r);
+    const f3 vel1 = tof3(mk3(a.vx[pA], a.vy[pA], a.vz[pA])), vel2 = tof3(mk3(a.vx[pB], a.vy[pB], a.vz[pB]));
+    const f3 av1 = tof3(mk3(a.wx[pA], a.wy[pA], a.wz[pA])), av2 = tof3(mk3(a.wx[pB], a.wy[pB], a.wz[pB]));
+    f3 T0, T1, T2, gvel;
+    if (!real) {
+      const f3 locX = normal;
+      const f3 e = fabsf(locX.y) < fabsf(locX.z) ? mkf3(0.f, 1.f, 0.f) : mkf3(0.f, 0.f, 1.f);
+      f3 locY = cross3(locX, e);
+      locY = sub3(locY, mul3(locX, dot3(locY, locX)));
+      locY = normalized3(locY);
+      T0 = locX;
+      T1 = locY;
+      T2 = cross3(locX, locY);
+      gvel = mkf3(0.f, 0.f, 0.f);
+    } else {
+      const f3 prevY = mkf3(oY0[ko], oY1[ko], oY2[ko]);
+      const float* og = ogp + (int64_t)GP32 * ko;
+      const f3 prevCA = mkf3(og[3], og[4], og[5]);
+      const f3 normRotVec = cross3(prevNormal, normal);
+      f3 midNormal = mul3(add3(prevNormal, normal), .5f);
+      if (sqrtf(sqn3(midNormal)) == 0.f) midNormal = normal;
+      else midNormal = normalized3(midNormal);
+      const float twist = dot3(midNormal, add3(av1, av2));
+      const f3 normTwistVec = mul3(mul3(mul3(midNormal, dt), .5f), twist);
+      const f3 rt = add3(normRotVec, normTwistVec);
+      const f3 M0 = midNormal;
+      const f3 M1 = sub3(prevY, div3(cross3(prevY, rt), 2.f));
+      const f3 M2 = cross3(M0, M1);
+      T0 = normal;
+      T1 = sub3(prevY, cross3(M1, rt));
+      T2 = cross3(T0, T1);
+      if (a.renorm) {
+        T0 = normalized3(T0);
+        T1 = sub3(T1, mul3(T0, dot3(T1, T0)));
+        T1 = normalized3(T1);
+        T2 = cross3(T0, T1);
+      }
+      const f3 c1x = mul3(add3(prevCA, cA), .5f);
+      const f3 c2x = add3(sub3(c1x, r), mul3(sub3(vel2, vel1), .5f * dt));
+      const f3 relVel = sub3(add3(vel2, cross3(av2, c2x)), add3(vel1, cross3(av1, c1x)));
+      gvel = mkf3(dot3(M0, relVel), dot3(M1, relVel), dot3(M2, relVel));
+    }
+    // Cp2 parameters in FP64 (functions of radii, materials and masses), the law in float
+    double l0, l1, A;
+    lens_area((double)r1, (double)r2, l0, l1, A);
+    const wdg_material mA = a.mats[a.mat[pA]], mB = a.mats[a.mat[pB]];
+    float Fn, Ft0 = real ? oF1[ko] : 0.f, Ft1 = real ? oF2[ko] : 0.f;
+    if (LAW == WDG_LAW_IDEALELPL) {
+      PhysVals ph;
+      phys_frict(mA, mB, l0, l1, A, ph);
+      const float kn = (float)ph.kn, kt = (float)ph.kt, tanPhi = (float)ph.tanPhi;
+      Fn = kn * uN;
+      if (a.noFrict || tanPhi == 0.f) {
+        Ft0 = 0.f;
+        Ft1 = 0.f;
+      } else {
+        const float dk = dt * kt;
+        Ft0 = Ft0 + dk * gvel.y;
+        Ft1 = Ft1 + dk * gvel.z;
+        const float maxFt = fabsf(Fn) * tanPhi;
+        if (Ft0 * Ft0 + Ft1 * Ft1 > maxFt * maxFt && !a.noSlip) {
+          const float FtNorm = sqrtf(Ft0 * Ft0 + Ft1 * Ft1);
+          float ratio = maxFt / FtNorm;
+          if (FtNorm == 0.f) ratio = 0.f;
+          Ft0 = Ft0 * ratio;
+          Ft1 = Ft1 * ratio;
+        }
+        if (isnan(Fn) || isnan(Ft0) || isnan(Ft1)) atomicOr(&a.dflags[1], ERR_NAN_FORCE);
+      }
+    } else {
+      PhysVals ph;
+      phys_hertz(mA, mB, l0, l1, a.mass[pA], a.mass[pB], a.poisson, a.en, a.viscAlpha, ph);
+      const float velN = gvel.x;
+      const float kn0 = (float)(ph.K * sqrt(ph.R));
+      const float s = sqrtf(-uN);
+      const float Fne = -kn0 * ((s * s) * s);
+      const float Fna = (float)(4 * M_PI * ph.R * ph.gamma);
+      const float aMK = (float)ph.alpha_sqrtMK;
+      const float eta = (aMK > 0.f ? aMK * sqrtf(sqrtf(fmaxf(-uN, 0.f))) : 0.f);
+      float Fnv = eta * velN;
+      if (ph.alpha == 0. && a.noAttraction && Fne + Fnv > 0.f) Fnv = -Fne;
+      Fn = Fne + Fna + Fnv;
+      const float kt = (float)ph.kt0 * sqrtf(uN < 0.f ? -uN : 0.f);
+      const float dk = dt * kt;
+      Ft0 = Ft0 + dk * gvel.y;
+      Ft1 = Ft1 + dk * gvel.z;
+      const float maxFt = fabsf(fminf(0.f, Fn) * (float)ph.tanPhi);
+      if (Ft0 * Ft0 + Ft1 * Ft1 > maxFt * maxFt) {
+        const float FtNorm = sqrtf(Ft0 * Ft0 + Ft1 * Ft1);
+        const float ratio = maxFt / FtNorm;
+        Ft0 = Ft0 * ratio;
+        Ft1 = Ft1 * ratio;
+      } else {
+        Ft0 = Ft0 + eta * gvel.y;
+        Ft1 = Ft1 + eta * gvel.z;
+      }
+    }
+    N0[k] = T0.x; N1[k] = T0.y; N2[k] = T0.z;
+    Y0[k] = T1.x; Y1[k] = T1.y; Y2[k] = T1.z;
+    F0[k] = Fn; F1[k] = Ft0; F2[k] = Ft1;
+    UN[k] = uN;
+    const f3 G = quat_rot(quat_from_cols(T0, T1, T2), mkf3(Fn, Ft0, Ft1));
+    float4* g4 = reinterpret_cast<float4*>(gp + (int64_t)GP32 * k);
+    g4[0] = make_float4(G.x, G.y, G.z, cA.x);
+    g4[1] = make_float4(cA.y, cA.z, cB.x, cB.y);
+    g4[2] = make_float4(cB.z, 0.f, 0.f, 0.f);
+  }
+}
+
 static ContactArgs contact_args(Ctx& c, int from, int to) {
   Contacts& C = c.con();
   Reals& O = c.R[from];
@@ -397,6 +632,28 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
   const bool per = c.field.periodic != 0;
   const bool hertz = p->lawKind == WDG_LAW_HERTZDMT;
   const dim3 gfil((unsigned)ng), bfil(GROUP_BS);
+  if (c.precision == 32) {
+    const dim3 gforce((unsigned)(ng * CF_BLOCKS)), bforce(CF_BS);
+    if (hertz) {
+      if (per) hipLaunchKernelGGL((k_cfilter32<true, WDG_LAW_HERTZDMT>), gfil, bfil, 0, c.stream, a);
+      else hipLaunchKernelGGL((k_cfilter32<false, WDG_LAW_HERTZDMT>), gfil, bfil, 0, c.stream, a);
+    } else {
+      if (per) hipLaunchKernelGGL((k_cfilter32<true, WDG_LAW_IDEALELPL>), gfil, bfil, 0, c.stream, a);
+      else hipLaunchKernelGGL((k_cfilter32<false, WDG_LAW_IDEALELPL>), gfil, bfil, 0, c.stream, a);
+    }
+    WDG_LAUNCH_CHECK();
+    if (hertz) {
+      if (per) hipLaunchKernelGGL((k_cforce32<true, WDG_LAW_HERTZDMT>), gforce, bforce, 0, c.stream, a);
+      else hipLaunchKernelGGL((k_cforce32<false, WDG_LAW_HERTZDMT>), gforce, bforce, 0, c.stream, a);
+    } else {
+      if (per) hipLaunchKernelGGL((k_cforce32<true, WDG_LAW_IDEALELPL>), gforce, bforce, 0, c.stream, a);
+      else hipLaunchKernelGGL((k_cforce32<false, WDG_LAW_IDEALELPL>), gforce, bforce, 0, c.stream, a);
+    }
+    WDG_LAUNCH_CHECK();
+    c.rcur = to;
+    c.forcesPending = p->applyForces != 0;
+    return;
+  }
   if (hertz) {
     if (per) hipLaunchKernelGGL((k_cfilter<true, WDG_LAW_HERTZDMT>), gfil, bfil, 0, c.stream, a);
     else hipLaunchKernelGGL((k_cfilter<false, WDG_LAW_HERTZDMT>), gfil, bfil, 0, c.stream, a);

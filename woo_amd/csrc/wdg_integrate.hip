@@ -23,13 +23,26 @@ struct GatherArgs {
   double L[3];
 };
 
-template <bool PERIODIC>
+// F32: the FP32 contact form's record (global force, branch vectors from pA and from pB's
+// image as floats); the sums stay FP64
+template <bool PERIODIC, bool F32>
 __device__ __forceinline__ void gather_all(const GatherArgs& g, int64_t i, d3 pos, d3& F, d3& T) {
   const uint32_t k0 = g.adjOff[i], k1 = g.adjOff[i + 1];
   for (uint32_t k = k0; k < k1; k++) {
     const uint32_t e = g.adj[k];
     const uint32_t slot = e & ~ADJ_PA;
     if (!rv_real(g.rv, slot)) continue;
+    if (F32) {
+      const float4* r =
+          reinterpret_cast<const float4*>(reinterpret_cast<const float*>(g.gp) + (int64_t)GP32 * rv_index(g.rv, slot));
+      const float4 r0 = r[0], r1 = r[1], r2 = r[2];
+      const bool isPA = (e & ADJ_PA) != 0u;
+      const d3 Fc = mul3(mk3(r0.x, r0.y, r0.z), isPA ? 1. : -1.);
+      const d3 xc = isPA ? mk3(r0.w, r1.x, r1.y) : mk3(r1.z, r1.w, r2.x);
+      F = add3(F, Fc);
+      T = add3(T, cross3(xc, Fc));
+      continue;
+    }
     const double2* r = reinterpret_cast<const double2*>(g.gp + (int64_t)GP * rv_index(g.rv, slot));
     const double2 r0 = r[0], r1 = r[1], r2 = r[2];
     const bool isPA = (e & ADJ_PA) != 0u;
@@ -47,7 +60,7 @@ __device__ __forceinline__ void gather_all(const GatherArgs& g, int64_t i, d3 po
   }
 }
 
-template <bool PERIODIC>
+template <bool PERIODIC, bool F32>
 __global__ __launch_bounds__(256) void k_gather(GatherArgs g, double* fx, double* fy, double* fz, double* tx,
                                                  double* ty, double* tz, const double* __restrict__ px,
                                                  const double* __restrict__ py, const double* __restrict__ pz) {
@@ -55,7 +68,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs g, double* fx, double
   if (i >= g.n) return;
   if (g.adjOff[i] == g.adjOff[i + 1]) return;
   d3 F = mk3(fx[i], fy[i], fz[i]), T = mk3(tx[i], ty[i], tz[i]);
-  gather_all<PERIODIC>(g, i, mk3(px[i], py[i], pz[i]), F, T);
+  gather_all<PERIODIC, F32>(g, i, mk3(px[i], py[i], pz[i]), F, T);
   fx[i] = F.x; fy[i] = F.y; fz[i] = F.z;
   tx[i] = T.x; ty[i] = T.y; tz[i] = T.z;
 }
@@ -78,7 +91,7 @@ struct LeapArgs {
   uint32_t* dirtyNext;  // cleared for the next launch
 };
 
-template <bool PERIODIC>
+template <bool PERIODIC, bool F32>
 __global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
   const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   bool dirty = false;
@@ -97,7 +110,7 @@ __global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
       F = mk3(a.fx[i], a.fy[i], a.fz[i]);
       T = mk3(a.tx[i], a.ty[i], a.tz[i]);
     }
-    if (a.gather) gather_all<PERIODIC>(g, i, pos, F, T);
+    if (a.gather) gather_all<PERIODIC, F32>(g, i, pos, F, T);
     if (!integrated) {
       if (a.gather) {
         a.fx[i] = F.x; a.fy[i] = F.y; a.fz[i] = F.z;
@@ -205,12 +218,9 @@ void flush_forces(Ctx& c) {
   if (c.con().n == 0 || c.n == 0) return;
   GatherArgs g = gather_args(c);
   dim3 grid(nblocks(c.n, 256)), block(256);
-  if (c.field.periodic)
-    hipLaunchKernelGGL(k_gather<true>, grid, block, 0, c.stream, g, c.force[0].p, c.force[1].p, c.force[2].p,
-                       c.torque[0].p, c.torque[1].p, c.torque[2].p, c.pos[0].p, c.pos[1].p, c.pos[2].p);
-  else
-    hipLaunchKernelGGL(k_gather<false>, grid, block, 0, c.stream, g, c.force[0].p, c.force[1].p, c.force[2].p,
-                       c.torque[0].p, c.torque[1].p, c.torque[2].p, c.pos[0].p, c.pos[1].p, c.pos[2].p);
+  auto* kern = c.precision == 32 ? k_gather<false, true> : (c.field.periodic ? k_gather<true, false> : k_gather<false, false>);
+  hipLaunchKernelGGL(kern, grid, block, 0, c.stream, g, c.force[0].p, c.force[1].p, c.force[2].p, c.torque[0].p,
+                     c.torque[1].p, c.torque[2].p, c.pos[0].p, c.pos[1].p, c.pos[2].p);
   WDG_LAUNCH_CHECK();
 }
 
@@ -240,8 +250,9 @@ void leapfrog(Ctx& c, const wdg_leapfrog_params* p) {
   a.dirtyNext = c.dflags.p + DIRTY_PP + (1 - s);
   GatherArgs g = gather_args(c);
   dim3 grid(nblocks(c.n, 256)), block(256);
-  if (c.field.periodic) hipLaunchKernelGGL(k_leapfrog<true>, grid, block, 0, c.stream, a, g);
-  else hipLaunchKernelGGL(k_leapfrog<false>, grid, block, 0, c.stream, a, g);
+  auto* kern = c.precision == 32 ? k_leapfrog<false, true>
+                                 : (c.field.periodic ? k_leapfrog<true, false> : k_leapfrog<false, false>);
+  hipLaunchKernelGGL(kern, grid, block, 0, c.stream, a, g);
   WDG_LAUNCH_CHECK();
   c.lfParity = 1 - s;
   c.dirtyIdx = DIRTY_PP + s;
@@ -276,6 +287,7 @@ struct DynArgs {
   unsigned long long* out;
 };
 
+template <int PREC>
 __device__ __forceinline__ void dyn_one(const DynArgs& a, uint32_t e, d3& kt, d3& kr) {
   const uint32_t s = e & ~ADJ_PA;
   if (!rv_real(a.rv, s)) return;
@@ -295,7 +307,7 @@ __device__ __forceinline__ void dyn_one(const DynArgs& a, uint32_t e, d3& kt, d3
   if (a.hertz) {
     phys_hertz(mA, mB, l0, l1, a.mass[pA], a.mass[pB], a.poisson, a.en, a.viscAlpha, ph);
     double kn0 = ph.K * sqrt(ph.R);
-    double uN = a.uN[r];
+    double uN = ldr<PREC>(a.uN, r);
     kn = (3 / 2.) * kn0 * sqrt(-uN);
     ktt = ph.kt0 * sqrt(uN < 0 ? -uN : 0);
   } else {
@@ -303,7 +315,8 @@ __device__ __forceinline__ void dyn_one(const DynArgs& a, uint32_t e, d3& kt, d3
     kn = ph.kn;
     ktt = ph.kt;
   }
-  d3 T0 = mk3(a.N0[r], a.N1[r], a.N2[r]), T1 = mk3(a.Y0[r], a.Y1[r], a.Y2[r]);
+  d3 T0 = mk3(ldr<PREC>(a.N0, r), ldr<PREC>(a.N1, r), ldr<PREC>(a.N2, r)),
+     T1 = mk3(ldr<PREC>(a.Y0, r), ldr<PREC>(a.Y1, r), ldr<PREC>(a.Y2, r));
   q4 q = quat_from_cols(T0, T1, cross3(T0, T1));
   d3 n = quat_rot(q, mk3(1., 0., 0.));
   d3 n2 = mk3(n.x * n.x, n.y * n.y, n.z * n.z);
@@ -313,6 +326,7 @@ __device__ __forceinline__ void dyn_one(const DynArgs& a, uint32_t e, d3& kt, d3
   kr = mk3(kr.x + lk * (n2.y + n2.z), kr.y + lk * (n2.z + n2.x), kr.z + lk * (n2.x + n2.y));
 }
 
+template <int PREC>
 __global__ __launch_bounds__(256) void k_dyndt(DynArgs a) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double ret = INFINITY;
@@ -320,7 +334,7 @@ __global__ __launch_bounds__(256) void k_dyndt(DynArgs a) {
     const uint32_t fl = a.flags[i];
     if ((fl & F_INTEGRATED) && (fl & F_DOF_ALL) != F_DOF_ALL) {
       d3 kt = mk3(0., 0., 0.), kr = mk3(0., 0., 0.);
-      for (uint32_t k = a.adjOff[i]; k < a.adjOff[i + 1]; k++) dyn_one(a, a.adj[k], kt, kr);
+      for (uint32_t k = a.adjOff[i]; k < a.adjOff[i + 1]; k++) dyn_one<PREC>(a, a.adj[k], kt, kr);
       const double m = a.mass[i];
       const double I[3] = {a.ix[i], a.iy[i], a.iz[i]};
       const double ktv[3] = {kt.x, kt.y, kt.z}, krv[3] = {kr.x, kr.y, kr.z};
@@ -358,7 +372,8 @@ double crit_dt(Ctx& c) {
   unsigned long long init = (unsigned long long)0x7FF0000000000000ull;  // +inf
   WDG_HIP(hipMemcpyAsync(out, &init, sizeof(init), hipMemcpyHostToDevice, c.stream));
   a.out = out;
-  hipLaunchKernelGGL(k_dyndt, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, a);
+  hipLaunchKernelGGL(c.precision == 32 ? k_dyndt<32> : k_dyndt<64>, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream,
+                     a);
   WDG_LAUNCH_CHECK();
   unsigned long long hb;
   WDG_HIP(hipMemcpyAsync(&hb, out, sizeof(hb), hipMemcpyDeviceToHost, c.stream));
@@ -441,6 +456,144 @@ void observables(Ctx& c, const double lo[3], const double hi[3], double out[3]) 
   out[1] = (C2 - N1) / (N - N0 - N1);
   double V = ((hi[0] - lo[0]) * (hi[1] - lo[1])) * (hi[2] - lo[2]);
   out[2] = Vs / V;
+}
+
+// ------------------------------------------------------------------ stress, kinetic energy
+// DemFuncs::stressStiffness, stress part (Funcs.cpp:183-229): per real contact
+// d0 = |posA - shift - posB|, n = ori*e_x, F = ori*force (the gather record's global force),
+// fN = F.n, fT = F - n fN, sigma_ij += d0 (fN n_i n_j + (fT_i n_j + fT_j n_i)/2).
+// Block partial sums in slot order, added on the host in block order.
+struct StressArgs {
+  int64_t nc;
+  const uint32_t *A, *B, *info;
+  const uint64_t* dist;
+  RealView rv;
+  const double *N0, *N1, *N2, *Y0, *Y1, *Y2, *gp;
+  const double *px, *py, *pz;
+  double L[3];
+  int periodic;
+  double* part;
+};
+template <int PREC>
+__global__ __launch_bounds__(256) void k_stress(StressArgs a) {
+  __shared__ double sh[9][256];
+  const int tid = threadIdx.x;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + tid;
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (c < a.nc && rv_real(a.rv, (uint32_t)c)) {
+    const uint32_t r = rv_index(a.rv, (uint32_t)c);
+    const uint32_t inf = a.info[c];
+    const bool swap = (inf & CI_SWAP) != 0u;
+    const uint32_t pA = swap ? a.B[c] : a.A[c], pB = swap ? a.A[c] : a.B[c];
+    d3 posA = mk3(a.px[pA], a.py[pA], a.pz[pA]);
+    const d3 posB = mk3(a.px[pB], a.py[pB], a.pz[pB]);
+    if (a.periodic) {
+      const uint64_t cd = a.dist[c];
+      posA = sub3(posA, mk3(a.L[0] * unpack_cd(cd, 0), a.L[1] * unpack_cd(cd, 1), a.L[2] * unpack_cd(cd, 2)));
+    }
+    const double d0 = sqrt(sqn3(sub3(posA, posB)));
+    const d3 T0 = mk3(ldr<PREC>(a.N0, r), ldr<PREC>(a.N1, r), ldr<PREC>(a.N2, r)),
+             T1 = mk3(ldr<PREC>(a.Y0, r), ldr<PREC>(a.Y1, r), ldr<PREC>(a.Y2, r));
+    const q4 q = quat_from_cols(T0, T1, cross3(T0, T1));
+    const d3 n = quat_rot(q, mk3(1., 0., 0.));
+    d3 F;
+    if (PREC == 32) {
+      const float* g = reinterpret_cast<const float*>(a.gp) + (int64_t)GP32 * r;
+      F = mk3(g[0], g[1], g[2]);
+    } else {
+      const double* g = a.gp + (int64_t)GP * r;
+      F = mk3(g[0], g[1], g[2]);
+    }
+    const double fN = dot3(F, n);
+    const d3 fT = sub3(F, mul3(n, fN));
+    const double nn[3] = {n.x, n.y, n.z}, tt[3] = {fT.x, fT.y, fT.z};
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) v[3 * i + j] = d0 * (fN * nn[i] * nn[j] + .5 * (tt[i] * nn[j] + tt[j] * nn[i]));
+  }
+  for (int q = 0; q < 9; q++) sh[q][tid] = v[q];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st)
+      for (int q = 0; q < 9; q++) sh[q][tid] += sh[q][tid + st];
+    __syncthreads();
+  }
+  if (tid == 0)
+    for (int q = 0; q < 9; q++) a.part[(int64_t)blockIdx.x * 9 + q] = sh[q][0];
+}
+
+void stress(Ctx& c, double volume, double out[9]) {
+  for (int k = 0; k < 9; k++) out[k] = 0;
+  if (volume <= 0) {
+    if (!c.field.periodic) throw Error(WDG_E_ARG, "a positive volume must be given for aperiodic scenes");
+    volume = c.field.cell[0] * c.field.cell[1] * c.field.cell[2];
+  }
+  Contacts& C = c.con();
+  if (C.n == 0) return;
+  Reals& R = c.reals();
+  const unsigned nb = nblocks(C.n, 256);
+  DBuf<double> part;
+  part.ensure((size_t)nb * 9);
+  StressArgs a{};
+  a.nc = C.n; a.A = C.A.p; a.B = C.B.p; a.info = C.info.p; a.dist = C.dist.p; a.rv = c.rview();
+  a.N0 = R.hN[0].p; a.N1 = R.hN[1].p; a.N2 = R.hN[2].p; a.Y0 = R.hY[0].p; a.Y1 = R.hY[1].p; a.Y2 = R.hY[2].p;
+  a.gp = R.gp.p;
+  a.px = c.pos[0].p; a.py = c.pos[1].p; a.pz = c.pos[2].p;
+  for (int k = 0; k < 3; k++) a.L[k] = c.field.cell[k];
+  a.periodic = c.field.periodic;
+  a.part = part.p;
+  hipLaunchKernelGGL(c.precision == 32 ? k_stress<32> : k_stress<64>, dim3(nb), dim3(256), 0, c.stream, a);
+  WDG_LAUNCH_CHECK();
+  std::vector<double> h((size_t)nb * 9);
+  WDG_HIP(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  WDG_HIP(hipStreamSynchronize(c.stream));
+  part.release();
+  for (unsigned b = 0; b < nb; b++)
+    for (int k = 0; k < 9; k++) out[k] += h[(size_t)b * 9 + k];
+  for (int k = 0; k < 9; k++) out[k] /= volume;
+}
+
+// sum over integrated nodes of m v.v/2 + w.(I w)/2
+__global__ __launch_bounds__(256) void k_ek(int64_t n, const uint32_t* __restrict__ flags, const double* __restrict__ m,
+                                          const double* __restrict__ vx, const double* __restrict__ vy,
+                                          const double* __restrict__ vz, const double* __restrict__ wx,
+                                          const double* __restrict__ wy, const double* __restrict__ wz,
+                                          const double* __restrict__ ix, const double* __restrict__ iy,
+                                          const double* __restrict__ iz, double* part) {
+  __shared__ double sh[256];
+  const int tid = threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
+  double e = 0;
+  if (i < n && (flags[i] & F_INTEGRATED)) {
+    const d3 v = mk3(vx[i], vy[i], vz[i]), w = mk3(wx[i], wy[i], wz[i]);
+    const double et = .5 * m[i] * sqn3(v);
+    const double er = 0.5 * dot3(w, mk3(ix[i] * w.x, iy[i] * w.y, iz[i] * w.z));
+    e = et + er;
+  }
+  sh[tid] = e;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) sh[tid] += sh[tid + st];
+    __syncthreads();
+  }
+  if (tid == 0) part[blockIdx.x] = sh[0];
+}
+
+double kinetic_energy(Ctx& c) {
+  if (c.n == 0) return 0;
+  const unsigned nb = nblocks(c.n, 256);
+  DBuf<double> part;
+  part.ensure(nb);
+  hipLaunchKernelGGL(k_ek, dim3(nb), dim3(256), 0, c.stream, c.n, c.flags.p, c.mass.p, c.vel[0].p, c.vel[1].p,
+                     c.vel[2].p, c.angVel[0].p, c.angVel[1].p, c.angVel[2].p, c.inertia[0].p, c.inertia[1].p,
+                     c.inertia[2].p, part.p);
+  WDG_LAUNCH_CHECK();
+  std::vector<double> h(nb);
+  WDG_HIP(hipMemcpyAsync(h.data(), part.p, nb * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  WDG_HIP(hipStreamSynchronize(c.stream));
+  part.release();
+  double s = 0;
+  for (double x : h) s += x;
+  return s;
 }
 
 int64_t count_real(Ctx& c) {

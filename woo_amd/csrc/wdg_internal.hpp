@@ -72,7 +72,7 @@ __host__ __device__ inline int unpack_cd(uint64_t v, int k) {
 constexpr int DIRTY_PP = 12;
 
 // device error bits
-enum : uint32_t { ERR_NAN_FORCE = 1u, ERR_DOUBLE_IMAGE = 2u, ERR_CD_RANGE = 4u };
+enum : uint32_t { ERR_NAN_FORCE = 1u, ERR_DOUBLE_IMAGE = 2u, ERR_CD_RANGE = 4u, ERR_ADJ = 8u };
 
 // ------------------------------------------------------------------ device buffers
 template <class T>
@@ -130,7 +130,8 @@ constexpr int GROUP = 4096;   // slots per group (16 tiles)
 constexpr int GROUP_BS = 1024;  // threads of a group kernel
 inline int64_t ntiles(int64_t nslots) { return (nslots + TILE - 1) / TILE; }
 inline int64_t ngroups(int64_t nslots) { return (nslots + GROUP - 1) / GROUP; }
-constexpr int GP = 8;         // doubles per gather record
+constexpr int GP = 8;         // doubles per gather record (FP64 contact form)
+constexpr int GP32 = 12;      // floats per gather record (FP32 contact form)
 struct Reals {
   DBuf<unsigned long long> bits;  // 64 words per group
   DBuf<uint32_t> tpre;            // per tile: real slots of its group before the tile
@@ -167,6 +168,12 @@ struct RealView {
 constexpr uint32_t GROUP_SKEW = 2531;
 __host__ __device__ inline uint32_t state_index(uint32_t g, uint32_t local) {
   return g * (uint32_t)GROUP + ((local + g * GROUP_SKEW) & (uint32_t)(GROUP - 1));
+}
+// element i of a contact-state array: double in the FP64 form, float (first half) in FP32
+template <int PREC>
+__device__ __forceinline__ double ldr(const double* p, uint32_t i) {
+  if (PREC == 32) return (double)reinterpret_cast<const float*>(p)[i];
+  return p[i];
 }
 __device__ __forceinline__ bool rv_real(const RealView& v, uint32_t s) { return (v.bits[s >> 6] >> (s & 63)) & 1ull; }
 // state index of real slot s
@@ -213,12 +220,13 @@ struct Ctx {
   // broadphase scratch
   DBuf<uint32_t> cellKey, sortIdx, cellKeyTmp, sortIdxTmp, perm, invPerm;
   DBuf<uint32_t> cellStart, cellEnd;
-  DBuf<uint32_t> scount, soff;
-  DBuf<uint64_t> pairKey, pairKeyTmp, pairDist;
-  DBuf<uint32_t> pairInfo, pairVal, pairValTmp;
-  DBuf<uint32_t> runFlag, runOff;
-  DBuf<uint64_t> adjKey, adjKeyTmp;
-  DBuf<uint32_t> adjValTmp, adjPart;
+  DBuf<uint32_t> scount, soff;          // pairs per generating sphere, their offsets
+  DBuf<uint32_t> stPartner, stInfo;     // staged pairs, a fixed number per sphere
+  DBuf<uint64_t> stDist;
+  DBuf<uint32_t> stNb, sdeg;            // staged neighbours per sphere, their number
+  DBuf<float4> posf;                    // float candidate records of the last refresh
+  DBuf<uint64_t> adjEnt;                // adjacency entries with their sort key
+  DBuf<uint32_t> adjDeg;
   DBuf<uint32_t> sortHist, scanAux;
   DBuf<uint32_t> act;     // slot of each state index (contact loop), per-slot flags (relaxation)
   DBuf<double> red;       // reductions
@@ -247,6 +255,8 @@ struct Ctx {
   double verletDist = NAN;
   int64_t nRefresh = 0, nCollide = 0;
   int64_t dynStepLast = -1, dynStepPrev = -1;
+  // contact form: 64 (plain FP64) or 32 (contact geometry and law in FP32)
+  int precision = 64;
   // contact model of the last contact loop (DynDt and contact export need it)
   int lastPhysKind = WDG_PHYS_FRICT;
   double lastPoisson = .2, lastEn = NAN, lastViscAlpha = 0;
@@ -268,6 +278,8 @@ void flush_forces(Ctx& c);
 double crit_dt(Ctx& c);
 void observables(Ctx& c, const double lo[3], const double hi[3], double out[3]);
 int64_t count_real(Ctx& c);
+void stress(Ctx& c, double volume, double out[9]);
+double kinetic_energy(Ctx& c);
 // bits + tile prefix of R[dst] from a per-slot flag array (nonzero = real)
 void reals_from_flags(Ctx& c, const uint32_t* flag, int64_t nslots, int dst);
 void check_errors(Ctx& c);

@@ -233,6 +233,20 @@ std::array<double, 3> DemField::observables(const Vec3& lo, const Vec3& hi) {
   return out;
 }
 
+std::array<double, 9> DemField::stress(double volume) {
+  if (!dev || structureChanged) throw std::runtime_error("no simulation state on the device yet (run a step first)");
+  std::array<double, 9> out{};
+  check(wdg_stress(dev->ctx, volume, out.data()));
+  return out;
+}
+
+double DemField::kineticEnergy() {
+  if (!dev || structureChanged) throw std::runtime_error("no simulation state on the device yet (run a step first)");
+  double e = 0;
+  check(wdg_kinetic_energy(dev->ctx, &e));
+  return e;
+}
+
 // ------------------------------------------------------------------ engines
 double DynDt::critDt(Scene& S) {
   auto f = S.dem();
@@ -380,7 +394,7 @@ void Scene::run(int64_t steps) {
       if (!e || e->dead) continue;
       if (auto* pr = dynamic_cast<PyRunner*>(e.get())) {
         if (pr->isActivated(step) && pr->command) {
-          pr->command();
+          pr->command(*this);
           f->push(*this);
           f->contactsValid = false;
         }

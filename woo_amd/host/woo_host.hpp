@@ -198,7 +198,7 @@ struct DynDt : PeriodicEngine {
 };
 // runs host code between device steps (the role of woo.core.PyRunner)
 struct PyRunner : PeriodicEngine {
-  std::function<void()> command;
+  std::function<void(Scene&)> command;
   std::string source;  // for display
   const char* className() const override { return "PyRunner"; }
 };
@@ -245,6 +245,8 @@ struct DemField : Object {
   int64_t countReal();
   double pWaveDt() const;        // DemFuncs::pWaveDt (Funcs.cpp:151-175)
   std::array<double, 3> observables(const Vec3& lo, const Vec3& hi);
+  std::array<double, 9> stress(double volume);  // DemFuncs::stressStiffness, stress part
+  double kineticEnergy();
 };
 
 struct Scene : Object {

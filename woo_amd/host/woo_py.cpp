@@ -336,6 +336,15 @@ PYBIND11_MODULE(_woo, m) {
       })
       .def("critDt", &DemField::pWaveDt)
       .def("observables", [](DemField& f, py::handle lo, py::handle hi) { return f.observables(vec3(lo), vec3(hi)); })
+      .def("stress",
+           [](DemField& f, double volume) {
+             auto s = f.stress(volume);
+             py::array_t<double> a({3, 3});
+             std::copy(s.begin(), s.end(), a.mutable_data());
+             return a;
+           },
+           py::arg("volume") = -1.)
+      .def("kineticEnergy", &DemField::kineticEnergy)
       .def_property_readonly_static("defaultMovableMask", [](py::object) { return 0b0101; })
       .def_property_readonly_static("defaultBoundaryMask", [](py::object) { return 0b0011; })
       .def_property_readonly_static("defaultLoneMask", [](py::object) { return 0b0010; });
@@ -435,12 +444,27 @@ PYBIND11_MODULE(_woo, m) {
       }))
       .def_property(
           "command", [](PyRunner& r) { return r.source; },
-          [](py::object self, py::object cmd) {
-            PyRunner& r = self.cast<PyRunner&>();
-            if (py::isinstance<py::str>(cmd)) r.source = cmd.cast<std::string>();
-            else r.source = py::str(py::repr(cmd)).cast<std::string>();
-            // the Scene binds the callable before running (it needs S in the namespace)
-            self.attr("_command") = cmd;
+          [](PyRunner& r, py::object cmd) {
+            // the engine keeps the command itself (its Python wrapper may not survive); the
+            // reference is dropped with the GIL held
+            std::shared_ptr<py::object> keep(new py::object(cmd), [](py::object* o) {
+              py::gil_scoped_acquire gil;
+              delete o;
+            });
+            if (py::isinstance<py::str>(cmd)) {
+              r.source = cmd.cast<std::string>();
+              // a string runs with S and woo in its namespace (PyRunner.command)
+              r.command = [keep](Scene& S) {
+                py::dict g;
+                g["S"] = py::cast(&S, py::return_value_policy::reference);
+                g["woo"] = py::module_::import("woo_amd.woo");
+                g["__builtins__"] = py::module_::import("builtins");
+                py::exec(keep->cast<std::string>(), g);
+              };
+            } else {
+              r.source = py::str(py::repr(cmd)).cast<std::string>();
+              r.command = [keep](Scene&) { (*keep)(); };
+            }
           });
 
   // ---------------------------------------------------------------- Scene
@@ -462,30 +486,8 @@ PYBIND11_MODULE(_woo, m) {
         return f;
       })
       .def_property_readonly("lab", &labels)
-      .def("run",
-           [](py::object self, int64_t steps, bool /*wait*/) {
-             Scene& S = self.cast<Scene&>();
-             // PyRunner commands run with S (and woo) in their namespace
-             for (auto& e : S.engines) {
-               if (auto pr = std::dynamic_pointer_cast<PyRunner>(e)) {
-                 py::object pe = py::cast(pr);
-                 if (!py::hasattr(pe, "_command")) continue;
-                 py::object cmd = pe.attr("_command");
-                 if (py::isinstance<py::str>(cmd)) {
-                   py::dict g;
-                   g["S"] = self;
-                   g["woo"] = py::module_::import("woo_amd.woo");
-                   g["__builtins__"] = py::module_::import("builtins");
-                   std::string src = cmd.cast<std::string>();
-                   pr->command = [g, src]() { py::exec(src, g); };
-                 } else {
-                   pr->command = [cmd]() { cmd(); };
-                 }
-               }
-             }
-             S.run(steps);
-           },
-           py::arg("steps"), py::arg("wait") = true)
+      .def("run", [](Scene& S, int64_t steps, bool /*wait*/) { S.run(steps); }, py::arg("steps"),
+           py::arg("wait") = true)
       .def("one", [](py::object self) { self.attr("run")(1); })
       .def("wait", [](Scene&) {})
       .def("stop", &Scene::stop);

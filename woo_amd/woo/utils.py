@@ -126,3 +126,14 @@ def porosity(dem, box):
     """Solid fraction of spheres whose centres lie in box ((min), (max)) (Funcs.cpp:104-114;
     the reference's ``porosity`` returns Vs/V)."""
     return dem.observables(box[0], box[1])[2]
+
+
+def stress(S, volume=-1.):
+    """Cauchy stress tensor from the contacts (DemFuncs::stressStiffness, stress part);
+    volume <= 0 takes the periodic cell's volume."""
+    return S.dem.stress(volume)
+
+
+def kineticEnergy(S):
+    """Kinetic energy of the integrated nodes (DemData::getEk_any, spherical inertia)."""
+    return S.dem.kineticEnergy()
