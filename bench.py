@@ -127,7 +127,7 @@ def time_kernels(dev, wl, step, dt, k):
     return tc / k * 1e-3, tl / k * 1e-3, step
 
 
-def cpu_baseline(wl, n_sample_steps_max=20, budget_s=15.):
+def cpu_baseline(wl, n_sample_steps_max=2000, budget_s=12.):
     from oracle import Oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
@@ -201,6 +201,51 @@ def fp32_form(wl, args, value64, obs64):
                 observable_errors_vs_fp64=err)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary (FETCH_SIZE and
+    WRITE_SIZE, each scaled by what it reads for a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_1M_hertz_fp64.md")
+    try:
+        for line in open(path):
+            if line.startswith(f"{kernel} traffic_bytes_per_launch:"):
+                return float(line.split(":")[1]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
+def pmc_probe(dev, wl, args):
+    dt, cp = wl["dt"], contact_params()
+    step, _ = dev.run(ENGINES, args.warmup, 0, dt, contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
+    dev.sync()
+    st = dev.stats()
+    for _ in range(args.pmc_probe):
+        dev.collide(-.05)
+        dev.contact_loop(step, dt, **cp)
+        dev.leapfrog(dt, 0., True)
+        step += 1
+    dev.sync()
+    # calibration: a plain copy of known size (the runtime's device-to-device copy kernel)
+    L = C.CDLL("libamdhip64.so")
+    L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    L.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    L.hipFree.argtypes = [C.c_void_p]
+    nbytes = 512 << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert L.hipMalloc(C.byref(a), nbytes) == 0 and L.hipMalloc(C.byref(b), nbytes) == 0
+    assert L.hipMemset(a, 1, nbytes) == 0
+    assert L.hipDeviceSynchronize() == 0
+    assert L.hipMemcpy(b, a, nbytes, 3) == 0  # hipMemcpyDeviceToDevice
+    assert L.hipDeviceSynchronize() == 0
+    L.hipFree(a)
+    L.hipFree(b)
+    B = algorithmic_bytes(args.n, st["nCandidates"], st["nReal"])
+    print(json.dumps({"pmc_probe": args.pmc_probe, "copy_bytes_each_way": nbytes, "algorithmic_bytes": B,
+                      "contact_slots": st["nCandidates"], "real_contacts": st["nReal"]}), flush=True)
+    dev.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,6 +255,10 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=50)
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--fp32", type=int, default=1, help="also time the FP32 contact form")
+    ap.add_argument("--pmc-probe", type=int, default=0,
+                    help="for rocprofv3 --pmc runs: warm up, launch the contact loop and leapfrog this many "
+                         "times, then a device-to-device hipMemcpy of 512 MiB whose FETCH_SIZE/WRITE_SIZE "
+                         "calibrate the counters; prints no benchmark line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
@@ -223,6 +272,9 @@ def main():
     dt = wl["dt"]
     step = 0
     cp = contact_params()
+    if args.pmc_probe:
+        pmc_probe(dev, wl, args)
+        return
     log(f"dt={dt:.4e} s, warmup {args.warmup} steps")
     step, _ = dev.run(ENGINES, args.warmup, step, dt, contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
     dev.sync()
@@ -254,8 +306,9 @@ def main():
     log(f"kernel times: contact loop {tc * 1e6:.1f} us ({B['contact_loop'] / tc / 1e9:.0f} GB/s), "
         f"leapfrog+gather {tl * 1e6:.1f} us ({B['k_leapfrog'] / tl / 1e9:.0f} GB/s); slots {st2['nCandidates']}, "
         f"real {st2['nReal']}")
+    traffic, tsrc = measured_traffic(dom) if args.n == 1_000_000 else (None, None)
     roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
-                traffic=None, kernel=dom, bytes_per_launch=B[dom], launch_us=times[dom] * 1e6,
+                traffic=traffic, traffic_source=tsrc, kernel=dom, bytes_per_launch=B[dom], launch_us=times[dom] * 1e6,
                 other_kernels={k: dict(bytes_per_launch=B[k], launch_us=times[k] * 1e6,
                                        achieved_GBs=B[k] / times[k] / 1e9) for k in times if k != dom})
     base = None

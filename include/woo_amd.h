@@ -11,6 +11,7 @@
  *   wdg_contact_loop   ContactLoop::run                  src/dem/ContactLoop.cpp:93-268
  *                      (Cg2_Sphere_Sphere_L6Geom        src/dem/Sphere.cpp:92-117,
  *                       Cg2_Wall_Sphere_L6Geom          src/dem/Wall.cpp:50-84,
+ *                       Cg2_Facet_Sphere_L6Geom         src/dem/Facet.cpp:333-390,
  *                       Cp2_FrictMat_FrictPhys          src/dem/FrictMat.cpp:20-44,
  *                       Cp2_HertzMat_HertzPhys          src/dem/Hertz.cpp:34-77,
  *                       Law2_L6Geom_FrictPhys_IdealElPl src/dem/IdealElPl.cpp:15-125,
@@ -48,7 +49,7 @@ extern "C" {
 typedef struct wdg_ctx wdg_ctx;
 
 /* particle shapes */
-enum { WDG_SPHERE = 0, WDG_WALL = 1 };
+enum { WDG_SPHERE = 0, WDG_WALL = 1, WDG_FACET = 2 };
 /* material kinds */
 enum { WDG_FRICTMAT = 0, WDG_HERTZMAT = 1 };
 /* contact models */
@@ -80,7 +81,7 @@ typedef struct {
 /* bulk particle description, one node per particle; arrays of length n (x3 / x4 where noted) */
 typedef struct {
   int64_t n;
-  const int32_t* shape;       /* WDG_SPHERE | WDG_WALL */
+  const int32_t* shape;       /* WDG_SPHERE | WDG_WALL | WDG_FACET */
   const double* radius;       /* Sphere.radius (ignored for walls) */
   const int32_t* wallAxis;    /* Wall.axis (ignored for spheres); may be NULL if no walls */
   const int32_t* wallSense;   /* Wall.sense */
@@ -96,6 +97,12 @@ typedef struct {
   const double* torque;       /* x3, may be NULL */
   const double* mass;         /* x1 */
   const double* inertia;      /* x3 */
+  /* Facet (Facet.hpp): fixed triangles with static vertices; a facet's node position is its
+   * vertex 0 (pos is ignored for facets), its velocities must be zero, and it takes no force
+   * from the contact loop (multinodal, ContactLoop.cpp:270-276). Aperiodic scenes only. */
+  const double* facetVerts;   /* x9 (vertices 0,1,2), may be NULL if there are no facets */
+  const double* fakeVel;      /* x3, Facet.fakeVel, may be NULL (zero) */
+  const double* halfThick;    /* x1, Facet.halfThick, may be NULL (zero) */
 } wdg_particles;
 
 typedef struct {

@@ -147,6 +147,10 @@ struct Particle {
   double mass = 0, inertia[3] = {0, 0, 0};
   uint32_t flags = 0;
   bool integrated = false;  // member of DemField.nodes
+  // Facet: static vertices (node 0 is pos), Facet.fakeVel, Facet.halfThick
+  double fv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double fakeVel[3] = {0, 0, 0};
+  double halfThick = 0;
   // Aabb
   bool hasBound = false;
   double bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
@@ -263,10 +267,10 @@ struct orc_scene {
     if ((A.mask & B.mask & prm.loneMask) != 0) return false;
     return true;
   }
-  // canonical roles: functor order for mixed types (Wall before Sphere), min id first otherwise
+  // canonical roles: functor order for mixed types (Wall or Facet before Sphere), min id first otherwise
   void canonicalRoles(int a, int b, int& pA, int& pB) const {
     if (par[a].shape != par[b].shape) {
-      if (par[a].shape == ORC_WALL) { pA = a; pB = b; }
+      if (par[a].shape != ORC_SPHERE) { pA = a; pB = b; }
       else { pA = b; pB = a; }
     } else {
       pA = std::min(a, b); pB = std::max(a, b);
@@ -328,11 +332,22 @@ struct orc_scene {
     d2 = std::max(d2, sqn(sub(ld3(p.lastPos), ld3(p.pos))));
     return d2 + 0. > p.maxD2;
   }
-  // Bo1_Sphere_Aabb (Sphere.cpp:52-82), Bo1_Wall_Aabb (Wall.cpp:25-38), updateAabb (Collision.cpp:125-162)
+  // Bo1_Sphere_Aabb (Sphere.cpp:52-82), Bo1_Wall_Aabb (Wall.cpp:25-38), Bo1_Facet_Aabb (Facet.cpp:275-298,
+  // aperiodic), updateAabb (Collision.cpp:125-162)
   void updateAabb(Particle& p) {
     if (p.shape == ORC_SPHERE) {
       double hs = (prm.distFactor > 0 ? prm.distFactor : 1.) * p.radius;
       for (int k = 0; k < 3; k++) { p.bmin[k] = p.pos[k] - hs; p.bmax[k] = p.pos[k] + hs; }
+    } else if (p.shape == ORC_FACET) {
+      for (int k = 0; k < 3; k++) {
+        p.bmin[k] = p.fv[k] - p.halfThick;
+        p.bmax[k] = p.fv[k] + p.halfThick;
+      }
+      for (int i = 1; i < 3; i++)
+        for (int k = 0; k < 3; k++) {
+          p.bmin[k] = std::min(p.bmin[k], p.fv[3 * i + k] - p.halfThick);
+          p.bmax[k] = std::max(p.bmax[k], p.fv[3 * i + k] + p.halfThick);
+        }
     } else {
       for (int k = 0; k < 3; k++) { p.bmin[k] = -Inf; p.bmax[k] = Inf; }
       p.bmin[p.axis] = p.bmax[p.axis] = p.pos[p.axis];
@@ -442,7 +457,7 @@ struct orc_scene {
     int n = (int)par.size();
     std::vector<int> spheres, walls;
     for (int i = 0; i < n; i++) (par[i].shape == ORC_SPHERE ? spheres : walls).push_back(i);
-    // walls against every sphere
+    // walls and facets against every sphere
     for (int w : walls)
       for (int s : spheres) {
         if (!mayCollide(w, s)) continue;
@@ -722,6 +737,56 @@ struct orc_scene {
                       -radius, radius);
     return true;
   }
+  // CompUtils::closestSegmentPt (CompUtils.cpp:59-65)
+  static V3 closestSegmentPt(const V3& P, const V3& A, const V3& B) {
+    V3 BA = sub(B, A);
+    if (sqn(BA) == 0.) return A;
+    double u = (dot(P, BA) - dot(A, BA)) / sqn(BA);
+    return add(A, smul(std::min(1., std::max(0., u)), BA));
+  }
+  // Cg2_Facet_Sphere_L6Geom::go (Facet.cpp:333-390) for a facet with static vertices:
+  // Facet::getNormal (74-77), getOuterVectors (113-118), interpolatePtLinAngVel (161-177) with
+  // zero node velocities (linear velocity = fakeVel, or zero when fakeVel.x is NaN; angular
+  // velocity zero). Facet.n21lim is NaN (no neighbour angles), so adjustBoundaryContactGeom
+  // (220-265) returns at once and is not restated.
+  bool geomFacetSphere(Contact& c, const V3& shift2) const {
+    const Particle &f = par[c.pA], &s = par[c.pB];
+    const V3 v[3] = {ld3(f.fv), ld3(f.fv + 3), ld3(f.fv + 6)};
+    const V3 sC = add(ld3(s.pos), shift2);
+    const V3 nn = cross(sub(v[1], v[0]), sub(v[2], v[0]));
+    V3 fNormal = nn;
+    normalize(fNormal);
+    const double planeDist = dot(sub(sC, v[0]), fNormal);
+    if (std::fabs(planeDist) > (s.radius + f.halfThick) && !c.isReal()) return false;
+    const V3 fC = sub(sC, smul(planeDist, fNormal));
+    const V3 outVec[3] = {cross(sub(v[1], v[0]), nn), cross(sub(v[2], v[1]), nn), cross(sub(v[0], v[2]), nn)};
+    double ll[3];
+    for (int i = 0; i < 3; i++) ll[i] = dot(outVec[i], sub(fC, v[i]));
+    const int w = (ll[0] > 0 ? 1 : 0) + (ll[1] > 0 ? 2 : 0) + (ll[2] > 0 ? 4 : 0);
+    V3 contPt;
+    switch (w) {
+      case 0: contPt = fC; break;
+      case 1: contPt = closestSegmentPt(fC, v[0], v[1]); break;
+      case 2: contPt = closestSegmentPt(fC, v[1], v[2]); break;
+      case 4: contPt = closestSegmentPt(fC, v[2], v[0]); break;
+      case 3: contPt = v[1]; break;
+      case 5: contPt = v[0]; break;
+      case 6: contPt = v[2]; break;
+      default: throw std::logic_error("Cg2_Facet_Sphere_L6Geom: impossible sphere-facet intersection");
+    }
+    V3 normal = sub(sC, contPt);
+    if (sqn(normal) > pow2(s.radius + f.halfThick) && !c.isReal()) return false;
+    const double dist = nrm(normal);
+    if (dist != 0) normal = divs(normal, dist);
+    else normal = c.geom ? quatRot(c.nodeOri, mk(1, 0, 0)) : mk(1, 0, 0);
+    if (f.halfThick > 0) contPt = add(contPt, mul(normal, f.halfThick));
+    const double uN = (dist - s.radius) - f.halfThick;
+    V3 linVel = zero3();
+    if (!std::isnan(f.fakeVel[0])) linVel = ld3(f.fakeVel);
+    handleSpheresLike(c, contPt, linVel, zero3(), sC, ld3(s.vel), ld3(s.angVel), normal, contPt, uN,
+                      std::max(f.halfThick, s.radius), s.radius);
+    return true;
+  }
   // ---------------------------------------------------------- phys
   // Cp2_FrictMat_FrictPhys::updateFrictPhys (FrictMat.cpp:20-44)
   void physFrict(Contact& c) const {
@@ -854,8 +919,11 @@ struct orc_scene {
         if (!C.isReal() && !isFresh(C) && C.minDist00Sq > 0 &&
             sqn(sub(ld3(par[C.pA].pos), add(ld3(par[C.pB].pos), shift2))) > C.minDist00Sq)
           continue;
+        // geometry functor by shape pair (pB is always the sphere); other pairs have no functor
+        if (par[C.pB].shape != ORC_SPHERE) continue;
         bool geomOk;
         if (par[C.pA].shape == ORC_WALL) geomOk = geomWallSphere(C, shift2);
+        else if (par[C.pA].shape == ORC_FACET) geomOk = geomFacetSphere(C, shift2);
         else geomOk = geomSphereSphere(C, shift2);
         if (!geomOk) continue;
         if (!C.phys) {
@@ -872,6 +940,8 @@ struct orc_scene {
         }
         if (prm.applyForces && C.isReal() && !det) {
           for (int pid : {C.pA, C.pB}) {
+            // applyForceUninodal: multinodal particles (facets) get no force here
+            if (par[pid].shape == ORC_FACET) continue;
             V3 F, T;
             forceBranch(C, pid, F, T);
             std::lock_guard<std::mutex> lk(locks[pid]);
@@ -892,6 +962,7 @@ struct orc_scene {
 #pragma omp parallel for schedule(guided)
       for (long pid = 0; pid < (long)par.size(); pid++) {
         Particle& p = par[pid];
+        if (p.shape == ORC_FACET) continue;
         for (const auto& oc : p.contacts) {
           const Contact& C = con[oc.second];
           if (!C.isReal()) continue;
@@ -1108,6 +1179,25 @@ int orc_add_wall(orc_scene* s, const double pos[3], int axis, int sense, int mat
   return (int)s->par.size() - 1;
 }
 
+// woo.utils.facet (utils.py): fixed; Facet::lumpMassInertia gives nothing with halfThick=0 and
+// the nodes are fixed, so the facet has no mass of its own here
+int orc_add_facet(orc_scene* s, const double verts[9], const double fakeVel[3], double halfThick, int mat,
+                  uint32_t mask) {
+  Particle p;
+  p.shape = ORC_FACET;
+  p.mat = mat;
+  p.mask = mask;
+  for (int k = 0; k < 9; k++) p.fv[k] = verts[k];
+  for (int k = 0; k < 3; k++) p.fakeVel[k] = fakeVel ? fakeVel[k] : 0.;
+  p.halfThick = halfThick;
+  for (int k = 0; k < 3; k++) p.pos[k] = verts[k];
+  p.mass = 0;
+  p.flags = DOF_ALL;
+  p.integrated = false;
+  s->par.push_back(p);
+  return (int)s->par.size() - 1;
+}
+
 int orc_num_particles(const orc_scene* s) { return (int)s->par.size(); }
 
 int orc_get(const orc_scene* s, int field, double* out) {
@@ -1263,6 +1353,8 @@ int orc_stress(const orc_scene* s, double volume, double out[9]) {
     V3 posA = ld3(A.pos), posB = ld3(B.pos);
     if (s->prm.periodic)
       posA = sub(posA, mk(s->prm.cell[0] * C.cellDist[0], s->prm.cell[1] * C.cellDist[1], s->prm.cell[2] * C.cellDist[2]));
+    // multinodal (skipMultinodal=false): the contact point stands for the facet
+    if (A.shape == ORC_FACET) posA = C.nodePos;
     const double d0 = std::sqrt(sqn(sub(posA, posB)));
     const V3 n = quatRot(C.nodeOri, mk(1, 0, 0));
     const V3 F = quatRot(C.nodeOri, C.force);

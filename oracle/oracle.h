@@ -12,6 +12,9 @@
  *   src/dem/InsertionSortCollider.cpp:35-613 + .hpp:86-218  SAP with Verlet-enlarged AABBs
  *   src/dem/Sphere.cpp:26-117             mass/inertia, Bo1_Sphere_Aabb, Cg2_Sphere_Sphere_L6Geom
  *   src/dem/Wall.cpp:25-84                Bo1_Wall_Aabb, Cg2_Wall_Sphere_L6Geom
+ *   src/dem/Facet.cpp:74-118,161-177,275-298,333-390  Facet normal/outer vectors, velocity
+ *                                         interpolation, Bo1_Facet_Aabb, Cg2_Facet_Sphere_L6Geom
+ *   src/supp/base/CompUtils.cpp:59-65     closestSegmentPt
  *   src/dem/L6Geom.cpp:25-216             handleSpheresLikeContact
  *   src/dem/FrictMat.cpp:20-44            Cp2_FrictMat_FrictPhys
  *   src/dem/IdealElPl.cpp:15-125          Law2_L6Geom_FrictPhys_IdealElPl
@@ -33,7 +36,7 @@ extern "C" {
 
 typedef struct orc_scene orc_scene;
 
-enum { ORC_SPHERE = 0, ORC_WALL = 1 };
+enum { ORC_SPHERE = 0, ORC_WALL = 1, ORC_FACET = 2 };
 enum { ORC_FRICTMAT = 0, ORC_HERTZMAT = 1 };
 enum { ORC_LAW_IDEALELPL = 0, ORC_LAW_HERTZDMT = 1 };
 enum { ORC_PHYS_FRICT = 0, ORC_PHYS_HERTZ = 1 };
@@ -84,6 +87,10 @@ int orc_add_material(orc_scene* s, int kind, double density, double young, doubl
 /* returns particle id */
 int orc_add_sphere(orc_scene* s, const double pos[3], double radius, int mat, int fixed, uint32_t mask);
 int orc_add_wall(orc_scene* s, const double pos[3], int axis, int sense, int mat, uint32_t mask);
+/* fixed facet (3 static vertices, verts 9 doubles); fakeVel may be NULL (zero); the
+ * particle's node position is vertex 0 */
+int orc_add_facet(orc_scene* s, const double verts[9], const double fakeVel[3], double halfThick, int mat,
+                  uint32_t mask);
 /* bulk: n free spheres (pos n*3, radius n), one material and mask; returns the first id */
 int orc_add_spheres(orc_scene* s, int64_t n, const double* pos, const double* radius, int mat, uint32_t mask);
 

@@ -52,6 +52,7 @@ def _load():
     L.orc_add_material.argtypes = [P, C.c_int] + [C.c_double] * 6; L.orc_add_material.restype = C.c_int
     L.orc_add_sphere.argtypes = [P, D, C.c_double, C.c_int, C.c_int, C.c_uint32]; L.orc_add_sphere.restype = C.c_int
     L.orc_add_wall.argtypes = [P, D, C.c_int, C.c_int, C.c_int, C.c_uint32]; L.orc_add_wall.restype = C.c_int
+    L.orc_add_facet.argtypes = [P, D, D, C.c_double, C.c_int, C.c_uint32]; L.orc_add_facet.restype = C.c_int
     L.orc_add_spheres.argtypes = [P, C.c_int64, D, D, C.c_int, C.c_uint32]; L.orc_add_spheres.restype = C.c_int
     L.orc_num_particles.argtypes = [P]; L.orc_num_particles.restype = C.c_int
     L.orc_get.argtypes = [P, C.c_int, D]; L.orc_get.restype = C.c_int
@@ -125,6 +126,11 @@ class Oracle:
     def wall(self, pos, axis, sense=0, mat=0, mask=0b0011):
         a = np.ascontiguousarray(pos, dtype=np.float64)
         return self.L.orc_add_wall(self.h, _dp(a), axis, sense, mat, mask)
+
+    def facet(self, verts, fakeVel=None, halfThick=0., mat=0, mask=0b0011):
+        v = np.ascontiguousarray(np.asarray(verts, dtype=np.float64).reshape(9))
+        fv = None if fakeVel is None else np.ascontiguousarray(np.asarray(fakeVel, dtype=np.float64).reshape(3))
+        return self.L.orc_add_facet(self.h, _dp(v), None if fv is None else _dp(fv), float(halfThick), mat, mask)
 
     @property
     def n(self):

@@ -150,12 +150,82 @@ def periodic_bidisperse(n_side=18, r_small=1e-3, r_big=6e-3, n_big=3, seed=4, rh
         dt=0.3 * pwave, physKind=1, lawKind=1, damping=0., poisson=poisson, en=en, box=L)
 
 
+def drum(n=2000, seed=11, R=0.04, length=0.03, div=64, omega=2., rmin=1.99e-3, rmax=2e-3, pitch=3.99e-3, rho=3000.,
+         young=1e8, tanPhi=.4, poisson=.25, en=.6, vel_sigma=0.05, caps=True, halfThick=0.):
+    """Rotating drum (woo.triangulated.cylinder with fakeVel, triangulated.py:11-79): axis x,
+    2*div facets spanning the length, fakeVel = -R*omega*(n x axis), end walls; spheres on a
+    lattice in the lower part, the outermost pushed onto the facets so that facet contacts
+    exist from the first step. Hertz law, gravity -z."""
+    rng = np.random.default_rng(seed)
+    ax = np.array([1., 0., 0.])
+    th = np.linspace(0, 2 * math.pi, div + 1)
+    facets = []
+    for k in range(div):
+        y0, z0, y1, z1 = R * math.cos(th[k]), R * math.sin(th[k]), R * math.cos(th[k + 1]), R * math.sin(th[k + 1])
+        A0, A1 = (-length / 2, y0, z0), (-length / 2, y1, z1)
+        B0, B1 = (length / 2, y0, z0), (length / 2, y1, z1)
+        for tri in ((A0, B0, B1), (A0, B1, A1)):
+            v = np.array(tri)
+            nrm = np.cross(v[1] - v[0], v[2] - v[0])
+            nrm /= np.linalg.norm(nrm)
+            facets.append(dict(verts=v, fakeVel=-R * omega * np.cross(nrm, ax), halfThick=halfThick))
+    Rin = R * math.cos(math.pi / div)
+    a = pitch
+    xs = np.arange(-length / 2 + rmax * 1.05, length / 2 - rmax * 1.05 + 1e-12, a)
+    ys = np.arange(-Rin, Rin + 1e-12, a)
+    g = np.array([(x, y, z) for z in ys for y in ys for x in xs if math.hypot(y, z) < Rin - 0.6 * rmax])
+    g = g[np.argsort(g[:, 2], kind="stable")][:n]
+    n = len(g)
+    radii = np.exp(rng.uniform(math.log(rmin), math.log(rmax), size=n))
+    pos = g + rng.uniform(-0.02 * rmin, 0.02 * rmin, size=(n, 3))
+    # spheres near the mantle moved out until they overlap the plane of their facet by 0.1 % of r
+    rho_ax = np.hypot(pos[:, 1], pos[:, 2])
+    phi = np.arctan2(pos[:, 2], pos[:, 1]) % (2 * math.pi)
+    dphi = phi - (np.floor(phi / (2 * math.pi / div)) + 0.5) * (2 * math.pi / div)
+    out = rho_ax * np.cos(dphi) > Rin - 2 * radii - halfThick
+    scale = (Rin - 0.999 * radii[out] - halfThick) / np.cos(dphi[out]) / rho_ax[out]
+    pos[out, 1] *= scale
+    pos[out, 2] *= scale
+    vel = rng.normal(0, vel_sigma, size=(n, 3))
+    mass, inertia = sphere_mass_inertia(radii, rho)
+    walls = []
+    if caps:
+        walls = [dict(pos=np.array([-length / 2, 0., 0.]), axis=0, sense=1),
+                 dict(pos=np.array([length / 2, 0., 0.]), axis=0, sense=-1)]
+    pwave = rmin / math.sqrt(young / rho)
+    return dict(
+        materials=[dict(kind=1, density=rho, young=young, ktDivKn=.2, tanPhi=tanPhi, surfEnergy=0., alpha=0.)],
+        pos=pos, radius=radii, vel=vel, mass=mass, inertia=np.repeat(inertia[:, None], 3, 1),
+        walls=walls, facets=facets, walls_first=True, gravity=(0., 0., -9.81), periodic=False, cell=(1., 1., 1.),
+        dt=0.3 * pwave, physKind=1, lawKind=1, damping=0., poisson=poisson, en=en, box=2 * R)
+
+
+def facet_box(n=800, seed=12, **kw):
+    """deposition() with each of the 6 walls replaced by two facets of the box face"""
+    sc = deposition(n=n, seed=seed, **kw)
+    b = sc["box"]
+    c = [np.array([x, y, z]) for z in (0., b) for y in (0., b) for x in (0., b)]  # corner index x + 2y + 4z
+    faces = [(0, 2, 6, 4), (1, 3, 7, 5), (0, 1, 5, 4), (2, 3, 7, 6), (0, 1, 3, 2), (4, 5, 7, 6)]
+    facets = []
+    for q in faces:
+        v = [c[i] for i in q]
+        facets.append(dict(verts=np.array([v[0], v[1], v[2]])))
+        facets.append(dict(verts=np.array([v[0], v[2], v[3]])))
+    sc["walls"] = []
+    sc["facets"] = facets
+    return sc
+
+
 def _particle_order(sc):
-    """list of ('s', i) / ('w', j) in id order"""
+    """list of ('s', i) / ('w', j) / ('f', k) in id order"""
     n = len(sc["radius"])
     sph = [("s", i) for i in range(n)]
-    wal = [("w", j) for j in range(len(sc["walls"]))]
+    wal = [("w", j) for j in range(len(sc["walls"]))] + [("f", k) for k in range(len(sc.get("facets", [])))]
     return wal + sph if sc["walls_first"] else sph + wal
+
+
+def n_boundary(sc):
+    return len(sc["walls"]) + len(sc.get("facets", []))
 
 
 def build_oracle(sc, **params):
@@ -171,9 +241,12 @@ def build_oracle(sc, **params):
     for kind, i in _particle_order(sc):
         if kind == "s":
             o.sphere(sc["pos"][i], sc["radius"][i], 0, False, MOVABLE_MASK)
-        else:
+        elif kind == "w":
             w = sc["walls"][i]
             o.wall(w["pos"], w["axis"], w["sense"], 0, BOUNDARY_MASK)
+        else:
+            f = sc["facets"][i]
+            o.facet(f["verts"], f.get("fakeVel"), f.get("halfThick", 0.), 0, BOUNDARY_MASK)
     order = _particle_order(sc)
     o.set("vel", _gather(sc, order, "vel", 3))
     o.set("mass", _gather(sc, order, "mass", 1))
@@ -192,8 +265,11 @@ def _gather(sc, order, key, w):
 def gpu_arrays(sc):
     order = _particle_order(sc)
     n = len(order)
-    shape = np.array([0 if k == "s" else 1 for k, _ in order], dtype=np.int32)
+    shape = np.array([{"s": 0, "w": 1, "f": 2}[k] for k, _ in order], dtype=np.int32)
     pos = np.zeros((n, 3))
+    fverts = np.zeros((n, 9))
+    fvel = np.zeros((n, 3))
+    fht = np.zeros(n)
     radius = np.zeros(n)
     axis = np.zeros(n, dtype=np.int32)
     sense = np.zeros(n, dtype=np.int32)
@@ -206,16 +282,28 @@ def gpu_arrays(sc):
             radius[k] = sc["radius"][i]
             mask[k] = MOVABLE_MASK
             integ[k] = 1
-        else:
+        elif kind == "w":
             w = sc["walls"][i]
             pos[k] = w["pos"]
             axis[k] = w["axis"]
             sense[k] = w["sense"]
             mask[k] = BOUNDARY_MASK
             flags[k] = DOF_ALL
-    return dict(shape=shape, pos=pos, radius=radius, wallAxis=axis, wallSense=sense, material=np.zeros(n, np.int32),
-                mask=mask, flags=flags, integrated=integ, vel=_gather(sc, order, "vel", 3),
-                mass=_gather(sc, order, "mass", 1), inertia=_gather(sc, order, "inertia", 3))
+        else:
+            f = sc["facets"][i]
+            fverts[k] = np.asarray(f["verts"], dtype=float).reshape(9)
+            pos[k] = fverts[k, :3]
+            if f.get("fakeVel") is not None:
+                fvel[k] = f["fakeVel"]
+            fht[k] = f.get("halfThick", 0.)
+            mask[k] = BOUNDARY_MASK
+            flags[k] = DOF_ALL
+    out = dict(shape=shape, pos=pos, radius=radius, wallAxis=axis, wallSense=sense, material=np.zeros(n, np.int32),
+               mask=mask, flags=flags, integrated=integ, vel=_gather(sc, order, "vel", 3),
+               mass=_gather(sc, order, "mass", 1), inertia=_gather(sc, order, "inertia", 3))
+    if sc.get("facets"):
+        out.update(facetVerts=fverts, fakeVel=fvel, halfThick=fht)
+    return out
 
 
 def build_gpu(sc, device=0):

@@ -105,3 +105,35 @@ def test_precision_switch_restarts_contacts():
     assert st["nReal"] > 0 and n64 > 0
     with pytest.raises(Exception):
         g.d.set_precision(16)
+
+
+def test_fp32_drum_facet_contacts_close_to_fp64():
+    # facet-sphere geometry in float on positions relative to the facet's vertex 0
+    from tests.scenes import drum, n_boundary
+
+    sc = drum(n=2000, seed=11)
+    g64, g32 = GpuRunner(sc), GpuRunner(sc)
+    g32.d.set_precision(32)
+    for g in (g64, g32):
+        g.run(ENG, 20)
+    for g in (g64, g32):
+        g.run(["Collider", "ContactLoop"])
+    nb = n_boundary(sc)
+    i64, _, d64 = g64.d.contacts(real_only=True)
+    i32, _, d32 = g32.d.contacts(real_only=True)
+    k64 = {tuple(x): j for j, x in enumerate(i64.tolist())}
+    rows = [(k64[tuple(x)], j) for j, x in enumerate(i32.tolist()) if tuple(x) in k64 and 2 <= x[0] < nb]
+    assert len(rows) > 20
+    a = np.array([r[0] for r in rows])
+    b = np.array([r[1] for r in rows])
+    # normal, contact point and force of the facet contacts
+    nerr = np.max(np.abs(d32[b, 0:3] - d64[a, 0:3]))
+    perr = np.max(np.abs(d32[b, 6:9] - d64[a, 6:9]))
+    fscale = np.max(np.abs(d64[a, 9]))
+    ferr = np.max(np.abs(d32[b, 9:12] - d64[a, 9:12])) / fscale
+    print("facet contacts", len(rows), "normal err", nerr, "contact point err", perr, "force err / max", ferr)
+    assert nerr < 1e-4 and perr < 1e-6 and ferr < 1e-2
+    # trajectories stay close over the short run
+    dp = np.max(np.linalg.norm(g32.d.get("pos")[nb:] - g64.d.get("pos")[nb:], axis=1) / sc["radius"])
+    print("max position deviation / r", dp)
+    assert dp < 1e-3

@@ -13,7 +13,8 @@ import math
 import numpy as np
 import pytest
 
-from tests.scenes import GpuRunner, build_oracle, deposition, periodic_bidisperse, periodic_hertz
+from tests.scenes import (GpuRunner, build_oracle, deposition, drum, facet_box, n_boundary, periodic_bidisperse,
+                          periodic_hertz)
 
 pytestmark = pytest.mark.gpu
 
@@ -56,8 +57,8 @@ def force_metric(o, g):
 
 def traj_metric(o, g, sc, nw_first):
     Po, Pg = o.get("pos"), g.d.get("pos")
-    r = np.concatenate([np.ones(len(sc["walls"])), sc["radius"]]) if nw_first else np.concatenate(
-        [sc["radius"], np.ones(len(sc["walls"]))])
+    nb = n_boundary(sc)
+    r = np.concatenate([np.ones(nb), sc["radius"]]) if nw_first else np.concatenate([sc["radius"], np.ones(nb)])
     return np.max(np.linalg.norm(Pg - Po, axis=1) / r), int(np.sum(np.any(Pg != Po, axis=1)))
 
 
@@ -322,3 +323,50 @@ def test_periodic_celldist_cases():
         ids, cd, _ = g.d.contacts()
         assert ids.tolist() == [[0, 1]]
         assert cd.tolist() == [[-s for s in shift2]]
+
+
+# ---------------------------------------------------------------- facets
+def test_broadphase_pairs_drum_facets():
+    # facet AABBs (vertices +- halfThick, Verlet-enlarged) against sphere AABBs: the potential
+    # set equals the oracle's and the brute-force overlap set
+    sc = drum(n=1500, seed=13, halfThick=2e-4)
+    o = build_oracle(sc)
+    g = GpuRunner(sc)
+    o.run(["Collider"])
+    g.run(["Collider"])
+    compare_contact_sets(o, g, real_only=False)
+    ids = g.d.contacts(data=False)[0]
+    nb = n_boundary(sc)
+    print("candidates", len(ids), "with a facet", int(np.sum((ids[:, 0] >= 2) & (ids[:, 0] < nb))))
+    assert int(np.sum((ids[:, 0] >= 2) & (ids[:, 0] < nb))) > 100
+    assert _sets(o.brute_aabb_pairs()) == _sets(ids)
+
+
+@pytest.mark.parametrize("halfThick", [0., 3e-4])
+def test_trajectory_drum_facets(halfThick):
+    # rotating drum (fakeVel) with end walls, Hertz: sets, trajectories and forces as the oracle
+    sc = drum(n=2000, seed=11, halfThick=halfThick)
+    o, g, worst = _run_both(sc, ENG, 200, check_every=20)
+    ids = o.contacts(real_only=True, data=False)[0]
+    nb = n_boundary(sc)
+    nf = int(np.sum((ids[:, 0] >= 2) & (ids[:, 0] < nb)))
+    ef, et, nd = force_metric(o, g)
+    print("drum real contacts", len(ids), "facet contacts", nf, "worst position err", worst, "force", ef, et)
+    assert nf > 20
+    assert worst <= TRAJ_TOL
+    assert ef <= FORCE_TOL and et <= FORCE_TOL
+    # facets take no force from the contact loop
+    F = g.d.get("force")
+    assert np.all(F[2:nb] == 0.)
+
+
+def test_facet_box_deposition_matches_oracle():
+    sc = facet_box(n=800, dense=True, seed=9, vel_sigma=0.05)
+    o, g, worst = _run_both(sc, ENG, 100)
+    ids = o.contacts(real_only=True, data=False)[0]
+    nf = int(np.sum(ids[:, 0] >= len(sc["radius"])))  # (pA, pB): the facet is pA
+    print("facet box: real", len(ids), "facet contacts", nf, "worst", worst)
+    assert nf > 10
+    assert worst <= TRAJ_TOL
+    ef, et, _ = force_metric(o, g)
+    assert ef <= FORCE_TOL and et <= FORCE_TOL

@@ -349,3 +349,101 @@ def test_column_contact_force_sign():
     k = pairs.index([4, 5])
     # global normal points from pA to pB (+z); local normal force is compressive (negative)
     assert d[k, 2] > 0.999 and d[k, 9] < 0
+
+
+# ---------------------------------------------------------------- facets
+def _floor_square(o, m, half=1., fakeVel=None, z=0.):
+    # two facets tiling [-half, half]^2 in the plane z
+    a, b, c, d = (-half, -half, z), (half, -half, z), (half, half, z), (-half, half, z)
+    o.facet([a, b, c], fakeVel, 0., m)
+    o.facet([a, c, d], fakeVel, 0., m)
+
+
+@pytest.mark.parametrize("law", ["idealelpl", "hertz"])
+def test_facet_face_contact_equals_wall(law):
+    # a sphere in the face region of a facet sees the same contact as on a wall in that plane:
+    # normal and uN from the plane, lens (r, r), contA pi r^2, force on the sphere only
+    runs = []
+    for kind in ("wall", "facet"):
+        if law == "hertz":
+            o = Oracle(physKind=1, lawKind=1, poisson=.25, en=.6, damping=0., gravity=(0, 0, -9.81))
+            m = o.material(kind=1, density=2500., young=1e7, ktDivKn=.2, tanPhi=.4)
+        else:
+            o = Oracle(damping=0., gravity=(0, 0, -9.81))
+            m = o.material(density=2500., young=1e7, ktDivKn=.2, tanPhi=.4)
+        if kind == "wall":
+            o.wall((0, 0, 0), 2, 1, m)
+            o.wall((0, 0, 5), 2, -1, m, mask=0)  # keeps the particle count and ids equal
+        else:
+            _floor_square(o, m)
+        r = .01
+        o.sphere((0.01, -0.02, r * (1 - 1e-4)), r, m)
+        o.set("vel", np.array([[0, 0, 0], [0, 0, 0], [0.2, 0.05, 0.]]))
+        o.set_params(dt=.2 * o.pwave_dt())
+        o.run(ENG, 3000)
+        runs.append((o.get("pos")[2], o.get("vel")[2], o.get("angVel")[2], len(o.contacts(real_only=True)[0])))
+    (pw, vw, ww, nw), (pf, vf, wf, nf) = runs
+    assert nw == 1 and nf == 1
+    assert np.max(np.abs(pf - pw)) <= 1e-12 * .01
+    assert np.max(np.abs(vf - vw)) <= 1e-9 * np.max(np.abs(vw))
+    assert np.max(np.abs(wf - ww)) <= 1e-9 * np.max(np.abs(ww))
+
+
+def test_facet_fakevel_drags_resting_sphere_to_two_sevenths():
+    # Facet.fakeVel u moves the surface under a sphere at rest: friction accelerates it until the
+    # contact point moves with the surface (v - w_y r = u), after which it rolls at 2/7 u
+    u, r, rho, E, tanPhi = .1, .01, 2500., 1e7, .5
+    o = Oracle(damping=0., gravity=(0, 0, -9.81))
+    m = o.material(density=rho, young=E, ktDivKn=.2, tanPhi=tanPhi)
+    _floor_square(o, m, fakeVel=(u, 0, 0))
+    mass = (4 / 3.) * math.pi * r ** 3 * rho
+    o.sphere((0, 0, r - mass * 9.81 / (E * math.pi * r / 2)), r, m)
+    o.set_params(dt=.2 * o.pwave_dt())
+    t_slip = 2 / 7. * u / (tanPhi * 9.81)
+    o.run(ENG, int(3 * t_slip / o.dt))
+    v, w = o.get("vel")[2], o.get("angVel")[2]
+    assert v[0] == pytest.approx(2 / 7. * u, rel=0.01)
+    assert v[0] - w[1] * r == pytest.approx(u, rel=0.01)
+    # the facets are fixed and get no force from the contact loop (multinodal particles)
+    assert np.all(o.get("force")[:2] == 0.)
+
+
+@pytest.mark.parametrize("centre,closest", [
+    ((0.3, 0.2, 0.006), (0.3, 0.2, 0.)),        # face
+    ((0.5, -0.003, 0.002), (0.5, 0., 0.)),       # edge v0-v1
+    ((0.5, 0.502, -0.001), (0.499, 0.501, 0.)),  # hypotenuse v1-v2, below the plane
+    ((-0.003, 0.4, 0.001), (0., 0.4, 0.)),       # edge v2-v0
+    ((-0.002, -0.002, 0.001), (0., 0., 0.)),     # vertex v0
+    ((1.002, -0.001, 0.002), (1., 0., 0.)),      # vertex v1
+    ((-0.001, 1.003, -0.002), (0., 1., 0.)),     # vertex v2
+])
+def test_facet_voronoi_regions(centre, closest):
+    # closest point of the triangle by region, normal from it to the centre, uN = |d| - r
+    r = .008
+    o = Oracle(damping=0., gravity=(0, 0, 0))
+    m = o.material(density=2500., young=1e7, ktDivKn=.2, tanPhi=.4)
+    o.facet([(0, 0, 0), (1, 0, 0), (0, 1, 0)], None, 0., m)
+    o.sphere(centre, r, m)
+    o.run(["Collider", "ContactLoop"], 1)
+    ids, _, d = o.contacts(real_only=True)
+    assert ids.tolist() == [[0, 1]]
+    dvec = np.array(centre) - np.array(closest)
+    dist = np.linalg.norm(dvec)
+    np.testing.assert_allclose(d[0, 0:3], dvec / dist, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(d[0, 6:9], closest, rtol=0, atol=1e-15)
+    assert d[0, 12] == pytest.approx(dist - r, rel=1e-12)
+
+
+def test_facet_halfthick_shifts_contact():
+    # halfThick thickens the facet on both sides: uN = |d| - r - h, contact point moved by h
+    r, h = .008, .001
+    o = Oracle(damping=0., gravity=(0, 0, 0))
+    m = o.material(density=2500., young=1e7, ktDivKn=.2, tanPhi=.4)
+    o.facet([(0, 0, 0), (1, 0, 0), (0, 1, 0)], None, h, m)
+    o.sphere((0.3, 0.2, -0.0085), r, m)
+    o.run(["Collider", "ContactLoop"], 1)
+    ids, _, d = o.contacts(real_only=True)
+    assert ids.tolist() == [[0, 1]]
+    np.testing.assert_allclose(d[0, 0:3], (0, 0, -1), atol=1e-15)
+    np.testing.assert_allclose(d[0, 6:9], (0.3, 0.2, -h), atol=1e-15)
+    assert d[0, 12] == pytest.approx(0.0085 - r - h, rel=1e-10)

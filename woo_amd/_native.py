@@ -10,7 +10,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libwoo_amd.so")
 
-SPHERE, WALL = 0, 1
+SPHERE, WALL, FACET = 0, 1, 2
 FRICTMAT, HERTZMAT = 0, 1
 PHYS_FRICT, PHYS_HERTZ = 0, 1
 LAW_IDEALELPL, LAW_HERTZDMT = 0, 1
@@ -47,7 +47,8 @@ class Particles(C.Structure):
                 ("mask", _P(C.c_uint32)), ("flags", _P(C.c_uint32)), ("integrated", _P(C.c_uint8)),
                 ("pos", _P(C.c_double)), ("vel", _P(C.c_double)), ("angVel", _P(C.c_double)),
                 ("ori", _P(C.c_double)), ("force", _P(C.c_double)), ("torque", _P(C.c_double)),
-                ("mass", _P(C.c_double)), ("inertia", _P(C.c_double))]
+                ("mass", _P(C.c_double)), ("inertia", _P(C.c_double)), ("facetVerts", _P(C.c_double)),
+                ("fakeVel", _P(C.c_double)), ("halfThick", _P(C.c_double))]
 
 
 class CollideParams(C.Structure):
@@ -171,7 +172,7 @@ class Device:
 
     def upload_particles(self, shape, pos, mass, inertia, radius=None, wallAxis=None, wallSense=None, material=None,
                          mask=None, flags=None, integrated=None, vel=None, angVel=None, ori=None, force=None,
-                         torque=None):
+                         torque=None, facetVerts=None, fakeVel=None, halfThick=None):
         n = len(shape)
         keep = []
 
@@ -202,6 +203,9 @@ class Device:
         p.torque = dptr(arr(torque, np.float64, 3))
         p.mass = dptr(arr(mass, np.float64))
         p.inertia = dptr(arr(inertia, np.float64, 3))
+        p.facetVerts = dptr(arr(facetVerts, np.float64, 9))
+        p.fakeVel = dptr(arr(fakeVel, np.float64, 3))
+        p.halfThick = dptr(arr(halfThick, np.float64))
         check(self.L.wdg_upload_particles(self.h, C.byref(p)))
         self.n = n
 

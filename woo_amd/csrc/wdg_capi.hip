@@ -36,6 +36,7 @@ struct ExportArgs {
   const double *px, *py, *pz;
   const int32_t* mat;
   const wdg_material* mats;
+  const double* fgeo;
   int hertz;
   double poisson, en, viscAlpha;
   double* out;
@@ -56,8 +57,7 @@ __global__ void k_export(ExportArgs a) {
   const bool swap = (inf & CI_SWAP) != 0u;
   const uint32_t pA = swap ? a.B[s] : a.A[s], pB = swap ? a.A[s] : a.B[s];
   double r1, r2;
-  if ((inf & CI_TYPE_MASK) == CT_SS) { r1 = a.rad[pA]; r2 = a.rad[pB]; }
-  else { r1 = -a.rad[pB]; r2 = a.rad[pB]; }
+  contact_radii(inf, pA, pB, a.rad, a.fgeo, r1, r2);
   double l0, l1, A;
   lens_area(r1, r2, l0, l1, A);
   const wdg_material mA = a.mats[a.mat[pA]], mB = a.mats[a.mat[pB]];
@@ -189,7 +189,7 @@ static std::vector<Record> collect_contacts(Ctx& c, int which, std::vector<doubl
       a.N0 = RS.hN[0].p; a.N1 = RS.hN[1].p; a.N2 = RS.hN[2].p;
       a.Y0 = RS.hY[0].p; a.Y1 = RS.hY[1].p; a.Y2 = RS.hY[2].p;
       a.F0 = RS.hF[0].p; a.F1 = RS.hF[1].p; a.F2 = RS.hF[2].p; a.uN = RS.huN.p; a.gp = RS.gp.p;
-      a.rad = c.radius.p; a.mass = c.mass.p; a.mat = c.mat.p; a.mats = c.dMats.p;
+      a.rad = c.radius.p; a.mass = c.mass.p; a.mat = c.mat.p; a.mats = c.dMats.p; a.fgeo = c.fgeo.p;
       a.px = c.pos[0].p; a.py = c.pos[1].p; a.pz = c.pos[2].p;
       a.hertz = c.lastPhysKind == WDG_PHYS_HERTZ;
       a.poisson = c.lastPoisson; a.en = c.lastEn; a.viscAlpha = c.lastViscAlpha;
@@ -287,6 +287,8 @@ int wdg_ctx_destroy(wdg_ctx* ctx) {
     c.perm.release(); c.invPerm.release(); c.cellStart.release(); c.cellEnd.release();
     c.scount.release(); c.soff.release(); c.stPartner.release(); c.stInfo.release(); c.stDist.release();
     c.adjEnt.release(); c.adjDeg.release(); c.stNb.release(); c.sdeg.release(); c.posf.release();
+    c.fgeo.release(); c.bnd.release(); c.bigOff.release(); c.bigRank.release(); c.bigVal.release();
+    c.bigValTmp.release(); c.bigDest.release(); c.bigKey.release(); c.bigKeyTmp.release();
     c.sortHist.release(); c.scanAux.release(); c.red.release(); c.xfer.release(); c.dflags.release();
     if (c.hflags) (void)hipHostFree(c.hflags);
     if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -333,13 +335,15 @@ int wdg_upload_particles(wdg_ctx* ctx, const wdg_particles* p) {
     const int64_t n = p->n;
     if (n >= (int64_t)1 << 31) throw Error(WDG_E_CAPACITY, "too many particles");
     WDG_HIP(hipStreamSynchronize(c.stream));
-    // storage order: walls (user-id order), then spheres (user-id order until the first refresh)
+    // storage order: boundary bodies (walls, facets; user-id order), then spheres (user-id
+    // order until the first refresh)
     std::vector<uint32_t> order;
     for (int pass = 0; pass < 2; pass++)
       for (int64_t i = 0; i < n; i++) {
         int shape = p->shape[i];
-        if (shape != WDG_SPHERE && shape != WDG_WALL) throw Error(WDG_E_ARG, "unsupported shape " + std::to_string(shape));
-        if ((pass == 0) == (shape == WDG_WALL)) order.push_back((uint32_t)i);
+        if (shape != WDG_SPHERE && shape != WDG_WALL && shape != WDG_FACET)
+          throw Error(WDG_E_ARG, "unsupported shape " + std::to_string(shape));
+        if ((pass == 0) == (shape != WDG_SPHERE)) order.push_back((uint32_t)i);
       }
     std::vector<uint32_t> flags((size_t)n), mask((size_t)n), uid((size_t)n), sto((size_t)n);
     std::vector<int32_t> mat((size_t)n);
@@ -347,7 +351,8 @@ int wdg_upload_particles(wdg_ctx* ctx, const wdg_particles* p) {
     for (auto& v : col) v.resize((size_t)n);
     c.rmin = INFINITY;
     c.rmax = 0;
-    int64_t nW = 0;
+    int64_t nW = 0, nF = 0;
+    std::vector<double> fgeo;
     for (int64_t s = 0; s < n; s++) {
       const int64_t i = order[s];
       const int shape = p->shape[i];
@@ -363,6 +368,22 @@ int wdg_upload_particles(wdg_ctx* ctx, const wdg_particles* p) {
         f |= (uint32_t)ax << F_AXIS_SHIFT;
         f |= (uint32_t)(se + 1) << F_SENSE_SHIFT;
         nW++;
+      } else if (shape == WDG_FACET) {
+        if (!p->facetVerts) throw Error(WDG_E_ARG, "facets need facetVerts");
+        for (int k = 0; k < 3; k++)
+          if ((p->vel && p->vel[3 * i + k] != 0.) || (p->angVel && p->angVel[3 * i + k] != 0.))
+            throw Error(WDG_E_ARG, "Facet #" + std::to_string(i) + ": moving facet nodes are not supported (use fakeVel)");
+        const double ht = p->halfThick ? p->halfThick[i] : 0.;
+        if (!(ht >= 0.)) throw Error(WDG_E_ARG, "Facet #" + std::to_string(i) + ": halfThick must be >= 0");
+        // fixed, outside DemField.nodes: the vertices never move
+        f = (f & ~(uint32_t)F_INTEGRATED) | F_DOF_ALL;
+        fgeo.resize((size_t)FGEO * (nW + 1), 0.);
+        double* g = fgeo.data() + (size_t)FGEO * nW;
+        for (int k = 0; k < 9; k++) g[k] = p->facetVerts[9 * i + k];
+        for (int k = 0; k < 3; k++) g[9 + k] = p->fakeVel ? p->fakeVel[3 * i + k] : 0.;
+        g[12] = ht;
+        nW++;
+        nF++;
       } else {
         r = p->radius ? p->radius[i] : 0.;
         if (!(r > 0)) throw Error(WDG_E_ARG, "Sphere #" + std::to_string(i) + ": radius must be positive");
@@ -370,7 +391,7 @@ int wdg_upload_particles(wdg_ctx* ctx, const wdg_particles* p) {
         c.rmax = std::max(c.rmax, r);
       }
       flags[s] = f;
-      mask[s] = p->mask ? p->mask[i] : (shape == WDG_WALL ? 3u : 5u);
+      mask[s] = p->mask ? p->mask[i] : (shape != WDG_SPHERE ? 3u : 5u);
       mat[s] = m;
       uid[s] = (uint32_t)i;
       sto[i] = (uint32_t)s;
@@ -386,9 +407,16 @@ int wdg_upload_particles(wdg_ctx* ctx, const wdg_particles* p) {
       for (int k = 0; k < 4; k++) col[9 + k][s] = p->ori ? p->ori[4 * i + k] : (k == 0 ? 1. : 0.);
       col[22][s] = p->mass[i];
       col[23][s] = r;
+      if (shape == WDG_FACET)
+        for (int k = 0; k < 3; k++) col[k][s] = p->facetVerts[9 * i + k];  // the node is vertex 0
     }
     c.n = n;
     c.nW = nW;
+    c.nFacet = nF;
+    fgeo.resize((size_t)FGEO * (nW + 1), 0.);
+    c.fgeo.ensure(fgeo.size());
+    WDG_HIP(hipMemcpy(c.fgeo.p, fgeo.data(), fgeo.size() * sizeof(double), hipMemcpyHostToDevice));
+    c.bnd.ensure((size_t)6 * (nW + 1));
     DBuf<double>* d[NPD] = {&c.pos[0], &c.pos[1], &c.pos[2], &c.vel[0], &c.vel[1], &c.vel[2], &c.angVel[0],
                             &c.angVel[1], &c.angVel[2], &c.ori[0], &c.ori[1], &c.ori[2], &c.ori[3], &c.force[0],
                             &c.force[1], &c.force[2], &c.torque[0], &c.torque[1], &c.torque[2], &c.inertia[0],

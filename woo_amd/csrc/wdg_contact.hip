@@ -38,6 +38,7 @@ struct ContactArgs {
   const uint32_t* flags;
   const int32_t* mat;
   const wdg_material* mats;
+  const double* fgeo;  // facet geometry per boundary body
   // parameters
   double L[3];
   double absDf;
@@ -54,6 +55,42 @@ __device__ __forceinline__ d3 slot_shift(const ContactArgs& a, int64_t c, uint64
   if (!PERIODIC) return mk3(0., 0., 0.);
   cd = a.dist[c];
   return mk3(a.L[0] * unpack_cd(cd, 0), a.L[1] * unpack_cd(cd, 1), a.L[2] * unpack_cd(cd, 2));
+}
+
+// CompUtils::closestSegmentPt (CompUtils.cpp:59-65)
+template <class V, class T>
+__device__ __forceinline__ V closest_segment_pt(V P, V A, V B) {
+  const V BA = sub3(B, A);
+  if (sqn3(BA) == T(0)) return A;
+  const T u = (dot3(P, BA) - dot3(A, BA)) / sqn3(BA);
+  return add3(A, mul3(BA, fmin(T(1), fmax(T(0), u))));
+}
+
+// Closest point of a facet to the sphere centre sC (Cg2_Facet_Sphere_L6Geom::go,
+// Facet.cpp:336-360, with Facet::getNormal 74-77 and getOuterVectors 113-118): the facet
+// normal, the signed plane distance, and the closest point by Voronoi region of the
+// triangle v0 v1 v2. Returns false for the impossible region (outside all three edges).
+template <class V, class T>
+__device__ __forceinline__ bool facet_closest(V v0, V v1, V v2, V sC, V& fNormal, T& planeDist, V& contPt) {
+  const V nn = cross3(sub3(v1, v0), sub3(v2, v0));
+  fNormal = normalized3(nn);
+  planeDist = dot3(sub3(sC, v0), fNormal);
+  const V fC = sub3(sC, mul3(fNormal, planeDist));
+  const T l0 = dot3(cross3(sub3(v1, v0), nn), sub3(fC, v0));
+  const T l1 = dot3(cross3(sub3(v2, v1), nn), sub3(fC, v1));
+  const T l2 = dot3(cross3(sub3(v0, v2), nn), sub3(fC, v2));
+  const int w = (l0 > 0 ? 1 : 0) + (l1 > 0 ? 2 : 0) + (l2 > 0 ? 4 : 0);
+  switch (w) {
+    case 0: contPt = fC; break;
+    case 1: contPt = closest_segment_pt<V, T>(fC, v0, v1); break;
+    case 2: contPt = closest_segment_pt<V, T>(fC, v1, v2); break;
+    case 4: contPt = closest_segment_pt<V, T>(fC, v2, v0); break;
+    case 3: contPt = v1; break;
+    case 5: contPt = v0; break;
+    case 6: contPt = v2; break;
+    default: contPt = fC; return false;
+  }
+  return true;
 }
 
 // Law2 break test: IdealElPl.cpp:26-33 (unless noBreak), Hertz.cpp:105-108
@@ -113,6 +150,19 @@ __device__ __forceinline__ bool filter_slot(const ContactArgs& a, int64_t c) {
         const double unDistSq = sqn3(relPos) - p2(a.absDf * (r1 + r2));
         pass = !(unDistSq > 0 && !real);
         uN = sqrt(sqn3(relPos)) - (r1 + r2);
+      } else if ((info & CI_TYPE_MASK) == CT_FS) {
+        // Cg2_Facet_Sphere_L6Geom::go (Facet.cpp:333-363, 373), pA is the facet
+        const double* g = a.fgeo + (int64_t)FGEO * pA;
+        const double radius = a.rad[pB], ht = g[12];
+        d3 fNormal, contPt;
+        double planeDist;
+        const bool ok = facet_closest<d3, double>(mk3(g[0], g[1], g[2]), mk3(g[3], g[4], g[5]), mk3(g[6], g[7], g[8]),
+                                                  pos2, fNormal, planeDist, contPt);
+        const bool near = !(fabs(planeDist) > (radius + ht) && !real);
+        if (near && !ok) atomicOr(&a.dflags[1], ERR_FACET);
+        const d3 nv = sub3(pos2, contPt);
+        pass = near && ok && !(sqn3(nv) > p2(radius + ht) && !real);
+        uN = (sqrt(sqn3(nv)) - radius) - ht;
       } else {
         // Cg2_Wall_Sphere_L6Geom::go (Wall.cpp:50-71)
         const uint32_t fl = a.flags[pA];
@@ -181,6 +231,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
 
     d3 normal, contPt, pos1, pos2;
     double uN, r1, r2;
+    d3 vel1 = mk3(a.vx[pA], a.vy[pA], a.vz[pA]), av1 = mk3(a.wx[pA], a.wy[pA], a.wz[pA]);
     if (type == CT_SS) {
       // Cg2_Sphere_Sphere_L6Geom::go (Sphere.cpp:92-117)
       r1 = a.rad[pA];
@@ -192,6 +243,32 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
       uN = dist - (r1 + r2);
       normal = div3(relPos, dist);
       contPt = add3(posA, mul3(normal, r1 + 0.5 * uN));
+    } else if (type == CT_FS) {
+      // Cg2_Facet_Sphere_L6Geom::go (Facet.cpp:333-390), pA is the facet; n21lim is NaN
+      // (adjustBoundaryContactGeom returns at once); static nodes: the velocity at the contact
+      // point is fakeVel (zero when its x is NaN), the angular velocity zero
+      const double* g = a.fgeo + (int64_t)FGEO * pA;
+      const double radius = a.rad[pB], ht = g[12];
+      pos2 = add3(posB, shift2);
+      d3 fNormal;
+      double planeDist;
+      facet_closest<d3, double>(mk3(g[0], g[1], g[2]), mk3(g[3], g[4], g[5]), mk3(g[6], g[7], g[8]), pos2, fNormal,
+                                planeDist, contPt);
+      const d3 nv = sub3(pos2, contPt);
+      const double dist = sqrt(sqn3(nv));
+      if (dist != 0) normal = div3(nv, dist);
+      else if (real) {
+        // the previous normal as node.ori * UnitX
+        const d3 pY = mk3(a.oY0[ko], a.oY1[ko], a.oY2[ko]);
+        normal = quat_rot(quat_from_cols(prevNormal, pY, cross3(prevNormal, pY)), mk3(1., 0., 0.));
+      } else normal = mk3(1., 0., 0.);
+      if (ht > 0) contPt = add3(contPt, mul3(normal, ht));
+      uN = (dist - radius) - ht;
+      pos1 = contPt;
+      r1 = fmax(ht, radius);
+      r2 = radius;
+      vel1 = isnan(g[9]) ? mk3(0., 0., 0.) : mk3(g[9], g[10], g[11]);
+      av1 = mk3(0., 0., 0.);
     } else {
       // Cg2_Wall_Sphere_L6Geom::go (Wall.cpp:50-84), pA is the wall
       const uint32_t fl = a.flags[pA];
@@ -211,8 +288,8 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
       r1 = -radius;
       r2 = radius;
     }
-    const d3 vel1 = mk3(a.vx[pA], a.vy[pA], a.vz[pA]), vel2 = mk3(a.vx[pB], a.vy[pB], a.vz[pB]);
-    const d3 av1 = mk3(a.wx[pA], a.wy[pA], a.wz[pA]), av2 = mk3(a.wx[pB], a.wy[pB], a.wz[pB]);
+    const d3 vel2 = mk3(a.vx[pB], a.vy[pB], a.vz[pB]);
+    const d3 av2 = mk3(a.wx[pB], a.wy[pB], a.wz[pB]);
 
     // Cg2_Any_Any_L6Geom__Base::handleSpheresLikeContact (L6Geom.cpp:49-216)
     d3 T0, T1, T2, gvel;
@@ -362,6 +439,20 @@ __device__ __forceinline__ bool filter_slot32(const ContactArgs& a, int64_t c) {
         const float reach = (float)a.absDf * (r1 + r2);
         pass = !(sqn3(r) - reach * reach > 0.f && !real);
         uN = sqrtf(sqn3(r)) - (r1 + r2);
+      } else if ((info & CI_TYPE_MASK) == CT_FS) {
+        // facet vertices relative to vertex 0 (the facet's node)
+        const double* g = a.fgeo + (int64_t)FGEO * pA;
+        const float radius = (float)a.rad[pB], ht = (float)g[12];
+        const f3 e1 = tof3(mk3(g[3] - g[0], g[4] - g[1], g[5] - g[2]));
+        const f3 e2 = tof3(mk3(g[6] - g[0], g[7] - g[1], g[8] - g[2]));
+        f3 fNormal, contPt;
+        float planeDist;
+        const bool ok = facet_closest<f3, float>(mkf3(0.f, 0.f, 0.f), e1, e2, r, fNormal, planeDist, contPt);
+        const bool near = !(fabsf(planeDist) > (radius + ht) && !real);
+        if (near && !ok) atomicOr(&a.dflags[1], ERR_FACET);
+        const f3 nv = sub3(r, contPt);
+        pass = near && ok && !(sqn3(nv) > (radius + ht) * (radius + ht) && !real);
+        uN = (sqrtf(sqn3(nv)) - radius) - ht;
       } else {
         const uint32_t fl = a.flags[pA];
         const int ax = flag_axis(fl), sense = flag_sense(fl);
@@ -432,6 +523,8 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
     if (real) prevNormal = mkf3(oN0[ko], oN1[ko], oN2[ko]);
     f3 normal, cA;
     float uN, r1, r2;
+    f3 vel1 = tof3(mk3(a.vx[pA], a.vy[pA], a.vz[pA])), av1 = tof3(mk3(a.wx[pA], a.wy[pA], a.wz[pA]));
+    f3 vmove1 = vel1;  // the velocity pA's node moved with over the last step
     if ((info & CI_TYPE_MASK) == CT_SS) {
       r1 = (float)a.rad[pA];
       r2 = (float)a.rad[pB];
@@ -439,6 +532,28 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
       uN = dist - (r1 + r2);
       normal = div3(r, dist);
       cA = mul3(normal, r1 + 0.5f * uN);
+    } else if ((info & CI_TYPE_MASK) == CT_FS) {
+      const double* g = a.fgeo + (int64_t)FGEO * pA;
+      const float radius = (float)a.rad[pB], ht = (float)g[12];
+      const f3 e1 = tof3(mk3(g[3] - g[0], g[4] - g[1], g[5] - g[2]));
+      const f3 e2 = tof3(mk3(g[6] - g[0], g[7] - g[1], g[8] - g[2]));
+      f3 fNormal;
+      float planeDist;
+      facet_closest<f3, float>(mkf3(0.f, 0.f, 0.f), e1, e2, r, fNormal, planeDist, cA);
+      const f3 nv = sub3(r, cA);
+      const float dist = sqrtf(sqn3(nv));
+      if (dist != 0.f) normal = div3(nv, dist);
+      else if (real) {
+        const f3 pY = mkf3(oY0[ko], oY1[ko], oY2[ko]);
+        normal = quat_rot(quat_from_cols(prevNormal, pY, cross3(prevNormal, pY)), mkf3(1.f, 0.f, 0.f));
+      } else normal = mkf3(1.f, 0.f, 0.f);
+      if (ht > 0.f) cA = add3(cA, mul3(normal, ht));
+      uN = (dist - radius) - ht;
+      r1 = fmaxf(ht, radius);
+      r2 = radius;
+      vel1 = isnan(g[9]) ? mkf3(0.f, 0.f, 0.f) : tof3(mk3(g[9], g[10], g[11]));
+      av1 = mkf3(0.f, 0.f, 0.f);
+      vmove1 = mkf3(0.f, 0.f, 0.f);
     } else {
       const uint32_t fl = a.flags[pA];
       const int ax = flag_axis(fl), sense = flag_sense(fl);
@@ -457,8 +572,8 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
       r2 = radius;
     }
     const f3 cB = sub3(cA, r);
-    const f3 vel1 = tof3(mk3(a.vx[pA], a.vy[pA], a.vz[pA])), vel2 = tof3(mk3(a.vx[pB], a.vy[pB], a.vz[pB]));
-    const f3 av1 = tof3(mk3(a.wx[pA], a.wy[pA], a.wz[pA])), av2 = tof3(mk3(a.wx[pB], a.wy[pB], a.wz[pB]));
+    const f3 vel2 = tof3(mk3(a.vx[pB], a.vy[pB], a.vz[pB]));
+    const f3 av2 = tof3(mk3(a.wx[pB], a.wy[pB], a.wz[pB]));
     f3 T0, T1, T2, gvel;
     if (!real) {
       const f3 locX = normal;
@@ -493,8 +608,12 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
         T1 = normalized3(T1);
         T2 = cross3(T0, T1);
       }
-      const f3 c1x = mul3(add3(prevCA, cA), .5f);
-      const f3 c2x = add3(sub3(c1x, r), mul3(sub3(vel2, vel1), .5f * dt));
+      // midContPt - midPos1 and midContPt - midPos2 relative to pA's node, which moved by
+      // vmove1*dt since the last step (a facet's does not). pos1 is pA's node (sphere, wall),
+      // or the contact point for a facet
+      const f3 mid = mul3(add3(prevCA, cA), .5f);
+      const f3 c1x = (info & CI_TYPE_MASK) == CT_FS ? add3(sub3(mid, cA), mul3(vel1, .5f * dt)) : mid;
+      const f3 c2x = add3(sub3(mid, r), mul3(sub3(vel2, vmove1), .5f * dt));
       const f3 relVel = sub3(add3(vel2, cross3(av2, c2x)), add3(vel1, cross3(av1, c1x)));
       gvel = mkf3(dot3(M0, relVel), dot3(M1, relVel), dot3(M2, relVel));
     }
@@ -588,6 +707,7 @@ static ContactArgs contact_args(Ctx& c, int from, int to) {
   a.vx = c.vel[0].p; a.vy = c.vel[1].p; a.vz = c.vel[2].p;
   a.wx = c.angVel[0].p; a.wy = c.angVel[1].p; a.wz = c.angVel[2].p;
   a.rad = c.radius.p; a.mass = c.mass.p; a.flags = c.flags.p; a.mat = c.mat.p; a.mats = c.dMats.p;
+  a.fgeo = c.fgeo.p;
   for (int k = 0; k < 3; k++) a.L[k] = c.field.cell[k];
   a.absDf = std::fabs(c.field.distFactor);
   a.dflags = c.dflags.p;

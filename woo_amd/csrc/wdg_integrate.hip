@@ -18,6 +18,7 @@ namespace wdg {
 struct GatherArgs {
   int64_t n;
   const uint32_t *adj, *adjOff;
+  const uint32_t* flags;
   RealView rv;
   const double* gp;
   double L[3];
@@ -67,6 +68,8 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs g, double* fx, double
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.n) return;
   if (g.adjOff[i] == g.adjOff[i + 1]) return;
+  // multinodal particles (facets) take no force from the contact loop (ContactLoop.cpp:270-276)
+  if (flag_shape(g.flags[i]) == WDG_FACET) return;
   d3 F = mk3(fx[i], fy[i], fz[i]), T = mk3(tx[i], ty[i], tz[i]);
   gather_all<PERIODIC, F32>(g, i, mk3(px[i], py[i], pz[i]), F, T);
   fx[i] = F.x; fy[i] = F.y; fz[i] = F.z;
@@ -110,7 +113,8 @@ __global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
       F = mk3(a.fx[i], a.fy[i], a.fz[i]);
       T = mk3(a.tx[i], a.ty[i], a.tz[i]);
     }
-    if (a.gather) gather_all<PERIODIC, F32>(g, i, pos, F, T);
+    // multinodal particles (facets) take no force from the contact loop (ContactLoop.cpp:270-276)
+    if (a.gather && flag_shape(fl) != WDG_FACET) gather_all<PERIODIC, F32>(g, i, pos, F, T);
     if (!integrated) {
       if (a.gather) {
         a.fx[i] = F.x; a.fy[i] = F.y; a.fz[i] = F.z;
@@ -206,7 +210,7 @@ static GatherArgs gather_args(Ctx& c) {
   Contacts& C = c.con();
   GatherArgs g{};
   g.n = c.n;
-  g.adj = C.adj.p; g.adjOff = C.adjOff.p; g.rv = c.rview(); g.gp = c.reals().gp.p;
+  g.adj = C.adj.p; g.adjOff = C.adjOff.p; g.rv = c.rview(); g.gp = c.reals().gp.p; g.flags = c.flags.p;
   for (int k = 0; k < 3; k++) g.L[k] = c.field.cell[k];
   return g;
 }
@@ -282,6 +286,7 @@ struct DynArgs {
   const uint32_t* flags;
   const int32_t* mat;
   const wdg_material* mats;
+  const double* fgeo;
   int hertz;
   double poisson, en, viscAlpha;
   unsigned long long* out;
@@ -297,8 +302,7 @@ __device__ __forceinline__ void dyn_one(const DynArgs& a, uint32_t e, d3& kt, d3
   const uint32_t pA = swap ? a.B[s] : a.A[s], pB = swap ? a.A[s] : a.B[s];
   const int ix = (e & ADJ_PA) ? 0 : 1;
   double r1, r2;
-  if ((inf & CI_TYPE_MASK) == CT_SS) { r1 = a.rad[pA]; r2 = a.rad[pB]; }
-  else { r1 = -a.rad[pB]; r2 = a.rad[pB]; }
+  contact_radii(inf, pA, pB, a.rad, a.fgeo, r1, r2);
   double l0, l1, A;
   lens_area(r1, r2, l0, l1, A);
   const wdg_material mA = a.mats[a.mat[pA]], mB = a.mats[a.mat[pB]];
@@ -365,7 +369,7 @@ double crit_dt(Ctx& c) {
   a.N0 = R.hN[0].p; a.N1 = R.hN[1].p; a.N2 = R.hN[2].p;
   a.Y0 = R.hY[0].p; a.Y1 = R.hY[1].p; a.Y2 = R.hY[2].p; a.uN = R.huN.p;
   a.rad = c.radius.p; a.mass = c.mass.p; a.ix = c.inertia[0].p; a.iy = c.inertia[1].p; a.iz = c.inertia[2].p;
-  a.flags = c.flags.p; a.mat = c.mat.p; a.mats = c.dMats.p;
+  a.flags = c.flags.p; a.mat = c.mat.p; a.mats = c.dMats.p; a.fgeo = c.fgeo.p;
   a.hertz = c.lastPhysKind == WDG_PHYS_HERTZ;
   a.poisson = c.lastPoisson; a.en = c.lastEn; a.viscAlpha = c.lastViscAlpha;
   unsigned long long* out = reinterpret_cast<unsigned long long*>(c.red.p);
@@ -490,6 +494,16 @@ __global__ __launch_bounds__(256) void k_stress(StressArgs a) {
     if (a.periodic) {
       const uint64_t cd = a.dist[c];
       posA = sub3(posA, mk3(a.L[0] * unpack_cd(cd, 0), a.L[1] * unpack_cd(cd, 1), a.L[2] * unpack_cd(cd, 2)));
+    }
+    // a multinodal pA (facet) is represented by the contact point (skipMultinodal=false)
+    if ((inf & CI_TYPE_MASK) == CT_FS) {
+      if (PREC == 32) {
+        const float* g = reinterpret_cast<const float*>(a.gp) + (int64_t)GP32 * r;
+        posA = add3(posA, mk3(g[3], g[4], g[5]));
+      } else {
+        const double* g = a.gp + (int64_t)GP * r;
+        posA = mk3(g[3], g[4], g[5]);
+      }
     }
     const double d0 = sqrt(sqn3(sub3(posA, posB)));
     const d3 T0 = mk3(ldr<PREC>(a.N0, r), ldr<PREC>(a.N1, r), ldr<PREC>(a.N2, r)),

@@ -39,7 +39,7 @@ enum : uint32_t {
   F_GRAVITY_SKIP = 512u,
   F_DAMPING_SKIP = 2048u,
   F_INTEGRATED = 1u << 16,
-  F_SHAPE_SHIFT = 17,   // 2 bits: 0 sphere, 1 wall
+  F_SHAPE_SHIFT = 17,   // 2 bits: 0 sphere, 1 wall, 2 facet
   F_AXIS_SHIFT = 19,    // 2 bits: wall axis
   F_SENSE_SHIFT = 21,   // 2 bits: wall sense + 1
 };
@@ -49,11 +49,13 @@ __host__ __device__ inline int flag_sense(uint32_t f) { return (int)((f >> F_SEN
 
 // ------------------------------------------------------------------ contact info word
 enum : uint32_t {
-  CI_TYPE_MASK = 3u,   // 0 sphere-sphere, 1 wall-sphere
+  CI_TYPE_MASK = 3u,   // 0 sphere-sphere, 1 wall-sphere, 2 facet-sphere
   CI_SWAP = 4u,        // geometric first particle (pA) is B (the larger storage index)
   CI_NEAR = 8u,        // close enough at the last refresh to touch before the next one
 };
-enum { CT_SS = 0, CT_WS = 1 };
+enum { CT_SS = 0, CT_WS = 1, CT_FS = 2 };
+// per boundary body (storage [0, nW)): facet vertices v0 v1 v2, fakeVel, halfThick, unused
+constexpr int FGEO = 16;
 constexpr uint32_t ADJ_PA = 0x80000000u;  // adjacency entry: this particle is pA of the slot
 
 // periodic image offsets packed 3 x 21 bits (two's complement)
@@ -72,7 +74,7 @@ __host__ __device__ inline int unpack_cd(uint64_t v, int k) {
 constexpr int DIRTY_PP = 12;
 
 // device error bits
-enum : uint32_t { ERR_NAN_FORCE = 1u, ERR_DOUBLE_IMAGE = 2u, ERR_CD_RANGE = 4u, ERR_ADJ = 8u };
+enum : uint32_t { ERR_NAN_FORCE = 1u, ERR_DOUBLE_IMAGE = 2u, ERR_CD_RANGE = 4u, ERR_ADJ = 8u, ERR_FACET = 16u };
 
 // ------------------------------------------------------------------ device buffers
 template <class T>
@@ -197,7 +199,10 @@ struct Ctx {
 
   // particles, storage order
   int64_t n = 0;
-  int64_t nW = 0;       // walls occupy storage [0, nW)
+  int64_t nW = 0;       // boundary bodies (walls, facets) occupy storage [0, nW)
+  int64_t nFacet = 0;
+  DBuf<double> fgeo;    // FGEO doubles per boundary body (facet geometry)
+  DBuf<double> bnd;     // enlarged AABB of each boundary body at the last refresh (min3, max3)
   double rmin = INFINITY, rmax = 0;
   DBuf<double> pos[3], vel[3], angVel[3], ori[4], force[3], torque[3], inertia[3], lastPos[3];
   DBuf<double> mass, radius;
@@ -227,6 +232,9 @@ struct Ctx {
   DBuf<float4> posf;                    // float candidate records of the last refresh
   DBuf<uint64_t> adjEnt;                // adjacency entries with their sort key
   DBuf<uint32_t> adjDeg;
+  // adjacency of high-degree particles (boundary bodies), sorted by one radix sort
+  DBuf<uint32_t> bigOff, bigRank, bigVal, bigValTmp, bigDest;
+  DBuf<uint64_t> bigKey, bigKeyTmp;
   DBuf<uint32_t> sortHist, scanAux;
   DBuf<uint32_t> act;     // slot of each state index (contact loop), per-slot flags (relaxation)
   DBuf<double> red;       // reductions

@@ -23,6 +23,18 @@ __device__ __forceinline__ void lens_area(double r1, double r2, double& l0, doub
   A = M_PI * (rr * rr);
 }
 
+// the radii the geometry functor passes to handleSpheresLikeContact: both radii for two
+// spheres (Sphere.cpp:117), (-r, r) for a wall (Wall.cpp:82), (max(halfThick, r), r) for a
+// facet (Facet.cpp:389)
+__device__ __forceinline__ void contact_radii(uint32_t info, uint32_t pA, uint32_t pB, const double* rad,
+                                              const double* fgeo, double& r1, double& r2) {
+  const uint32_t t = info & CI_TYPE_MASK;
+  r2 = rad[pB];
+  if (t == CT_SS) r1 = rad[pA];
+  else if (t == CT_FS) r1 = fmax(fgeo[(int64_t)FGEO * pA + 12], r2);
+  else r1 = -r2;
+}
+
 // Cp2_FrictMat_FrictPhys::updateFrictPhys (FrictMat.cpp:20-44)
 __device__ __forceinline__ void phys_frict(const wdg_material& m1, const wdg_material& m2, double l0, double l1,
                                            double A, PhysVals& ph) {

@@ -241,5 +241,6 @@ void radix_sort_pairs(Ctx& c, K* keys, uint32_t* vals, K* keysTmp, uint32_t* val
 }
 
 template void radix_sort_pairs<uint32_t>(Ctx&, uint32_t*, uint32_t*, uint32_t*, uint32_t*, int64_t, int);
+template void radix_sort_pairs<uint64_t>(Ctx&, uint64_t*, uint32_t*, uint64_t*, uint32_t*, int64_t, int);
 
 }  // namespace wdg
