@@ -8,7 +8,7 @@ import math
 import numpy as np
 import pytest
 
-from tests.scenes import _particle_order, build_oracle, deposition, periodic_hertz
+from tests.scenes import _particle_order, build_oracle, deposition, drum, periodic_hertz
 
 pytestmark = pytest.mark.gpu
 
@@ -36,9 +36,13 @@ def host_scene(sc, engines=None):
             p.mass = sc["mass"][i]
             p.inertia = sc["inertia"][i]
             p.vel = sc["vel"][i]
-        else:
+        elif kind == "w":
             w = sc["walls"][i]
             p = woo.utils.wall(w["pos"], axis=w["axis"], sense=w["sense"], mat=mat)
+        else:
+            f = sc["facets"][i]
+            p = woo.utils.facet([tuple(v) for v in f["verts"]], fakeVel=f.get("fakeVel"),
+                                halfThick=f.get("halfThick", 0.), mat=mat)
         pars.append(p)
     S.dem.par.add(pars)
     if engines is None:
@@ -54,7 +58,7 @@ def host_scene(sc, engines=None):
 
 
 def positions(S):
-    return np.array([p.pos for p in S.dem.par])
+    return np.array([p.pos if type(p.shape).__name__ != "Facet" else p.shape.nodes[0].pos for p in S.dem.par])
 
 
 def traj_err(o, S, sc):
@@ -165,3 +169,44 @@ def test_unsupported_combinations_raise():
                                                             [Law2_L6Geom_FrictPhys_IdealElPl()])]
     with pytest.raises(RuntimeError):
         S.run(1)
+
+
+def test_scripted_drum_with_facets_matches_oracle():
+    # woo.utils.facet with fakeVel (the rotating mantle of woo.triangulated.cylinder) and walls
+    sc = drum(n=1500, seed=41)
+    S = host_scene(sc)
+    o = build_oracle(sc)
+    for _ in range(3):
+        S.run(20)
+        o.run(MINIMAL, 20)
+        e = traj_err(o, S, sc)
+        assert e <= TRAJ_TOL, e
+    nf = sum(1 for c in S.dem.con if c.real and type(S.dem.par[c.id1].shape).__name__ == "Facet")
+    assert nf > 10
+
+
+def test_triangulated_cylinder_drum_runs():
+    # the reference's way of building a drum: triangulated.cylinder(..., wallCaps, angVel)
+    import woo_amd.woo as woo
+    from woo_amd.woo.dem import Cp2_HertzMat_HertzPhys, DemField, HertzMat, Law2_L6Geom_HertzPhys_DMT
+
+    m = HertzMat(density=3000., young=1e8, tanPhi=.4)
+    S = woo.core.Scene(fields=[DemField(gravity=(0, 0, -9.81))])
+    S.dem.par.add(woo.triangulated.cylinder((-.02, 0, 0), (.02, 0, 0), .05, div=49, wallCaps=True, capA=True,
+                                            capB=True, angVel=2., mat=m))
+    rng = np.random.default_rng(3)
+    for x in np.arange(-.015, .016, .006):
+        for y in np.arange(-.03, .031, .006):
+            for z in np.arange(-.04, -.005, .006):
+                if math.hypot(y, z) < .043:
+                    S.dem.par.add(woo.utils.sphere((x, y, z), .0025 + .0003 * rng.uniform(), mat=m))
+    S.engines = woo.utils.defaultEngines(damping=0., dynDtPeriod=0, cp2=Cp2_HertzMat_HertzPhys(poisson=.25, en=.6),
+                                         law=Law2_L6Geom_HertzPhys_DMT())
+    S.dt = .3 * woo.utils.pWaveDt(S)
+    S.run(3000)
+    S.wait()
+    nf = sum(1 for c in S.dem.con if c.real and type(S.dem.par[c.id1].shape).__name__ == "Facet")
+    z = np.array([p.pos[2] for p in S.dem.par if type(p.shape).__name__ == "Sphere"])
+    print("facet contacts", nf, "lowest sphere z", z.min())
+    # spheres fell onto the mantle and stay inside it
+    assert nf > 5 and z.min() > -.05

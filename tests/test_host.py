@@ -80,7 +80,9 @@ def test_engines_functors_labels(woo):
     assert S.lab.leapfrog.damping == .1 and S.lab.leapfrog.reset
     assert S.lab.cp2.poisson == .25 and S.lab.dynDt.stepPeriod == 10
     cl = S.lab.contactLoop
-    assert [type(f).__name__ for f in cl.geoDisp] == ["Cg2_Sphere_Sphere_L6Geom", "Cg2_Wall_Sphere_L6Geom"]
+    # the reference's list restricted to the shapes this engine has (utils.py:116-125)
+    assert [type(f).__name__ for f in cl.geoDisp] == ["Cg2_Sphere_Sphere_L6Geom", "Cg2_Facet_Sphere_L6Geom",
+                                                      "Cg2_Wall_Sphere_L6Geom"]
     assert cl.geoDisp[0].trsfRenorm == 100
     pr = woo.core.PyRunner(7, "pass")
     assert pr.stepPeriod == 7 and pr.command == "pass"
@@ -109,3 +111,23 @@ def test_run_needs_the_device_engine(woo):
     S.engines = woo.utils.defaultEngines()
     with pytest.raises(RuntimeError):
         S.run(1)
+
+
+def test_facet_constructors(woo):
+    from woo_amd.woo.dem import Facet, FrictMat
+
+    m = FrictMat(density=2000.)
+    p = Facet.make([(0, 0, 0), (1, 0, 0), (0, 1, 0)], fakeVel=(.1, 0, 0), halfThick=.001, mat=m)
+    assert type(p.shape).__name__ == "Facet" and len(p.shape.nodes) == 3
+    assert np.allclose(p.shape.getNormal(), (0, 0, 1)) and p.shape.halfThick == .001
+    assert p.mask == 3 and all(n.dem.blocked == "xyzXYZ" for n in p.shape.nodes)
+    # triangulated.cylinder: 2 facets per circumferential segment, caps first; fakeVel tangential
+    # with magnitude radius*angVel (triangulated.py:66-74)
+    ff = woo.triangulated.cylinder((0, 0, 0), (.4, 0, 0), .5, div=9, wallCaps=True, capA=True, capB=True, angVel=2.)
+    assert [type(f.shape).__name__ for f in ff[:2]] == ["Wall", "Wall"] and len(ff) == 2 + 2 * 8
+    for f in ff[2:]:
+        v = np.asarray(f.shape.fakeVel)
+        assert abs(np.linalg.norm(v) - .5 * 2.) < 1e-12 and abs(v[0]) < 1e-15
+        assert abs(np.dot(v, f.shape.getNormal())) < 1e-12
+    with pytest.raises(NotImplementedError):
+        woo.triangulated.cylinder((0, 0, 0), (.4, 0, 0), .5, realRot=True, angVel=1.)

@@ -36,17 +36,33 @@ DeviceState::~DeviceState() {
 
 // ------------------------------------------------------------------ DemField
 int64_t DemField::add(const std::shared_ptr<Particle>& p, int nodes) {
-  if (!p || !p->shape || !p->material || p->shape->nodes.size() != 1 || !p->shape->nodes[0])
-    throw std::invalid_argument("particle needs a material and a shape with one node");
+  if (!p || !p->shape || !p->material) throw std::invalid_argument("particle needs a material and a shape");
   if (p->id >= 0) throw std::invalid_argument("particle was already added (id " + std::to_string(p->id) + ")");
-  if (!dynamic_cast<Sphere*>(p->shape.get()) && !dynamic_cast<Wall*>(p->shape.get()))
-    throw std::invalid_argument(std::string("shape ") + p->shape->className() + " is not supported by the device engine");
+  if (auto* f = dynamic_cast<Facet*>(p->shape.get())) {
+    if (f->nodes.size() != 3 || !f->nodes[0] || !f->nodes[1] || !f->nodes[2])
+      throw std::invalid_argument("Facet needs 3 nodes");
+    for (auto& n : f->nodes) {
+      const DemData& d = n->dem;
+      if ((d.flags & DOF_ALL) != DOF_ALL || d.vel != Vec3{} || d.angVel != Vec3{})
+        throw std::invalid_argument("only fixed facets with static nodes are supported (use Facet.fakeVel)");
+    }
+  } else {
+    if (p->shape->nodes.size() != 1 || !p->shape->nodes[0])
+      throw std::invalid_argument("particle needs a shape with one node");
+    if (!dynamic_cast<Sphere*>(p->shape.get()) && !dynamic_cast<Wall*>(p->shape.get()))
+      throw std::invalid_argument(std::string("shape ") + p->shape->className() +
+                                  " is not supported by the device engine");
+  }
   pull();
   p->id = (int64_t)particles.size();
-  p->node().field = this;
+  for (auto& n : p->shape->nodes) n->field = this;
   // ParticleContainer::pyAppend (ParticleContainer.cpp:176-201): nodes=1 always, 0 never,
-  // -1 by DemData::guessMoving
-  p->node().integrated = nodes == 1 || (nodes == -1 && p->node().dem.guessMoving());
+  // -1 by DemData::guessMoving; facet nodes here are fixed and static
+  if (dynamic_cast<Facet*>(p->shape.get())) {
+    for (auto& n : p->shape->nodes) n->integrated = false;
+  } else {
+    p->node().integrated = nodes == 1 || (nodes == -1 && p->node().dem.guessMoving());
+  }
   particles.push_back(p);
   structureChanged = true;
   contactsValid = false;
@@ -80,7 +96,7 @@ void DemField::pull() {
     const int w = field_width(f);
     buf.resize(n * w);
     check(wdg_download(dev->ctx, f, buf.data()));
-    for (size_t i = 0; i < n; i++) std::copy(&buf[i * w], &buf[i * w] + w, node_field(particles[i]->node(), f));
+    for (size_t i = 0; i < n; i++) std::copy(&buf[i * w], &buf[i * w] + w, node_field(particles[i]->node0(), f));
   }
 }
 
@@ -104,7 +120,7 @@ void DemField::push(Scene& S) {
     std::vector<int32_t> shape(n), axis(n), sense(n), mat(n);
     std::vector<uint32_t> mask(n), flags(n);
     std::vector<uint8_t> integ(n);
-    std::vector<double> radius(n), bufs[8];
+    std::vector<double> radius(n), bufs[8], fverts(9 * n, 0.), fvel(3 * n, 0.), fht(n, 0.);
     for (size_t i = 0; i < n; i++) {
       const Particle& p = *particles[i];
       auto it = midx.find(p.material.get());
@@ -128,6 +144,12 @@ void DemField::push(Scene& S) {
       if (const auto* s = dynamic_cast<const Sphere*>(p.shape.get())) {
         shape[i] = WDG_SPHERE;
         radius[i] = s->radius;
+      } else if (const auto* f = dynamic_cast<const Facet*>(p.shape.get())) {
+        shape[i] = WDG_FACET;
+        for (int v = 0; v < 3; v++)
+          for (int k = 0; k < 3; k++) fverts[9 * i + 3 * v + k] = f->nodes[v]->pos[k];
+        for (int k = 0; k < 3; k++) fvel[3 * i + k] = f->fakeVel[k];
+        fht[i] = f->halfThick;
       } else {
         const auto* w = dynamic_cast<const Wall*>(p.shape.get());
         shape[i] = WDG_WALL;
@@ -135,14 +157,14 @@ void DemField::push(Scene& S) {
         sense[i] = w->sense;
       }
       mask[i] = p.mask;
-      flags[i] = p.node().dem.flags;
-      integ[i] = p.node().integrated ? 1 : 0;
+      flags[i] = p.node0().dem.flags;
+      integ[i] = p.node0().integrated ? 1 : 0;
     }
     for (int fi = 0; fi < 8; fi++) {
       const int w = field_width(FIELDS[fi]);
       bufs[fi].resize(n * w);
       for (size_t i = 0; i < n; i++) {
-        const double* src = node_field(particles[i]->node(), FIELDS[fi]);
+        const double* src = node_field(particles[i]->node0(), FIELDS[fi]);
         std::copy(src, src + w, &bufs[fi][i * w]);
       }
     }
@@ -153,6 +175,7 @@ void DemField::push(Scene& S) {
     pd.material = mat.data(); pd.mask = mask.data(); pd.flags = flags.data(); pd.integrated = integ.data();
     pd.pos = bufs[0].data(); pd.vel = bufs[1].data(); pd.angVel = bufs[2].data(); pd.ori = bufs[3].data();
     pd.force = bufs[4].data(); pd.torque = bufs[5].data(); pd.mass = bufs[6].data(); pd.inertia = bufs[7].data();
+    pd.facetVerts = fverts.data(); pd.fakeVel = fvel.data(); pd.halfThick = fht.data();
     check(wdg_upload_particles(dev->ctx, &pd));
     structureChanged = hostDirty = false;
   } else if (hostDirty) {
@@ -161,7 +184,7 @@ void DemField::push(Scene& S) {
       const int w = field_width(f);
       buf.resize(n * w);
       for (size_t i = 0; i < n; i++) {
-        const double* src = node_field(particles[i]->node(), f);
+        const double* src = node_field(particles[i]->node0(), f);
         std::copy(src, src + w, &buf[i * w]);
       }
       check(wdg_upload(dev->ctx, f, buf.data()));
@@ -284,9 +307,10 @@ wdg_program Scene::program(bool& hostEngines) {
   wdg_program p{};
   hostEngines = false;
   auto f = dem();
-  bool anyWall = false, anySphere = false, allHertz = true;
+  bool anyWall = false, anySphere = false, anyFacet = false, allHertz = true;
   for (auto& q : f->particles) {
     if (dynamic_cast<Wall*>(q->shape.get())) anyWall = true;
+    else if (dynamic_cast<Facet*>(q->shape.get())) anyFacet = true;
     else anySphere = true;
     if (!dynamic_cast<HertzMat*>(q->material.get())) allHertz = false;
   }
@@ -298,6 +322,8 @@ wdg_program Scene::program(bool& hostEngines) {
         throw std::runtime_error("InsertionSortCollider: spheres need Bo1_Sphere_Aabb");
       if (anyWall && !find_functor(c->boundDispatcher, "Bo1_Wall_Aabb"))
         throw std::runtime_error("InsertionSortCollider: walls need Bo1_Wall_Aabb");
+      if (anyFacet && !find_functor(c->boundDispatcher, "Bo1_Facet_Aabb"))
+        throw std::runtime_error("InsertionSortCollider: facets need Bo1_Facet_Aabb");
       p.collide.verletDist = c->verletDist;
       p.collide.forceInit = c->forceInitSort ? 1 : 0;
       p.engines[p.nEngines++] = WDG_E_COLLIDER;
@@ -306,6 +332,8 @@ wdg_program Scene::program(bool& hostEngines) {
         throw std::runtime_error("ContactLoop: spheres need Cg2_Sphere_Sphere_L6Geom");
       if (anyWall && !find_functor(cl->geoDisp, "Cg2_Wall_Sphere_L6Geom"))
         throw std::runtime_error("ContactLoop: walls need Cg2_Wall_Sphere_L6Geom");
+      if (anyFacet && !find_functor(cl->geoDisp, "Cg2_Facet_Sphere_L6Geom"))
+        throw std::runtime_error("ContactLoop: facets need Cg2_Facet_Sphere_L6Geom");
       p.contact.trsfRenorm = cl->geoDisp.empty() ? 100 : cl->geoDisp[0]->trsfRenorm;
       p.contact.applyForces = cl->applyForces ? 1 : 0;
       // CPhys dispatch by material type: HertzMat pairs get HertzPhys when its functor is

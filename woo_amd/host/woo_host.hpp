@@ -67,6 +67,12 @@ struct Wall : Shape {
   int axis = 0, sense = 0;
   const char* className() const override { return "Wall"; }
 };
+// Facet.hpp: triangle of 3 nodes; this engine supports fixed facets with static nodes
+struct Facet : Shape {
+  Vec3 fakeVel{};
+  double halfThick = 0;
+  const char* className() const override { return "Facet"; }
+};
 
 struct DemField;
 
@@ -107,6 +113,11 @@ struct Particle : Object {
       throw std::runtime_error("particle has no shape with exactly one node");
     return *shape->nodes[0];
   }
+  // the node that stands for the particle on the device (node 0; a facet's vertex 0)
+  Node& node0() const {
+    if (!shape || shape->nodes.empty() || !shape->nodes[0]) throw std::runtime_error("particle has no nodes");
+    return *shape->nodes[0];
+  }
   const char* className() const override { return "Particle"; }
 };
 
@@ -121,6 +132,9 @@ struct Bo1_Sphere_Aabb : BoundFunctor {
 struct Bo1_Wall_Aabb : BoundFunctor {
   const char* className() const override { return "Bo1_Wall_Aabb"; }
 };
+struct Bo1_Facet_Aabb : BoundFunctor {
+  const char* className() const override { return "Bo1_Facet_Aabb"; }
+};
 struct CGeomFunctor : Functor {
   int trsfRenorm = 100;  // Cg2_Any_Any_L6Geom__Base.trsfRenorm (L6Geom.hpp:37)
 };
@@ -129,6 +143,9 @@ struct Cg2_Sphere_Sphere_L6Geom : CGeomFunctor {
 };
 struct Cg2_Wall_Sphere_L6Geom : CGeomFunctor {
   const char* className() const override { return "Cg2_Wall_Sphere_L6Geom"; }
+};
+struct Cg2_Facet_Sphere_L6Geom : CGeomFunctor {
+  const char* className() const override { return "Cg2_Facet_Sphere_L6Geom"; }
 };
 struct CPhysFunctor : Functor {};
 struct Cp2_FrictMat_FrictPhys : CPhysFunctor {

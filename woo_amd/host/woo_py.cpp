@@ -184,6 +184,27 @@ PYBIND11_MODULE(_woo, m) {
       .def(py::init(&make_kw<Wall>))
       .def_readwrite("axis", &Wall::axis)
       .def_readwrite("sense", &Wall::sense);
+  auto restructure_shape = [](Shape& s) {
+    if (!s.nodes.empty() && s.nodes[0] && s.nodes[0]->field) s.nodes[0]->field->restructure();
+  };
+  py::class_<Facet, Shape, std::shared_ptr<Facet>>(m, "Facet", py::dynamic_attr())
+      .def(py::init(&make_kw<Facet>))
+      .def_property(
+          "fakeVel", [](Facet& f) { return arr(f.fakeVel.data(), 3); },
+          [restructure_shape](Facet& f, py::handle v) { restructure_shape(f); f.fakeVel = vec3(v); })
+      .def_property(
+          "halfThick", [](Facet& f) { return f.halfThick; },
+          [restructure_shape](Facet& f, double h) { restructure_shape(f); f.halfThick = h; })
+      // Facet::getNormal (Facet.cpp:74-77)
+      .def("getNormal", [](Facet& f) {
+        if (f.nodes.size() != 3) throw std::runtime_error("Facet needs 3 nodes");
+        const Vec3 &a = f.nodes[0]->pos, &b = f.nodes[1]->pos, &c = f.nodes[2]->pos;
+        const Vec3 u{b[0] - a[0], b[1] - a[1], b[2] - a[2]}, v{c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        Vec3 n{u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+        const double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        for (auto& x : n) x /= l;
+        return arr(n.data(), 3);
+      });
 
   // ---------------------------------------------------------------- particles
   py::class_<Particle, Object, std::shared_ptr<Particle>>(m, "Particle", py::dynamic_attr())
@@ -193,19 +214,19 @@ PYBIND11_MODULE(_woo, m) {
       .def_property(
           "material", [](Particle& p) { return p.material; },
           [](Particle& p, std::shared_ptr<Material> m) {
-            if (p.id >= 0) p.node().field->restructure();
+            if (p.id >= 0) p.node0().field->restructure();
             p.material = m;
           })
       .def_property(
           "mat", [](Particle& p) { return p.material; },
           [](Particle& p, std::shared_ptr<Material> m) {
-            if (p.id >= 0) p.node().field->restructure();
+            if (p.id >= 0) p.node0().field->restructure();
             p.material = m;
           })
       .def_property(
           "mask", [](Particle& p) { return p.mask; },
           [](Particle& p, uint32_t v) {
-            if (p.id >= 0) p.node().field->restructure();
+            if (p.id >= 0) p.node0().field->restructure();
             p.mask = v;
           })
       // Particle shorthands for the single node (Particle.hpp:100-120)
@@ -331,7 +352,7 @@ PYBIND11_MODULE(_woo, m) {
       .def_property_readonly("nodes", [](DemField& f) {
         py::list l;
         for (auto& p : f.particles)
-          if (p->node().integrated) l.append(p->shape->nodes[0]);
+          if (p->node0().integrated) l.append(p->shape->nodes[0]);
         return l;
       })
       .def("critDt", &DemField::pWaveDt)
@@ -356,6 +377,8 @@ PYBIND11_MODULE(_woo, m) {
       .def(py::init(&make_kw<Bo1_Sphere_Aabb>));
   py::class_<Bo1_Wall_Aabb, BoundFunctor, std::shared_ptr<Bo1_Wall_Aabb>>(m, "Bo1_Wall_Aabb", py::dynamic_attr())
       .def(py::init(&make_kw<Bo1_Wall_Aabb>));
+  py::class_<Bo1_Facet_Aabb, BoundFunctor, std::shared_ptr<Bo1_Facet_Aabb>>(m, "Bo1_Facet_Aabb", py::dynamic_attr())
+      .def(py::init(&make_kw<Bo1_Facet_Aabb>));
   py::class_<CGeomFunctor, Functor, std::shared_ptr<CGeomFunctor>>(m, "CGeomFunctor", py::dynamic_attr())
       .def_readwrite("trsfRenorm", &CGeomFunctor::trsfRenorm);
   py::class_<Cg2_Sphere_Sphere_L6Geom, CGeomFunctor, std::shared_ptr<Cg2_Sphere_Sphere_L6Geom>>(
@@ -364,6 +387,9 @@ PYBIND11_MODULE(_woo, m) {
   py::class_<Cg2_Wall_Sphere_L6Geom, CGeomFunctor, std::shared_ptr<Cg2_Wall_Sphere_L6Geom>>(
       m, "Cg2_Wall_Sphere_L6Geom", py::dynamic_attr())
       .def(py::init(&make_kw<Cg2_Wall_Sphere_L6Geom>));
+  py::class_<Cg2_Facet_Sphere_L6Geom, CGeomFunctor, std::shared_ptr<Cg2_Facet_Sphere_L6Geom>>(
+      m, "Cg2_Facet_Sphere_L6Geom", py::dynamic_attr())
+      .def(py::init(&make_kw<Cg2_Facet_Sphere_L6Geom>));
   py::class_<CPhysFunctor, Functor, std::shared_ptr<CPhysFunctor>>(m, "CPhysFunctor", py::dynamic_attr());
   py::class_<Cp2_FrictMat_FrictPhys, CPhysFunctor, std::shared_ptr<Cp2_FrictMat_FrictPhys>>(
       m, "Cp2_FrictMat_FrictPhys", py::dynamic_attr())

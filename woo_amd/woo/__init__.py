@@ -14,14 +14,14 @@ drive the HIP kernels through the C ABI in include/woo_amd.h.
 """
 import sys
 
-from . import core, dem, utils  # noqa: F401
+from . import core, dem, triangulated, utils  # noqa: F401
 
 
 def install():
-    """Register this package as ``woo`` (and ``woo.core``, ``woo.dem``, ``woo.utils``),
+    """Register this package as ``woo`` (and ``woo.core``, ``woo.dem``, ``woo.utils``, ``woo.triangulated``),
     so that scripts written for the reference import it unchanged."""
     me = sys.modules[__name__]
     sys.modules.setdefault("woo", me)
-    for sub in ("core", "dem", "utils"):
+    for sub in ("core", "dem", "utils", "triangulated"):
         sys.modules.setdefault("woo." + sub, sys.modules[__name__ + "." + sub])
     return me

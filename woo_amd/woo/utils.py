@@ -1,9 +1,10 @@
 """woo.utils: particle constructors and the default engine list (py/utils.py:77-290)."""
 import math
 
-from woo_amd._woo import (Bo1_Sphere_Aabb, Bo1_Wall_Aabb, Cg2_Sphere_Sphere_L6Geom, Cg2_Wall_Sphere_L6Geom,
-                          ContactLoop, Cp2_FrictMat_FrictPhys, DemField, DynDt, FrictMat, InsertionSortCollider,
-                          Law2_L6Geom_FrictPhys_IdealElPl, Leapfrog, Material, Node, Particle, Sphere, Wall)
+from woo_amd._woo import (Bo1_Facet_Aabb, Bo1_Sphere_Aabb, Bo1_Wall_Aabb, Cg2_Facet_Sphere_L6Geom,
+                          Cg2_Sphere_Sphere_L6Geom, Cg2_Wall_Sphere_L6Geom, ContactLoop, Cp2_FrictMat_FrictPhys,
+                          DemField, DynDt, Facet, FrictMat, InsertionSortCollider, Law2_L6Geom_FrictPhys_IdealElPl,
+                          Leapfrog, Material, Node, Particle, Sphere, Wall)
 
 
 def spherePWaveDt(radius, density, young):
@@ -108,10 +109,34 @@ def defaultEngines(damping=0., gravity=None, verletDist=-.05, kinSplit=False, do
         setattr(law[0], k, v)
     return [
         Leapfrog(damping=damping, reset=True, label='leapfrog'),
-        InsertionSortCollider([Bo1_Sphere_Aabb(), Bo1_Wall_Aabb()], label='collider', verletDist=verletDist),
-        ContactLoop([Cg2_Sphere_Sphere_L6Geom(), Cg2_Wall_Sphere_L6Geom()], cp2, law, applyForces=True,
-                    label='contactLoop'),
+        InsertionSortCollider([Bo1_Sphere_Aabb(), Bo1_Facet_Aabb(), Bo1_Wall_Aabb()], label='collider',
+                              verletDist=verletDist),
+        ContactLoop([Cg2_Sphere_Sphere_L6Geom(), Cg2_Facet_Sphere_L6Geom(), Cg2_Wall_Sphere_L6Geom()], cp2, law,
+                    applyForces=True, label='contactLoop'),
     ] + ([DynDt(stepPeriod=dynDtPeriod, label='dynDt')] if dynDtPeriod > 0 else [])
+
+
+def facet(vertices, fakeVel=None, halfThick=0., fixed=True, wire=True, color=None, highlight=False,
+          mat=defaultMaterial, visible=True, mask=DemField.defaultBoundaryMask):
+    """Triangle particle from 3 vertices (utils.py:318-344). The device engine supports fixed
+    facets with static nodes; a moving surface is modelled by ``fakeVel``."""
+    if not fixed:
+        raise NotImplementedError("only fixed facets are supported (model moving surfaces with fakeVel)")
+    p = Particle()
+    nodes = [v if isinstance(v, Node) else Node(pos=v) for v in vertices]
+    if len(nodes) != 3:
+        raise ValueError("a facet needs 3 vertices")
+    p.shape = Facet(halfThick=halfThick)
+    if fakeVel is not None:
+        p.shape.fakeVel = fakeVel
+    p.mat = _material(mat)
+    p.shape.nodes = nodes
+    for n in nodes:
+        n.dem.mass = 0.
+        n.dem.inertia = (0., 0., 0.)
+        n.dem.blocked = "xyzXYZ"
+    p.mask = mask
+    return p
 
 
 def coordNumber(dem=None, skipFree=False, S=None):
