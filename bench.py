@@ -214,6 +214,109 @@ def measured_traffic(kernel):
     return None, None
 
 
+def drum_workload(n, dev, seed=4):
+    """Rotating drum: 1024 facets (512 circumferential strips of 2 triangles, R=0.5 m, length
+    0.4 m, axis x) whose rotation at 2 rad/s is given by Facet.fakeVel (triangulated.cylinder
+    with angVel, realRot=False), end walls at x=+-0.2 m, n spheres with radius
+    lognormal(median 1.5 mm, sigma 0.3) clipped to [0.8, 4] mm packed into the lower part,
+    velocities N(0, 0.1 m/s), gravity -z, Hertz as the periodic workload."""
+    from woo_amd.pack import drum_fill
+    from woo_amd._native import FACET, SPHERE, WALL
+
+    R, length, div, omega = 0.5, 0.4, 512, 2.0
+    Rin = R * math.cos(math.pi / div)
+    t0 = time.perf_counter()
+    pos, radius, info = drum_fill(n, Rin, length, 0.8e-3, 4e-3, 1.5e-3, 0.3, seed=seed, dev=dev)
+    log(f"drum packing: n={n} fill={info['fill']:.3f} box spheres={info['boxSpheres']} kept={info['kept']} "
+        f"({time.perf_counter() - t0:.1f} s)")
+    ax = np.array([1., 0., 0.])
+    th = np.linspace(0, 2 * math.pi, div + 1)
+    verts, fake = [], []
+    for k in range(div):
+        a0 = (-length / 2, R * math.cos(th[k]), R * math.sin(th[k]))
+        a1 = (-length / 2, R * math.cos(th[k + 1]), R * math.sin(th[k + 1]))
+        b0, b1 = (length / 2,) + a0[1:], (length / 2,) + a1[1:]
+        for tri in ((a0, b0, b1), (a0, b1, a1)):
+            v = np.array(tri)
+            nrm = np.cross(v[1] - v[0], v[2] - v[0])
+            nrm /= np.linalg.norm(nrm)
+            verts.append(v.reshape(9))
+            fake.append(-R * omega * np.cross(nrm, ax))
+    nf, nw = len(verts), 2
+    N = nw + nf + n
+    shape = np.concatenate([np.full(nw, WALL), np.full(nf, FACET), np.full(n, SPHERE)]).astype(np.int32)
+    P = np.zeros((N, 3))
+    P[0, 0], P[1, 0] = -length / 2, length / 2
+    P[nw:nw + nf] = np.array(verts)[:, :3]
+    P[nw + nf:] = pos
+    fv = np.zeros((N, 9))
+    fv[nw:nw + nf] = np.array(verts)
+    fk = np.zeros((N, 3))
+    fk[nw:nw + nf] = np.array(fake)
+    rad = np.zeros(N)
+    rad[nw + nf:] = radius
+    mass = np.zeros(N)
+    mass[nw + nf:] = (4 / 3.) * math.pi * radius ** 3 * RHO
+    inertia = np.repeat((0. + 1. * (2. / 5.) * mass * (rad * rad))[:, None], 3, 1)
+    rng = np.random.default_rng(5)
+    vel = np.zeros((N, 3))
+    vel[nw + nf:] = rng.normal(0., 0.1, size=(n, 3))
+    flags = np.zeros(N, np.uint32)
+    flags[:nw + nf] = 63
+    integ = np.ones(N, np.uint8)
+    integ[:nw + nf] = 0
+    mask = np.full(N, 5, np.uint32)
+    mask[:nw + nf] = 3
+    axis = np.zeros(N, np.int32)
+    sense = np.zeros(N, np.int32)
+    sense[0], sense[1] = 1, -1
+    dt = 0.3 * float(np.min(radius)) / math.sqrt(YOUNG / RHO)
+    mat = dict(kind=1, density=RHO, young=YOUNG, ktDivKn=.2, tanPhi=TANPHI, surfEnergy=0., alpha=0.)
+    arrays = dict(shape=shape, pos=P, radius=rad, wallAxis=axis, wallSense=sense, mask=mask, flags=flags,
+                  integrated=integ, vel=vel, mass=mass, inertia=inertia, facetVerts=fv, fakeVel=fk)
+    return dict(arrays=arrays, mat=mat, dt=dt, n=n, facets=nf, fill=info["fill"])
+
+
+def drum_form(args):
+    """The drum workload in the FP64 and FP32 contact forms: throughput of each, and the
+    FP32 observables against FP64 (kinetic energy, coordination number)."""
+    from woo_amd._native import Device
+
+    dev = Device(0)
+    wl = drum_workload(args.drum_n, dev)
+    out = dict(workload=f"rotating drum, {wl['facets']} facets (fakeVel 2 rad/s) + 2 end walls, {args.drum_n} "
+                        f"spheres r~lognormal(1.5 mm, 0.3) in [0.8, 4] mm, gravity, Hertz",
+               n=args.drum_n, dt=wl["dt"], fill=wl["fill"], steps=args.drum_steps, warmup=args.warmup)
+    cp = contact_params()
+    for prec in (64, 32):
+        dev.set_field(gravity=(0., 0., -9.81), periodic=False)
+        dev.set_materials([wl["mat"]])
+        dev.upload_particles(**wl["arrays"])
+        dev.set_precision(prec)
+        step, _ = dev.run(ENGINES, args.warmup, 0, wl["dt"], contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
+        dev.sync()
+        st0 = dev.stats()
+        t0 = time.perf_counter()
+        step, _ = dev.run(ENGINES, args.drum_steps, step, wl["dt"], contact=cp, leapfrog=dict(damping=0.),
+                          dynDtPeriod=0)
+        dev.sync()
+        el = time.perf_counter() - t0
+        st1 = dev.stats()
+        big = 1e300
+        ob = dev.observables((-big,) * 3, (big,) * 3)
+        out[f"fp{prec}"] = dict(value=args.drum_n * args.drum_steps / el, unit="particle-steps/s",
+                                ms_per_step=el / args.drum_steps * 1e3, real_contacts=st1["nReal"],
+                                contact_slots=st1["nCandidates"], refreshes=st1["nRefresh"] - st0["nRefresh"],
+                                kinetic_energy=dev.kinetic_energy(), coord_number=ob["coordNumber"])
+        log(f"drum fp{prec}: {out[f'fp{prec}']}")
+        dev.set_precision(64)
+    dev.close()
+    a, b = out["fp64"], out["fp32"]
+    out["fp32_speedup_vs_fp64"] = b["value"] / a["value"]
+    out["fp32_rel_err"] = {k: abs(b[k] - a[k]) / abs(a[k]) for k in ("kinetic_energy", "coord_number")}
+    return out
+
+
 def pmc_probe(dev, wl, args):
     dt, cp = wl["dt"], contact_params()
     step, _ = dev.run(ENGINES, args.warmup, 0, dt, contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
@@ -255,6 +358,9 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=50)
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--fp32", type=int, default=1, help="also time the FP32 contact form")
+    ap.add_argument("--drum", type=int, default=1, help="also time the rotating-drum workload (facets)")
+    ap.add_argument("--drum-n", type=int, default=4_000_000)
+    ap.add_argument("--drum-steps", type=int, default=500)
     ap.add_argument("--pmc-probe", type=int, default=0,
                     help="for rocprofv3 --pmc runs: warm up, launch the contact loop and leapfrog this many "
                          "times, then a device-to-device hipMemcpy of 512 MiB whose FETCH_SIZE/WRITE_SIZE "
@@ -311,6 +417,11 @@ def main():
                 traffic=traffic, traffic_source=tsrc, kernel=dom, bytes_per_launch=B[dom], launch_us=times[dom] * 1e6,
                 other_kernels={k: dict(bytes_per_launch=B[k], launch_us=times[k] * 1e6,
                                        achieved_GBs=B[k] / times[k] / 1e9) for k in times if k != dom})
+    drum = None
+    if args.drum:
+        dev.close()
+        drum = drum_form(args)
+        log(f"drum form: {drum}")
     base = None
     if args.cpu_baseline:
         log("timing the CPU oracle on the same workload ...")
@@ -328,9 +439,11 @@ def main():
         "roofline": roof,
         "cpu_baseline": base,
         "fp32_form": fp32,
+        "drum_form": drum,
     }
     print(json.dumps(out), flush=True)
-    dev.close()
+    if not args.drum:
+        dev.close()
 
 
 if __name__ == "__main__":

@@ -197,9 +197,11 @@ def test_triangulated_cylinder_drum_runs():
     rng = np.random.default_rng(3)
     for x in np.arange(-.015, .016, .006):
         for y in np.arange(-.03, .031, .006):
-            for z in np.arange(-.04, -.005, .006):
-                if math.hypot(y, z) < .043:
-                    S.dem.par.add(woo.utils.sphere((x, y, z), .0025 + .0003 * rng.uniform(), mat=m))
+            for z in np.arange(-.046, -.005, .006):
+                if math.hypot(y, z) < .046:
+                    p = woo.utils.sphere((x, y, z), .0025 + .0003 * rng.uniform(), mat=m)
+                    p.vel = (0., 0., -.5)
+                    S.dem.par.add(p)
     S.engines = woo.utils.defaultEngines(damping=0., dynDtPeriod=0, cp2=Cp2_HertzMat_HertzPhys(poisson=.25, en=.6),
                                          law=Law2_L6Geom_HertzPhys_DMT())
     S.dt = .3 * woo.utils.pWaveDt(S)

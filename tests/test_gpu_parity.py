@@ -227,7 +227,11 @@ def test_force_state_changes_mid_run():
     e, n_diff = traj_metric(o, g, sc, sc["walls_first"])
     print("positions differing", n_diff, "worst", e)
     assert e <= TRAJ_TOL
-    np.testing.assert_array_equal(o.get("force"), g.d.get("force"))
+    # spheres sum in the canonical order: identical; the walls' thousands of contacts are
+    # summed by a parallel tree, equal up to rounding
+    ns = len(sc["radius"])
+    np.testing.assert_array_equal(o.get("force")[:ns], g.d.get("force")[:ns])
+    np.testing.assert_allclose(g.d.get("force")[ns:], o.get("force")[ns:], rtol=1e-12, atol=0)
 
 
 def test_stress_and_kinetic_energy_match_oracle():

@@ -287,8 +287,8 @@ int wdg_ctx_destroy(wdg_ctx* ctx) {
     c.perm.release(); c.invPerm.release(); c.cellStart.release(); c.cellEnd.release();
     c.scount.release(); c.soff.release(); c.stPartner.release(); c.stInfo.release(); c.stDist.release();
     c.adjEnt.release(); c.adjDeg.release(); c.stNb.release(); c.sdeg.release(); c.posf.release();
-    c.fgeo.release(); c.bnd.release(); c.bigOff.release(); c.bigRank.release(); c.bigVal.release();
-    c.bigValTmp.release(); c.bigDest.release(); c.bigKey.release(); c.bigKeyTmp.release();
+    c.fgeo.release(); c.bnd.release(); c.pairMax.release(); c.bigOff.release(); c.bigRank.release(); c.bigVal.release();
+    c.bigValTmp.release(); c.bigDest.release(); c.bigKey.release(); c.bigKeyTmp.release(); c.bigList.release();
     c.sortHist.release(); c.scanAux.release(); c.red.release(); c.xfer.release(); c.dflags.release();
     if (c.hflags) (void)hipHostFree(c.hflags);
     if (c.stream) (void)hipStreamDestroy(c.stream);

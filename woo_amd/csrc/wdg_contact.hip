@@ -208,7 +208,9 @@ __global__ __launch_bounds__(GROUP_BS) void k_cfilter(ContactArgs a) {
 constexpr int CF_BS = 256;
 constexpr int CF_BLOCKS = 4;
 
-template <bool PERIODIC, int LAW>
+// FACETS: the scene has facets (a separate instantiation keeps the facet geometry's
+// registers out of the other scenes' kernel)
+template <bool PERIODIC, int LAW, bool FACETS>
 __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
   const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
   const int64_t g = lb / CF_BLOCKS;
@@ -243,7 +245,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
       uN = dist - (r1 + r2);
       normal = div3(relPos, dist);
       contPt = add3(posA, mul3(normal, r1 + 0.5 * uN));
-    } else if (type == CT_FS) {
+    } else if (FACETS && type == CT_FS) {
       // Cg2_Facet_Sphere_L6Geom::go (Facet.cpp:333-390), pA is the facet; n21lim is NaN
       // (adjustBoundaryContactGeom returns at once); static nodes: the velocity at the contact
       // point is fakeVel (zero when its x is NaN), the angular velocity zero
@@ -493,7 +495,7 @@ __global__ __launch_bounds__(GROUP_BS) void k_cfilter32(ContactArgs a) {
   if (threadIdx.x == 0) a.ngcnt[g] = running;
 }
 
-template <bool PERIODIC, int LAW>
+template <bool PERIODIC, int LAW, bool FACETS>
 __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
   const float *oN0 = reinterpret_cast<const float*>(a.oN0), *oN1 = reinterpret_cast<const float*>(a.oN1),
               *oN2 = reinterpret_cast<const float*>(a.oN2), *oY0 = reinterpret_cast<const float*>(a.oY0),
@@ -532,7 +534,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
       uN = dist - (r1 + r2);
       normal = div3(r, dist);
       cA = mul3(normal, r1 + 0.5f * uN);
-    } else if ((info & CI_TYPE_MASK) == CT_FS) {
+    } else if (FACETS && (info & CI_TYPE_MASK) == CT_FS) {
       const double* g = a.fgeo + (int64_t)FGEO * pA;
       const float radius = (float)a.rad[pB], ht = (float)g[12];
       const f3 e1 = tof3(mk3(g[3] - g[0], g[4] - g[1], g[5] - g[2]));
@@ -612,7 +614,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
       // vmove1*dt since the last step (a facet's does not). pos1 is pA's node (sphere, wall),
       // or the contact point for a facet
       const f3 mid = mul3(add3(prevCA, cA), .5f);
-      const f3 c1x = (info & CI_TYPE_MASK) == CT_FS ? add3(sub3(mid, cA), mul3(vel1, .5f * dt)) : mid;
+      const f3 c1x = FACETS && (info & CI_TYPE_MASK) == CT_FS ? add3(sub3(mid, cA), mul3(vel1, .5f * dt)) : mid;
       const f3 c2x = add3(sub3(mid, r), mul3(sub3(vel2, vmove1), .5f * dt));
       const f3 relVel = sub3(add3(vel2, cross3(av2, c2x)), add3(vel1, cross3(av1, c1x)));
       gvel = mkf3(dot3(M0, relVel), dot3(M1, relVel), dot3(M2, relVel));
@@ -750,6 +752,7 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
   a.viscAlpha = viscAlpha;
   a.noSlip = p->noSlip; a.noBreak = p->noBreak; a.noFrict = p->noFrict; a.noAttraction = p->noAttraction;
   const bool per = c.field.periodic != 0;
+  const bool fac = c.nFacet > 0;
   const bool hertz = p->lawKind == WDG_LAW_HERTZDMT;
   const dim3 gfil((unsigned)ng), bfil(GROUP_BS);
   if (c.precision == 32) {
@@ -763,11 +766,13 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
     }
     WDG_LAUNCH_CHECK();
     if (hertz) {
-      if (per) hipLaunchKernelGGL((k_cforce32<true, WDG_LAW_HERTZDMT>), gforce, bforce, 0, c.stream, a);
-      else hipLaunchKernelGGL((k_cforce32<false, WDG_LAW_HERTZDMT>), gforce, bforce, 0, c.stream, a);
+      if (per) hipLaunchKernelGGL((k_cforce32<true, WDG_LAW_HERTZDMT, false>), gforce, bforce, 0, c.stream, a);
+      else if (fac) hipLaunchKernelGGL((k_cforce32<false, WDG_LAW_HERTZDMT, true>), gforce, bforce, 0, c.stream, a);
+      else hipLaunchKernelGGL((k_cforce32<false, WDG_LAW_HERTZDMT, false>), gforce, bforce, 0, c.stream, a);
     } else {
-      if (per) hipLaunchKernelGGL((k_cforce32<true, WDG_LAW_IDEALELPL>), gforce, bforce, 0, c.stream, a);
-      else hipLaunchKernelGGL((k_cforce32<false, WDG_LAW_IDEALELPL>), gforce, bforce, 0, c.stream, a);
+      if (per) hipLaunchKernelGGL((k_cforce32<true, WDG_LAW_IDEALELPL, false>), gforce, bforce, 0, c.stream, a);
+      else if (fac) hipLaunchKernelGGL((k_cforce32<false, WDG_LAW_IDEALELPL, true>), gforce, bforce, 0, c.stream, a);
+      else hipLaunchKernelGGL((k_cforce32<false, WDG_LAW_IDEALELPL, false>), gforce, bforce, 0, c.stream, a);
     }
     WDG_LAUNCH_CHECK();
     c.rcur = to;
@@ -784,11 +789,13 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
   WDG_LAUNCH_CHECK();
   const dim3 gforce((unsigned)(ng * CF_BLOCKS)), bforce(CF_BS);
   if (hertz) {
-    if (per) hipLaunchKernelGGL((k_cforce<true, WDG_LAW_HERTZDMT>), gforce, bforce, 0, c.stream, a);
-    else hipLaunchKernelGGL((k_cforce<false, WDG_LAW_HERTZDMT>), gforce, bforce, 0, c.stream, a);
+    if (per) hipLaunchKernelGGL((k_cforce<true, WDG_LAW_HERTZDMT, false>), gforce, bforce, 0, c.stream, a);
+    else if (fac) hipLaunchKernelGGL((k_cforce<false, WDG_LAW_HERTZDMT, true>), gforce, bforce, 0, c.stream, a);
+    else hipLaunchKernelGGL((k_cforce<false, WDG_LAW_HERTZDMT, false>), gforce, bforce, 0, c.stream, a);
   } else {
-    if (per) hipLaunchKernelGGL((k_cforce<true, WDG_LAW_IDEALELPL>), gforce, bforce, 0, c.stream, a);
-    else hipLaunchKernelGGL((k_cforce<false, WDG_LAW_IDEALELPL>), gforce, bforce, 0, c.stream, a);
+    if (per) hipLaunchKernelGGL((k_cforce<true, WDG_LAW_IDEALELPL, false>), gforce, bforce, 0, c.stream, a);
+    else if (fac) hipLaunchKernelGGL((k_cforce<false, WDG_LAW_IDEALELPL, true>), gforce, bforce, 0, c.stream, a);
+    else hipLaunchKernelGGL((k_cforce<false, WDG_LAW_IDEALELPL, false>), gforce, bforce, 0, c.stream, a);
   }
   WDG_LAUNCH_CHECK();
   c.rcur = to;

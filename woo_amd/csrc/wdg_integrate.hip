@@ -16,7 +16,7 @@
 namespace wdg {
 
 struct GatherArgs {
-  int64_t n;
+  int64_t n, nW;
   const uint32_t *adj, *adjOff;
   const uint32_t* flags;
   RealView rv;
@@ -27,12 +27,10 @@ struct GatherArgs {
 // F32: the FP32 contact form's record (global force, branch vectors from pA and from pB's
 // image as floats); the sums stay FP64
 template <bool PERIODIC, bool F32>
-__device__ __forceinline__ void gather_all(const GatherArgs& g, int64_t i, d3 pos, d3& F, d3& T) {
-  const uint32_t k0 = g.adjOff[i], k1 = g.adjOff[i + 1];
-  for (uint32_t k = k0; k < k1; k++) {
-    const uint32_t e = g.adj[k];
+__device__ __forceinline__ void gather_one(const GatherArgs& g, uint32_t e, d3 pos, d3& F, d3& T) {
+  {
     const uint32_t slot = e & ~ADJ_PA;
-    if (!rv_real(g.rv, slot)) continue;
+    if (!rv_real(g.rv, slot)) return;
     if (F32) {
       const float4* r =
           reinterpret_cast<const float4*>(reinterpret_cast<const float*>(g.gp) + (int64_t)GP32 * rv_index(g.rv, slot));
@@ -42,7 +40,7 @@ __device__ __forceinline__ void gather_all(const GatherArgs& g, int64_t i, d3 po
       const d3 xc = isPA ? mk3(r0.w, r1.x, r1.y) : mk3(r1.z, r1.w, r2.x);
       F = add3(F, Fc);
       T = add3(T, cross3(xc, Fc));
-      continue;
+      return;
     }
     const double2* r = reinterpret_cast<const double2*>(g.gp + (int64_t)GP * rv_index(g.rv, slot));
     const double2 r0 = r[0], r1 = r[1], r2 = r[2];
@@ -62,14 +60,52 @@ __device__ __forceinline__ void gather_all(const GatherArgs& g, int64_t i, d3 po
 }
 
 template <bool PERIODIC, bool F32>
+__device__ __forceinline__ void gather_all(const GatherArgs& g, int64_t i, d3 pos, d3& F, d3& T) {
+  const uint32_t k0 = g.adjOff[i], k1 = g.adjOff[i + 1];
+  for (uint32_t k = k0; k < k1; k++) gather_one<PERIODIC, F32>(g, g.adj[k], pos, F, T);
+}
+
+// Contact forces on the boundary bodies (storage [0, nW)), which may touch thousands of
+// spheres: one workgroup per body, each thread summing every 256th entry of the body's list
+// in order, then a fixed-order tree in LDS, added to the value in the force arrays (which
+// always hold the current force: the reset value, or the gathered one). The spheres'
+// gathers (k_gather, k_leapfrog) skip these bodies. Deterministic; the summation order
+// differs from the sequential one, by rounding.
+template <bool PERIODIC, bool F32>
+__global__ __launch_bounds__(256) void k_gather_bnd(GatherArgs g, double* fx, double* fy, double* fz, double* tx,
+                                                     double* ty, double* tz, const double* __restrict__ px,
+                                                     const double* __restrict__ py, const double* __restrict__ pz) {
+  __shared__ double sh[6][256];
+  const int64_t i = blockIdx.x;
+  const uint32_t k0 = g.adjOff[i], k1 = g.adjOff[i + 1];
+  if (k0 == k1 || flag_shape(g.flags[i]) == WDG_FACET) return;
+  const d3 pos = mk3(px[i], py[i], pz[i]);
+  d3 F = mk3(0., 0., 0.), T = mk3(0., 0., 0.);
+  for (uint32_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) gather_one<PERIODIC, F32>(g, g.adj[k], pos, F, T);
+  const int t = threadIdx.x;
+  sh[0][t] = F.x; sh[1][t] = F.y; sh[2][t] = F.z;
+  sh[3][t] = T.x; sh[4][t] = T.y; sh[5][t] = T.z;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (t < st)
+      for (int q = 0; q < 6; q++) sh[q][t] += sh[q][t + st];
+    __syncthreads();
+  }
+  if (t == 0) {
+    fx[i] = fx[i] + sh[0][0]; fy[i] = fy[i] + sh[1][0]; fz[i] = fz[i] + sh[2][0];
+    tx[i] = tx[i] + sh[3][0]; ty[i] = ty[i] + sh[4][0]; tz[i] = tz[i] + sh[5][0];
+  }
+}
+
+template <bool PERIODIC, bool F32>
 __global__ __launch_bounds__(256) void k_gather(GatherArgs g, double* fx, double* fy, double* fz, double* tx,
                                                  double* ty, double* tz, const double* __restrict__ px,
                                                  const double* __restrict__ py, const double* __restrict__ pz) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.n) return;
   if (g.adjOff[i] == g.adjOff[i + 1]) return;
-  // multinodal particles (facets) take no force from the contact loop (ContactLoop.cpp:270-276)
-  if (flag_shape(g.flags[i]) == WDG_FACET) return;
+  // boundary bodies: k_gather_bnd; facets take no force from the contact loop (multinodal)
+  if (i < g.nW) return;
   d3 F = mk3(fx[i], fy[i], fz[i]), T = mk3(tx[i], ty[i], tz[i]);
   gather_all<PERIODIC, F32>(g, i, mk3(px[i], py[i], pz[i]), F, T);
   fx[i] = F.x; fy[i] = F.y; fz[i] = F.z;
@@ -102,9 +138,11 @@ __global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
     const uint32_t fl = a.flags[i];
     const bool integrated = (fl & F_INTEGRATED) != 0u;
     const bool grav = a.hasGravity && !(fl & F_GRAVITY_SKIP);
+    // a boundary body's contact forces were summed into its arrays by k_gather_bnd
+    const bool pre = a.gather && i < g.nW;
     d3 pos = mk3(a.px[i], a.py[i], a.pz[i]);
     d3 F, T;
-    if (a.clean && integrated) {
+    if (a.clean && integrated && !pre) {
       // the value the last reset stored (ForceResetter / Leapfrog.reset)
       const double m = a.mass[i];
       F = grav ? mk3(m * a.g[0], m * a.g[1], m * a.g[2]) : mk3(0., 0., 0.);
@@ -113,10 +151,9 @@ __global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
       F = mk3(a.fx[i], a.fy[i], a.fz[i]);
       T = mk3(a.tx[i], a.ty[i], a.tz[i]);
     }
-    // multinodal particles (facets) take no force from the contact loop (ContactLoop.cpp:270-276)
-    if (a.gather && flag_shape(fl) != WDG_FACET) gather_all<PERIODIC, F32>(g, i, pos, F, T);
+    if (a.gather && !pre) gather_all<PERIODIC, F32>(g, i, pos, F, T);
     if (!integrated) {
-      if (a.gather) {
+      if (a.gather && !pre) {
         a.fx[i] = F.x; a.fy[i] = F.y; a.fz[i] = F.z;
         a.tx[i] = T.x; a.ty[i] = T.y; a.tz[i] = T.z;
       }
@@ -157,7 +194,7 @@ __global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
       a.wx[i] = av.x; a.wy[i] = av.y; a.wz[i] = av.z;
       // Leapfrog.cpp:240-244; when the arrays already hold the reset value nothing is stored
       if (a.reset) {
-        if (!a.clean) {
+        if (!a.clean || pre) {
           F = grav ? mk3(m * a.g[0], m * a.g[1], m * a.g[2]) : mk3(0., 0., 0.);
           a.fx[i] = F.x; a.fy[i] = F.y; a.fz[i] = F.z;
           a.tx[i] = 0.; a.ty[i] = 0.; a.tz[i] = 0.;
@@ -211,8 +248,18 @@ static GatherArgs gather_args(Ctx& c) {
   GatherArgs g{};
   g.n = c.n;
   g.adj = C.adj.p; g.adjOff = C.adjOff.p; g.rv = c.rview(); g.gp = c.reals().gp.p; g.flags = c.flags.p;
+  g.nW = c.nW;
   for (int k = 0; k < 3; k++) g.L[k] = c.field.cell[k];
   return g;
+}
+
+static void gather_boundary(Ctx& c, const GatherArgs& g) {
+  if (c.nW == 0) return;
+  auto* kern = c.precision == 32 ? k_gather_bnd<false, true>
+                                 : (c.field.periodic ? k_gather_bnd<true, false> : k_gather_bnd<false, false>);
+  hipLaunchKernelGGL(kern, dim3((unsigned)c.nW), dim3(256), 0, c.stream, g, c.force[0].p, c.force[1].p, c.force[2].p,
+                     c.torque[0].p, c.torque[1].p, c.torque[2].p, c.pos[0].p, c.pos[1].p, c.pos[2].p);
+  WDG_LAUNCH_CHECK();
 }
 
 void flush_forces(Ctx& c) {
@@ -221,6 +268,7 @@ void flush_forces(Ctx& c) {
   c.forceClean = false;
   if (c.con().n == 0 || c.n == 0) return;
   GatherArgs g = gather_args(c);
+  gather_boundary(c, g);
   dim3 grid(nblocks(c.n, 256)), block(256);
   auto* kern = c.precision == 32 ? k_gather<false, true> : (c.field.periodic ? k_gather<true, false> : k_gather<false, false>);
   hipLaunchKernelGGL(kern, grid, block, 0, c.stream, g, c.force[0].p, c.force[1].p, c.force[2].p, c.torque[0].p,
@@ -253,6 +301,7 @@ void leapfrog(Ctx& c, const wdg_leapfrog_params* p) {
   a.dirty = c.dflags.p + DIRTY_PP + s;
   a.dirtyNext = c.dflags.p + DIRTY_PP + (1 - s);
   GatherArgs g = gather_args(c);
+  if (a.gather) gather_boundary(c, g);
   dim3 grid(nblocks(c.n, 256)), block(256);
   auto* kern = c.precision == 32 ? k_leapfrog<false, true>
                                  : (c.field.periodic ? k_leapfrog<true, false> : k_leapfrog<false, false>);

@@ -229,11 +229,12 @@ struct Ctx {
   DBuf<uint32_t> stPartner, stInfo;     // staged pairs, a fixed number per sphere
   DBuf<uint64_t> stDist;
   DBuf<uint32_t> stNb, sdeg;            // staged neighbours per sphere, their number
+  DBuf<uint32_t> pairMax;               // per pair-generation workgroup: largest pair / neighbour count
   DBuf<float4> posf;                    // float candidate records of the last refresh
   DBuf<uint64_t> adjEnt;                // adjacency entries with their sort key
   DBuf<uint32_t> adjDeg;
   // adjacency of high-degree particles (boundary bodies), sorted by one radix sort
-  DBuf<uint32_t> bigOff, bigRank, bigVal, bigValTmp, bigDest;
+  DBuf<uint32_t> bigOff, bigRank, bigVal, bigValTmp, bigDest, bigList;
   DBuf<uint64_t> bigKey, bigKeyTmp;
   DBuf<uint32_t> sortHist, scanAux;
   DBuf<uint32_t> act;     // slot of each state index (contact loop), per-slot flags (relaxation)
