@@ -317,6 +317,158 @@ def drum_form(args):
     return out
 
 
+def batched_scenes(n_scenes=256, seed=5, r=5e-3, rho=3000.):
+    """Parametric study of gravity depositions (the 2000-sphere wall-box scene): particle count
+    log-uniform in [500, 50000], tanPhi uniform in [0.1, 0.9], E log-uniform in [1e6, 1e8] Pa,
+    box edge 0.2*(N/2000)^(1/3) m, dt = 0.5 pWaveDt. Non-overlapping random placement: a random
+    subset of the sites of a cubic lattice filling the box, each jittered within its cell.
+    Scenes are laid out side by side (sorted by size) so that each occupies its own region."""
+    rng = np.random.default_rng(seed)
+    N = np.exp(rng.uniform(math.log(500), math.log(50000), size=n_scenes)).astype(np.int64)
+    tanPhi = rng.uniform(.1, .9, size=n_scenes)
+    E = np.exp(rng.uniform(math.log(1e6), math.log(1e8), size=n_scenes))
+    scenes = []
+    for i in range(n_scenes):
+        box = 0.2 * (N[i] / 2000.) ** (1 / 3.)
+        m = int(math.ceil(N[i] ** (1 / 3.)))
+        a = box / m
+        g = np.stack(np.meshgrid(*[np.arange(m)] * 3, indexing="ij"), -1).reshape(-1, 3)
+        g = g[rng.permutation(len(g))[:N[i]]]
+        pos = (g + 0.5) * a + rng.uniform(-1, 1, size=(N[i], 3)) * (0.49 * a - r)
+        scenes.append(dict(n=int(N[i]), box=box, pos=pos, tanPhi=float(tanPhi[i]), young=float(E[i]),
+                           dt=0.5 * r / math.sqrt(E[i] / rho)))
+    # side by side: layers of rows, largest first
+    order = np.argsort([-sc["box"] for sc in scenes], kind="stable")
+    side = int(math.ceil(n_scenes ** (1 / 3.)))
+    gap = 0.05
+    x = y = z = 0.
+    rowH = layH = 0.
+    for k, i in enumerate(order):
+        sc = scenes[i]
+        if k % side == 0 and k:
+            x = 0.
+            y += rowH + gap
+            rowH = 0.
+        if k % (side * side) == 0 and k:
+            y = 0.
+            z += layH + gap
+            layH = 0.
+        sc["offset"] = np.array([x, y, z])
+        x += sc["box"] + gap
+        rowH = max(rowH, sc["box"])
+        layH = max(layH, sc["box"])
+    return scenes, r, rho
+
+
+def batch_upload(dev, scenes, r, rho, idx=None):
+    """scenes idx (all by default) into one context: spheres then the 6 walls of each scene"""
+    from woo_amd._native import SPHERE, WALL
+
+    idx = range(len(scenes)) if idx is None else idx
+    parts = dict(shape=[], pos=[], radius=[], wallAxis=[], wallSense=[], material=[], mask=[], flags=[],
+                 integrated=[], mass=[], inertia=[], scene=[])
+    mats, dts = [], []
+    for k, i in enumerate(idx):
+        sc = scenes[i]
+        n, off, b = sc["n"], sc["offset"], sc["box"]
+        m = (4 / 3.) * math.pi * r ** 3 * rho
+        wp = []
+        for sense, coord in ((1, 0.), (-1, b)):
+            for ax in (0, 1, 2):
+                p = np.zeros(3)
+                p[ax] = coord
+                wp.append((p + off, ax, sense))
+        parts["shape"].append(np.concatenate([np.full(n, SPHERE), np.full(6, WALL)]))
+        parts["pos"].append(np.concatenate([sc["pos"] + off, np.array([w[0] for w in wp])]))
+        parts["radius"].append(np.concatenate([np.full(n, r), np.zeros(6)]))
+        parts["wallAxis"].append(np.concatenate([np.zeros(n), [w[1] for w in wp]]))
+        parts["wallSense"].append(np.concatenate([np.zeros(n), [w[2] for w in wp]]))
+        parts["material"].append(np.full(n + 6, k))
+        parts["mask"].append(np.concatenate([np.full(n, 5), np.full(6, 3)]))
+        parts["flags"].append(np.concatenate([np.zeros(n), np.full(6, 63)]))
+        parts["integrated"].append(np.concatenate([np.ones(n), np.zeros(6)]))
+        parts["mass"].append(np.concatenate([np.full(n, m), np.zeros(6)]))
+        parts["inertia"].append(np.concatenate([np.full((n, 3), 0. + 1. * (2. / 5.) * m * (r * r)), np.zeros((6, 3))]))
+        parts["scene"].append(np.full(n + 6, k))
+        mats.append(dict(kind=0, density=rho, young=sc["young"], ktDivKn=.2, tanPhi=sc["tanPhi"], surfEnergy=0.,
+                         alpha=0.))
+        dts.append(sc["dt"])
+    arr = {k: np.concatenate(v) for k, v in parts.items()}
+    arr["inertia"] = arr["inertia"].reshape(-1, 3)
+    dev.set_field(gravity=(0., 0., -9.81), periodic=False)
+    dev.set_materials(mats)
+    dev.upload_particles(**arr)
+    dev.set_scenes(dts)
+    return arr
+
+
+def bed_fraction(pos, scenes, idx, r):
+    """per scene: sphere volume over the box cross-section times the bed height (top sphere)"""
+    out, o = [], 0
+    for i in idx:
+        sc = scenes[i]
+        p = pos[o:o + sc["n"]] - sc["offset"]
+        o += sc["n"] + 6
+        h = float(np.max(p[:, 2])) + r
+        out.append(sc["n"] * (4 / 3.) * math.pi * r ** 3 / (sc["box"] ** 2 * h))
+    return np.array(out)
+
+
+def batched_form(args):
+    """256 scenes in one context, stepped by the graph runner after the autotuner; against the
+    plain GPU engine running a sample of the scenes one at a time (same steps): throughput of
+    both, and the sampled scenes' final states, which must be identical."""
+    from woo_amd._native import Device
+
+    scenes, r, rho = batched_scenes()
+    ntot = sum(sc["n"] for sc in scenes)
+    steps = args.batch_steps
+    kw = dict(contact=dict(physKind=0, lawKind=0, trsfRenorm=100), leapfrog=dict(damping=.2), dynDtPeriod=0)
+    dev = Device(0)
+    batch_upload(dev, scenes, r, rho)
+    t0 = time.perf_counter()
+    tun, step, taken = dev.tune(ENGINES, 0, scenes[0]["dt"], **kw)
+    t_tune = time.perf_counter() - t0
+    st0 = dev.stats()
+    dev.sync()
+    t0 = time.perf_counter()
+    step, _ = dev.run(ENGINES, steps, step, scenes[0]["dt"], graph=True, **kw)
+    dev.sync()
+    el = time.perf_counter() - t0
+    st1 = dev.stats()
+    value = ntot * steps / el
+    total = taken + steps
+    pos = dev.get("pos")
+    dev.close()
+    # serial sample: scenes spread over the sizes, each alone for the same steps
+    order = np.argsort([sc["n"] for sc in scenes])
+    sample = [int(order[k]) for k in np.linspace(0, len(scenes) - 1, args.batch_sample).astype(int)]
+    t_ser, n_ser, same = 0., 0, True
+    offs = np.cumsum([0] + [scenes[i]["n"] + 6 for i in range(len(scenes))])
+    for i in sample:
+        d = Device(0)
+        batch_upload(d, scenes, r, rho, idx=[i])
+        d.set_tuning(**tun)
+        d.sync()
+        t0 = time.perf_counter()
+        d.run(ENGINES, total, 0, scenes[i]["dt"], **kw)
+        d.sync()
+        t_ser += time.perf_counter() - t0
+        n_ser += scenes[i]["n"]
+        same = same and bool(np.array_equal(d.get("pos"), pos[offs[i]:offs[i + 1]]))
+        d.close()
+    serial = n_ser * total / t_ser
+    phi_b = bed_fraction(pos, scenes, range(len(scenes)), r)
+    return dict(value=value, unit="particle-steps/s", scenes=len(scenes), particles=ntot, steps=steps,
+                ms_per_step=el / steps * 1e3, tuning=tun, tuning_steps=taken, tuning_s=t_tune,
+                refreshes=st1["nRefresh"] - st0["nRefresh"], serial_plain_engine=dict(
+                    value=serial, unit="particle-steps/s", scenes=len(sample), particles=n_ser,
+                    note="each sampled scene alone on the plain engine loop (wdg_run), the same steps "
+                         "(tuning steps included)"),
+                speedup_vs_serial=value / serial, sampled_scenes_identical=same,
+                bed_solid_fraction=dict(mean=float(phi_b.mean()), min=float(phi_b.min()), max=float(phi_b.max())))
+
+
 def pmc_probe(dev, wl, args):
     dt, cp = wl["dt"], contact_params()
     step, _ = dev.run(ENGINES, args.warmup, 0, dt, contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
@@ -361,6 +513,9 @@ def main():
     ap.add_argument("--drum", type=int, default=1, help="also time the rotating-drum workload (facets)")
     ap.add_argument("--drum-n", type=int, default=4_000_000)
     ap.add_argument("--drum-steps", type=int, default=500)
+    ap.add_argument("--batch", type=int, default=1, help="also time the batched 256-scene study")
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--batch-sample", type=int, default=8, help="scenes also run one at a time")
     ap.add_argument("--pmc-probe", type=int, default=0,
                     help="for rocprofv3 --pmc runs: warm up, launch the contact loop and leapfrog this many "
                          "times, then a device-to-device hipMemcpy of 512 MiB whose FETCH_SIZE/WRITE_SIZE "
@@ -417,11 +572,14 @@ def main():
                 traffic=traffic, traffic_source=tsrc, kernel=dom, bytes_per_launch=B[dom], launch_us=times[dom] * 1e6,
                 other_kernels={k: dict(bytes_per_launch=B[k], launch_us=times[k] * 1e6,
                                        achieved_GBs=B[k] / times[k] / 1e9) for k in times if k != dom})
-    drum = None
+    drum = batch = None
+    dev.close()
     if args.drum:
-        dev.close()
         drum = drum_form(args)
         log(f"drum form: {drum}")
+    if args.batch:
+        batch = batched_form(args)
+        log(f"batched form: {batch}")
     base = None
     if args.cpu_baseline:
         log("timing the CPU oracle on the same workload ...")
@@ -440,10 +598,9 @@ def main():
         "cpu_baseline": base,
         "fp32_form": fp32,
         "drum_form": drum,
+        "batched_form": batch,
     }
     print(json.dumps(out), flush=True)
-    if not args.drum:
-        dev.close()
 
 
 if __name__ == "__main__":

@@ -103,6 +103,9 @@ typedef struct {
   const double* facetVerts;   /* x9 (vertices 0,1,2), may be NULL if there are no facets */
   const double* fakeVel;      /* x3, Facet.fakeVel, may be NULL (zero) */
   const double* halfThick;    /* x1, Facet.halfThick, may be NULL (zero) */
+  /* Batched independent scenes: the scene of each particle (0..nScenes-1, see
+   * wdg_set_scenes); particles of different scenes never collide. May be NULL (all 0). */
+  const int32_t* scene;
 } wdg_particles;
 
 typedef struct {
@@ -199,6 +202,30 @@ int wdg_run(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t* step
 int wdg_relax(wdg_ctx* ctx, int32_t maxIter, double inflate, double tol, double* maxOverlap, int32_t* iters);
 
 int wdg_sync(wdg_ctx* ctx);
+
+/* ---- batched scenes and graph replay (no counterpart in the reference: many independent
+ * Scene objects stepped together, for parametric studies).
+ * wdg_set_scenes: nScenes independent scenes sharing this context, each with its own
+ * Scene.dt (dt[nScenes]); the contact loop and leapfrog then take dt per particle from its
+ * scene (the dt passed with the engine parameters is ignored). nScenes = 0 switches back.
+ * Only the engine list [Collider, ContactLoop, Leapfrog] is run in this mode (no DynDt). */
+int wdg_set_scenes(wdg_ctx* ctx, int32_t nScenes, const double* dt);
+/* tuning of the graph runner: workgroups per contact group of the force kernel, steps per
+ * captured graph (even) */
+typedef struct {
+  int32_t cfBlocks;
+  int32_t graphSteps;
+} wdg_tuning;
+/* wdg_run for the engine list [Collider, ContactLoop, Leapfrog(reset)] with the contact loop
+ * and leapfrog of graphSteps steps captured once as a hipGraph and replayed; a step whose
+ * collider would rebuild (a particle left its Verlet skin) ends the replay early on the
+ * device and the host runs the rebuild. Same results as wdg_run. */
+int wdg_run_graph(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t* step, double* dt);
+/* autotuner: times each candidate tuning on the live state (real simulation steps, counted
+ * in *step) and keeps the fastest; out receives it, stepsTaken the steps run */
+int wdg_tune(wdg_ctx* ctx, const wdg_program* prog, int64_t* step, double* dt, wdg_tuning* out, int64_t* stepsTaken);
+int wdg_get_tuning(wdg_ctx* ctx, wdg_tuning* out);
+int wdg_set_tuning(wdg_ctx* ctx, const wdg_tuning* t);
 /* HIP stream the engine launches on (hipStream_t), for callers that time it with events */
 void* wdg_stream(wdg_ctx* ctx);
 

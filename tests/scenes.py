@@ -345,3 +345,39 @@ class GpuRunner:
                 else:
                     raise ValueError(e)
             self.step += 1
+
+
+def translated(sc, offset):
+    """the scene moved by offset (positions of spheres and walls)"""
+    out = dict(sc)
+    off = np.asarray(offset, dtype=float)
+    out["pos"] = sc["pos"] + off
+    out["walls"] = [dict(w, pos=np.asarray(w["pos"], dtype=float) + off) for w in sc["walls"]]
+    return out
+
+
+def batch_arrays(scenes):
+    """one upload holding several independent scenes: each scene's particles in its own id
+    order, scene after scene; material k belongs to scene k; returns (arrays, materials, dt)"""
+    parts = [gpu_arrays(sc) for sc in scenes]
+    keys = parts[0].keys()
+    out = {}
+    for k in keys:
+        out[k] = np.concatenate([p[k] for p in parts])
+    out["material"] = np.concatenate([np.full(len(p["shape"]), i, np.int32) for i, p in enumerate(parts)])
+    out["scene"] = np.concatenate([np.full(len(p["shape"]), i, np.int32) for i, p in enumerate(parts)])
+    mats = [sc["materials"][0] for sc in scenes]
+    return out, mats, np.array([sc["dt"] for sc in scenes])
+
+
+def build_batch(scenes, device=0):
+    from woo_amd._native import Device
+
+    d = Device(device)
+    sc0 = scenes[0]
+    d.set_field(gravity=sc0["gravity"], periodic=False)
+    arrays, mats, dts = batch_arrays(scenes)
+    d.set_materials(mats)
+    d.upload_particles(**arrays)
+    d.set_scenes(dts)
+    return d, arrays

@@ -24,7 +24,7 @@ EXPORTS = [
     "wdg_upload_particles", "wdg_upload", "wdg_download", "wdg_collide", "wdg_contact_loop", "wdg_leapfrog",
     "wdg_force_reset", "wdg_crit_dt", "wdg_contact_count", "wdg_contact_list", "wdg_observables", "wdg_stats",
     "wdg_run", "wdg_relax", "wdg_sync", "wdg_stream", "wdg_stress", "wdg_kinetic_energy",
-    "wdg_set_precision",
+    "wdg_set_precision", "wdg_set_scenes", "wdg_run_graph", "wdg_tune", "wdg_get_tuning", "wdg_set_tuning",
 ]
 
 
@@ -48,7 +48,11 @@ class Particles(C.Structure):
                 ("pos", _P(C.c_double)), ("vel", _P(C.c_double)), ("angVel", _P(C.c_double)),
                 ("ori", _P(C.c_double)), ("force", _P(C.c_double)), ("torque", _P(C.c_double)),
                 ("mass", _P(C.c_double)), ("inertia", _P(C.c_double)), ("facetVerts", _P(C.c_double)),
-                ("fakeVel", _P(C.c_double)), ("halfThick", _P(C.c_double))]
+                ("fakeVel", _P(C.c_double)), ("halfThick", _P(C.c_double)), ("scene", _P(C.c_int32))]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("cfBlocks", C.c_int32), ("graphSteps", C.c_int32)]
 
 
 class CollideParams(C.Structure):
@@ -95,6 +99,11 @@ def lib():
     L.wdg_set_field.argtypes = [V, _P(Field)]
     L.wdg_set_materials.argtypes = [V, _P(Material), C.c_int32]
     L.wdg_upload_particles.argtypes = [V, _P(Particles)]
+    L.wdg_set_scenes.argtypes = [V, C.c_int32, _P(C.c_double)]
+    L.wdg_run_graph.argtypes = [V, _P(Program), C.c_int64, _P(C.c_int64), _P(C.c_double)]
+    L.wdg_tune.argtypes = [V, _P(Program), _P(C.c_int64), _P(C.c_double), _P(Tuning), _P(C.c_int64)]
+    L.wdg_get_tuning.argtypes = [V, _P(Tuning)]
+    L.wdg_set_tuning.argtypes = [V, _P(Tuning)]
     L.wdg_upload.argtypes = [V, C.c_int32, _P(C.c_double)]
     L.wdg_download.argtypes = [V, C.c_int32, _P(C.c_double)]
     L.wdg_collide.argtypes = [V, _P(CollideParams)]
@@ -172,7 +181,7 @@ class Device:
 
     def upload_particles(self, shape, pos, mass, inertia, radius=None, wallAxis=None, wallSense=None, material=None,
                          mask=None, flags=None, integrated=None, vel=None, angVel=None, ori=None, force=None,
-                         torque=None, facetVerts=None, fakeVel=None, halfThick=None):
+                         torque=None, facetVerts=None, fakeVel=None, halfThick=None, scene=None):
         n = len(shape)
         keep = []
 
@@ -206,6 +215,7 @@ class Device:
         p.facetVerts = dptr(arr(facetVerts, np.float64, 9))
         p.fakeVel = dptr(arr(fakeVel, np.float64, 3))
         p.halfThick = dptr(arr(halfThick, np.float64))
+        p.scene = iptr(arr(scene, np.int32))
         check(self.L.wdg_upload_particles(self.h, C.byref(p)))
         self.n = n
 
@@ -280,8 +290,8 @@ class Device:
         check(self.L.wdg_stats(self.h, C.byref(s)))
         return {k: (list(getattr(s, k)) if k == "gridCells" else getattr(s, k)) for k, _ in Counters._fields_}
 
-    def run(self, engines, nSteps, step, dt, collide=None, contact=None, leapfrog=None, dynDtPeriod=100,
-            maxRelInc=1e-4, dtSafety=.9):
+    def program(self, engines, dt, collide=None, contact=None, leapfrog=None, dynDtPeriod=100, maxRelInc=1e-4,
+                dtSafety=.9):
         prog = Program()
         prog.nEngines = len(engines)
         for i, e in enumerate(engines):
@@ -297,10 +307,40 @@ class Device:
         prog.dynDtPeriod = dynDtPeriod
         prog.maxRelInc = maxRelInc
         prog.dtSafety = dtSafety
+        return prog
+
+    def run(self, engines, nSteps, step, dt, graph=False, **kw):
+        """wdg_run (or wdg_run_graph with graph=True); kw: collide, contact, leapfrog, dynDtPeriod, ..."""
+        prog = self.program(engines, dt, **kw)
         st = C.c_int64(step)
         d = C.c_double(dt)
-        check(self.L.wdg_run(self.h, C.byref(prog), nSteps, C.byref(st), C.byref(d)))
+        fn = self.L.wdg_run_graph if graph else self.L.wdg_run
+        check(fn(self.h, C.byref(prog), nSteps, C.byref(st), C.byref(d)))
         return st.value, d.value
+
+    def set_scenes(self, dt):
+        """independent scenes, dt of each (empty: back to one scene)"""
+        a = np.ascontiguousarray(dt, dtype=np.float64)
+        check(self.L.wdg_set_scenes(self.h, len(a), dptr(a) if len(a) else None))
+
+    def tune(self, engines, step, dt, **kw):
+        """autotuner; returns (tuning dict, new step, steps taken)"""
+        prog = self.program(engines, dt, **kw)
+        st = C.c_int64(step)
+        d = C.c_double(dt)
+        t = Tuning()
+        n = C.c_int64()
+        check(self.L.wdg_tune(self.h, C.byref(prog), C.byref(st), C.byref(d), C.byref(t), C.byref(n)))
+        return dict(cfBlocks=t.cfBlocks, graphSteps=t.graphSteps), st.value, n.value
+
+    def tuning(self):
+        t = Tuning()
+        check(self.L.wdg_get_tuning(self.h, C.byref(t)))
+        return dict(cfBlocks=t.cfBlocks, graphSteps=t.graphSteps)
+
+    def set_tuning(self, cfBlocks, graphSteps):
+        t = Tuning(cfBlocks, graphSteps)
+        check(self.L.wdg_set_tuning(self.h, C.byref(t)))
 
     def relax(self, maxIter=2000, inflate=0.01, tol=0.):
         ov = C.c_double()

@@ -287,7 +287,10 @@ int wdg_ctx_destroy(wdg_ctx* ctx) {
     c.perm.release(); c.invPerm.release(); c.cellStart.release(); c.cellEnd.release();
     c.scount.release(); c.soff.release(); c.stPartner.release(); c.stInfo.release(); c.stDist.release();
     c.adjEnt.release(); c.adjDeg.release(); c.stNb.release(); c.sdeg.release(); c.posf.release();
-    c.fgeo.release(); c.bnd.release(); c.pairMax.release(); c.bigOff.release(); c.bigRank.release(); c.bigVal.release();
+    c.fgeo.release(); c.bnd.release(); c.pairMax.release(); c.scene.release(); c.dtScene.release();
+    c.gstate.release();
+    if (c.graph.exec) (void)hipGraphExecDestroy(c.graph.exec);
+    c.graph.exec = nullptr; c.bigOff.release(); c.bigRank.release(); c.bigVal.release();
     c.bigValTmp.release(); c.bigDest.release(); c.bigKey.release(); c.bigKeyTmp.release(); c.bigList.release();
     c.sortHist.release(); c.scanAux.release(); c.red.release(); c.xfer.release(); c.dflags.release();
     if (c.hflags) (void)hipHostFree(c.hflags);
@@ -345,7 +348,8 @@ int wdg_upload_particles(wdg_ctx* ctx, const wdg_particles* p) {
           throw Error(WDG_E_ARG, "unsupported shape " + std::to_string(shape));
         if ((pass == 0) == (shape != WDG_SPHERE)) order.push_back((uint32_t)i);
       }
-    std::vector<uint32_t> flags((size_t)n), mask((size_t)n), uid((size_t)n), sto((size_t)n);
+    std::vector<uint32_t> flags((size_t)n), mask((size_t)n), uid((size_t)n), sto((size_t)n), scn((size_t)n);
+    int64_t maxScene = 0;
     std::vector<int32_t> mat((size_t)n);
     std::vector<double> col[NPD];
     for (auto& v : col) v.resize((size_t)n);
@@ -391,6 +395,10 @@ int wdg_upload_particles(wdg_ctx* ctx, const wdg_particles* p) {
         c.rmax = std::max(c.rmax, r);
       }
       flags[s] = f;
+      const int32_t sc = p->scene ? p->scene[i] : 0;
+      if (sc < 0) throw Error(WDG_E_ARG, "negative scene index");
+      scn[s] = (uint32_t)sc;
+      maxScene = std::max<int64_t>(maxScene, sc);
       mask[s] = p->mask ? p->mask[i] : (shape != WDG_SPHERE ? 3u : 5u);
       mat[s] = m;
       uid[s] = (uint32_t)i;
@@ -430,6 +438,10 @@ int wdg_upload_particles(wdg_ctx* ctx, const wdg_particles* p) {
       if (n) WDG_HIP(hipMemcpy(c.lastPos[k].p, col[k].data(), n * sizeof(double), hipMemcpyHostToDevice));
     }
     c.flags.ensure(n + 1); c.mask.ensure(n + 1); c.mat.ensure(n + 1); c.uid.ensure(n + 1); c.sto.ensure(n + 1);
+    c.scene.ensure(n + 1);
+    c.maxScene = maxScene;
+    c.gen++;
+    if (n) WDG_HIP(hipMemcpy(c.scene.p, scn.data(), n * 4, hipMemcpyHostToDevice));
     if (n) {
       WDG_HIP(hipMemcpy(c.flags.p, flags.data(), n * 4, hipMemcpyHostToDevice));
       WDG_HIP(hipMemcpy(c.mask.p, mask.data(), n * 4, hipMemcpyHostToDevice));
@@ -533,7 +545,52 @@ int wdg_set_precision(wdg_ctx* ctx, int32_t bits) {
     if (c.forcesPending) flush_forces(c);
     drop_contacts(c);
     c.precision = bits;
+    c.gen++;
   });
+}
+
+int wdg_set_scenes(wdg_ctx* ctx, int32_t nScenes, const double* dt) {
+  if (!ctx || nScenes < 0 || (nScenes > 0 && !dt)) { g_err = "bad arguments"; return WDG_E_ARG; }
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    for (int32_t k = 0; k < nScenes; k++)
+      if (!(dt[k] > 0)) throw Error(WDG_E_ARG, "scene " + std::to_string(k) + ": dt must be positive");
+    if (c.forcesPending) flush_forces(c);
+    c.nScenes = nScenes;
+    c.dtScene.ensure((size_t)std::max(nScenes, 1));
+    if (nScenes) WDG_HIP(hipMemcpy(c.dtScene.p, dt, nScenes * sizeof(double), hipMemcpyHostToDevice));
+    c.gen++;
+  });
+}
+
+int wdg_run_graph(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t* step, double* dt) {
+  if (!ctx || !prog || !step || !dt || nSteps < 0) { g_err = "bad arguments"; return WDG_E_ARG; }
+  return guard(ctx, [&] { run_graph(ctx->c, prog, nSteps, step, dt); });
+}
+
+int wdg_tune(wdg_ctx* ctx, const wdg_program* prog, int64_t* step, double* dt, wdg_tuning* out, int64_t* stepsTaken) {
+  if (!ctx || !prog || !step || !dt) { g_err = "bad arguments"; return WDG_E_ARG; }
+  return guard(ctx, [&] {
+    const int64_t n = tune(ctx->c, prog, step, dt);
+    if (out) *out = ctx->c.tune;
+    if (stepsTaken) *stepsTaken = n;
+  });
+}
+
+int wdg_get_tuning(wdg_ctx* ctx, wdg_tuning* out) {
+  if (!ctx || !out) { g_err = "NULL argument"; return WDG_E_ARG; }
+  *out = ctx->c.tune;
+  return WDG_OK;
+}
+
+int wdg_set_tuning(wdg_ctx* ctx, const wdg_tuning* t) {
+  if (!ctx || !t || !(t->cfBlocks == 1 || t->cfBlocks == 2 || t->cfBlocks == 4 || t->cfBlocks == 8) ||
+      t->graphSteps < 2 || (t->graphSteps & 1) || t->graphSteps > 256) {
+    g_err = "tuning: cfBlocks in {1,2,4,8}, graphSteps even in [2,256]";
+    return WDG_E_ARG;
+  }
+  ctx->c.tune = *t;
+  return WDG_OK;
 }
 
 int wdg_stress(wdg_ctx* ctx, double volume, double out[9]) {

@@ -39,6 +39,13 @@ struct ContactArgs {
   const int32_t* mat;
   const wdg_material* mats;
   const double* fgeo;  // facet geometry per boundary body
+  // batched scenes: dt of each particle's scene (else null, dt above)
+  const double* dtScene;
+  const uint32_t* scene;
+  // graph replay: a set stop word skips the kernel; the step's renorm flag from the device
+  const uint32_t* gstop;
+  const uint32_t* renormPtr;
+  int cfBlocks;  // workgroups per group of slots in the force kernel
   // parameters
   double L[3];
   double absDf;
@@ -190,6 +197,7 @@ __device__ __forceinline__ bool filter_slot(const ContactArgs& a, int64_t c) {
 
 template <bool PERIODIC, int LAW>
 __global__ __launch_bounds__(GROUP_BS) void k_cfilter(ContactArgs a) {
+  if (a.gstop && *a.gstop) return;
   const uint32_t g = xcd_block(blockIdx.x, gridDim.x);
   const int64_t base = (int64_t)g * GROUP;
   // all slots of the thread first (their loads overlap), then the bookkeeping passes
@@ -204,18 +212,20 @@ __global__ __launch_bounds__(GROUP_BS) void k_cfilter(ContactArgs a) {
   if (threadIdx.x == 0) a.ngcnt[g] = running;
 }
 
-// CF_BLOCKS workgroups per group of slots
+// a.cfBlocks workgroups of CF_BS per group of slots
 constexpr int CF_BS = 256;
-constexpr int CF_BLOCKS = 4;
 
 // FACETS: the scene has facets (a separate instantiation keeps the facet geometry's
 // registers out of the other scenes' kernel)
 template <bool PERIODIC, int LAW, bool FACETS>
 __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
+  if (a.gstop && *a.gstop) return;
+  const int renorm = a.renormPtr ? (int)*a.renormPtr : a.renorm;
   const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t g = lb / CF_BLOCKS;
+  const uint32_t nblk = (uint32_t)a.cfBlocks;
+  const int64_t g = lb / nblk;
   const uint32_t cnt = a.ngcnt[g];
-  for (uint32_t loc = (lb % CF_BLOCKS) * CF_BS + threadIdx.x; loc < cnt; loc += CF_BS * CF_BLOCKS) {
+  for (uint32_t loc = (lb % nblk) * CF_BS + threadIdx.x; loc < cnt; loc += CF_BS * nblk) {
     const int64_t k = state_index((uint32_t)g, loc);
     const uint32_t c = a.act[k];
     const uint32_t info = a.info[c];
@@ -224,6 +234,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
     const bool swap = (info & CI_SWAP) != 0u;
     const uint32_t pA = swap ? a.B[c] : a.A[c], pB = swap ? a.A[c] : a.B[c];
     const int type = (int)(info & CI_TYPE_MASK);
+    const double dt = a.dtScene ? a.dtScene[a.scene[pA]] : a.dt;
     uint64_t cd;
     const d3 shift2 = slot_shift<PERIODIC>(a, c, cd);
     const d3 posA = mk3(a.px[pA], a.py[pA], a.pz[pA]);
@@ -307,7 +318,6 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
       T2 = cross3(locX, locY);
       gvel = mk3(0., 0., 0.);
     } else {
-      const double dt = a.dt;
       const d3 prevY = mk3(a.oY0[ko], a.oY1[ko], a.oY2[ko]);
       const double* og = a.ogp + (int64_t)GP * ko;
       const d3 prevContPt = mk3(og[3], og[4], og[5]);
@@ -325,7 +335,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
       T0 = currNormal;
       T1 = sub3(prevY, cross3(M1, rt));
       T2 = cross3(T0, T1);
-      if (a.renorm) {
+      if (renorm) {
         T0 = normalized3(T0);
         T1 = sub3(T1, mul3(T0, dot3(T1, T0)));
         T1 = normalized3(T1);
@@ -354,7 +364,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
         Ft0 = 0.;
         Ft1 = 0.;
       } else {
-        const double dk = a.dt * ph.kt;
+        const double dk = dt * ph.kt;
         Ft0 = Ft0 + dk * gvel.y;
         Ft1 = Ft1 + dk * gvel.z;
         const double maxFt = fabs(Fn) * ph.tanPhi;
@@ -381,7 +391,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
       if (ph.alpha == 0. && a.noAttraction && Fne + Fnv > 0) Fnv = -Fne;
       Fn = Fne + Fna + Fnv;
       const double kt = ph.kt0 * sqrt(uN < 0 ? -uN : 0);
-      const double dk = a.dt * kt;
+      const double dk = dt * kt;
       Ft0 = Ft0 + dk * gvel.y;
       Ft1 = Ft1 + dk * gvel.z;
       const double maxFt = fabs(fmin(0., Fn) * ph.tanPhi);
@@ -482,6 +492,7 @@ __device__ __forceinline__ bool filter_slot32(const ContactArgs& a, int64_t c) {
 
 template <bool PERIODIC, int LAW>
 __global__ __launch_bounds__(GROUP_BS) void k_cfilter32(ContactArgs a) {
+  if (a.gstop && *a.gstop) return;
   const uint32_t g = xcd_block(blockIdx.x, gridDim.x);
   const int64_t base = (int64_t)g * GROUP;
   constexpr int NP = GROUP / GROUP_BS;
@@ -506,11 +517,13 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
         *N2 = reinterpret_cast<float*>(a.N2), *Y0 = reinterpret_cast<float*>(a.Y0), *Y1 = reinterpret_cast<float*>(a.Y1),
         *Y2 = reinterpret_cast<float*>(a.Y2), *F0 = reinterpret_cast<float*>(a.F0), *F1 = reinterpret_cast<float*>(a.F1),
         *F2 = reinterpret_cast<float*>(a.F2), *UN = reinterpret_cast<float*>(a.uN), *gp = reinterpret_cast<float*>(a.gp);
-  const float dt = (float)a.dt;
+  if (a.gstop && *a.gstop) return;
+  const int renorm = a.renormPtr ? (int)*a.renormPtr : a.renorm;
   const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t g = lb / CF_BLOCKS;
+  const uint32_t nblk = (uint32_t)a.cfBlocks;
+  const int64_t g = lb / nblk;
   const uint32_t cnt = a.ngcnt[g];
-  for (uint32_t loc = (lb % CF_BLOCKS) * CF_BS + threadIdx.x; loc < cnt; loc += CF_BS * CF_BLOCKS) {
+  for (uint32_t loc = (lb % nblk) * CF_BS + threadIdx.x; loc < cnt; loc += CF_BS * nblk) {
     const int64_t k = state_index((uint32_t)g, loc);
     const uint32_t c = a.act[k];
     const uint32_t info = a.info[c];
@@ -518,6 +531,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
     const uint32_t ko = real ? rv_index(a.ov, c) : 0u;
     const bool swap = (info & CI_SWAP) != 0u;
     const uint32_t pA = swap ? a.B[c] : a.A[c], pB = swap ? a.A[c] : a.B[c];
+    const float dt = (float)(a.dtScene ? a.dtScene[a.scene[pA]] : a.dt);
     uint64_t cd;
     const d3 shift2 = slot_shift<PERIODIC>(a, c, cd);
     const f3 r = tof3(sub3(add3(mk3(a.px[pB], a.py[pB], a.pz[pB]), shift2), mk3(a.px[pA], a.py[pA], a.pz[pA])));
@@ -604,7 +618,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
       T0 = normal;
       T1 = sub3(prevY, cross3(M1, rt));
       T2 = cross3(T0, T1);
-      if (a.renorm) {
+      if (renorm) {
         T0 = normalized3(T0);
         T1 = sub3(T1, mul3(T0, dot3(T1, T0)));
         T1 = normalized3(T1);
@@ -710,6 +724,13 @@ static ContactArgs contact_args(Ctx& c, int from, int to) {
   a.wx = c.angVel[0].p; a.wy = c.angVel[1].p; a.wz = c.angVel[2].p;
   a.rad = c.radius.p; a.mass = c.mass.p; a.flags = c.flags.p; a.mat = c.mat.p; a.mats = c.dMats.p;
   a.fgeo = c.fgeo.p;
+  a.dtScene = c.nScenes > 0 ? c.dtScene.p : nullptr;
+  a.scene = c.scene.p;
+  a.cfBlocks = c.tune.cfBlocks;
+  if (c.graphCapture) {
+    a.gstop = c.gstate.p;
+    a.renormPtr = c.gstate.p + 2;
+  }
   for (int k = 0; k < 3; k++) a.L[k] = c.field.cell[k];
   a.absDf = std::fabs(c.field.distFactor);
   a.dflags = c.dflags.p;
@@ -756,7 +777,7 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
   const bool hertz = p->lawKind == WDG_LAW_HERTZDMT;
   const dim3 gfil((unsigned)ng), bfil(GROUP_BS);
   if (c.precision == 32) {
-    const dim3 gforce((unsigned)(ng * CF_BLOCKS)), bforce(CF_BS);
+    const dim3 gforce((unsigned)(ng * a.cfBlocks)), bforce(CF_BS);
     if (hertz) {
       if (per) hipLaunchKernelGGL((k_cfilter32<true, WDG_LAW_HERTZDMT>), gfil, bfil, 0, c.stream, a);
       else hipLaunchKernelGGL((k_cfilter32<false, WDG_LAW_HERTZDMT>), gfil, bfil, 0, c.stream, a);
@@ -787,7 +808,7 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
     else hipLaunchKernelGGL((k_cfilter<false, WDG_LAW_IDEALELPL>), gfil, bfil, 0, c.stream, a);
   }
   WDG_LAUNCH_CHECK();
-  const dim3 gforce((unsigned)(ng * CF_BLOCKS)), bforce(CF_BS);
+  const dim3 gforce((unsigned)(ng * a.cfBlocks)), bforce(CF_BS);
   if (hertz) {
     if (per) hipLaunchKernelGGL((k_cforce<true, WDG_LAW_HERTZDMT, false>), gforce, bforce, 0, c.stream, a);
     else if (fac) hipLaunchKernelGGL((k_cforce<false, WDG_LAW_HERTZDMT, true>), gforce, bforce, 0, c.stream, a);

@@ -1,0 +1,207 @@
+// Graph replay of the step loop and its autotuner.
+//
+// The engine list [Collider, ContactLoop, Leapfrog(reset)] runs as: the collider on the
+// host (its dirty test reads one flag word, a rebuild sizes buffers), then K steps of contact
+// loop and leapfrog captured once as a hipGraph and replayed. Before step j > 0 of a replay a
+// one-thread kernel reads the dirty word the leapfrog of step j-1 set: when a particle left
+// its Verlet skin it sets the stop word, and every captured kernel after it returns at once,
+// so the replay ends after step j-1 and the host rebuilds before the next step, exactly as
+// Scene::doOneStep would. The same kernel counts the executed steps and sets the step's
+// trsfRenorm flag from the device-resident step number. Host bookkeeping (which state
+// buffer is current, the parity of the dirty words) advances by the executed count.
+//
+// A graph is valid while the slot list, the buffers and the parameters it captured stay the
+// same: it is captured again after a rebuild or any change of state layout (Ctx::gen), of
+// the tuning, or of the parity of the state buffers. K is even so that a replay ends on the
+// buffer parity it started with.
+//
+// The autotuner times candidate tunings (workgroups per contact group in the force kernel,
+// steps per graph) on the live state: each trial is a run of real steps, timed by the host
+// clock around whole replays, the best kept.
+#include <chrono>
+
+#include "wdg_internal.hpp"
+
+namespace wdg {
+
+// gstate words: [0] stop, [1] executed, [2] renorm, [3..4] first step (int64), [5] trsfRenorm,
+// [6] dirty word index of the previous step (set per step at capture)
+__global__ void k_gate(uint32_t* gs, const uint32_t* dflags, int j, int dirtyWord) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (gs[0]) return;
+  if (j > 0 && dflags[dirtyWord]) {
+    gs[0] = 1u;
+    return;
+  }
+  gs[1] = (uint32_t)(j + 1);
+  const int64_t step = (int64_t)(((uint64_t)gs[4] << 32) | gs[3]) + j;
+  const int64_t tr = (int64_t)gs[5];
+  gs[2] = (tr > 0 && (step % tr) == 0) ? 1u : 0u;
+}
+
+static void check_program(const wdg_program* prog) {
+  if (prog->nEngines != 3 || prog->engines[0] != WDG_E_COLLIDER || prog->engines[1] != WDG_E_CONTACTLOOP ||
+      prog->engines[2] != WDG_E_LEAPFROG)
+    throw Error(WDG_E_ARG, "graph replay runs the engine list [Collider, ContactLoop, Leapfrog]");
+  if (!prog->leapfrog.reset) throw Error(WDG_E_ARG, "graph replay needs Leapfrog.reset");
+}
+
+static void plain_step(Ctx& c, const wdg_program* prog, int64_t step, double dt) {
+  wdg_contact_params cp = prog->contact;
+  cp.step = step;
+  cp.dt = dt;
+  contact_loop(c, &cp);
+  wdg_leapfrog_params lp = prog->leapfrog;
+  lp.dt = dt;
+  leapfrog(c, &lp);
+}
+
+// host state a captured (and so not executed) sequence of launches changes
+struct HostState {
+  int rcur, lfParity, dirtyIdx;
+  bool forcesPending, forceClean, dirtyKnown;
+};
+static HostState save(const Ctx& c) {
+  return HostState{c.rcur, c.lfParity, c.dirtyIdx, c.forcesPending, c.forceClean, c.dirtyKnown};
+}
+static void restore(Ctx& c, const HostState& h) {
+  c.rcur = h.rcur; c.lfParity = h.lfParity; c.dirtyIdx = h.dirtyIdx;
+  c.forcesPending = h.forcesPending; c.forceClean = h.forceClean; c.dirtyKnown = h.dirtyKnown;
+}
+
+static void capture(Ctx& c, const wdg_program* prog, int K, int64_t step, double dt) {
+  auto& G = c.graph;
+  if (G.exec) {
+    WDG_HIP(hipGraphExecDestroy(G.exec));
+    G.exec = nullptr;
+  }
+  // every buffer a captured launch touches is sized now (no allocation while capturing)
+  const int64_t nc = c.con().n;
+  c.R[0].ensure(nc);
+  c.R[1].ensure(nc);
+  c.act.ensure(ngroups(nc) * GROUP);
+  const HostState h = save(c);
+  hipGraph_t graph = nullptr;
+  c.graphCapture = true;
+  WDG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  try {
+    for (int j = 0; j < K; j++) {
+      // the dirty word the previous step's leapfrog sets
+      const int prevWord = DIRTY_PP + (1 - c.lfParity);
+      hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, c.stream, c.gstate.p, (const uint32_t*)c.dflags.p, j, prevWord);
+      WDG_LAUNCH_CHECK();
+      plain_step(c, prog, step + j, dt);
+    }
+  } catch (...) {
+    c.graphCapture = false;
+    (void)hipStreamEndCapture(c.stream, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    restore(c, h);
+    throw;
+  }
+  c.graphCapture = false;
+  WDG_HIP(hipStreamEndCapture(c.stream, &graph));
+  WDG_HIP(hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0));
+  WDG_HIP(hipGraphDestroy(graph));
+  restore(c, h);
+  G.gen = c.gen;
+  G.K = K;
+  G.cfBlocks = c.tune.cfBlocks;
+  G.rcur = c.rcur;
+  G.parity = c.lfParity;
+  G.dt = dt;
+  G.damping = prog->leapfrog.damping;
+  G.cp = prog->contact;
+}
+
+static bool graph_valid(const Ctx& c, const wdg_program* prog, int K, double dt) {
+  const auto& G = c.graph;
+  const wdg_contact_params& a = G.cp;
+  const wdg_contact_params& b = prog->contact;
+  return G.exec && G.gen == c.gen && G.K == K && G.cfBlocks == c.tune.cfBlocks && G.rcur == c.rcur &&
+         G.parity == c.lfParity && G.dt == dt && G.damping == prog->leapfrog.damping && a.physKind == b.physKind &&
+         a.lawKind == b.lawKind && a.applyForces == b.applyForces && a.trsfRenorm == b.trsfRenorm &&
+         a.poisson == b.poisson && (a.en == b.en || (a.en != a.en && b.en != b.en)) && a.noSlip == b.noSlip &&
+         a.noBreak == b.noBreak && a.noFrict == b.noFrict && a.noAttraction == b.noAttraction;
+}
+
+// one replay of up to K steps; returns the steps executed
+static int replay(Ctx& c, const wdg_program* prog, int K, int64_t step, double dt) {
+  if (!graph_valid(c, prog, K, dt)) capture(c, prog, K, step, dt);
+  uint32_t w[6] = {0u, 0u, 0u, (uint32_t)((uint64_t)step & 0xFFFFFFFFu), (uint32_t)((uint64_t)step >> 32),
+                   (uint32_t)std::max(prog->contact.trsfRenorm, 0)};
+  WDG_HIP(hipMemcpyAsync(c.gstate.p, w, sizeof(w), hipMemcpyHostToDevice, c.stream));
+  WDG_HIP(hipGraphLaunch(c.graph.exec, c.stream));
+  WDG_HIP(hipMemcpyAsync(c.hflags + 32, c.gstate.p + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+  WDG_HIP(hipStreamSynchronize(c.stream));
+  const int e = (int)c.hflags[32];
+  // host state after e steps of contact loop + leapfrog
+  if (e > 0) {
+    if (c.con().n > 0) c.rcur = (e & 1) ? 1 - c.rcur : c.rcur;
+    const int lastParity = (c.lfParity + e - 1) & 1;
+    c.lfParity = (c.lfParity + e) & 1;
+    c.dirtyIdx = DIRTY_PP + lastParity;
+    c.dirtyKnown = true;
+    c.forcesPending = false;
+    c.forceClean = true;
+  }
+  return e;
+}
+
+void run_graph(Ctx& c, const wdg_program* prog, int64_t nSteps, int64_t* step, double* dt) {
+  check_program(prog);
+  c.gstate.ensure(16);
+  int64_t done = 0;
+  const int K = c.tune.graphSteps;
+  while (done < nSteps) {
+    collide(c, &prog->collide);
+    // a replay starts from a state its capture saw: forces at their reset value, the
+    // contact loop's forces gathered; otherwise one plain step
+    const int64_t left = nSteps - done;
+    if (!c.forceClean || c.forcesPending || left < K || c.n == 0) {
+      plain_step(c, prog, *step, *dt);
+      *step += 1;
+      done += 1;
+      continue;
+    }
+    const int e = replay(c, prog, K, *step, *dt);
+    if (e == 0) throw Error(WDG_E_STATE, "graph replay executed no step");
+    *step += e;
+    done += e;
+  }
+}
+
+// ------------------------------------------------------------------ autotuner
+int64_t tune(Ctx& c, const wdg_program* prog, int64_t* step, double* dt) {
+  check_program(prog);
+  const int cfCand[] = {2, 4, 8};
+  const int kCand[] = {4, 8, 16};
+  wdg_tuning best = c.tune;
+  double bestT = INFINITY;
+  int64_t taken = 0;
+  // warm-up: one plain step so that forces are at their reset value
+  run_graph(c, prog, 2, step, dt);
+  taken += 2;
+  for (int cf : cfCand)
+    for (int K : kCand) {
+      c.tune = wdg_tuning{cf, K};
+      // the first replay captures; the timed ones replay
+      run_graph(c, prog, K, step, dt);
+      taken += K;
+      const int reps = 3;
+      WDG_HIP(hipStreamSynchronize(c.stream));
+      const auto t0 = std::chrono::steady_clock::now();
+      run_graph(c, prog, (int64_t)reps * K, step, dt);
+      WDG_HIP(hipStreamSynchronize(c.stream));
+      const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / (reps * K);
+      taken += (int64_t)reps * K;
+      if (t < bestT) {
+        bestT = t;
+        best = c.tune;
+      }
+    }
+  c.tune = best;
+  return taken;
+}
+
+}  // namespace wdg

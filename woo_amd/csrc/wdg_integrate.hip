@@ -19,6 +19,7 @@ struct GatherArgs {
   int64_t n, nW;
   const uint32_t *adj, *adjOff;
   const uint32_t* flags;
+  const uint32_t* gstop;  // graph replay: a set stop word skips the kernel
   RealView rv;
   const double* gp;
   double L[3];
@@ -76,6 +77,7 @@ __global__ __launch_bounds__(256) void k_gather_bnd(GatherArgs g, double* fx, do
                                                      double* ty, double* tz, const double* __restrict__ px,
                                                      const double* __restrict__ py, const double* __restrict__ pz) {
   __shared__ double sh[6][256];
+  if (g.gstop && *g.gstop) return;
   const int64_t i = blockIdx.x;
   const uint32_t k0 = g.adjOff[i], k1 = g.adjOff[i + 1];
   if (k0 == k1 || flag_shape(g.flags[i]) == WDG_FACET) return;
@@ -128,10 +130,14 @@ struct LeapArgs {
   int clean;  // force/torque of integrated particles hold the reset value
   uint32_t* dirty;      // set when a particle left its Verlet skin
   uint32_t* dirtyNext;  // cleared for the next launch
+  const double* dtScene;  // batched scenes: dt of each particle's scene (else null, dt above)
+  const uint32_t* scene;
+  const uint32_t* gstop;  // graph replay: a set stop word skips the kernel
 };
 
 template <bool PERIODIC, bool F32>
 __global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
+  if (a.gstop && *a.gstop) return;
   const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   bool dirty = false;
   if (i < a.n) {
@@ -158,7 +164,7 @@ __global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
         a.tx[i] = T.x; a.ty[i] = T.y; a.tz[i] = T.z;
       }
     } else {
-      const double dt = a.dt, damping = a.damping;
+      const double dt = a.dtScene ? a.dtScene[a.scene[i]] : a.dt, damping = a.damping;
       const bool damp = (damping != 0. && !(fl & F_DAMPING_SKIP));
       const uint32_t bl = fl & F_DOF_ALL;
       const double m = a.mass[i];
@@ -249,6 +255,7 @@ static GatherArgs gather_args(Ctx& c) {
   g.n = c.n;
   g.adj = C.adj.p; g.adjOff = C.adjOff.p; g.rv = c.rview(); g.gp = c.reals().gp.p; g.flags = c.flags.p;
   g.nW = c.nW;
+  if (c.graphCapture) g.gstop = c.gstate.p;
   for (int k = 0; k < 3; k++) g.L[k] = c.field.cell[k];
   return g;
 }
@@ -296,6 +303,9 @@ void leapfrog(Ctx& c, const wdg_leapfrog_params* p) {
   a.checkDirty = c.boundsValid && c.verletDist > 0 ? 1 : 0;
   a.maxD2 = c.verletDist * c.verletDist;
   a.clean = c.forceClean ? 1 : 0;
+  a.dtScene = c.nScenes > 0 ? c.dtScene.p : nullptr;
+  a.scene = c.scene.p;
+  if (c.graphCapture) a.gstop = c.gstate.p;
   // the flag word of this launch was cleared by the previous launch (or at creation)
   const int s = c.lfParity;
   a.dirty = c.dflags.p + DIRTY_PP + s;
@@ -408,6 +418,7 @@ __global__ __launch_bounds__(256) void k_dyndt(DynArgs a) {
 }
 
 double crit_dt(Ctx& c) {
+  if (c.nScenes > 0) throw Error(WDG_E_STATE, "DynDt is not available for batched scenes");
   if (c.n == 0) return INFINITY;
   Contacts& C = c.con();
   if (C.n == 0) return INFINITY;

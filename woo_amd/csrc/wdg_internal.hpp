@@ -188,7 +188,7 @@ __device__ __forceinline__ uint32_t rv_index(const RealView& v, uint32_t s) {
 }
 
 constexpr int NPD = 24;  // permuted double arrays per particle
-constexpr int NPU = 4;   // permuted u32 arrays per particle
+constexpr int NPU = 5;   // permuted u32 arrays per particle
 
 struct Ctx {
   int device = 0;
@@ -207,6 +207,23 @@ struct Ctx {
   DBuf<double> pos[3], vel[3], angVel[3], ori[4], force[3], torque[3], inertia[3], lastPos[3];
   DBuf<double> mass, radius;
   DBuf<uint32_t> flags, mask, uid, sto;
+  DBuf<uint32_t> scene;  // scene of each particle (batched independent scenes)
+  int32_t nScenes = 0;   // 0: one scene; else dt per scene in dtScene
+  int64_t maxScene = 0;  // largest scene index uploaded
+  DBuf<double> dtScene;
+  // graph runner: device words [0] stop, [1] steps executed, [2] renorm flag of the step,
+  // [3..4] first step number (int64), [5] trsfRenorm
+  DBuf<uint32_t> gstate;
+  wdg_tuning tune{4, 8};
+  int64_t gen = 0;  // bumped whenever buffers a captured graph uses may change
+  bool graphCapture = false;  // kernels being launched are captured for replay
+  struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    int64_t gen = -1;
+    int K = 0, cfBlocks = 0, rcur = -1, parity = -1;
+    double dt = 0, damping = 0;
+    wdg_contact_params cp{};
+  } graph;
   DBuf<int32_t> mat;
   DBuf<double> spareD[NPD];
   DBuf<uint32_t> spareU[NPU];
@@ -284,6 +301,8 @@ void contact_loop(Ctx& c, const wdg_contact_params* p);
 void leapfrog(Ctx& c, const wdg_leapfrog_params* p);
 void force_reset(Ctx& c);
 void flush_forces(Ctx& c);
+void run_graph(Ctx& c, const wdg_program* prog, int64_t nSteps, int64_t* step, double* dt);
+int64_t tune(Ctx& c, const wdg_program* prog, int64_t* step, double* dt);
 double crit_dt(Ctx& c);
 void observables(Ctx& c, const double lo[3], const double hi[3], double out[3]);
 int64_t count_real(Ctx& c);
