@@ -427,12 +427,15 @@ def batched_form(args):
     dev = Device(0)
     batch_upload(dev, scenes, r, rho)
     t0 = time.perf_counter()
-    tun, step, taken = dev.tune(ENGINES, 0, scenes[0]["dt"], **kw)
+    if args.batch_graph:
+        tun, step, taken = dev.tune(ENGINES, 0, scenes[0]["dt"], **kw)
+    else:
+        tun, step, taken = dev.tuning(), 0, 0
     t_tune = time.perf_counter() - t0
     st0 = dev.stats()
     dev.sync()
     t0 = time.perf_counter()
-    step, _ = dev.run(ENGINES, steps, step, scenes[0]["dt"], graph=True, **kw)
+    step, _ = dev.run(ENGINES, steps, step, scenes[0]["dt"], graph=bool(args.batch_graph), **kw)
     dev.sync()
     el = time.perf_counter() - t0
     st1 = dev.stats()
@@ -516,6 +519,8 @@ def main():
     ap.add_argument("--batch", type=int, default=1, help="also time the batched 256-scene study")
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--batch-sample", type=int, default=8, help="scenes also run one at a time")
+    ap.add_argument("--batch-graph", type=int, default=1,
+                    help="0: step the batch with the plain engine loop (no tuning, no graph), e.g. under a tracer")
     ap.add_argument("--pmc-probe", type=int, default=0,
                     help="for rocprofv3 --pmc runs: warm up, launch the contact loop and leapfrog this many "
                          "times, then a device-to-device hipMemcpy of 512 MiB whose FETCH_SIZE/WRITE_SIZE "

@@ -288,6 +288,7 @@ int wdg_ctx_destroy(wdg_ctx* ctx) {
     c.scount.release(); c.soff.release(); c.stPartner.release(); c.stInfo.release(); c.stDist.release();
     c.adjEnt.release(); c.adjDeg.release(); c.stNb.release(); c.sdeg.release(); c.posf.release();
     c.fgeo.release(); c.bnd.release(); c.pairMax.release(); c.scene.release(); c.dtScene.release();
+    c.sceneBodyOff.release(); c.sceneBody.release();
     c.gstate.release();
     if (c.graph.exec) (void)hipGraphExecDestroy(c.graph.exec);
     c.graph.exec = nullptr; c.bigOff.release(); c.bigRank.release(); c.bigVal.release();
@@ -327,6 +328,8 @@ int wdg_set_materials(wdg_ctx* ctx, const wdg_material* mats, int32_t n) {
     if (n) WDG_HIP(hipMemcpy(c.dMats.p, mats, n * sizeof(wdg_material), hipMemcpyHostToDevice));
   });
 }
+
+static void scene_bodies(Ctx& c);
 
 int wdg_upload_particles(wdg_ctx* ctx, const wdg_particles* p) {
   if (!ctx || !p || p->n < 0 || (p->n > 0 && (!p->shape || !p->pos || !p->mass || !p->inertia))) {
@@ -440,6 +443,8 @@ int wdg_upload_particles(wdg_ctx* ctx, const wdg_particles* p) {
     c.flags.ensure(n + 1); c.mask.ensure(n + 1); c.mat.ensure(n + 1); c.uid.ensure(n + 1); c.sto.ensure(n + 1);
     c.scene.ensure(n + 1);
     c.maxScene = maxScene;
+    c.bodyScene.assign(scn.begin(), scn.begin() + nW);
+    scene_bodies(c);
     c.gen++;
     if (n) WDG_HIP(hipMemcpy(c.scene.p, scn.data(), n * 4, hipMemcpyHostToDevice));
     if (n) {
@@ -549,6 +554,20 @@ int wdg_set_precision(wdg_ctx* ctx, int32_t bits) {
   });
 }
 
+// boundary bodies grouped by scene (their storage indices never change)
+static void scene_bodies(Ctx& c) {
+  const int64_t ns = std::max<int64_t>(c.nScenes, c.maxScene + 1);
+  std::vector<uint32_t> off((size_t)ns + 1, 0u), body(c.bodyScene.size());
+  for (uint32_t sc : c.bodyScene) off[sc + 1]++;
+  for (int64_t k = 0; k < ns; k++) off[k + 1] += off[k];
+  std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+  for (size_t w = 0; w < c.bodyScene.size(); w++) body[fill[c.bodyScene[w]]++] = (uint32_t)w;
+  c.sceneBodyOff.ensure(off.size());
+  c.sceneBody.ensure(std::max<size_t>(body.size(), 1));
+  WDG_HIP(hipMemcpy(c.sceneBodyOff.p, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+  if (!body.empty()) WDG_HIP(hipMemcpy(c.sceneBody.p, body.data(), body.size() * 4, hipMemcpyHostToDevice));
+}
+
 int wdg_set_scenes(wdg_ctx* ctx, int32_t nScenes, const double* dt) {
   if (!ctx || nScenes < 0 || (nScenes > 0 && !dt)) { g_err = "bad arguments"; return WDG_E_ARG; }
   return guard(ctx, [&] {
@@ -557,6 +576,7 @@ int wdg_set_scenes(wdg_ctx* ctx, int32_t nScenes, const double* dt) {
       if (!(dt[k] > 0)) throw Error(WDG_E_ARG, "scene " + std::to_string(k) + ": dt must be positive");
     if (c.forcesPending) flush_forces(c);
     c.nScenes = nScenes;
+    scene_bodies(c);
     c.dtScene.ensure((size_t)std::max(nScenes, 1));
     if (nScenes) WDG_HIP(hipMemcpy(c.dtScene.p, dt, nScenes * sizeof(double), hipMemcpyHostToDevice));
     c.gen++;

@@ -210,6 +210,8 @@ struct Ctx {
   DBuf<uint32_t> scene;  // scene of each particle (batched independent scenes)
   int32_t nScenes = 0;   // 0: one scene; else dt per scene in dtScene
   int64_t maxScene = 0;  // largest scene index uploaded
+  std::vector<uint32_t> bodyScene;        // scene of each boundary body (host)
+  DBuf<uint32_t> sceneBodyOff, sceneBody;  // boundary bodies by scene (batched scenes)
   DBuf<double> dtScene;
   // graph runner: device words [0] stop, [1] steps executed, [2] renorm flag of the step,
   // [3..4] first step number (int64), [5] trsfRenorm
