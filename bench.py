@@ -472,6 +472,70 @@ def batched_form(args):
                 bed_solid_fraction=dict(mean=float(phi_b.mean()), min=float(phi_b.min()), max=float(phi_b.max())))
 
 
+def surrogate_form(args):
+    """Learned coarse step on the granular column collapse: the network is trained on the GPU on
+    trajectories of seeded small columns (DEM, this engine), then advances the 200,000-sphere
+    column by coarse steps of 10 DEM steps with a DEM re-simulation check every 50 coarse
+    steps (fallback to DEM when the position error exceeds 0.1 r_min); against plain DEM over
+    the same simulated time: throughput, run-out and height profile."""
+    import woo_amd.surrogate as S
+    from woo_amd._native import Device
+
+    t0 = time.perf_counter()
+    samples, _ = S.column_samples(n_columns=args.sur_columns, snapshots=20, every=1500, seed=100)
+    held, _ = S.column_samples(n_columns=32, snapshots=20, every=1500, seed=5000)
+    t_data = time.perf_counter() - t0
+    net = S.Network(seed=1)
+    t0 = time.perf_counter()
+    hist = net.train([(g, a) for g, a, _, _ in samples], epochs=args.sur_epochs, lr=1e-3)
+    t_train = time.perf_counter() - t0
+
+    def rmse(fn):
+        return float(np.sqrt(np.mean([np.mean(np.sum((fn(g) - a) ** 2, axis=1)) for g, a, _, _ in held])))
+
+    err_trained = rmse(net.predict)
+    err_ballistic = rmse(lambda g: np.tile([0., 0., -9.81], (g.N, 1)))
+    dev = Device(0)
+    arr, rad = S.collapse_arrays()
+    dt = S.dem_dt()
+    coarse = args.sur_steps // S.COARSE
+    center = (0.2706 / 2, 0.2706 / 2)
+    # plain DEM over 1 + coarse * COARSE steps
+    S.upload(dev, arr)
+    dev.sync()
+    t0 = time.perf_counter()
+    S.dem_run(dev, 1 + coarse * S.COARSE, 0, dt)
+    dev.sync()
+    t_dem = time.perf_counter() - t0
+    p_dem = dev.get("pos")[1:]
+    # learned: one DEM step (collider, contacts), then coarse steps
+    S.upload(dev, arr)
+    S.dem_run(dev, 1, 0, dt)
+    dev.sync()
+    t0 = time.perf_counter()
+    st = S.run_learned(dev, net, arr, coarse, dt)
+    dev.sync()
+    t_sur = time.perf_counter() - t0
+    p_sur = dev.get("pos")[1:]
+    dev.close()
+    ro_dem, pr_dem = S.runout_profile(p_dem, rad, center)
+    ro_sur, pr_sur = S.runout_profile(p_sur, rad, center)
+    n = len(rad)
+    v_dem = n * (coarse * S.COARSE) / t_dem
+    v_sur = n * (coarse * S.COARSE) / t_sur
+    return dict(value=v_sur, unit="particle-steps/s (DEM-step equivalents)", particles=n,
+                simulated_time_s=(1 + coarse * S.COARSE) * dt, dem_steps=1 + coarse * S.COARSE,
+                plain_dem=dict(value=v_dem, unit="particle-steps/s", seconds=t_dem),
+                speedup_vs_plain_dem=v_sur / v_dem, seconds=t_sur, checks=st["checks"], fallbacks=st["fallbacks"],
+                check_error_mm=dict(max=1e3 * max(st["errors"]), mean=1e3 * float(np.mean(st["errors"]))),
+                runout_m=dict(dem=ro_dem, learned=ro_sur, rel_err=abs(ro_sur - ro_dem) / ro_dem),
+                profile_rms_diff_m=float(np.sqrt(np.mean((pr_sur - pr_dem) ** 2))),
+                position_diff_max_mm=1e3 * float(np.max(np.linalg.norm(p_sur - p_dem, axis=1))),
+                training=dict(columns=args.sur_columns, snapshots=len(samples), epochs=args.sur_epochs,
+                              loss_first=hist[0], loss_last=hist[-1], data_s=t_data, train_s=t_train,
+                              heldout_rmse_trained=err_trained, heldout_rmse_ballistic=err_ballistic))
+
+
 def pmc_probe(dev, wl, args):
     dt, cp = wl["dt"], contact_params()
     step, _ = dev.run(ENGINES, args.warmup, 0, dt, contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
@@ -521,6 +585,10 @@ def main():
     ap.add_argument("--batch-sample", type=int, default=8, help="scenes also run one at a time")
     ap.add_argument("--batch-graph", type=int, default=1,
                     help="0: step the batch with the plain engine loop (no tuning, no graph), e.g. under a tracer")
+    ap.add_argument("--surrogate", type=int, default=1, help="also run the learned coarse step (column collapse)")
+    ap.add_argument("--sur-columns", type=int, default=500)
+    ap.add_argument("--sur-epochs", type=int, default=30)
+    ap.add_argument("--sur-steps", type=int, default=20000, help="DEM steps of simulated time")
     ap.add_argument("--pmc-probe", type=int, default=0,
                     help="for rocprofv3 --pmc runs: warm up, launch the contact loop and leapfrog this many "
                          "times, then a device-to-device hipMemcpy of 512 MiB whose FETCH_SIZE/WRITE_SIZE "
@@ -585,6 +653,10 @@ def main():
     if args.batch:
         batch = batched_form(args)
         log(f"batched form: {batch}")
+    sur = None
+    if args.surrogate:
+        sur = surrogate_form(args)
+        log(f"surrogate form: {sur}")
     base = None
     if args.cpu_baseline:
         log("timing the CPU oracle on the same workload ...")
@@ -604,6 +676,7 @@ def main():
         "fp32_form": fp32,
         "drum_form": drum,
         "batched_form": batch,
+        "surrogate_form": sur,
     }
     print(json.dumps(out), flush=True)
 

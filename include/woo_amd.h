@@ -226,6 +226,38 @@ int wdg_run_graph(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t
 int wdg_tune(wdg_ctx* ctx, const wdg_program* prog, int64_t* step, double* dt, wdg_tuning* out, int64_t* stepsTaken);
 int wdg_get_tuning(wdg_ctx* ctx, wdg_tuning* out);
 int wdg_set_tuning(wdg_ctx* ctx, const wdg_tuning* t);
+
+/* ---- learned coarse step (no counterpart in the reference): a message-passing network on
+ * the contact graph of the spheres, BF16 MFMA kernels with FP32 accumulation, trained on
+ * trajectories of this engine; predicts each sphere's mean acceleration over a coarse step.
+ * Graph: CSR by receiver (rowOff[N+1]), send[E], rev[E] = the opposite edge; node inputs
+ * x[N x 5] (velocity / 1 m/s, radius / 2.5 mm, floor overlap 100 (1 - height / r) clipped to
+ * +-20), edge inputs ef[E x 9] (relative position / (ri+rj), overlap 100 (1 - d / (ri+rj)),
+ * relative velocity / 1 m/s, both radii / 2.5 mm); outputs and targets in units of g.
+ * Errors: wdg_mpnn_last_error. */
+typedef struct wdg_mpnn wdg_mpnn;
+const char* wdg_mpnn_last_error(void);
+int wdg_mpnn_create(int device, uint64_t seed, wdg_mpnn** out);
+int wdg_mpnn_destroy(wdg_mpnn* m);
+int wdg_mpnn_num_params(wdg_mpnn* m, int64_t* n);
+/* parameters (FP32 masters) and the last gradients, host arrays of wdg_mpnn_num_params */
+int wdg_mpnn_get_params(wdg_mpnn* m, float* params, float* grads);
+int wdg_mpnn_set_params(wdg_mpnn* m, const float* params);
+int wdg_mpnn_set_graph(wdg_mpnn* m, int64_t N, int64_t E, const float* x, const int32_t* rowOff,
+                       const int32_t* send, const int32_t* rev, const float* ef, const float* targets);
+/* forward pass; out [N x 3] (may be NULL): node order of the graph, or for a graph from an
+ * engine context its spheres in user-id order */
+int wdg_mpnn_forward(wdg_mpnn* m, float* out);
+/* forward and backward on the current graph and targets: mean squared error, gradients */
+int wdg_mpnn_grad(wdg_mpnn* m, double* loss);
+int wdg_mpnn_adam(wdg_mpnn* m, double lr, double beta1, double beta2, double eps);
+/* graph of the spheres of an (aperiodic) engine context, edges closer than cutoff (ri+rj)
+ * taken from its slot list: rebuilt first when forced or when a sphere moved more than maxDisp
+ * since the last rebuild (the context's Verlet distance must cover 0.05 (ri+rj) + maxDisp) */
+int wdg_mpnn_graph_from_ctx(wdg_mpnn* m, wdg_ctx* ctx, double cutoff, double floorZ, double maxDisp, int force,
+                            int64_t* nNodes, int64_t* nEdges, int* rebuilt);
+/* coarse step of the context's spheres by the last forward pass: v += a dT, x += v dT */
+int wdg_mpnn_advance(wdg_mpnn* m, wdg_ctx* ctx, double dT);
 /* HIP stream the engine launches on (hipStream_t), for callers that time it with events */
 void* wdg_stream(wdg_ctx* ctx);
 

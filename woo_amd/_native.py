@@ -25,6 +25,9 @@ EXPORTS = [
     "wdg_force_reset", "wdg_crit_dt", "wdg_contact_count", "wdg_contact_list", "wdg_observables", "wdg_stats",
     "wdg_run", "wdg_relax", "wdg_sync", "wdg_stream", "wdg_stress", "wdg_kinetic_energy",
     "wdg_set_precision", "wdg_set_scenes", "wdg_run_graph", "wdg_tune", "wdg_get_tuning", "wdg_set_tuning",
+    "wdg_mpnn_last_error", "wdg_mpnn_create", "wdg_mpnn_destroy", "wdg_mpnn_num_params", "wdg_mpnn_get_params",
+    "wdg_mpnn_set_params", "wdg_mpnn_set_graph", "wdg_mpnn_forward", "wdg_mpnn_grad", "wdg_mpnn_adam",
+    "wdg_mpnn_graph_from_ctx", "wdg_mpnn_advance",
 ]
 
 

@@ -23,7 +23,7 @@ FLAGS = [
     "-Wno-unused-result", "-I", os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["wdg_sort.hip", "wdg_collide.hip", "wdg_contact.hip", "wdg_integrate.hip", "wdg_pack.hip", "wdg_graph.hip",
+SOURCES = ["wdg_sort.hip", "wdg_collide.hip", "wdg_contact.hip", "wdg_integrate.hip", "wdg_pack.hip", "wdg_graph.hip", "wdg_mpnn.hip",
            "wdg_capi.hip"]
 HEADERS = ["wdg_internal.hpp", "wdg_math.hpp", "wdg_math32.hpp", "wdg_physics.hpp"]
 

@@ -12,6 +12,7 @@
 struct wdg_ctx {
   wdg::Ctx c;
 };
+wdg::Ctx& wdg::ctx_of(wdg_ctx* h) { return h->c; }
 
 namespace wdg {
 

@@ -304,6 +304,7 @@ void leapfrog(Ctx& c, const wdg_leapfrog_params* p);
 void force_reset(Ctx& c);
 void flush_forces(Ctx& c);
 void run_graph(Ctx& c, const wdg_program* prog, int64_t nSteps, int64_t* step, double* dt);
+uint32_t read_u32(Ctx& c, const uint32_t* d);
 int64_t tune(Ctx& c, const wdg_program* prog, int64_t* step, double* dt);
 double crit_dt(Ctx& c);
 void observables(Ctx& c, const double lo[3], const double hi[3], double out[3]);
@@ -334,4 +335,6 @@ inline int bits_for(uint64_t maxval) {
   return b;
 }
 
+// the engine state behind a C-ABI handle
+Ctx& ctx_of(wdg_ctx* h);
 }  // namespace wdg
