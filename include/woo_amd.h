@@ -231,9 +231,14 @@ int wdg_set_tuning(wdg_ctx* ctx, const wdg_tuning* t);
  * the contact graph of the spheres, BF16 MFMA kernels with FP32 accumulation, trained on
  * trajectories of this engine; predicts each sphere's mean acceleration over a coarse step.
  * Graph: CSR by receiver (rowOff[N+1]), send[E], rev[E] = the opposite edge; node inputs
- * x[N x 5] (velocity / 1 m/s, radius / 2.5 mm, floor overlap 100 (1 - height / r) clipped to
- * +-20), edge inputs ef[E x 9] (relative position / (ri+rj), overlap 100 (1 - d / (ri+rj)),
- * relative velocity / 1 m/s, both radii / 2.5 mm); outputs and targets in units of g.
+ * x[N x 7] (velocity / 1 m/s, radius / 2.5 mm, floor overlap ov = 100 (1 - height / r) clipped
+ * to +-2, then max(ov,0)^(3/2) / a and v_z max(ov,0)^(1/4) / a with a = r / 2.5 mm), edge
+ * inputs ef[E x 15] (relative position / (ri+rj), overlap ov = 100 (1 - d / (ri+rj)) clipped
+ * to +-2, relative
+ * velocity / 1 m/s, both radii / 2.5 mm, then the normal scaled by the elastic and by the
+ * viscous Hertz term per unit mass of the receiver); outputs and targets in units of 100 g
+ * (981 m/s^2).
+ * woo_amd/csrc/wdg_mpnn.hip (edge_inputs, node_inputs) gives the exact expressions.
  * Errors: wdg_mpnn_last_error. */
 typedef struct wdg_mpnn wdg_mpnn;
 const char* wdg_mpnn_last_error(void);
@@ -245,6 +250,16 @@ int wdg_mpnn_get_params(wdg_mpnn* m, float* params, float* grads);
 int wdg_mpnn_set_params(wdg_mpnn* m, const float* params);
 int wdg_mpnn_set_graph(wdg_mpnn* m, int64_t N, int64_t E, const float* x, const int32_t* rowOff,
                        const int32_t* send, const int32_t* rev, const float* ef, const float* targets);
+/* targets [N x 3] of the current graph, in the output order (see wdg_mpnn_forward) */
+int wdg_mpnn_set_targets(wdg_mpnn* m, const float* targets);
+/* the current graph: sizes, CSR, inputs (BF16 as stored, widened to float) and the output row of
+ * each node; every array may be NULL */
+int wdg_mpnn_get_graph(wdg_mpnn* m, int64_t* N, int64_t* E, int32_t* rowOff, int32_t* send, int32_t* rev, float* x,
+                       float* ef, int64_t* nodeRow);
+/* keep the current graph with its targets on the device under a slot (a training set), and make
+ * a kept graph current again (its outputs then in node order) */
+int wdg_mpnn_keep_graph(wdg_mpnn* m, int32_t slot);
+int wdg_mpnn_use_graph(wdg_mpnn* m, int32_t slot);
 /* forward pass; out [N x 3] (may be NULL): node order of the graph, or for a graph from an
  * engine context its spheres in user-id order */
 int wdg_mpnn_forward(wdg_mpnn* m, float* out);
@@ -252,12 +267,24 @@ int wdg_mpnn_forward(wdg_mpnn* m, float* out);
 int wdg_mpnn_grad(wdg_mpnn* m, double* loss);
 int wdg_mpnn_adam(wdg_mpnn* m, double lr, double beta1, double beta2, double eps);
 /* graph of the spheres of an (aperiodic) engine context, edges closer than cutoff (ri+rj)
- * taken from its slot list: rebuilt first when forced or when a sphere moved more than maxDisp
- * since the last rebuild (the context's Verlet distance must cover 0.05 (ri+rj) + maxDisp) */
+ * taken from its slot list. The slot list is refreshed first when forced, or when a sphere
+ * moved so far since its last refresh that it may miss an edge; a refresh made here raises the
+ * context's Verlet distance to (cutoff - 1) rmax + maxDisp if that is larger, so that spheres
+ * can move maxDisp before the next one. */
 int wdg_mpnn_graph_from_ctx(wdg_mpnn* m, wdg_ctx* ctx, double cutoff, double floorZ, double maxDisp, int force,
                             int64_t* nNodes, int64_t* nEdges, int* rebuilt);
 /* coarse step of the context's spheres by the last forward pass: v += a dT, x += v dT */
 int wdg_mpnn_advance(wdg_mpnn* m, wdg_ctx* ctx, double dT);
+/* In-memory snapshots of the kinematic state, the role of Scene::saveTmp / loadTmp
+ * (src/core/Scene.cpp:291-294) for the device: wdg_snapshot keeps positions, velocities,
+ * angular velocities, orientations, forces and torques of every particle on the device in slot
+ * 0..3; wdg_restore writes them back and restarts the contact history and the collider (the
+ * next collider call rebuilds, resolving its verletDist again, as after an upload);
+ * wdg_snapshot_distance gives the largest distance between a sphere's position and its
+ * position in the snapshot. A snapshot is valid until the next wdg_upload_particles. */
+int wdg_snapshot(wdg_ctx* ctx, int32_t slot);
+int wdg_restore(wdg_ctx* ctx, int32_t slot);
+int wdg_snapshot_distance(wdg_ctx* ctx, int32_t slot, double* maxDist);
 /* HIP stream the engine launches on (hipStream_t), for callers that time it with events */
 void* wdg_stream(wdg_ctx* ctx);
 

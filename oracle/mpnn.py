@@ -16,6 +16,7 @@ import numpy as np
 
 H, KX, KE, NOUT = 64, 32, 32, 3
 KM, KU = 2 * H + KE, 2 * H
+NIN, NEI = 7, 15  # node and edge input columns in use (the rest of KX, KE are zero padding)
 
 
 def layout():
@@ -62,9 +63,9 @@ class Net:
         N, E = len(rowOff) - 1, len(send)
         recv = np.repeat(np.arange(N), np.diff(rowOff))
         X = np.zeros((N, KX))
-        X[:, :5] = self._r(x)
+        X[:, :x.shape[1]] = self._r(x)
         F = np.zeros((E, KE))
-        F[:, :9] = self._r(ef)
+        F[:, :ef.shape[1]] = self._r(ef)
         c = dict(X=X, F=F, recv=recv, send=send, N=N, E=E)
         c["h"] = [self._r(np.maximum(X @ self.Wimg("encW", (KX, H)) + self.W("encB", (H,)), 0.))]
         c["m"], c["agg"], c["u"], c["Z"] = [], [], [], []

@@ -16,7 +16,7 @@ def as_f32(a):
 
 @pytest.mark.parametrize("seed", [11, 12])
 def test_gradients_match_cpu_oracle(seed):
-    from woo_amd.surrogate import Graph, Network
+    from woo_amd.surrogate import A0, Graph, Network
 
     rng = np.random.default_rng(seed)
     rowOff, send, rev, x, ef, t = random_graph(rng, N=700, deg=6)
@@ -26,7 +26,7 @@ def test_gradients_match_cpu_oracle(seed):
     net.set_params(p)
     g = Graph(rowOff, send, rev, x, ef)
     net.set_graph(g, t)
-    loss_gpu = net.loss_grad(g, t * 9.81)
+    loss_gpu = net.loss_grad(g, t * A0)
     grad_gpu = net.grads().astype(np.float64)
     loss_cpu, grad_cpu = Net(p, emulate=True).loss_grad(x, rowOff, send, rev, ef, t)
     print("loss gpu", loss_gpu, "cpu", loss_cpu)
@@ -39,30 +39,68 @@ def test_gradients_match_cpu_oracle(seed):
         print(f"{name:6s} rel {rel:.2e} cos {cos:.6f}")
         assert rel < 2e-2 and cos > 0.9995, name
     # forward outputs against the oracle
-    out = net.predict(g) / 9.81
+    out = net.predict(g) / A0
     a_cpu, _ = Net(p, emulate=True).forward(x, rowOff, send, ef)
     assert np.max(np.abs(out - a_cpu)) <= 1e-2 * np.max(np.abs(a_cpu))
 
 
 def test_device_graph_equals_host_graph():
+    # several columns as independent scenes of one context, far enough into the collapse that the
+    # forced refresh of the graph builder re-sorts the particles
     from woo_amd._native import Device
-    from woo_amd.surrogate import (Network, column, contact_graph, dem_dt, dem_run, engine_arrays, material)
+    from woo_amd.surrogate import (Network, column, contact_graph, contact_params, dem_dt, dem_run, engine_arrays,
+                                   material)
 
-    pos, rad = column(0.04, 0.08, seed=5)
+    parts = []
+    for c in range(4):
+        p, r = column(0.03, 0.3, seed=5000 + c, offset=((c % 2) * 0.08, (c // 2) * 0.08, 0.))
+        parts.append(engine_arrays(p, r))
+    arr = {k: np.concatenate([q[k] for q in parts]) for k in parts[0]}
+    arr["scene"] = np.concatenate([np.full(len(q["shape"]), i, np.int32) for i, q in enumerate(parts)])
+    sph = arr["shape"] == 0
+    rad = arr["radius"][sph]
     d = Device(0)
     d.set_field(gravity=(0., 0., -9.81), periodic=False)
     d.set_materials([material()])
-    d.upload_particles(**engine_arrays(pos, rad))
-    step = dem_run(d, 4000, 0, dem_dt())
+    d.upload_particles(**arr)
+    d.set_scenes(np.full(4, dem_dt()))
+    step = dem_run(d, 12000, 0, dem_dt())
     net = Network(seed=3)
+    # random weights everywhere (the decoder of a new network starts at zero)
+    net.set_params(as_f32(random_params(np.random.default_rng(4))))
     n, e, rb = net.graph_from(d, floorZ=0., force=True)
     a_dev = net.forward()
-    P, V = d.get("pos")[1:], d.get("vel")[1:]
+    gd, row = net.graph()
+    P, V = d.get("pos")[sph], d.get("vel")[sph]
     g = contact_graph(P, V, rad)
     print("nodes", n, "edges", e, "host edges", g.E)
-    assert n == len(rad) and e == g.E and e > 100
+    assert n == len(rad) and e == g.E and e > 1000
+    # the same edges, with the same inputs (as stored in BF16)
+    recv_d = np.repeat(np.arange(gd.N), np.diff(gd.rowOff))
+    recv_h = np.repeat(np.arange(g.N), np.diff(g.rowOff))
+    kd = row[recv_d] * n + row[gd.send]
+    kh = recv_h.astype(np.int64) * n + g.send
+    od, oh = np.argsort(kd), np.argsort(kh)
+    assert np.array_equal(kd[od], kh[oh])
+    bf = as_bf16
+    np.testing.assert_array_equal(gd.ef[od], bf(g.ef[oh]))
+    xd = np.empty_like(gd.x)
+    xd[row] = gd.x
+    np.testing.assert_array_equal(xd, bf(g.x))
     a_host = net.predict(g)
     np.testing.assert_allclose(a_dev, a_host, rtol=1e-5, atol=1e-6 * np.max(np.abs(a_host)))
+    # a refresh re-sorts the context's particles; the outputs of the graph built before stay in
+    # the user-id order of its spheres
+    net.graph_from(d, floorZ=0., force=True)
+    d.run(["Collider", "ContactLoop", "Leapfrog"], 1, step, dem_dt(), collide=dict(verletDist=-.16, forceInit=1),
+          contact=contact_params(), leapfrog=dict(damping=0.), dynDtPeriod=0)
+    np.testing.assert_array_equal(net.forward(), a_dev)
+
+
+def as_bf16(a):
+    """float32 values rounded to BF16 (nearest even), as the device stores inputs"""
+    from oracle.mpnn import bf16
+    return bf16(np.asarray(a, dtype=np.float32)).astype(np.float32)
 
 
 def test_training_gains_over_untrained_and_ballistic():
@@ -85,3 +123,20 @@ def test_training_gains_over_untrained_and_ballistic():
     # free fall (the coarse step without it) and than the untrained network
     assert hist[-1] < 0.8 * hist[0]
     assert after < 0.9 * ballistic and after < 0.85 * before
+
+
+def test_kept_graphs_train_like_host_graphs():
+    # a graph built on the device and kept there gives the same loss as the current graph, and
+    # training over kept graphs lowers the loss
+    from woo_amd.surrogate import Network, column_training_set
+
+    net = Network(seed=5)
+    slots, ball, n = column_training_set(net, 0, n_columns=16, width=0.03, height=0.1, snapshots=4, every=1200,
+                                         seed=300)
+    assert n > 100 and len(slots) == 4
+    l0 = [net.kept_loss(k) for k in slots]
+    assert l0 == [net.kept_loss(k) for k in slots]  # deterministic
+    hist = net.train_kept(slots, epochs=60, lr=1e-3, lr_end=1e-4)
+    l1 = [net.kept_loss(k) for k in slots]
+    print("loss before", np.mean(l0), "after", np.mean(l1), "ballistic", np.mean(ball))
+    assert np.mean(l1) < 0.5 * np.mean(l0) and np.mean(l1) < np.mean(ball)

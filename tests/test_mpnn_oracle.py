@@ -4,7 +4,7 @@ and the BF16-emulating mode stays close to it."""
 import numpy as np
 import pytest
 
-from oracle.mpnn import Net, bf16, layout
+from oracle.mpnn import NEI, NIN, Net, bf16, layout
 
 
 def random_graph(rng, N=40, deg=4):
@@ -21,8 +21,8 @@ def random_graph(rng, N=40, deg=4):
     rowOff = np.concatenate([[0], np.cumsum(np.bincount(recv, minlength=N))])
     key = recv * N + send
     rev = np.searchsorted(key, send * N + recv)
-    x = rng.normal(size=(N, 5))
-    ef = rng.normal(size=(len(send), 9))
+    x = rng.normal(size=(N, NIN))
+    ef = rng.normal(size=(len(send), NEI))
     t = rng.normal(size=(N, 3))
     return rowOff, send, rev, x, ef, t
 
@@ -32,10 +32,10 @@ def random_params(rng, scale=.15):
     p = rng.normal(0, scale, size=n)
     # the padding rows of the inputs stay zero, as on the device
     o, s = off["encW"]
-    p[o:o + s].reshape(32, 64)[5:] = 0.
+    p[o:o + s].reshape(32, 64)[NIN:] = 0.
     for l in range(2):
         o, s = off[f"mW{l}"]
-        p[o:o + s].reshape(160, 64)[128 + 9:] = 0.
+        p[o:o + s].reshape(160, 64)[128 + NEI:] = 0.
     for name in ("encB", "mB0", "mB1", "uB0", "uB1"):
         o, s = off[name]
         p[o:o + s] = np.abs(p[o:o + s]) + .05  # keep units active
@@ -68,6 +68,11 @@ def test_backward_matches_finite_differences():
 def test_bf16_rounding_and_emulation():
     # ties to even, and a few exact values
     assert bf16(1.0) == 1.0 and bf16(1 + 2 ** -8) == 1.0 and bf16(1 + 3 * 2 ** -8) == 1 + 2 ** -6
+    # against torch's float32 -> bfloat16 conversion (round to nearest even) on random values
+    import torch
+    v = np.random.default_rng(5).standard_normal(100000).astype(np.float32) * np.float32(1e3)
+    ref = torch.from_numpy(v).to(torch.bfloat16).to(torch.float64).numpy()
+    assert np.array_equal(bf16(v), ref)
     rng = np.random.default_rng(4)
     rowOff, send, rev, x, ef, t = random_graph(rng)
     p = random_params(rng)

@@ -27,7 +27,8 @@ EXPORTS = [
     "wdg_set_precision", "wdg_set_scenes", "wdg_run_graph", "wdg_tune", "wdg_get_tuning", "wdg_set_tuning",
     "wdg_mpnn_last_error", "wdg_mpnn_create", "wdg_mpnn_destroy", "wdg_mpnn_num_params", "wdg_mpnn_get_params",
     "wdg_mpnn_set_params", "wdg_mpnn_set_graph", "wdg_mpnn_forward", "wdg_mpnn_grad", "wdg_mpnn_adam",
-    "wdg_mpnn_graph_from_ctx", "wdg_mpnn_advance",
+    "wdg_mpnn_graph_from_ctx", "wdg_mpnn_advance", "wdg_snapshot", "wdg_restore", "wdg_snapshot_distance",
+    "wdg_mpnn_set_targets", "wdg_mpnn_keep_graph", "wdg_mpnn_use_graph", "wdg_mpnn_get_graph",
 ]
 
 
@@ -126,6 +127,9 @@ def lib():
     L.wdg_sync.argtypes = [V]
     L.wdg_stream.argtypes = [V]
     L.wdg_stream.restype = V
+    L.wdg_snapshot.argtypes = [V, C.c_int32]
+    L.wdg_restore.argtypes = [V, C.c_int32]
+    L.wdg_snapshot_distance.argtypes = [V, C.c_int32, _P(C.c_double)]
     _lib = L
     return L
 
@@ -353,6 +357,20 @@ class Device:
 
     def sync(self):
         check(self.L.wdg_sync(self.h))
+
+    def snapshot(self, slot):
+        """keep the kinematic state (pos, vel, angVel, ori, force, torque) on the device"""
+        check(self.L.wdg_snapshot(self.h, slot))
+
+    def restore(self, slot):
+        """load a snapshot back; the contact history restarts"""
+        check(self.L.wdg_restore(self.h, slot))
+
+    def snapshot_distance(self, slot):
+        """largest distance of a sphere from its position in the snapshot"""
+        d = C.c_double()
+        check(self.L.wdg_snapshot_distance(self.h, slot, C.byref(d)))
+        return d.value
 
     def stream(self):
         return self.L.wdg_stream(self.h)

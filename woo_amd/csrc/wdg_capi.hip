@@ -294,6 +294,7 @@ int wdg_ctx_destroy(wdg_ctx* ctx) {
     if (c.graph.exec) (void)hipGraphExecDestroy(c.graph.exec);
     c.graph.exec = nullptr; c.bigOff.release(); c.bigRank.release(); c.bigVal.release();
     c.bigValTmp.release(); c.bigDest.release(); c.bigKey.release(); c.bigKeyTmp.release(); c.bigList.release();
+    for (auto& b : c.snap) b.release();
     c.sortHist.release(); c.scanAux.release(); c.red.release(); c.xfer.release(); c.dflags.release();
     if (c.hflags) (void)hipHostFree(c.hflags);
     if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -457,6 +458,7 @@ int wdg_upload_particles(wdg_ctx* ctx, const wdg_particles* p) {
     }
     drop_contacts(c);
     WDG_HIP(hipStreamSynchronize(c.stream));
+    c.uploads++;
     c.forceClean = false;
     c.verletDist = NAN;
     c.dynStepLast = c.dynStepPrev = -1;
@@ -717,6 +719,21 @@ int wdg_sync(wdg_ctx* ctx) {
     WDG_HIP(hipStreamSynchronize(ctx->c.stream));
     check_errors(ctx->c);
   });
+}
+
+int wdg_snapshot(wdg_ctx* ctx, int32_t slot) {
+  if (!ctx) { g_err = "NULL argument"; return WDG_E_ARG; }
+  return guard(ctx, [&] { snapshot_save(ctx->c, slot); });
+}
+
+int wdg_restore(wdg_ctx* ctx, int32_t slot) {
+  if (!ctx) { g_err = "NULL argument"; return WDG_E_ARG; }
+  return guard(ctx, [&] { snapshot_load(ctx->c, slot); });
+}
+
+int wdg_snapshot_distance(wdg_ctx* ctx, int32_t slot, double* maxDist) {
+  if (!ctx || !maxDist) { g_err = "NULL argument"; return WDG_E_ARG; }
+  return guard(ctx, [&] { *maxDist = snapshot_distance(ctx->c, slot); });
 }
 
 void* wdg_stream(wdg_ctx* ctx) { return ctx ? (void*)ctx->c.stream : nullptr; }

@@ -670,6 +670,111 @@ double kinetic_energy(Ctx& c) {
   return s;
 }
 
+// ------------------------------------------------------------------ in-memory snapshots
+// Scene::saveTmp / loadTmp (src/core/Scene.cpp:291-294) for the kinematic state: positions,
+// velocities, angular velocities, orientations, forces and torques, kept on the device by user
+// id (so a snapshot survives the re-sorts of the broadphase). Loading one restarts the contact
+// history, as loading a saved scene without its contacts would.
+constexpr int SNAP_FIELDS = 19;
+struct SnapArgs {
+  double* f[SNAP_FIELDS];
+  double* snap;
+  const uint32_t* uid;
+  int64_t n;
+};
+static SnapArgs snap_args(Ctx& c, int slot) {
+  SnapArgs a{};
+  int q = 0;
+  for (int k = 0; k < 3; k++) a.f[q++] = c.pos[k].p;
+  for (int k = 0; k < 3; k++) a.f[q++] = c.vel[k].p;
+  for (int k = 0; k < 3; k++) a.f[q++] = c.angVel[k].p;
+  for (int k = 0; k < 4; k++) a.f[q++] = c.ori[k].p;
+  for (int k = 0; k < 3; k++) a.f[q++] = c.force[k].p;
+  for (int k = 0; k < 3; k++) a.f[q++] = c.torque[k].p;
+  a.snap = c.snap[slot].p;
+  a.uid = c.uid.p;
+  a.n = c.n;
+  return a;
+}
+__global__ void k_snap_save(SnapArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int64_t u = a.uid[i];
+#pragma unroll
+  for (int q = 0; q < SNAP_FIELDS; q++) a.snap[q * a.n + u] = a.f[q][i];
+}
+__global__ void k_snap_load(SnapArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int64_t u = a.uid[i];
+#pragma unroll
+  for (int q = 0; q < SNAP_FIELDS; q++) a.f[q][i] = a.snap[q * a.n + u];
+}
+// largest squared distance between the particles [i0, n) and their snapshot positions
+__global__ __launch_bounds__(256) void k_snap_dist(SnapArgs a, int64_t i0, unsigned long long* out) {
+  __shared__ double wm[4];
+  double m = 0.;
+  for (int64_t i = i0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t u = a.uid[i];
+    const double dx = a.f[0][i] - a.snap[u], dy = a.f[1][i] - a.snap[a.n + u], dz = a.f[2][i] - a.snap[2 * a.n + u];
+    m = fmax(m, (dx * dx + dy * dy) + dz * dz);
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    atomicMax(out, (unsigned long long)__double_as_longlong(fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]))));
+}
+
+static void snap_check(Ctx& c, int slot, bool mustExist) {
+  if (slot < 0 || slot >= Ctx::NSNAP) throw Error(WDG_E_ARG, "snapshot slot must be 0.." + std::to_string(Ctx::NSNAP - 1));
+  if (mustExist && (c.snapN[slot] != c.n || c.snapUpload[slot] != c.uploads))
+    throw Error(WDG_E_STATE, "no snapshot of the loaded particles in slot " + std::to_string(slot));
+}
+
+void snapshot_save(Ctx& c, int slot) {
+  snap_check(c, slot, false);
+  if (c.forcesPending) flush_forces(c);
+  c.snap[slot].ensure((size_t)std::max<int64_t>(c.n, 1) * SNAP_FIELDS);
+  if (c.n > 0) {
+    hipLaunchKernelGGL(k_snap_save, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, snap_args(c, slot));
+    WDG_LAUNCH_CHECK();
+  }
+  c.snapN[slot] = c.n;
+  c.snapUpload[slot] = c.uploads;
+}
+
+void snapshot_load(Ctx& c, int slot) {
+  snap_check(c, slot, true);
+  if (c.forcesPending) flush_forces(c);
+  if (c.n > 0) {
+    hipLaunchKernelGGL(k_snap_load, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, snap_args(c, slot));
+    WDG_LAUNCH_CHECK();
+  }
+  drop_contacts(c);
+  c.forceClean = false;
+  c.dirtyKnown = false;
+  // the next collider call resolves its Verlet distance again, as after an upload
+  c.verletDist = NAN;
+}
+
+double snapshot_distance(Ctx& c, int slot) {
+  snap_check(c, slot, true);
+  const int64_t m = c.n - c.nW;
+  if (m <= 0) return 0.;
+  unsigned long long* d = reinterpret_cast<unsigned long long*>(c.red.p);
+  WDG_HIP(hipMemsetAsync(d, 0, 8, c.stream));
+  hipLaunchKernelGGL(k_snap_dist, dim3((unsigned)std::min<int64_t>(nblocks(m, 256), 256)), dim3(256), 0, c.stream,
+                     snap_args(c, slot), c.nW, d);
+  WDG_LAUNCH_CHECK();
+  unsigned long long hb = 0;
+  WDG_HIP(hipMemcpyAsync(&hb, d, 8, hipMemcpyDeviceToHost, c.stream));
+  WDG_HIP(hipStreamSynchronize(c.stream));
+  double d2;
+  memcpy(&d2, &hb, 8);
+  return sqrt(d2);
+}
+
 int64_t count_real(Ctx& c) {
   Contacts& C = c.con();
   if (C.n == 0) return 0;

@@ -257,6 +257,13 @@ struct Ctx {
   DBuf<uint64_t> bigKey, bigKeyTmp;
   DBuf<uint32_t> sortHist, scanAux;
   DBuf<uint32_t> act;     // slot of each state index (contact loop), per-slot flags (relaxation)
+  // in-memory snapshots of the kinematic state (pos, vel, angVel, ori; 13 doubles per
+  // particle, by user id), valid while the particle set they were taken of is loaded
+  static constexpr int NSNAP = 4;
+  DBuf<double> snap[NSNAP];
+  int64_t snapN[NSNAP] = {-1, -1, -1, -1};
+  uint64_t snapUpload[NSNAP] = {0, 0, 0, 0};
+  uint64_t uploads = 0;   // particle uploads so far
   DBuf<double> red;       // reductions
   DBuf<double> xfer;      // user-id-ordered staging for upload/download
   DBuf<uint32_t> dflags;  // [0] dirty, [1] error bits, [2..] scratch counters
@@ -316,6 +323,9 @@ void reals_from_flags(Ctx& c, const uint32_t* flag, int64_t nslots, int dst);
 void check_errors(Ctx& c);
 void relax(Ctx& c, int maxIter, double inflate, double tol, double* maxOverlap, int* iters);
 void drop_contacts(Ctx& c);
+void snapshot_save(Ctx& c, int slot);
+void snapshot_load(Ctx& c, int slot);
+double snapshot_distance(Ctx& c, int slot);
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 

@@ -35,15 +35,63 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 typedef uint16_t bf16;
 
 constexpr int H = 64;     // hidden width
-constexpr int KX = 32;    // node input columns (5 used)
-constexpr int KE = 32;    // edge input columns (9 used)
+constexpr int KX = 32;    // node input columns (NIN used)
+constexpr int KE = 32;    // edge input columns (NEI used)
+constexpr int NIN = 7, NEI = 15;
 constexpr int KM = 2 * H + KE;  // message GEMM inner dimension (160)
 constexpr int KU = 2 * H;       // update GEMM inner dimension (128)
 constexpr int NOUT = 3;
-// input scales: lengths by R0, velocities by V0 (outputs are accelerations in units of g)
+// input scales: lengths by R0, velocities by V0 (outputs are accelerations in units of MPNN_A0)
 // overlaps (fractions of the contact distance, 1e-3 .. 1e-2) are scaled by MPNN_OV so that BF16
-// resolves them; the floor overlap is clipped to +-MPNN_OV_CLIP
-constexpr double MPNN_R0 = 2.5e-3, MPNN_V0 = 1.0, MPNN_OV = 100.0, MPNN_OV_CLIP = 20.0;
+// resolves them, and clipped to +-MPNN_OV_CLIP as inputs of their own (gaps of spheres that are
+// far apart would otherwise dominate the first layers); the force inputs use them unclipped
+constexpr double MPNN_R0 = 2.5e-3, MPNN_V0 = 1.0, MPNN_OV = 100.0, MPNN_OV_CLIP = 2.0;
+// unit of the outputs and targets: 100 g, the order of contact accelerations, so that outputs are
+// of order one
+constexpr double MPNN_A0 = 100.0 * 9.81;
+
+// Inputs of edge e (receiver i, sender j), all scaled, computed in double:
+//   0-2   (x_j - x_i) / (r_i + r_j)
+//   3     overlap ov = (1 - d / (r_i + r_j)) OV, clipped to +-OV_CLIP
+//   4-6   (v_j - v_i) / V0
+//   7, 8  r_i / R0, r_j / R0
+//   9-11  -n ov+^(3/2) sqrt(a b) (a + b) / a^3            (n = (x_j - x_i) / d, a = r_i / R0,
+//                                                          b = r_j / R0, ov+ = max(ov, 0)):
+//         the direction and radius scaling of a Hertz force on i, divided by i's mass
+//   12-14 n vN ov+^(1/4) (a + b)^(1/4) sqrt(mb sqrt(Rb)) / a^3   (vN = (v_j - v_i).n / V0,
+//         mb = a^3 b^3 / (a^3 + b^3), Rb = a b / (a + b)): the same for the viscous term
+// Inputs of node i (floor at height zf, h = z_i - zf):
+//   0-2   v_i / V0;  3  r_i / R0;  4  clip((1 - h / r_i) OV, +-OV_CLIP)
+//   5     ovf+^(3/2) / a;  6  (v_i)_z / V0 ovf+^(1/4) / a   (ovf+ = max((1 - h / r_i) OV, 0))
+// The same expressions, in the same order, build host graphs (woo_amd/surrogate.py).
+__host__ __device__ inline void edge_inputs(double xi, double yi, double zi, double xj, double yj, double zj,
+                                            double vxi, double vyi, double vzi, double vxj, double vyj,
+                                            double vzj, double ri, double rj, float* f) {
+  const double rs = ri + rj;
+  const double dx = (xj - xi) / rs, dy = (yj - yi) / rs, dz = (zj - zi) / rs;
+  const double dn = sqrt((dx * dx + dy * dy) + dz * dz);
+  const double ov = (1. - dn) * MPNN_OV;
+  const double wx = (vxj - vxi) / MPNN_V0, wy = (vyj - vyi) / MPNN_V0, wz = (vzj - vzi) / MPNN_V0;
+  const double a = ri / MPNN_R0, b = rj / MPNN_R0;
+  const double nx = dn > 0 ? dx / dn : 0., ny = dn > 0 ? dy / dn : 0., nz = dn > 0 ? dz / dn : 0.;
+  const double ovp = ov > 0 ? ov : 0.;
+  const double a3 = a * a * a, b3 = b * b * b;
+  const double hz = ovp * sqrt(ovp) * (sqrt(a * b) * (a + b) / a3);
+  const double mb = a3 * b3 / (a3 + b3), Rb = a * b / (a + b);
+  const double vn = (wx * nx + wy * ny) + wz * nz;
+  const double dp = vn * sqrt(sqrt(ovp)) * (sqrt(sqrt(a + b)) * sqrt(mb * sqrt(Rb)) / a3);
+  const double ovc = fmin(fmax(ov, -MPNN_OV_CLIP), MPNN_OV_CLIP);
+  const double v[NEI] = {dx, dy, dz, ovc, wx, wy, wz, a, b, -nx * hz, -ny * hz, -nz * hz, nx * dp, ny * dp, nz * dp};
+  for (int q = 0; q < NEI; q++) f[q] = (float)v[q];
+}
+__host__ __device__ inline void node_inputs(double vx, double vy, double vz, double r, double h, float* f) {
+  const double a = r / MPNN_R0;
+  const double ov = (1. - h / r) * MPNN_OV;
+  const double ovp = ov > 0 ? ov : 0.;
+  const double v[NIN] = {vx / MPNN_V0, vy / MPNN_V0, vz / MPNN_V0, a, fmin(fmax(ov, -MPNN_OV_CLIP), MPNN_OV_CLIP),
+                         ovp * sqrt(ovp) / a, vz / MPNN_V0 * sqrt(sqrt(ovp)) / a};
+  for (int q = 0; q < NIN; q++) f[q] = (float)v[q];
+}
 
 // round to nearest even (finite inputs)
 __host__ __device__ inline bf16 f2bf(float f) {
@@ -116,8 +164,9 @@ template <int MODE, bool RESID>
 __global__ __launch_bounds__(256) void k_mm_fwd(ARows a, int64_t M, const bf16* __restrict__ Wt,
                                                  const float* __restrict__ bias, bf16* y, bf16* hOut) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // every wavefront reaches the barrier of the epilogue; rows at or past M are neither read
+  // nor written
   const int64_t r0 = ((int64_t)blockIdx.x * 4 + wv) * 16;
-  if (r0 >= M) return;
   const int64_t r = r0 + (lane & 15);
   const bool live = r < M;
   const int kq = 8 * (lane >> 4);
@@ -132,19 +181,35 @@ __global__ __launch_bounds__(256) void k_mm_fwd(ARows a, int64_t M, const bf16* 
       acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[cb], 0, 0, 0);
     }
   }
-  // C layout: row 4*(lane/16)+q, column lane%16 of each block
+  // C layout: row 4*(lane/16)+q, column lane%16 of each block. The wavefront's 16 x 64 tile
+  // goes through LDS so that rows leave as 16-byte stores.
+  __shared__ bf16 tile[4][16][H + 8];
 #pragma unroll
   for (int cb = 0; cb < 4; cb++) {
     const int col = cb * 16 + (lane & 15);
     const float b = bias[col];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int64_t rr = r0 + 4 * (lane >> 4) + q;
-      if (rr >= M) continue;
-      const float v = fmaxf(acc[cb][q] + b, 0.f);
-      const bf16 vb = f2bf(v);
-      y[rr * H + col] = vb;
-      if (RESID) hOut[rr * H + col] = f2bf(bf2f(a.h[rr * H + col]) + bf2f(vb));
+    for (int q = 0; q < 4; q++) tile[wv][4 * (lane >> 4) + q][col] = f2bf(fmaxf(acc[cb][q] + b, 0.f));
+  }
+  __syncthreads();
+#pragma unroll
+  for (int half = 0; half < 2; half++) {
+    const int row = half * 8 + (lane >> 3), c0 = (lane & 7) * 8;
+    const int64_t rr = r0 + row;
+    if (rr >= M) continue;
+    const uint4 v = *reinterpret_cast<const uint4*>(&tile[wv][row][c0]);
+    *reinterpret_cast<uint4*>(y + rr * H + c0) = v;
+    if (RESID) {
+      const uint4 hv = *reinterpret_cast<const uint4*>(a.h + rr * H + c0);
+      const uint32_t uw[4] = {v.x, v.y, v.z, v.w}, hw[4] = {hv.x, hv.y, hv.z, hv.w};
+      uint32_t o[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const bf16 lo = f2bf(bf2f((bf16)(hw[q] & 0xFFFFu)) + bf2f((bf16)(uw[q] & 0xFFFFu)));
+        const bf16 hi = f2bf(bf2f((bf16)(hw[q] >> 16)) + bf2f((bf16)(uw[q] >> 16)));
+        o[q] = (uint32_t)lo | ((uint32_t)hi << 16);
+      }
+      *reinterpret_cast<uint4*>(hOut + rr * H + c0) = make_uint4(o[0], o[1], o[2], o[3]);
     }
   }
 }
@@ -272,15 +337,29 @@ __global__ void k_sum_partials(const float* __restrict__ partial, int64_t nblk, 
 }
 
 // ------------------------------------------------------------------ graph sums
-// agg_i = sum of m_e over edges received by i (one wavefront per node, lane = feature)
+// agg_i = sum of m_e over edges received by i, in edge order (eight threads per node, eight
+// features each, 16-byte loads and stores)
 __global__ __launch_bounds__(256) void k_agg(int64_t N, const int32_t* __restrict__ rowOff,
                                               const bf16* __restrict__ m, bf16* agg) {
-  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int f = threadIdx.x & 63;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t >> 3;
+  const int f0 = (int)(t & 7) * 8;
   if (i >= N) return;
-  float s = 0.f;
-  for (int32_t e = rowOff[i]; e < rowOff[i + 1]; e++) s += bf2f(m[(int64_t)e * H + f]);
-  agg[i * H + f] = f2bf(s);
+  float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int32_t e1 = rowOff[i + 1];
+  for (int32_t e = rowOff[i]; e < e1; e++) {
+    const uint4 v = *reinterpret_cast<const uint4*>(m + (int64_t)e * H + f0);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      s[2 * q] += bf2f((bf16)(w[q] & 0xFFFFu));
+      s[2 * q + 1] += bf2f((bf16)(w[q] >> 16));
+    }
+  }
+  uint32_t o[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) o[q] = (uint32_t)f2bf(s[2 * q]) | ((uint32_t)f2bf(s[2 * q + 1]) << 16);
+  *reinterpret_cast<uint4*>(agg + i * H + f0) = make_uint4(o[0], o[1], o[2], o[3]);
 }
 // dY_m,e = dagg_recv(e) * (m_e > 0), BF16
 __global__ void k_edge_dy(int64_t E, const int32_t* __restrict__ recv, const float* __restrict__ dU,
@@ -415,11 +494,47 @@ struct Mpnn {
   std::vector<int64_t> outRow;
   Ctx* fromCtx = nullptr;  // the engine context of the current graph (storage order of its nodes)
   DBuf<uint32_t> degScan;
+  DBuf<uint32_t> nodeUid;  // user id of each node of a graph from an engine context
   // activations
   DBuf<bf16> h[3], m[2], agg[2], u[2];
   DBuf<float> out, dA, dhA, dhB, dU, dZ, part, partB, partLoss;
   DBuf<bf16> dY, dYe;
+  hipEvent_t done = nullptr;  // orders the network's stream and an engine context's stream
+  // graphs kept on the device (training sets): inputs, structure and targets
+  struct Kept {
+    int64_t N = -1, E = 0;
+    DBuf<int32_t> rowOff, recv, send, rev;
+    DBuf<bf16> X, EF;
+    DBuf<float> T;
+    void release() {
+      rowOff.release(); recv.release(); send.release(); rev.release(); X.release(); EF.release(); T.release();
+      N = -1;
+    }
+  };
+  std::vector<Kept> kept;
 };
+
+// output row of each node: its own index for graphs given by the caller, the rank of its user
+// id among the spheres for a graph from an engine context
+static void out_rows(Mpnn& s) {
+  if (!s.fromCtx || !s.outRow.empty() || s.N == 0) return;
+  // the user ids of the nodes as they were when the graph was built (the context may have
+  // re-sorted its particles since)
+  std::vector<uint32_t> uid((size_t)s.N);
+  WDG_HIP(hipMemcpyAsync(uid.data(), s.nodeUid.p, s.N * 4, hipMemcpyDeviceToHost, s.stream));
+  WDG_HIP(hipStreamSynchronize(s.stream));
+  std::vector<int64_t> idx((size_t)s.N);
+  for (int64_t t = 0; t < s.N; t++) idx[t] = t;
+  std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return uid[a] < uid[b]; });
+  s.outRow.assign((size_t)s.N, 0);
+  for (int64_t r = 0; r < s.N; r++) s.outRow[idx[r]] = r;
+}
+
+template <class T>
+static void copy_buf(DBuf<T>& dst, const DBuf<T>& src, size_t n, hipStream_t st) {
+  dst.ensure(std::max<size_t>(n, 1));
+  if (n) WDG_HIP(hipMemcpyAsync(dst.p, src.p, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+}
 
 static void images(Mpnn& s) {
   const int Ks[5] = {KX, KM, KU, KM, KU};
@@ -476,7 +591,7 @@ static void forward(Mpnn& s) {
                          (bf16*)nullptr);
       WDG_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_agg, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, s.stream, N, (const int32_t*)s.rowOff.p,
+    hipLaunchKernelGGL(k_agg, dim3(nblocks(N * 8, 256)), dim3(256), 0, s.stream, N, (const int32_t*)s.rowOff.p,
                        (const bf16*)s.m[l].p, s.agg[l].p);
     WDG_LAUNCH_CHECK();
     ARows au{};
@@ -585,6 +700,17 @@ struct GraphArgs {
   int32_t* edgeOfSlot;  // [2 * nslots]
   bf16 *X, *EF;
 };
+// a row of K inputs as BF16, in 16-byte stores
+template <int K>
+__device__ __forceinline__ void store_row(bf16* o, const float* f) {
+#pragma unroll
+  for (int c = 0; c < K; c += 8) {
+    uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) w[q] = (uint32_t)f2bf(f[c + 2 * q]) | ((uint32_t)f2bf(f[c + 2 * q + 1]) << 16);
+    *reinterpret_cast<uint4*>(o + c) = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+}
 __device__ __forceinline__ bool edge_keep(const GraphArgs& g, uint32_t s, uint32_t slot, uint32_t& other) {
   if ((g.info[slot] & CI_TYPE_MASK) != CT_SS) return false;
   const uint32_t a = g.A[slot], b = g.B[slot];
@@ -616,26 +742,17 @@ __global__ void k_mg_fill(GraphArgs g) {
     const int side = g.A[slot] == s ? 0 : 1;
     g.slotSide[e] = (int32_t)(2 * slot + side);
     g.edgeOfSlot[2 * slot + side] = e;
-    // edge inputs: relative position / (ri + rj), overlap (1 - d / (ri + rj)) * OV_SCALE, relative
-    // velocity, radii
-    const double rs = g.rad[s] + g.rad[other];
-    const double dx = (g.px[other] - g.px[s]) / rs, dy = (g.py[other] - g.py[s]) / rs, dz = (g.pz[other] - g.pz[s]) / rs;
-    const float f[9] = {(float)dx, (float)dy, (float)dz, (float)((1. - sqrt((dx * dx + dy * dy) + dz * dz)) * MPNN_OV),
-                        (float)((g.vx[other] - g.vx[s]) / MPNN_V0), (float)((g.vy[other] - g.vy[s]) / MPNN_V0),
-                        (float)((g.vz[other] - g.vz[s]) / MPNN_V0), (float)(g.rad[s] / MPNN_R0),
-                        (float)(g.rad[other] / MPNN_R0)};
-    bf16* o = g.EF + (int64_t)e * KE;
-    for (int q = 0; q < 9; q++) o[q] = f2bf(f[q]);
-    for (int q = 9; q < KE; q++) o[q] = 0;
+    float f[KE];
+    for (int q = NEI; q < KE; q++) f[q] = 0.f;
+    edge_inputs(g.px[s], g.py[s], g.pz[s], g.px[other], g.py[other], g.pz[other], g.vx[s], g.vy[s], g.vz[s],
+                g.vx[other], g.vy[other], g.vz[other], g.rad[s], g.rad[other], f);
+    store_row<KE>(g.EF + (int64_t)e * KE, f);
     e++;
   }
-  // node inputs: velocity, radius, overlap with the floor (1 - height / r) * OV_SCALE, clipped
-  const double ov = (1. - (g.pz[s] - g.floorZ) / g.rad[s]) * MPNN_OV;
-  const float x[5] = {(float)(g.vx[s] / MPNN_V0), (float)(g.vy[s] / MPNN_V0), (float)(g.vz[s] / MPNN_V0),
-                      (float)(g.rad[s] / MPNN_R0), (float)fmin(fmax(ov, -MPNN_OV_CLIP), MPNN_OV_CLIP)};
-  bf16* o = g.X + t * KX;
-  for (int q = 0; q < 5; q++) o[q] = f2bf(x[q]);
-  for (int q = 5; q < KX; q++) o[q] = 0;
+  float x[KX];
+  for (int q = NIN; q < KX; q++) x[q] = 0.f;
+  node_inputs(g.vx[s], g.vy[s], g.vz[s], g.rad[s], g.pz[s] - g.floorZ, x);
+  store_row<KX>(g.X + t * KX, x);
 }
 __global__ void k_mg_rev(int64_t E, const int32_t* __restrict__ slotSide, const int32_t* __restrict__ edgeOfSlot,
                          int32_t* rev) {
@@ -643,7 +760,7 @@ __global__ void k_mg_rev(int64_t E, const int32_t* __restrict__ slotSide, const 
   if (e >= E) return;
   rev[e] = edgeOfSlot[slotSide[e] ^ 1];
 }
-// coarse step: v += a dT, x += v dT for the spheres (a in units of g)
+// coarse step: v += a dT, x += v dT for the spheres (a in units of the given g)
 __global__ void k_mg_advance(int64_t nW, int64_t nSph, const float* __restrict__ a, double dT, double g, double* px,
                              double* py, double* pz, double* vx, double* vy, double* vz, const uint32_t* flags) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -660,13 +777,18 @@ __global__ __launch_bounds__(256) void k_mg_disp(int64_t n, const double* __rest
                                                   const double* __restrict__ pz, const double* __restrict__ lx,
                                                   const double* __restrict__ ly, const double* __restrict__ lz,
                                                   unsigned long long* out) {
+  __shared__ double wm[4];
   double m = 0.;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const double dx = px[i] - lx[i], dy = py[i] - ly[i], dz = pz[i] - lz[i];
     m = fmax(m, (dx * dx + dy * dy) + dz * dz);
   }
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  // one device-wide atomic per workgroup
+  if (threadIdx.x == 0)
+    atomicMax(out, (unsigned long long)__double_as_longlong(fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]))));
 }
 
 }  // namespace wdg
@@ -715,9 +837,11 @@ int wdg_mpnn_create(int device, uint64_t seed, wdg_mpnn** out) {
     s.device = device;
     WDG_HIP(hipSetDevice(device));
     WDG_HIP(hipStreamCreate(&s.stream));
+    WDG_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
     const size_t n = s.L.total;
     s.P.ensure(n); s.G.ensure(n); s.Am.ensure(n); s.Av.ensure(n);
-    // Glorot-uniform weights over the used input rows, zero biases and padding rows
+    // Glorot-uniform weights over the used input rows, zero biases and padding rows (the
+    // decoder below)
     std::vector<float> p(n, 0.f);
     uint64_t st = seed;
     auto fill = [&](size_t off, int K, int Kused, int O) {
@@ -728,15 +852,17 @@ int wdg_mpnn_create(int device, uint64_t seed, wdg_mpnn** out) {
           p[off + (size_t)k * O + o] = k < Kused ? (float)((2 * u - 1) * lim) : 0.f;
         }
     };
-    fill(s.L.encW, KX, 5, H);
+    fill(s.L.encW, KX, NIN, H);
     for (int l = 0; l < 2; l++) {
-      fill(s.L.mW[l], KM, 2 * H + 9, H);
+      fill(s.L.mW[l], KM, 2 * H + NEI, H);
       // rows of the unused edge-input columns are zero
-      for (int k = 2 * H + 9; k < KM; k++)
+      for (int k = 2 * H + NEI; k < KM; k++)
         for (int o = 0; o < H; o++) p[s.L.mW[l] + (size_t)k * H + o] = 0.f;
       fill(s.L.uW[l], KU, KU, H);
     }
-    fill(s.L.decW, H, H, NOUT);
+    // the decoder starts at zero weights and a bias of free fall: the untrained network is the
+    // ballistic coarse step, and training learns the contact accelerations on top of it
+    p[s.L.decB + 2] = (float)(-9.81 / MPNN_A0);
     WDG_HIP(hipMemcpy(s.P.p, p.data(), n * 4, hipMemcpyHostToDevice));
     WDG_HIP(hipMemset(s.Am.p, 0, n * 4));
     WDG_HIP(hipMemset(s.Av.p, 0, n * 4));
@@ -762,6 +888,10 @@ int wdg_mpnn_destroy(wdg_mpnn* m) {
                       &s.u[0], &s.u[1], &s.dY, &s.dYe};
   for (auto* b : bb) b->release();
   s.rowOff.release(); s.recv.release(); s.send.release(); s.rev.release();
+  s.deg.release(); s.slotSide.release(); s.edgeOfSlot.release(); s.degScan.release(); s.nodeUid.release();
+  for (auto& k : s.kept) k.release();
+  if (s.stream) (void)hipStreamSynchronize(s.stream);
+  if (s.done) (void)hipEventDestroy(s.done);
   if (s.stream) (void)hipStreamDestroy(s.stream);
   delete m;
   return WDG_OK;
@@ -793,8 +923,8 @@ int wdg_mpnn_set_params(wdg_mpnn* m, const float* p) {
   });
 }
 
-/* graph from host arrays: x [N x 5] node inputs, CSR by receiver rowOff[N+1], send[E], rev[E],
- * ef [E x 9] edge inputs, targets t [N x 3] (may be NULL) */
+/* graph from host arrays: x [N x 7] node inputs, CSR by receiver rowOff[N+1], send[E], rev[E],
+ * ef [E x 15] edge inputs, targets t [N x 3] (may be NULL) */
 int wdg_mpnn_set_graph(wdg_mpnn* m, int64_t N, int64_t E, const float* x, const int32_t* rowOff,
                        const int32_t* send, const int32_t* rev, const float* ef, const float* t) {
   if (!m || N < 0 || E < 0 || (N > 0 && (!x || !rowOff)) || (E > 0 && (!send || !rev || !ef))) return WDG_E_ARG;
@@ -805,12 +935,12 @@ int wdg_mpnn_set_graph(wdg_mpnn* m, int64_t N, int64_t E, const float* x, const 
     std::vector<int32_t> recv((size_t)std::max<int64_t>(E, 1));
     for (int64_t i = 0; i < N; i++) {
       if (rowOff[i + 1] < rowOff[i]) throw Error(WDG_E_ARG, "rowOff must not decrease");
-      for (int c = 0; c < 5; c++) X[i * KX + c] = f2bf(x[i * 5 + c]);
+      for (int c = 0; c < NIN; c++) X[i * KX + c] = f2bf(x[i * NIN + c]);
       for (int32_t e = rowOff[i]; e < rowOff[i + 1]; e++) recv[e] = (int32_t)i;
     }
     for (int64_t e = 0; e < E; e++) {
       if (send[e] < 0 || send[e] >= N || rev[e] < 0 || rev[e] >= E) throw Error(WDG_E_ARG, "edge index out of range");
-      for (int c = 0; c < 9; c++) F[e * KE + c] = f2bf(ef[e * 9 + c]);
+      for (int c = 0; c < NEI; c++) F[e * KE + c] = f2bf(ef[e * NEI + c]);
     }
     s.N = N;
     s.E = E;
@@ -837,17 +967,7 @@ int wdg_mpnn_forward(wdg_mpnn* m, float* out) {
   return mguard([&] {
     Mpnn& s = m->s;
     forward(s);
-    if (out && s.N > 0 && s.fromCtx && s.outRow.empty()) {
-      Ctx& c = *s.fromCtx;
-      std::vector<uint32_t> uid((size_t)s.N);
-      WDG_HIP(hipMemcpyAsync(uid.data(), c.uid.p + c.nW, s.N * 4, hipMemcpyDeviceToHost, c.stream));
-      WDG_HIP(hipStreamSynchronize(c.stream));
-      std::vector<int64_t> idx((size_t)s.N);
-      for (int64_t t = 0; t < s.N; t++) idx[t] = t;
-      std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return uid[a] < uid[b]; });
-      s.outRow.assign((size_t)s.N, 0);
-      for (int64_t r = 0; r < s.N; r++) s.outRow[idx[r]] = r;
-    }
+    if (out) out_rows(s);
     if (out && s.N > 0) {
       std::vector<float> o((size_t)s.N * NOUT);
       WDG_HIP(hipMemcpyAsync(o.data(), s.out.p, o.size() * 4, hipMemcpyDeviceToHost, s.stream));
@@ -857,7 +977,94 @@ int wdg_mpnn_forward(wdg_mpnn* m, float* out) {
         for (int c2 = 0; c2 < NOUT; c2++) out[r * NOUT + c2] = o[t * NOUT + c2];
       }
     }
+    // without an output the pass stays queued: wdg_mpnn_advance orders the engine's stream after it
+    if (!s.fromCtx) WDG_HIP(hipStreamSynchronize(s.stream));
+  });
+}
+
+/* targets t [N x 3] of the current graph, in its output order */
+int wdg_mpnn_set_targets(wdg_mpnn* m, const float* t) {
+  if (!m || !t) return WDG_E_ARG;
+  return mguard([&] {
+    Mpnn& s = m->s;
+    out_rows(s);
+    std::vector<float> T((size_t)std::max<int64_t>(s.N, 1) * NOUT);
+    for (int64_t q = 0; q < s.N; q++) {
+      const int64_t r = s.outRow.empty() ? q : s.outRow[q];
+      for (int c2 = 0; c2 < NOUT; c2++) T[q * NOUT + c2] = t[r * NOUT + c2];
+    }
+    s.T.ensure(T.size());
+    WDG_HIP(hipMemcpyAsync(s.T.p, T.data(), (size_t)s.N * NOUT * 4, hipMemcpyHostToDevice, s.stream));
     WDG_HIP(hipStreamSynchronize(s.stream));
+  });
+}
+
+/* the current graph: sizes, and (each array may be NULL) its CSR, inputs as stored (BF16
+ * widened to float) and the output row of each node */
+int wdg_mpnn_get_graph(wdg_mpnn* m, int64_t* N, int64_t* E, int32_t* rowOff, int32_t* send, int32_t* rev, float* x,
+                       float* ef, int64_t* nodeRow) {
+  if (!m) return WDG_E_ARG;
+  return mguard([&] {
+    Mpnn& s = m->s;
+    WDG_HIP(hipStreamSynchronize(s.stream));
+    if (N) *N = s.N;
+    if (E) *E = s.E;
+    if (rowOff) WDG_HIP(hipMemcpy(rowOff, s.rowOff.p, ((size_t)s.N + 1) * 4, hipMemcpyDeviceToHost));
+    if (send && s.E) WDG_HIP(hipMemcpy(send, s.send.p, (size_t)s.E * 4, hipMemcpyDeviceToHost));
+    if (rev && s.E) WDG_HIP(hipMemcpy(rev, s.rev.p, (size_t)s.E * 4, hipMemcpyDeviceToHost));
+    auto widen = [&](const DBuf<bf16>& b, int64_t rows, int K, int used, float* out) {
+      std::vector<bf16> h((size_t)rows * K);
+      if (rows) WDG_HIP(hipMemcpy(h.data(), b.p, h.size() * 2, hipMemcpyDeviceToHost));
+      for (int64_t r = 0; r < rows; r++)
+        for (int c2 = 0; c2 < used; c2++) out[r * used + c2] = bf2f(h[(size_t)r * K + c2]);
+    };
+    if (x) widen(s.X, s.N, KX, NIN, x);
+    if (ef) widen(s.EF, s.E, KE, NEI, ef);
+    if (nodeRow) {
+      out_rows(s);
+      for (int64_t q = 0; q < s.N; q++) nodeRow[q] = s.outRow.empty() ? q : s.outRow[q];
+    }
+  });
+}
+
+/* keep the current graph (inputs, structure, targets) on the device under slot */
+int wdg_mpnn_keep_graph(wdg_mpnn* m, int32_t slot) {
+  if (!m || slot < 0) return WDG_E_ARG;
+  return mguard([&] {
+    Mpnn& s = m->s;
+    if ((size_t)slot >= s.kept.size()) s.kept.resize((size_t)slot + 1);
+    Mpnn::Kept& k = s.kept[slot];
+    copy_buf(k.rowOff, s.rowOff, (size_t)s.N + 1, s.stream);
+    copy_buf(k.recv, s.recv, (size_t)s.E, s.stream);
+    copy_buf(k.send, s.send, (size_t)s.E, s.stream);
+    copy_buf(k.rev, s.rev, (size_t)s.E, s.stream);
+    copy_buf(k.X, s.X, (size_t)s.N * KX, s.stream);
+    copy_buf(k.EF, s.EF, (size_t)s.E * KE, s.stream);
+    copy_buf(k.T, s.T, (size_t)s.N * NOUT, s.stream);
+    k.N = s.N;
+    k.E = s.E;
+    WDG_HIP(hipStreamSynchronize(s.stream));
+  });
+}
+
+/* make a kept graph current (outputs in its node order) */
+int wdg_mpnn_use_graph(wdg_mpnn* m, int32_t slot) {
+  if (!m || slot < 0) return WDG_E_ARG;
+  return mguard([&] {
+    Mpnn& s = m->s;
+    if ((size_t)slot >= s.kept.size() || s.kept[slot].N < 0) throw Error(WDG_E_ARG, "no graph kept in this slot");
+    Mpnn::Kept& k = s.kept[slot];
+    s.N = k.N;
+    s.E = k.E;
+    copy_buf(s.rowOff, k.rowOff, (size_t)s.N + 1, s.stream);
+    copy_buf(s.recv, k.recv, (size_t)s.E, s.stream);
+    copy_buf(s.send, k.send, (size_t)s.E, s.stream);
+    copy_buf(s.rev, k.rev, (size_t)s.E, s.stream);
+    copy_buf(s.X, k.X, (size_t)s.N * KX, s.stream);
+    copy_buf(s.EF, k.EF, (size_t)s.E * KE, s.stream);
+    copy_buf(s.T, k.T, (size_t)s.N * NOUT, s.stream);
+    s.outRow.clear();
+    s.fromCtx = nullptr;
   });
 }
 
@@ -898,47 +1105,72 @@ int wdg_mpnn_graph_from_ctx(wdg_mpnn* m, wdg_ctx* ctx, double cutoff, double flo
     if (c.field.periodic) throw Error(WDG_E_ARG, "the learned step supports aperiodic scenes");
     if (c.forcesPending) flush_forces(c);
     const int64_t nSph = c.n - c.nW;
-    bool rb = force != 0 || !c.boundsValid;
-    if (!rb && nSph > 0) {
-      unsigned long long* d = reinterpret_cast<unsigned long long*>(c.red.p);
-      WDG_HIP(hipMemsetAsync(d, 0, 8, c.stream));
-      hipLaunchKernelGGL(k_mg_disp, dim3((unsigned)std::min<int64_t>(nblocks(nSph, 256), 1024)), dim3(256), 0,
-                         c.stream, nSph, (const double*)(c.pos[0].p + c.nW), (const double*)(c.pos[1].p + c.nW),
-                         (const double*)(c.pos[2].p + c.nW), (const double*)(c.lastPos[0].p + c.nW),
-                         (const double*)(c.lastPos[1].p + c.nW), (const double*)(c.lastPos[2].p + c.nW), d);
-      WDG_LAUNCH_CHECK();
-      unsigned long long hb = 0;
-      WDG_HIP(hipMemcpyAsync(&hb, d, 8, hipMemcpyDeviceToHost, c.stream));
-      WDG_HIP(hipStreamSynchronize(c.stream));
-      double d2;
-      memcpy(&d2, &hb, 8);
-      rb = d2 > maxDisp * maxDisp;
-    }
-    if (rb) {
-      if (std::isnan(c.verletDist)) throw Error(WDG_E_STATE, "run the collider once before building graphs");
-      wdg_collide_params p{c.verletDist, 1};
-      collide(c, &p);
-    }
-    if (rebuilt) *rebuilt = rb ? 1 : 0;
-    Contacts& C = c.con();
+    // the network's kernels of the previous graph (on its own stream) end before the graph changes
+    WDG_HIP(hipEventRecord(s.done, s.stream));
+    WDG_HIP(hipStreamWaitEvent(c.stream, s.done, 0));
     s.deg.ensure((size_t)nSph + 1);
     s.rowOff.ensure((size_t)nSph + 1);
-    s.edgeOfSlot.ensure((size_t)std::max<int64_t>(2 * C.n, 2));
     GraphArgs g{};
     g.nW = c.nW; g.nSph = nSph;
-    g.adj = C.adj.p; g.adjOff = C.adjOff.p; g.A = C.A.p; g.B = C.B.p; g.info = C.info.p;
-    g.px = c.pos[0].p; g.py = c.pos[1].p; g.pz = c.pos[2].p;
-    g.vx = c.vel[0].p; g.vy = c.vel[1].p; g.vz = c.vel[2].p; g.rad = c.radius.p;
     g.cutoff = cutoff; g.floorZ = floorZ;
     g.deg = s.deg.p; g.rowOff = s.rowOff.p;
-    int64_t E = 0;
-    if (nSph > 0) {
+    // the context's arrays as they are now: a refresh permutes the particles into buffers of its
+    // own, so these are bound again after every collider call
+    auto bind = [&]() {
+      Contacts& C = c.con();
+      g.adj = C.adj.p; g.adjOff = C.adjOff.p; g.A = C.A.p; g.B = C.B.p; g.info = C.info.p;
+      g.px = c.pos[0].p; g.py = c.pos[1].p; g.pz = c.pos[2].p;
+      g.vx = c.vel[0].p; g.vy = c.vel[1].p; g.vz = c.vel[2].p; g.rad = c.radius.p;
+    };
+    // edge counts over the current slot list, and in the same round trip the largest
+    // displacement since its refresh (the slot list holds every edge while that stays below maxDisp)
+    auto count = [&](bool disp, double& d2) -> int64_t {
+      bind();
+      unsigned long long* d = reinterpret_cast<unsigned long long*>(c.red.p);
+      if (disp) {
+        WDG_HIP(hipMemsetAsync(d, 0, 8, c.stream));
+        hipLaunchKernelGGL(k_mg_disp, dim3((unsigned)std::min<int64_t>(nblocks(nSph, 256), 256)), dim3(256), 0,
+                           c.stream, nSph, (const double*)(c.pos[0].p + c.nW), (const double*)(c.pos[1].p + c.nW),
+                           (const double*)(c.pos[2].p + c.nW), (const double*)(c.lastPos[0].p + c.nW),
+                           (const double*)(c.lastPos[1].p + c.nW), (const double*)(c.lastPos[2].p + c.nW), d);
+        WDG_LAUNCH_CHECK();
+      }
       hipLaunchKernelGGL(k_mg_count, dim3(nblocks(nSph, 256)), dim3(256), 0, c.stream, g);
       WDG_LAUNCH_CHECK();
       scan_exclusive(c, reinterpret_cast<const uint32_t*>(s.deg.p), reinterpret_cast<uint32_t*>(s.rowOff.p), nSph,
                      reinterpret_cast<uint32_t*>(s.rowOff.p + nSph));
-      E = read_u32(c, reinterpret_cast<const uint32_t*>(s.rowOff.p + nSph));
+      uint32_t hE = 0;
+      unsigned long long hb = 0;
+      WDG_HIP(hipMemcpyAsync(&hE, s.rowOff.p + nSph, 4, hipMemcpyDeviceToHost, c.stream));
+      if (disp) WDG_HIP(hipMemcpyAsync(&hb, d, 8, hipMemcpyDeviceToHost, c.stream));
+      WDG_HIP(hipStreamSynchronize(c.stream));
+      memcpy(&d2, &hb, 8);
+      return (int64_t)hE;
+    };
+    if (std::isnan(c.verletDist)) throw Error(WDG_E_STATE, "run the collider once before building graphs");
+    // the slot list holds every pair closer than r1 + r2 + 2 verletDist at its refresh, so it
+    // holds every edge while no sphere has moved farther than verletDist - need since then
+    const double need = 0.5 * (cutoff - 1.) * 2. * c.rmax;
+    const double allow = c.verletDist - need;
+    bool rb = force != 0 || !c.boundsValid || allow < 0;
+    int64_t E = 0;
+    if (!rb && nSph > 0) {
+      double d2 = 0.;
+      E = count(true, d2);
+      rb = d2 > allow * allow;
     }
+    if (rb) {
+      // rebuild with room for maxDisp of motion before the next one
+      c.verletDist = std::max(c.verletDist, need + std::max(maxDisp, 0.));
+      wdg_collide_params p{c.verletDist, 1};
+      collide(c, &p);
+      double d2;
+      if (nSph > 0) E = count(false, d2);
+    }
+    if (rebuilt) *rebuilt = rb ? 1 : 0;
+    bind();
+    Contacts& C = c.con();
+    s.edgeOfSlot.ensure((size_t)std::max<int64_t>(2 * C.n, 2));
     s.recv.ensure((size_t)std::max<int64_t>(E, 1)); s.send.ensure((size_t)std::max<int64_t>(E, 1));
     s.rev.ensure((size_t)std::max<int64_t>(E, 1)); s.slotSide.ensure((size_t)std::max<int64_t>(E, 1));
     s.X.ensure((size_t)std::max<int64_t>(nSph, 1) * KX);
@@ -955,6 +1187,12 @@ int wdg_mpnn_graph_from_ctx(wdg_mpnn* m, wdg_ctx* ctx, double cutoff, double flo
                          (const int32_t*)s.edgeOfSlot.p, s.rev.p);
       WDG_LAUNCH_CHECK();
     }
+    s.nodeUid.ensure((size_t)std::max<int64_t>(nSph, 1));
+    if (nSph > 0)
+      WDG_HIP(hipMemcpyAsync(s.nodeUid.p, c.uid.p + c.nW, nSph * 4, hipMemcpyDeviceToDevice, c.stream));
+    // the network's next kernels (its own stream) start after the graph is built
+    WDG_HIP(hipEventRecord(s.done, c.stream));
+    WDG_HIP(hipStreamWaitEvent(s.stream, s.done, 0));
     // output rows (spheres in user-id order) are found when an output is asked for
     s.outRow.clear();
     s.fromCtx = &c;
@@ -972,15 +1210,17 @@ int wdg_mpnn_advance(wdg_mpnn* m, wdg_ctx* ctx, double dT) {
     Ctx& c = ctx_of(ctx);
     const int64_t nSph = c.n - c.nW;
     if (s.N != nSph) throw Error(WDG_E_STATE, "the network's graph is not of this context");
-    WDG_HIP(hipStreamSynchronize(s.stream));
+    if (s.fromCtx != &c) throw Error(WDG_E_STATE, "the network's graph was not built from this context");
+    // the forward pass (network stream) ends before the engine's stream moves the spheres
+    WDG_HIP(hipEventRecord(s.done, s.stream));
+    WDG_HIP(hipStreamWaitEvent(c.stream, s.done, 0));
     if (c.forcesPending) flush_forces(c);
     if (nSph > 0) {
       hipLaunchKernelGGL(k_mg_advance, dim3(nblocks(nSph, 256)), dim3(256), 0, c.stream, c.nW, nSph,
-                         (const float*)s.out.p, dT, 9.81, c.pos[0].p, c.pos[1].p, c.pos[2].p, c.vel[0].p, c.vel[1].p,
+                         (const float*)s.out.p, dT, MPNN_A0, c.pos[0].p, c.pos[1].p, c.pos[2].p, c.vel[0].p, c.vel[1].p,
                          c.vel[2].p, (const uint32_t*)c.flags.p);
       WDG_LAUNCH_CHECK();
     }
-    WDG_HIP(hipStreamSynchronize(c.stream));
     c.dirtyKnown = false;
   });
 }

@@ -2,14 +2,20 @@
 MFMA kernels in csrc/wdg_mpnn.hip, driven through the C ABI).
 
 The network maps the current state of a granular packing (contact graph with edges between
-spheres closer than 1.1 (ri + rj), node inputs velocity / radius / overlap with the floor,
-edge inputs relative position / overlap / relative velocity / radii) to each sphere's mean
-acceleration over a coarse step of 10 DEM steps. ``Surrogate.advance`` moves the spheres
-over that step; ``run_with_fallback`` checks the learned steps against DEM re-simulation
-every ``check_every`` coarse steps and falls back to the DEM result when the position error
-exceeds the bound.
+spheres closer than 1.1 (ri + rj); node inputs velocity / radius / overlap with the floor,
+edge inputs relative position / overlap / relative velocity / radii, with the contact normal
+scaled by the elastic and viscous Hertz terms) to each sphere's mean acceleration over a
+coarse step of 10 DEM steps. ``Network.advance`` moves the spheres over that step;
+``run_learned`` checks the learned steps against DEM re-simulation every ``check_every``
+coarse steps and falls back to DEM when the position error exceeds the bound.
 
-Scales: lengths by R0 (2.5 mm), velocities by V0 (1 m/s), accelerations by G (9.81 m/s^2).
+Training data come from the engine itself: ``column_training_set`` runs seeded columns as
+independent scenes of one context, builds the graph of each snapshot on the device and keeps
+it there with its targets, so that training never moves a graph through the host.
+
+Scales: lengths by R0 (2.5 mm), velocities by V0 (1 m/s), accelerations (outputs and targets)
+by A0 = 100 g (981 m/s^2), the order of the contact accelerations, so that the network's
+outputs are of order one.
 """
 import ctypes as C
 import math
@@ -19,10 +25,12 @@ import numpy as np
 from ._native import lib as _lib, dptr, iptr
 
 R0, V0, G = 2.5e-3, 1.0, 9.81
+A0 = 100. * G  # unit of the network's outputs (m/s^2); MPNN_A0 in csrc/wdg_mpnn.hip
 CUTOFF = 1.1
-NODE_IN, EDGE_IN = 5, 9
-# overlaps are fractions 1e-3..1e-2 of the contact distance: scaled so that BF16 resolves them
-OV_SCALE, OV_CLIP = 100., 20.
+NODE_IN, EDGE_IN = 7, 15
+# overlaps are fractions 1e-3..1e-2 of the contact distance: scaled so that BF16 resolves them,
+# and clipped as inputs of their own (the force inputs use them unclipped)
+OV_SCALE, OV_CLIP = 100., 2.
 
 
 def _L():
@@ -44,6 +52,11 @@ def _L():
         L.wdg_mpnn_graph_from_ctx.argtypes = [V, V, C.c_double, C.c_double, C.c_double, C.c_int, P(C.c_int64),
                                               P(C.c_int64), P(C.c_int)]
         L.wdg_mpnn_advance.argtypes = [V, V, C.c_double]
+        L.wdg_mpnn_set_targets.argtypes = [V, P(C.c_float)]
+        L.wdg_mpnn_keep_graph.argtypes = [V, C.c_int32]
+        L.wdg_mpnn_use_graph.argtypes = [V, C.c_int32]
+        L.wdg_mpnn_get_graph.argtypes = [V, P(C.c_int64), P(C.c_int64), P(C.c_int32), P(C.c_int32), P(C.c_int32),
+                                         P(C.c_float), P(C.c_float), P(C.c_int64)]
         L._mpnn_ready = True
     return L
 
@@ -59,7 +72,7 @@ def _fptr(a):
 
 class Graph:
     """contact graph in CSR by receiver: rowOff[N+1], send[E], rev[E] (opposite edge),
-    node inputs x[N,5], edge inputs ef[E,9] (float32)"""
+    node inputs x[N,7], edge inputs ef[E,15] (float32)"""
 
     def __init__(self, rowOff, send, rev, x, ef):
         self.rowOff = np.ascontiguousarray(rowOff, dtype=np.int32)
@@ -105,19 +118,46 @@ def contact_graph(pos, vel, radius, floor_z=0.):
     rkey = send.astype(np.int64) * n + recv
     rev = np.searchsorted(key, rkey)
     x = node_inputs(vel, radius, pos[:, 2] - floor_z)
-    rs = radius[recv] + radius[send]
-    dx = (pos[send] - pos[recv]) / rs[:, None]
-    dist = np.sqrt((dx[:, 0] * dx[:, 0] + dx[:, 1] * dx[:, 1]) + dx[:, 2] * dx[:, 2])
-    ef = np.concatenate([dx, ((1. - dist) * OV_SCALE)[:, None], (vel[send] - vel[recv]) / V0,
-                         (radius[recv] / R0)[:, None], (radius[send] / R0)[:, None]], axis=1)
+    vel = np.asarray(vel, dtype=np.float64)
+    ef = edge_inputs(pos[recv], pos[send], vel[recv], vel[send], radius[recv], radius[send])
     return Graph(rowOff, send, rev, x, ef)
 
 
+def edge_inputs(pi, pj, vi, vj, ri, rj):
+    """inputs of edges with receivers i and senders j: the expressions of edge_inputs in
+    csrc/wdg_mpnn.hip, in the same order (so that host and device graphs agree)"""
+    rs = ri + rj
+    dx, dy, dz = (pj[:, 0] - pi[:, 0]) / rs, (pj[:, 1] - pi[:, 1]) / rs, (pj[:, 2] - pi[:, 2]) / rs
+    dn = np.sqrt((dx * dx + dy * dy) + dz * dz)
+    ov = (1. - dn) * OV_SCALE
+    wx, wy, wz = (vj[:, 0] - vi[:, 0]) / V0, (vj[:, 1] - vi[:, 1]) / V0, (vj[:, 2] - vi[:, 2]) / V0
+    a, b = ri / R0, rj / R0
+    pos_dn = dn > 0
+    dsafe = np.where(pos_dn, dn, 1.)
+    nx, ny, nz = (np.where(pos_dn, d / dsafe, 0.) for d in (dx, dy, dz))
+    ovp = np.where(ov > 0, ov, 0.)
+    a3, b3 = a * a * a, b * b * b
+    hz = ovp * np.sqrt(ovp) * (np.sqrt(a * b) * (a + b) / a3)
+    mb, Rb = a3 * b3 / (a3 + b3), a * b / (a + b)
+    vn = (wx * nx + wy * ny) + wz * nz
+    dp = vn * np.sqrt(np.sqrt(ovp)) * (np.sqrt(np.sqrt(a + b)) * np.sqrt(mb * np.sqrt(Rb)) / a3)
+    ovc = np.minimum(np.maximum(ov, -OV_CLIP), OV_CLIP)
+    cols = [dx, dy, dz, ovc, wx, wy, wz, a, b, -nx * hz, -ny * hz, -nz * hz, nx * dp, ny * dp, nz * dp]
+    return np.stack(cols, axis=1).astype(np.float32)
+
+
 def node_inputs(vel, radius, height):
-    """velocity / V0, radius / R0, overlap with the floor (1 - height / r) * OV_SCALE clipped"""
-    radius = np.asarray(radius)
-    ov = np.clip((1. - np.asarray(height) / radius) * OV_SCALE, -OV_CLIP, OV_CLIP)
-    return np.concatenate([np.asarray(vel) / V0, (radius / R0)[:, None], ov[:, None]], axis=1)
+    """velocity / V0, radius / R0, overlap with the floor ov = (1 - height / r) * OV_SCALE
+    clipped, ov+^(3/2) / a and v_z ov+^(1/4) / a (a = r / R0): node_inputs in csrc/wdg_mpnn.hip"""
+    radius = np.asarray(radius, dtype=np.float64)
+    vel = np.asarray(vel, dtype=np.float64)
+    a = radius / R0
+    ov = (1. - np.asarray(height) / radius) * OV_SCALE
+    ovp = np.where(ov > 0, ov, 0.)
+    vz = vel[:, 2] / V0
+    cols = [vel[:, 0] / V0, vel[:, 1] / V0, vz, a, np.minimum(np.maximum(ov, -OV_CLIP), OV_CLIP),
+            ovp * np.sqrt(ovp) / a, vz * np.sqrt(np.sqrt(ovp)) / a]
+    return np.stack(cols, axis=1).astype(np.float32)
 
 
 class Network:
@@ -167,11 +207,11 @@ class Network:
         self.N = g.N
         out = np.empty((g.N, 3), dtype=np.float32)
         _chk(self.L.wdg_mpnn_forward(self.h, _fptr(out)))
-        return out.astype(np.float64) * G
+        return out.astype(np.float64) * A0
 
     def loss_grad(self, g, target_acc):
         """loss against target accelerations (m/s^2) and the gradients on the device"""
-        self.set_graph(g, np.asarray(target_acc) / G)
+        self.set_graph(g, np.asarray(target_acc) / A0)
         loss = C.c_double()
         _chk(self.L.wdg_mpnn_grad(self.h, C.byref(loss)))
         return loss.value
@@ -189,7 +229,7 @@ class Network:
         """forward pass on the current graph; accelerations (m/s^2) when out"""
         o = np.empty((self.N, 3), dtype=np.float32) if out else None
         _chk(self.L.wdg_mpnn_forward(self.h, _fptr(o)))
-        return None if o is None else o.astype(np.float64) * G
+        return None if o is None else o.astype(np.float64) * A0
 
     def advance(self, dev, dT):
         """the context's spheres over the coarse step dT with the last forward pass"""
@@ -197,6 +237,56 @@ class Network:
 
     def adam(self, lr=1e-3, b1=.9, b2=.999, eps=1e-8):
         _chk(self.L.wdg_mpnn_adam(self.h, lr, b1, b2, eps))
+
+    def set_targets(self, acc):
+        """targets (m/s^2) of the current graph, in its output order"""
+        t = np.ascontiguousarray(np.asarray(acc) / A0, dtype=np.float32)
+        _chk(self.L.wdg_mpnn_set_targets(self.h, _fptr(t)))
+
+    def keep(self, slot):
+        """keep the current graph and its targets on the device"""
+        _chk(self.L.wdg_mpnn_keep_graph(self.h, slot))
+
+    def use(self, slot):
+        _chk(self.L.wdg_mpnn_use_graph(self.h, slot))
+
+    def graph(self):
+        """the current graph as a host Graph (inputs as stored in BF16) and the output row of
+        each node"""
+        n, e = C.c_int64(), C.c_int64()
+        _chk(self.L.wdg_mpnn_get_graph(self.h, C.byref(n), C.byref(e), None, None, None, None, None, None))
+        N, E = n.value, e.value
+        rowOff, send, rev = np.empty(N + 1, np.int32), np.empty(max(E, 1), np.int32), np.empty(max(E, 1), np.int32)
+        x, ef = np.empty((max(N, 1), NODE_IN), np.float32), np.empty((max(E, 1), EDGE_IN), np.float32)
+        row = np.empty(max(N, 1), np.int64)
+        _chk(self.L.wdg_mpnn_get_graph(self.h, None, None, iptr(rowOff), iptr(send), iptr(rev), _fptr(x), _fptr(ef),
+                                       row.ctypes.data_as(C.POINTER(C.c_int64))))
+        return Graph(rowOff, send[:E], rev[:E], x[:N], ef[:E]), row[:N]
+
+    def kept_loss(self, slot):
+        """loss (mean over nodes of |a - t|^2, in units of A0^2) on a kept graph; also leaves the
+        gradients"""
+        self.use(slot)
+        loss = C.c_double()
+        _chk(self.L.wdg_mpnn_grad(self.h, C.byref(loss)))
+        return loss.value
+
+    def train_kept(self, slots, epochs=10, lr=1e-3, lr_end=None, seed=0, log=None):
+        """Adam over graphs kept on the device, one step per graph, shuffled each epoch; the
+        learning rate falls geometrically from lr to lr_end; returns the mean loss per epoch"""
+        rng = np.random.default_rng(seed)
+        lr_end = lr if lr_end is None else lr_end
+        hist = []
+        for ep in range(epochs):
+            r = lr * (lr_end / lr) ** (ep / max(epochs - 1, 1))
+            tot = 0.
+            for k in rng.permutation(len(slots)):
+                tot += self.kept_loss(slots[k])
+                self.adam(r)
+            hist.append(tot / len(slots))
+            if log:
+                log(f"epoch {ep}: lr {r:.2e} loss {hist[-1]:.5f}")
+        return hist
 
     def train(self, samples, epochs=10, lr=1e-3, seed=0, log=None):
         """samples: list of (Graph, target accelerations); one Adam step per sample, shuffled
@@ -318,6 +408,48 @@ def column_samples(n_columns=500, width=0.03, height=0.06, snapshots=20, every=1
     return samples, rad
 
 
+def column_training_set(net, slot0=0, n_columns=500, width=0.03, height=0.6, snapshots=20, every=1500, seed=100,
+                        device=0, log=None):
+    """Training data kept on the device: n_columns seeded columns (width x width x height,
+    spheres on a jittered lattice, released onto their floors) run side by side as independent
+    scenes of one context. At every ``every`` steps the graph of all columns is built on the
+    device from the engine's slot list, and COARSE steps later the mean acceleration over the
+    coarse step becomes its target; the graph is kept in slot slot0 + k of the network.
+    Returns (slots, ballistic mean squared error per snapshot (in units of A0^2), spheres)."""
+    from ._native import Device
+
+    side = int(math.ceil(math.sqrt(n_columns)))
+    parts = []
+    for c in range(n_columns):
+        off = (c % side) * (width + 0.05), (c // side) * (width + 0.05), 0.
+        p, r = column(width, height, seed=seed + c, offset=off)
+        parts.append(engine_arrays(p, r))
+    arr = {k: np.concatenate([q[k] for q in parts]) for k in parts[0]}
+    arr["scene"] = np.concatenate([np.full(len(q["shape"]), i, np.int32) for i, q in enumerate(parts)])
+    dev = Device(device)
+    dt = dem_dt()
+    dev.set_field(gravity=(0., 0., -G), periodic=False)
+    dev.set_materials([material()])
+    dev.upload_particles(**arr)
+    dev.set_scenes(np.full(n_columns, dt))
+    sph = arr["shape"] == 0
+    slots, ball, step = [], [], 0
+    for k in range(snapshots):
+        step = dem_run(dev, every if k else 1, step, dt)
+        n, e, _ = net.graph_from(dev, floorZ=0., force=True)
+        v0 = dev.get("vel")[sph]
+        step = dem_run(dev, COARSE, step, dt)
+        acc = (dev.get("vel")[sph] - v0) / (COARSE * dt)
+        net.set_targets(acc)
+        net.keep(slot0 + k)
+        slots.append(slot0 + k)
+        ball.append(float(np.mean(np.sum((acc - np.array([0., 0., -G])) ** 2, axis=1))) / A0 ** 2)
+        if log:
+            log(f"snapshot {k}: step {step}, nodes {n}, edges {e}, ballistic rms {math.sqrt(ball[-1]) * A0:.2f} m/s^2")
+    dev.close()
+    return slots, ball, int(np.sum(sph))
+
+
 # ---------------------------------------------------------------- column collapse
 def collapse_arrays(width=0.2706, height=0.6, seed=4):
     """the granular column: spheres (r uniform in [2, 3] mm) on a lattice filling a
@@ -340,45 +472,43 @@ def snapshot(dev):
     return dev.get("pos"), dev.get("vel"), dev.get("angVel"), dev.get("ori")
 
 
-def run_learned(dev, net, arr, coarse_steps, dt, check_every=50, bound=0.2e-3, log=None):
-    """Coarse steps of COARSE DEM steps by the network, on the context dev. Every check_every
-    coarse steps the last coarse step is re-simulated by DEM from its start; when the largest
-    position difference exceeds ``bound`` the segment since the previous check is redone by
-    DEM (fallback). The state after a check is the DEM one. Returns statistics."""
-    skin = 0.  # the graph reuses the slot list while no sphere moved farther than this
-    # the context's Verlet distance covers the edge cutoff with room for this displacement
-    rmin = float(np.min(arr["radius"][arr["shape"] == 0]))
-    skin = abs(VERLET) * rmin - 0.05 * 2 * float(np.max(arr["radius"]))
+def run_learned(dev, net, radius, coarse_steps, dt, check_every=50, bound=0.2e-3, step=0, log=None):
+    """Coarse steps of COARSE DEM steps by the network, on the context dev (its spheres have the
+    given radii). Every check_every coarse steps the last coarse step is redone by DEM from its
+    start; when the largest position difference exceeds ``bound``, the segment since the
+    previous check is redone by DEM (fallback). The state after a check is the DEM one. The
+    states are kept as device snapshots (slots 0-2). Returns statistics."""
+    # room for motion between two slot-list refreshes of the graph builder
+    skin = 0.25 * float(np.min(radius))
     stats = dict(checks=0, fallbacks=0, errors=[], learned_coarse_steps=0, dem_steps=0)
-    step, k = 0, 0
-    sph = arr["shape"] == 0
+    k = 0
     while k < coarse_steps:
         seg = min(check_every, coarse_steps - k)
-        start = snapshot(dev)
-        pre = None
+        dev.snapshot(0)
         for j in range(seg):
             if j == seg - 1:
-                pre = snapshot(dev)
-            net.graph_from(dev, floorZ=0., maxDisp=max(skin, 0.))
+                dev.snapshot(1)
+            net.graph_from(dev, floorZ=0., maxDisp=skin)
             net.forward(out=False)
             net.advance(dev, COARSE * dt)
             stats["learned_coarse_steps"] += 1
-        p_sur = dev.get("pos")[sph]
-        upload(dev, arr, pre)
+        dev.snapshot(2)
+        dev.restore(1)
         step = dem_run(dev, COARSE, step, dt)
         stats["dem_steps"] += COARSE
-        err = float(np.max(np.linalg.norm(dev.get("pos")[sph] - p_sur, axis=1)))
+        err = dev.snapshot_distance(2)
         stats["checks"] += 1
         stats["errors"].append(err)
         if err > bound:
             stats["fallbacks"] += 1
-            upload(dev, arr, start)
+            dev.restore(0)
             step = dem_run(dev, seg * COARSE, step, dt)
             stats["dem_steps"] += seg * COARSE
         if log:
             log(f"coarse {k + seg}/{coarse_steps}: check error {err * 1e3:.4f} mm"
                 f"{' -> DEM fallback' if err > bound else ''}")
         k += seg
+    stats["step"] = step
     return stats
 
 
