@@ -1,6 +1,7 @@
 // extern "C" entry points of the DEM step engine (see include/woo_amd.h).
 // The ABI speaks user ids (Particle.id) and user-ordered arrays; the device keeps
 // particles in storage order (walls, then spheres sorted by cell), mapped by uid/sto.
+#include <atomic>
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -248,6 +249,8 @@ int wdg_ctx_create(int device, wdg_ctx** out) {
   *out = nullptr;
   wdg_ctx* ctx = new wdg_ctx();
   ctx->c.device = device;
+  static std::atomic<uint64_t> serials{0};
+  ctx->c.serial = ++serials;
   int rc = guard(ctx, [&] {
     int ndev = 0;
     WDG_HIP(hipGetDeviceCount(&ndev));

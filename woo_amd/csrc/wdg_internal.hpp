@@ -264,6 +264,7 @@ struct Ctx {
   int64_t snapN[NSNAP] = {-1, -1, -1, -1};
   uint64_t snapUpload[NSNAP] = {0, 0, 0, 0};
   uint64_t uploads = 0;   // particle uploads so far
+  uint64_t serial = 0;    // unique per context of the process
   DBuf<double> red;       // reductions
   DBuf<double> xfer;      // user-id-ordered staging for upload/download
   DBuf<uint32_t> dflags;  // [0] dirty, [1] error bits, [2..] scratch counters

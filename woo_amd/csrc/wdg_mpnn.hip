@@ -160,58 +160,90 @@ __device__ __forceinline__ bf16x8 as_bf8(uint4 v) {
 // ------------------------------------------------------------------ forward GEMM
 // Y = relu(A W + b) over M rows, 64 columns. Out: y (BF16, M x 64). With RESID (update
 // layer): u = relu(.) to y, h_new = round(h + u) to hOut.
+// The workgroup stages the BF16 weight image in LDS once; each wavefront then walks 16-row
+// tiles of A (all K/32 row loads of a tile issued before its MFMAs), and writes each output
+// tile through its own LDS tile as 16-byte row stores.
+template <int MODE>
+constexpr int k_of() { return MODE == A_PLAIN ? KX : (MODE == A_EDGE ? KM : KU); }
+// LDS ordering among the lanes of one wavefront (the tile is private to the wavefront)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 template <int MODE, bool RESID>
 __global__ __launch_bounds__(256) void k_mm_fwd(ARows a, int64_t M, const bf16* __restrict__ Wt,
                                                  const float* __restrict__ bias, bf16* y, bf16* hOut) {
+  constexpr int K = k_of<MODE>(), KS = K / 32, WP = K + 8;  // padded LDS rows (bank spread)
+  __shared__ bf16 Ws[H * WP];
+  __shared__ bf16 tile[4][16][H + 8];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  // every wavefront reaches the barrier of the epilogue; rows at or past M are neither read
-  // nor written
-  const int64_t r0 = ((int64_t)blockIdx.x * 4 + wv) * 16;
-  const int64_t r = r0 + (lane & 15);
-  const bool live = r < M;
+  // weight image [64 x K] into LDS, 16 bytes per thread and step
+  for (int i = threadIdx.x; i < H * K / 8; i += blockDim.x) {
+    const int col = i / (K / 8), k8 = (i % (K / 8)) * 8;
+    *reinterpret_cast<uint4*>(&Ws[col * WP + k8]) = *reinterpret_cast<const uint4*>(Wt + (int64_t)col * K + k8);
+  }
+  __syncthreads();
   const int kq = 8 * (lane >> 4);
-  floatx4 acc[4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
-  for (int k0 = 0; k0 < a.K; k0 += 32) {
-    const uint4 av = live ? load8<MODE>(a, r, k0 + kq) : make_uint4(0, 0, 0, 0);
-    const bf16x8 A = as_bf8(av);
+  float bv[4];
+#pragma unroll
+  for (int cb = 0; cb < 4; cb++) bv[cb] = bias[cb * 16 + (lane & 15)];
+  const int64_t ntiles = (M + 15) / 16;
+  for (int64_t t = (int64_t)blockIdx.x * 4 + wv; t < ntiles; t += (int64_t)gridDim.x * 4) {
+    const int64_t r0 = t * 16;
+    const int64_t r = r0 + (lane & 15);
+    const bool live = r < M;
+    uint4 av[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) av[ks] = live ? load8<MODE>(a, r, ks * 32 + kq) : make_uint4(0, 0, 0, 0);
+    floatx4 acc[4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+      const bf16x8 A = as_bf8(av[ks]);
+#pragma unroll
+      for (int cb = 0; cb < 4; cb++) {
+        const int col = cb * 16 + (lane & 15);
+        const bf16x8 B = as_bf8(*reinterpret_cast<const uint4*>(&Ws[col * WP + ks * 32 + kq]));
+        acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[cb], 0, 0, 0);
+      }
+    }
+    // C layout: row 4*(lane/16)+q, column lane%16 of each block
 #pragma unroll
     for (int cb = 0; cb < 4; cb++) {
       const int col = cb * 16 + (lane & 15);
-      const bf16x8 B = as_bf8(*reinterpret_cast<const uint4*>(Wt + (int64_t)col * a.K + k0 + kq));
-      acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[cb], 0, 0, 0);
+#pragma unroll
+      for (int q = 0; q < 4; q++) tile[wv][4 * (lane >> 4) + q][col] = f2bf(fmaxf(acc[cb][q] + bv[cb], 0.f));
     }
-  }
-  // C layout: row 4*(lane/16)+q, column lane%16 of each block. The wavefront's 16 x 64 tile
-  // goes through LDS so that rows leave as 16-byte stores.
-  __shared__ bf16 tile[4][16][H + 8];
+    wave_lds_sync();
 #pragma unroll
-  for (int cb = 0; cb < 4; cb++) {
-    const int col = cb * 16 + (lane & 15);
-    const float b = bias[col];
+    for (int half = 0; half < 2; half++) {
+      const int row = half * 8 + (lane >> 3), c0 = (lane & 7) * 8;
+      const int64_t rr = r0 + row;
+      if (rr >= M) continue;
+      const uint4 v = *reinterpret_cast<const uint4*>(&tile[wv][row][c0]);
+      *reinterpret_cast<uint4*>(y + rr * H + c0) = v;
+      if (RESID) {
+        const uint4 hv = *reinterpret_cast<const uint4*>(a.h + rr * H + c0);
+        const uint32_t uw[4] = {v.x, v.y, v.z, v.w}, hw[4] = {hv.x, hv.y, hv.z, hv.w};
+        uint32_t o[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) tile[wv][4 * (lane >> 4) + q][col] = f2bf(fmaxf(acc[cb][q] + b, 0.f));
-  }
-  __syncthreads();
-#pragma unroll
-  for (int half = 0; half < 2; half++) {
-    const int row = half * 8 + (lane >> 3), c0 = (lane & 7) * 8;
-    const int64_t rr = r0 + row;
-    if (rr >= M) continue;
-    const uint4 v = *reinterpret_cast<const uint4*>(&tile[wv][row][c0]);
-    *reinterpret_cast<uint4*>(y + rr * H + c0) = v;
-    if (RESID) {
-      const uint4 hv = *reinterpret_cast<const uint4*>(a.h + rr * H + c0);
-      const uint32_t uw[4] = {v.x, v.y, v.z, v.w}, hw[4] = {hv.x, hv.y, hv.z, hv.w};
-      uint32_t o[4];
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const bf16 lo = f2bf(bf2f((bf16)(hw[q] & 0xFFFFu)) + bf2f((bf16)(uw[q] & 0xFFFFu)));
-        const bf16 hi = f2bf(bf2f((bf16)(hw[q] >> 16)) + bf2f((bf16)(uw[q] >> 16)));
-        o[q] = (uint32_t)lo | ((uint32_t)hi << 16);
+        for (int q = 0; q < 4; q++) {
+          const bf16 lo = f2bf(bf2f((bf16)(hw[q] & 0xFFFFu)) + bf2f((bf16)(uw[q] & 0xFFFFu)));
+          const bf16 hi = f2bf(bf2f((bf16)(hw[q] >> 16)) + bf2f((bf16)(uw[q] >> 16)));
+          o[q] = (uint32_t)lo | ((uint32_t)hi << 16);
+        }
+        *reinterpret_cast<uint4*>(hOut + rr * H + c0) = make_uint4(o[0], o[1], o[2], o[3]);
       }
-      *reinterpret_cast<uint4*>(hOut + rr * H + c0) = make_uint4(o[0], o[1], o[2], o[3]);
     }
+    // the tile is written again by the next iteration only after every lane has read it
+    wave_lds_sync();
   }
+}
+// workgroups for a forward GEMM over M rows: enough wavefronts to fill the chip, each walking a
+// few tiles so that the weight staging is amortised
+static unsigned fwd_blocks(int64_t M) {
+  const int64_t tiles = (M + 15) / 16;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 1024));
 }
 
 // ------------------------------------------------------------------ input-gradient GEMM
@@ -393,15 +425,29 @@ __global__ void k_relu_bwd(int64_t n, const float* __restrict__ g, const bf16* _
 }
 
 // ------------------------------------------------------------------ decoder and loss
-__global__ void k_dec(int64_t N, const bf16* __restrict__ h, const float* __restrict__ W, const float* __restrict__ b,
-                      float* out) {
+// a_i = h_i W_dec + b_dec (FP32), summed over k in order; one thread per node, the row read
+// as eight 16-byte loads, the weights from LDS
+__global__ __launch_bounds__(256) void k_dec(int64_t N, const bf16* __restrict__ h, const float* __restrict__ W,
+                                              const float* __restrict__ b, float* out) {
+  __shared__ float Ws[H * NOUT];
+  for (int q = threadIdx.x; q < H * NOUT; q += blockDim.x) Ws[q] = W[q];
+  __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  float o[NOUT] = {b[0], b[1], b[2]};
-  for (int k = 0; k < H; k++) {
-    const float x = bf2f(h[i * H + k]);
+  uint4 hv[H / 8];
 #pragma unroll
-    for (int c = 0; c < NOUT; c++) o[c] += x * W[k * NOUT + c];
+  for (int c8 = 0; c8 < H / 8; c8++) hv[c8] = *reinterpret_cast<const uint4*>(h + i * H + c8 * 8);
+  float o[NOUT] = {b[0], b[1], b[2]};
+#pragma unroll
+  for (int c8 = 0; c8 < H / 8; c8++) {
+    const uint32_t w[4] = {hv[c8].x, hv[c8].y, hv[c8].z, hv[c8].w};
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int k = c8 * 8 + q;
+      const float x = bf2f((bf16)((q & 1) ? (w[q >> 1] >> 16) : (w[q >> 1] & 0xFFFFu)));
+#pragma unroll
+      for (int c = 0; c < NOUT; c++) o[c] += x * Ws[k * NOUT + c];
+    }
   }
 #pragma unroll
   for (int c = 0; c < NOUT; c++) out[i * NOUT + c] = o[c];
@@ -495,6 +541,9 @@ struct Mpnn {
   Ctx* fromCtx = nullptr;  // the engine context of the current graph (storage order of its nodes)
   DBuf<uint32_t> degScan;
   DBuf<uint32_t> nodeUid;  // user id of each node of a graph from an engine context
+  uint64_t uidCtx = 0;  // the context (serial), refresh and upload count nodeUid was taken at
+  int64_t uidRefresh = -1, uidN = -1;
+  uint64_t uidUploads = 0;
   // activations
   DBuf<bf16> h[3], m[2], agg[2], u[2];
   DBuf<float> out, dA, dhA, dhB, dU, dZ, part, partB, partLoss;
@@ -575,7 +624,7 @@ static void forward(Mpnn& s) {
   }
   s.out.ensure((size_t)std::max<int64_t>(N, 1) * NOUT);
   if (N == 0) return;
-  const unsigned gN = (unsigned)((N + 63) / 64), gE = (unsigned)((E + 63) / 64);
+  const unsigned gN = fwd_blocks(N), gE = fwd_blocks(E);
   ARows a{};
   a.x = s.X.p;
   a.K = KX;
@@ -728,6 +777,7 @@ __global__ void k_mg_count(GraphArgs g) {
   for (uint32_t k = g.adjOff[s]; k < g.adjOff[s + 1]; k++) c += edge_keep(g, s, g.adj[k] & ~ADJ_PA, other) ? 1 : 0;
   g.deg[t] = c;
 }
+// the structure of the graph: each sphere's kept edges in its adjacency order
 __global__ void k_mg_fill(GraphArgs g) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.nSph) return;
@@ -742,23 +792,29 @@ __global__ void k_mg_fill(GraphArgs g) {
     const int side = g.A[slot] == s ? 0 : 1;
     g.slotSide[e] = (int32_t)(2 * slot + side);
     g.edgeOfSlot[2 * slot + side] = e;
-    float f[KE];
-    for (int q = NEI; q < KE; q++) f[q] = 0.f;
-    edge_inputs(g.px[s], g.py[s], g.pz[s], g.px[other], g.py[other], g.pz[other], g.vx[s], g.vy[s], g.vz[s],
-                g.vx[other], g.vy[other], g.vz[other], g.rad[s], g.rad[other], f);
-    store_row<KE>(g.EF + (int64_t)e * KE, f);
     e++;
   }
-  float x[KX];
-  for (int q = NIN; q < KX; q++) x[q] = 0.f;
-  node_inputs(g.vx[s], g.vy[s], g.vz[s], g.rad[s], g.pz[s] - g.floorZ, x);
-  store_row<KX>(g.X + t * KX, x);
 }
-__global__ void k_mg_rev(int64_t E, const int32_t* __restrict__ slotSide, const int32_t* __restrict__ edgeOfSlot,
-                         int32_t* rev) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E) return;
-  rev[e] = edgeOfSlot[slotSide[e] ^ 1];
+// inputs of every edge (threads [0, E)) and node (threads [E, E + nSph)), and the opposite
+// edge of each edge; one thread per row, so that the double-precision input arithmetic runs
+// on full wavefronts
+__global__ __launch_bounds__(256) void k_mg_inputs(GraphArgs g, int64_t E) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < E) {
+    const int64_t i = g.nW + g.recv[q], j = g.nW + g.send[q];
+    g.rev[q] = g.edgeOfSlot[g.slotSide[q] ^ 1];
+    float f[KE];
+    for (int c = NEI; c < KE; c++) f[c] = 0.f;
+    edge_inputs(g.px[i], g.py[i], g.pz[i], g.px[j], g.py[j], g.pz[j], g.vx[i], g.vy[i], g.vz[i], g.vx[j], g.vy[j],
+                g.vz[j], g.rad[i], g.rad[j], f);
+    store_row<KE>(g.EF + q * KE, f);
+  } else if (q < E + g.nSph) {
+    const int64_t t = q - E, s = g.nW + t;
+    float x[KX];
+    for (int c = NIN; c < KX; c++) x[c] = 0.f;
+    node_inputs(g.vx[s], g.vy[s], g.vz[s], g.rad[s], g.pz[s] - g.floorZ, x);
+    store_row<KX>(g.X + t * KX, x);
+  }
 }
 // coarse step: v += a dT, x += v dT for the spheres (a in units of the given g)
 __global__ void k_mg_advance(int64_t nW, int64_t nSph, const float* __restrict__ a, double dT, double g, double* px,
@@ -1181,15 +1237,19 @@ int wdg_mpnn_graph_from_ctx(wdg_mpnn* m, wdg_ctx* ctx, double cutoff, double flo
     if (nSph > 0) {
       hipLaunchKernelGGL(k_mg_fill, dim3(nblocks(nSph, 256)), dim3(256), 0, c.stream, g);
       WDG_LAUNCH_CHECK();
-    }
-    if (E > 0) {
-      hipLaunchKernelGGL(k_mg_rev, dim3(nblocks(E, 256)), dim3(256), 0, c.stream, E, (const int32_t*)s.slotSide.p,
-                         (const int32_t*)s.edgeOfSlot.p, s.rev.p);
+      hipLaunchKernelGGL(k_mg_inputs, dim3(nblocks(E + nSph, 256)), dim3(256), 0, c.stream, g, E);
       WDG_LAUNCH_CHECK();
     }
-    s.nodeUid.ensure((size_t)std::max<int64_t>(nSph, 1));
-    if (nSph > 0)
-      WDG_HIP(hipMemcpyAsync(s.nodeUid.p, c.uid.p + c.nW, nSph * 4, hipMemcpyDeviceToDevice, c.stream));
+    // the user ids of the nodes change only with a refresh (or an upload) of the context
+    if (s.uidCtx != c.serial || s.uidRefresh != c.nRefresh || s.uidUploads != c.uploads || s.uidN != nSph) {
+      s.nodeUid.ensure((size_t)std::max<int64_t>(nSph, 1));
+      if (nSph > 0)
+        WDG_HIP(hipMemcpyAsync(s.nodeUid.p, c.uid.p + c.nW, nSph * 4, hipMemcpyDeviceToDevice, c.stream));
+      s.uidCtx = c.serial;
+      s.uidRefresh = c.nRefresh;
+      s.uidUploads = c.uploads;
+      s.uidN = nSph;
+    }
     // the network's next kernels (its own stream) start after the graph is built
     WDG_HIP(hipEventRecord(s.done, c.stream));
     WDG_HIP(hipStreamWaitEvent(s.stream, s.done, 0));
