@@ -484,15 +484,11 @@ def surrogate_form(args):
 
     net = S.Network(seed=1)
     t0 = time.perf_counter()
-    slots, ball, n_train = S.column_training_set(net, 0, n_columns=args.sur_columns, snapshots=20, every=1500,
-                                                 seed=100)
-    held, ball_h, _ = S.column_training_set(net, 1000, n_columns=32, snapshots=20, every=1500, seed=5000)
-    t_data = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    hist = net.train_kept(slots, epochs=args.sur_epochs, lr=1e-3, lr_end=1e-4)
+    hist, ball, n_train, n_graphs = S.train_on_columns(net, n_columns=args.sur_columns, epochs=args.sur_epochs,
+                                                       rounds=args.sur_rounds)
     t_train = time.perf_counter() - t0
-    err_trained = math.sqrt(float(np.mean([net.kept_loss(k) for k in held]))) * S.A0
-    err_ballistic = math.sqrt(float(np.mean(ball_h))) * S.A0
+    held, _, _ = S.column_training_set(net, 10000, n_columns=32, snapshots=20, every=1500, seed=5000)
+    err_trained, err_ballistic = S.held_out_rmse(net, held)
     dev = Device(0)
     arr, rad = S.collapse_arrays()
     dt = S.dem_dt()
@@ -534,10 +530,10 @@ def surrogate_form(args):
                 profile=dict(rms_diff_m=float(np.sqrt(np.mean((pr_sur - pr_dem) ** 2))),
                              max_rel_err=float(np.max(np.abs(pr_sur - pr_dem)[occupied] / pr_dem[occupied]))),
                 position_diff_max_mm=1e3 * float(np.max(np.linalg.norm(p_sur - p_dem, axis=1))),
-                training=dict(columns=args.sur_columns, spheres=n_train, snapshots=len(slots), epochs=args.sur_epochs,
-                              loss_first=hist[0], loss_last=hist[-1], data_s=t_data, train_s=t_train,
-                              heldout_rmse_trained=err_trained, heldout_rmse_ballistic=err_ballistic,
-                              unit="m/s^2"))
+                training=dict(columns=args.sur_columns, spheres=n_train, graphs=n_graphs, epochs=args.sur_epochs,
+                              rollout_rounds=args.sur_rounds, loss_first=hist[0], loss_last=hist[-1],
+                              data_and_train_s=t_train, heldout_rmse_learned=err_trained,
+                              heldout_rmse_ballistic=err_ballistic, unit="m/s^2"))
 
 
 def pmc_probe(dev, wl, args):
@@ -601,6 +597,7 @@ def main():
     ap.add_argument("--surrogate", type=int, default=1, help="also run the learned coarse step (column collapse)")
     ap.add_argument("--sur-columns", type=int, default=500)
     ap.add_argument("--sur-epochs", type=int, default=30)
+    ap.add_argument("--sur-rounds", type=int, default=2, help="rounds of training on the network's own rollouts")
     ap.add_argument("--sur-steps", type=int, default=20000, help="DEM steps of simulated time")
     ap.add_argument("--pmc-probe", type=int, default=0,
                     help="for rocprofv3 --pmc runs: warm up, launch the contact loop and leapfrog this many "

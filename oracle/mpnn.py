@@ -87,10 +87,18 @@ class Net:
         c["a"] = a
         return a, c
 
+    def contact(self, c):
+        """nodes in contact: a positive floor overlap input (column 4) or a positive overlap input
+        (column 3) on one of their edges; the others move in free flight and carry no loss"""
+        flag = c["X"][:, 4] > 0
+        hit = np.zeros(c["N"], dtype=bool)
+        np.logical_or.at(hit, c["recv"], c["F"][:, 3] > 0)
+        return flag | hit
+
     def loss_grad(self, x, rowOff, send, rev, ef, t):
         a, c = self.forward(x, rowOff, send, ef)
         N, recv = c["N"], c["recv"]
-        d = a - t
+        d = (a - t) * self.contact(c)[:, None]
         loss = float(np.sum(d * d) / N)
         dA = 2 * d / N
         g = np.zeros(self.n)

@@ -114,21 +114,26 @@ def test_training_gains_over_untrained_and_ballistic():
         e = [np.mean(np.sum((pred_fn(g) - a) ** 2, axis=1)) for g, a, _, _ in test]
         return float(np.sqrt(np.mean(e)))
 
-    before = rmse(net.predict)
+    # the learned step: the network's accelerations for spheres in contact, free flight for the
+    # others (the untrained network is the ballistic step)
+    before = rmse(net.step_acc)
     ballistic = rmse(lambda g: np.tile([0., 0., -9.81], (g.N, 1)))
     hist = net.train([(g, a) for g, a, _, _ in train], epochs=100, lr=1e-3)
-    after = rmse(net.predict)
+    after = rmse(net.step_acc)
     print("rmse untrained", before, "ballistic", ballistic, "trained", after, "loss", hist[0], hist[-1])
     # held-out columns: the trained network predicts the coarse-step accelerations better than
     # free fall (the coarse step without it) and than the untrained network
     assert hist[-1] < 0.8 * hist[0]
-    assert after < 0.9 * ballistic and after < 0.85 * before
+    assert abs(before - ballistic) < 1e-3 * ballistic
+    assert after < 0.85 * ballistic
 
 
 def test_kept_graphs_train_like_host_graphs():
     # a graph built on the device and kept there gives the same loss as the current graph, and
     # training over kept graphs lowers the loss
-    from woo_amd.surrogate import Network, column_training_set
+    import math
+
+    from woo_amd.surrogate import A0, Network, column_training_set, held_out_rmse
 
     net = Network(seed=5)
     slots, ball, n = column_training_set(net, 0, n_columns=16, width=0.03, height=0.1, snapshots=4, every=1200,
@@ -138,5 +143,8 @@ def test_kept_graphs_train_like_host_graphs():
     assert l0 == [net.kept_loss(k) for k in slots]  # deterministic
     hist = net.train_kept(slots, epochs=60, lr=1e-3, lr_end=1e-4)
     l1 = [net.kept_loss(k) for k in slots]
-    print("loss before", np.mean(l0), "after", np.mean(l1), "ballistic", np.mean(ball))
-    assert np.mean(l1) < 0.5 * np.mean(l0) and np.mean(l1) < np.mean(ball)
+    learned, ballistic = held_out_rmse(net, slots)
+    print("loss before", np.mean(l0), "after", np.mean(l1), "rmse learned", learned, "ballistic", ballistic)
+    # the untrained network is the ballistic step; training lowers the error of the spheres in contact
+    assert abs(math.sqrt(np.mean(l0)) * A0 - ballistic) < 1e-3 * ballistic
+    assert np.mean(l1) < 0.7 * np.mean(l0)

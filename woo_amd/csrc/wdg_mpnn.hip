@@ -453,14 +453,26 @@ __global__ __launch_bounds__(256) void k_dec(int64_t N, const bf16* __restrict__
   for (int c = 0; c < NOUT; c++) out[i * NOUT + c] = o[c];
 }
 // loss = mean over nodes of |a - t|^2; dA = 2 (a - t) / N; per-block partial loss
+// A node is in contact when its floor overlap input or the overlap input of one of its edges
+// is positive. Nodes not in contact move in free flight (the network's output is not used for
+// them), so they carry no loss.
+__global__ __launch_bounds__(256) void k_contact(int64_t N, const int32_t* __restrict__ rowOff,
+                                                  const bf16* __restrict__ X, const bf16* __restrict__ EF,
+                                                  uint8_t* flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  bool c = bf2f(X[i * KX + 4]) > 0.f;
+  for (int32_t e = rowOff[i]; e < rowOff[i + 1] && !c; e++) c = bf2f(EF[(int64_t)e * KE + 3]) > 0.f;
+  flag[i] = c ? 1 : 0;
+}
 __global__ __launch_bounds__(256) void k_loss(int64_t N, const float* __restrict__ a, const float* __restrict__ t,
-                                               float* dA, float* partLoss) {
+                                               const uint8_t* __restrict__ flag, float* dA, float* partLoss) {
   __shared__ float sm[256];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   float l = 0.f;
   if (i < N)
     for (int c = 0; c < NOUT; c++) {
-      const float d = a[i * NOUT + c] - t[i * NOUT + c];
+      const float d = flag[i] ? a[i * NOUT + c] - t[i * NOUT + c] : 0.f;
       l += d * d;
       dA[i * NOUT + c] = 2.f * d / (float)N;
     }
@@ -548,6 +560,7 @@ struct Mpnn {
   DBuf<bf16> h[3], m[2], agg[2], u[2];
   DBuf<float> out, dA, dhA, dhB, dU, dZ, part, partB, partLoss;
   DBuf<bf16> dY, dYe;
+  DBuf<uint8_t> cflag;  // node in contact (else in free flight)
   hipEvent_t done = nullptr;  // orders the network's stream and an engine context's stream
   // graphs kept on the device (training sets): inputs, structure and targets
   struct Kept {
@@ -623,7 +636,11 @@ static void forward(Mpnn& s) {
     s.u[l].ensure((size_t)std::max<int64_t>(N, 1) * H);
   }
   s.out.ensure((size_t)std::max<int64_t>(N, 1) * NOUT);
+  s.cflag.ensure((size_t)std::max<int64_t>(N, 1));
   if (N == 0) return;
+  hipLaunchKernelGGL(k_contact, dim3(nblocks(N, 256)), dim3(256), 0, s.stream, N, (const int32_t*)s.rowOff.p,
+                     (const bf16*)s.X.p, (const bf16*)s.EF.p, s.cflag.p);
+  WDG_LAUNCH_CHECK();
   const unsigned gN = fwd_blocks(N), gE = fwd_blocks(E);
   ARows a{};
   a.x = s.X.p;
@@ -677,7 +694,7 @@ static double backward(Mpnn& s) {
   }
   WDG_HIP(hipMemsetAsync(s.G.p, 0, s.L.total * sizeof(float), s.stream));
   hipLaunchKernelGGL(k_loss, dim3((unsigned)nbN), dim3(256), 0, s.stream, N, (const float*)s.out.p,
-                     (const float*)s.T.p, s.dA.p, s.partLoss.p);
+                     (const float*)s.T.p, (const uint8_t*)s.cflag.p, s.dA.p, s.partLoss.p);
   WDG_LAUNCH_CHECK();
   // decoder
   hipLaunchKernelGGL(k_dec_bwd, dim3((unsigned)nbN), dim3(256), 0, s.stream, N, (const bf16*)s.h[2].p,
@@ -816,14 +833,18 @@ __global__ __launch_bounds__(256) void k_mg_inputs(GraphArgs g, int64_t E) {
     store_row<KX>(g.X + t * KX, x);
   }
 }
-// coarse step: v += a dT, x += v dT for the spheres (a in units of the given g)
-__global__ void k_mg_advance(int64_t nW, int64_t nSph, const float* __restrict__ a, double dT, double g, double* px,
-                             double* py, double* pz, double* vx, double* vy, double* vz, const uint32_t* flags) {
+// coarse step: v += a dT, x += v dT for the spheres; a is the network's output (in units of
+// unit) for a sphere in contact, the field's gravity for one in free flight
+__global__ void k_mg_advance(int64_t nW, int64_t nSph, const float* __restrict__ a, const uint8_t* __restrict__ inContact,
+                             double dT, double unit, double gx, double gy, double gz, double* px, double* py,
+                             double* pz, double* vx, double* vy, double* vz, const uint32_t* flags) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nSph) return;
   const int64_t s = nW + t;
   if (!(flags[s] & F_INTEGRATED)) return;
-  const double ax = g * a[t * NOUT], ay = g * a[t * NOUT + 1], az = g * a[t * NOUT + 2];
+  const bool c = inContact[t] != 0;
+  const double ax = c ? unit * a[t * NOUT] : gx, ay = c ? unit * a[t * NOUT + 1] : gy,
+               az = c ? unit * a[t * NOUT + 2] : gz;
   const double v[3] = {vx[s] + ax * dT, vy[s] + ay * dT, vz[s] + az * dT};
   vx[s] = v[0]; vy[s] = v[1]; vz[s] = v[2];
   px[s] = px[s] + v[0] * dT; py[s] = py[s] + v[1] * dT; pz[s] = pz[s] + v[2] * dT;
@@ -943,6 +964,7 @@ int wdg_mpnn_destroy(wdg_mpnn* m) {
   DBuf<bf16>* bb[] = {&s.img, &s.X, &s.EF, &s.h[0], &s.h[1], &s.h[2], &s.m[0], &s.m[1], &s.agg[0], &s.agg[1],
                       &s.u[0], &s.u[1], &s.dY, &s.dYe};
   for (auto* b : bb) b->release();
+  s.cflag.release();
   s.rowOff.release(); s.recv.release(); s.send.release(); s.rev.release();
   s.deg.release(); s.slotSide.release(); s.edgeOfSlot.release(); s.degScan.release(); s.nodeUid.release();
   for (auto& k : s.kept) k.release();
@@ -1277,7 +1299,8 @@ int wdg_mpnn_advance(wdg_mpnn* m, wdg_ctx* ctx, double dT) {
     if (c.forcesPending) flush_forces(c);
     if (nSph > 0) {
       hipLaunchKernelGGL(k_mg_advance, dim3(nblocks(nSph, 256)), dim3(256), 0, c.stream, c.nW, nSph,
-                         (const float*)s.out.p, dT, MPNN_A0, c.pos[0].p, c.pos[1].p, c.pos[2].p, c.vel[0].p, c.vel[1].p,
+                         (const float*)s.out.p, (const uint8_t*)s.cflag.p, dT, MPNN_A0, c.field.gravity[0],
+                         c.field.gravity[1], c.field.gravity[2], c.pos[0].p, c.pos[1].p, c.pos[2].p, c.vel[0].p, c.vel[1].p,
                          c.vel[2].p, (const uint32_t*)c.flags.p);
       WDG_LAUNCH_CHECK();
     }

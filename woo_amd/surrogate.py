@@ -160,6 +160,15 @@ def node_inputs(vel, radius, height):
     return np.stack(cols, axis=1).astype(np.float32)
 
 
+def contact_mask(g):
+    """spheres in contact (a positive floor overlap input or a positive overlap input on one of
+    their edges); the learned step moves the others in free flight"""
+    recv = np.repeat(np.arange(g.N), np.diff(g.rowOff))
+    hit = np.zeros(g.N, dtype=bool)
+    np.logical_or.at(hit, recv, g.ef[:, 3] > 0)
+    return (g.x[:, 4] > 0) | hit
+
+
 class Network:
     """the device network: parameters, Adam state, the current graph"""
 
@@ -208,6 +217,13 @@ class Network:
         out = np.empty((g.N, 3), dtype=np.float32)
         _chk(self.L.wdg_mpnn_forward(self.h, _fptr(out)))
         return out.astype(np.float64) * A0
+
+    def step_acc(self, g, gravity=(0., 0., -G)):
+        """the accelerations the learned step applies: the network's for spheres in contact,
+        gravity for the others"""
+        a = self.predict(g)
+        a[~contact_mask(g)] = gravity
+        return a
 
     def loss_grad(self, g, target_acc):
         """loss against target accelerations (m/s^2) and the gradients on the device"""
@@ -409,12 +425,18 @@ def column_samples(n_columns=500, width=0.03, height=0.06, snapshots=20, every=1
 
 
 def column_training_set(net, slot0=0, n_columns=500, width=0.03, height=0.6, snapshots=20, every=1500, seed=100,
-                        device=0, log=None):
+                        rollout=0, device=0, log=None):
     """Training data kept on the device: n_columns seeded columns (width x width x height,
     spheres on a jittered lattice, released onto their floors) run side by side as independent
     scenes of one context. At every ``every`` steps the graph of all columns is built on the
     device from the engine's slot list, and COARSE steps later the mean acceleration over the
     coarse step becomes its target; the graph is kept in slot slot0 + k of the network.
+
+    With rollout > 0 the states come from the network itself: from each DEM snapshot the
+    network advances the columns by 1..rollout coarse steps (varying with k), and the DEM
+    target is computed from where it arrived (contact history restarted), so that training sees
+    the drift of its own coarse steps and learns to correct it; the DEM trajectory then resumes
+    from its snapshot.
     Returns (slots, ballistic mean squared error per snapshot (in units of A0^2), spheres)."""
     from ._native import Device
 
@@ -434,12 +456,26 @@ def column_training_set(net, slot0=0, n_columns=500, width=0.03, height=0.6, sna
     dev.set_scenes(np.full(n_columns, dt))
     sph = arr["shape"] == 0
     slots, ball, step = [], [], 0
+    skin = 0.25 * float(np.min(arr["radius"][sph]))
     for k in range(snapshots):
         step = dem_run(dev, every if k else 1, step, dt)
+        if rollout:
+            dev.snapshot(0)
+            for j in range(1 + (7 * k) % rollout):
+                net.graph_from(dev, floorZ=0., maxDisp=skin)
+                net.forward(out=False)
+                net.advance(dev, COARSE * dt)
+            dev.snapshot(1)
+            dev.restore(1)
+            dev.collide(VERLET)
         n, e, _ = net.graph_from(dev, floorZ=0., force=True)
         v0 = dev.get("vel")[sph]
-        step = dem_run(dev, COARSE, step, dt)
+        dem_run(dev, COARSE, step, dt)
         acc = (dev.get("vel")[sph] - v0) / (COARSE * dt)
+        if rollout:
+            dev.restore(0)
+        else:
+            step += COARSE
         net.set_targets(acc)
         net.keep(slot0 + k)
         slots.append(slot0 + k)
@@ -448,6 +484,41 @@ def column_training_set(net, slot0=0, n_columns=500, width=0.03, height=0.6, sna
             log(f"snapshot {k}: step {step}, nodes {n}, edges {e}, ballistic rms {math.sqrt(ball[-1]) * A0:.2f} m/s^2")
     dev.close()
     return slots, ball, int(np.sum(sph))
+
+
+def held_out_rmse(net, slots):
+    """root mean squared acceleration error (m/s^2) on kept graphs of the learned step and of the
+    ballistic step (free fall for every sphere): the errors of spheres in contact, averaged over
+    all spheres (a sphere in free flight moves the same in both). The ballistic step is the
+    network with a zero decoder and the free-fall bias."""
+    trained = float(np.mean([net.kept_loss(k) for k in slots]))
+    p = net.params()
+    q = p.copy()
+    n = len(p)
+    # the decoder is the last H * 3 + 3 parameters: weights then bias
+    q[n - 3 - 64 * 3:n - 3] = 0.
+    q[n - 3:] = (0., 0., -G / A0)
+    net.set_params(q)
+    ballistic = float(np.mean([net.kept_loss(k) for k in slots]))
+    net.set_params(p)
+    return math.sqrt(trained) * A0, math.sqrt(ballistic) * A0
+
+
+def train_on_columns(net, n_columns=500, epochs=30, rounds=1, rollout=50, snapshots=20, every=1500, seed=100,
+                     log=None):
+    """Train the network on DEM trajectories of seeded columns (column_training_set), then for
+    each of ``rounds`` rounds add the states its own coarse steps reach from the same snapshots
+    (rollout) with their DEM targets, and train on the union. Returns the loss history and the
+    ballistic loss of the DEM states."""
+    slots, ball, n = column_training_set(net, 0, n_columns=n_columns, snapshots=snapshots, every=every, seed=seed,
+                                         log=log)
+    hist = net.train_kept(slots, epochs=epochs, lr=1e-3, lr_end=1e-4, log=log)
+    for r in range(rounds):
+        more, _, _ = column_training_set(net, 100 * (r + 1), n_columns=n_columns, snapshots=snapshots, every=every,
+                                         seed=seed, rollout=rollout, log=log)
+        slots = slots + more
+        hist += net.train_kept(slots, epochs=max(epochs // 2, 1), lr=3e-4, lr_end=3e-5, seed=r + 1, log=log)
+    return hist, ball, n, len(slots)
 
 
 # ---------------------------------------------------------------- column collapse
