@@ -33,6 +33,7 @@ struct ContactArgs {
   unsigned long long* nbits;
   uint32_t *ntpre, *ngcnt;
   uint32_t* act;
+  uint32_t* sidx;  // state index of each real slot of the new set
   // particles
   const double *px, *py, *pz, *vx, *vy, *vz, *wx, *wy, *wz, *rad, *mass;
   const uint32_t* flags;
@@ -111,7 +112,8 @@ __device__ __forceinline__ bool law_keeps(double uN, int noBreak) {
 // once per pass of GROUP_BS slots: real bits of the pass, tile prefixes, and (when act
 // is given) the slot of every state index. running = real slots of the group so far.
 __device__ __forceinline__ void group_pass(int64_t c, bool keep, int64_t base, uint32_t& running,
-                                           unsigned long long* bits, uint32_t* tpre, uint32_t* act) {
+                                           unsigned long long* bits, uint32_t* tpre, uint32_t* act,
+                                           uint32_t* sidx) {
   __shared__ uint32_t wc[GROUP_BS / 64];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const unsigned long long b = __ballot(keep);
@@ -129,7 +131,9 @@ __device__ __forceinline__ void group_pass(int64_t c, bool keep, int64_t base, u
   if (lane == 0 && (wv & 3) == 0) tpre[c >> 8] = running + wpre;
   if (keep && act) {
     const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-    act[state_index((uint32_t)(base / GROUP), running + wpre + (uint32_t)__popcll(b & lt))] = (uint32_t)c;
+    const uint32_t st = state_index((uint32_t)(base / GROUP), running + wpre + (uint32_t)__popcll(b & lt));
+    act[st] = (uint32_t)c;
+    sidx[c] = st;
   }
   running += tot;
   __syncthreads();
@@ -208,7 +212,7 @@ __global__ __launch_bounds__(GROUP_BS) void k_cfilter(ContactArgs a) {
   uint32_t running = 0;
 #pragma unroll
   for (int pass = 0; pass < NP; pass++)
-    group_pass(base + pass * GROUP_BS + threadIdx.x, keep[pass], base, running, a.nbits, a.ntpre, a.act);
+    group_pass(base + pass * GROUP_BS + threadIdx.x, keep[pass], base, running, a.nbits, a.ntpre, a.act, a.sidx);
   if (threadIdx.x == 0) a.ngcnt[g] = running;
 }
 
@@ -502,7 +506,7 @@ __global__ __launch_bounds__(GROUP_BS) void k_cfilter32(ContactArgs a) {
   uint32_t running = 0;
 #pragma unroll
   for (int pass = 0; pass < NP; pass++)
-    group_pass(base + pass * GROUP_BS + threadIdx.x, keep[pass], base, running, a.nbits, a.ntpre, a.act);
+    group_pass(base + pass * GROUP_BS + threadIdx.x, keep[pass], base, running, a.nbits, a.ntpre, a.act, a.sidx);
   if (threadIdx.x == 0) a.ngcnt[g] = running;
 }
 
@@ -719,6 +723,7 @@ static ContactArgs contact_args(Ctx& c, int from, int to) {
   a.ntpre = N.tpre.p;
   a.ngcnt = N.gcnt.p;
   a.act = c.act.p;
+  a.sidx = N.sidx.p;
   a.px = c.pos[0].p; a.py = c.pos[1].p; a.pz = c.pos[2].p;
   a.vx = c.vel[0].p; a.vy = c.vel[1].p; a.vz = c.vel[2].p;
   a.wx = c.angVel[0].p; a.wy = c.angVel[1].p; a.wz = c.angVel[2].p;
@@ -796,6 +801,7 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
       else hipLaunchKernelGGL((k_cforce32<false, WDG_LAW_IDEALELPL, false>), gforce, bforce, 0, c.stream, a);
     }
     WDG_LAUNCH_CHECK();
+    c.R[to].sidxValid = true;
     c.rcur = to;
     c.forcesPending = p->applyForces != 0;
     return;
@@ -819,6 +825,7 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
     else hipLaunchKernelGGL((k_cforce<false, WDG_LAW_IDEALELPL, false>), gforce, bforce, 0, c.stream, a);
   }
   WDG_LAUNCH_CHECK();
+  c.R[to].sidxValid = true;
   c.rcur = to;
   c.forcesPending = p->applyForces != 0;
 }
@@ -830,7 +837,7 @@ __global__ __launch_bounds__(GROUP_BS) void k_flag_bits(int64_t nc, const uint32
   uint32_t running = 0;
   for (int pass = 0; pass < GROUP / GROUP_BS; pass++) {
     const int64_t c = base + pass * GROUP_BS + threadIdx.x;
-    group_pass(c, c < nc && flag[c] != 0u, base, running, bits, tpre, nullptr);
+    group_pass(c, c < nc && flag[c] != 0u, base, running, bits, tpre, nullptr, nullptr);
   }
   if (threadIdx.x == 0) gcnt[blockIdx.x] = running;
 }
@@ -838,6 +845,7 @@ __global__ __launch_bounds__(GROUP_BS) void k_flag_bits(int64_t nc, const uint32
 void reals_from_flags(Ctx& c, const uint32_t* flag, int64_t nslots, int dst) {
   Reals& N = c.R[dst];
   N.ensure(nslots);
+  N.sidxValid = false;
   const int64_t ng = ngroups(nslots);
   if (ng > 0) {
     hipLaunchKernelGGL(k_flag_bits, dim3((unsigned)ng), dim3(GROUP_BS), 0, c.stream, nslots, flag, N.bits.p,

@@ -143,6 +143,10 @@ struct Reals {
   DBuf<double> huN;               // L6Geom.uN
   // gather record per contact: global force on pA (3), contact point (3), cellDist bits, unused
   DBuf<double> gp;
+  // state index of every real slot, written by the contact loop's filter (stale for the other
+  // slots); valid only for a set that filter produced
+  DBuf<uint32_t> sidx;
+  bool sidxValid = false;
   // room for every slot of a list of nslots being real
   void ensure(int64_t nslots) {
     const size_t g = (size_t)ngroups(nslots);
@@ -153,11 +157,13 @@ struct Reals {
     for (int k = 0; k < 3; k++) { hN[k].ensure(c); hY[k].ensure(c); hF[k].ensure(c); }
     huN.ensure(c);
     gp.ensure(GP * c);
+    sidx.ensure(c);
   }
   void release() {
     bits.release(); tpre.release(); gcnt.release();
     for (int k = 0; k < 3; k++) { hN[k].release(); hY[k].release(); hF[k].release(); }
-    huN.release(); gp.release();
+    huN.release(); gp.release(); sidx.release();
+    sidxValid = false;
   }
 };
 struct RealView {
