@@ -359,13 +359,22 @@ __global__ __launch_bounds__(256) void k_mm_dw(ARows a, int64_t M, const bf16* _
     partialB[(int64_t)blockIdx.x * H + threadIdx.x] =
         ((bsum[0][threadIdx.x] + bsum[1][threadIdx.x]) + bsum[2][threadIdx.x]) + bsum[3][threadIdx.x];
 }
-// grad[i] (+)= sum over blocks of partial[blk][i], in block order
+// grad[i] = sum over blocks of partial[blk][i], in block order
 __global__ void k_sum_partials(const float* __restrict__ partial, int64_t nblk, int64_t n, float* grad) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float s = 0.f;
   for (int64_t b = 0; b < nblk; b++) s += partial[b * n + i];
   grad[i] = s;
+}
+// first level of a two-level sum for many blocks: segment y of len blocks, in block order
+__global__ void k_sum_seg(const float* __restrict__ partial, int64_t nblk, int64_t n, int64_t len, float* seg) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t b0 = (int64_t)blockIdx.y * len, b1 = b0 + len < nblk ? b0 + len : nblk;
+  float s = 0.f;
+  for (int64_t b = b0; b < b1; b++) s += partial[b * n + i];
+  seg[(int64_t)blockIdx.y * n + i] = s;
 }
 
 // ------------------------------------------------------------------ graph sums
@@ -558,7 +567,7 @@ struct Mpnn {
   uint64_t uidUploads = 0;
   // activations
   DBuf<bf16> h[3], m[2], agg[2], u[2];
-  DBuf<float> out, dA, dhA, dhB, dU, dZ, part, partB, partLoss;
+  DBuf<float> out, dA, dhA, dhB, dU, dZ, part, partB, partLoss, partSeg;
   DBuf<bf16> dY, dYe;
   DBuf<uint8_t> cflag;  // node in contact (else in free flight)
   hipEvent_t done = nullptr;  // orders the network's stream and an engine context's stream
@@ -614,17 +623,32 @@ static void images(Mpnn& s) {
   }
 }
 
+// sums of per-block partials in a fixed order: directly for few blocks, else over SUM_SEGS
+// segments of consecutive blocks and then over the segments
+constexpr int64_t SUM_SEGS = 64;
+static void sum_partials(Mpnn& s, const float* partial, int64_t nblk, int64_t n, float* out) {
+  if (nblk <= 2 * SUM_SEGS) {
+    hipLaunchKernelGGL(k_sum_partials, dim3(nblocks(n, 256)), dim3(256), 0, s.stream, partial, nblk, n, out);
+    WDG_LAUNCH_CHECK();
+    return;
+  }
+  const int64_t len = (nblk + SUM_SEGS - 1) / SUM_SEGS;
+  hipLaunchKernelGGL(k_sum_seg, dim3(nblocks(n, 256), (unsigned)SUM_SEGS), dim3(256), 0, s.stream, partial, nblk, n,
+                     len, s.partSeg.p);
+  WDG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sum_partials, dim3(nblocks(n, 256)), dim3(256), 0, s.stream, (const float*)s.partSeg.p,
+                     SUM_SEGS, n, out);
+  WDG_LAUNCH_CHECK();
+}
+
 template <int MODE, int KP>
 static void dw(Mpnn& s, const ARows& a, int64_t M, const bf16* dY, float* gW, float* gB) {
   const int64_t rowsPer = (int64_t)DW_CHUNKS * 32;
   const int64_t nblk = std::max<int64_t>((M + rowsPer - 1) / rowsPer, 1);
   hipLaunchKernelGGL((k_mm_dw<MODE, KP>), dim3((unsigned)nblk), dim3(256), 0, s.stream, a, M, dY, s.part.p, s.partB.p);
   WDG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sum_partials, dim3(nblocks((int64_t)KP * H, 256)), dim3(256), 0, s.stream,
-                     (const float*)s.part.p, nblk, (int64_t)KP * H, gW);
-  WDG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s.stream, (const float*)s.partB.p, nblk, (int64_t)H, gB);
-  WDG_LAUNCH_CHECK();
+  sum_partials(s, s.part.p, nblk, (int64_t)KP * H, gW);
+  sum_partials(s, s.partB.p, nblk, (int64_t)H, gB);
 }
 
 static void forward(Mpnn& s) {
@@ -691,6 +715,7 @@ static double backward(Mpnn& s) {
     const size_t need = std::max({(size_t)bE * KM * H, (size_t)bN * KU * H, (size_t)nbN * (H + 1) * NOUT});
     s.part.ensure(need);
     s.partB.ensure((size_t)std::max(bE, bN) * H);
+    s.partSeg.ensure((size_t)SUM_SEGS * KM * H);
   }
   WDG_HIP(hipMemsetAsync(s.G.p, 0, s.L.total * sizeof(float), s.stream));
   hipLaunchKernelGGL(k_loss, dim3((unsigned)nbN), dim3(256), 0, s.stream, N, (const float*)s.out.p,
@@ -700,9 +725,7 @@ static double backward(Mpnn& s) {
   hipLaunchKernelGGL(k_dec_bwd, dim3((unsigned)nbN), dim3(256), 0, s.stream, N, (const bf16*)s.h[2].p,
                      (const float*)(s.P.p + s.L.decW), (const float*)s.dA.p, s.dhA.p, s.part.p);
   WDG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s.stream, (const float*)s.part.p, nbN,
-                     (int64_t)(H + 1) * NOUT, s.G.p + s.L.decW);  // decW then decB are adjacent
-  WDG_LAUNCH_CHECK();
+  sum_partials(s, s.part.p, nbN, (int64_t)(H + 1) * NOUT, s.G.p + s.L.decW);  // decW then decB are adjacent
   float* dh = s.dhA.p;   // gradient of the current layer output
   float* dh2 = s.dhB.p;
   const unsigned gN = (unsigned)((N + 63) / 64), gE = (unsigned)((E + 63) / 64);
@@ -959,7 +982,7 @@ int wdg_mpnn_destroy(wdg_mpnn* m) {
   if (!m) return WDG_OK;
   Mpnn& s = m->s;
   DBuf<float>* fb[] = {&s.P, &s.G, &s.Am, &s.Av, &s.T, &s.out, &s.dA, &s.dhA, &s.dhB, &s.dU, &s.dZ, &s.part, &s.partB,
-                       &s.partLoss};
+                       &s.partLoss, &s.partSeg};
   for (auto* b : fb) b->release();
   DBuf<bf16>* bb[] = {&s.img, &s.X, &s.EF, &s.h[0], &s.h[1], &s.h[2], &s.m[0], &s.m[1], &s.agg[0], &s.agg[1],
                       &s.u[0], &s.u[1], &s.dY, &s.dYe};
