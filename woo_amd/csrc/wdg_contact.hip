@@ -34,6 +34,9 @@ struct ContactArgs {
   uint32_t *ntpre, *ngcnt;
   uint32_t* act;
   uint32_t* sidx;  // state index of each real slot of the new set
+  const uint32_t* osidx;  // the same for the old set, when its filter made it (else null)
+  uint4* desc;           // per new state index: info, pA, pB, old state index (or NO_STATE)
+  uint64_t* descDist;    // per new state index: packed cellDist (periodic cells)
   // particles
   const double *px, *py, *pz, *vx, *vy, *vz, *wx, *wy, *wz, *rad, *mass;
   const uint32_t* flags;
@@ -108,12 +111,24 @@ __device__ __forceinline__ bool law_keeps(double uN, int noBreak) {
   return !(uN > 0);
 }
 
+// What the force pass needs of a kept slot, found by the filter: its type and role bits, the
+// two particles in role order (pA, pB), the state index of its previous step (NO_STATE when
+// it was not real) and, in periodic cells, its packed cellDist. The filter writes it at the
+// slot's new state index, so the force pass starts from one coalesced record.
+constexpr uint32_t NO_STATE = 0xFFFFFFFFu;
+struct SlotD {
+  uint32_t info, pA, pB, ko;
+  uint64_t cd;
+};
+
 // Group kernel tail, one workgroup of GROUP_BS threads per group of GROUP slots, called
 // once per pass of GROUP_BS slots: real bits of the pass, tile prefixes, and (when act
-// is given) the slot of every state index. running = real slots of the group so far.
+// is given) the slot of every state index, the state index of every real slot and the
+// force pass's record. running = real slots of the group so far.
 __device__ __forceinline__ void group_pass(int64_t c, bool keep, int64_t base, uint32_t& running,
                                            unsigned long long* bits, uint32_t* tpre, uint32_t* act,
-                                           uint32_t* sidx) {
+                                           uint32_t* sidx, const SlotD& d = SlotD{}, uint4* desc = nullptr,
+                                           uint64_t* descDist = nullptr) {
   __shared__ uint32_t wc[GROUP_BS / 64];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const unsigned long long b = __ballot(keep);
@@ -134,13 +149,20 @@ __device__ __forceinline__ void group_pass(int64_t c, bool keep, int64_t base, u
     const uint32_t st = state_index((uint32_t)(base / GROUP), running + wpre + (uint32_t)__popcll(b & lt));
     act[st] = (uint32_t)c;
     sidx[c] = st;
+    desc[st] = make_uint4(d.info, d.pA, d.pB, d.ko);
+    if (descDist) descDist[st] = d.cd;
   }
   running += tot;
   __syncthreads();
 }
 
+// state index of slot c in the old real set (a real slot)
+__device__ __forceinline__ uint32_t old_state(const ContactArgs& a, int64_t c) {
+  return a.osidx ? a.osidx[c] : rv_index(a.ov, (uint32_t)c);
+}
+
 template <bool PERIODIC, int LAW>
-__device__ __forceinline__ bool filter_slot(const ContactArgs& a, int64_t c) {
+__device__ __forceinline__ bool filter_slot(const ContactArgs& a, int64_t c, SlotD& d) {
   bool keep = false;
   if (c < a.nc) {
     const uint32_t info = a.info[c];
@@ -150,6 +172,8 @@ __device__ __forceinline__ bool filter_slot(const ContactArgs& a, int64_t c) {
       const uint32_t pA = swap ? a.B[c] : a.A[c], pB = swap ? a.A[c] : a.B[c];
       uint64_t cd;
       const d3 shift2 = slot_shift<PERIODIC>(a, c, cd);
+      d.info = info; d.pA = pA; d.pB = pB; d.cd = cd;
+      d.ko = real ? old_state(a, c) : NO_STATE;
       const d3 posA = mk3(a.px[pA], a.py[pA], a.pz[pA]);
       const d3 pos2 = add3(mk3(a.px[pB], a.py[pB], a.pz[pB]), shift2);
       bool pass;
@@ -185,7 +209,7 @@ __device__ __forceinline__ bool filter_slot(const ContactArgs& a, int64_t c) {
         if (sense == 0) {
           if (!real) nvv = dist > 0 ? 1. : -1.;
           else {
-            const uint32_t ko = rv_index(a.ov, (uint32_t)c);
+            const uint32_t ko = d.ko;
             nvv = ax == 0 ? a.oN0[ko] : (ax == 1 ? a.oN1[ko] : a.oN2[ko]);
           }
         } else {
@@ -207,12 +231,15 @@ __global__ __launch_bounds__(GROUP_BS) void k_cfilter(ContactArgs a) {
   // all slots of the thread first (their loads overlap), then the bookkeeping passes
   constexpr int NP = GROUP / GROUP_BS;
   bool keep[NP];
+  SlotD d[NP];
 #pragma unroll
-  for (int pass = 0; pass < NP; pass++) keep[pass] = filter_slot<PERIODIC, LAW>(a, base + pass * GROUP_BS + threadIdx.x);
+  for (int pass = 0; pass < NP; pass++)
+    keep[pass] = filter_slot<PERIODIC, LAW>(a, base + pass * GROUP_BS + threadIdx.x, d[pass]);
   uint32_t running = 0;
 #pragma unroll
   for (int pass = 0; pass < NP; pass++)
-    group_pass(base + pass * GROUP_BS + threadIdx.x, keep[pass], base, running, a.nbits, a.ntpre, a.act, a.sidx);
+    group_pass(base + pass * GROUP_BS + threadIdx.x, keep[pass], base, running, a.nbits, a.ntpre, a.act, a.sidx,
+               d[pass], a.desc, PERIODIC ? a.descDist : nullptr);
   if (threadIdx.x == 0) a.ngcnt[g] = running;
 }
 
@@ -231,16 +258,19 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
   const uint32_t cnt = a.ngcnt[g];
   for (uint32_t loc = (lb % nblk) * CF_BS + threadIdx.x; loc < cnt; loc += CF_BS * nblk) {
     const int64_t k = state_index((uint32_t)g, loc);
-    const uint32_t c = a.act[k];
-    const uint32_t info = a.info[c];
-    const bool real = rv_real(a.ov, c);
-    const uint32_t ko = real ? rv_index(a.ov, c) : 0u;
-    const bool swap = (info & CI_SWAP) != 0u;
-    const uint32_t pA = swap ? a.B[c] : a.A[c], pB = swap ? a.A[c] : a.B[c];
+    // the filter's record of the slot: type bits, particles in role order, old state index
+    const uint4 dk = a.desc[k];
+    const uint32_t info = dk.x, pA = dk.y, pB = dk.z;
+    const bool real = dk.w != NO_STATE;
+    const uint32_t ko = real ? dk.w : 0u;
     const int type = (int)(info & CI_TYPE_MASK);
     const double dt = a.dtScene ? a.dtScene[a.scene[pA]] : a.dt;
-    uint64_t cd;
-    const d3 shift2 = slot_shift<PERIODIC>(a, c, cd);
+    uint64_t cd = 0;
+    d3 shift2 = mk3(0., 0., 0.);
+    if (PERIODIC) {
+      cd = a.descDist[k];
+      shift2 = mk3(a.L[0] * unpack_cd(cd, 0), a.L[1] * unpack_cd(cd, 1), a.L[2] * unpack_cd(cd, 2));
+    }
     const d3 posA = mk3(a.px[pA], a.py[pA], a.pz[pA]);
     const d3 posB = mk3(a.px[pB], a.py[pB], a.pz[pB]);
     d3 prevNormal = mk3(0., 0., 0.);
@@ -437,7 +467,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
 __device__ __forceinline__ f3 tof3(d3 v) { return mkf3((float)v.x, (float)v.y, (float)v.z); }
 
 template <bool PERIODIC, int LAW>
-__device__ __forceinline__ bool filter_slot32(const ContactArgs& a, int64_t c) {
+__device__ __forceinline__ bool filter_slot32(const ContactArgs& a, int64_t c, SlotD& d) {
   bool keep = false;
   if (c < a.nc) {
     const uint32_t info = a.info[c];
@@ -447,6 +477,8 @@ __device__ __forceinline__ bool filter_slot32(const ContactArgs& a, int64_t c) {
       const uint32_t pA = swap ? a.B[c] : a.A[c], pB = swap ? a.A[c] : a.B[c];
       uint64_t cd;
       const d3 shift2 = slot_shift<PERIODIC>(a, c, cd);
+      d.info = info; d.pA = pA; d.pB = pB; d.cd = cd;
+      d.ko = real ? old_state(a, c) : NO_STATE;
       const f3 r = tof3(sub3(add3(mk3(a.px[pB], a.py[pB], a.pz[pB]), shift2), mk3(a.px[pA], a.py[pA], a.pz[pA])));
       bool pass;
       float uN;
@@ -479,7 +511,7 @@ __device__ __forceinline__ bool filter_slot32(const ContactArgs& a, int64_t c) {
         if (sense == 0) {
           if (!real) nvv = dist > 0.f ? 1.f : -1.f;
           else {
-            const uint32_t ko = rv_index(a.ov, (uint32_t)c);
+            const uint32_t ko = d.ko;
             const float* on = reinterpret_cast<const float*>(ax == 0 ? a.oN0 : (ax == 1 ? a.oN1 : a.oN2));
             nvv = on[ko];
           }
@@ -501,12 +533,15 @@ __global__ __launch_bounds__(GROUP_BS) void k_cfilter32(ContactArgs a) {
   const int64_t base = (int64_t)g * GROUP;
   constexpr int NP = GROUP / GROUP_BS;
   bool keep[NP];
+  SlotD d[NP];
 #pragma unroll
-  for (int pass = 0; pass < NP; pass++) keep[pass] = filter_slot32<PERIODIC, LAW>(a, base + pass * GROUP_BS + threadIdx.x);
+  for (int pass = 0; pass < NP; pass++)
+    keep[pass] = filter_slot32<PERIODIC, LAW>(a, base + pass * GROUP_BS + threadIdx.x, d[pass]);
   uint32_t running = 0;
 #pragma unroll
   for (int pass = 0; pass < NP; pass++)
-    group_pass(base + pass * GROUP_BS + threadIdx.x, keep[pass], base, running, a.nbits, a.ntpre, a.act, a.sidx);
+    group_pass(base + pass * GROUP_BS + threadIdx.x, keep[pass], base, running, a.nbits, a.ntpre, a.act, a.sidx,
+               d[pass], a.desc, PERIODIC ? a.descDist : nullptr);
   if (threadIdx.x == 0) a.ngcnt[g] = running;
 }
 
@@ -529,15 +564,18 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
   const uint32_t cnt = a.ngcnt[g];
   for (uint32_t loc = (lb % nblk) * CF_BS + threadIdx.x; loc < cnt; loc += CF_BS * nblk) {
     const int64_t k = state_index((uint32_t)g, loc);
-    const uint32_t c = a.act[k];
-    const uint32_t info = a.info[c];
-    const bool real = rv_real(a.ov, c);
-    const uint32_t ko = real ? rv_index(a.ov, c) : 0u;
-    const bool swap = (info & CI_SWAP) != 0u;
-    const uint32_t pA = swap ? a.B[c] : a.A[c], pB = swap ? a.A[c] : a.B[c];
+    // the filter's record of the slot: type bits, particles in role order, old state index
+    const uint4 dk = a.desc[k];
+    const uint32_t info = dk.x, pA = dk.y, pB = dk.z;
+    const bool real = dk.w != NO_STATE;
+    const uint32_t ko = real ? dk.w : 0u;
     const float dt = (float)(a.dtScene ? a.dtScene[a.scene[pA]] : a.dt);
-    uint64_t cd;
-    const d3 shift2 = slot_shift<PERIODIC>(a, c, cd);
+    uint64_t cd = 0;
+    d3 shift2 = mk3(0., 0., 0.);
+    if (PERIODIC) {
+      cd = a.descDist[k];
+      shift2 = mk3(a.L[0] * unpack_cd(cd, 0), a.L[1] * unpack_cd(cd, 1), a.L[2] * unpack_cd(cd, 2));
+    }
     const f3 r = tof3(sub3(add3(mk3(a.px[pB], a.py[pB], a.pz[pB]), shift2), mk3(a.px[pA], a.py[pA], a.pz[pA])));
     f3 prevNormal = mkf3(0.f, 0.f, 0.f);
     if (real) prevNormal = mkf3(oN0[ko], oN1[ko], oN2[ko]);
@@ -724,6 +762,9 @@ static ContactArgs contact_args(Ctx& c, int from, int to) {
   a.ngcnt = N.gcnt.p;
   a.act = c.act.p;
   a.sidx = N.sidx.p;
+  a.osidx = O.sidxValid ? O.sidx.p : nullptr;
+  a.desc = c.desc.p;
+  a.descDist = c.descDist.p;
   a.px = c.pos[0].p; a.py = c.pos[1].p; a.pz = c.pos[2].p;
   a.vx = c.vel[0].p; a.vy = c.vel[1].p; a.vz = c.vel[2].p;
   a.wx = c.angVel[0].p; a.wy = c.angVel[1].p; a.wz = c.angVel[2].p;
@@ -770,6 +811,8 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
   c.R[to].ensure(C.n);
   const int64_t ng = ngroups(C.n);
   c.act.ensure(ng * GROUP);
+  c.desc.ensure(ng * GROUP);
+  if (c.field.periodic) c.descDist.ensure(ng * GROUP);
   ContactArgs a = contact_args(c, from, to);
   a.dt = p->dt;
   a.renorm = (p->trsfRenorm > 0 && (p->step % p->trsfRenorm) == 0) ? 1 : 0;

@@ -60,13 +60,16 @@ static void plain_step(Ctx& c, const wdg_program* prog, int64_t step, double dt)
 struct HostState {
   int rcur, lfParity, dirtyIdx;
   bool forcesPending, forceClean, dirtyKnown;
+  bool sidxValid[2];
 };
 static HostState save(const Ctx& c) {
-  return HostState{c.rcur, c.lfParity, c.dirtyIdx, c.forcesPending, c.forceClean, c.dirtyKnown};
+  return HostState{c.rcur, c.lfParity, c.dirtyIdx, c.forcesPending, c.forceClean, c.dirtyKnown,
+                   {c.R[0].sidxValid, c.R[1].sidxValid}};
 }
 static void restore(Ctx& c, const HostState& h) {
   c.rcur = h.rcur; c.lfParity = h.lfParity; c.dirtyIdx = h.dirtyIdx;
   c.forcesPending = h.forcesPending; c.forceClean = h.forceClean; c.dirtyKnown = h.dirtyKnown;
+  c.R[0].sidxValid = h.sidxValid[0]; c.R[1].sidxValid = h.sidxValid[1];
 }
 
 static void capture(Ctx& c, const wdg_program* prog, int K, int64_t step, double dt) {
@@ -80,6 +83,8 @@ static void capture(Ctx& c, const wdg_program* prog, int K, int64_t step, double
   c.R[0].ensure(nc);
   c.R[1].ensure(nc);
   c.act.ensure(ngroups(nc) * GROUP);
+  c.desc.ensure(ngroups(nc) * GROUP);
+  if (c.field.periodic) c.descDist.ensure(ngroups(nc) * GROUP);
   const HostState h = save(c);
   hipGraph_t graph = nullptr;
   c.graphCapture = true;
@@ -137,7 +142,12 @@ static int replay(Ctx& c, const wdg_program* prog, int K, int64_t step, double d
   const int e = (int)c.hflags[32];
   // host state after e steps of contact loop + leapfrog
   if (e > 0) {
-    if (c.con().n > 0) c.rcur = (e & 1) ? 1 - c.rcur : c.rcur;
+    if (c.con().n > 0) {
+      // the real sets the executed filters wrote carry their per-slot state indices
+      c.R[1 - c.rcur].sidxValid = true;
+      if (e > 1) c.R[c.rcur].sidxValid = true;
+      c.rcur = (e & 1) ? 1 - c.rcur : c.rcur;
+    }
     const int lastParity = (c.lfParity + e - 1) & 1;
     c.lfParity = (c.lfParity + e) & 1;
     c.dirtyIdx = DIRTY_PP + lastParity;

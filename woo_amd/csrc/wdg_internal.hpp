@@ -263,6 +263,8 @@ struct Ctx {
   DBuf<uint64_t> bigKey, bigKeyTmp;
   DBuf<uint32_t> sortHist, scanAux;
   DBuf<uint32_t> act;     // slot of each state index (contact loop), per-slot flags (relaxation)
+  DBuf<uint4> desc;       // contact loop: the filter's record of each real slot, by state index
+  DBuf<uint64_t> descDist;
   // in-memory snapshots of the kinematic state (pos, vel, angVel, ori; 13 doubles per
   // particle, by user id), valid while the particle set they were taken of is loaded
   static constexpr int NSNAP = 4;
