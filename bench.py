@@ -516,6 +516,9 @@ def surrogate_form(args):
     net.close()
     ro_dem, pr_dem = S.runout_profile(p_dem, rad, center)
     ro_sur, pr_sur = S.runout_profile(p_sur, rad, center)
+    # the deposit's front without its few farthest spheres: 99.9th percentile of the distance
+    front = lambda p: float(np.percentile(np.hypot(p[:, 0] - center[0], p[:, 1] - center[1]) + rad, 99.9))
+    fr_dem, fr_sur = front(p_dem), front(p_sur)
     n = len(rad)
     v_dem = n * (coarse * S.COARSE) / t_dem
     v_sur = n * (coarse * S.COARSE) / t_sur
@@ -526,7 +529,9 @@ def surrogate_form(args):
                 speedup_vs_plain_dem=v_sur / v_dem, seconds=t_sur, checks=st["checks"], fallbacks=st["fallbacks"],
                 learned_coarse_steps=st["learned_coarse_steps"], dem_steps_in_checks_and_fallbacks=st["dem_steps"],
                 check_error_mm=dict(max=1e3 * max(st["errors"]), mean=1e3 * float(np.mean(st["errors"]))),
-                runout_m=dict(dem=ro_dem, learned=ro_sur, rel_err=abs(ro_sur - ro_dem) / ro_dem),
+                runout_m=dict(dem=ro_dem, learned=ro_sur, rel_err=abs(ro_sur - ro_dem) / ro_dem,
+                              definition="largest horizontal distance of a sphere's edge from the column axis"),
+                front_p999_m=dict(dem=fr_dem, learned=fr_sur, rel_err=abs(fr_sur - fr_dem) / fr_dem),
                 profile=dict(rms_diff_m=float(np.sqrt(np.mean((pr_sur - pr_dem) ** 2))),
                              max_rel_err=float(np.max(np.abs(pr_sur - pr_dem)[occupied] / pr_dem[occupied]))),
                 position_diff_max_mm=1e3 * float(np.max(np.linalg.norm(p_sur - p_dem, axis=1))),
@@ -596,8 +601,8 @@ def main():
                     help="0: step the batch with the plain engine loop (no tuning, no graph), e.g. under a tracer")
     ap.add_argument("--surrogate", type=int, default=1, help="also run the learned coarse step (column collapse)")
     ap.add_argument("--sur-columns", type=int, default=500)
-    ap.add_argument("--sur-epochs", type=int, default=30)
-    ap.add_argument("--sur-rounds", type=int, default=2, help="rounds of training on the network's own rollouts")
+    ap.add_argument("--sur-epochs", type=int, default=20)
+    ap.add_argument("--sur-rounds", type=int, default=5, help="rounds of training on the network's own rollouts")
     ap.add_argument("--sur-steps", type=int, default=20000, help="DEM steps of simulated time")
     ap.add_argument("--pmc-probe", type=int, default=0,
                     help="for rocprofv3 --pmc runs: warm up, launch the contact loop and leapfrog this many "
