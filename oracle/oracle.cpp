@@ -132,7 +132,7 @@ struct Material { int kind; double density, young, ktDivKn, tanPhi, surfEnergy, 
 
 enum : uint32_t {
   DOF_X = 1, DOF_Y = 2, DOF_Z = 4, DOF_RX = 8, DOF_RY = 16, DOF_RZ = 32, DOF_ALL = 63,
-  GRAVITY_SKIP = 512, DAMPING_SKIP = 2048
+  ENERGY_SKIP = 256, GRAVITY_SKIP = 512, DAMPING_SKIP = 2048
 };
 
 struct Particle {
@@ -158,6 +158,8 @@ struct Particle {
   double maxD2 = NaN;
   // Particle::contacts: other id -> index into the contact vector
   std::map<int, int> contacts;
+  // energy terms of this node in the last leapfrog: kinetic, grav, nonviscDamp
+  double en[3] = {0, 0, 0};
 };
 
 struct Contact {
@@ -178,6 +180,8 @@ struct Contact {
   V3 force = {{0, 0, 0}}, torque = {{0, 0, 0}};
   double kn = NaN, kt = NaN, tanPhi = NaN;
   double kt0 = 0, alpha_sqrtMK = 0, R = 0, K = 0, gamma = 0, alpha = 0;
+  // energy terms of this contact in the last contact loop: elast, plast, viscN, viscT
+  double en[4] = {0, 0, 0, 0};
   bool isReal() const { return geom && phys; }
 };
 
@@ -219,6 +223,8 @@ struct orc_scene {
   // DynDt PeriodicEngine bookkeeping
   int64_t dynStepLast = -1, dynStepPrev = -1;
   std::vector<std::mutex> locks;
+  // Scene.energy, ORC_E_* order
+  double energy[ORC_E_N] = {0, 0, 0, 0, 0, 0, 0};
 
   // ---------------------------------------------------------- contacts
   int find(int a, int b) const {
@@ -840,12 +846,21 @@ struct orc_scene {
         double FtNorm = std::sqrt(ft0 * ft0 + ft1 * ft1);
         double ratio = maxFt / FtNorm;
         if (FtNorm == 0.) ratio = 0.;
+        // IdealElPl.cpp:62-73: dissipation of the slip, while Ft is still the trial value
+        if (prm.trackEnergy) c.en[1] = (c.kt != 0. ? maxFt * (FtNorm - maxFt) / c.kt : 0.);
         ft0 = ft0 * ratio; ft1 = ft1 * ratio;
       }
       c.force[1] = ft0; c.force[2] = ft1;
       if (std::isnan(std::max(std::max(c.force[0], c.force[1]), c.force[2])) || std::isnan(c.force[0]) ||
           std::isnan(ft0) || std::isnan(ft1))
         throw std::runtime_error("##" + std::to_string(c.pA) + "+" + std::to_string(c.pB) + " has NaN force!");
+    }
+    // IdealElPl.cpp:85-92: elastic potential, 0 when it is NaN
+    if (prm.trackEnergy) {
+      double elast = 0.5 * (pow2(c.force[0]) / c.kn +
+                            (c.kt != 0. ? (c.force[1] * c.force[1] + c.force[2] * c.force[2]) / c.kt : 0.));
+      if (std::isnan(elast)) elast = 0.;
+      c.en[0] = elast;
     }
     return true;
   }
@@ -867,6 +882,8 @@ struct orc_scene {
     if (c.alpha == 0. && prm.noAttraction && Fne + Fnv > 0) Fnv = -Fne;
     double Fn = Fne + Fna + Fnv;
     c.force[0] = Fn;
+    // Hertz.cpp:181: normal viscous dissipation
+    if (prm.trackEnergy) c.en[2] = Fnv * velN * dt;
     c.kt = c.kt0 * std::sqrt(c.uN < 0 ? -c.uN : 0);
     double dk = dt * c.kt;
     double ft0 = c.force[1] + dk * vt0, ft1 = c.force[2] + dk * vt1;
@@ -874,11 +891,18 @@ struct orc_scene {
     if (ft0 * ft0 + ft1 * ft1 > pow2(maxFt)) {
       double FtNorm = std::sqrt(ft0 * ft0 + ft1 * ft1);
       double ratio = maxFt / FtNorm;
+      // Hertz.cpp:191: sliding dissipation
+      if (prm.trackEnergy) c.en[1] = (.5 * (FtNorm - maxFt) + maxFt) * (FtNorm - maxFt) / c.kt;
       ft0 = ft0 * ratio; ft1 = ft1 * ratio;
     } else {
       ft0 = ft0 + eta * vt0; ft1 = ft1 + eta * vt1;
+      // Hertz.cpp:198: tangential viscous dissipation
+      if (prm.trackEnergy) c.en[3] = eta * (vt0 * vt0 + vt1 * vt1) * dt;
     }
     c.force[1] = ft0; c.force[2] = ft1;
+    // Hertz.cpp:203-206: elastic potential (in compression)
+    if (prm.trackEnergy && c.uN < 0)
+      c.en[0] = kn0 * (2 / 5.) * pown(std::sqrt(-c.uN), 5) + 0.5 * (ft0 * ft0 + ft1 * ft1) / c.kt;
     return true;
   }
   // Contact::getForceTorqueBranch (Contact.cpp:24-34); phys.torque is zero for both laws
@@ -902,6 +926,8 @@ struct orc_scene {
     std::vector<std::string> errMsg(TH);
     if (!det && locks.size() != par.size()) locks = std::vector<std::mutex>(par.size());
     long size = (long)con.size();
+    if (prm.trackEnergy)
+      for (auto& C : con) C.en[0] = C.en[1] = C.en[2] = C.en[3] = 0.;
 #pragma omp parallel for schedule(guided)
     for (long i = 0; i < size; i++) {
       int t = 0;
@@ -957,6 +983,17 @@ struct orc_scene {
       if (!m.empty()) throw std::runtime_error(m);
     for (auto& v : pendT)
       for (auto& pr : v) pending.push_back(pr);
+    if (prm.trackEnergy) {
+      // EnergyTracker: elast (resettable) and the dissipations (increments) are each one sum over
+      // the contacts, in ContactContainer order, added to Scene.energy
+      double e[4] = {0, 0, 0, 0};
+      for (const auto& C : con)
+        for (int q = 0; q < 4; q++) e[q] += C.en[q];
+      energy[ORC_E_ELAST] += e[0];
+      energy[ORC_E_PLAST] += e[1];
+      energy[ORC_E_VISCN] += e[2];
+      energy[ORC_E_VISCT] += e[3];
+    }
     if (det && prm.applyForces) {
       // canonical order: each particle sums its real contacts in increasing order of the other id
 #pragma omp parallel for schedule(guided)
@@ -990,13 +1027,40 @@ struct orc_scene {
   // Leapfrog::run, spherical integrator (Leapfrog.cpp:104-297)
   void leapfrog() {
     const double dt = prm.dt, damping = prm.damping;
+    // Leapfrog.cpp:132 (gradV is zero here, so a periodic scene is not tracked at step 0)
+    const bool track = prm.trackEnergy && (!prm.periodic || step > 0);
+    const bool hasGravity = !(prm.gravity[0] == 0 && prm.gravity[1] == 0 && prm.gravity[2] == 0);
 #pragma omp parallel for schedule(guided)
     for (long i = 0; i < (long)par.size(); i++) {
       Particle& p = par[i];
       if (!p.integrated) continue;
       bool damp = (damping != 0. && !(p.flags & DAMPING_SKIP));
       const uint32_t bl = p.flags & DOF_ALL;
+      p.en[0] = p.en[1] = p.en[2] = 0.;
+      if (track) {
+        // Leapfrog::doDampingDissipation (Leapfrog.cpp:53-65), with the values before the update
+        if (damp && !(p.flags & ENERGY_SKIP)) {
+          double fv = (std::fabs(p.vel[0]) * std::fabs(p.force[0]) + std::fabs(p.vel[1]) * std::fabs(p.force[1])) +
+                      std::fabs(p.vel[2]) * std::fabs(p.force[2]);
+          double tw = (std::fabs(p.angVel[0]) * std::fabs(p.torque[0]) + std::fabs(p.angVel[1]) * std::fabs(p.torque[1])) +
+                      std::fabs(p.angVel[2]) * std::fabs(p.torque[2]);
+          p.en[2] = fv * damping * dt + tw * damping * dt;
+        }
+        // Leapfrog::doGravityWork (Leapfrog.cpp:67-74)
+        if (hasGravity && !(p.flags & GRAVITY_SKIP)) {
+          double gr = 0;
+          if (bl == 0)
+            gr = -((prm.gravity[0] * p.vel[0] + prm.gravity[1] * p.vel[1]) + prm.gravity[2] * p.vel[2]) * p.mass * dt;
+          else
+            for (int k = 0; k < 3; k++)
+              if (!(bl & (1u << k))) gr -= prm.gravity[k] * p.vel[k] * p.mass * dt;
+          p.en[1] = gr;
+        }
+      }
+      // velocities before the update and accelerations, for the kinetic energy
+      double pv[3] = {0, 0, 0}, pw[3] = {0, 0, 0}, lacc[3] = {0, 0, 0}, aacc[3] = {0, 0, 0};
       if (bl != DOF_ALL) {
+        for (int k = 0; k < 3; k++) { pv[k] = p.vel[k]; pw[k] = p.angVel[k]; }
         double la[3], aa[3] = {0, 0, 0};
         if (bl == 0) for (int k = 0; k < 3; k++) la[k] = p.force[k] / p.mass;
         else for (int k = 0; k < 3; k++) { la[k] = 0; if (!(bl & (1u << k))) la[k] += p.force[k] / p.mass; }
@@ -1010,6 +1074,16 @@ struct orc_scene {
             for (int k = 0; k < 3; k++) aa[k] *= 1 - damping * signum(p.torque[k] * (p.angVel[k] + 0.5 * dt * aa[k]));
           for (int k = 0; k < 3; k++) p.angVel[k] += dt * aa[k];
         }
+        for (int k = 0; k < 3; k++) { lacc[k] = la[k]; aacc[k] = aa[k]; }
+      }
+      // Leapfrog::doKineticEnergy (Leapfrog.cpp:76-98), spherical: on-step velocities
+      if (track && !(p.flags & ENERGY_SKIP)) {
+        double cv[3], cw[3];
+        for (int k = 0; k < 3; k++) { cv[k] = pv[k] + .5 * dt * lacc[k]; cw[k] = pw[k] + .5 * dt * aacc[k]; }
+        double Et = .5 * p.mass * ((cv[0] * cv[0] + cv[1] * cv[1]) + cv[2] * cv[2]);
+        double Er = 0.5 * ((cw[0] * (p.inertia[0] * cw[0]) + cw[1] * (p.inertia[1] * cw[1])) + cw[2] * (p.inertia[2] * cw[2]));
+        if (std::isnan(Er) || std::isinf(std::max(std::max(p.inertia[0], p.inertia[1]), p.inertia[2]))) Er = 0;
+        p.en[0] = Et + Er;
       }
       for (int k = 0; k < 3; k++) p.pos[k] += p.vel[k] * dt;
       // leapfrogSphericalRotate
@@ -1026,6 +1100,16 @@ struct orc_scene {
       double qn = std::sqrt(((ori.w * ori.w + ori.x * ori.x) + ori.y * ori.y) + ori.z * ori.z);
       p.ori[0] = ori.w / qn; p.ori[1] = ori.x / qn; p.ori[2] = ori.y / qn; p.ori[3] = ori.z / qn;
       if (prm.reset) resetForce(p);
+    }
+    if (track) {
+      // kinetic (resettable), grav and nonviscDamp (increments), each one sum in node order
+      double e[3] = {0, 0, 0};
+      for (const auto& p : par)
+        if (p.integrated)
+          for (int q = 0; q < 3; q++) e[q] += p.en[q];
+      energy[ORC_E_KINETIC] += e[0];
+      energy[ORC_E_GRAV] += e[1];
+      energy[ORC_E_NONVISCDAMP] += e[2];
     }
   }
   // ---------------------------------------------------------- DynDt
@@ -1079,6 +1163,8 @@ struct orc_scene {
   }
   // ---------------------------------------------------------- step driver (Scene::doOneStep)
   void doOneStep(const int* eng, int ne) {
+    // Scene.cpp:416: the resettable terms start every step at zero
+    if (prm.trackEnergy) energy[ORC_E_KINETIC] = energy[ORC_E_ELAST] = 0.;
     for (int e = 0; e < ne; e++) {
       switch (eng[e]) {
         case ORC_E_FORCERESETTER: forceResetter(); break;
@@ -1154,6 +1240,12 @@ int orc_add_sphere(orc_scene* s, const double pos[3], double radius, int mat, in
   p.integrated = !fixed;
   s->par.push_back(p);
   return (int)s->par.size() - 1;
+}
+
+int orc_set_flags(orc_scene* s, int64_t i, uint32_t flags) {
+  if (i < 0 || i >= (int64_t)s->par.size()) { g_err = "particle id out of range"; return -1; }
+  s->par[i].flags = flags;
+  return 0;
 }
 
 int orc_add_spheres(orc_scene* s, int64_t n, const double* pos, const double* radius, int mat, uint32_t mask) {
@@ -1328,6 +1420,10 @@ double orc_solid_fraction(const orc_scene* s, const double boxMin[3], const doub
 }
 
 // sum over integrated nodes of 1/2 m v^2 + 1/2 w.(I w)
+void orc_energy(const orc_scene* s, double out[7]) {
+  for (int q = 0; q < ORC_E_N; q++) out[q] = s->energy[q];
+}
+
 double orc_kinetic_energy(const orc_scene* s) {
   double E = 0;
   for (const auto& p : s->par) {

@@ -75,7 +75,16 @@ typedef struct {
   /* DynDt */
   int dynDtPeriod;
   double maxRelInc;
+  /* Scene.trackEnergy */
+  int trackEnergy;
 } orc_params;
+
+/* Scene.energy (EnergyTracker, src/core/EnergyTracker.hpp:30-60): the terms the path's engines
+ * add, in this order. kinetic and elast hold the last step's value (IsResettable); the others
+ * accumulate (IsIncrement). */
+enum { ORC_E_KINETIC = 0, ORC_E_ELAST = 1, ORC_E_GRAV = 2, ORC_E_NONVISCDAMP = 3, ORC_E_PLAST = 4, ORC_E_VISCN = 5,
+       ORC_E_VISCT = 6, ORC_E_N = 7 };
+void orc_energy(const orc_scene* s, double out[7]);
 
 void orc_default_params(orc_params* p);
 orc_scene* orc_create(const orc_params* p);
@@ -93,6 +102,8 @@ int orc_add_facet(orc_scene* s, const double verts[9], const double fakeVel[3], 
                   uint32_t mask);
 /* bulk: n free spheres (pos n*3, radius n), one material and mask; returns the first id */
 int orc_add_spheres(orc_scene* s, int64_t n, const double* pos, const double* radius, int mat, uint32_t mask);
+/* DemData.flags of particle i (blocked DOFs and the *Skip bits, Particle.hpp:196-199) */
+int orc_set_flags(orc_scene* s, int64_t i, uint32_t flags);
 
 int orc_num_particles(const orc_scene* s);
 /* bulk node data (one node per particle), n*3 or n*4 or n doubles */

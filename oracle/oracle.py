@@ -25,8 +25,11 @@ class OrcParams(C.Structure):
         ("poisson", C.c_double), ("en", C.c_double),
         ("noSlip", C.c_int), ("noBreak", C.c_int), ("noFrict", C.c_int), ("noAttraction", C.c_int),
         ("damping", C.c_double), ("reset", C.c_int),
-        ("dynDtPeriod", C.c_int), ("maxRelInc", C.c_double),
+        ("dynDtPeriod", C.c_int), ("maxRelInc", C.c_double), ("trackEnergy", C.c_int),
     ]
+
+# Scene.energy terms, in the order orc_energy and wdg_energy give them
+ENERGY_TERMS = ("kinetic", "elast", "grav", "nonviscDamp", "plast", "viscN", "viscT")
 
 
 def build_oracle():
@@ -67,6 +70,8 @@ def _load():
     L.orc_coord_number.argtypes = [P, C.c_int]; L.orc_coord_number.restype = C.c_double
     L.orc_solid_fraction.argtypes = [P, D, D]; L.orc_solid_fraction.restype = C.c_double
     L.orc_kinetic_energy.argtypes = [P]; L.orc_kinetic_energy.restype = C.c_double
+    L.orc_energy.argtypes = [P, C.POINTER(C.c_double)]
+    L.orc_set_flags.argtypes = [P, C.c_int64, C.c_uint32]
     L.orc_stress.argtypes = [P, C.c_double, D]; L.orc_stress.restype = C.c_int
     L.orc_pwave_dt.argtypes = [P]; L.orc_pwave_dt.restype = C.c_double
     L.orc_crit_dt.argtypes = [P]; L.orc_crit_dt.restype = C.c_double
@@ -132,6 +137,10 @@ class Oracle:
         fv = None if fakeVel is None else np.ascontiguousarray(np.asarray(fakeVel, dtype=np.float64).reshape(3))
         return self.L.orc_add_facet(self.h, _dp(v), None if fv is None else _dp(fv), float(halfThick), mat, mask)
 
+    def set_flags(self, i, flags):
+        if self.L.orc_set_flags(self.h, int(i), int(flags)) != 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
+
     @property
     def n(self):
         return self.L.orc_num_particles(self.h)
@@ -193,6 +202,12 @@ class Oracle:
 
     def kinetic_energy(self):
         return self.L.orc_kinetic_energy(self.h)
+
+    def energy(self):
+        """Scene.energy as a dict (trackEnergy=1 in the parameters)"""
+        out = (C.c_double * 7)()
+        self.L.orc_energy(self.h, out)
+        return dict(zip(ENERGY_TERMS, list(out)))
 
     def stress(self, volume=-1.):
         """Cauchy stress tensor (3x3) from the real contacts (DemFuncs::stressStiffness)."""
