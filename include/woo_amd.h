@@ -27,6 +27,9 @@
  *   wdg_observables    DemFuncs::coordNumber/porosity    src/dem/Funcs.cpp:84-114
  *   wdg_stress         DemFuncs::stressStiffness (stress) src/dem/Funcs.cpp:183-229
  *   wdg_kinetic_energy DemData kinetic energy             src/dem/Particle.cpp:247-265
+ *   wdg_set_energy_tracking / wdg_energy / wdg_energy_clear
+ *                      Scene.trackEnergy, Scene.energy (EnergyTracker)
+ *                                                        src/core/Scene.hpp:204,223, src/core/EnergyTracker.hpp:30-61
  *   wdg_run            Scene::doOneStep (engine loop)    src/core/Scene.cpp:366-467
  */
 #ifndef WOO_AMD_H
@@ -59,6 +62,12 @@ enum { WDG_LAW_IDEALELPL = 0, WDG_LAW_HERTZDMT = 1 };
 enum {
   WDG_F_POS = 0, WDG_F_VEL = 1, WDG_F_ANGVEL = 2, WDG_F_ORI = 3, WDG_F_FORCE = 4, WDG_F_TORQUE = 5,
   WDG_F_MASS = 6, WDG_F_INERTIA = 7
+};
+/* Scene.energy terms, in the order wdg_energy gives them. kinetic and elast are the last
+ * step's values (EnergyTracker::IsResettable); the others accumulate (IsIncrement). */
+enum {
+  WDG_EN_KINETIC = 0, WDG_EN_ELAST = 1, WDG_EN_GRAV = 2, WDG_EN_NONVISCDAMP = 3, WDG_EN_PLAST = 4,
+  WDG_EN_VISCN = 5, WDG_EN_VISCT = 6, WDG_EN_N = 7
 };
 /* engine codes for wdg_run */
 enum { WDG_E_FORCERESETTER = 1, WDG_E_COLLIDER = 2, WDG_E_CONTACTLOOP = 3, WDG_E_LEAPFROG = 4, WDG_E_DYNDT = 5 };
@@ -189,6 +198,18 @@ int wdg_set_precision(wdg_ctx* ctx, int32_t bits);
 int wdg_stress(wdg_ctx* ctx, double volume, double out[9]);
 /* sum over integrated nodes of m v.v/2 + w.(I w)/2 */
 int wdg_kinetic_energy(wdg_ctx* ctx, double* ek);
+/* Scene.trackEnergy (src/core/Scene.hpp:204): while on, wdg_run adds the terms of
+ * Leapfrog.cpp:53-98 (kinetic at the on-step velocity, grav, nonviscDamp; a periodic scene
+ * from step 1), IdealElPl.cpp:62-92 (elast, plast) and Hertz.cpp:181-206 (elast, plast,
+ * viscN, viscT) to the context's energy terms; each is a sum over the workgroups' partials
+ * in a fixed order (not ContactContainer order: the last bits differ from a serial sum).
+ * One scene per context only; wdg_run_graph and wdg_tune refuse a context that tracks.
+ * Turning it on does not clear the terms. */
+int wdg_set_energy_tracking(wdg_ctx* ctx, int32_t on);
+/* Scene.energy: out[WDG_EN_N] in WDG_EN_* order (synchronises the context's stream) */
+int wdg_energy(wdg_ctx* ctx, double out[7]);
+/* EnergyTracker.clear (src/core/EnergyTracker.hpp:60): every term to zero */
+int wdg_energy_clear(wdg_ctx* ctx);
 int wdg_stats(wdg_ctx* ctx, wdg_counters* out);
 
 /* run nSteps of Scene::doOneStep with a fixed engine list; *step and *dt are the

@@ -79,7 +79,7 @@ typedef struct {
   int trackEnergy;
 } orc_params;
 
-/* Scene.energy (EnergyTracker, src/core/EnergyTracker.hpp:30-60): the terms the path's engines
+/* Scene.energy (EnergyTracker, src/core/EnergyTracker.hpp:30-61): the terms the path's engines
  * add, in this order. kinetic and elast hold the last step's value (IsResettable); the others
  * accumulate (IsIncrement). */
 enum { ORC_E_KINETIC = 0, ORC_E_ELAST = 1, ORC_E_GRAV = 2, ORC_E_NONVISCDAMP = 3, ORC_E_PLAST = 4, ORC_E_VISCN = 5,

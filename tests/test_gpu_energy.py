@@ -1,0 +1,162 @@
+"""Energy tracking (Scene.energy) on the HIP engine against the CPU oracle.
+
+Both sides evaluate each contact's and each particle's terms with the same expressions in
+the same order; the engine sums them over workgroups in a fixed tree where the oracle sums
+serially, so the terms agree to summation roundoff: |gpu - cpu| <= 1e-10 * max(|cpu|,
+1e-3 * sum of |cpu| over the terms) for the FP64 engine. The FP32 contact form evaluates
+the law's terms from its float values (FP64 sums): 1e-3 on kinetic and elast, 2e-2 on the
+dissipations, over a short run.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import ENERGY_TERMS
+from tests.scenes import GpuRunner, build_gpu, build_oracle, contact_kw, deposition, drum, gpu_arrays, periodic_hertz
+
+pytestmark = pytest.mark.gpu
+
+ENG = ["Collider", "ContactLoop", "Leapfrog"]
+TOL = 1e-10
+
+
+def compare(eo, eg, tol=TOL, tol_diss=None):
+    tot = sum(abs(v) for v in eo.values())
+    worst = 0.
+    for k in ENERGY_TERMS:
+        t = tol if tol_diss is None or k in ("kinetic", "elast") else tol_diss
+        den = max(abs(eo[k]), 1e-3 * tot)
+        err = abs(eg[k] - eo[k]) / den if den > 0 else abs(eg[k])
+        assert err <= t, (k, eo[k], eg[k])
+        worst = max(worst, err)
+    return worst
+
+
+def run_pair(sc, steps, every, precision=64, **oracle_kw):
+    o = build_oracle(sc, trackEnergy=1, **oracle_kw)
+    g = GpuRunner(sc)
+    if precision != 64:
+        g.d.set_precision(precision)
+    g.d.set_energy_tracking(True)
+    worst = 0.
+    for _ in range(steps // every):
+        o.run(ENG, every)
+        g.run(ENG, every)
+        eo, eg = o.energy(), g.d.energy()
+        worst = max(worst, compare(eo, eg))
+    return o, g, worst
+
+
+def test_frict_deposition_with_damping():
+    # IdealElPl, walls, gravity, non-viscous damping .2: every term but viscN/viscT
+    sc = deposition(n=900, dense=True, seed=51, vel_sigma=0.05)
+    o, g, worst = run_pair(sc, 200, 25)
+    e = o.energy()
+    print("energy", e, "worst rel", worst)
+    assert e["elast"] > 0 and e["plast"] > 0 and e["nonviscDamp"] > 0 and e["grav"] != 0
+
+
+def test_hertz_periodic_viscous():
+    # Hertz with viscosity in a periodic cell: elast, plast, viscN, viscT; step 0 is not tracked
+    sc = periodic_hertz(n=1000, spacing=0.95, seed=52, en=.6)
+    o = build_oracle(sc, trackEnergy=1)
+    g = GpuRunner(sc)
+    g.d.set_energy_tracking(True)
+    o.run(ENG, 1)
+    g.run(ENG, 1)
+    assert o.energy()["kinetic"] == 0. and g.d.energy()["kinetic"] == 0.
+    compare(o.energy(), g.d.energy())
+    for _ in range(6):
+        o.run(ENG, 25)
+        g.run(ENG, 25)
+        compare(o.energy(), g.d.energy())
+    e = o.energy()
+    print("energy", e)
+    assert e["viscN"] > 0 and e["viscT"] > 0 and e["elast"] > 0 and e["kinetic"] > 0
+
+
+def test_drum_facets():
+    # rotating drum (facets with fakeVel, end walls), Hertz and gravity
+    sc = drum(n=1500, seed=53)
+    o, g, worst = run_pair(sc, 100, 20)
+    print("energy", o.energy(), "worst rel", worst)
+    assert o.energy()["elast"] > 0
+
+
+def test_program_loop_matches_oracle():
+    # wdg_run: the resettable terms are reset at each step's start (Scene.cpp:416)
+    sc = deposition(n=600, dense=True, seed=54, vel_sigma=0.05)
+    o = build_oracle(sc, trackEnergy=1)
+    o.run(ENG, 40)
+    g = GpuRunner(sc)
+    g.d.set_energy_tracking(True)
+    step, _ = g.d.run(ENG, 40, 0, sc["dt"], contact=dict(physKind=0, lawKind=0),
+                      leapfrog=dict(damping=sc["damping"]), dynDtPeriod=0)
+    assert step == 40
+    compare(o.energy(), g.d.energy())
+    # a program without the contact loop leaves elast at zero (reset, nothing added)
+    o.run(["Leapfrog"], 1)
+    g.d.run(["Leapfrog"], 1, 40, sc["dt"], leapfrog=dict(damping=sc["damping"]), dynDtPeriod=0)
+    assert o.energy()["elast"] == 0. and g.d.energy()["elast"] == 0.
+    compare(o.energy(), g.d.energy())
+
+
+def test_skip_flags():
+    # DemData.gravitySkip / dampingSkip / energySkip on some spheres (py/tests/test_energy.py:68-89)
+    sc = deposition(n=600, dense=True, seed=55, vel_sigma=0.05)
+    arr = gpu_arrays(sc)
+    nb = len(sc["walls"])
+    rng = np.random.default_rng(5)
+    sph = np.nonzero(arr["shape"] == 0)[0]
+    extra = np.zeros(len(arr["shape"]), np.uint32)
+    extra[sph] = rng.choice(np.array([0, 256, 512, 2048, 256 | 2048], np.uint32), size=len(sph))
+    arr["flags"] = arr["flags"] | extra
+    o = build_oracle(sc, trackEnergy=1)
+    for i in sph:
+        o.set_flags(int(i), int(arr["flags"][i]))
+    from woo_amd._native import Device
+    d = Device(0)
+    d.set_field(gravity=sc["gravity"], periodic=sc["periodic"], cell=sc["cell"])
+    d.set_materials(sc["materials"])
+    d.upload_particles(**arr)
+    d.set_energy_tracking(True)
+    o.run(ENG, 60)
+    d.run(ENG, 60, 0, sc["dt"], contact=dict(physKind=0, lawKind=0), leapfrog=dict(damping=sc["damping"]),
+          dynDtPeriod=0)
+    np.testing.assert_array_equal(d.get("pos"), o.get("pos"))
+    compare(o.energy(), d.energy())
+    assert nb > 0
+
+
+def test_fp32_form():
+    sc = deposition(n=900, dense=True, seed=56, vel_sigma=0.05)
+    o = build_oracle(sc, trackEnergy=1)
+    g = GpuRunner(sc)
+    g.d.set_precision(32)
+    g.d.set_energy_tracking(True)
+    o.run(ENG, 20)
+    g.run(ENG, 20)
+    eo, eg = o.energy(), g.d.energy()
+    print("oracle", eo, "\nfp32", eg)
+    compare(eo, eg, tol=1e-3, tol_diss=2e-2)
+
+
+def test_tracking_leaves_trajectory_and_refuses_graphs():
+    sc = deposition(n=500, dense=True, seed=57, vel_sigma=0.05)
+    g1, g2 = GpuRunner(sc), GpuRunner(sc)
+    g1.d.set_energy_tracking(True)
+    g1.run(ENG, 30)
+    g2.run(ENG, 30)
+    np.testing.assert_array_equal(g1.d.get("pos"), g2.d.get("pos"))
+    assert all(v == 0. for v in g2.d.energy().values())
+    kw = dict(contact=contact_kw(sc), leapfrog=dict(damping=sc["damping"]), dynDtPeriod=0)
+    with pytest.raises(RuntimeError, match="energy"):
+        g1.d.run(ENG, 4, 30, sc["dt"], graph=True, **kw)
+    with pytest.raises(RuntimeError, match="energy"):
+        g1.d.tune(ENG, 30, sc["dt"], **kw)
+    # cleared, and switched off: the terms stay as they are
+    g1.d.energy_clear()
+    assert all(v == 0. for v in g1.d.energy().values())
+    g1.d.set_energy_tracking(False)
+    g1.run(ENG, 5)
+    assert all(v == 0. for v in g1.d.energy().values())
+    build_gpu(sc)  # a fresh context tracks nothing

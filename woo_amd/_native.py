@@ -29,7 +29,11 @@ EXPORTS = [
     "wdg_mpnn_set_params", "wdg_mpnn_set_graph", "wdg_mpnn_forward", "wdg_mpnn_grad", "wdg_mpnn_adam",
     "wdg_mpnn_graph_from_ctx", "wdg_mpnn_advance", "wdg_snapshot", "wdg_restore", "wdg_snapshot_distance",
     "wdg_mpnn_set_targets", "wdg_mpnn_keep_graph", "wdg_mpnn_use_graph", "wdg_mpnn_get_graph",
+    "wdg_set_energy_tracking", "wdg_energy", "wdg_energy_clear",
 ]
+
+# Scene.energy terms, in WDG_EN_* order
+ENERGY_TERMS = ("kinetic", "elast", "grav", "nonviscDamp", "plast", "viscN", "viscT")
 
 
 class Field(C.Structure):
@@ -121,6 +125,9 @@ def lib():
     L.wdg_stats.argtypes = [V, _P(Counters)]
     L.wdg_stress.argtypes = [V, C.c_double, _P(C.c_double)]
     L.wdg_kinetic_energy.argtypes = [V, _P(C.c_double)]
+    L.wdg_set_energy_tracking.argtypes = [V, C.c_int32]
+    L.wdg_energy.argtypes = [V, _P(C.c_double)]
+    L.wdg_energy_clear.argtypes = [V]
     L.wdg_set_precision.argtypes = [V, C.c_int32]
     L.wdg_run.argtypes = [V, _P(Program), C.c_int64, _P(C.c_int64), _P(C.c_double)]
     L.wdg_relax.argtypes = [V, C.c_int32, C.c_double, C.c_double, _P(C.c_double), _P(C.c_int32)]
@@ -291,6 +298,19 @@ class Device:
         e = C.c_double()
         check(self.L.wdg_kinetic_energy(self.h, C.byref(e)))
         return e.value
+
+    def set_energy_tracking(self, on=True):
+        """Scene.trackEnergy: wdg_run adds the EnergyTracker terms while on"""
+        check(self.L.wdg_set_energy_tracking(self.h, int(bool(on))))
+
+    def energy(self):
+        """Scene.energy as a dict of the terms (ENERGY_TERMS)"""
+        out = np.empty(7, dtype=np.float64)
+        check(self.L.wdg_energy(self.h, dptr(out)))
+        return dict(zip(ENERGY_TERMS, out.tolist()))
+
+    def energy_clear(self):
+        check(self.L.wdg_energy_clear(self.h))
 
     def stats(self):
         s = Counters()

@@ -279,7 +279,7 @@ int wdg_ctx_destroy(wdg_ctx* ctx) {
     (void)hipStreamSynchronize(c.stream);
     for (auto& s : c.C) s.release();
     for (auto& r : c.R) r.release();
-    c.act.release(); c.desc.release(); c.descDist.release();
+    c.act.release(); c.desc.release(); c.descDist.release(); c.energy.release(); c.enPart.release();
     DBuf<double>* arrs[] = {c.pos, c.vel, c.angVel, c.force, c.torque, c.inertia, c.lastPos};
     for (auto* a : arrs) for (int k = 0; k < 3; k++) a[k].release();
     for (int k = 0; k < 4; k++) c.ori[k].release();
@@ -591,12 +591,16 @@ int wdg_set_scenes(wdg_ctx* ctx, int32_t nScenes, const double* dt) {
 
 int wdg_run_graph(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t* step, double* dt) {
   if (!ctx || !prog || !step || !dt || nSteps < 0) { g_err = "bad arguments"; return WDG_E_ARG; }
-  return guard(ctx, [&] { run_graph(ctx->c, prog, nSteps, step, dt); });
+  return guard(ctx, [&] {
+    if (ctx->c.trackEnergy) throw Error(WDG_E_STATE, "graph replay does not track energy (wdg_set_energy_tracking 0)");
+    run_graph(ctx->c, prog, nSteps, step, dt);
+  });
 }
 
 int wdg_tune(wdg_ctx* ctx, const wdg_program* prog, int64_t* step, double* dt, wdg_tuning* out, int64_t* stepsTaken) {
   if (!ctx || !prog || !step || !dt) { g_err = "bad arguments"; return WDG_E_ARG; }
   return guard(ctx, [&] {
+    if (ctx->c.trackEnergy) throw Error(WDG_E_STATE, "the tuner replays graphs, which do not track energy");
     const int64_t n = tune(ctx->c, prog, step, dt);
     if (out) *out = ctx->c.tune;
     if (stepsTaken) *stepsTaken = n;
@@ -627,6 +631,21 @@ int wdg_stress(wdg_ctx* ctx, double volume, double out[9]) {
 int wdg_kinetic_energy(wdg_ctx* ctx, double* ek) {
   if (!ctx || !ek) { g_err = "NULL argument"; return WDG_E_ARG; }
   return guard(ctx, [&] { *ek = kinetic_energy(ctx->c); });
+}
+
+int wdg_set_energy_tracking(wdg_ctx* ctx, int32_t on) {
+  if (!ctx) { g_err = "NULL argument"; return WDG_E_ARG; }
+  return guard(ctx, [&] { energy_enable(ctx->c, on != 0); });
+}
+
+int wdg_energy(wdg_ctx* ctx, double out[7]) {
+  if (!ctx || !out) { g_err = "NULL argument"; return WDG_E_ARG; }
+  return guard(ctx, [&] { energy_read(ctx->c, out); });
+}
+
+int wdg_energy_clear(wdg_ctx* ctx) {
+  if (!ctx) { g_err = "NULL argument"; return WDG_E_ARG; }
+  return guard(ctx, [&] { energy_clear(ctx->c); });
 }
 
 int wdg_stats(wdg_ctx* ctx, wdg_counters* out) {
@@ -661,8 +680,11 @@ int wdg_run(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t* step
   }
   return guard(ctx, [&] {
     Ctx& c = ctx->c;
+    if (c.trackEnergy && c.nScenes > 1) throw Error(WDG_E_STATE, "energy tracking needs a context of one scene");
     for (int64_t s = 0; s < nSteps; s++) {
       double nextDt = NAN;
+      c.stepNow = *step;
+      if (c.trackEnergy) energy_step_begin(c);
       for (int e = 0; e < prog->nEngines; e++) {
         switch (prog->engines[e]) {
           case WDG_E_FORCERESETTER: force_reset(c); break;

@@ -58,6 +58,7 @@ struct ContactArgs {
   double poisson, en, viscAlpha;
   int noSlip, noBreak, noFrict, noAttraction;
   uint32_t* dflags;
+  double* enPart;  // energy tracking: 4 partial sums per force workgroup
 };
 
 template <bool PERIODIC>
@@ -248,14 +249,18 @@ constexpr int CF_BS = 256;
 
 // FACETS: the scene has facets (a separate instantiation keeps the facet geometry's
 // registers out of the other scenes' kernel)
-template <bool PERIODIC, int LAW, bool FACETS>
+// ENERGY: the law's energy terms (elast, plast, viscN, viscT) of the workgroup's contacts are
+// summed into a.enPart[4 * blockIdx.x ..]; every thread reaches the sum (no early exit: graph
+// replay, the only user of gstop, does not track energy)
+template <bool PERIODIC, int LAW, bool FACETS, bool ENERGY>
 __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
-  if (a.gstop && *a.gstop) return;
+  if (!ENERGY && a.gstop && *a.gstop) return;
   const int renorm = a.renormPtr ? (int)*a.renormPtr : a.renorm;
   const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t nblk = (uint32_t)a.cfBlocks;
   const int64_t g = lb / nblk;
   const uint32_t cnt = a.ngcnt[g];
+  double en[4] = {0., 0., 0., 0.};
   for (uint32_t loc = (lb % nblk) * CF_BS + threadIdx.x; loc < cnt; loc += CF_BS * nblk) {
     const int64_t k = state_index((uint32_t)g, loc);
     // the filter's record of the slot: type bits, particles in role order, old state index
@@ -406,10 +411,18 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
           const double FtNorm = sqrt(Ft0 * Ft0 + Ft1 * Ft1);
           double ratio = maxFt / FtNorm;
           if (FtNorm == 0.) ratio = 0.;
+          // IdealElPl.cpp:62-73: slip dissipation, while Ft is the trial value
+          if (ENERGY) en[1] += (ph.kt != 0. ? maxFt * (FtNorm - maxFt) / ph.kt : 0.);
           Ft0 = Ft0 * ratio;
           Ft1 = Ft1 * ratio;
         }
         if (isnan(Fn) || isnan(Ft0) || isnan(Ft1)) atomicOr(&a.dflags[1], ERR_NAN_FORCE);
+      }
+      if (ENERGY) {
+        // IdealElPl.cpp:85-92: elastic potential, 0 when it is NaN
+        double el = 0.5 * (p2(Fn) / ph.kn + (ph.kt != 0. ? (Ft0 * Ft0 + Ft1 * Ft1) / ph.kt : 0.));
+        if (isnan(el)) el = 0.;
+        en[0] += el;
       }
     } else {
       // Law2_L6Geom_HertzPhys_DMT::go (Hertz.cpp:85-210), DMT branch; breaking decided by k_cfilter
@@ -424,6 +437,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
       double Fnv = eta * velN;
       if (ph.alpha == 0. && a.noAttraction && Fne + Fnv > 0) Fnv = -Fne;
       Fn = Fne + Fna + Fnv;
+      if (ENERGY) en[2] += Fnv * velN * dt;  // Hertz.cpp:181
       const double kt = ph.kt0 * sqrt(uN < 0 ? -uN : 0);
       const double dk = dt * kt;
       Ft0 = Ft0 + dk * gvel.y;
@@ -432,11 +446,18 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
       if (Ft0 * Ft0 + Ft1 * Ft1 > p2(maxFt)) {
         const double FtNorm = sqrt(Ft0 * Ft0 + Ft1 * Ft1);
         const double ratio = maxFt / FtNorm;
+        if (ENERGY) en[1] += (.5 * (FtNorm - maxFt) + maxFt) * (FtNorm - maxFt) / kt;  // Hertz.cpp:193
         Ft0 = Ft0 * ratio;
         Ft1 = Ft1 * ratio;
       } else {
         Ft0 = Ft0 + eta * gvel.y;
         Ft1 = Ft1 + eta * gvel.z;
+        if (ENERGY) en[3] += eta * (gvel.y * gvel.y + gvel.z * gvel.z) * dt;  // Hertz.cpp:199
+      }
+      if (ENERGY && uN < 0) {
+        // Hertz.cpp:203-206: kn0 (2/5) (-uN)^(5/2) with pown's square-and-multiply
+        const double q = s * s;
+        en[0] += kn0 * (2 / 5.) * ((q * q) * s) + 0.5 * (Ft0 * Ft0 + Ft1 * Ft1) / kt;
       }
     }
     a.N0[k] = T0.x; a.N1[k] = T0.y; a.N2[k] = T0.z;
@@ -452,6 +473,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
     g2[2] = make_double2(contPt.y, contPt.z);
     g2[3] = make_double2(__longlong_as_double((long long)cd), 0.);
   }
+  if (ENERGY) block_sum_store<4, CF_BS>(en, a.enPart + (int64_t)4 * blockIdx.x);
 }
 
 // ================================================================== FP32 contact form
@@ -545,7 +567,7 @@ __global__ __launch_bounds__(GROUP_BS) void k_cfilter32(ContactArgs a) {
   if (threadIdx.x == 0) a.ngcnt[g] = running;
 }
 
-template <bool PERIODIC, int LAW, bool FACETS>
+template <bool PERIODIC, int LAW, bool FACETS, bool ENERGY>
 __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
   const float *oN0 = reinterpret_cast<const float*>(a.oN0), *oN1 = reinterpret_cast<const float*>(a.oN1),
               *oN2 = reinterpret_cast<const float*>(a.oN2), *oY0 = reinterpret_cast<const float*>(a.oY0),
@@ -556,12 +578,14 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
         *N2 = reinterpret_cast<float*>(a.N2), *Y0 = reinterpret_cast<float*>(a.Y0), *Y1 = reinterpret_cast<float*>(a.Y1),
         *Y2 = reinterpret_cast<float*>(a.Y2), *F0 = reinterpret_cast<float*>(a.F0), *F1 = reinterpret_cast<float*>(a.F1),
         *F2 = reinterpret_cast<float*>(a.F2), *UN = reinterpret_cast<float*>(a.uN), *gp = reinterpret_cast<float*>(a.gp);
-  if (a.gstop && *a.gstop) return;
+  if (!ENERGY && a.gstop && *a.gstop) return;
   const int renorm = a.renormPtr ? (int)*a.renormPtr : a.renorm;
   const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t nblk = (uint32_t)a.cfBlocks;
   const int64_t g = lb / nblk;
   const uint32_t cnt = a.ngcnt[g];
+  // energy terms from the float law's values, evaluated and summed in FP64
+  double en[4] = {0., 0., 0., 0.};
   for (uint32_t loc = (lb % nblk) * CF_BS + threadIdx.x; loc < cnt; loc += CF_BS * nblk) {
     const int64_t k = state_index((uint32_t)g, loc);
     // the filter's record of the slot: type bits, particles in role order, old state index
@@ -697,10 +721,17 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
           const float FtNorm = sqrtf(Ft0 * Ft0 + Ft1 * Ft1);
           float ratio = maxFt / FtNorm;
           if (FtNorm == 0.f) ratio = 0.f;
+          if (ENERGY) en[1] += (kt != 0.f ? (double)maxFt * ((double)FtNorm - maxFt) / kt : 0.);
           Ft0 = Ft0 * ratio;
           Ft1 = Ft1 * ratio;
         }
         if (isnan(Fn) || isnan(Ft0) || isnan(Ft1)) atomicOr(&a.dflags[1], ERR_NAN_FORCE);
+      }
+      if (ENERGY) {
+        const double f0 = Fn, f1 = Ft0, f2 = Ft1;
+        double el = 0.5 * (f0 * f0 / kn + (kt != 0.f ? (f1 * f1 + f2 * f2) / kt : 0.));
+        if (isnan(el)) el = 0.;
+        en[0] += el;
       }
     } else {
       PhysVals ph;
@@ -715,6 +746,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
       float Fnv = eta * velN;
       if (ph.alpha == 0. && a.noAttraction && Fne + Fnv > 0.f) Fnv = -Fne;
       Fn = Fne + Fna + Fnv;
+      if (ENERGY) en[2] += (double)Fnv * velN * dt;
       const float kt = (float)ph.kt0 * sqrtf(uN < 0.f ? -uN : 0.f);
       const float dk = dt * kt;
       Ft0 = Ft0 + dk * gvel.y;
@@ -723,11 +755,20 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
       if (Ft0 * Ft0 + Ft1 * Ft1 > maxFt * maxFt) {
         const float FtNorm = sqrtf(Ft0 * Ft0 + Ft1 * Ft1);
         const float ratio = maxFt / FtNorm;
+        if (ENERGY) {
+          const double d = (double)FtNorm - maxFt;
+          en[1] += (.5 * d + maxFt) * d / kt;
+        }
         Ft0 = Ft0 * ratio;
         Ft1 = Ft1 * ratio;
       } else {
         Ft0 = Ft0 + eta * gvel.y;
         Ft1 = Ft1 + eta * gvel.z;
+        if (ENERGY) en[3] += (double)eta * ((double)gvel.y * gvel.y + (double)gvel.z * gvel.z) * dt;
+      }
+      if (ENERGY && uN < 0.f) {
+        const double sd = sqrt(-(double)uN), q = sd * sd, f1 = Ft0, f2 = Ft1;
+        en[0] += ph.K * sqrt(ph.R) * (2 / 5.) * ((q * q) * sd) + 0.5 * (f1 * f1 + f2 * f2) / kt;
       }
     }
     N0[k] = T0.x; N1[k] = T0.y; N2[k] = T0.z;
@@ -740,6 +781,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
     g4[1] = make_float4(cA.y, cA.z, cB.x, cB.y);
     g4[2] = make_float4(cB.z, 0.f, 0.f, 0.f);
   }
+  if (ENERGY) block_sum_store<4, CF_BS>(en, a.enPart + (int64_t)4 * blockIdx.x);
 }
 
 static ContactArgs contact_args(Ctx& c, int from, int to) {
@@ -783,6 +825,33 @@ static ContactArgs contact_args(Ctx& c, int from, int to) {
   return a;
 }
 
+template <bool E>
+static void launch_cforce(Ctx& c, const ContactArgs& a, dim3 grid, bool f32, bool hertz, bool per, bool fac) {
+  const dim3 block(CF_BS);
+  if (f32) {
+    if (hertz) {
+      if (per) hipLaunchKernelGGL((k_cforce32<true, WDG_LAW_HERTZDMT, false, E>), grid, block, 0, c.stream, a);
+      else if (fac) hipLaunchKernelGGL((k_cforce32<false, WDG_LAW_HERTZDMT, true, E>), grid, block, 0, c.stream, a);
+      else hipLaunchKernelGGL((k_cforce32<false, WDG_LAW_HERTZDMT, false, E>), grid, block, 0, c.stream, a);
+    } else {
+      if (per) hipLaunchKernelGGL((k_cforce32<true, WDG_LAW_IDEALELPL, false, E>), grid, block, 0, c.stream, a);
+      else if (fac) hipLaunchKernelGGL((k_cforce32<false, WDG_LAW_IDEALELPL, true, E>), grid, block, 0, c.stream, a);
+      else hipLaunchKernelGGL((k_cforce32<false, WDG_LAW_IDEALELPL, false, E>), grid, block, 0, c.stream, a);
+    }
+  } else {
+    if (hertz) {
+      if (per) hipLaunchKernelGGL((k_cforce<true, WDG_LAW_HERTZDMT, false, E>), grid, block, 0, c.stream, a);
+      else if (fac) hipLaunchKernelGGL((k_cforce<false, WDG_LAW_HERTZDMT, true, E>), grid, block, 0, c.stream, a);
+      else hipLaunchKernelGGL((k_cforce<false, WDG_LAW_HERTZDMT, false, E>), grid, block, 0, c.stream, a);
+    } else {
+      if (per) hipLaunchKernelGGL((k_cforce<true, WDG_LAW_IDEALELPL, false, E>), grid, block, 0, c.stream, a);
+      else if (fac) hipLaunchKernelGGL((k_cforce<false, WDG_LAW_IDEALELPL, true, E>), grid, block, 0, c.stream, a);
+      else hipLaunchKernelGGL((k_cforce<false, WDG_LAW_IDEALELPL, false, E>), grid, block, 0, c.stream, a);
+    }
+  }
+  WDG_LAUNCH_CHECK();
+}
+
 void contact_loop(Ctx& c, const wdg_contact_params* p) {
   Contacts& C = c.con();
   if (c.mats.empty() || !c.dMats.p) throw Error(WDG_E_STATE, "no materials set (wdg_set_materials)");
@@ -799,6 +868,7 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
                                ? -std::sqrt(5) * std::log(p->en) /
                                      (std::sqrt(std::log(p->en) * std::log(p->en) + M_PI * M_PI))
                                : 0.;
+  c.stepNow = p->step;
   c.lastPhysKind = p->physKind;
   c.lastPoisson = p->poisson;
   c.lastEn = p->en;
@@ -825,7 +895,6 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
   const bool hertz = p->lawKind == WDG_LAW_HERTZDMT;
   const dim3 gfil((unsigned)ng), bfil(GROUP_BS);
   if (c.precision == 32) {
-    const dim3 gforce((unsigned)(ng * a.cfBlocks)), bforce(CF_BS);
     if (hertz) {
       if (per) hipLaunchKernelGGL((k_cfilter32<true, WDG_LAW_HERTZDMT>), gfil, bfil, 0, c.stream, a);
       else hipLaunchKernelGGL((k_cfilter32<false, WDG_LAW_HERTZDMT>), gfil, bfil, 0, c.stream, a);
@@ -833,41 +902,27 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
       if (per) hipLaunchKernelGGL((k_cfilter32<true, WDG_LAW_IDEALELPL>), gfil, bfil, 0, c.stream, a);
       else hipLaunchKernelGGL((k_cfilter32<false, WDG_LAW_IDEALELPL>), gfil, bfil, 0, c.stream, a);
     }
-    WDG_LAUNCH_CHECK();
+  } else {
     if (hertz) {
-      if (per) hipLaunchKernelGGL((k_cforce32<true, WDG_LAW_HERTZDMT, false>), gforce, bforce, 0, c.stream, a);
-      else if (fac) hipLaunchKernelGGL((k_cforce32<false, WDG_LAW_HERTZDMT, true>), gforce, bforce, 0, c.stream, a);
-      else hipLaunchKernelGGL((k_cforce32<false, WDG_LAW_HERTZDMT, false>), gforce, bforce, 0, c.stream, a);
+      if (per) hipLaunchKernelGGL((k_cfilter<true, WDG_LAW_HERTZDMT>), gfil, bfil, 0, c.stream, a);
+      else hipLaunchKernelGGL((k_cfilter<false, WDG_LAW_HERTZDMT>), gfil, bfil, 0, c.stream, a);
     } else {
-      if (per) hipLaunchKernelGGL((k_cforce32<true, WDG_LAW_IDEALELPL, false>), gforce, bforce, 0, c.stream, a);
-      else if (fac) hipLaunchKernelGGL((k_cforce32<false, WDG_LAW_IDEALELPL, true>), gforce, bforce, 0, c.stream, a);
-      else hipLaunchKernelGGL((k_cforce32<false, WDG_LAW_IDEALELPL, false>), gforce, bforce, 0, c.stream, a);
+      if (per) hipLaunchKernelGGL((k_cfilter<true, WDG_LAW_IDEALELPL>), gfil, bfil, 0, c.stream, a);
+      else hipLaunchKernelGGL((k_cfilter<false, WDG_LAW_IDEALELPL>), gfil, bfil, 0, c.stream, a);
     }
-    WDG_LAUNCH_CHECK();
-    c.R[to].sidxValid = true;
-    c.rcur = to;
-    c.forcesPending = p->applyForces != 0;
-    return;
-  }
-  if (hertz) {
-    if (per) hipLaunchKernelGGL((k_cfilter<true, WDG_LAW_HERTZDMT>), gfil, bfil, 0, c.stream, a);
-    else hipLaunchKernelGGL((k_cfilter<false, WDG_LAW_HERTZDMT>), gfil, bfil, 0, c.stream, a);
-  } else {
-    if (per) hipLaunchKernelGGL((k_cfilter<true, WDG_LAW_IDEALELPL>), gfil, bfil, 0, c.stream, a);
-    else hipLaunchKernelGGL((k_cfilter<false, WDG_LAW_IDEALELPL>), gfil, bfil, 0, c.stream, a);
   }
   WDG_LAUNCH_CHECK();
-  const dim3 gforce((unsigned)(ng * a.cfBlocks)), bforce(CF_BS);
-  if (hertz) {
-    if (per) hipLaunchKernelGGL((k_cforce<true, WDG_LAW_HERTZDMT, false>), gforce, bforce, 0, c.stream, a);
-    else if (fac) hipLaunchKernelGGL((k_cforce<false, WDG_LAW_HERTZDMT, true>), gforce, bforce, 0, c.stream, a);
-    else hipLaunchKernelGGL((k_cforce<false, WDG_LAW_HERTZDMT, false>), gforce, bforce, 0, c.stream, a);
+  const dim3 gforce((unsigned)(ng * a.cfBlocks));
+  if (c.trackEnergy) {
+    c.enPart.ensure((size_t)4 * gforce.x);
+    a.enPart = c.enPart.p;
+    launch_cforce<true>(c, a, gforce, c.precision == 32, hertz, per, fac);
+    // EnergyTracker: elast is resettable (this step's sum replaces it), the dissipations accumulate
+    static const int tgt[4] = {WDG_EN_ELAST, WDG_EN_PLAST, WDG_EN_VISCN, WDG_EN_VISCT};
+    energy_add(c, gforce.x, 4, tgt, 1);
   } else {
-    if (per) hipLaunchKernelGGL((k_cforce<true, WDG_LAW_IDEALELPL, false>), gforce, bforce, 0, c.stream, a);
-    else if (fac) hipLaunchKernelGGL((k_cforce<false, WDG_LAW_IDEALELPL, true>), gforce, bforce, 0, c.stream, a);
-    else hipLaunchKernelGGL((k_cforce<false, WDG_LAW_IDEALELPL, false>), gforce, bforce, 0, c.stream, a);
+    launch_cforce<false>(c, a, gforce, c.precision == 32, hertz, per, fac);
   }
-  WDG_LAUNCH_CHECK();
   c.R[to].sidxValid = true;
   c.rcur = to;
   c.forcesPending = p->applyForces != 0;

@@ -182,13 +182,20 @@ struct LeapArgs {
   const double* dtScene;  // batched scenes: dt of each particle's scene (else null, dt above)
   const uint32_t* scene;
   const uint32_t* gstop;  // graph replay: a set stop word skips the kernel
+  double* enPart;         // energy tracking: kinetic, grav, nonviscDamp partials per workgroup
 };
 
-template <bool PERIODIC, bool F32>
+// std::max's comparison, so that a NaN propagates as in the reference's isinf(maxCoeff)
+__device__ __forceinline__ double max_cmp(double x, double y) { return x < y ? y : x; }
+
+// ENERGY: Leapfrog::doDampingDissipation, doGravityWork and doKineticEnergy (Leapfrog.cpp:50-98,
+// called at Leapfrog.cpp:156-159 and 218) summed over the workgroup into a.enPart[3 * blockIdx.x ..]
+template <bool PERIODIC, bool F32, bool ENERGY>
 __global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
-  if (a.gstop && *a.gstop) return;
+  if (!ENERGY && a.gstop && *a.gstop) return;
   const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   bool dirty = false;
+  double en[3] = {0., 0., 0.};
   if (i < a.n) {
     const uint32_t fl = a.flags[i];
     const bool integrated = (fl & F_INTEGRATED) != 0u;
@@ -219,6 +226,27 @@ __global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
       const double m = a.mass[i];
       d3 vel = mk3(a.vx[i], a.vy[i], a.vz[i]);
       d3 av = mk3(a.wx[i], a.wy[i], a.wz[i]);
+      // on-step velocities v(t) = v(t-dt/2) + a dt/2 for the kinetic energy (zero when all
+      // DOFs are blocked)
+      d3 cv = mk3(0., 0., 0.), cw = mk3(0., 0., 0.);
+      if (ENERGY) {
+        if (damp && !(fl & F_ENERGY_SKIP)) {
+          const double fv = (fabs(vel.x) * fabs(F.x) + fabs(vel.y) * fabs(F.y)) + fabs(vel.z) * fabs(F.z);
+          const double tw = (fabs(av.x) * fabs(T.x) + fabs(av.y) * fabs(T.y)) + fabs(av.z) * fabs(T.z);
+          en[2] = fv * damping * dt + tw * damping * dt;
+        }
+        if (grav) {
+          if (bl == 0) {
+            en[1] = -((a.g[0] * vel.x + a.g[1] * vel.y) + a.g[2] * vel.z) * m * dt;
+          } else {
+            const double v[3] = {vel.x, vel.y, vel.z};
+            double gr = 0.;
+            for (int k = 0; k < 3; k++)
+              if (!(bl & (1u << k))) gr -= a.g[k] * v[k] * m * dt;
+            en[1] = gr;
+          }
+        }
+      }
       if (bl != F_DOF_ALL) {
         double f[3] = {F.x, F.y, F.z}, t[3] = {T.x, T.y, T.z};
         double v[3] = {vel.x, vel.y, vel.z}, w[3] = {av.x, av.y, av.z};
@@ -239,9 +267,22 @@ __global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
           if (damp)
             for (int k = 0; k < 3; k++) aa[k] *= 1 - damping * sgn(t[k] * (w[k] + 0.5 * dt * aa[k]));
           for (int k = 0; k < 3; k++) w[k] += dt * aa[k];
+        } else {
+          aa[0] = aa[1] = aa[2] = 0.;
+        }
+        if (ENERGY) {
+          cv = mk3(vel.x + .5 * dt * la[0], vel.y + .5 * dt * la[1], vel.z + .5 * dt * la[2]);
+          cw = mk3(av.x + .5 * dt * aa[0], av.y + .5 * dt * aa[1], av.z + .5 * dt * aa[2]);
         }
         vel = mk3(v[0], v[1], v[2]);
         av = mk3(w[0], w[1], w[2]);
+      }
+      if (ENERGY && !(fl & F_ENERGY_SKIP)) {
+        const double Ix = a.ix[i], Iy = a.iy[i], Iz = a.iz[i];
+        const double Et = .5 * m * ((cv.x * cv.x + cv.y * cv.y) + cv.z * cv.z);
+        double Er = 0.5 * ((cw.x * (Ix * cw.x) + cw.y * (Iy * cw.y)) + cw.z * (Iz * cw.z));
+        if (isnan(Er) || isinf(max_cmp(max_cmp(Ix, Iy), Iz))) Er = 0;
+        en[0] = Et + Er;
       }
       pos = mk3(pos.x + vel.x * dt, pos.y + vel.y * dt, pos.z + vel.z * dt);
       a.px[i] = pos.x; a.py[i] = pos.y; a.pz[i] = pos.z;
@@ -281,6 +322,7 @@ __global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) *a.dirtyNext = 0u;
   if (__ballot(dirty) != 0ull && (threadIdx.x & 63) == 0) atomicOr(a.dirty, 1u);
+  if (ENERGY) block_sum_store<3, 256>(en, a.enPart + (int64_t)3 * blockIdx.x);
 }
 
 __global__ void k_force_reset(int64_t n, const uint32_t* __restrict__ flags, const double* __restrict__ mass,
@@ -363,10 +405,25 @@ void leapfrog(Ctx& c, const wdg_leapfrog_params* p) {
   GatherArgs g = gather_args(c);
   if (a.gather) gather_boundary(c, g);
   dim3 grid(nblocks(c.n, 256)), block(256);
-  auto* kern = c.precision == 32 ? k_leapfrog<false, true>
-                                 : (c.field.periodic ? k_leapfrog<true, false> : k_leapfrog<false, false>);
-  hipLaunchKernelGGL(kern, grid, block, 0, c.stream, a, g);
-  WDG_LAUNCH_CHECK();
+  // Leapfrog.cpp:132: a periodic scene is not tracked at step 0 (gradV is zero here)
+  const bool track = c.trackEnergy && (!c.field.periodic || c.stepNow > 0);
+  if (track) {
+    c.enPart.ensure((size_t)3 * grid.x);
+    a.enPart = c.enPart.p;
+    auto* kern = c.precision == 32 ? k_leapfrog<false, true, true>
+                                   : (c.field.periodic ? k_leapfrog<true, false, true> : k_leapfrog<false, false, true>);
+    hipLaunchKernelGGL(kern, grid, block, 0, c.stream, a, g);
+    WDG_LAUNCH_CHECK();
+    // kinetic is resettable: this step's sum replaces it (the same as the reset at the step's
+    // start and one add); grav and nonviscDamp accumulate
+    static const int tgt[3] = {WDG_EN_KINETIC, WDG_EN_GRAV, WDG_EN_NONVISCDAMP};
+    energy_add(c, grid.x, 3, tgt, 1);
+  } else {
+    auto* kern = c.precision == 32 ? k_leapfrog<false, true, false>
+                                   : (c.field.periodic ? k_leapfrog<true, false, false> : k_leapfrog<false, false, false>);
+    hipLaunchKernelGGL(kern, grid, block, 0, c.stream, a, g);
+    WDG_LAUNCH_CHECK();
+  }
   c.lfParity = 1 - s;
   c.dirtyIdx = DIRTY_PP + s;
   c.forcesPending = false;
@@ -834,6 +891,72 @@ int64_t count_real(Ctx& c) {
   int64_t s = 0;
   for (uint32_t x : g) s += x;
   return s;
+}
+
+// ------------------------------------------------------------------ energy tracking (EnergyTracker)
+// one workgroup: thread t sums the partials of workgroups t, t+256, ... in order, then the
+// threads' sums are added in a fixed tree; run to run the result is the same
+// (bit q of setMask: the sum replaces energy[target q] instead of being added to it)
+__global__ __launch_bounds__(256) void k_energy_add(const double* __restrict__ part, int64_t nBlocks, int nTerms,
+                                                    int4 target, int setMask, double* energy) {
+  __shared__ double sh[256];
+  for (int q = 0; q < nTerms; q++) {
+    double s = 0.;
+    for (int64_t b = threadIdx.x; b < nBlocks; b += 256) s += part[b * nTerms + q];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const int t = q == 0 ? target.x : q == 1 ? target.y : q == 2 ? target.z : target.w;
+      energy[t] = ((setMask >> q) & 1) ? sh[0] : energy[t] + sh[0];
+    }
+    __syncthreads();
+  }
+}
+
+// Scene.cpp:416 (EnergyTracker::resetResettables): kinetic and elast start every step at zero
+__global__ void k_energy_reset(double* energy, int clearAll) {
+  const int t = threadIdx.x;
+  if (t < WDG_EN_N && (clearAll || t == WDG_EN_KINETIC || t == WDG_EN_ELAST)) energy[t] = 0.;
+}
+
+void energy_enable(Ctx& c, bool on) {
+  if (on && !c.energy.p) {
+    c.energy.ensure(WDG_EN_N);
+    energy_clear(c);
+  }
+  c.trackEnergy = on;
+}
+
+void energy_clear(Ctx& c) {
+  if (!c.energy.p) return;
+  hipLaunchKernelGGL(k_energy_reset, dim3(1), dim3(64), 0, c.stream, c.energy.p, 1);
+  WDG_LAUNCH_CHECK();
+}
+
+void energy_step_begin(Ctx& c) {
+  hipLaunchKernelGGL(k_energy_reset, dim3(1), dim3(64), 0, c.stream, c.energy.p, 0);
+  WDG_LAUNCH_CHECK();
+}
+
+void energy_add(Ctx& c, int64_t nBlocks, int nTerms, const int* target, int setMask) {
+  if (nBlocks <= 0) return;
+  const int4 t = make_int4(target[0], nTerms > 1 ? target[1] : 0, nTerms > 2 ? target[2] : 0, nTerms > 3 ? target[3] : 0);
+  hipLaunchKernelGGL(k_energy_add, dim3(1), dim3(256), 0, c.stream, c.enPart.p, nBlocks, nTerms, t, setMask,
+                     c.energy.p);
+  WDG_LAUNCH_CHECK();
+}
+
+void energy_read(Ctx& c, double out[7]) {
+  if (!c.energy.p) {
+    for (int q = 0; q < WDG_EN_N; q++) out[q] = 0.;
+    return;
+  }
+  WDG_HIP(hipMemcpyAsync(out, c.energy.p, WDG_EN_N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  WDG_HIP(hipStreamSynchronize(c.stream));
 }
 
 }  // namespace wdg

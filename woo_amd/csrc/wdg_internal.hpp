@@ -36,6 +36,7 @@ struct Error : std::runtime_error {
 // low bits are DemData.flags (Particle.hpp:196-199); high bits are engine-private
 enum : uint32_t {
   F_DOF_ALL = 63u,
+  F_ENERGY_SKIP = 256u,
   F_GRAVITY_SKIP = 512u,
   F_DAMPING_SKIP = 2048u,
   F_INTEGRATED = 1u << 16,
@@ -273,6 +274,13 @@ struct Ctx {
   uint64_t snapUpload[NSNAP] = {0, 0, 0, 0};
   uint64_t uploads = 0;   // particle uploads so far
   uint64_t serial = 0;    // unique per context of the process
+  // Scene.energy (EnergyTracker) while trackEnergy is set: the terms in WDG_EN_* order, and
+  // per-workgroup partial sums of the kernel that last produced them
+  bool trackEnergy = false;
+  int64_t stepNow = 0;    // Scene.step of the step being run, from wdg_run or the last contact
+                          // loop (Leapfrog does not track a periodic scene's step 0)
+  DBuf<double> energy;
+  DBuf<double> enPart;
   DBuf<double> red;       // reductions
   DBuf<double> xfer;      // user-id-ordered staging for upload/download
   DBuf<uint32_t> dflags;  // [0] dirty, [1] error bits, [2..] scratch counters
@@ -335,6 +343,34 @@ void drop_contacts(Ctx& c);
 void snapshot_save(Ctx& c, int slot);
 void snapshot_load(Ctx& c, int slot);
 double snapshot_distance(Ctx& c, int slot);
+void energy_enable(Ctx& c, bool on);
+void energy_clear(Ctx& c);
+void energy_step_begin(Ctx& c);
+// adds the sums over nBlocks workgroup partials of nTerms terms to energy[target[q]] (bit q of
+// setMask: stores it instead)
+void energy_add(Ctx& c, int64_t nBlocks, int nTerms, const int* target, int setMask);
+void energy_read(Ctx& c, double out[7]);
+
+// the sum over a workgroup of BS threads of NT values per thread, stored by thread 0..NT-1 at
+// out[0..NT-1]; a fixed order (butterfly in the wavefront, then wavefronts in order), so the
+// result does not vary from run to run. Every thread of the workgroup must call it.
+template <int NT, int BS>
+__device__ __forceinline__ void block_sum_store(double (&v)[NT], double* out) {
+  __shared__ double sh[BS / 64][NT];
+#pragma unroll
+  for (int q = 0; q < NT; q++)
+    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < NT; q++) sh[w][q] = v[q];
+  __syncthreads();
+  if (threadIdx.x < NT) {
+    double s = 0.;
+    for (int k = 0; k < BS / 64; k++) s += sh[k][threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
