@@ -24,6 +24,8 @@
  *                                                        src/dem/ParticleContainer.cpp:176-201, src/dem/Particle.hpp:281-292
  *   wdg_contact_count / wdg_contact_list
  *                      ContactContainer size / linView   src/dem/ContactContainer.hpp:16-141
+ *   wdg_restore_contacts
+ *                      ContactContainer of a loaded Scene src/core/Scene.cpp:285-294
  *   wdg_observables    DemFuncs::coordNumber/porosity    src/dem/Funcs.cpp:84-114
  *   wdg_stress         DemFuncs::stressStiffness (stress) src/dem/Funcs.cpp:183-229
  *   wdg_kinetic_energy DemData kinetic energy             src/dem/Particle.cpp:247-265
@@ -185,6 +187,18 @@ int wdg_crit_dt(wdg_ctx* ctx, double* critDt);
  * Sorted by (min id, max id). Any output pointer may be NULL. */
 int wdg_contact_count(wdg_ctx* ctx, int32_t which, int64_t* n);
 int wdg_contact_list(wdg_ctx* ctx, int32_t which, int32_t* ids, int32_t* cellDist, double* data);
+/* Load half of a checkpoint of the contact history: the ContactContainer saved with the Scene
+ * (Scene::boostSave / saveTmp, src/core/Scene.cpp:285-294; per contact Contact.geom L6Geom
+ * uN and trsf, src/dem/L6Geom.hpp:18-21, the contact point Contact.geom.node, Contact.phys
+ * force, src/dem/Contact.hpp:21,32). The save half is wdg_contact_list(which=1) and
+ * wdg_download of the node fields. Call after wdg_upload_particles with the saved node state:
+ * it drops the current contacts, runs a full collider refresh (verletDist as in
+ * wdg_collide_params) and makes each listed contact real with its saved state. ids n*2 in role
+ * order, cellDist n*3 (periodic cells; may be NULL), data n*16 in the wdg_contact_list layout
+ * (kn, kt and the real flag are not read). FP64 contact form only. Stepping on from the
+ * restored state gives the trajectory of the uninterrupted run, bit for bit. */
+int wdg_restore_contacts(wdg_ctx* ctx, int64_t n, const int32_t* ids, const int32_t* cellDist, const double* data,
+                         double verletDist);
 
 /* coordNumber (skipFree=0 and 1) and solid fraction in a box */
 int wdg_observables(wdg_ctx* ctx, const double boxMin[3], const double boxMax[3], double out[3]);

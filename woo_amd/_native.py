@@ -29,7 +29,7 @@ EXPORTS = [
     "wdg_mpnn_set_params", "wdg_mpnn_set_graph", "wdg_mpnn_forward", "wdg_mpnn_grad", "wdg_mpnn_adam",
     "wdg_mpnn_graph_from_ctx", "wdg_mpnn_advance", "wdg_snapshot", "wdg_restore", "wdg_snapshot_distance",
     "wdg_mpnn_set_targets", "wdg_mpnn_keep_graph", "wdg_mpnn_use_graph", "wdg_mpnn_get_graph",
-    "wdg_set_energy_tracking", "wdg_energy", "wdg_energy_clear",
+    "wdg_set_energy_tracking", "wdg_energy", "wdg_energy_clear", "wdg_restore_contacts",
 ]
 
 # Scene.energy terms, in WDG_EN_* order
@@ -128,6 +128,7 @@ def lib():
     L.wdg_set_energy_tracking.argtypes = [V, C.c_int32]
     L.wdg_energy.argtypes = [V, _P(C.c_double)]
     L.wdg_energy_clear.argtypes = [V]
+    L.wdg_restore_contacts.argtypes = [V, C.c_int64, _P(C.c_int32), _P(C.c_int32), _P(C.c_double), C.c_double]
     L.wdg_set_precision.argtypes = [V, C.c_int32]
     L.wdg_run.argtypes = [V, _P(Program), C.c_int64, _P(C.c_int64), _P(C.c_double)]
     L.wdg_relax.argtypes = [V, C.c_int32, C.c_double, C.c_double, _P(C.c_double), _P(C.c_int32)]
@@ -166,6 +167,8 @@ class Device:
         self.h = C.c_void_p()
         check(self.L.wdg_ctx_create(device, C.byref(self.h)))
         self.n = 0
+        # what was set up, kept for checkpoint()
+        self._field, self._mats, self._static, self._scenes = None, None, None, None
 
     def close(self):
         if self.h:
@@ -186,12 +189,15 @@ class Device:
         f.periodic = int(bool(periodic))
         f.cell[:] = list(cell)
         check(self.L.wdg_set_field(self.h, C.byref(f)))
+        self._field = dict(gravity=[float(x) for x in gravity], distFactor=float(distFactor), loneMask=int(loneMask),
+                           periodic=bool(periodic), cell=[float(x) for x in cell])
 
     def set_materials(self, mats):
         arr = (Material * max(1, len(mats)))()
         for i, m in enumerate(mats):
             arr[i] = Material(**m)
         check(self.L.wdg_set_materials(self.h, arr, len(mats)))
+        self._mats = [{k: (float(v) if k != "kind" else int(v)) for k, v in m.items()} for m in mats]
 
     def upload_particles(self, shape, pos, mass, inertia, radius=None, wallAxis=None, wallSense=None, material=None,
                          mask=None, flags=None, integrated=None, vel=None, angVel=None, ori=None, force=None,
@@ -232,6 +238,10 @@ class Device:
         p.scene = iptr(arr(scene, np.int32))
         check(self.L.wdg_upload_particles(self.h, C.byref(p)))
         self.n = n
+        static = dict(shape=shape, radius=radius, wallAxis=wallAxis, wallSense=wallSense, material=material, mask=mask,
+                      flags=flags, integrated=integrated, facetVerts=facetVerts, fakeVel=fakeVel, halfThick=halfThick,
+                      scene=scene)
+        self._static = {k: np.array(v) for k, v in static.items() if v is not None}
 
     def get(self, field):
         w = FIELD_W[field]
@@ -276,6 +286,49 @@ class Device:
         d = np.empty((n, 16), dtype=np.float64) if data else None
         check(self.L.wdg_contact_list(self.h, w, iptr(ids), iptr(cd), dptr(d)))
         return ids, cd, d
+
+    def restore_contacts(self, ids, cellDist, data, verletDist=-.05):
+        """wdg_restore_contacts: a full collider refresh, then the listed contacts are real again with
+        their saved state (the arrays of contacts(real_only=True))"""
+        ids = np.ascontiguousarray(ids, dtype=np.int32).reshape(-1, 2)
+        cd = np.ascontiguousarray(cellDist, dtype=np.int32).reshape(-1, 3) if cellDist is not None else None
+        d = np.ascontiguousarray(data, dtype=np.float64).reshape(-1, 16)
+        check(self.L.wdg_restore_contacts(self.h, len(ids), iptr(ids), iptr(cd) if cd is not None else None, dptr(d),
+                                          float(verletDist)))
+
+    def checkpoint(self):
+        """The context's state as a dict of numpy arrays (for np.savez): the set-up (field,
+        materials, particle shapes and flags, scenes), the node fields and the real contacts with
+        their history. Device.from_checkpoint continues from it bit for bit."""
+        import json
+        if self._static is None:
+            raise WdgError("checkpoint: no particles uploaded through this Device")
+        ck = {"meta": np.array(json.dumps(dict(field=self._field, materials=self._mats)))}
+        for k, v in self._static.items():
+            ck["p_" + k] = v
+        for f in ("pos", "vel", "angVel", "ori", "force", "torque", "mass", "inertia"):
+            ck["p_" + f] = self.get(f)
+        if self._scenes is not None:
+            ck["scenes_dt"] = self._scenes
+        ids, cd, data = self.contacts(real_only=True)
+        ck.update(c_ids=ids, c_cellDist=cd, c_data=data)
+        return ck
+
+    @classmethod
+    def from_checkpoint(cls, ck, device=0, verletDist=-.05):
+        """a new context holding the state of checkpoint() (a dict or an np.load result)"""
+        import json
+        meta = json.loads(str(ck["meta"]))
+        d = cls(device)
+        if meta["field"] is not None:
+            d.set_field(**meta["field"])
+        if meta["materials"] is not None:
+            d.set_materials(meta["materials"])
+        d.upload_particles(**{k[2:]: np.asarray(ck[k]) for k in ck if k.startswith("p_")})
+        if "scenes_dt" in ck:
+            d.set_scenes(np.asarray(ck["scenes_dt"]))
+        d.restore_contacts(ck["c_ids"], ck["c_cellDist"], ck["c_data"], verletDist)
+        return d
 
     def observables(self, lo, hi):
         lo = np.asarray(lo, dtype=np.float64)
@@ -349,6 +402,7 @@ class Device:
         """independent scenes, dt of each (empty: back to one scene)"""
         a = np.ascontiguousarray(dt, dtype=np.float64)
         check(self.L.wdg_set_scenes(self.h, len(a), dptr(a) if len(a) else None))
+        self._scenes = a.copy()
 
     def tune(self, engines, step, dt, **kw):
         """autotuner; returns (tuning dict, new step, steps taken)"""

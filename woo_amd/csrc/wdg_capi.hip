@@ -2,6 +2,7 @@
 // The ABI speaks user ids (Particle.id) and user-ordered arrays; the device keeps
 // particles in storage order (walls, then spheres sorted by cell), mapped by uid/sto.
 #include <atomic>
+#include <unordered_map>
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -88,6 +89,115 @@ __global__ void k_export(ExportArgs a) {
   }
   o[9] = ldr<PREC>(a.F0, r); o[10] = ldr<PREC>(a.F1, r); o[11] = ldr<PREC>(a.F2, r);
   o[12] = uN; o[13] = kn; o[14] = kt; o[15] = 1.;
+}
+
+// a checkpoint's real contacts written back into the current real set: per slot, 1 + the
+// index of its record (0: not real), records in the wdg_contact_list layout
+struct RestoreArgs {
+  int64_t nc;
+  const uint32_t* rec;
+  const double* data;
+  RealView rv;
+  const uint64_t* dist;  // packed cellDist per slot (periodic cells; else null)
+  double *N0, *N1, *N2, *Y0, *Y1, *Y2, *F0, *F1, *F2, *uN, *gp;
+};
+
+__global__ void k_restore(RestoreArgs a) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.nc) return;
+  const uint32_t q = a.rec[s];
+  if (q == 0u) return;
+  const double* d = a.data + (int64_t)16 * (q - 1u);
+  const uint32_t k = rv_index(a.rv, (uint32_t)s);
+  const d3 T0 = mk3(d[0], d[1], d[2]), T1 = mk3(d[3], d[4], d[5]);
+  a.N0[k] = T0.x; a.N1[k] = T0.y; a.N2[k] = T0.z;
+  a.Y0[k] = T1.x; a.Y1[k] = T1.y; a.Y2[k] = T1.z;
+  a.F0[k] = d[9]; a.F1[k] = d[10]; a.F2[k] = d[11];
+  a.uN[k] = d[12];
+  // the gather record as the force pass writes it: global force on pA, contact point, cellDist
+  const d3 G = quat_rot(quat_from_cols(T0, T1, cross3(T0, T1)), mk3(d[9], d[10], d[11]));
+  double2* g2 = reinterpret_cast<double2*>(a.gp + (int64_t)GP * k);
+  g2[0] = make_double2(G.x, G.y);
+  g2[1] = make_double2(G.z, d[6]);
+  g2[2] = make_double2(d[7], d[8]);
+  g2[3] = make_double2(__longlong_as_double((long long)(a.dist ? a.dist[s] : 0ull)), 0.);
+}
+
+// Contact history of a checkpoint (the save half is wdg_contact_list(which=1)): a full
+// collider refresh at the uploaded positions, then each saved contact's slot is found by its
+// particles' user ids and made real with the saved state
+void restore_contacts(Ctx& c, int64_t n, const int32_t* ids, const int32_t* cellDist, const double* data,
+                      double verletDist) {
+  if (c.precision != 64) throw Error(WDG_E_STATE, "restoring contacts needs the FP64 contact form");
+  if (c.forcesPending) flush_forces(c);
+  drop_contacts(c);
+  if (c.n == 0) {
+    if (n > 0) throw Error(WDG_E_ARG, "no particles to restore contacts between");
+    return;
+  }
+  wdg_collide_params cp{verletDist, 1};
+  collide(c, &cp);
+  Contacts& C = c.con();
+  const int64_t nc = C.n;
+  std::vector<uint32_t> A((size_t)nc), B((size_t)nc), info((size_t)nc), uid((size_t)c.n);
+  std::vector<uint64_t> dist;
+  WDG_HIP(hipStreamSynchronize(c.stream));
+  WDG_HIP(hipMemcpy(uid.data(), c.uid.p, c.n * 4, hipMemcpyDeviceToHost));
+  if (nc) {
+    WDG_HIP(hipMemcpy(A.data(), C.A.p, nc * 4, hipMemcpyDeviceToHost));
+    WDG_HIP(hipMemcpy(B.data(), C.B.p, nc * 4, hipMemcpyDeviceToHost));
+    WDG_HIP(hipMemcpy(info.data(), C.info.p, nc * 4, hipMemcpyDeviceToHost));
+    if (c.field.periodic) {
+      dist.resize((size_t)nc);
+      WDG_HIP(hipMemcpy(dist.data(), C.dist.p, nc * 8, hipMemcpyDeviceToHost));
+    }
+  }
+  std::unordered_map<uint64_t, int64_t> slot;
+  slot.reserve((size_t)nc);
+  for (int64_t s = 0; s < nc; s++) {
+    const uint64_t a = uid[A[s]], b = uid[B[s]];
+    slot[(std::min(a, b) << 32) | std::max(a, b)] = s;
+  }
+  std::vector<uint32_t> rec((size_t)std::max<int64_t>(nc, 1), 0u);
+  for (int64_t k = 0; k < n; k++) {
+    const int64_t ia = ids[2 * k], ib = ids[2 * k + 1];
+    const std::string name = "##" + std::to_string(ia) + "+" + std::to_string(ib);
+    if (ia < 0 || ib < 0 || ia >= c.n || ib >= c.n || ia == ib) throw Error(WDG_E_ARG, name + ": bad particle ids");
+    const uint64_t lo = (uint64_t)std::min(ia, ib), hi = (uint64_t)std::max(ia, ib);
+    auto it = slot.find((lo << 32) | hi);
+    if (it == slot.end())
+      throw Error(WDG_E_ARG, name + ": the pair's bounding boxes do not overlap (positions not those of the checkpoint?)");
+    const int64_t s = it->second;
+    const bool swap = (info[s] & CI_SWAP) != 0u;
+    if ((int64_t)uid[swap ? B[s] : A[s]] != ia) throw Error(WDG_E_ARG, name + ": particles not in the contact's role order");
+    if (c.field.periodic && cellDist)
+      for (int m = 0; m < 3; m++)
+        if (unpack_cd(dist[s], m) != cellDist[3 * k + m]) throw Error(WDG_E_ARG, name + ": cellDist differs from the nearest image");
+    if (rec[s]) throw Error(WDG_E_ARG, name + ": listed twice");
+    rec[s] = (uint32_t)(k + 1);
+  }
+  if (nc == 0) return;
+  DBuf<uint32_t> drec;
+  DBuf<double> ddata;
+  drec.ensure((size_t)nc);
+  ddata.ensure((size_t)std::max<int64_t>(n, 1) * 16);
+  WDG_HIP(hipMemcpy(drec.p, rec.data(), nc * 4, hipMemcpyHostToDevice));
+  if (n) WDG_HIP(hipMemcpy(ddata.p, data, n * 16 * sizeof(double), hipMemcpyHostToDevice));
+  reals_from_flags(c, drec.p, nc, c.rcur);
+  Reals& R = c.reals();
+  RestoreArgs a{};
+  a.nc = nc; a.rec = drec.p; a.data = ddata.p; a.rv = c.rview();
+  a.dist = c.field.periodic ? C.dist.p : nullptr;
+  a.N0 = R.hN[0].p; a.N1 = R.hN[1].p; a.N2 = R.hN[2].p;
+  a.Y0 = R.hY[0].p; a.Y1 = R.hY[1].p; a.Y2 = R.hY[2].p;
+  a.F0 = R.hF[0].p; a.F1 = R.hF[1].p; a.F2 = R.hF[2].p; a.uN = R.huN.p; a.gp = R.gp.p;
+  hipLaunchKernelGGL(k_restore, dim3(nblocks(nc, 256)), dim3(256), 0, c.stream, a);
+  WDG_LAUNCH_CHECK();
+  WDG_HIP(hipStreamSynchronize(c.stream));
+  drec.release();
+  ddata.release();
+  c.forcesPending = false;
+  c.gen++;
 }
 
 }  // namespace wdg
@@ -540,6 +650,12 @@ int wdg_contact_list(wdg_ctx* ctx, int32_t which, int32_t* ids, int32_t* cellDis
       }
     }
   });
+}
+
+int wdg_restore_contacts(wdg_ctx* ctx, int64_t n, const int32_t* ids, const int32_t* cellDist, const double* data,
+                         double verletDist) {
+  if (!ctx || n < 0 || (n > 0 && (!ids || !data))) { g_err = "bad arguments"; return WDG_E_ARG; }
+  return guard(ctx, [&] { restore_contacts(ctx->c, n, ids, cellDist, data, verletDist); });
 }
 
 int wdg_observables(wdg_ctx* ctx, const double boxMin[3], const double boxMax[3], double out[3]) {
