@@ -59,6 +59,8 @@ struct ContactArgs {
   int noSlip, noBreak, noFrict, noAttraction;
   uint32_t* dflags;
   double* enPart;  // energy tracking: 4 partial sums per force workgroup
+  uint32_t* eidx;          // per adjacency entry: its slot when real, else NO_STATE (null: not kept)
+  const uint32_t* entPos;  // per slot: positions of its two entries
 };
 
 template <bool PERIODIC>
@@ -116,7 +118,6 @@ __device__ __forceinline__ bool law_keeps(double uN, int noBreak) {
 // two particles in role order (pA, pB), the state index of its previous step (NO_STATE when
 // it was not real) and, in periodic cells, its packed cellDist. The filter writes it at the
 // slot's new state index, so the force pass starts from one coalesced record.
-constexpr uint32_t NO_STATE = 0xFFFFFFFFu;
 struct SlotD {
   uint32_t info, pA, pB, ko;
   uint64_t cd;
@@ -157,6 +158,27 @@ __device__ __forceinline__ void group_pass(int64_t c, bool keep, int64_t base, u
   __syncthreads();
 }
 
+// The per-entry real slots of the filter's slots, written after its last barrier (the loads
+// of the entry positions delay no other wavefront): the entries of a slot that became real or
+// stopped being real, and after a reset (EIDX_FULL) those of every real slot; the entries of
+// the other slots hold their value already. Only changes are written: scattered stores are
+// costly, and a contact's slot, unlike its state index, stays the same from step to step.
+// wk: bit p = slot of pass p kept, bit 8+p = it was real (one register across the passes).
+template <int NP>
+__device__ __forceinline__ void eidx_tail(const ContactArgs& a, int64_t base, uint32_t wk) {
+  const bool full = a.dflags[EIDX_FULL] != 0u;
+#pragma unroll
+  for (int pass = 0; pass < NP; pass++) {
+    const int64_t c = base + pass * GROUP_BS + threadIdx.x;
+    const bool keep = (wk >> pass) & 1u, was = (wk >> (8 + pass)) & 1u;
+    if (keep ? (full || !was) : was) {
+      const uint2 ep = reinterpret_cast<const uint2*>(a.entPos)[c];
+      a.eidx[ep.x] = keep ? ((uint32_t)c | EIDX_PA) : NO_STATE;
+      a.eidx[ep.y] = keep ? (uint32_t)c : NO_STATE;
+    }
+  }
+}
+
 // state index of slot c in the old real set (a real slot)
 __device__ __forceinline__ uint32_t old_state(const ContactArgs& a, int64_t c) {
   return a.osidx ? a.osidx[c] : rv_index(a.ov, (uint32_t)c);
@@ -165,6 +187,7 @@ __device__ __forceinline__ uint32_t old_state(const ContactArgs& a, int64_t c) {
 template <bool PERIODIC, int LAW>
 __device__ __forceinline__ bool filter_slot(const ContactArgs& a, int64_t c, SlotD& d) {
   bool keep = false;
+  d.ko = NO_STATE;
   if (c < a.nc) {
     const uint32_t info = a.info[c];
     const bool real = rv_real(a.ov, (uint32_t)c);
@@ -175,6 +198,7 @@ __device__ __forceinline__ bool filter_slot(const ContactArgs& a, int64_t c, Slo
       const d3 shift2 = slot_shift<PERIODIC>(a, c, cd);
       d.info = info; d.pA = pA; d.pB = pB; d.cd = cd;
       d.ko = real ? old_state(a, c) : NO_STATE;
+
       const d3 posA = mk3(a.px[pA], a.py[pA], a.pz[pA]);
       const d3 pos2 = add3(mk3(a.px[pB], a.py[pB], a.pz[pB]), shift2);
       bool pass;
@@ -236,12 +260,17 @@ __global__ __launch_bounds__(GROUP_BS) void k_cfilter(ContactArgs a) {
 #pragma unroll
   for (int pass = 0; pass < NP; pass++)
     keep[pass] = filter_slot<PERIODIC, LAW>(a, base + pass * GROUP_BS + threadIdx.x, d[pass]);
+  uint32_t wk = 0;
+#pragma unroll
+  for (int pass = 0; pass < NP; pass++)
+    wk |= (keep[pass] ? 1u : 0u) << pass | (d[pass].ko != NO_STATE ? 1u : 0u) << (8 + pass);
   uint32_t running = 0;
 #pragma unroll
   for (int pass = 0; pass < NP; pass++)
     group_pass(base + pass * GROUP_BS + threadIdx.x, keep[pass], base, running, a.nbits, a.ntpre, a.act, a.sidx,
                d[pass], a.desc, PERIODIC ? a.descDist : nullptr);
   if (threadIdx.x == 0) a.ngcnt[g] = running;
+  if (a.eidx) eidx_tail<NP>(a, base, wk);
 }
 
 // a.cfBlocks workgroups of CF_BS per group of slots
@@ -261,6 +290,8 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
   const int64_t g = lb / nblk;
   const uint32_t cnt = a.ngcnt[g];
   double en[4] = {0., 0., 0., 0.};
+  // the filter has written the entries of every real slot
+  if (a.eidx && blockIdx.x == 0 && threadIdx.x == 0) a.dflags[EIDX_FULL] = 0u;
   for (uint32_t loc = (lb % nblk) * CF_BS + threadIdx.x; loc < cnt; loc += CF_BS * nblk) {
     const int64_t k = state_index((uint32_t)g, loc);
     // the filter's record of the slot: type bits, particles in role order, old state index
@@ -491,6 +522,7 @@ __device__ __forceinline__ f3 tof3(d3 v) { return mkf3((float)v.x, (float)v.y, (
 template <bool PERIODIC, int LAW>
 __device__ __forceinline__ bool filter_slot32(const ContactArgs& a, int64_t c, SlotD& d) {
   bool keep = false;
+  d.ko = NO_STATE;
   if (c < a.nc) {
     const uint32_t info = a.info[c];
     const bool real = rv_real(a.ov, (uint32_t)c);
@@ -501,6 +533,7 @@ __device__ __forceinline__ bool filter_slot32(const ContactArgs& a, int64_t c, S
       const d3 shift2 = slot_shift<PERIODIC>(a, c, cd);
       d.info = info; d.pA = pA; d.pB = pB; d.cd = cd;
       d.ko = real ? old_state(a, c) : NO_STATE;
+
       const f3 r = tof3(sub3(add3(mk3(a.px[pB], a.py[pB], a.pz[pB]), shift2), mk3(a.px[pA], a.py[pA], a.pz[pA])));
       bool pass;
       float uN;
@@ -559,12 +592,17 @@ __global__ __launch_bounds__(GROUP_BS) void k_cfilter32(ContactArgs a) {
 #pragma unroll
   for (int pass = 0; pass < NP; pass++)
     keep[pass] = filter_slot32<PERIODIC, LAW>(a, base + pass * GROUP_BS + threadIdx.x, d[pass]);
+  uint32_t wk = 0;
+#pragma unroll
+  for (int pass = 0; pass < NP; pass++)
+    wk |= (keep[pass] ? 1u : 0u) << pass | (d[pass].ko != NO_STATE ? 1u : 0u) << (8 + pass);
   uint32_t running = 0;
 #pragma unroll
   for (int pass = 0; pass < NP; pass++)
     group_pass(base + pass * GROUP_BS + threadIdx.x, keep[pass], base, running, a.nbits, a.ntpre, a.act, a.sidx,
                d[pass], a.desc, PERIODIC ? a.descDist : nullptr);
   if (threadIdx.x == 0) a.ngcnt[g] = running;
+  if (a.eidx) eidx_tail<NP>(a, base, wk);
 }
 
 template <bool PERIODIC, int LAW, bool FACETS, bool ENERGY>
@@ -586,6 +624,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
   const uint32_t cnt = a.ngcnt[g];
   // energy terms from the float law's values, evaluated and summed in FP64
   double en[4] = {0., 0., 0., 0.};
+  if (a.eidx && blockIdx.x == 0 && threadIdx.x == 0) a.dflags[EIDX_FULL] = 0u;
   for (uint32_t loc = (lb % nblk) * CF_BS + threadIdx.x; loc < cnt; loc += CF_BS * nblk) {
     const int64_t k = state_index((uint32_t)g, loc);
     // the filter's record of the slot: type bits, particles in role order, old state index
@@ -825,6 +864,20 @@ static ContactArgs contact_args(Ctx& c, int from, int to) {
   return a;
 }
 
+__global__ void k_fill_u32(int64_t n, uint32_t* p, uint32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void eidx_clear(Ctx& c) {
+  Contacts& C = c.con();
+  if (C.n == 0) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3(nblocks(2 * C.n, 256)), dim3(256), 0, c.stream, 2 * C.n, C.eidx.p, NO_STATE);
+  WDG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c.stream, (int64_t)1, c.dflags.p + EIDX_FULL, 1u);
+  WDG_LAUNCH_CHECK();
+}
+
 template <bool E>
 static void launch_cforce(Ctx& c, const ContactArgs& a, dim3 grid, bool f32, bool hertz, bool per, bool fac) {
   const dim3 block(CF_BS);
@@ -884,6 +937,14 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
   c.desc.ensure(ng * GROUP);
   if (c.field.periodic) c.descDist.ensure(ng * GROUP);
   ContactArgs a = contact_args(c, from, to);
+  // per-entry real slots for the gather (slot indices must stay below EIDX_PA); the filter
+  // writes the entries of slots whose real state changes (all real ones after a reset), so
+  // the others must already hold their value
+  if (ng * GROUP < (int64_t)EIDX_PA) {
+    a.eidx = C.eidx.p;
+    a.entPos = C.entPos.p;
+    if (!c.eidxValid && !c.eidxClean) eidx_clear(c);
+  }
   a.dt = p->dt;
   a.renorm = (p->trsfRenorm > 0 && (p->step % p->trsfRenorm) == 0) ? 1 : 0;
   a.poisson = p->poisson;
@@ -924,6 +985,8 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
     launch_cforce<false>(c, a, gforce, c.precision == 32, hertz, per, fac);
   }
   c.R[to].sidxValid = true;
+  c.eidxValid = a.eidx != nullptr;
+  c.eidxClean = false;
   c.rcur = to;
   c.forcesPending = p->applyForces != 0;
 }
@@ -944,6 +1007,8 @@ void reals_from_flags(Ctx& c, const uint32_t* flag, int64_t nslots, int dst) {
   Reals& N = c.R[dst];
   N.ensure(nslots);
   N.sidxValid = false;
+  c.eidxValid = false;
+  c.eidxClean = false;
   const int64_t ng = ngroups(nslots);
   if (ng > 0) {
     hipLaunchKernelGGL(k_flag_bits, dim3((unsigned)ng), dim3(GROUP_BS), 0, c.stream, nslots, flag, N.bits.p,

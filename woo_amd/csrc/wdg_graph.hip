@@ -61,15 +61,18 @@ struct HostState {
   int rcur, lfParity, dirtyIdx;
   bool forcesPending, forceClean, dirtyKnown;
   bool sidxValid[2];
+  bool eidxValid, eidxClean;
 };
 static HostState save(const Ctx& c) {
   return HostState{c.rcur, c.lfParity, c.dirtyIdx, c.forcesPending, c.forceClean, c.dirtyKnown,
-                   {c.R[0].sidxValid, c.R[1].sidxValid}};
+                   {c.R[0].sidxValid, c.R[1].sidxValid}, c.eidxValid, c.eidxClean};
 }
 static void restore(Ctx& c, const HostState& h) {
   c.rcur = h.rcur; c.lfParity = h.lfParity; c.dirtyIdx = h.dirtyIdx;
   c.forcesPending = h.forcesPending; c.forceClean = h.forceClean; c.dirtyKnown = h.dirtyKnown;
   c.R[0].sidxValid = h.sidxValid[0]; c.R[1].sidxValid = h.sidxValid[1];
+  c.eidxValid = h.eidxValid;
+  c.eidxClean = h.eidxClean;
 }
 
 static void capture(Ctx& c, const wdg_program* prog, int K, int64_t step, double dt) {
@@ -132,6 +135,11 @@ static bool graph_valid(const Ctx& c, const wdg_program* prog, int K, double dt)
 
 // one replay of up to K steps; returns the steps executed
 static int replay(Ctx& c, const wdg_program* prog, int K, int64_t step, double dt) {
+  // the captured filters keep the per-entry state indices of a clean or current array
+  if (!c.eidxValid && !c.eidxClean && c.con().n > 0) {
+    eidx_clear(c);
+    c.eidxClean = true;
+  }
   if (!graph_valid(c, prog, K, dt)) capture(c, prog, K, step, dt);
   uint32_t w[6] = {0u, 0u, 0u, (uint32_t)((uint64_t)step & 0xFFFFFFFFu), (uint32_t)((uint64_t)step >> 32),
                    (uint32_t)std::max(prog->contact.trsfRenorm, 0)};
@@ -146,6 +154,9 @@ static int replay(Ctx& c, const wdg_program* prog, int K, int64_t step, double d
       // the real sets the executed filters wrote carry their per-slot state indices
       c.R[1 - c.rcur].sidxValid = true;
       if (e > 1) c.R[c.rcur].sidxValid = true;
+      // the last executed filter wrote the per-entry state indices of the set now current
+      c.eidxValid = ngroups(c.con().n) * GROUP < (int64_t)EIDX_PA;
+      c.eidxClean = false;
       c.rcur = (e & 1) ? 1 - c.rcur : c.rcur;
     }
     const int lastParity = (c.lfParity + e - 1) & 1;

@@ -19,6 +19,7 @@ struct GatherArgs {
   int64_t n, nW;
   const uint32_t *adj, *adjOff;
   const uint32_t* sidx;  // state index of each real slot (null: found from the real bits)
+  const uint32_t* eidx;  // per adjacency entry its slot when real, else NO_STATE (null: not kept)
   const uint32_t* flags;
   const uint32_t* gstop;  // graph replay: a set stop word skips the kernel
   RealView rv;
@@ -61,58 +62,56 @@ __device__ __forceinline__ void gather_one(const GatherArgs& g, uint32_t e, d3 p
   }
 }
 
-// A particle's list is walked GATHER_BATCH entries at a time: the entries of the batch, then
-// their real bits and state indices (the filter's per-slot index) are all loaded before any
-// record, so that the dependent loads of the batch are in flight together; the records of the
-// real entries (usually none or one) are then loaded and summed in list order, the order of
-// the entry-by-entry walk, so the sums are bit-identical to it.
-constexpr int GATHER_BATCH = 2;
+// one contact's force and torque on the particle at pos: the record at state index q, the
+// particle being the contact's pA or pB
+template <bool PERIODIC, bool F32>
+__device__ __forceinline__ void gather_record(const GatherArgs& g, uint32_t q, bool isPA, d3 pos, d3& F, d3& T) {
+  if (F32) {
+    const float4* r = reinterpret_cast<const float4*>(reinterpret_cast<const float*>(g.gp) + (int64_t)GP32 * q);
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2];
+    const d3 Fc = mul3(mk3(r0.x, r0.y, r0.z), isPA ? 1. : -1.);
+    const d3 xc = isPA ? mk3(r0.w, r1.x, r1.y) : mk3(r1.z, r1.w, r2.x);
+    F = add3(F, Fc);
+    T = add3(T, cross3(xc, Fc));
+  } else {
+    const double2* r = reinterpret_cast<const double2*>(g.gp + (int64_t)GP * q);
+    const double2 r0 = r[0], r1 = r[1], r2 = r[2];
+    const d3 Fc = mul3(mk3(r0.x, r0.y, r1.x), isPA ? 1. : -1.);
+    d3 shift = mk3(0., 0., 0.);
+    if (PERIODIC && !isPA) {
+      const uint64_t cd = (uint64_t)__double_as_longlong(r[3].x);
+      shift = mk3(g.L[0] * unpack_cd(cd, 0), g.L[1] * unpack_cd(cd, 1), g.L[2] * unpack_cd(cd, 2));
+    }
+    // Contact::getForceTorqueBranch: xc = contact point - (node pos + periodic shift of pB)
+    const d3 xc = sub3(mk3(r1.y, r2.x, r2.y), add3(pos, shift));
+    F = add3(F, Fc);
+    T = add3(T, cross3(xc, Fc));
+  }
+}
+
+// With the per-entry real slots the list is one contiguous array: EIDX_BATCH of its entries
+// are loaded together, then the state indices of the real ones (usually none or one), then
+// their records, summed in list order (the order of the entry-by-entry walk, so the sums are
+// bit-identical to it).
+constexpr int EIDX_BATCH = 4;
 template <bool PERIODIC, bool F32>
 __device__ __forceinline__ void gather_all(const GatherArgs& g, int64_t i, d3 pos, d3& F, d3& T) {
   const uint32_t k0 = g.adjOff[i], k1 = g.adjOff[i + 1];
-  if (!g.sidx) {
-    for (uint32_t k = k0; k < k1; k++) gather_one<PERIODIC, F32>(g, g.adj[k], pos, F, T);
+  if (g.eidx && g.sidx) {
+    for (uint32_t k = k0; k < k1; k += EIDX_BATCH) {
+      uint32_t v[EIDX_BATCH], q[EIDX_BATCH];
+#pragma unroll
+      for (int j = 0; j < EIDX_BATCH; j++) v[j] = k + j < k1 ? g.eidx[k + j] : NO_STATE;
+#pragma unroll
+      for (int j = 0; j < EIDX_BATCH; j++) q[j] = v[j] != NO_STATE ? g.sidx[v[j] & ~EIDX_PA] : 0u;
+#pragma unroll
+      for (int j = 0; j < EIDX_BATCH; j++)
+        if (v[j] != NO_STATE) gather_record<PERIODIC, F32>(g, q[j], (v[j] & EIDX_PA) != 0u, pos, F, T);
+    }
     return;
   }
-  const uint32_t* bits32 = reinterpret_cast<const uint32_t*>(g.rv.bits);
-  for (uint32_t k = k0; k < k1; k += GATHER_BATCH) {
-    uint32_t e[GATHER_BATCH], w[GATHER_BATCH], idx[GATHER_BATCH];
-#pragma unroll
-    for (int j = 0; j < GATHER_BATCH; j++) e[j] = k + j < k1 ? g.adj[k + j] : 0u;
-#pragma unroll
-    for (int j = 0; j < GATHER_BATCH; j++) {
-      const uint32_t sl = e[j] & ~ADJ_PA;
-      w[j] = k + j < k1 ? bits32[sl >> 5] : 0u;
-      idx[j] = k + j < k1 ? g.sidx[sl] : 0u;
-    }
-#pragma unroll
-    for (int j = 0; j < GATHER_BATCH; j++) {
-      if (!((w[j] >> ((e[j] & ~ADJ_PA) & 31)) & 1u)) continue;
-      const bool isPA = (e[j] & ADJ_PA) != 0u;
-      if (F32) {
-        const float4* r =
-            reinterpret_cast<const float4*>(reinterpret_cast<const float*>(g.gp) + (int64_t)GP32 * idx[j]);
-        const float4 r0 = r[0], r1 = r[1], r2 = r[2];
-        const d3 Fc = mul3(mk3(r0.x, r0.y, r0.z), isPA ? 1. : -1.);
-        const d3 xc = isPA ? mk3(r0.w, r1.x, r1.y) : mk3(r1.z, r1.w, r2.x);
-        F = add3(F, Fc);
-        T = add3(T, cross3(xc, Fc));
-      } else {
-        const double2* r = reinterpret_cast<const double2*>(g.gp + (int64_t)GP * idx[j]);
-        const double2 r0 = r[0], r1 = r[1], r2 = r[2];
-        const d3 Fc = mul3(mk3(r0.x, r0.y, r1.x), isPA ? 1. : -1.);
-        d3 shift = mk3(0., 0., 0.);
-        if (PERIODIC && !isPA) {
-          const uint64_t cd = (uint64_t)__double_as_longlong(r[3].x);
-          shift = mk3(g.L[0] * unpack_cd(cd, 0), g.L[1] * unpack_cd(cd, 1), g.L[2] * unpack_cd(cd, 2));
-        }
-        // Contact::getForceTorqueBranch: xc = contact point - (node pos + periodic shift of pB)
-        const d3 xc = sub3(mk3(r1.y, r2.x, r2.y), add3(pos, shift));
-        F = add3(F, Fc);
-        T = add3(T, cross3(xc, Fc));
-      }
-    }
-  }
+  // without them (a real set not made by the contact loop): entry by entry through the real bits
+  for (uint32_t k = k0; k < k1; k++) gather_one<PERIODIC, F32>(g, g.adj[k], pos, F, T);
 }
 
 // Contact forces on the boundary bodies (storage [0, nW)), which may touch thousands of
@@ -346,6 +345,7 @@ static GatherArgs gather_args(Ctx& c) {
   g.n = c.n;
   g.adj = C.adj.p; g.adjOff = C.adjOff.p; g.rv = c.rview(); g.gp = c.reals().gp.p; g.flags = c.flags.p;
   g.sidx = c.reals().sidxValid ? c.reals().sidx.p : nullptr;
+  g.eidx = c.eidxValid ? C.eidx.p : nullptr;
   g.nW = c.nW;
   if (c.graphCapture) g.gstop = c.gstate.p;
   for (int k = 0; k < 3; k++) g.L[k] = c.field.cell[k];

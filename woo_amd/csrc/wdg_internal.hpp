@@ -32,6 +32,14 @@ struct Error : std::runtime_error {
   } while (0)
 #define WDG_LAUNCH_CHECK() WDG_HIP(hipGetLastError())
 
+// a slot's contact is not real (per-slot and per-entry state indices)
+constexpr uint32_t NO_STATE = 0xFFFFFFFFu;
+// the entry of the contact's pA (Contacts::eidx); slot indices stay below it
+constexpr uint32_t EIDX_PA = 0x80000000u;
+// dflags word: set when Contacts::eidx was reset, so that the next filter writes the entries of
+// every real slot (else only of the slots whose real state changed); cleared by the force pass
+constexpr int EIDX_FULL = 48;
+
 // ------------------------------------------------------------------ particle flag bits
 // low bits are DemData.flags (Particle.hpp:196-199); high bits are engine-private
 enum : uint32_t {
@@ -111,15 +119,22 @@ struct Contacts {
   // per-particle adjacency: entries slot|ADJ_PA grouped by storage index, each group in
   // increasing order of the other particle's user id (the canonical summation order)
   DBuf<uint32_t> adj, adjOff;
+  // per adjacency entry: its slot (| EIDX_PA for pA's entry) when the slot is real, else
+  // NO_STATE, kept by the contact loop's filter; and per slot the positions of its two entries
+  // (pA's, then pB's), written at the refresh
+  DBuf<uint32_t> eidx, entPos;
   void ensure(int64_t nc, int64_t npar, bool periodic) {
     size_t c = (size_t)(nc > 0 ? nc : 1);
     A.ensure(c); B.ensure(c); info.ensure(c); carry.ensure(c);
     if (periodic) dist.ensure(c);
     adj.ensure(2 * c);
+    eidx.ensure(2 * c);
+    entPos.ensure(2 * c);
     adjOff.ensure((size_t)npar + 1);
   }
   void release() {
     A.release(); B.release(); info.release(); dist.release(); carry.release(); adj.release(); adjOff.release();
+    eidx.release(); entPos.release();
   }
 };
 
@@ -304,6 +319,9 @@ struct Ctx {
   // force/torque of every integrated particle hold the reset value (m*gravity unless
   // gravity-skipped, zero torque), so the integrator computes them instead of loading
   bool forceClean = false;
+  // the slot list's per-entry real slots (Contacts::eidx) describe the current real set;
+  // eidxClean: they all hold NO_STATE (after a refresh)
+  bool eidxValid = false, eidxClean = false;
   double verletDist = NAN;
   int64_t nRefresh = 0, nCollide = 0;
   int64_t dynStepLast = -1, dynStepPrev = -1;
@@ -343,6 +361,8 @@ void drop_contacts(Ctx& c);
 void snapshot_save(Ctx& c, int slot);
 void snapshot_load(Ctx& c, int slot);
 double snapshot_distance(Ctx& c, int slot);
+// every per-entry slot of the current slot list to NO_STATE (and EIDX_FULL set)
+void eidx_clear(Ctx& c);
 void energy_enable(Ctx& c, bool on);
 void energy_clear(Ctx& c);
 void energy_step_begin(Ctx& c);
