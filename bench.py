@@ -28,6 +28,10 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 METRIC = "particle-steps/s, 1M Hertz spheres on 1 MI355X; % HBM roofline; speedup vs FP64"
 RHO, YOUNG, TANPHI, POISSON, EN = 3000., 1e8, .4, .25, .6
 ENGINES = ["Collider", "ContactLoop", "Leapfrog"]
+# steps wdg_tune takes: one plain pair, then for 3 force-pass widths x graph lengths 4, 8, 16 a
+# capturing replay and 3 timed ones (2 + 3 * 4 * (4 + 8 + 16)); the run after it is the same as
+# after this many plain steps (graph replay is bit-identical to the plain loop)
+TUNE_STEPS = 338
 
 
 def log(*a):
@@ -165,7 +169,7 @@ def observables(dev, wl):
                 solid_fraction=ob["solidFraction"], stress=dev.stress(-1.))
 
 
-def fp32_form(wl, args, value64, obs64):
+def fp32_form(wl, args, value64, obs64, tuning):
     """The FP32 contact form on the same initial state and step count: throughput and its
     speedup over FP64, and each observable's relative error against the FP64 run (which
     is bit-identical to the FP64 CPU oracle)."""
@@ -181,8 +185,15 @@ def fp32_form(wl, args, value64, obs64):
     cp = contact_params()
     step, _ = d.run(ENGINES, args.warmup, 0, wl["dt"], contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
     d.sync()
+    if tuning:
+        # the FP64 run's tuner steps, so that both runs compare at the same step, then its tuning
+        step, _ = d.run(ENGINES, tuning["tuning_steps"], step, wl["dt"], contact=cp, leapfrog=dict(damping=0.),
+                        dynDtPeriod=0)
+        d.set_tuning(cfBlocks=tuning["cfBlocks"], graphSteps=tuning["graphSteps"])
+        d.sync()
     t0 = time.perf_counter()
-    step, _ = d.run(ENGINES, args.steps, step, wl["dt"], contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
+    step, _ = d.run(ENGINES, args.steps, step, wl["dt"], graph=bool(args.graph), contact=cp,
+                    leapfrog=dict(damping=0.), dynDtPeriod=0)
     d.sync()
     el = time.perf_counter() - t0
     v32 = n * args.steps / el
@@ -546,8 +557,10 @@ def pmc_probe(dev, wl, args):
     kernel-timing steps), the last pmc_probe of which are the probed launches: the engine is
     deterministic, so they see the same state as the launches the benchmark times."""
     dt, cp = wl["dt"], contact_params()
-    step, _ = dev.run(ENGINES, args.warmup, 0, dt, contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
-    step, _ = dev.run(ENGINES, args.steps, step, dt, contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
+    # plain steps throughout (a profiler collecting counters does not follow graph replays): the
+    # tuner's steps as plain steps, then the timed ones
+    n = args.warmup + (TUNE_STEPS if args.graph else 0) + args.steps
+    step, _ = dev.run(ENGINES, n, 0, dt, contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
     for _ in range(max(args.kernel_steps - args.pmc_probe, 0)):
         dev.collide(-.05)
         dev.contact_loop(step, dt, **cp)
@@ -590,6 +603,10 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--kernel-steps", type=int, default=50)
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--graph", type=int, default=1,
+                    help="1: after the warm-up the autotuner picks the force-pass width and the steps per "
+                         "hipGraph, and the timed steps are graph replays (bit-identical to the plain loop); "
+                         "0: the plain engine loop")
     ap.add_argument("--fp32", type=int, default=1, help="also time the FP32 contact form")
     ap.add_argument("--drum", type=int, default=1, help="also time the rotating-drum workload (facets)")
     ap.add_argument("--drum-n", type=int, default=4_000_000)
@@ -627,10 +644,19 @@ def main():
     log(f"dt={dt:.4e} s, warmup {args.warmup} steps")
     step, _ = dev.run(ENGINES, args.warmup, step, dt, contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
     dev.sync()
+    tuning = None
+    if args.graph:
+        # untimed, after the warm-up: the tuner's trial steps advance the run like any other
+        tun, step, taken = dev.tune(ENGINES, step, dt, contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
+        dev.sync()
+        assert taken == TUNE_STEPS, taken
+        tuning = dict(tun, tuning_steps=taken)
+        log(f"tuning: {tuning}")
     st0 = dev.stats()
     log(f"after warmup: {st0}")
     t0 = time.perf_counter()
-    step, _ = dev.run(ENGINES, args.steps, step, dt, contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
+    step, _ = dev.run(ENGINES, args.steps, step, dt, graph=bool(args.graph), contact=cp, leapfrog=dict(damping=0.),
+                      dynDtPeriod=0)
     dev.sync()
     el = time.perf_counter() - t0
     st1 = dev.stats()
@@ -640,7 +666,7 @@ def main():
     obs64 = observables(dev, wl)
     fp32 = None
     if args.fp32:
-        fp32 = fp32_form(wl, args, value, obs64)
+        fp32 = fp32_form(wl, args, value, obs64, tuning)
         log(f"fp32 form: {fp32}")
 
     # per-kernel device time on the engine's stream
@@ -685,7 +711,9 @@ def main():
         "config": {"workload": "1M polydisperse Hertz spheres, periodic cube, phi=0.55, FP64 (Collider, ContactLoop, "
                                "Leapfrog)", "n": args.n, "dt": dt, "cell": wl["L"],
                    "contact_slots": st2["nCandidates"], "real_contacts": st2["nReal"],
-                   "refreshes_in_timed_region": st1["nRefresh"] - st0["nRefresh"]},
+                   "refreshes_in_timed_region": st1["nRefresh"] - st0["nRefresh"],
+                   "stepping": "hipGraph replay, tuned" if args.graph else "plain engine loop",
+                   "tuning": tuning},
         "roofline": roof,
         "cpu_baseline": base,
         "fp32_form": fp32,
