@@ -217,8 +217,9 @@ int wdg_kinetic_energy(wdg_ctx* ctx, double* ek);
  * from step 1), IdealElPl.cpp:62-92 (elast, plast) and Hertz.cpp:181-206 (elast, plast,
  * viscN, viscT) to the context's energy terms; each is a sum over the workgroups' partials
  * in a fixed order (not ContactContainer order: the last bits differ from a serial sum).
- * One scene per context only; wdg_run_graph and wdg_tune refuse a context that tracks.
- * Turning it on does not clear the terms. */
+ * One scene per context only. Graph replay (wdg_run_graph, wdg_tune) tracks too, with the
+ * same sums; a periodic scene's step 0 is then run plainly. Turning it on does not clear the
+ * terms. */
 int wdg_set_energy_tracking(wdg_ctx* ctx, int32_t on);
 /* Scene.energy: out[WDG_EN_N] in WDG_EN_* order (synchronises the context's stream) */
 int wdg_energy(wdg_ctx* ctx, double out[7]);

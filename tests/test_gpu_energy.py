@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import ENERGY_TERMS
-from tests.scenes import GpuRunner, build_gpu, build_oracle, contact_kw, deposition, drum, gpu_arrays, periodic_hertz
+from tests.scenes import GpuRunner, build_oracle, contact_kw, deposition, drum, gpu_arrays, periodic_hertz
 
 pytestmark = pytest.mark.gpu
 
@@ -140,7 +140,7 @@ def test_fp32_form():
     compare(eo, eg, tol=1e-3, tol_diss=2e-2)
 
 
-def test_tracking_leaves_trajectory_and_refuses_graphs():
+def test_tracking_leaves_trajectory_and_clears():
     sc = deposition(n=500, dense=True, seed=57, vel_sigma=0.05)
     g1, g2 = GpuRunner(sc), GpuRunner(sc)
     g1.d.set_energy_tracking(True)
@@ -148,15 +148,38 @@ def test_tracking_leaves_trajectory_and_refuses_graphs():
     g2.run(ENG, 30)
     np.testing.assert_array_equal(g1.d.get("pos"), g2.d.get("pos"))
     assert all(v == 0. for v in g2.d.energy().values())
-    kw = dict(contact=contact_kw(sc), leapfrog=dict(damping=sc["damping"]), dynDtPeriod=0)
-    with pytest.raises(RuntimeError, match="energy"):
-        g1.d.run(ENG, 4, 30, sc["dt"], graph=True, **kw)
-    with pytest.raises(RuntimeError, match="energy"):
-        g1.d.tune(ENG, 30, sc["dt"], **kw)
     # cleared, and switched off: the terms stay as they are
     g1.d.energy_clear()
     assert all(v == 0. for v in g1.d.energy().values())
     g1.d.set_energy_tracking(False)
     g1.run(ENG, 5)
     assert all(v == 0. for v in g1.d.energy().values())
-    build_gpu(sc)  # a fresh context tracks nothing
+
+
+@pytest.mark.parametrize("kind", ["frict_deposition", "hertz_periodic"])
+def test_graph_replay_tracks_like_the_plain_loop(kind):
+    # the same kernels and partial sums in the same order: the terms are bit-identical; a
+    # periodic scene's step 0 (not tracked in the leapfrog) is run plainly
+    sc = (deposition(n=700, dense=True, seed=58, vel_sigma=0.05) if kind == "frict_deposition"
+          else periodic_hertz(n=1000, spacing=0.95, seed=59, en=.6))
+    kw = dict(contact=contact_kw(sc), leapfrog=dict(damping=sc["damping"]), dynDtPeriod=0)
+    runs = []
+    for graph in (False, True):
+        g = GpuRunner(sc)
+        g.d.set_tuning(cfBlocks=4, graphSteps=8)
+        g.d.set_energy_tracking(True)
+        step, _ = g.d.run(ENG, 150, 0, sc["dt"], graph=graph, **kw)
+        assert step == 150
+        runs.append((g.d.get("pos"), g.d.energy()))
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    assert runs[0][1] == runs[1][1]
+    o = build_oracle(sc, trackEnergy=1)
+    o.run(ENG, 150)
+    compare(o.energy(), runs[1][1])
+    # the tuner replays graphs too
+    g = GpuRunner(sc)
+    g.d.set_energy_tracking(True)
+    tun, step, taken = g.d.tune(ENG, 0, sc["dt"], **kw)
+    o2 = build_oracle(sc, trackEnergy=1)
+    o2.run(ENG, taken)
+    compare(o2.energy(), g.d.energy())

@@ -708,7 +708,8 @@ int wdg_set_scenes(wdg_ctx* ctx, int32_t nScenes, const double* dt) {
 int wdg_run_graph(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t* step, double* dt) {
   if (!ctx || !prog || !step || !dt || nSteps < 0) { g_err = "bad arguments"; return WDG_E_ARG; }
   return guard(ctx, [&] {
-    if (ctx->c.trackEnergy) throw Error(WDG_E_STATE, "graph replay does not track energy (wdg_set_energy_tracking 0)");
+    if (ctx->c.trackEnergy && ctx->c.nScenes > 1)
+      throw Error(WDG_E_STATE, "energy tracking needs a context of one scene");
     run_graph(ctx->c, prog, nSteps, step, dt);
   });
 }
@@ -716,7 +717,8 @@ int wdg_run_graph(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t
 int wdg_tune(wdg_ctx* ctx, const wdg_program* prog, int64_t* step, double* dt, wdg_tuning* out, int64_t* stepsTaken) {
   if (!ctx || !prog || !step || !dt) { g_err = "bad arguments"; return WDG_E_ARG; }
   return guard(ctx, [&] {
-    if (ctx->c.trackEnergy) throw Error(WDG_E_STATE, "the tuner replays graphs, which do not track energy");
+    if (ctx->c.trackEnergy && ctx->c.nScenes > 1)
+      throw Error(WDG_E_STATE, "energy tracking needs a context of one scene");
     const int64_t n = tune(ctx->c, prog, step, dt);
     if (out) *out = ctx->c.tune;
     if (stepsTaken) *stepsTaken = n;

@@ -279,11 +279,11 @@ constexpr int CF_BS = 256;
 // FACETS: the scene has facets (a separate instantiation keeps the facet geometry's
 // registers out of the other scenes' kernel)
 // ENERGY: the law's energy terms (elast, plast, viscN, viscT) of the workgroup's contacts are
-// summed into a.enPart[4 * blockIdx.x ..]; every thread reaches the sum (no early exit: graph
-// replay, the only user of gstop, does not track energy)
+// summed into a.enPart[4 * blockIdx.x ..]; every thread reaches the sum (the only early exit,
+// a stopped graph replay, is taken by the whole workgroup)
 template <bool PERIODIC, int LAW, bool FACETS, bool ENERGY>
 __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
-  if (!ENERGY && a.gstop && *a.gstop) return;
+  if (a.gstop && *a.gstop) return;
   const int renorm = a.renormPtr ? (int)*a.renormPtr : a.renorm;
   const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t nblk = (uint32_t)a.cfBlocks;
@@ -616,7 +616,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
         *N2 = reinterpret_cast<float*>(a.N2), *Y0 = reinterpret_cast<float*>(a.Y0), *Y1 = reinterpret_cast<float*>(a.Y1),
         *Y2 = reinterpret_cast<float*>(a.Y2), *F0 = reinterpret_cast<float*>(a.F0), *F1 = reinterpret_cast<float*>(a.F1),
         *F2 = reinterpret_cast<float*>(a.F2), *UN = reinterpret_cast<float*>(a.uN), *gp = reinterpret_cast<float*>(a.gp);
-  if (!ENERGY && a.gstop && *a.gstop) return;
+  if (a.gstop && *a.gstop) return;
   const int renorm = a.renormPtr ? (int)*a.renormPtr : a.renorm;
   const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t nblk = (uint32_t)a.cfBlocks;

@@ -88,6 +88,8 @@ static void capture(Ctx& c, const wdg_program* prog, int K, int64_t step, double
   c.act.ensure(ngroups(nc) * GROUP);
   c.desc.ensure(ngroups(nc) * GROUP);
   if (c.field.periodic) c.descDist.ensure(ngroups(nc) * GROUP);
+  if (c.trackEnergy)
+    c.enPart.ensure(std::max<size_t>((size_t)4 * ngroups(nc) * c.tune.cfBlocks, (size_t)3 * nblocks(c.n, 256)));
   const HostState h = save(c);
   hipGraph_t graph = nullptr;
   c.graphCapture = true;
@@ -115,6 +117,7 @@ static void capture(Ctx& c, const wdg_program* prog, int K, int64_t step, double
   G.gen = c.gen;
   G.K = K;
   G.cfBlocks = c.tune.cfBlocks;
+  G.track = c.trackEnergy;
   G.rcur = c.rcur;
   G.parity = c.lfParity;
   G.dt = dt;
@@ -126,7 +129,8 @@ static bool graph_valid(const Ctx& c, const wdg_program* prog, int K, double dt)
   const auto& G = c.graph;
   const wdg_contact_params& a = G.cp;
   const wdg_contact_params& b = prog->contact;
-  return G.exec && G.gen == c.gen && G.K == K && G.cfBlocks == c.tune.cfBlocks && G.rcur == c.rcur &&
+  return G.exec && G.gen == c.gen && G.K == K && G.cfBlocks == c.tune.cfBlocks && G.track == c.trackEnergy &&
+         G.rcur == c.rcur &&
          G.parity == c.lfParity && G.dt == dt && G.damping == prog->leapfrog.damping && a.physKind == b.physKind &&
          a.lawKind == b.lawKind && a.applyForces == b.applyForces && a.trsfRenorm == b.trsfRenorm &&
          a.poisson == b.poisson && (a.en == b.en || (a.en != a.en && b.en != b.en)) && a.noSlip == b.noSlip &&
@@ -179,7 +183,9 @@ void run_graph(Ctx& c, const wdg_program* prog, int64_t nSteps, int64_t* step, d
     // a replay starts from a state its capture saw: forces at their reset value, the
     // contact loop's forces gathered; otherwise one plain step
     const int64_t left = nSteps - done;
-    if (!c.forceClean || c.forcesPending || left < K || c.n == 0) {
+    // (a periodic scene's step 0 is a plain one when tracking energy: its leapfrog adds no terms,
+    // and a graph captured there would skip them at every replay)
+    if (!c.forceClean || c.forcesPending || left < K || c.n == 0 || (c.trackEnergy && c.field.periodic && *step == 0)) {
       plain_step(c, prog, *step, *dt);
       *step += 1;
       done += 1;

@@ -191,7 +191,7 @@ __device__ __forceinline__ double max_cmp(double x, double y) { return x < y ? y
 // called at Leapfrog.cpp:156-159 and 218) summed over the workgroup into a.enPart[3 * blockIdx.x ..]
 template <bool PERIODIC, bool F32, bool ENERGY>
 __global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
-  if (!ENERGY && a.gstop && *a.gstop) return;
+  if (a.gstop && *a.gstop) return;
   const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   bool dirty = false;
   double en[3] = {0., 0., 0.};
@@ -898,7 +898,9 @@ int64_t count_real(Ctx& c) {
 // threads' sums are added in a fixed tree; run to run the result is the same
 // (bit q of setMask: the sum replaces energy[target q] instead of being added to it)
 __global__ __launch_bounds__(256) void k_energy_add(const double* __restrict__ part, int64_t nBlocks, int nTerms,
-                                                    int4 target, int setMask, double* energy) {
+                                                    int4 target, int setMask, double* energy, const uint32_t* gstop) {
+  // a stopped graph replay: the step did not happen
+  if (gstop && *gstop) return;
   __shared__ double sh[256];
   for (int q = 0; q < nTerms; q++) {
     double s = 0.;
@@ -946,7 +948,7 @@ void energy_add(Ctx& c, int64_t nBlocks, int nTerms, const int* target, int setM
   if (nBlocks <= 0) return;
   const int4 t = make_int4(target[0], nTerms > 1 ? target[1] : 0, nTerms > 2 ? target[2] : 0, nTerms > 3 ? target[3] : 0);
   hipLaunchKernelGGL(k_energy_add, dim3(1), dim3(256), 0, c.stream, c.enPart.p, nBlocks, nTerms, t, setMask,
-                     c.energy.p);
+                     c.energy.p, c.graphCapture ? (const uint32_t*)c.gstate.p : nullptr);
   WDG_LAUNCH_CHECK();
 }
 

@@ -245,6 +245,7 @@ struct Ctx {
     hipGraphExec_t exec = nullptr;
     int64_t gen = -1;
     int K = 0, cfBlocks = 0, rcur = -1, parity = -1;
+    bool track = false;  // captured with energy tracking
     double dt = 0, damping = 0;
     wdg_contact_params cp{};
   } graph;
