@@ -686,6 +686,20 @@ def main():
                 traffic=traffic, traffic_source=tsrc, kernel=dom, bytes_per_launch=B[dom], launch_us=times[dom] * 1e6,
                 other_kernels={k: dict(bytes_per_launch=B[k], launch_us=times[k] * 1e6,
                                        achieved_GBs=B[k] / times[k] / 1e9) for k in times if k != dom})
+    # the cost of energy tracking (Scene.trackEnergy) on the same stepping, 200 steps each
+    kw = dict(graph=bool(args.graph), contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
+    energy = {}
+    for on in (False, True):
+        dev.set_energy_tracking(on)
+        step, _ = dev.run(ENGINES, 8, step, dt, **kw)
+        dev.sync()
+        t0 = time.perf_counter()
+        step, _ = dev.run(ENGINES, 200, step, dt, **kw)
+        dev.sync()
+        energy["ms_per_step_" + ("tracking" if on else "plain")] = (time.perf_counter() - t0) / 200 * 1e3
+    energy["terms"] = dev.energy()
+    energy["overhead"] = energy["ms_per_step_tracking"] / energy["ms_per_step_plain"] - 1.
+    log(f"energy tracking: {energy}")
     drum = batch = None
     dev.close()
     if args.drum:
@@ -717,6 +731,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": base,
         "fp32_form": fp32,
+        "energy_tracking": energy,
         "drum_form": drum,
         "batched_form": batch,
         "surrogate_form": sur,

@@ -901,21 +901,30 @@ __global__ __launch_bounds__(256) void k_energy_add(const double* __restrict__ p
                                                     int4 target, int setMask, double* energy, const uint32_t* gstop) {
   // a stopped graph replay: the step did not happen
   if (gstop && *gstop) return;
-  __shared__ double sh[256];
-  for (int q = 0; q < nTerms; q++) {
-    double s = 0.;
-    for (int64_t b = threadIdx.x; b < nBlocks; b += 256) s += part[b * nTerms + q];
-    sh[threadIdx.x] = s;
+  __shared__ double sh[4][256];
+  const int t = threadIdx.x;
+  double acc[4] = {0., 0., 0., 0.};
+  // all terms of a partial at once (their loads are independent); per term the order is
+  // workgroups t, t+256, ...
+#pragma unroll 4
+  for (int64_t b = t; b < nBlocks; b += 256) {
+    const double* p = part + b * nTerms;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (q < nTerms) acc[q] += p[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++) sh[q][t] = acc[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w)
+#pragma unroll
+      for (int q = 0; q < 4; q++) sh[q][t] += sh[q][t + w];
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      const int t = q == 0 ? target.x : q == 1 ? target.y : q == 2 ? target.z : target.w;
-      energy[t] = ((setMask >> q) & 1) ? sh[0] : energy[t] + sh[0];
-    }
-    __syncthreads();
+  }
+  if (t < nTerms) {
+    const int e = t == 0 ? target.x : t == 1 ? target.y : t == 2 ? target.z : target.w;
+    energy[e] = ((setMask >> t) & 1) ? sh[t][0] : energy[e] + sh[t][0];
   }
 }
 
