@@ -29,7 +29,7 @@
  *   wdg_observables    DemFuncs::coordNumber/porosity    src/dem/Funcs.cpp:84-114
  *   wdg_stress         DemFuncs::stressStiffness (stress) src/dem/Funcs.cpp:183-229
  *   wdg_kinetic_energy DemData kinetic energy             src/dem/Particle.cpp:247-265
- *   wdg_set_energy_tracking / wdg_energy / wdg_energy_clear
+ *   wdg_set_energy_tracking / wdg_energy / wdg_energy_clear / wdg_energy_set
  *                      Scene.trackEnergy, Scene.energy (EnergyTracker)
  *                                                        src/core/Scene.hpp:204,223, src/core/EnergyTracker.hpp:30-61
  *   wdg_run            Scene::doOneStep (engine loop)    src/core/Scene.cpp:366-467
@@ -225,6 +225,9 @@ int wdg_set_energy_tracking(wdg_ctx* ctx, int32_t on);
 int wdg_energy(wdg_ctx* ctx, double out[7]);
 /* EnergyTracker.clear (src/core/EnergyTracker.hpp:60): every term to zero */
 int wdg_energy_clear(wdg_ctx* ctx);
+/* EnergyTracker.set / __setitem__ (src/core/EnergyTracker.hpp:46-49,57): the terms from
+ * in[WDG_EN_N] in WDG_EN_* order, e.g. those of a checkpoint */
+int wdg_energy_set(wdg_ctx* ctx, const double in[7]);
 int wdg_stats(wdg_ctx* ctx, wdg_counters* out);
 
 /* run nSteps of Scene::doOneStep with a fixed engine list; *step and *dt are the

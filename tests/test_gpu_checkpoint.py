@@ -90,3 +90,31 @@ def test_restore_rejects_inconsistent_contacts():
     dup["c_data"] = np.concatenate([ck["c_data"], ck["c_data"][:1]])
     with pytest.raises(WdgError, match="twice"):
         Device.from_checkpoint(dup)
+
+
+def test_resume_keeps_energy_terms(tmp_path):
+    # Scene.energy travels with the checkpoint; the resumed terms equal the uninterrupted run's
+    # up to summation order (the sums run over the workgroups of a different storage order)
+    from woo_amd._native import Device
+
+    sc = scene("frict_deposition")
+    g = GpuRunner(sc)
+    g.d.set_energy_tracking(True)
+    g.run(ENG, 100)
+    saved = g.d.checkpoint()
+    np.savez(tmp_path / "ck.npz", **saved)
+    ck = dict(np.load(tmp_path / "ck.npz"))
+    assert bool(ck["track_energy"]) and ck["energy"][4] > 0  # plast
+    g.run(ENG, 60)
+    r = GpuRunner(sc)
+    r.d.close()
+    r.d = Device.from_checkpoint(ck, verletDist=r.vd)
+    r.step = 100
+    e0 = r.d.energy()
+    assert [e0[k] for k in e0] == list(ck["energy"])
+    r.run(ENG, 60)
+    np.testing.assert_array_equal(r.d.get("pos"), g.d.get("pos"))
+    a, b = g.d.energy(), r.d.energy()
+    tot = sum(abs(v) for v in a.values())
+    for k in a:
+        assert abs(a[k] - b[k]) <= 1e-11 * max(abs(a[k]), 1e-3 * tot), (k, a[k], b[k])

@@ -29,7 +29,7 @@ EXPORTS = [
     "wdg_mpnn_set_params", "wdg_mpnn_set_graph", "wdg_mpnn_forward", "wdg_mpnn_grad", "wdg_mpnn_adam",
     "wdg_mpnn_graph_from_ctx", "wdg_mpnn_advance", "wdg_snapshot", "wdg_restore", "wdg_snapshot_distance",
     "wdg_mpnn_set_targets", "wdg_mpnn_keep_graph", "wdg_mpnn_use_graph", "wdg_mpnn_get_graph",
-    "wdg_set_energy_tracking", "wdg_energy", "wdg_energy_clear", "wdg_restore_contacts",
+    "wdg_set_energy_tracking", "wdg_energy", "wdg_energy_clear", "wdg_restore_contacts", "wdg_energy_set",
 ]
 
 # Scene.energy terms, in WDG_EN_* order
@@ -128,6 +128,7 @@ def lib():
     L.wdg_set_energy_tracking.argtypes = [V, C.c_int32]
     L.wdg_energy.argtypes = [V, _P(C.c_double)]
     L.wdg_energy_clear.argtypes = [V]
+    L.wdg_energy_set.argtypes = [V, _P(C.c_double)]
     L.wdg_restore_contacts.argtypes = [V, C.c_int64, _P(C.c_int32), _P(C.c_int32), _P(C.c_double), C.c_double]
     L.wdg_set_precision.argtypes = [V, C.c_int32]
     L.wdg_run.argtypes = [V, _P(Program), C.c_int64, _P(C.c_int64), _P(C.c_double)]
@@ -169,6 +170,7 @@ class Device:
         self.n = 0
         # what was set up, kept for checkpoint()
         self._field, self._mats, self._static, self._scenes = None, None, None, None
+        self._tracking = False
 
     def close(self):
         if self.h:
@@ -312,6 +314,9 @@ class Device:
             ck["scenes_dt"] = self._scenes
         ids, cd, data = self.contacts(real_only=True)
         ck.update(c_ids=ids, c_cellDist=cd, c_data=data)
+        # Scene.trackEnergy and Scene.energy
+        ck["energy"] = np.array([self.energy()[k] for k in ENERGY_TERMS])
+        ck["track_energy"] = np.array(self._tracking)
         return ck
 
     @classmethod
@@ -328,6 +333,9 @@ class Device:
         if "scenes_dt" in ck:
             d.set_scenes(np.asarray(ck["scenes_dt"]))
         d.restore_contacts(ck["c_ids"], ck["c_cellDist"], ck["c_data"], verletDist)
+        if "energy" in ck:
+            d.energy_set(np.asarray(ck["energy"]))
+            d.set_energy_tracking(bool(ck["track_energy"]))
         return d
 
     def observables(self, lo, hi):
@@ -355,6 +363,7 @@ class Device:
     def set_energy_tracking(self, on=True):
         """Scene.trackEnergy: wdg_run adds the EnergyTracker terms while on"""
         check(self.L.wdg_set_energy_tracking(self.h, int(bool(on))))
+        self._tracking = bool(on)
 
     def energy(self):
         """Scene.energy as a dict of the terms (ENERGY_TERMS)"""
@@ -364,6 +373,13 @@ class Device:
 
     def energy_clear(self):
         check(self.L.wdg_energy_clear(self.h))
+
+    def energy_set(self, terms):
+        """the terms from a dict of ENERGY_TERMS (missing ones zero) or a sequence in that order"""
+        if isinstance(terms, dict):
+            terms = [terms.get(k, 0.) for k in ENERGY_TERMS]
+        a = np.ascontiguousarray(terms, dtype=np.float64).reshape(7)
+        check(self.L.wdg_energy_set(self.h, dptr(a)))
 
     def stats(self):
         s = Counters()

@@ -761,6 +761,16 @@ int wdg_energy(wdg_ctx* ctx, double out[7]) {
   return guard(ctx, [&] { energy_read(ctx->c, out); });
 }
 
+int wdg_energy_set(wdg_ctx* ctx, const double in[7]) {
+  if (!ctx || !in) { g_err = "NULL argument"; return WDG_E_ARG; }
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    c.energy.ensure(WDG_EN_N);
+    WDG_HIP(hipMemcpyAsync(c.energy.p, in, WDG_EN_N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    WDG_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
 int wdg_energy_clear(wdg_ctx* ctx) {
   if (!ctx) { g_err = "NULL argument"; return WDG_E_ARG; }
   return guard(ctx, [&] { energy_clear(ctx->c); });
