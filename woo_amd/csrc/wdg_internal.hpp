@@ -144,7 +144,7 @@ struct Contacts {
 // index of s is the number of real slots of the group before s. Reads and writes of the
 // state therefore stream, and no global prefix sum is needed to find an index.
 constexpr int TILE = 256;     // slots per prefix-count tile (4 words)
-constexpr int GROUP = 4096;   // slots per group (16 tiles)
+constexpr int GROUP = 2048;   // slots per group (8 tiles)
 constexpr int GROUP_BS = 1024;  // threads of a group kernel
 inline int64_t ntiles(int64_t nslots) { return (nslots + TILE - 1) / TILE; }
 inline int64_t ngroups(int64_t nslots) { return (nslots + GROUP - 1) / GROUP; }
@@ -189,7 +189,7 @@ struct RealView {
 // The used part of a group's range starts at a rotated offset (golden-ratio skew), so
 // that the densely used regions of consecutive groups spread over all HBM channels
 // instead of all starting at the same power-of-two alignment.
-constexpr uint32_t GROUP_SKEW = 2531;
+constexpr uint32_t GROUP_SKEW = 1265;
 __host__ __device__ inline uint32_t state_index(uint32_t g, uint32_t local) {
   return g * (uint32_t)GROUP + ((local + g * GROUP_SKEW) & (uint32_t)(GROUP - 1));
 }
@@ -238,7 +238,7 @@ struct Ctx {
   // graph runner: device words [0] stop, [1] steps executed, [2] renorm flag of the step,
   // [3..4] first step number (int64), [5] trsfRenorm
   DBuf<uint32_t> gstate;
-  wdg_tuning tune{4, 8};
+  wdg_tuning tune{2, 8};  // until wdg_tune / wdg_set_tuning: 2 force workgroups per group of slots
   int64_t gen = 0;  // bumped whenever buffers a captured graph uses may change
   bool graphCapture = false;  // kernels being launched are captured for replay
   struct GraphCache {
