@@ -249,8 +249,9 @@ int wdg_sync(wdg_ctx* ctx);
  * scene (the dt passed with the engine parameters is ignored). nScenes = 0 switches back.
  * Only the engine list [Collider, ContactLoop, Leapfrog] is run in this mode (no DynDt). */
 int wdg_set_scenes(wdg_ctx* ctx, int32_t nScenes, const double* dt);
-/* tuning of the graph runner: workgroups per contact group of the force kernel, steps per
- * captured graph (even) */
+/* tuning of the engine: workgroups of the force kernel per group of 2048 contact slots (also
+ * used by wdg_run and wdg_contact_loop), steps per captured graph (even); {2, 8} until
+ * wdg_tune or wdg_set_tuning changes it */
 typedef struct {
   int32_t cfBlocks;
   int32_t graphSteps;
