@@ -103,7 +103,7 @@ def test_autotuner_picks_a_candidate_and_keeps_results_exact():
     d1, _ = build_batch(scenes)
     t, step, taken = d1.tune(ENG, 0, sc["dt"], **kw)
     print("tuned", t, "steps taken", taken)
-    assert t["cfBlocks"] in (2, 4, 8) and t["graphSteps"] in (4, 8, 16)
+    assert t["cfBlocks"] in (1, 2, 4) and t["graphSteps"] in (4, 8, 16)
     assert step == taken
     step, _ = d1.run(ENG, 100, step, sc["dt"], graph=True, **kw)
     d0, _ = build_batch(scenes)

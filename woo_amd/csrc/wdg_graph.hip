@@ -201,7 +201,7 @@ void run_graph(Ctx& c, const wdg_program* prog, int64_t nSteps, int64_t* step, d
 // ------------------------------------------------------------------ autotuner
 int64_t tune(Ctx& c, const wdg_program* prog, int64_t* step, double* dt) {
   check_program(prog);
-  const int cfCand[] = {2, 4, 8};
+  const int cfCand[] = {1, 2, 4};
   const int kCand[] = {4, 8, 16};
   wdg_tuning best = c.tune;
   double bestT = INFINITY;
