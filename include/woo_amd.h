@@ -318,10 +318,16 @@ int wdg_mpnn_advance(wdg_mpnn* m, wdg_ctx* ctx, double dT);
 /* In-memory snapshots of the kinematic state, the role of Scene::saveTmp / loadTmp
  * (src/core/Scene.cpp:291-294) for the device: wdg_snapshot keeps positions, velocities,
  * angular velocities, orientations, forces and torques of every particle on the device in slot
- * 0..3; wdg_restore writes them back and restarts the contact history and the collider (the
- * next collider call rebuilds, resolving its verletDist again, as after an upload);
- * wdg_snapshot_distance gives the largest distance between a sphere's position and its
- * position in the snapshot. A snapshot is valid until the next wdg_upload_particles. */
+ * 0..3, and in the FP64 contact form the contact history (every real contact's L6Geom and
+ * force state, keyed by the particles' user ids); wdg_restore writes them back: the slot list
+ * is rebuilt at the snapshot's positions and Verlet distance and the saved contacts are real
+ * again with their state, so stepping on from a restored snapshot gives the trajectory the
+ * state would have had, bit for bit (FP32 form, or a snapshot without contacts: the history
+ * restarts and the next collider call rebuilds, resolving its verletDist again, as after an
+ * upload); wdg_snapshot_distance gives the largest distance between a sphere's position and
+ * its position in the snapshot. A snapshot is valid until the next wdg_upload_particles.
+ * wdg_mpnn_advance empties the contact history (the coarse step replaces the state it
+ * described). */
 int wdg_snapshot(wdg_ctx* ctx, int32_t slot);
 int wdg_restore(wdg_ctx* ctx, int32_t slot);
 int wdg_snapshot_distance(wdg_ctx* ctx, int32_t slot, double* maxDist);

@@ -109,3 +109,77 @@ def test_autotuner_picks_a_candidate_and_keeps_results_exact():
     d0, _ = build_batch(scenes)
     d0.run(ENG, step, 0, sc["dt"], **kw)
     np.testing.assert_array_equal(d1.get("pos"), d0.get("pos"))
+
+
+def scenes_c4_sample(k=16, seed=71, nmin=500, nmax=5000):
+    """C4-shaped scenes: N log-uniform in [nmin, nmax] (the smallest and largest forced in),
+    E log-uniform in [1e6, 1e8] Pa, tanPhi uniform in [0.1, 0.9], each with its own box and
+    dt = 0.5 pWaveDt; most start as a dense lattice (contacts from the first step), some as
+    a loose random placement in a box of edge 0.2 (N / 2000)^(1/3)"""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(nmin), np.log(nmax), size=k)).astype(int)
+    ns[0], ns[1] = nmin, nmax
+    out = []
+    for i, n in enumerate(ns):
+        E = float(np.exp(rng.uniform(np.log(1e6), np.log(1e8))))
+        tp = float(rng.uniform(.1, .9))
+        if i % 4 == 3:
+            sc = deposition(n=int(n), seed=seed + i, box=0.2 * (n / 2000.) ** (1 / 3.), young=E, tanPhi=tp,
+                            vel_sigma=0.3)
+        else:
+            sc = deposition(n=int(n), seed=seed + i, dense=True, vel_sigma=0.08, young=E, tanPhi=tp)
+        out.append(translated(sc, (0.3 * (i % 4), 0.3 * (i // 4), 0.)))
+    return out
+
+
+def test_c4_batch_against_oracle_per_scene():
+    """16 C4-shaped scenes (500-5000 spheres, own E, tanPhi and dt) stepped together, graph
+    replayed, through several refreshes; each scene against the oracle running it alone:
+    real-contact sets equal, positions within 1e-9 r (Scene::doOneStep per scene,
+    Scene.cpp:366-467)."""
+    from tests.scenes import build_oracle
+
+    scenes = scenes_c4_sample()
+    steps = 240
+    d, _ = batch_run(scenes, steps, graph=True, tuning=dict(cfBlocks=2, graphSteps=8))
+    st = d.stats()
+    print("batch: particles", st["nParticles"], "real", st["nReal"], "refreshes", st["nRefresh"], flush=True)
+    assert st["nRefresh"] >= 3, "the window must hold at least two refreshes after the first"
+    P = d.get("pos")
+    ids = d.contacts(real_only=True, data=False)[0]
+    total_real = 0
+    for i, (sc, sl) in enumerate(zip(scenes, slices(scenes))):
+        o = build_oracle(sc)
+        o.run(ENG, steps)
+        r = sc["radius"][0]
+        err = float(np.max(np.abs(P[sl] - o.get("pos")))) / r
+        mine = ids[(ids[:, 0] >= sl.start) & (ids[:, 0] < sl.stop)] - sl.start
+        theirs = o.contacts(real_only=True, data=False)[0]
+        total_real += len(theirs)
+        print(f"scene {i}: N={len(sc['radius'])} E={sc['materials'][0]['young']:.3g} "
+              f"dt={sc['dt']:.3g} real={len(theirs)} pos err/r={err:.2e}", flush=True)
+        # both lists are in role order (pA, pB), sorted by (min id, max id)
+        assert np.array_equal(mine, theirs), f"scene {i}: real contacts differ"
+        assert err <= 1e-9, f"scene {i}: positions differ by {err} r"
+    assert total_real > 10000
+
+
+def test_graph_replay_follows_field_and_material_changes():
+    # gravity is captured by value in a step graph and the material table by address: changing
+    # either recaptures, so the replay stays the plain loop bit for bit
+    sc = deposition(n=600, seed=66, dense=True, vel_sigma=0.05)
+    kw = dict(contact=contact_kw(sc), leapfrog=dict(damping=sc["damping"]), dynDtPeriod=0)
+    out = []
+    for graph in (False, True):
+        g = GpuRunner(sc)
+        g.d.set_tuning(cfBlocks=2, graphSteps=8)
+        step, _ = g.d.run(ENG, 40, 0, sc["dt"], graph=graph, **kw)
+        g.d.set_field(gravity=(0., 3., -5.), periodic=False)
+        step, _ = g.d.run(ENG, 40, step, sc["dt"], graph=graph, **kw)
+        # 70 materials: the device table is reallocated; material 0 stays the scene's
+        mats = [dict(sc["materials"][0], tanPhi=0.2)] + [sc["materials"][0]] * 69
+        g.d.set_materials(mats)
+        g.d.run(ENG, 40, step, sc["dt"], graph=graph, **kw)
+        out.append({f: g.d.get(f) for f in ("pos", "vel", "angVel")})
+    for f in out[0]:
+        np.testing.assert_array_equal(out[1][f], out[0][f], err_msg=f)

@@ -183,3 +183,23 @@ def test_graph_replay_tracks_like_the_plain_loop(kind):
     o2 = build_oracle(sc, trackEnergy=1)
     o2.run(ENG, taken)
     compare(o2.energy(), g.d.energy())
+
+
+def test_graph_replay_zeroes_elast_without_contact_slots():
+    # two spheres fly apart from an overlap: once the slot list is empty, every step's elast is
+    # the empty sum (EnergyTracker resets it each step), in graph replay as in the plain loop
+    sc = deposition(n=2, dense=True, seed=67)
+    sc = dict(sc, walls=[], gravity=(0., 0., 0.), vel=np.array([[0., 0., -.1], [0., 0., .1]]), damping=0.)
+    kw = dict(contact=contact_kw(sc), leapfrog=dict(damping=0.), dynDtPeriod=0)
+    runs = []
+    for graph in (False, True):
+        g = GpuRunner(sc)
+        g.d.set_tuning(cfBlocks=1, graphSteps=4)
+        g.d.set_energy_tracking(True)
+        step, _ = g.d.run(ENG, 1, 0, sc["dt"], graph=graph, **kw)
+        assert g.d.energy()["elast"] > 0.  # touching
+        g.d.run(ENG, 60, step, sc["dt"], graph=graph, **kw)
+        assert g.d.stats()["nCandidates"] == 0
+        runs.append((g.d.get("pos"), g.d.energy()))
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    assert runs[1][1]["elast"] == 0. and runs[0][1] == runs[1][1]

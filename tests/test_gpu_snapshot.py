@@ -20,10 +20,16 @@ def test_restore_replays_the_same_trajectory():
     d.snapshot(1)
     assert d.snapshot_distance(1) == 0.
     p0 = d.get("pos")
-    # a first continuation from the snapshot (restored, so the contact history restarts)
+    # the uninterrupted run
+    d.run(eng, 80, step, sc["dt"], **kw)
+    pu, vu = d.get("pos"), d.get("vel")
+    # a continuation from the snapshot: the contact history comes back with it, so it is the
+    # uninterrupted run bit for bit
     d.restore(1)
+    assert np.array_equal(d.get("pos"), p0)
     d.run(eng, 80, step, sc["dt"], **kw)
     p1, v1 = d.get("pos"), d.get("vel")
+    assert np.array_equal(p1, pu) and np.array_equal(v1, vu)
     moved = d.snapshot_distance(1)
     sph = a["shape"] == 0
     assert moved == pytest.approx(np.max(np.linalg.norm((p1 - p0)[sph], axis=1)), rel=1e-12)

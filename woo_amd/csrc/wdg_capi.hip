@@ -91,6 +91,23 @@ __global__ void k_export(ExportArgs a) {
   o[12] = uN; o[13] = kn; o[14] = kt; o[15] = 1.;
 }
 
+// the record export of the current slot list into out (16 doubles per slot)
+static ExportArgs export_args(Ctx& c, double* out) {
+  Contacts& C = c.con();
+  Reals& RS = c.reals();
+  ExportArgs a{};
+  a.nc = C.n; a.A = C.A.p; a.B = C.B.p; a.info = C.info.p; a.rv = c.rview();
+  a.N0 = RS.hN[0].p; a.N1 = RS.hN[1].p; a.N2 = RS.hN[2].p;
+  a.Y0 = RS.hY[0].p; a.Y1 = RS.hY[1].p; a.Y2 = RS.hY[2].p;
+  a.F0 = RS.hF[0].p; a.F1 = RS.hF[1].p; a.F2 = RS.hF[2].p; a.uN = RS.huN.p; a.gp = RS.gp.p;
+  a.rad = c.radius.p; a.mass = c.mass.p; a.mat = c.mat.p; a.mats = c.dMats.p; a.fgeo = c.fgeo.p;
+  a.px = c.pos[0].p; a.py = c.pos[1].p; a.pz = c.pos[2].p;
+  a.hertz = c.lastPhysKind == WDG_PHYS_HERTZ;
+  a.poisson = c.lastPoisson; a.en = c.lastEn; a.viscAlpha = c.lastViscAlpha;
+  a.out = out;
+  return a;
+}
+
 // a checkpoint's real contacts written back into the current real set: per slot, 1 + the
 // index of its record (0: not real), records in the wdg_contact_list layout
 struct RestoreArgs {
@@ -200,6 +217,116 @@ void restore_contacts(Ctx& c, int64_t n, const int32_t* ids, const int32_t* cell
   c.gen++;
 }
 
+// ------------------------------------------------------------------ contact history of snapshots
+// Per slot: its key (min uid << 32 | max uid) when real, else nkey (above every real key), the
+// slot itself as the value, and its packed cellDist
+__global__ void k_snap_keys(int64_t nc, const uint32_t* __restrict__ A, const uint32_t* __restrict__ B, RealView rv,
+                            const uint32_t* __restrict__ uid, uint64_t nkey, const uint64_t* __restrict__ dist,
+                            uint64_t* key, uint32_t* val, uint64_t* cd) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nc) return;
+  uint64_t k = nkey;
+  if (rv_real(rv, (uint32_t)s)) {
+    const uint64_t a = uid[A[s]], b = uid[B[s]];
+    k = (a < b ? a : b) << 32 | (a < b ? b : a);
+  }
+  key[s] = k;
+  val[s] = (uint32_t)s;
+  if (dist) cd[s] = dist[s];
+}
+
+// Per slot of the refreshed list: 1 + the slot of the same pair among the snapshot's real
+// contacts (binary search over the sorted keys), or 0. The pair's periodic image must be the
+// saved one.
+__global__ void k_snap_match(int64_t nc, const uint32_t* __restrict__ A, const uint32_t* __restrict__ B,
+                             const uint32_t* __restrict__ uid, const uint64_t* __restrict__ dist,
+                             const uint64_t* __restrict__ key, const uint32_t* __restrict__ slot, int64_t nOld,
+                             const uint64_t* __restrict__ oldCd, uint32_t* rec, uint32_t* dflags) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nc) return;
+  const uint64_t a = uid[A[s]], b = uid[B[s]];
+  const uint64_t k = (a < b ? a : b) << 32 | (a < b ? b : a);
+  int64_t lo = 0, hi = nOld;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (key[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  uint32_t r = 0u;
+  if (lo < nOld && key[lo] == k) {
+    const uint32_t os = slot[lo];
+    if (dist && oldCd[os] != dist[s]) atomicOr(&dflags[1], ERR_SNAP_IMAGE);
+    r = os + 1u;
+  }
+  rec[s] = r;
+}
+
+void snapshot_contacts_save(Ctx& c, int slot) {
+  c.snapReal[slot] = 0;
+  Contacts& C = c.con();
+  if (c.precision != 64 || C.n == 0 || c.n == 0) return;
+  const int64_t nReal = count_real(c);
+  if (nReal == 0) return;
+  const int64_t nc = C.n;
+  const bool per = c.field.periodic != 0;
+  c.snapKey[slot].ensure(nc);
+  c.snapSlot[slot].ensure(nc);
+  c.snapRec[slot].ensure((size_t)nc * 16);
+  if (per) c.snapCd[slot].ensure(nc);
+  c.bigKeyTmp.ensure(nc);
+  c.bigValTmp.ensure(nc);
+  hipLaunchKernelGGL(k_snap_keys, dim3(nblocks(nc, 256)), dim3(256), 0, c.stream, nc, (const uint32_t*)C.A.p,
+                     (const uint32_t*)C.B.p, c.rview(), (const uint32_t*)c.uid.p, (uint64_t)c.n << 32,
+                     per ? (const uint64_t*)C.dist.p : nullptr, c.snapKey[slot].p, c.snapSlot[slot].p,
+                     per ? c.snapCd[slot].p : nullptr);
+  WDG_LAUNCH_CHECK();
+  radix_sort_pairs<uint64_t>(c, c.snapKey[slot].p, c.snapSlot[slot].p, c.bigKeyTmp.p, c.bigValTmp.p, nc,
+                             32 + bits_for((uint64_t)c.n));
+  hipLaunchKernelGGL(k_export<64>, dim3(nblocks(nc, 256)), dim3(256), 0, c.stream, export_args(c, c.snapRec[slot].p));
+  WDG_LAUNCH_CHECK();
+  c.snapReal[slot] = nReal;
+  c.snapVerlet[slot] = c.verletDist;
+}
+
+// after the kinematic state of the snapshot was loaded and the contacts dropped
+void snapshot_contacts_load(Ctx& c, int slot) {
+  const int64_t nReal = c.snapReal[slot];
+  if (nReal == 0 || c.precision != 64) return;
+  // the slot list of the snapshot's Verlet distance, rebuilt at its positions: it holds every
+  // saved real contact (their particles touch, so their boxes overlap)
+  c.verletDist = c.snapVerlet[slot];
+  wdg_collide_params cp{c.verletDist, 1};
+  collide(c, &cp);
+  Contacts& C = c.con();
+  const int64_t nc = C.n;
+  const bool per = c.field.periodic != 0;
+  c.snapMatch.ensure((size_t)std::max<int64_t>(nc, 1));
+  if (nc > 0) {
+    hipLaunchKernelGGL(k_snap_match, dim3(nblocks(nc, 256)), dim3(256), 0, c.stream, nc, (const uint32_t*)C.A.p,
+                       (const uint32_t*)C.B.p, (const uint32_t*)c.uid.p, per ? (const uint64_t*)C.dist.p : nullptr,
+                       (const uint64_t*)c.snapKey[slot].p, (const uint32_t*)c.snapSlot[slot].p, nReal,
+                       per ? (const uint64_t*)c.snapCd[slot].p : nullptr, c.snapMatch.p, c.dflags.p);
+    WDG_LAUNCH_CHECK();
+    reals_from_flags(c, c.snapMatch.p, nc, c.rcur);
+    Reals& R = c.reals();
+    RestoreArgs a{};
+    a.nc = nc; a.rec = c.snapMatch.p; a.data = c.snapRec[slot].p; a.rv = c.rview();
+    a.dist = per ? C.dist.p : nullptr;
+    a.N0 = R.hN[0].p; a.N1 = R.hN[1].p; a.N2 = R.hN[2].p;
+    a.Y0 = R.hY[0].p; a.Y1 = R.hY[1].p; a.Y2 = R.hY[2].p;
+    a.F0 = R.hF[0].p; a.F1 = R.hF[1].p; a.F2 = R.hF[2].p; a.uN = R.huN.p; a.gp = R.gp.p;
+    hipLaunchKernelGGL(k_restore, dim3(nblocks(nc, 256)), dim3(256), 0, c.stream, a);
+    WDG_LAUNCH_CHECK();
+  }
+  check_errors(c);
+  const int64_t got = count_real(c);
+  if (got != nReal)
+    throw Error(WDG_E_STATE, "snapshot: " + std::to_string(nReal - got) +
+                                 " saved contacts are not in the slot list rebuilt at the snapshot's positions");
+  c.forcesPending = false;
+  c.gen++;
+}
+
 }  // namespace wdg
 
 namespace {
@@ -296,16 +423,7 @@ static std::vector<Record> collect_contacts(Ctx& c, int which, std::vector<doubl
     if (rec) {
       DBuf<double> out;
       out.ensure((size_t)nc * 16);
-      ExportArgs a{};
-      a.nc = nc; a.A = C.A.p; a.B = C.B.p; a.info = C.info.p; a.rv = c.rview();
-      a.N0 = RS.hN[0].p; a.N1 = RS.hN[1].p; a.N2 = RS.hN[2].p;
-      a.Y0 = RS.hY[0].p; a.Y1 = RS.hY[1].p; a.Y2 = RS.hY[2].p;
-      a.F0 = RS.hF[0].p; a.F1 = RS.hF[1].p; a.F2 = RS.hF[2].p; a.uN = RS.huN.p; a.gp = RS.gp.p;
-      a.rad = c.radius.p; a.mass = c.mass.p; a.mat = c.mat.p; a.mats = c.dMats.p; a.fgeo = c.fgeo.p;
-      a.px = c.pos[0].p; a.py = c.pos[1].p; a.pz = c.pos[2].p;
-      a.hertz = c.lastPhysKind == WDG_PHYS_HERTZ;
-      a.poisson = c.lastPoisson; a.en = c.lastEn; a.viscAlpha = c.lastViscAlpha;
-      a.out = out.p;
+      ExportArgs a = export_args(c, out.p);
       hipLaunchKernelGGL(c.precision == 32 ? k_export<32> : k_export<64>, dim3(nblocks(nc, 256)), dim3(256), 0,
                          c.stream, a);
       WDG_LAUNCH_CHECK();
@@ -408,6 +526,10 @@ int wdg_ctx_destroy(wdg_ctx* ctx) {
     c.graph.exec = nullptr; c.bigOff.release(); c.bigRank.release(); c.bigVal.release();
     c.bigValTmp.release(); c.bigDest.release(); c.bigKey.release(); c.bigKeyTmp.release(); c.bigList.release();
     for (auto& b : c.snap) b.release();
+    for (int k = 0; k < Ctx::NSNAP; k++) {
+      c.snapKey[k].release(); c.snapCd[k].release(); c.snapSlot[k].release(); c.snapRec[k].release();
+    }
+    c.snapMatch.release();
     c.sortHist.release(); c.scanAux.release(); c.red.release(); c.xfer.release(); c.dflags.release();
     if (c.hflags) (void)hipHostFree(c.hflags);
     if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -426,11 +548,19 @@ int wdg_set_field(wdg_ctx* ctx, const wdg_field* f) {
     bool changed = f->periodic != c.field.periodic;
     for (int k = 0; k < 3; k++) changed = changed || f->cell[k] != c.field.cell[k];
     changed = changed || f->distFactor != c.field.distFactor || f->loneMask != c.field.loneMask;
-    // the reset value of the forces depends on gravity
-    for (int k = 0; k < 3; k++)
-      if (f->gravity[k] != c.field.gravity[k]) c.forceClean = false;
+    // the reset value of the forces depends on gravity, and a captured step graph holds it by
+    // value in its leapfrog arguments
+    bool gravChanged = false;
+    for (int k = 0; k < 3; k++) gravChanged = gravChanged || f->gravity[k] != c.field.gravity[k];
+    if (gravChanged) {
+      c.forceClean = false;
+      c.gen++;
+    }
     c.field = *f;
-    if (changed) c.boundsValid = false;
+    if (changed) {
+      c.boundsValid = false;
+      c.gen++;
+    }
   });
 }
 
@@ -438,8 +568,13 @@ int wdg_set_materials(wdg_ctx* ctx, const wdg_material* mats, int32_t n) {
   if (!ctx || (n > 0 && !mats) || n < 0) { g_err = "bad materials"; return WDG_E_ARG; }
   return guard(ctx, [&] {
     Ctx& c = ctx->c;
+    // kernels still queued on the engine's stream may read the table
+    WDG_HIP(hipStreamSynchronize(c.stream));
     c.mats.assign(mats, mats + n);
+    const wdg_material* before = c.dMats.p;
     c.dMats.ensure((size_t)std::max(n, 1));
+    // a captured step graph holds the table's address
+    if (c.dMats.p != before) c.gen++;
     if (n) WDG_HIP(hipMemcpy(c.dMats.p, mats, n * sizeof(wdg_material), hipMemcpyHostToDevice));
   });
 }
@@ -697,6 +832,7 @@ int wdg_set_scenes(wdg_ctx* ctx, int32_t nScenes, const double* dt) {
     for (int32_t k = 0; k < nScenes; k++)
       if (!(dt[k] > 0)) throw Error(WDG_E_ARG, "scene " + std::to_string(k) + ": dt must be positive");
     if (c.forcesPending) flush_forces(c);
+    WDG_HIP(hipStreamSynchronize(c.stream));
     c.nScenes = nScenes;
     scene_bodies(c);
     c.dtScene.ensure((size_t)std::max(nScenes, 1));

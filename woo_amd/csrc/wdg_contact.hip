@@ -927,6 +927,8 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
   c.lastEn = p->en;
   c.lastViscAlpha = viscAlpha;
   if (C.n == 0 || c.n == 0) {
+    // no contact: this step's elast is zero (EnergyTracker resets it every step)
+    if (c.trackEnergy) energy_zero(c, WDG_EN_ELAST);
     c.forcesPending = false;
     return;
   }

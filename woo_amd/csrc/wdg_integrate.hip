@@ -849,6 +849,7 @@ void snapshot_save(Ctx& c, int slot) {
   }
   c.snapN[slot] = c.n;
   c.snapUpload[slot] = c.uploads;
+  snapshot_contacts_save(c, slot);
 }
 
 void snapshot_load(Ctx& c, int slot) {
@@ -861,8 +862,10 @@ void snapshot_load(Ctx& c, int slot) {
   drop_contacts(c);
   c.forceClean = false;
   c.dirtyKnown = false;
-  // the next collider call resolves its Verlet distance again, as after an upload
+  // without saved contacts the next collider call resolves its Verlet distance again, as after
+  // an upload; with them the slot list is rebuilt now at the snapshot's Verlet distance
   c.verletDist = NAN;
+  snapshot_contacts_load(c, slot);
 }
 
 double snapshot_distance(Ctx& c, int slot) {
@@ -945,6 +948,19 @@ void energy_enable(Ctx& c, bool on) {
 void energy_clear(Ctx& c) {
   if (!c.energy.p) return;
   hipLaunchKernelGGL(k_energy_reset, dim3(1), dim3(64), 0, c.stream, c.energy.p, 1);
+  WDG_LAUNCH_CHECK();
+}
+
+// one term to zero, skipped by a stopped graph replay (a step of a contact loop without slots:
+// its elast is the empty sum)
+__global__ void k_energy_zero(double* energy, int term, const uint32_t* gstop) {
+  if (gstop && *gstop) return;
+  if (threadIdx.x == 0) energy[term] = 0.;
+}
+
+void energy_zero(Ctx& c, int term) {
+  hipLaunchKernelGGL(k_energy_zero, dim3(1), dim3(64), 0, c.stream, c.energy.p, term,
+                     c.graphCapture ? (const uint32_t*)c.gstate.p : nullptr);
   WDG_LAUNCH_CHECK();
 }
 

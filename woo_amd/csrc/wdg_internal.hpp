@@ -83,7 +83,9 @@ __host__ __device__ inline int unpack_cd(uint64_t v, int k) {
 constexpr int DIRTY_PP = 12;
 
 // device error bits
-enum : uint32_t { ERR_NAN_FORCE = 1u, ERR_DOUBLE_IMAGE = 2u, ERR_CD_RANGE = 4u, ERR_ADJ = 8u, ERR_FACET = 16u };
+enum : uint32_t {
+  ERR_NAN_FORCE = 1u, ERR_DOUBLE_IMAGE = 2u, ERR_CD_RANGE = 4u, ERR_ADJ = 8u, ERR_FACET = 16u, ERR_SNAP_IMAGE = 32u
+};
 
 // ------------------------------------------------------------------ device buffers
 template <class T>
@@ -288,6 +290,16 @@ struct Ctx {
   DBuf<double> snap[NSNAP];
   int64_t snapN[NSNAP] = {-1, -1, -1, -1};
   uint64_t snapUpload[NSNAP] = {0, 0, 0, 0};
+  // contact history of each snapshot (FP64 contact form; none in the FP32 form): the keys
+  // (min uid << 32 | max uid) of the slot list's slots, sorted, real contacts first (the others
+  // hold n << 32), with the slot each had; per slot its record in the wdg_contact_list layout
+  // and its packed cellDist; the number of real contacts and the Verlet distance of that list
+  DBuf<uint64_t> snapKey[NSNAP], snapCd[NSNAP];
+  DBuf<uint32_t> snapSlot[NSNAP];
+  DBuf<double> snapRec[NSNAP];
+  int64_t snapReal[NSNAP] = {0, 0, 0, 0};
+  double snapVerlet[NSNAP] = {NAN, NAN, NAN, NAN};
+  DBuf<uint32_t> snapMatch;  // per slot of a restored list: 1 + its slot in the snapshot, or 0
   uint64_t uploads = 0;   // particle uploads so far
   uint64_t serial = 0;    // unique per context of the process
   // Scene.energy (EnergyTracker) while trackEnergy is set: the terms in WDG_EN_* order, and
@@ -360,6 +372,12 @@ void check_errors(Ctx& c);
 void relax(Ctx& c, int maxIter, double inflate, double tol, double* maxOverlap, int* iters);
 void drop_contacts(Ctx& c);
 void snapshot_save(Ctx& c, int slot);
+// the contact history part of a snapshot: kept on the device (FP64 form), and restored after a
+// full refresh at the snapshot's positions
+void snapshot_contacts_save(Ctx& c, int slot);
+void snapshot_contacts_load(Ctx& c, int slot);
+// the real set of the current slot list emptied (contact history restarts; the slots stay)
+void clear_history(Ctx& c);
 void snapshot_load(Ctx& c, int slot);
 double snapshot_distance(Ctx& c, int slot);
 // every per-entry slot of the current slot list to NO_STATE (and EIDX_FULL set)
@@ -367,6 +385,8 @@ void eidx_clear(Ctx& c);
 void energy_enable(Ctx& c, bool on);
 void energy_clear(Ctx& c);
 void energy_step_begin(Ctx& c);
+// energy[term] = 0 (graph-replay safe)
+void energy_zero(Ctx& c, int term);
 // adds the sums over nBlocks workgroup partials of nTerms terms to energy[target[q]] (bit q of
 // setMask: stores it instead)
 void energy_add(Ctx& c, int64_t nBlocks, int nTerms, const int* target, int setMask);

@@ -1320,6 +1320,9 @@ int wdg_mpnn_advance(wdg_mpnn* m, wdg_ctx* ctx, double dT) {
     WDG_HIP(hipEventRecord(s.done, s.stream));
     WDG_HIP(hipStreamWaitEvent(c.stream, s.done, 0));
     if (c.forcesPending) flush_forces(c);
+    // the contact history describes the state before the coarse step, which the network
+    // replaces: the next contact loop starts it afresh
+    clear_history(c);
     if (nSph > 0) {
       hipLaunchKernelGGL(k_mg_advance, dim3(nblocks(nSph, 256)), dim3(256), 0, c.stream, c.nW, nSph,
                          (const float*)s.out.p, (const uint8_t*)s.cflag.p, dT, MPNN_A0, c.field.gravity[0],

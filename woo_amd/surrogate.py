@@ -555,6 +555,10 @@ def run_learned(dev, net, radius, coarse_steps, dt, check_every=50, bound=0.2e-3
     k = 0
     while k < coarse_steps:
         seg = min(check_every, coarse_steps - k)
+        # the scene's step number at the segment's start: each coarse step stands for COARSE
+        # DEM steps, so the check's DEM redo starts at step0 + (seg - 1) COARSE and a fallback
+        # at step0 (the trsfRenorm phase of dem_run follows the true step)
+        step0 = step
         dev.snapshot(0)
         for j in range(seg):
             if j == seg - 1:
@@ -565,7 +569,7 @@ def run_learned(dev, net, radius, coarse_steps, dt, check_every=50, bound=0.2e-3
             stats["learned_coarse_steps"] += 1
         dev.snapshot(2)
         dev.restore(1)
-        step = dem_run(dev, COARSE, step, dt)
+        dem_run(dev, COARSE, step0 + (seg - 1) * COARSE, dt)
         stats["dem_steps"] += COARSE
         err = dev.snapshot_distance(2)
         stats["checks"] += 1
@@ -573,8 +577,9 @@ def run_learned(dev, net, radius, coarse_steps, dt, check_every=50, bound=0.2e-3
         if err > bound:
             stats["fallbacks"] += 1
             dev.restore(0)
-            step = dem_run(dev, seg * COARSE, step, dt)
+            dem_run(dev, seg * COARSE, step0, dt)
             stats["dem_steps"] += seg * COARSE
+        step = step0 + seg * COARSE
         if log:
             log(f"coarse {k + seg}/{coarse_steps}: check error {err * 1e3:.4f} mm"
                 f"{' -> DEM fallback' if err > bound else ''}")
