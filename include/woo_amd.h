@@ -212,6 +212,13 @@ int wdg_set_precision(wdg_ctx* ctx, int32_t bits);
 int wdg_stress(wdg_ctx* ctx, double volume, double out[9]);
 /* sum over integrated nodes of m v.v/2 + w.(I w)/2 */
 int wdg_kinetic_energy(wdg_ctx* ctx, double* ek);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops): wdg_kinetic_energy with the same bits, written to dOut in
+ * device memory (8 bytes, caller-owned) on the context's stream; the call does not wait for it.
+ * It calls no hipMalloc or hipFree: the per-workgroup partials live in a fixed device array of
+ * 2 MiB per GPU (2^18 doubles), held until the process ends, so at most 2^26 particles; more is
+ * WDG_E_ARG. All calls on one GPU share that array: calls that overlap on one GPU race. */
+int wdg_kinetic_energy_noalloc(wdg_ctx* ctx, double* dOut);
 /* Scene.trackEnergy (src/core/Scene.hpp:204): while on, wdg_run adds the terms of
  * Leapfrog.cpp:53-98 (kinetic at the on-step velocity, grav, nonviscDamp; a periodic scene
  * from step 1), IdealElPl.cpp:62-92 (elast, plast) and Hertz.cpp:181-206 (elast, plast,

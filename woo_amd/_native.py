@@ -30,6 +30,7 @@ EXPORTS = [
     "wdg_mpnn_graph_from_ctx", "wdg_mpnn_advance", "wdg_snapshot", "wdg_restore", "wdg_snapshot_distance",
     "wdg_mpnn_set_targets", "wdg_mpnn_keep_graph", "wdg_mpnn_use_graph", "wdg_mpnn_get_graph",
     "wdg_set_energy_tracking", "wdg_energy", "wdg_energy_clear", "wdg_restore_contacts", "wdg_energy_set",
+    "wdg_kinetic_energy_noalloc",
 ]
 
 # Scene.energy terms, in WDG_EN_* order
@@ -125,6 +126,7 @@ def lib():
     L.wdg_stats.argtypes = [V, _P(Counters)]
     L.wdg_stress.argtypes = [V, C.c_double, _P(C.c_double)]
     L.wdg_kinetic_energy.argtypes = [V, _P(C.c_double)]
+    L.wdg_kinetic_energy_noalloc.argtypes = [V, V]
     L.wdg_set_energy_tracking.argtypes = [V, C.c_int32]
     L.wdg_energy.argtypes = [V, _P(C.c_double)]
     L.wdg_energy_clear.argtypes = [V]
@@ -359,6 +361,11 @@ class Device:
         e = C.c_double()
         check(self.L.wdg_kinetic_energy(self.h, C.byref(e)))
         return e.value
+
+    def kinetic_energy_noalloc(self, dOut):
+        """wdg_kinetic_energy into 8 bytes of device memory at address dOut, enqueued on the
+        engine's stream without any device allocation (no wait)"""
+        check(self.L.wdg_kinetic_energy_noalloc(self.h, C.c_void_p(dOut)))
 
     def set_energy_tracking(self, on=True):
         """Scene.trackEnergy: wdg_run adds the EnergyTracker terms while on"""

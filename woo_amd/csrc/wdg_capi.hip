@@ -488,6 +488,8 @@ int wdg_ctx_create(int device, wdg_ctx** out) {
     WDG_HIP(hipMemset(ctx->c.dflags.p, 0, 64 * sizeof(uint32_t)));
     WDG_HIP(hipHostMalloc((void**)&ctx->c.hflags, 64 * sizeof(uint32_t), hipHostMallocDefault));
     ctx->c.red.ensure(64);
+    // the device's copy of the no-allocation entry points' scratch array
+    scratch_setup(device);
     ctx->c.field.distFactor = -1;
     ctx->c.field.loneMask = 2;
     ctx->c.field.cell[0] = ctx->c.field.cell[1] = ctx->c.field.cell[2] = 1;
@@ -885,6 +887,11 @@ int wdg_stress(wdg_ctx* ctx, double volume, double out[9]) {
 int wdg_kinetic_energy(wdg_ctx* ctx, double* ek) {
   if (!ctx || !ek) { g_err = "NULL argument"; return WDG_E_ARG; }
   return guard(ctx, [&] { *ek = kinetic_energy(ctx->c); });
+}
+
+int wdg_kinetic_energy_noalloc(wdg_ctx* ctx, double* dOut) {
+  if (!ctx || !dOut) { g_err = "NULL argument"; return WDG_E_ARG; }
+  return guard(ctx, [&] { kinetic_energy_noalloc(ctx->c, dOut); });
 }
 
 int wdg_set_energy_tracking(wdg_ctx* ctx, int32_t on) {

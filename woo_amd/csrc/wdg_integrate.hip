@@ -8,6 +8,7 @@
 // order of the other particle's user id: the order of Particle::contacts (a
 // std::map keyed by the other id), so sums are reproducible bit for bit.
 #include <cstring>
+#include <mutex>
 
 #include "wdg_internal.hpp"
 #include "wdg_math.hpp"
@@ -775,6 +776,53 @@ double kinetic_energy(Ctx& c) {
   double s = 0;
   for (double x : h) s += x;
   return s;
+}
+
+// ------------------------------------------------------------------ kinetic energy, no allocation
+// For callers that cannot allocate device memory (code under HIP graph capture, serving loops):
+// k_ek's per-workgroup partials go to a file-scope device array instead of a fresh buffer, and
+// one thread adds them in block order, left to right, as kinetic_energy does on the host, so
+// both give the same bits. Every GPU holds its own copy of the array until the process ends;
+// its address on each device is looked up once, with that device current, at context creation.
+// All calls on one GPU share the array: calls that overlap on one GPU race. The partials are
+// written in full before they are read, so nothing needs clearing.
+constexpr int64_t EK_SCRATCH_MAX = int64_t(1) << 18;  // workgroup partials: up to 2^26 particles
+constexpr int SCRATCH_DEVICES = 64;
+static __device__ __align__(256) double scratch[EK_SCRATCH_MAX];
+static double* g_scratch[SCRATCH_DEVICES];
+static std::mutex g_scratch_mu;
+
+void scratch_setup(int device) {
+  if (device < 0 || device >= SCRATCH_DEVICES) throw Error(WDG_E_ARG, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return;
+  void* p = nullptr;
+  WDG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch[device] = static_cast<double*>(p);
+}
+
+__global__ void k_sum_in_order(const double* __restrict__ part, int64_t nb, double* out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double s = 0;
+  for (int64_t b = 0; b < nb; b++) s += part[b];
+  *out = s;
+}
+
+void kinetic_energy_noalloc(Ctx& c, double* dOut) {
+  const int64_t nb = nblocks(c.n, 256);
+  if (nb > EK_SCRATCH_MAX)
+    throw Error(WDG_E_ARG, "wdg_kinetic_energy_noalloc: " + std::to_string(c.n) + " particles exceed its maximum of " +
+                               std::to_string(EK_SCRATCH_MAX * 256));
+  double* part = (c.device >= 0 && c.device < SCRATCH_DEVICES) ? g_scratch[c.device] : nullptr;
+  if (!part) throw Error(WDG_E_STATE, "the scratch array of this device was not set up");
+  if (nb > 0) {
+    hipLaunchKernelGGL(k_ek, dim3((unsigned)nb), dim3(256), 0, c.stream, c.n, c.flags.p, c.mass.p, c.vel[0].p,
+                       c.vel[1].p, c.vel[2].p, c.angVel[0].p, c.angVel[1].p, c.angVel[2].p, c.inertia[0].p,
+                       c.inertia[1].p, c.inertia[2].p, part);
+    WDG_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_sum_in_order, dim3(1), dim3(64), 0, c.stream, part, nb, dOut);
+  WDG_LAUNCH_CHECK();
 }
 
 // ------------------------------------------------------------------ in-memory snapshots

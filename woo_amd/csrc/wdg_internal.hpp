@@ -366,6 +366,8 @@ void observables(Ctx& c, const double lo[3], const double hi[3], double out[3]);
 int64_t count_real(Ctx& c);
 void stress(Ctx& c, double volume, double out[9]);
 double kinetic_energy(Ctx& c);
+void scratch_setup(int device);
+void kinetic_energy_noalloc(Ctx& c, double* dOut);
 // bits + tile prefix of R[dst] from a per-slot flag array (nonzero = real)
 void reals_from_flags(Ctx& c, const uint32_t* flag, int64_t nslots, int dst);
 void check_errors(Ctx& c);
