@@ -42,6 +42,7 @@ struct ContactArgs {
   const uint32_t* flags;
   const int32_t* mat;
   const wdg_material* mats;
+  int nMats;
   const double* fgeo;  // facet geometry per boundary body
   // batched scenes: dt of each particle's scene (else null, dt above)
   const double* dtScene;
@@ -281,9 +282,22 @@ constexpr int CF_BS = 256;
 // ENERGY: the law's energy terms (elast, plast, viscN, viscT) of the workgroup's contacts are
 // summed into a.enPart[4 * blockIdx.x ..]; every thread reaches the sum (the only early exit,
 // a stopped graph replay, is taken by the whole workgroup)
+// materials staged in LDS by the force kernels (a larger table is read from HBM beyond this)
+constexpr int MAT_LDS = 32;
+__device__ __forceinline__ void stage_materials(const ContactArgs& a, wdg_material* smat) {
+  const int nm = a.nMats < MAT_LDS ? a.nMats : MAT_LDS;
+  for (int t = threadIdx.x; t < nm; t += blockDim.x) smat[t] = a.mats[t];
+  __syncthreads();
+}
+__device__ __forceinline__ wdg_material material(const ContactArgs& a, const wdg_material* smat, uint32_t m) {
+  return m < (uint32_t)MAT_LDS ? smat[m] : a.mats[m];
+}
+
 template <bool PERIODIC, int LAW, bool FACETS, bool ENERGY>
 __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
+  __shared__ wdg_material smat[MAT_LDS];
   if (a.gstop && *a.gstop) return;
+  stage_materials(a, smat);
   const int renorm = a.renormPtr ? (int)*a.renormPtr : a.renorm;
   const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t nblk = (uint32_t)a.cfBlocks;
@@ -300,25 +314,36 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
     const bool real = dk.w != NO_STATE;
     const uint32_t ko = real ? dk.w : 0u;
     const int type = (int)(info & CI_TYPE_MASK);
-    const double dt = a.dtScene ? a.dtScene[a.scene[pA]] : a.dt;
-    uint64_t cd = 0;
-    d3 shift2 = mk3(0., 0., 0.);
-    if (PERIODIC) {
-      cd = a.descDist[k];
-      shift2 = mk3(a.L[0] * unpack_cd(cd, 0), a.L[1] * unpack_cd(cd, 1), a.L[2] * unpack_cd(cd, 2));
-    }
+    // every load of the contact at once: both particles and the previous state (read at
+    // index 0 for a new contact and not used)
     const d3 posA = mk3(a.px[pA], a.py[pA], a.pz[pA]);
     const d3 posB = mk3(a.px[pB], a.py[pB], a.pz[pB]);
-    d3 prevNormal = mk3(0., 0., 0.);
-    if (real) prevNormal = mk3(a.oN0[ko], a.oN1[ko], a.oN2[ko]);
+    const double radA = a.rad[pA], radB = a.rad[pB];
+    const d3 velA = mk3(a.vx[pA], a.vy[pA], a.vz[pA]), avA = mk3(a.wx[pA], a.wy[pA], a.wz[pA]);
+    const d3 vel2 = mk3(a.vx[pB], a.vy[pB], a.vz[pB]);
+    const d3 av2 = mk3(a.wx[pB], a.wy[pB], a.wz[pB]);
+    const double massA = a.mass[pA], massB = a.mass[pB];
+    const uint32_t matA = (uint32_t)a.mat[pA], matB = (uint32_t)a.mat[pB];
+    const d3 prevNormalL = mk3(a.oN0[ko], a.oN1[ko], a.oN2[ko]);
+    const d3 prevYL = mk3(a.oY0[ko], a.oY1[ko], a.oY2[ko]);
+    const double oFt0 = a.oF1[ko], oFt1 = a.oF2[ko];
+    const double* og = a.ogp + (int64_t)GP * ko;
+    const double2 ogc = reinterpret_cast<const double2*>(og)[2];
+    const d3 prevContPtL = mk3(og[3], ogc.x, ogc.y);
+    uint64_t cd = 0;
+    if (PERIODIC) cd = a.descDist[k];
+    const double dt = a.dtScene ? a.dtScene[a.scene[pA]] : a.dt;
+    d3 shift2 = mk3(0., 0., 0.);
+    if (PERIODIC) shift2 = mk3(a.L[0] * unpack_cd(cd, 0), a.L[1] * unpack_cd(cd, 1), a.L[2] * unpack_cd(cd, 2));
+    const d3 prevNormal = real ? prevNormalL : mk3(0., 0., 0.);
 
     d3 normal, contPt, pos1, pos2;
     double uN, r1, r2;
-    d3 vel1 = mk3(a.vx[pA], a.vy[pA], a.vz[pA]), av1 = mk3(a.wx[pA], a.wy[pA], a.wz[pA]);
+    d3 vel1 = velA, av1 = avA;
     if (type == CT_SS) {
       // Cg2_Sphere_Sphere_L6Geom::go (Sphere.cpp:92-117)
-      r1 = a.rad[pA];
-      r2 = a.rad[pB];
+      r1 = radA;
+      r2 = radB;
       pos1 = posA;
       pos2 = add3(posB, shift2);
       const d3 relPos = sub3(pos2, posA);
@@ -331,7 +356,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
       // (adjustBoundaryContactGeom returns at once); static nodes: the velocity at the contact
       // point is fakeVel (zero when its x is NaN), the angular velocity zero
       const double* g = a.fgeo + (int64_t)FGEO * pA;
-      const double radius = a.rad[pB], ht = g[12];
+      const double radius = radB, ht = g[12];
       pos2 = add3(posB, shift2);
       d3 fNormal;
       double planeDist;
@@ -342,7 +367,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
       if (dist != 0) normal = div3(nv, dist);
       else if (real) {
         // the previous normal as node.ori * UnitX
-        const d3 pY = mk3(a.oY0[ko], a.oY1[ko], a.oY2[ko]);
+        const d3 pY = prevYL;
         normal = quat_rot(quat_from_cols(prevNormal, pY, cross3(prevNormal, pY)), mk3(1., 0., 0.));
       } else normal = mk3(1., 0., 0.);
       if (ht > 0) contPt = add3(contPt, mul3(normal, ht));
@@ -356,7 +381,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
       // Cg2_Wall_Sphere_L6Geom::go (Wall.cpp:50-84), pA is the wall
       const uint32_t fl = a.flags[pA];
       const int ax = flag_axis(fl), sense = flag_sense(fl);
-      const double radius = a.rad[pB];
+      const double radius = radB;
       pos1 = posA;
       pos2 = add3(posB, shift2);
       const double dist = comp3(pos2, ax) - comp3(posA, ax);
@@ -371,8 +396,6 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
       r1 = -radius;
       r2 = radius;
     }
-    const d3 vel2 = mk3(a.vx[pB], a.vy[pB], a.vz[pB]);
-    const d3 av2 = mk3(a.wx[pB], a.wy[pB], a.wz[pB]);
 
     // Cg2_Any_Any_L6Geom__Base::handleSpheresLikeContact (L6Geom.cpp:49-216)
     d3 T0, T1, T2, gvel;
@@ -388,9 +411,8 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
       T2 = cross3(locX, locY);
       gvel = mk3(0., 0., 0.);
     } else {
-      const d3 prevY = mk3(a.oY0[ko], a.oY1[ko], a.oY2[ko]);
-      const double* og = a.ogp + (int64_t)GP * ko;
-      const d3 prevContPt = mk3(og[3], og[4], og[5]);
+      const d3 prevY = prevYL;
+      const d3 prevContPt = prevContPtL;
       const d3& currNormal = normal;
       const d3 normRotVec = cross3(prevNormal, currNormal);
       d3 midNormal = mul3(add3(prevNormal, currNormal), .5);
@@ -423,8 +445,8 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
     // Cp2 (recomputed; constant over the life of the contact)
     double l0, l1, A;
     lens_area(r1, r2, l0, l1, A);
-    const wdg_material mA = a.mats[a.mat[pA]], mB = a.mats[a.mat[pB]];
-    double Fn, Ft0 = real ? a.oF1[ko] : 0., Ft1 = real ? a.oF2[ko] : 0.;
+    const wdg_material mA = material(a, smat, matA), mB = material(a, smat, matB);
+    double Fn, Ft0 = real ? oFt0 : 0., Ft1 = real ? oFt1 : 0.;
     if (LAW == WDG_LAW_IDEALELPL) {
       // Law2_L6Geom_FrictPhys_IdealElPl::go (IdealElPl.cpp:15-125); breaking decided by k_cfilter
       PhysVals ph;
@@ -458,7 +480,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
     } else {
       // Law2_L6Geom_HertzPhys_DMT::go (Hertz.cpp:85-210), DMT branch; breaking decided by k_cfilter
       PhysVals ph;
-      phys_hertz(mA, mB, l0, l1, a.mass[pA], a.mass[pB], a.poisson, a.en, a.viscAlpha, ph);
+      phys_hertz(mA, mB, l0, l1, massA, massB, a.poisson, a.en, a.viscAlpha, ph);
       const double velN = gvel.x;
       const double kn0 = ph.K * sqrt(ph.R);
       const double s = sqrt(-uN);
@@ -850,6 +872,7 @@ static ContactArgs contact_args(Ctx& c, int from, int to) {
   a.vx = c.vel[0].p; a.vy = c.vel[1].p; a.vz = c.vel[2].p;
   a.wx = c.angVel[0].p; a.wy = c.angVel[1].p; a.wz = c.angVel[2].p;
   a.rad = c.radius.p; a.mass = c.mass.p; a.flags = c.flags.p; a.mat = c.mat.p; a.mats = c.dMats.p;
+  a.nMats = (int)c.mats.size();
   a.fgeo = c.fgeo.p;
   a.dtScene = c.nScenes > 0 ? c.dtScene.p : nullptr;
   a.scene = c.scene.p;
