@@ -518,7 +518,7 @@ int wdg_ctx_destroy(wdg_ctx* ctx) {
     c.mass.release(); c.radius.release(); c.flags.release(); c.mask.release(); c.mat.release();
     c.uid.release(); c.sto.release(); c.dMats.release();
     c.cellKey.release(); c.sortIdx.release(); c.cellKeyTmp.release(); c.sortIdxTmp.release();
-    c.perm.release(); c.invPerm.release(); c.cellStart.release(); c.cellEnd.release();
+    c.perm.release(); c.invPerm.release(); c.cellStart.release();
     c.scount.release(); c.soff.release(); c.stPartner.release(); c.stInfo.release(); c.stDist.release();
     c.adjEnt.release(); c.adjDeg.release(); c.stNb.release(); c.sdeg.release(); c.posf.release();
     c.fgeo.release(); c.bnd.release(); c.pairMax.release(); c.scene.release(); c.dtScene.release();

@@ -268,7 +268,7 @@ struct Ctx {
 
   // broadphase scratch
   DBuf<uint32_t> cellKey, sortIdx, cellKeyTmp, sortIdxTmp, perm, invPerm;
-  DBuf<uint32_t> cellStart, cellEnd;
+  DBuf<uint32_t> cellStart;  // per cell key (and one past the last): first sphere of that cell or later
   DBuf<uint32_t> scount, soff;          // pairs per generating sphere, their offsets
   DBuf<uint32_t> stPartner, stInfo;     // staged pairs, a fixed number per sphere
   DBuf<uint64_t> stDist;
