@@ -211,6 +211,14 @@ __device__ __forceinline__ uint32_t rv_index(const RealView& v, uint32_t s) {
   return state_index(s / (uint32_t)GROUP, r);
 }
 
+// A wavefront's LDS writes become visible to its other lanes (no workgroup barrier): used
+// where each wavefront reads and writes only its own LDS region.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 constexpr int NPD = 24;  // permuted double arrays per particle
 constexpr int NPU = 5;   // permuted u32 arrays per particle
 

@@ -165,12 +165,6 @@ __device__ __forceinline__ bf16x8 as_bf8(uint4 v) {
 // tile through its own LDS tile as 16-byte row stores.
 template <int MODE>
 constexpr int k_of() { return MODE == A_PLAIN ? KX : (MODE == A_EDGE ? KM : KU); }
-// LDS ordering among the lanes of one wavefront (the tile is private to the wavefront)
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 template <int MODE, bool RESID>
 __global__ __launch_bounds__(256) void k_mm_fwd(ARows a, int64_t M, const bf16* __restrict__ Wt,
                                                  const float* __restrict__ bias, bf16* y, bf16* hOut) {
