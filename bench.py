@@ -8,8 +8,11 @@ dt = 0.3 pWaveDt, engines [InsertionSortCollider, ContactLoop, Leapfrog(damping=
 plain FP64. One step = one pass of those engines over all particles.
 
 Prints one JSON line: whole-job particle-steps/s over K timed steps (inputs resident
-in HBM), the roofline of the dominant kernel (HIP-event timed on the engine's stream),
-and the CPU oracle timed on a bounded sample of the same workload.
+in HBM), with the broadphase refresh counted at its measured cost and rate; the roofline
+of the dominant kernel (HIP-event timed on the engine's stream, at a fixed state that the
+committed rocprofv3 counter summary was taken at) and of the whole step (BASELINE.md's
+algorithmic bytes per particle-step); and the CPU oracle timed on a bounded sample of the
+same workload.
 """
 import argparse
 import ctypes as C
@@ -32,6 +35,11 @@ ENGINES = ["Collider", "ContactLoop", "Leapfrog"]
 # capturing replay and 3 timed ones (2 + 3 * 4 * (4 + 8 + 16)); the run after it is the same as
 # after this many plain steps (graph replay is bit-identical to the plain loop)
 TUNE_STEPS = 338
+# the state the kernels are timed at, and the committed counter profile was taken at: this many
+# plain steps from the initial state, then KERNEL_LAUNCHES event-timed steps
+FIX_STEPS = 400
+KERNEL_LAUNCHES = 20
+PMC_PROFILE = os.path.join("profiles", "pmc_traffic_1M_hertz_fp64.md")
 
 
 def log(*a):
@@ -110,11 +118,30 @@ def algorithmic_bytes(n, slots, real):
     }
 
 
-def time_kernels(dev, wl, step, dt, k):
-    """Average device time of the contact and the (fused gather) leapfrog launches."""
+def fixed_state(wl):
+    """A device at the kernel-timing state: the workload's initial state, FIX_STEPS plain steps."""
+    from woo_amd._native import Device, SPHERE
+
+    d = Device(0)
+    n = len(wl["radius"])
+    d.set_field(periodic=True, cell=(wl["L"],) * 3)
+    d.set_materials([wl["mat"]])
+    d.upload_particles(shape=np.full(n, SPHERE, np.int32), pos=wl["pos"], radius=wl["radius"], mass=wl["mass"],
+                       inertia=wl["inertia"], vel=wl["vel"])
+    step, _ = d.run(ENGINES, FIX_STEPS, 0, wl["dt"], contact=contact_params(), leapfrog=dict(damping=0.),
+                    dynDtPeriod=0)
+    d.sync()
+    return d, step
+
+
+def time_kernels(dev, wl, step, k):
+    """Average device time of the contact loop (k_cfilter + k_cforce) and of the leapfrog (gather
+    fused) over k steps, HIP events on the engine's stream; the state (slots, real contacts)
+    after them."""
     ev = HipEvents()
     s = dev.stream()
     cp = contact_params()
+    dt = wl["dt"]
     e = [ev.new() for _ in range(4)]
     tc = tl = 0.
     for i in range(k):
@@ -131,33 +158,89 @@ def time_kernels(dev, wl, step, dt, k):
     return tc / k * 1e-3, tl / k * 1e-3, step
 
 
-def cpu_baseline(wl, n_sample_steps_max=2000, budget_s=12.):
-    from oracle import Oracle
+def refresh_cost(dev, step, dt, k=3):
+    """Average device time of a broadphase refresh (a forced collider run: cell keys, sort,
+    permutation, pair pass, history merge, adjacency), HIP events on the engine's stream; one
+    step between refreshes."""
+    ev = HipEvents()
+    s = dev.stream()
+    cp = contact_params()
+    a, b = ev.new(), ev.new()
+    t = 0.
+    for _ in range(k):
+        dev.sync()
+        ev.record(a, s)
+        dev.collide(-.05, forceInit=True)
+        ev.record(b, s)
+        t += ev.elapsed_ms(a, b)
+        dev.contact_loop(step, dt, **cp)
+        dev.leapfrog(dt, 0., True)
+        step += 1
+    return t / k * 1e-3, step
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-    threads = min(threads, 16)
+
+def whole_step_bytes(n, slots, real, k_refresh):
+    """BASELINE.md's algorithmic bytes per particle-step (FP64): 272 integrate + 80 contact-kernel
+    particle read + 48 gather write + (C/N) * 280 per contact + (104 + 8 P/N) / k refresh, with the
+    run's own contacts per particle C/N, candidate pairs per particle P/N and refresh interval k."""
+    cn, pn = real / n, slots / n
+    return 272 + 80 + 48 + cn * (8 + 176 + 96) + (104 + 8 * pn) / k_refresh, cn, pn
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(wl, budget_s=5.):
+    """The OpenMP CPU oracle (a restatement of Woo's algorithm: chunked insertion-sort collider,
+    guided contact loop with per-node mutex force accumulation, guided leapfrog), compiled here
+    with -march=native, on the same workload: all cores the process may use, then 1 thread; each
+    the median of 3 samples of whole steps (about budget_s each), after one untimed first step
+    (initial sort and contact creation, excluded as in Woo's own timings)."""
+    from oracle import Oracle, native_library
+
+    lib = native_library()
+    cores = len(os.sched_getaffinity(0))
     n = len(wl["radius"])
-    o = Oracle(dt=wl["dt"], gravity=(0, 0, 0), periodic=1, cell=(wl["L"],) * 3, physKind=1, lawKind=1,
-               damping=0., poisson=POISSON, en=EN, deterministic=0, nThreads=threads)
-    m = wl["mat"]
-    o.material(**m)
+    o = Oracle(lib=lib, dt=wl["dt"], gravity=(0, 0, 0), periodic=1, cell=(wl["L"],) * 3, physKind=1, lawKind=1,
+               damping=0., poisson=POISSON, en=EN, deterministic=0, nThreads=cores)
+    o.material(**wl["mat"])
     o.spheres(wl["pos"], wl["radius"], 0)
     o.set("vel", wl["vel"])
     o.set("mass", wl["mass"])
     o.set("inertia", wl["inertia"])
     t0 = time.perf_counter()
-    o.run(ENGINES, 1)  # initial sort + contact creation, excluded like Woo's t1
+    o.run(ENGINES, 1)
     t_first = time.perf_counter() - t0
-    steps = 0
-    t0 = time.perf_counter()
-    while steps < n_sample_steps_max and (time.perf_counter() - t0) < budget_s:
+    out = {}
+    for threads in (cores, 1):
+        o.set_params(nThreads=threads)
+        t0 = time.perf_counter()
         o.run(ENGINES, 1)
-        steps += 1
-    el = time.perf_counter() - t0
-    return dict(value=n * steps / el, unit="particle-steps/s", cores=threads, kind="port",
-                sample=f"{steps} steps of the same {n}-sphere workload after 1 untimed first step "
-                       f"({t_first:.1f} s), OpenMP oracle (per-node mutex force accumulation), "
-                       f"{el:.1f} s wall")
+        per = time.perf_counter() - t0
+        steps = max(1, int(budget_s / per))
+        rates = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            o.run(ENGINES, steps)
+            rates.append(n * steps / (time.perf_counter() - t0))
+        out[threads] = dict(value=float(np.median(rates)), samples=rates, steps_per_sample=steps)
+        log(f"cpu oracle, {threads} threads: {out[threads]}")
+    return dict(value=out[cores]["value"], unit="particle-steps/s", cores=cores, kind="port",
+                cpu=cpu_model(), os_cpu_count=os.cpu_count(), one_thread=out[1]["value"],
+                samples={"all_cores": out[cores]["samples"], "one_thread": out[1]["samples"]},
+                sample=(f"the same {n}-sphere workload after one untimed first step ({t_first:.1f} s); median of 3 "
+                        f"samples of {out[cores]['steps_per_sample']} steps on {cores} threads (all the process may "
+                        f"use) and of {out[1]['steps_per_sample']} steps on 1 thread; OpenMP oracle built with "
+                        f"-march=native on this host: Woo's algorithm restated, except that its periodic collider "
+                        f"rebuilds the pairs from a cell grid at each refresh instead of Woo's periodic "
+                        f"insertion sort"))
 
 
 def observables(dev, wl):
@@ -212,17 +295,24 @@ def fp32_form(wl, args, value64, obs64, tuning):
                 observable_errors_vs_fp64=err)
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, slots, real):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary (FETCH_SIZE and
-    WRITE_SIZE, each scaled by what it reads for a copy of known size), or None."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_1M_hertz_fp64.md")
+    WRITE_SIZE, each scaled by what it reads for a copy of known size), when that summary was taken
+    at the same state (contact slots and real contacts) as the launches timed here; else None."""
+    path = os.path.join(ROOT, PMC_PROFILE)
+    state, traffic = None, None
     try:
         for line in open(path):
+            if line.startswith("state:"):
+                kv = dict(x.split("=") for x in line.split(":", 1)[1].split())
+                state = (int(kv["slots"]), int(kv["real"]))
             if line.startswith(f"{kernel} traffic_bytes_per_launch:"):
-                return float(line.split(":")[1]), os.path.relpath(path, ROOT)
-    except OSError:
-        pass
-    return None, None
+                traffic = float(line.split(":")[1])
+    except (OSError, ValueError, KeyError):
+        return None, None
+    if traffic is None or state != (slots, real):
+        return None, None
+    return traffic, PMC_PROFILE
 
 
 def drum_workload(n, dev, seed=4):
@@ -552,22 +642,14 @@ def surrogate_form(args):
                               heldout_rmse_ballistic=err_ballistic, unit="m/s^2"))
 
 
-def pmc_probe(dev, wl, args):
-    """The main workload's steps as the benchmark runs them (warmup, timed steps, then the
-    kernel-timing steps), the last pmc_probe of which are the probed launches: the engine is
-    deterministic, so they see the same state as the launches the benchmark times."""
-    dt, cp = wl["dt"], contact_params()
-    # plain steps throughout (a profiler collecting counters does not follow graph replays): the
-    # tuner's steps as plain steps, then the timed ones
-    n = args.warmup + (TUNE_STEPS if args.graph else 0) + args.steps
-    step, _ = dev.run(ENGINES, n, 0, dt, contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
-    for _ in range(max(args.kernel_steps - args.pmc_probe, 0)):
-        dev.collide(-.05)
-        dev.contact_loop(step, dt, **cp)
-        dev.leapfrog(dt, 0., True)
-        step += 1
-    dev.sync()
-    for _ in range(args.pmc_probe):
+def pmc_probe(wl):
+    """For rocprofv3 --pmc runs: the kernel-timing launches of the benchmark (FIX_STEPS plain steps
+    from the initial state, then KERNEL_LAUNCHES steps), then a device-to-device copy of 512 MiB
+    whose counters calibrate FETCH_SIZE and WRITE_SIZE. The engine is deterministic, so the probed
+    launches see the state the benchmark times them at; prints that state."""
+    dev, step = fixed_state(wl)
+    cp, dt = contact_params(), wl["dt"]
+    for _ in range(KERNEL_LAUNCHES):
         dev.collide(-.05)
         dev.contact_loop(step, dt, **cp)
         dev.leapfrog(dt, 0., True)
@@ -589,9 +671,11 @@ def pmc_probe(dev, wl, args):
     assert L.hipDeviceSynchronize() == 0
     L.hipFree(a)
     L.hipFree(b)
-    B = algorithmic_bytes(args.n, st["nCandidates"], st["nReal"])
-    print(json.dumps({"pmc_probe": args.pmc_probe, "copy_bytes_each_way": nbytes, "algorithmic_bytes": B,
-                      "contact_slots": st["nCandidates"], "real_contacts": st["nReal"]}), flush=True)
+    n = len(wl["radius"])
+    B = algorithmic_bytes(n, st["nCandidates"], st["nReal"])
+    print(json.dumps({"pmc_probe": KERNEL_LAUNCHES, "fix_steps": FIX_STEPS, "copy_bytes_each_way": nbytes,
+                      "algorithmic_bytes": B, "contact_slots": st["nCandidates"], "real_contacts": st["nReal"]}),
+          flush=True)
     dev.close()
 
 
@@ -601,7 +685,6 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--kernel-steps", type=int, default=50)
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--graph", type=int, default=1,
                     help="1: after the warm-up the autotuner picks the force-pass width and the steps per "
@@ -622,9 +705,9 @@ def main():
     ap.add_argument("--sur-rounds", type=int, default=5, help="rounds of training on the network's own rollouts")
     ap.add_argument("--sur-steps", type=int, default=20000, help="DEM steps of simulated time")
     ap.add_argument("--pmc-probe", type=int, default=0,
-                    help="for rocprofv3 --pmc runs: warm up, launch the contact loop and leapfrog this many "
-                         "times, then a device-to-device hipMemcpy of 512 MiB whose FETCH_SIZE/WRITE_SIZE "
-                         "calibrate the counters; prints no benchmark line")
+                    help="1: for rocprofv3 --pmc runs, only the kernel-timing launches (at their fixed state) "
+                         "and a device-to-device hipMemcpy of 512 MiB whose FETCH_SIZE/WRITE_SIZE calibrate "
+                         "the counters; prints no benchmark line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
@@ -639,7 +722,8 @@ def main():
     step = 0
     cp = contact_params()
     if args.pmc_probe:
-        pmc_probe(dev, wl, args)
+        dev.close()
+        pmc_probe(wl)
         return
     log(f"dt={dt:.4e} s, warmup {args.warmup} steps")
     step, _ = dev.run(ENGINES, args.warmup, step, dt, contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
@@ -660,32 +744,56 @@ def main():
     dev.sync()
     el = time.perf_counter() - t0
     st1 = dev.stats()
-    value = args.n * args.steps / el
-    log(f"timed {args.steps} steps in {el:.3f} s: {value:.4e} particle-steps/s; refreshes {st1['nRefresh'] - st0['nRefresh']}")
-
+    raw_value = args.n * args.steps / el
+    r_window = st1["nRefresh"] - st0["nRefresh"]
+    # refresh interval over every step run so far (warm-up, tuner, timed: `step` of them), the first
+    # refresh (at step 0) aside
+    k_refresh = (step - 1) / max(st1["nRefresh"] - 1, 1)
+    # observables at the end of the timed steps (the FP32 form is compared there)
     obs64 = observables(dev, wl)
+    c_refresh, step = refresh_cost(dev, step, dt)
+    # the timed window with the refreshes it would hold at the run's rate: those it held replaced
+    # by K / k refreshes of the measured cost
+    el_adj = max(el + (args.steps / k_refresh - r_window) * c_refresh, 1e-12)
+    value = args.n * args.steps / el_adj
+    log(f"timed {args.steps} steps in {el:.4f} s ({raw_value:.4e} particle-steps/s, {r_window} refreshes); "
+        f"refresh every {k_refresh:.1f} steps at {c_refresh * 1e6:.0f} us -> {value:.4e} particle-steps/s")
+
     fp32 = None
     if args.fp32:
-        fp32 = fp32_form(wl, args, value, obs64, tuning)
+        fp32 = fp32_form(wl, args, raw_value, obs64, tuning)
         log(f"fp32 form: {fp32}")
 
-    # per-kernel device time on the engine's stream
-    tc, tl, step = time_kernels(dev, wl, step, dt, args.kernel_steps)
-    st2 = dev.stats()
+    # whole step: BASELINE.md's algorithmic bytes per particle-step at the run's state
+    B_ps, cn, pn = whole_step_bytes(args.n, st1["nCandidates"], st1["nReal"], k_refresh)
+    whole = dict(bytes_per_particle_step=B_ps, contacts_per_particle=cn, slots_per_particle=pn,
+                 refresh_interval_steps=k_refresh, refresh_us=c_refresh * 1e6, achieved_GBs=B_ps * value / 1e9,
+                 peak_GBs=HBM_PEAK / 1e9, frac=B_ps * value / HBM_PEAK,
+                 formula="272 + 80 + 48 + (C/N)*280 + (104 + 8*P/N)/k (BASELINE.md), P = contact slots")
+
+    # per-kernel device time on the engine's stream, at the fixed state of the committed profile
+    kdev, kstep = fixed_state(wl)
+    tc, tl, kstep = time_kernels(kdev, wl, kstep, KERNEL_LAUNCHES)
+    st2 = kdev.stats()
+    kdev.close()
     B = algorithmic_bytes(args.n, st2["nCandidates"], st2["nReal"])
     times = {"contact_loop": tc, "k_leapfrog": tl}
-    # the roofline is quoted for k_leapfrog: the largest single kernel of the step (the
-    # contact loop is two kernels, k_cfilter and k_cforce, timed together)
-    dom = "k_leapfrog"
+    # the roofline is quoted for the engine kernel that takes longest: the contact loop (k_cfilter
+    # + k_cforce, timed together) or the leapfrog with its fused gather
+    dom = max(times, key=times.get)
     achieved = B[dom] / times[dom]
-    log(f"kernel times: contact loop {tc * 1e6:.1f} us ({B['contact_loop'] / tc / 1e9:.0f} GB/s), "
+    log(f"kernel times at step {FIX_STEPS}: contact loop {tc * 1e6:.1f} us ({B['contact_loop'] / tc / 1e9:.0f} GB/s), "
         f"leapfrog+gather {tl * 1e6:.1f} us ({B['k_leapfrog'] / tl / 1e9:.0f} GB/s); slots {st2['nCandidates']}, "
         f"real {st2['nReal']}")
-    traffic, tsrc = measured_traffic(dom) if args.n == 1_000_000 else (None, None)
+    traffic, tsrc = measured_traffic(dom, st2["nCandidates"], st2["nReal"]) if args.n == 1_000_000 else (None, None)
     roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
                 traffic=traffic, traffic_source=tsrc, kernel=dom, bytes_per_launch=B[dom], launch_us=times[dom] * 1e6,
+                state=dict(step=FIX_STEPS, launches=KERNEL_LAUNCHES, contact_slots=st2["nCandidates"],
+                           real_contacts=st2["nReal"]),
                 other_kernels={k: dict(bytes_per_launch=B[k], launch_us=times[k] * 1e6,
-                                       achieved_GBs=B[k] / times[k] / 1e9) for k in times if k != dom})
+                                       achieved_GBs=B[k] / times[k] / 1e9,
+                                       traffic=measured_traffic(k, st2["nCandidates"], st2["nReal"])[0])
+                               for k in times if k != dom})
     # the cost of energy tracking (Scene.trackEnergy) on the same stepping, 200 steps each
     kw = dict(graph=bool(args.graph), contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
     energy = {}
@@ -719,16 +827,21 @@ def main():
         log(f"cpu baseline: {base}")
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "warmup": args.warmup, "ms_per_step": el_adj / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: seeded polydisperse periodic packing at solid fraction 0.55, Gaussian velocities",
         "config": {"workload": "1M polydisperse Hertz spheres, periodic cube, phi=0.55, FP64 (Collider, ContactLoop, "
                                "Leapfrog)", "n": args.n, "dt": dt, "cell": wl["L"],
-                   "contact_slots": st2["nCandidates"], "real_contacts": st2["nReal"],
-                   "refreshes_in_timed_region": st1["nRefresh"] - st0["nRefresh"],
+                   "contact_slots": st1["nCandidates"], "real_contacts": st1["nReal"],
+                   "refreshes_in_timed_region": r_window,
+                   "timed_window": {"seconds": el, "raw_value": raw_value, "refreshes": r_window,
+                                    "refresh_interval_steps": k_refresh, "refresh_us": c_refresh * 1e6,
+                                    "value_rule": "value = n*K / (window seconds + (K/k - refreshes in the window) "
+                                                  "* refresh seconds)"},
                    "stepping": "hipGraph replay, tuned" if args.graph else "plain engine loop",
                    "tuning": tuning},
         "roofline": roof,
+        "whole_step_roofline": whole,
         "cpu_baseline": base,
         "fp32_form": fp32,
         "energy_tracking": energy,

@@ -5,4 +5,4 @@ leg may import this package, and only as the checker (or the timed CPU
 baseline), never as the thing measured or shipped. See ``oracle.h`` for the
 reference functions it restates.
 """
-from .oracle import Oracle, OrcParams, ENG, FIELD, build_oracle, liboracle_path  # noqa: F401
+from .oracle import Oracle, OrcParams, ENG, FIELD, build_oracle, liboracle_path, native_library  # noqa: F401

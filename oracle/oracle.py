@@ -37,16 +37,37 @@ def build_oracle():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
-_lib = None
+def native_library():
+    """The oracle compiled with -march=native for the host it runs on (the CPU baseline's build;
+    the tests use the portable one): built by the Makefile's `native` target, a few seconds, under
+    a name tagged with this host's CPU model and flags, so that a build from another host is never
+    loaded here."""
+    import hashlib
+
+    ident = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith(("model name", "flags")):
+                ident += line
+            if line.strip() == "":
+                break
+    except OSError:
+        pass
+    tag = hashlib.sha1(ident.encode()).hexdigest()[:12]
+    subprocess.check_call(["make", "-s", "-C", _HERE, "native", f"NATIVE_TAG={tag}"])
+    return os.path.join(_HERE, "_build", f"liboracle_native_{tag}.so")
 
 
-def _load():
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(liboracle_path):
+_libs = {}
+
+
+def _load(path=None):
+    path = path or liboracle_path
+    if path in _libs:
+        return _libs[path]
+    if path == liboracle_path and not os.path.exists(liboracle_path):
         build_oracle()
-    L = C.CDLL(liboracle_path)
+    L = C.CDLL(path)
     P, D, I = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)
     L.orc_default_params.argtypes = [C.POINTER(OrcParams)]
     L.orc_create.argtypes = [C.POINTER(OrcParams)]; L.orc_create.restype = P
@@ -77,7 +98,7 @@ def _load():
     L.orc_crit_dt.argtypes = [P]; L.orc_crit_dt.restype = C.c_double
     L.orc_brute_aabb_pairs.argtypes = [P, I, C.c_int64]; L.orc_brute_aabb_pairs.restype = C.c_int64
     L.orc_last_error.restype = C.c_char_p
-    _lib = L
+    _libs[path] = L
     return L
 
 
@@ -92,8 +113,8 @@ def _ip(a):
 class Oracle:
     """One oracle scene. ``params`` are OrcParams field overrides."""
 
-    def __init__(self, **params):
-        L = _load()
+    def __init__(self, lib=None, **params):
+        L = _load(lib)
         self.p = OrcParams()
         L.orc_default_params(C.byref(self.p))
         self.p.en = math.nan

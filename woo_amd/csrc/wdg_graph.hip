@@ -24,17 +24,23 @@
 
 namespace wdg {
 
-// gstate words: [0] stop, [1] executed, [2] renorm, [3..4] first step (int64), [5] trsfRenorm,
-// [6] dirty word index of the previous step (set per step at capture)
-__global__ void k_gate(uint32_t* gs, const uint32_t* dflags, int j, int dirtyWord) {
+// gstate words: [0] stop, [1] steps executed in the batch, [2] renorm flag of the step, [3..4]
+// the batch's first step (int64), [5] trsfRenorm, [6] set until the batch's first step starts.
+// A batch is several replays launched back to back: once a gate has stopped, every later kernel
+// of the batch returns at once, so the replays after the stop cost only their launches.
+// dirtyWord: the word the previous step's leapfrog sets (its parity is fixed at capture; K is even)
+__global__ void k_gate(uint32_t* gs, const uint32_t* dflags, int dirtyWord) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (gs[0]) return;
-  if (j > 0 && dflags[dirtyWord]) {
+  // the batch's first step follows the host's collider call, which has rebuilt if needed
+  if (!gs[6] && dflags[dirtyWord]) {
     gs[0] = 1u;
     return;
   }
-  gs[1] = (uint32_t)(j + 1);
-  const int64_t step = (int64_t)(((uint64_t)gs[4] << 32) | gs[3]) + j;
+  gs[6] = 0u;
+  const uint32_t ex = gs[1];
+  gs[1] = ex + 1u;
+  const int64_t step = (int64_t)(((uint64_t)gs[4] << 32) | gs[3]) + ex;
   const int64_t tr = (int64_t)gs[5];
   gs[2] = (tr > 0 && (step % tr) == 0) ? 1u : 0u;
 }
@@ -98,7 +104,7 @@ static void capture(Ctx& c, const wdg_program* prog, int K, int64_t step, double
     for (int j = 0; j < K; j++) {
       // the dirty word the previous step's leapfrog sets
       const int prevWord = DIRTY_PP + (1 - c.lfParity);
-      hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, c.stream, c.gstate.p, (const uint32_t*)c.dflags.p, j, prevWord);
+      hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, c.stream, c.gstate.p, (const uint32_t*)c.dflags.p, prevWord);
       WDG_LAUNCH_CHECK();
       plain_step(c, prog, step + j, dt);
     }
@@ -137,21 +143,25 @@ static bool graph_valid(const Ctx& c, const wdg_program* prog, int K, double dt)
          a.noBreak == b.noBreak && a.noFrict == b.noFrict && a.noAttraction == b.noAttraction;
 }
 
-// one replay of up to K steps; returns the steps executed
-static int replay(Ctx& c, const wdg_program* prog, int K, int64_t step, double dt) {
+// one batch of R replays of the K-step graph (up to R*K steps); returns the steps executed. The
+// executed count and the dirty/error words come back in one round trip, and the collider call
+// that follows uses them without reading again.
+static int replay(Ctx& c, const wdg_program* prog, int K, int R, int64_t step, double dt) {
   // the captured filters keep the per-entry state indices of a clean or current array
   if (!c.eidxValid && !c.eidxClean && c.con().n > 0) {
     eidx_clear(c);
     c.eidxClean = true;
   }
   if (!graph_valid(c, prog, K, dt)) capture(c, prog, K, step, dt);
-  uint32_t w[6] = {0u, 0u, 0u, (uint32_t)((uint64_t)step & 0xFFFFFFFFu), (uint32_t)((uint64_t)step >> 32),
-                   (uint32_t)std::max(prog->contact.trsfRenorm, 0)};
+  uint32_t w[7] = {0u, 0u, 0u, (uint32_t)((uint64_t)step & 0xFFFFFFFFu), (uint32_t)((uint64_t)step >> 32),
+                   (uint32_t)std::max(prog->contact.trsfRenorm, 0), 1u};
   WDG_HIP(hipMemcpyAsync(c.gstate.p, w, sizeof(w), hipMemcpyHostToDevice, c.stream));
-  WDG_HIP(hipGraphLaunch(c.graph.exec, c.stream));
+  for (int r = 0; r < R; r++) WDG_HIP(hipGraphLaunch(c.graph.exec, c.stream));
   WDG_HIP(hipMemcpyAsync(c.hflags + 32, c.gstate.p + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+  WDG_HIP(hipMemcpyAsync(c.hflags, c.dflags.p, (DIRTY_PP + 2) * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
   WDG_HIP(hipStreamSynchronize(c.stream));
   const int e = (int)c.hflags[32];
+  c.hflagsFresh = true;
   // host state after e steps of contact loop + leapfrog
   if (e > 0) {
     if (c.con().n > 0) {
@@ -173,13 +183,22 @@ static int replay(Ctx& c, const wdg_program* prog, int K, int64_t step, double d
   return e;
 }
 
-void run_graph(Ctx& c, const wdg_program* prog, int64_t nSteps, int64_t* step, double* dt) {
+static void run_graph_steps(Ctx& c, const wdg_program* prog, int64_t nSteps, int64_t* step, double* dt) {
   check_program(prog);
   c.gstate.ensure(16);
   int64_t done = 0;
   const int K = c.tune.graphSteps;
   while (done < nSteps) {
+    const int64_t nr = c.nRefresh;
     collide(c, &prog->collide);
+    if (c.nRefresh != nr) {
+      // the run's refresh interval, a running mean over the last few refreshes
+      if (c.lastRefreshStep >= 0) {
+        const double iv = (double)(*step - c.lastRefreshStep);
+        c.refreshEvery = c.refreshEvery > 0 ? 0.75 * c.refreshEvery + 0.25 * iv : iv;
+      }
+      c.lastRefreshStep = *step;
+    }
     // a replay starts from a state its capture saw: forces at their reset value, the
     // contact loop's forces gathered; otherwise one plain step
     const int64_t left = nSteps - done;
@@ -191,11 +210,30 @@ void run_graph(Ctx& c, const wdg_program* prog, int64_t nSteps, int64_t* step, d
       done += 1;
       continue;
     }
-    const int e = replay(c, prog, K, *step, *dt);
+    // replays per batch: up to the next refresh the run's own refresh interval predicts (a
+    // batch that reaches a refresh earlier wastes only the launches of its remaining replays)
+    int R = 1;
+    if (c.refreshEvery > 0) {
+      const double toRefresh = c.refreshEvery - (double)(*step - c.lastRefreshStep);
+      R = (int)std::max(1., std::floor(toRefresh / K));
+    }
+    R = (int)std::min<int64_t>(std::min<int64_t>(R, left / K), 64);
+    const int e = replay(c, prog, K, R, *step, *dt);
     if (e == 0) throw Error(WDG_E_STATE, "graph replay executed no step");
     *step += e;
     done += e;
   }
+}
+
+void run_graph(Ctx& c, const wdg_program* prog, int64_t nSteps, int64_t* step, double* dt) {
+  // the host mirror of the flag words serves only the collider call right after a batch
+  try {
+    run_graph_steps(c, prog, nSteps, step, dt);
+  } catch (...) {
+    c.hflagsFresh = false;
+    throw;
+  }
+  c.hflagsFresh = false;
 }
 
 // ------------------------------------------------------------------ autotuner

@@ -345,6 +345,11 @@ struct Ctx {
   bool eidxValid = false, eidxClean = false;
   double verletDist = NAN;
   int64_t nRefresh = 0, nCollide = 0;
+  // graph runner: the step of the last refresh and the running mean of the refresh interval
+  int64_t lastRefreshStep = -1;
+  double refreshEvery = 0;
+  // hflags[0 .. DIRTY_PP + 1] mirror dflags after a graph batch (the next collider call uses them)
+  bool hflagsFresh = false;
   int64_t dynStepLast = -1, dynStepPrev = -1;
   // contact form: 64 (plain FP64) or 32 (contact geometry and law in FP32)
   int precision = 64;
