@@ -447,3 +447,110 @@ def test_facet_halfthick_shifts_contact():
     np.testing.assert_allclose(d[0, 0:3], (0, 0, -1), atol=1e-15)
     np.testing.assert_allclose(d[0, 6:9], (0.3, 0.2, -h), atol=1e-15)
     assert d[0, 12] == pytest.approx(0.0085 - r - h, rel=1e-10)
+
+
+def _rot(axis, ang):
+    """Rotation matrix about a unit axis (Rodrigues)."""
+    a = np.asarray(axis, float) / np.linalg.norm(axis)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + math.sin(ang) * K + (1 - math.cos(ang)) * K @ K
+
+
+def _corotating_pair(omega, steps, dt=1e-4, r=.01):
+    """Two touching spheres (normal along x) moved as one rigid body rotating at omega about
+    their midpoint, positions and velocities set each step (no integrator: the contact loop
+    sees exactly the rigid motion, with leapfrog's convention of velocities at mid-step).
+    Returns the contact's (normal, local y axis, local force) at creation and after `steps`."""
+    # no renormalisation (trsfRenorm 0): it would project the frame back onto the new normal
+    # and hide how the frame was carried
+    o = Oracle(physKind=0, lawKind=0, damping=0., gravity=(0, 0, 0), dt=dt, trsfRenorm=0)
+    mt = o.material(density=2000., young=1e7, ktDivKn=.2, tanPhi=.5)
+    half = r * (1 - 1e-7)
+    x0 = np.array([[-half, 0., 0.], [half, 0., 0.]])
+    o.sphere(x0[0], r, mt)
+    o.sphere(x0[1], r, mt)
+    w = np.asarray(omega, float)
+    wn = np.linalg.norm(w)
+
+    def place(t):
+        x = x0 @ _rot(w, wn * t).T
+        xm = x0 @ _rot(w, wn * (t - dt / 2)).T
+        o.set("pos", x)
+        o.set("vel", np.cross(w, xm))
+        o.set("angVel", np.tile(w, (2, 1)))
+
+    place(0.)
+    o.run(["Collider", "ContactLoop"], 1)
+    d0 = o.contacts(real_only=True)[2][0]
+    for k in range(1, steps + 1):
+        place(k * dt)
+        o.run(["ContactLoop"], 1)
+    ids, _, d = o.contacts(real_only=True)
+    assert len(ids) == 1
+    return d0, d[0], _rot(w, wn * steps * dt)
+
+
+def test_l6geom_corotation_perpendicular_to_normal():
+    # a touching pair turning rigidly about an axis perpendicular to its normal (x), and oblique
+    # to the frame's initial y axis (-y): the contact frame turns with it (normRotVec,
+    # L6Geom.cpp:74-117) and the material points at the contact do not slide, so the shear force
+    # stays at the chord error of the midpoint rule. A wrong sign of normRotVec turns the frame's
+    # y axis the other way about the rotation axis; a wrong branch vector gives velT ~ omega * 2r.
+    steps, dt, r, w = 400, 1e-4, .01, np.array([0., 2.5, 2.5]) / math.sqrt(2)
+    d0, d, R = _corotating_pair(w, steps, dt, r)
+    assert np.max(np.abs(d[0:3] - R @ d0[0:3])) < 1e-12
+    assert np.max(np.abs(d[3:6] - R @ d0[3:6])) < 1e-6
+    kt = d[14]
+    wrong = kt * np.linalg.norm(w) * 2 * r * dt * steps
+    assert math.hypot(d[10], d[11]) < 1e-6 * wrong
+
+
+def test_l6geom_twist_about_normal():
+    # both spheres spinning at the same rate about the normal: the frame's y axis turns about the
+    # normal by the spin angle (normTwistVec = n dt (n.(w1+w2))/2), no relative motion, no shear
+    steps, dt, w = 400, 1e-4, np.array([3.0, 0., 0.])
+    d0, d, R = _corotating_pair(w, steps, dt)
+    assert np.max(np.abs(d[0:3] - d0[0:3])) < 1e-15
+    assert np.max(np.abs(d[3:6] - R @ d0[3:6])) < 1e-6
+    # half the rate (a factor 1/2 too many) or the opposite sense would be off by >= 0.03 rad
+    assert np.max(np.abs(d[3:6] - d0[3:6])) > 0.05
+    assert math.hypot(d[10], d[11]) == 0.
+
+
+def test_trsf_orthonormal_after_renormalisation():
+    # trsfRenorm (L6Geom.cpp:124-136): at steps that are multiples of 100 the frame is
+    # re-orthonormalised; in between it drifts. Right after the renormalising step every real
+    # contact's columns 0 and 1 are unit and orthogonal to rounding.
+    sc = periodic_hertz(n=400, seed=5, spacing=.8, vel_sigma=.5)
+    o = build_oracle(sc)
+    o.run(ENG, 100)  # steps 0..99; the frame drifts
+    d99 = o.contacts(real_only=True)[2]
+    o.run(ENG, 1)  # step 100 renormalises
+    d = o.contacts(real_only=True)[2]
+    assert len(d) > 50
+
+    def dev(x):
+        n, y = x[:, 0:3], x[:, 3:6]
+        return max(np.max(np.abs(np.linalg.norm(n, axis=1) - 1)), np.max(np.abs(np.linalg.norm(y, axis=1) - 1)),
+                   np.max(np.abs(np.sum(n * y, axis=1))))
+
+    print("deviation before / after", dev(d99), dev(d))
+    assert dev(d) < 4e-16
+    assert dev(d99) > dev(d)
+
+
+def test_dyndt_rotational_stiffness_decides():
+    # one vertical wall contact with kt = 0.8 kn: the rotational term of DynDt.cpp:16-58,
+    # I / (l^2 kt) = (2/5) m / (0.8 kn) = 0.5 m / kn, is below m / kn, so the critical step is
+    # sqrt(2 * 0.5 m / kn). A rotational stiffness without l^2, with kn in place of kt, or left
+    # out, gives another value.
+    r, rho, E = .01, 2000., 1e7
+    o = Oracle(damping=0., gravity=(0, 0, 0))
+    mt = o.material(density=rho, young=E, ktDivKn=.8, tanPhi=.5)
+    o.wall((0, 0, 0), 2, 1, mt)
+    o.sphere((0, 0, r * (1 - 1e-4)), r, mt)
+    o.set_params(dt=.1 * o.pwave_dt())
+    o.run(["Collider", "ContactLoop"], 1)
+    mass = o.get("mass")[1]
+    kn = E * math.pi * r / 2
+    assert o.crit_dt() == pytest.approx(math.sqrt(2 * 0.5 * mass / kn), rel=1e-12)
