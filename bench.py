@@ -31,6 +31,7 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 METRIC = "particle-steps/s, 1M Hertz spheres on 1 MI355X; % HBM roofline; speedup vs FP64"
 RHO, YOUNG, TANPHI, POISSON, EN = 3000., 1e8, .4, .25, .6
 ENGINES = ["Collider", "ContactLoop", "Leapfrog"]
+DRUM_ENGINES = ["Collider", "ContactLoop", "IntraForce", "Leapfrog"]
 # steps wdg_tune takes: one plain pair, then for 3 force-pass widths x graph lengths 4, 8, 16 a
 # capturing replay and 3 timed ones (2 + 3 * 4 * (4 + 8 + 16)); the run after it is the same as
 # after this many plain steps (graph replay is bit-identical to the plain loop)
@@ -375,7 +376,15 @@ def drum_workload(n, dev, seed=4):
     mat = dict(kind=1, density=RHO, young=YOUNG, ktDivKn=.2, tanPhi=TANPHI, surfEnergy=0., alpha=0.)
     arrays = dict(shape=shape, pos=P, radius=rad, wallAxis=axis, wallSense=sense, mask=mask, flags=flags,
                   integrated=integ, vel=vel, mass=mass, inertia=inertia, facetVerts=fv, fakeVel=fk)
-    return dict(arrays=arrays, mat=mat, dt=dt, n=n, facets=nf, fill=info["fill"])
+    return dict(arrays=arrays, mat=mat, dt=dt, n=n, facets=nf, fill=info["fill"], verts=np.array(verts).reshape(-1, 3, 3),
+                facet_ids=np.arange(nw, nw + nf), omega=-omega)
+
+
+def drum_torque(dev, wl):
+    """moment about the drum axis (x through the origin) of the mantle's node forces and torques
+    from the last IntraForce (In2_Facet): what ReadForce summed over the facet nodes reads"""
+    F, T = dev.facet_nodes(wl["facet_ids"])
+    return float(np.sum(np.cross(wl["verts"], F)[..., 0]) + np.sum(T[..., 0]))
 
 
 def drum_form(args):
@@ -386,7 +395,8 @@ def drum_form(args):
     dev = Device(0)
     wl = drum_workload(args.drum_n, dev)
     out = dict(workload=f"rotating drum, {wl['facets']} facets (fakeVel 2 rad/s) + 2 end walls, {args.drum_n} "
-                        f"spheres r~lognormal(1.5 mm, 0.3) in [0.8, 4] mm, gravity, Hertz",
+                        f"spheres r~lognormal(1.5 mm, 0.3) in [0.8, 4] mm, gravity, Hertz; engines "
+                        f"{'+'.join(DRUM_ENGINES)}",
                n=args.drum_n, dt=wl["dt"], fill=wl["fill"], steps=args.drum_steps, warmup=args.warmup)
     cp = contact_params()
     for prec in (64, 32):
@@ -394,21 +404,26 @@ def drum_form(args):
         dev.set_materials([wl["mat"]])
         dev.upload_particles(**wl["arrays"])
         dev.set_precision(prec)
-        step, _ = dev.run(ENGINES, args.warmup, 0, wl["dt"], contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)
+        step, _ = dev.run(DRUM_ENGINES, args.warmup, 0, wl["dt"], contact=cp, leapfrog=dict(damping=0.),
+                          dynDtPeriod=0)
         dev.sync()
         st0 = dev.stats()
         t0 = time.perf_counter()
-        step, _ = dev.run(ENGINES, args.drum_steps, step, wl["dt"], contact=cp, leapfrog=dict(damping=0.),
+        step, _ = dev.run(DRUM_ENGINES, args.drum_steps, step, wl["dt"], contact=cp, leapfrog=dict(damping=0.),
                           dynDtPeriod=0)
         dev.sync()
         el = time.perf_counter() - t0
         st1 = dev.stats()
         big = 1e300
         ob = dev.observables((-big,) * 3, (big,) * 3)
+        M = drum_torque(dev, wl)
         out[f"fp{prec}"] = dict(value=args.drum_n * args.drum_steps / el, unit="particle-steps/s",
                                 ms_per_step=el / args.drum_steps * 1e3, real_contacts=st1["nReal"],
                                 contact_slots=st1["nCandidates"], refreshes=st1["nRefresh"] - st0["nRefresh"],
-                                kinetic_energy=dev.kinetic_energy(), coord_number=ob["coordNumber"])
+                                kinetic_energy=dev.kinetic_energy(), coord_number=ob["coordNumber"],
+                                # the charge's moment on the mantle about +x at the last step, and the
+                                # power the drive turning the mantle at omega (about +x) puts in
+                                drum_torque_Nm=M, drum_power_W=-M * wl["omega"])
         log(f"drum fp{prec}: {out[f'fp{prec}']}")
         dev.set_precision(64)
     dev.close()

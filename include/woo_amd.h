@@ -72,7 +72,10 @@ enum {
   WDG_EN_VISCN = 5, WDG_EN_VISCT = 6, WDG_EN_N = 7
 };
 /* engine codes for wdg_run */
-enum { WDG_E_FORCERESETTER = 1, WDG_E_COLLIDER = 2, WDG_E_CONTACTLOOP = 3, WDG_E_LEAPFROG = 4, WDG_E_DYNDT = 5 };
+enum {
+  WDG_E_FORCERESETTER = 1, WDG_E_COLLIDER = 2, WDG_E_CONTACTLOOP = 3, WDG_E_LEAPFROG = 4, WDG_E_DYNDT = 5,
+  WDG_E_INTRAFORCE = 6  /* IntraForce([In2_Facet]), after the ContactLoop */
+};
 
 /* DemField + Scene.cell (diagonal hSize only; gradV is zero) */
 typedef struct {
@@ -179,6 +182,21 @@ int wdg_collide(wdg_ctx* ctx, const wdg_collide_params* p);
 int wdg_contact_loop(wdg_ctx* ctx, const wdg_contact_params* p);
 int wdg_leapfrog(wdg_ctx* ctx, const wdg_leapfrog_params* p);
 int wdg_force_reset(wdg_ctx* ctx);
+/* IntraForce::run (src/dem/IntraForce.cpp:26-42) with In2_Facet (src/dem/Facet.cpp:301-328,
+ * In2_Facet::go / distributeForces with
+ * bary=true; src/dem/Contact.cpp:24-34 getForceTorqueBranch): each real contact of a facet
+ * adds w_i F and (c - v_i) x (w_i F) to the force and torque of the facet's node i, w being
+ * the barycentric coordinates (src/supp/base/CompUtils.cpp:94-112) of the contact point c
+ * projected onto the facet plane. The facet nodes are massless and take no gravity, so their
+ * reset value is zero: the sums kept are those of this call, made in Particle::contacts order
+ * (bit-identical to a sequential loop). Read them with wdg_facet_nodes. */
+int wdg_intra_force(wdg_ctx* ctx);
+/* The node forces and torques of facets, by user id, from the last IntraForce (zeros before
+ * the first): out n*18 doubles per facet, the force on nodes 0, 1, 2 (9), then the torque on
+ * nodes 0, 1, 2 (9) (DemData.force / DemData.torque of Facet.nodes, src/dem/Particle.hpp:
+ * 281-292, as ReadForce reads them before Leapfrog resets them, src/dem/Impose.cpp:143-158).
+ * An id that is not a facet is WDG_E_ARG. */
+int wdg_facet_nodes(wdg_ctx* ctx, const int64_t* ids, int64_t n, double* out);
 int wdg_crit_dt(wdg_ctx* ctx, double* critDt);
 
 /* which=0: potential+real (the ContactContainer view), which=1: real only.

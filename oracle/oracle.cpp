@@ -151,6 +151,8 @@ struct Particle {
   double fv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   double fakeVel[3] = {0, 0, 0};
   double halfThick = 0;
+  // force and torque on the facet's 3 nodes from the last IntraForce (In2_Facet)
+  double nodeF[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, nodeT[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   // Aabb
   bool hasBound = false;
   double bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
@@ -914,6 +916,55 @@ struct orc_scene {
     V3 xc = sub(c.nodePos, add(ld3(par[pid].pos), shift));
     T = cross(xc, F);
   }
+  // CompUtils::triangleBarycentrics (CompUtils.cpp:94-112): x - A = u (B - A) + v (C - A), dotted
+  // with B - A and C - A gives a 2x2 system, solved with the inverse of its matrix (adjugate over
+  // the determinant); returns (1 - u - v, u, v)
+  static V3 triangleBarycentrics(const V3& x, const V3& A, const V3& B, const V3& C) {
+    const V3 vB = sub(B, A), vC = sub(C, A), vX = sub(x, A);
+    const double m00 = dot(vB, vB), m01 = dot(vC, vB), m10 = dot(vB, vC), m11 = dot(vC, vC);
+    const double invdet = 1. / (m00 * m11 - m10 * m01);
+    const double r0 = dot(vX, vB), r1 = dot(vX, vC);
+    const double u = (m11 * invdet) * r0 + (-m01 * invdet) * r1;
+    const double v = (-m10 * invdet) * r0 + (m00 * invdet) * r1;
+    return mk(1. - u - v, u, v);
+  }
+  // IntraForce with In2_Facet (Facet.cpp:301-328, bary=true): every facet with contacts spreads
+  // each real contact's force over its 3 nodes, weighted by the barycentric coordinates of the
+  // contact point projected onto the facet plane; Contact::getForceTorqueBranch (Contact.cpp:
+  // 24-34) per node gives F = ori * force * sign and xc = contact point - node position (the
+  // facet is always pA, so no periodic shift); the phys torque is zero for these laws. Node
+  // i gets w_i F and xc_i x (w_i F). Contacts are visited in Particle::contacts order (the
+  // other id). The facet nodes are massless and outside gravity: the reset at the start of the
+  // step (ForceResetter, or Leapfrog.reset of the step before) sets them to zero, so the values
+  // kept are this step's sums, which ReadForce reads before Leapfrog resets them.
+  void intraForce() {
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long pid = 0; pid < (long)par.size(); pid++) {
+      Particle& f = par[pid];
+      if (f.shape != ORC_FACET) continue;
+      for (int k = 0; k < 9; k++) f.nodeF[k] = f.nodeT[k] = 0.;
+      if (f.contacts.empty()) continue;
+      const V3 v[3] = {ld3(f.fv), ld3(f.fv + 3), ld3(f.fv + 6)};
+      V3 normal = cross(sub(v[1], v[0]), sub(v[2], v[0]));  // Facet::getNormal (Facet.cpp:74-77)
+      normalize(normal);
+      for (const auto& oc : f.contacts) {
+        const Contact& C = con[oc.second];
+        if (!C.isReal()) continue;
+        const V3 c = C.nodePos;
+        const V3 p = sub(c, smul(dot(sub(c, v[0]), normal), normal));
+        const V3 w = triangleBarycentrics(p, v[0], v[1], v[2]);
+        const bool isPA = (int)pid == C.pA;
+        const V3 Fc = mul(quatRot(C.nodeOri, C.force), isPA ? 1. : -1.);
+        const V3 shift = (!isPA && prm.periodic) ? shiftPos(C.cellDist) : zero3();
+        for (int i = 0; i < 3; i++) {
+          const V3 xc = sub(c, add(v[i], shift));
+          const V3 F = mul(Fc, w[i]);
+          const V3 T = cross(xc, F);
+          for (int k = 0; k < 3; k++) { f.nodeF[3 * i + k] += F[k]; f.nodeT[3 * i + k] += T[k]; }
+        }
+      }
+    }
+  }
   // ContactLoop::run (ContactLoop.cpp:93-268)
   void contactLoop() {
     const bool det = prm.deterministic != 0;
@@ -1172,6 +1223,7 @@ struct orc_scene {
         case ORC_E_CONTACTLOOP: contactLoop(); break;
         case ORC_E_LEAPFROG: leapfrog(); break;
         case ORC_E_DYNDT: if (dynDtActivated()) dynDtRun(); break;
+        case ORC_E_INTRAFORCE: intraForce(); break;
         default: throw std::runtime_error("unknown engine code " + std::to_string(eng[e]));
       }
     }
@@ -1343,6 +1395,15 @@ int orc_run(orc_scene* s, const int* engines, int nEngines, int nSteps) {
     g_err = e.what();
     return -1;
   }
+}
+
+int orc_facet_nodes(const orc_scene* s, int64_t i, double out[18]) {
+  if (i < 0 || i >= (int64_t)s->par.size() || s->par[i].shape != ORC_FACET) {
+    g_err = "not a facet";
+    return -1;
+  }
+  for (int k = 0; k < 9; k++) { out[k] = s->par[i].nodeF[k]; out[9 + k] = s->par[i].nodeT[k]; }
+  return 0;
 }
 
 int64_t orc_step(const orc_scene* s) { return s->step; }

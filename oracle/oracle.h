@@ -14,7 +14,9 @@
  *   src/dem/Wall.cpp:25-84                Bo1_Wall_Aabb, Cg2_Wall_Sphere_L6Geom
  *   src/dem/Facet.cpp:74-118,161-177,275-298,333-390  Facet normal/outer vectors, velocity
  *                                         interpolation, Bo1_Facet_Aabb, Cg2_Facet_Sphere_L6Geom
+ *   src/dem/Facet.cpp:301-328             In2_Facet (IntraForce): contact forces spread to facet nodes
  *   src/supp/base/CompUtils.cpp:59-65     closestSegmentPt
+ *   src/supp/base/CompUtils.cpp:94-112    triangleBarycentrics
  *   src/dem/L6Geom.cpp:25-216             handleSpheresLikeContact
  *   src/dem/FrictMat.cpp:20-44            Cp2_FrictMat_FrictPhys
  *   src/dem/IdealElPl.cpp:15-125          Law2_L6Geom_FrictPhys_IdealElPl
@@ -41,7 +43,10 @@ enum { ORC_FRICTMAT = 0, ORC_HERTZMAT = 1 };
 enum { ORC_LAW_IDEALELPL = 0, ORC_LAW_HERTZDMT = 1 };
 enum { ORC_PHYS_FRICT = 0, ORC_PHYS_HERTZ = 1 };
 /* engine codes for orc_run */
-enum { ORC_E_FORCERESETTER = 1, ORC_E_COLLIDER = 2, ORC_E_CONTACTLOOP = 3, ORC_E_LEAPFROG = 4, ORC_E_DYNDT = 5 };
+enum {
+  ORC_E_FORCERESETTER = 1, ORC_E_COLLIDER = 2, ORC_E_CONTACTLOOP = 3, ORC_E_LEAPFROG = 4, ORC_E_DYNDT = 5,
+  ORC_E_INTRAFORCE = 6  /* IntraForce([In2_Facet]) */
+};
 /* node fields for orc_get / orc_set */
 enum { ORC_F_POS = 0, ORC_F_VEL = 1, ORC_F_ANGVEL = 2, ORC_F_ORI = 3, ORC_F_FORCE = 4, ORC_F_TORQUE = 5,
        ORC_F_MASS = 6, ORC_F_INERTIA = 7 };
@@ -112,6 +117,9 @@ int orc_set(orc_scene* s, int field, const double* in);
 
 /* run nSteps steps with the given engine list (Scene.engines order) */
 int orc_run(orc_scene* s, const int* engines, int nEngines, int nSteps);
+/* facet i's node forces and torques from the last IntraForce: F of nodes 0, 1, 2 (9), then
+ * T of nodes 0, 1, 2 (9) */
+int orc_facet_nodes(const orc_scene* s, int64_t i, double out[18]);
 int64_t orc_step(const orc_scene* s);
 double orc_dt(const orc_scene* s);
 double orc_verlet_dist(const orc_scene* s);

@@ -9,7 +9,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 liboracle_path = os.path.join(_HERE, "_build", "liboracle.so")
 
-ENG = dict(ForceResetter=1, Collider=2, ContactLoop=3, Leapfrog=4, DynDt=5)
+ENG = dict(ForceResetter=1, Collider=2, ContactLoop=3, Leapfrog=4, DynDt=5, IntraForce=6)
 FIELD = dict(pos=0, vel=1, angVel=2, ori=3, force=4, torque=5, mass=6, inertia=7)
 _FIELD_W = dict(pos=3, vel=3, angVel=3, ori=4, force=3, torque=3, mass=1, inertia=3)
 
@@ -96,6 +96,7 @@ def _load(path=None):
     L.orc_stress.argtypes = [P, C.c_double, D]; L.orc_stress.restype = C.c_int
     L.orc_pwave_dt.argtypes = [P]; L.orc_pwave_dt.restype = C.c_double
     L.orc_crit_dt.argtypes = [P]; L.orc_crit_dt.restype = C.c_double
+    L.orc_facet_nodes.argtypes = [P, C.c_int64, D]; L.orc_facet_nodes.restype = C.c_int
     L.orc_brute_aabb_pairs.argtypes = [P, I, C.c_int64]; L.orc_brute_aabb_pairs.restype = C.c_int64
     L.orc_last_error.restype = C.c_char_p
     _libs[path] = L
@@ -213,6 +214,13 @@ class Oracle:
         ids = np.empty((n, 2), dtype=np.int32)
         self.L.orc_brute_aabb_pairs(self.h, _ip(ids), n)
         return ids
+
+    def facet_nodes(self, i):
+        """Facet i's node forces (3x3, node by row) and torques (3x3) from the last IntraForce."""
+        out = np.empty(18, dtype=np.float64)
+        if self.L.orc_facet_nodes(self.h, int(i), _dp(out)) != 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
+        return out[:9].reshape(3, 3), out[9:].reshape(3, 3)
 
     def coord_number(self, skipFree=False):
         return self.L.orc_coord_number(self.h, int(skipFree))

@@ -342,6 +342,8 @@ class GpuRunner:
                     self.d.leapfrog(self.dt, self.sc["damping"], True)
                 elif e == "ForceResetter":
                     self.d.force_reset()
+                elif e == "IntraForce":
+                    self.d.intra_force()
                 else:
                     raise ValueError(e)
             self.step += 1

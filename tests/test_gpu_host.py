@@ -185,6 +185,36 @@ def test_scripted_drum_with_facets_matches_oracle():
     assert nf > 10
 
 
+def test_scripted_drum_intraforce_node_forces():
+    # IntraForce([In2_Facet()]) appended to the engines (as the reference's scripts do): after a
+    # run each facet node's DemData.force / torque hold what In2_Facet spread to it, as the oracle's
+    from woo_amd.woo.dem import In2_Facet, IntraForce
+
+    sc = drum(n=1500, seed=41)
+    S = host_scene(sc)
+    S.engines = S.engines + [IntraForce([In2_Facet()])]
+    o = build_oracle(sc)
+    S.run(40)
+    o.run(MINIMAL + ["IntraForce"], 40)
+    worst, touched = 0., 0
+    for p in S.dem.par:
+        if type(p.shape).__name__ != "Facet":
+            continue
+        Fo, To = o.facet_nodes(p.id)
+        Fg = np.array([n.dem.force for n in p.shape.nodes])
+        Tg = np.array([n.dem.torque for n in p.shape.nodes])
+        worst = max(worst, np.abs(Fg - Fo).max(), np.abs(Tg - To).max() / .04)
+        touched += bool(np.any(Fo != 0))
+    print("facets touched", touched, "largest node force difference", worst)
+    assert touched > 5
+    assert worst <= 1e-12 * max(1e-300, max(np.abs(o.facet_nodes(p.id)[0]).max() for p in S.dem.par
+                                            if type(p.shape).__name__ == "Facet"))
+    # In2_Facet needs its functor for facets
+    S.engines = S.engines[:-1] + [IntraForce([])]
+    with pytest.raises(RuntimeError):
+        S.run(1)
+
+
 def test_triangulated_cylinder_drum_runs():
     # the reference's way of building a drum: triangulated.cylinder(..., wallCaps, angVel)
     import woo_amd.woo as woo

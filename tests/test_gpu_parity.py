@@ -364,6 +364,55 @@ def test_trajectory_drum_facets(halfThick):
     assert np.all(F[2:nb] == 0.)
 
 
+def _drum_torque(sc, first, F, T):
+    """moment about the drum axis (x through the origin) of the facet node forces and torques"""
+    M = 0.
+    for k, f in enumerate(sc["facets"]):
+        for i in range(3):
+            M += np.cross(f["verts"][i], F[k, i])[0] + T[k, i, 0]
+    return M
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_drum_facet_node_forces_match_oracle(graph):
+    # IntraForce (In2_Facet) in the rotating drum: the force and torque of every facet node, and
+    # the drum torque they add up to, as the oracle's, stepped engine by engine and replayed from
+    # captured graphs
+    sc = drum(n=2000, seed=11)
+    eng = ["Collider", "ContactLoop", "IntraForce", "Leapfrog"]
+    o = build_oracle(sc)
+    first, nb = len(sc["walls"]), n_boundary(sc)
+    ids = np.arange(first, nb)
+    if graph:
+        from tests.scenes import build_gpu, contact_kw
+        d = build_gpu(sc)
+        step = 0
+    else:
+        g = GpuRunner(sc)
+        d = g.d
+    worst = 0.
+    for k in range(4):
+        o.run(eng, 50)
+        if graph:
+            step, _ = d.run(eng, 50, step, sc["dt"], graph=True, contact=contact_kw(sc),
+                            leapfrog=dict(damping=sc["damping"]))
+        else:
+            g.run(eng, 50)
+        Fg, Tg = d.facet_nodes(ids)
+        Fo = np.array([o.facet_nodes(i)[0] for i in ids])
+        To = np.array([o.facet_nodes(i)[1] for i in ids])
+        scale = np.abs(Fo).max()
+        err = max(np.abs(Fg - Fo).max(), np.abs(Tg - To).max() / 0.04) / scale  # torques over the drum radius
+        worst = max(worst, err)
+        Mo, Mg = _drum_torque(sc, first, Fo, To), _drum_torque(sc, first, Fg, Tg)
+        print(f"step {50 * (k + 1)}: node force error {err:.3e} (of the largest), drum torque {Mg:.9e} vs {Mo:.9e}, "
+              f"facets touched {int(np.sum(np.abs(Fo).sum((1, 2)) > 0))}", flush=True)
+        assert Mg == pytest.approx(Mo, rel=1e-12, abs=1e-12 * scale * .04)
+    assert worst <= FORCE_TOL
+    with pytest.raises(RuntimeError):
+        d.facet_nodes([nb])  # the first sphere
+
+
 def test_facet_box_deposition_matches_oracle():
     sc = facet_box(n=800, dense=True, seed=9, vel_sigma=0.05)
     o, g, worst = _run_both(sc, ENG, 100)

@@ -88,6 +88,12 @@ def test_engines_functors_labels(woo):
     assert pr.stepPeriod == 7 and pr.command == "pass"
     with pytest.raises(ValueError):
         woo.utils.defaultEngines(gravity=(0, 0, -9.81))
+    from woo_amd.woo.dem import In2_Facet, IntraForce
+    it = IntraForce([In2_Facet(label='in2')], label='intra')
+    S.engines = S.engines + [it]
+    assert [type(f).__name__ for f in S.lab.intra.funcs] == ["In2_Facet"] and S.lab.in2 is it.funcs[0]
+    with pytest.raises(TypeError):
+        IntraForce([In2_Facet()], [In2_Facet()])
 
 
 def test_install_alias(woo):

@@ -449,6 +449,113 @@ def test_facet_halfthick_shifts_contact():
     assert d[0, 12] == pytest.approx(0.0085 - r - h, rel=1e-10)
 
 
+def _global_force(d):
+    """global force on pA of each contact row of contacts(): local force in the frame
+    (normal, y axis, normal x y)"""
+    n, y = d[:, 0:3], d[:, 3:6]
+    z = np.cross(n, y)
+    return d[:, 9:10] * n + d[:, 10:11] * y + d[:, 11:12] * z
+
+
+@pytest.mark.parametrize("centre", [
+    (0.2, 0.3, 0.0079),     # face: weights (0.5, 0.2, 0.3)
+    (0.5, -0.002, 0.0075),  # edge v0-v1: node 2 gets nothing
+    (1.002, -0.001, 0.0075),  # vertex v1: node 1 gets everything, with no moment
+])
+def test_in2_facet_single_contact_weights(centre):
+    # In2_Facet on the unit right triangle (0,0,0), (1,0,0), (0,1,0): the barycentric coordinates
+    # of a point (x, y, 0) are (1 - x - y, x, y); node i gets w_i F and (c - v_i) x (w_i F),
+    # F being the force on the facet, which is minus the force on the only sphere
+    r = .008
+    o = Oracle(damping=0., gravity=(0, 0, 0))
+    m = o.material(density=2500., young=1e7, ktDivKn=.2, tanPhi=.4)
+    o.facet([(0, 0, 0), (1, 0, 0), (0, 1, 0)], None, 0., m)
+    o.sphere(centre, r, m)
+    o.set("vel", np.array([[0, 0, 0], [0.05, -0.03, 0.]]))
+    o.set_params(dt=.2 * o.pwave_dt())
+    o.run(["Collider", "ContactLoop", "IntraForce", "Leapfrog"], 4)
+    o.run(["ForceResetter", "Collider", "ContactLoop", "IntraForce"], 1)
+    ids, _, d = o.contacts(real_only=True)
+    assert ids.tolist() == [[0, 1]]
+    c = d[0, 6:9]
+    assert c[2] == 0.
+    F = -o.get("force")[1]
+    assert abs(F[0]) > 0 and abs(F[1]) > 0 and F[2] < 0  # friction acts and the sphere presses down
+    np.testing.assert_allclose(F, _global_force(d)[0], rtol=0, atol=1e-14 * np.abs(F).max())
+    w = np.array([1 - c[0] - c[1], c[0], c[1]])
+    nf, nt = o.facet_nodes(0)
+    V = np.array([(0, 0, 0), (1, 0, 0), (0, 1, 0)], float)
+    for i in range(3):
+        np.testing.assert_allclose(nf[i], w[i] * F, rtol=0, atol=1e-15 * np.abs(F).max())
+        np.testing.assert_allclose(nt[i], np.cross(c - V[i], w[i] * F), rtol=0, atol=1e-15 * np.abs(F).max())
+    if centre[0] > 1:
+        assert np.all(nf[[0, 2]] == 0) and np.all(nt == 0)
+    if centre[1] < 0:
+        assert np.all(nf[2] == 0) and np.all(nt[2] == 0)
+    with pytest.raises(RuntimeError):
+        o.facet_nodes(1)  # a sphere
+
+
+def test_in2_facet_weights_reproduce_contact_point_on_tilted_triangle():
+    # on a scalene triangle out of the coordinate planes, the weights read back from the node
+    # forces (w_i = F_i . F / F . F) are the barycentric coordinates of the contact point: they
+    # sum to one and interpolate the vertices to it
+    V = np.array([(0.1, -0.2, 0.05), (0.9, 0.1, 0.3), (0.2, 0.7, -0.1)])
+    n = np.cross(V[1] - V[0], V[2] - V[0])
+    n /= np.linalg.norm(n)
+    r = .008
+    o = Oracle(damping=0., gravity=(0, 0, 0))
+    m = o.material(density=2500., young=1e7, ktDivKn=.2, tanPhi=.4)
+    o.facet(V, None, 0., m)
+    target = .2 * V[0] + .5 * V[1] + .3 * V[2]
+    o.sphere(target + n * r * (1 - 1e-3), r, m)
+    o.set("vel", np.array([[0, 0, 0], [0.05, -0.03, 0.02]]))
+    o.set_params(dt=.2 * o.pwave_dt())
+    o.run(["Collider", "ContactLoop", "IntraForce", "Leapfrog"], 4)
+    ids, _, d = o.contacts(real_only=True)
+    assert ids.tolist() == [[0, 1]]
+    c = d[0, 6:9]
+    nf, nt = o.facet_nodes(0)
+    F = nf.sum(0)
+    w = nf @ F / (F @ F)
+    assert w.sum() == pytest.approx(1., abs=1e-14)
+    np.testing.assert_allclose(w @ V, c, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(w, (.2, .5, .3), atol=1e-3)  # the sphere has barely moved
+    for i in range(3):
+        np.testing.assert_allclose(nt[i], np.cross(c - V[i], nf[i]), rtol=0, atol=1e-15 * np.abs(F).max())
+
+
+def test_in2_facet_drum_force_and_moment_balance():
+    # In the rotating drum every facet's node forces add up to the force of its contacts, and the
+    # node forces and torques have the same moment about any point as the contact forces acting
+    # at their contact points (weights summing to one)
+    from tests.scenes import drum, n_boundary
+    sc = drum(n=500, seed=3, omega=4.)
+    o = build_oracle(sc)
+    eng = ["Collider", "ContactLoop", "IntraForce", "Leapfrog"]
+    o.run(eng, 300)
+    nW = n_boundary(sc)
+    facets = [(k + len(sc["walls"]), f) for k, f in enumerate(sc["facets"])]
+    ids, _, d = o.contacts(real_only=True)
+    Fg = _global_force(d)
+    pt = np.array([0.01, -0.02, 0.005])
+    touched = 0
+    for pid, f in facets:
+        nf, nt = o.facet_nodes(pid)
+        rows = np.nonzero(ids[:, 0] == pid)[0]
+        assert np.all(ids[rows, 1] >= nW)
+        Fc, cc = Fg[rows], d[rows, 6:9]
+        scale = max(np.abs(Fc).sum(), 1e-300)
+        np.testing.assert_allclose(nf.sum(0), Fc.sum(0), rtol=0, atol=1e-13 * scale)
+        M_nodes = sum(np.cross(f["verts"][i] - pt, nf[i]) + nt[i] for i in range(3))
+        M_cont = np.cross(cc - pt, Fc).sum(0) if len(rows) else np.zeros(3)
+        np.testing.assert_allclose(M_nodes, M_cont, rtol=0, atol=1e-13 * scale * .1)
+        if len(rows) == 0:
+            assert np.all(nf == 0) and np.all(nt == 0)
+        touched += len(rows) > 0
+    assert touched > 10
+
+
 def _rot(axis, ang):
     """Rotation matrix about a unit axis (Rodrigues)."""
     a = np.asarray(axis, float) / np.linalg.norm(axis)

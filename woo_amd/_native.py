@@ -16,7 +16,7 @@ PHYS_FRICT, PHYS_HERTZ = 0, 1
 LAW_IDEALELPL, LAW_HERTZDMT = 0, 1
 FIELD = dict(pos=0, vel=1, angVel=2, ori=3, force=4, torque=5, mass=6, inertia=7)
 FIELD_W = dict(pos=3, vel=3, angVel=3, ori=4, force=3, torque=3, mass=1, inertia=3)
-ENG = dict(ForceResetter=1, Collider=2, ContactLoop=3, Leapfrog=4, DynDt=5)
+ENG = dict(ForceResetter=1, Collider=2, ContactLoop=3, Leapfrog=4, DynDt=5, IntraForce=6)
 
 # every entry point include/woo_amd.h declares
 EXPORTS = [
@@ -30,7 +30,7 @@ EXPORTS = [
     "wdg_mpnn_graph_from_ctx", "wdg_mpnn_advance", "wdg_snapshot", "wdg_restore", "wdg_snapshot_distance",
     "wdg_mpnn_set_targets", "wdg_mpnn_keep_graph", "wdg_mpnn_use_graph", "wdg_mpnn_get_graph",
     "wdg_set_energy_tracking", "wdg_energy", "wdg_energy_clear", "wdg_restore_contacts", "wdg_energy_set",
-    "wdg_kinetic_energy_noalloc",
+    "wdg_kinetic_energy_noalloc", "wdg_intra_force", "wdg_facet_nodes",
 ]
 
 # Scene.energy terms, in WDG_EN_* order
@@ -119,6 +119,8 @@ def lib():
     L.wdg_contact_loop.argtypes = [V, _P(ContactParams)]
     L.wdg_leapfrog.argtypes = [V, _P(LeapfrogParams)]
     L.wdg_force_reset.argtypes = [V]
+    L.wdg_intra_force.argtypes = [V]
+    L.wdg_facet_nodes.argtypes = [V, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_double)]
     L.wdg_crit_dt.argtypes = [V, _P(C.c_double)]
     L.wdg_contact_count.argtypes = [V, C.c_int32, _P(C.c_int64)]
     L.wdg_contact_list.argtypes = [V, C.c_int32, _P(C.c_int32), _P(C.c_int32), _P(C.c_double)]
@@ -274,6 +276,18 @@ class Device:
 
     def force_reset(self):
         check(self.L.wdg_force_reset(self.h))
+
+    def intra_force(self):
+        """IntraForce([In2_Facet]): the facets' contact forces spread to their nodes"""
+        check(self.L.wdg_intra_force(self.h))
+
+    def facet_nodes(self, ids):
+        """node forces [n, 3, 3] and torques [n, 3, 3] (facet, node, component) of the facets ids
+        from the last IntraForce"""
+        a = np.ascontiguousarray(np.atleast_1d(ids), dtype=np.int64)
+        out = np.empty((len(a), 18), dtype=np.float64)
+        check(self.L.wdg_facet_nodes(self.h, a.ctypes.data_as(C.POINTER(C.c_int64)), len(a), dptr(out)))
+        return out[:, :9].reshape(-1, 3, 3), out[:, 9:].reshape(-1, 3, 3)
 
     def crit_dt(self):
         d = C.c_double()

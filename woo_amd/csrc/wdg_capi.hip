@@ -680,6 +680,9 @@ int wdg_upload_particles(wdg_ctx* ctx, const wdg_particles* p) {
     c.fgeo.ensure(fgeo.size());
     WDG_HIP(hipMemcpy(c.fgeo.p, fgeo.data(), fgeo.size() * sizeof(double), hipMemcpyHostToDevice));
     c.bnd.ensure((size_t)6 * (nW + 1));
+    c.fnode.ensure((size_t)18 * (nW + 1));
+    WDG_HIP(hipMemset(c.fnode.p, 0, (size_t)18 * (nW + 1) * sizeof(double)));
+    c.bodyUid.assign(order.begin(), order.begin() + nW);
     DBuf<double>* d[NPD] = {&c.pos[0], &c.pos[1], &c.pos[2], &c.vel[0], &c.vel[1], &c.vel[2], &c.angVel[0],
                             &c.angVel[1], &c.angVel[2], &c.ori[0], &c.ori[1], &c.ori[2], &c.ori[3], &c.force[0],
                             &c.force[1], &c.force[2], &c.torque[0], &c.torque[1], &c.torque[2], &c.inertia[0],
@@ -753,6 +756,32 @@ int wdg_contact_loop(wdg_ctx* ctx, const wdg_contact_params* p) {
 int wdg_leapfrog(wdg_ctx* ctx, const wdg_leapfrog_params* p) {
   if (!ctx || !p) { g_err = "NULL argument"; return WDG_E_ARG; }
   return guard(ctx, [&] { leapfrog(ctx->c, p); });
+}
+
+int wdg_intra_force(wdg_ctx* ctx) {
+  if (!ctx) { g_err = "NULL argument"; return WDG_E_ARG; }
+  return guard(ctx, [&] { intra_force(ctx->c); });
+}
+
+int wdg_facet_nodes(wdg_ctx* ctx, const int64_t* ids, int64_t n, double* out) {
+  if (!ctx || n < 0 || (n > 0 && (!ids || !out))) { g_err = "bad arguments"; return WDG_E_ARG; }
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    std::vector<int64_t> body((size_t)n);
+    std::unordered_map<uint32_t, int64_t> byUid;
+    for (int64_t b = 0; b < c.nW; b++) byUid[c.bodyUid[b]] = b;
+    std::vector<uint32_t> fl((size_t)c.nW);
+    if (c.nW) WDG_HIP(hipMemcpyAsync(fl.data(), c.flags.p, c.nW * 4, hipMemcpyDeviceToHost, c.stream));
+    std::vector<double> all((size_t)18 * c.nW);
+    if (c.nW) WDG_HIP(hipMemcpyAsync(all.data(), c.fnode.p, all.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    WDG_HIP(hipStreamSynchronize(c.stream));
+    for (int64_t j = 0; j < n; j++) {
+      auto it = ids[j] >= 0 && ids[j] < c.n ? byUid.find((uint32_t)ids[j]) : byUid.end();
+      if (it == byUid.end() || flag_shape(fl[it->second]) != WDG_FACET)
+        throw Error(WDG_E_ARG, "particle #" + std::to_string(ids[j]) + " is not a facet");
+      std::copy(&all[18 * it->second], &all[18 * it->second] + 18, out + 18 * j);
+    }
+  });
 }
 
 int wdg_force_reset(wdg_ctx* ctx) {
@@ -959,6 +988,7 @@ int wdg_run(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t* step
       for (int e = 0; e < prog->nEngines; e++) {
         switch (prog->engines[e]) {
           case WDG_E_FORCERESETTER: force_reset(c); break;
+          case WDG_E_INTRAFORCE: intra_force(c); break;
           case WDG_E_COLLIDER: collide(c, &prog->collide); break;
           case WDG_E_CONTACTLOOP: {
             wdg_contact_params p = prog->contact;

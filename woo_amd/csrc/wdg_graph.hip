@@ -45,10 +45,14 @@ __global__ void k_gate(uint32_t* gs, const uint32_t* dflags, int dirtyWord) {
   gs[2] = (tr > 0 && (step % tr) == 0) ? 1u : 0u;
 }
 
+// [Collider, ContactLoop, (IntraForce,) Leapfrog]
+static bool has_intra(const wdg_program* prog) { return prog->nEngines == 4; }
 static void check_program(const wdg_program* prog) {
-  if (prog->nEngines != 3 || prog->engines[0] != WDG_E_COLLIDER || prog->engines[1] != WDG_E_CONTACTLOOP ||
-      prog->engines[2] != WDG_E_LEAPFROG)
-    throw Error(WDG_E_ARG, "graph replay runs the engine list [Collider, ContactLoop, Leapfrog]");
+  const int n = prog->nEngines;
+  if ((n != 3 && n != 4) || prog->engines[0] != WDG_E_COLLIDER || prog->engines[1] != WDG_E_CONTACTLOOP ||
+      (n == 4 && prog->engines[2] != WDG_E_INTRAFORCE) || prog->engines[n - 1] != WDG_E_LEAPFROG)
+    throw Error(WDG_E_ARG, "graph replay runs the engine list [Collider, ContactLoop, Leapfrog] or [Collider, "
+                           "ContactLoop, IntraForce, Leapfrog]");
   if (!prog->leapfrog.reset) throw Error(WDG_E_ARG, "graph replay needs Leapfrog.reset");
 }
 
@@ -57,6 +61,7 @@ static void plain_step(Ctx& c, const wdg_program* prog, int64_t step, double dt)
   cp.step = step;
   cp.dt = dt;
   contact_loop(c, &cp);
+  if (has_intra(prog)) intra_force(c);
   wdg_leapfrog_params lp = prog->leapfrog;
   lp.dt = dt;
   leapfrog(c, &lp);
@@ -129,6 +134,7 @@ static void capture(Ctx& c, const wdg_program* prog, int K, int64_t step, double
   G.dt = dt;
   G.damping = prog->leapfrog.damping;
   G.cp = prog->contact;
+  G.intra = has_intra(prog);
 }
 
 static bool graph_valid(const Ctx& c, const wdg_program* prog, int K, double dt) {
@@ -136,6 +142,7 @@ static bool graph_valid(const Ctx& c, const wdg_program* prog, int K, double dt)
   const wdg_contact_params& a = G.cp;
   const wdg_contact_params& b = prog->contact;
   return G.exec && G.gen == c.gen && G.K == K && G.cfBlocks == c.tune.cfBlocks && G.track == c.trackEnergy &&
+         G.intra == has_intra(prog) &&
          G.rcur == c.rcur &&
          G.parity == c.lfParity && G.dt == dt && G.damping == prog->leapfrog.damping && a.physKind == b.physKind &&
          a.lawKind == b.lawKind && a.applyForces == b.applyForces && a.trsfRenorm == b.trsfRenorm &&

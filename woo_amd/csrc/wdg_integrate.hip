@@ -340,6 +340,106 @@ __global__ void k_force_reset(int64_t n, const uint32_t* __restrict__ flags, con
   tx[i] = 0.; ty[i] = 0.; tz[i] = 0.;
 }
 
+// IntraForce with In2_Facet (Facet.cpp:301-328, bary=true): one workgroup per facet spreads
+// each real contact's force over the facet's 3 nodes, weighted by the barycentric coordinates
+// (CompUtils::triangleBarycentrics, CompUtils.cpp:94-112: the 2x2 system of x - A = u (B - A)
+// + v (C - A) dotted with B - A and C - A, solved by the adjugate over the determinant) of the
+// contact point projected onto the facet plane (normal of Facet::getNormal, Facet.cpp:74-77).
+// Contact::getForceTorqueBranch (Contact.cpp:24-34) per node: F = the force on the facet,
+// xc = contact point - node position (minus the periodic shift when the facet is pB); node i
+// gets w_i F and xc x (w_i F) (the laws here have no contact torque). The workgroup takes its
+// facet's entries FN_BS at a time, each thread one entry, and 18 threads then add one
+// component each over them in list order: the sums run in the order of Particle::contacts
+// (increasing other id), as in the sequential loop, so they are bit-identical to it. A facet
+// without real contacts gets zeros (the reset value of its massless nodes).
+constexpr int FN_BS = 256;
+template <bool PERIODIC, bool F32>
+__global__ __launch_bounds__(FN_BS) void k_facet_nodes(GatherArgs g, const double* __restrict__ fgeo,
+                                                        const double* __restrict__ px, const double* __restrict__ py,
+                                                        const double* __restrict__ pz, double* __restrict__ out) {
+  __shared__ double sh[18][FN_BS];
+  if (g.gstop && *g.gstop) return;
+  const int64_t i = blockIdx.x;
+  if (flag_shape(g.flags[i]) != WDG_FACET) return;
+  const int t = threadIdx.x;
+  const uint32_t k0 = g.adjOff ? g.adjOff[i] : 0u, k1 = g.adjOff ? g.adjOff[i + 1] : 0u;
+  const double* fg = fgeo + (int64_t)FGEO * i;
+  const d3 v[3] = {mk3(fg[0], fg[1], fg[2]), mk3(fg[3], fg[4], fg[5]), mk3(fg[6], fg[7], fg[8])};
+  const d3 vB = sub3(v[1], v[0]), vC = sub3(v[2], v[0]);
+  const d3 normal = normalized3(cross3(vB, vC));
+  const double m00 = dot3(vB, vB), m01 = dot3(vC, vB), m10 = dot3(vB, vC), m11 = dot3(vC, vC);
+  const double invdet = 1. / (m00 * m11 - m10 * m01);
+  const double a00 = m11 * invdet, a01 = -m01 * invdet, a10 = -m10 * invdet, a11 = m00 * invdet;
+  double acc = 0.;
+  for (uint32_t base = k0; base < k1; base += FN_BS) {
+    double cv[18];
+#pragma unroll
+    for (int q = 0; q < 18; q++) cv[q] = 0.;
+    const uint32_t k = base + (uint32_t)t;
+    bool real = false, isPA = true;
+    uint32_t q = 0;
+    if (k < k1) {
+      if (g.eidx && g.sidx) {
+        const uint32_t e = g.eidx[k];
+        if (e != NO_STATE) {
+          real = true;
+          isPA = (e & EIDX_PA) != 0u;
+          q = g.sidx[e & ~EIDX_PA];
+        }
+      } else {
+        const uint32_t e = g.adj[k], slot = e & ~ADJ_PA;
+        if (rv_real(g.rv, slot)) {
+          real = true;
+          isPA = (e & ADJ_PA) != 0u;
+          q = rv_index(g.rv, slot);
+        }
+      }
+    }
+    if (real) {
+      d3 Fc, cp, shift = mk3(0., 0., 0.);
+      if (F32) {
+        // the FP32 record: force on pA, and the branches from pA and from pB's image
+        const float4* r = reinterpret_cast<const float4*>(reinterpret_cast<const float*>(g.gp) + (int64_t)GP32 * q);
+        const float4 r0 = r[0], r1 = r[1], r2 = r[2];
+        Fc = mul3(mk3(r0.x, r0.y, r0.z), isPA ? 1. : -1.);
+        const d3 br = isPA ? mk3(r0.w, r1.x, r1.y) : mk3(r1.z, r1.w, r2.x);
+        cp = add3(mk3(px[i], py[i], pz[i]), br);
+      } else {
+        const double2* r = reinterpret_cast<const double2*>(g.gp + (int64_t)GP * q);
+        const double2 r0 = r[0], r1 = r[1], r2 = r[2];
+        Fc = mul3(mk3(r0.x, r0.y, r1.x), isPA ? 1. : -1.);
+        cp = mk3(r1.y, r2.x, r2.y);
+        if (PERIODIC && !isPA) {
+          const uint64_t cd = (uint64_t)__double_as_longlong(r[3].x);
+          shift = mk3(g.L[0] * unpack_cd(cd, 0), g.L[1] * unpack_cd(cd, 1), g.L[2] * unpack_cd(cd, 2));
+        }
+      }
+      const d3 p = sub3(cp, mul3(normal, dot3(sub3(cp, v[0]), normal)));
+      const d3 vX = sub3(p, v[0]);
+      const double r0 = dot3(vX, vB), r1 = dot3(vX, vC);
+      const double u = a00 * r0 + a01 * r1, w2 = a10 * r0 + a11 * r1;
+      const double w[3] = {1. - u - w2, u, w2};
+#pragma unroll
+      for (int n = 0; n < 3; n++) {
+        const d3 xc = sub3(cp, add3(v[n], shift));
+        const d3 F = mul3(Fc, w[n]);
+        const d3 T = cross3(xc, F);
+        cv[3 * n] = F.x; cv[3 * n + 1] = F.y; cv[3 * n + 2] = F.z;
+        cv[9 + 3 * n] = T.x; cv[9 + 3 * n + 1] = T.y; cv[9 + 3 * n + 2] = T.z;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 18; c++) sh[c][t] = cv[c];
+    __syncthreads();
+    if (t < 18) {
+      const uint32_t m = k1 - base < (uint32_t)FN_BS ? k1 - base : (uint32_t)FN_BS;
+      for (uint32_t j = 0; j < m; j++) acc += sh[t][j];
+    }
+    __syncthreads();
+  }
+  if (t < 18) out[18 * i + t] = acc;
+}
+
 static GatherArgs gather_args(Ctx& c) {
   Contacts& C = c.con();
   GatherArgs g{};
@@ -359,6 +459,17 @@ static void gather_boundary(Ctx& c, const GatherArgs& g) {
                                  : (c.field.periodic ? k_gather_bnd<true, false> : k_gather_bnd<false, false>);
   hipLaunchKernelGGL(kern, dim3((unsigned)c.nW), dim3(256), 0, c.stream, g, c.force[0].p, c.force[1].p, c.force[2].p,
                      c.torque[0].p, c.torque[1].p, c.torque[2].p, c.pos[0].p, c.pos[1].p, c.pos[2].p);
+  WDG_LAUNCH_CHECK();
+}
+
+void intra_force(Ctx& c) {
+  if (c.nFacet == 0) return;
+  GatherArgs g = gather_args(c);
+  if (c.con().n == 0) g.adjOff = nullptr;  // no slot list yet: every facet gets zeros
+  auto* kern = c.precision == 32 ? k_facet_nodes<false, true>
+                                 : (c.field.periodic ? k_facet_nodes<true, false> : k_facet_nodes<false, false>);
+  hipLaunchKernelGGL(kern, dim3((unsigned)c.nW), dim3(FN_BS), 0, c.stream, g, (const double*)c.fgeo.p,
+                     (const double*)c.pos[0].p, (const double*)c.pos[1].p, (const double*)c.pos[2].p, c.fnode.p);
   WDG_LAUNCH_CHECK();
 }
 

@@ -234,6 +234,10 @@ struct Ctx {
   int64_t nW = 0;       // boundary bodies (walls, facets) occupy storage [0, nW)
   int64_t nFacet = 0;
   DBuf<double> fgeo;    // FGEO doubles per boundary body (facet geometry)
+  // per boundary body: force on its 3 nodes (9), then torque (9), from the last IntraForce
+  // (facets only)
+  DBuf<double> fnode;
+  std::vector<uint32_t> bodyUid;  // user id of each boundary body
   DBuf<double> bnd;     // enlarged AABB of each boundary body at the last refresh (min3, max3)
   double rmin = INFINITY, rmax = 0;
   DBuf<double> pos[3], vel[3], angVel[3], ori[4], force[3], torque[3], inertia[3], lastPos[3];
@@ -256,6 +260,7 @@ struct Ctx {
     int64_t gen = -1;
     int K = 0, cfBlocks = 0, rcur = -1, parity = -1;
     bool track = false;  // captured with energy tracking
+    bool intra = false;  // captured with IntraForce
     double dt = 0, damping = 0;
     wdg_contact_params cp{};
   } graph;
@@ -371,6 +376,8 @@ void contact_loop(Ctx& c, const wdg_contact_params* p);
 void leapfrog(Ctx& c, const wdg_leapfrog_params* p);
 void force_reset(Ctx& c);
 void flush_forces(Ctx& c);
+// IntraForce([In2_Facet]): the contact forces of each facet spread to its nodes (fnode)
+void intra_force(Ctx& c);
 void run_graph(Ctx& c, const wdg_program* prog, int64_t nSteps, int64_t* step, double* dt);
 uint32_t read_u32(Ctx& c, const uint32_t* d);
 int64_t tune(Ctx& c, const wdg_program* prog, int64_t* step, double* dt);

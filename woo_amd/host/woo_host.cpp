@@ -98,6 +98,21 @@ void DemField::pull() {
     check(wdg_download(dev->ctx, f, buf.data()));
     for (size_t i = 0; i < n; i++) std::copy(&buf[i * w], &buf[i * w] + w, node_field(particles[i]->node0(), f));
   }
+  // facet nodes: the force and torque of the last IntraForce (In2_Facet), zero without one
+  std::vector<int64_t> fid;
+  for (size_t i = 0; i < n; i++)
+    if (dynamic_cast<const Facet*>(particles[i]->shape.get())) fid.push_back((int64_t)i);
+  if (fid.empty()) return;
+  std::vector<double> fn(18 * fid.size());
+  check(wdg_facet_nodes(dev->ctx, fid.data(), (int64_t)fid.size(), fn.data()));
+  for (size_t j = 0; j < fid.size(); j++) {
+    auto& nodes = particles[fid[j]]->shape->nodes;
+    for (int k = 0; k < 3; k++)
+      for (int q = 0; q < 3; q++) {
+        nodes[k]->dem.force[q] = fn[18 * j + 3 * k + q];
+        nodes[k]->dem.torque[q] = fn[18 * j + 9 + 3 * k + q];
+      }
+  }
 }
 
 void DemField::push(Scene& S) {
@@ -367,6 +382,9 @@ wdg_program Scene::program(bool& hostEngines) {
       p.leapfrog.damping = lf->damping;
       p.leapfrog.reset = lf->reset ? 1 : 0;
       p.engines[p.nEngines++] = WDG_E_LEAPFROG;
+    } else if (auto* it = dynamic_cast<IntraForce*>(e.get())) {
+      if (anyFacet && !find_functor(it->funcs, "In2_Facet")) throw std::runtime_error("IntraForce: facets need In2_Facet");
+      p.engines[p.nEngines++] = WDG_E_INTRAFORCE;
     } else if (dynamic_cast<ForceResetter*>(e.get())) {
       p.engines[p.nEngines++] = WDG_E_FORCERESETTER;
     } else if (auto* dd = dynamic_cast<DynDt*>(e.get())) {

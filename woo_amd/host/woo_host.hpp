@@ -165,6 +165,12 @@ struct Law2_L6Geom_HertzPhys_DMT : LawFunctor {
   const char* className() const override { return "Law2_L6Geom_HertzPhys_DMT"; }
 };
 
+// IntraForce functors (Facet.hpp:66-76): In2_Facet spreads a facet's contact forces to its nodes
+struct IntraFunctor : Functor {};
+struct In2_Facet : IntraFunctor {
+  const char* className() const override { return "In2_Facet"; }
+};
+
 // ------------------------------------------------------------------ engines
 struct Scene;
 struct Engine : Object {
@@ -203,6 +209,11 @@ struct Leapfrog : Engine {
   double damping = .2;  // Leapfrog.hpp:50-51
   bool reset = false;
   const char* className() const override { return "Leapfrog"; }
+};
+// IntraForce (IntraForce.hpp): dispatches IntraFunctors by shape; the device runs In2_Facet
+struct IntraForce : Engine {
+  std::vector<std::shared_ptr<IntraFunctor>> funcs;
+  const char* className() const override { return "IntraForce"; }
 };
 struct ForceResetter : Engine {
   const char* className() const override { return "ForceResetter"; }

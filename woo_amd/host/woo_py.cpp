@@ -100,6 +100,8 @@ py::object labels(Scene& S) {
       for (auto& f : c->phyDisp) put(f);
       for (auto& f : c->lawDisp) put(f);
     }
+    if (auto* c = dynamic_cast<IntraForce*>(e.get()))
+      for (auto& f : c->funcs) put(f);
   }
   return ns;
 }
@@ -411,6 +413,10 @@ PYBIND11_MODULE(_woo, m) {
       .def(py::init(&make_kw<Law2_L6Geom_HertzPhys_DMT>))
       .def_readwrite("noAttraction", &Law2_L6Geom_HertzPhys_DMT::noAttraction);
 
+  py::class_<IntraFunctor, Functor, std::shared_ptr<IntraFunctor>>(m, "IntraFunctor", py::dynamic_attr());
+  py::class_<In2_Facet, IntraFunctor, std::shared_ptr<In2_Facet>>(m, "In2_Facet", py::dynamic_attr())
+      .def(py::init(&make_kw<In2_Facet>));
+
   // ---------------------------------------------------------------- engines
   py::class_<Engine, Object, std::shared_ptr<Engine>>(m, "Engine", py::dynamic_attr())
       .def_readwrite("dead", &Engine::dead)
@@ -451,6 +457,15 @@ PYBIND11_MODULE(_woo, m) {
       .def(py::init(&make_kw<Leapfrog>))
       .def_readwrite("damping", &Leapfrog::damping)
       .def_readwrite("reset", &Leapfrog::reset);
+  // IntraForce([In2_*]) (IntraForce.hpp)
+  py::class_<IntraForce, Engine, std::shared_ptr<IntraForce>>(m, "IntraForce", py::dynamic_attr())
+      .def(py::init([](py::args a, py::kwargs kw) {
+        auto o = make_kw<IntraForce>(kw);
+        if (a.size() > 1) throw py::type_error("IntraForce takes one positional argument (intra functors)");
+        if (a.size() == 1) o->funcs = a[0].cast<std::vector<std::shared_ptr<IntraFunctor>>>();
+        return o;
+      }))
+      .def_readwrite("funcs", &IntraForce::funcs);
   py::class_<ForceResetter, Engine, std::shared_ptr<ForceResetter>>(m, "ForceResetter", py::dynamic_attr())
       .def(py::init(&make_kw<ForceResetter>));
   py::class_<DynDt, PeriodicEngine, std::shared_ptr<DynDt>>(m, "DynDt", py::dynamic_attr())

@@ -5,7 +5,8 @@ from woo_amd._woo import (  # noqa: F401
     Bo1_Facet_Aabb, Bo1_Sphere_Aabb, Bo1_Wall_Aabb, BoundFunctor, Cg2_Facet_Sphere_L6Geom, Cg2_Sphere_Sphere_L6Geom,
     Cg2_Wall_Sphere_L6Geom, CGeomFunctor, Facet,
     Contact, ContactContainer, ContactLoop, Cp2_FrictMat_FrictPhys, Cp2_HertzMat_HertzPhys, CPhysFunctor, DemData,
-    DemField, DynDt, ElastMat, ForceResetter, FrictMat, FrictPhys, Functor, HertzMat, InsertionSortCollider, L6Geom,
+    DemField, DynDt, ElastMat, ForceResetter, FrictMat, FrictPhys, Functor, HertzMat, In2_Facet, InsertionSortCollider,
+    IntraForce, IntraFunctor, L6Geom,
     Law2_L6Geom_FrictPhys_IdealElPl, Law2_L6Geom_HertzPhys_DMT, LawFunctor, Leapfrog, Material, Particle,
     ParticleContainer, Shape, Sphere, Wall)
 
