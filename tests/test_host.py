@@ -137,3 +137,73 @@ def test_facet_constructors(woo):
         assert abs(np.dot(v, f.shape.getNormal())) < 1e-12
     with pytest.raises(NotImplementedError):
         woo.triangulated.cylinder((0, 0, 0), (.4, 0, 0), .5, realRot=True, angVel=1.)
+
+
+def test_background_run_error_is_raised_by_wait(woo):
+    # Scene.run(n, wait=False) runs on a worker thread; its exception (here: no device engine on a
+    # machine without a GPU) is kept and raised again by S.wait() (Scene.cpp:34-51, 64-83)
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from woo_amd.woo.dem import DemField
+
+    S = woo.core.Scene(fields=[DemField()])
+    S.dem.par.add(woo.utils.sphere((0, 0, 0), .01))
+    S.engines = woo.utils.defaultEngines()
+    S.run(5, wait=False)
+    with pytest.raises(RuntimeError):
+        S.wait()
+    assert not S.running
+    S.wait()  # the error was raised once
+    with pytest.raises(RuntimeError):
+        S.run(1)
+
+
+def test_savetmp_loadtmp_copies_without_device(woo):
+    # Scene.saveTmp / Scene.loadTmp: a copy in a named slot, a fresh copy on each load; a material
+    # shared by two particles stays shared (test_core.py:129-136 of the reference)
+    from woo_amd.woo.dem import DemField, In2_Facet, IntraForce
+
+    S = woo.core.Scene(fields=[DemField(gravity=(0, 0, -9.81))])
+    m = woo.utils.defaultMaterial()
+    S.dem.par.add([woo.utils.sphere((0, 0, 0), 1, mat=m), woo.utils.sphere((0, 0, 3), 2, mat=m)])
+    S.engines = woo.utils.defaultEngines(damping=.3) + [IntraForce([In2_Facet(label='in2')], label='intra')]
+    S.dt = 1e-4
+    S.step = 7
+    S.trackEnergy = True
+    S.saveTmp(quiet=True)
+    S2 = woo.core.Scene.loadTmp()
+    assert S2 is not S and S2.dt == 1e-4 and S2.step == 7 and S2.trackEnergy
+    assert list(S2.dem.gravity) == [0, 0, -9.81]
+    S2.dem.par[0].mat.young = 9087438484
+    assert S2.dem.par[0].mat.young == S2.dem.par[1].mat.young
+    assert S.dem.par[0].mat.young != 9087438484  # the original is untouched
+    assert [type(e).__name__ for e in S2.engines] == [type(e).__name__ for e in S.engines]
+    assert S2.lab.leapfrog.damping == .3 and S2.lab.leapfrog is not S.lab.leapfrog
+    assert type(S2.lab.in2).__name__ == "In2_Facet" and S2.lab.in2 is not S.lab.in2
+    S2.dem.par[1].pos = (1, 2, 3)
+    assert list(S.dem.par[1].pos) == [0, 0, 3]
+    S3 = woo.core.Scene.loadTmp('')
+    assert list(S3.dem.par[1].pos) == [0, 0, 3]
+    with pytest.raises(RuntimeError):
+        woo.core.Scene.loadTmp('no such slot')
+
+
+def test_energy_tracker_surface_without_device(woo):
+    # Scene.energy before any run: no terms; values set by EnergyTracker.__setitem__ are kept for
+    # the device (e.g. those of a checkpoint)
+    from woo_amd.woo.dem import DemField
+
+    S = woo.core.Scene(fields=[DemField()])
+    assert len(S.energy) == 0 and S.energy.keys() == [] and S.energy.total() == 0
+    with pytest.raises(KeyError):
+        S.energy['kinetic']
+    S.energy['plast'] = 2.5
+    S.energy['kinetic'] = -.5
+    assert S.energy.keys() == ['kinetic', 'plast'] and S.energy['plast'] == 2.5 and 'plast' in S.energy
+    assert S.energy.total() == 2. and S.energy.relErr() == pytest.approx(2. / 3.)
+    with pytest.raises(KeyError):
+        S.energy['gravV'] = 1.
+    S.energy.clear()
+    assert len(S.energy) == 0

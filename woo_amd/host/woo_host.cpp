@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
+#include <typeinfo>
 
 namespace woo {
 
@@ -35,7 +37,18 @@ DeviceState::~DeviceState() {
 }
 
 // ------------------------------------------------------------------ DemField
+// the scene whose steps the current thread runs in the background (none on other threads)
+static thread_local const Scene* tlsWorkerOf = nullptr;
+
+bool Scene::inWorker() const { return tlsWorkerOf == this; }
+
+void DemField::guard() const {
+  if (scene && scene->running() && tlsWorkerOf != scene)
+    throw std::runtime_error("the scene is running in the background (call S.wait() first)");
+}
+
 int64_t DemField::add(const std::shared_ptr<Particle>& p, int nodes) {
+  guard();
   if (!p || !p->shape || !p->material) throw std::invalid_argument("particle needs a material and a shape");
   if (p->id >= 0) throw std::invalid_argument("particle was already added (id " + std::to_string(p->id) + ")");
   if (auto* f = dynamic_cast<Facet*>(p->shape.get())) {
@@ -89,6 +102,7 @@ static int field_width(int f) { return f == WDG_F_ORI ? 4 : (f == WDG_F_MASS ? 1
 
 void DemField::pull() {
   if (!deviceNewer || !dev) return;
+  guard();
   deviceNewer = false;
   const size_t n = particles.size();
   std::vector<double> buf;
@@ -116,9 +130,11 @@ void DemField::pull() {
 }
 
 void DemField::push(Scene& S) {
+  guard();
   if (!dev) {
     dev = std::make_shared<DeviceState>(S.device);
     structureChanged = true;
+    devTrackEnergy = false;
   }
   wdg_field fd{};
   for (int k = 0; k < 3; k++) fd.gravity[k] = gravity[k];
@@ -193,6 +209,14 @@ void DemField::push(Scene& S) {
     pd.facetVerts = fverts.data(); pd.fakeVel = fvel.data(); pd.halfThick = fht.data();
     check(wdg_upload_particles(dev->ctx, &pd));
     structureChanged = hostDirty = false;
+    if (savedContacts) {
+      // a loaded copy: its real contacts come back with their history
+      const auto sc = savedContacts;
+      savedContacts.reset();
+      check(wdg_restore_contacts(dev->ctx, (int64_t)sc->ids.size() / 2, sc->ids.data(),
+                                 sc->cellDist.empty() ? nullptr : sc->cellDist.data(), sc->data.data(),
+                                 sc->verletDist));
+    }
   } else if (hostDirty) {
     std::vector<double> buf;
     for (int f : FIELDS) {
@@ -209,6 +233,7 @@ void DemField::push(Scene& S) {
 }
 
 const std::vector<ContactView>& DemField::contacts() {
+  guard();
   if (contactsValid) return contactCache;
   contactCache.clear();
   if (dev && !structureChanged) {
@@ -265,6 +290,7 @@ double DemField::pWaveDt() const {
 }
 
 std::array<double, 3> DemField::observables(const Vec3& lo, const Vec3& hi) {
+  guard();
   if (!dev || structureChanged) throw std::runtime_error("no simulation state on the device yet (run a step first)");
   std::array<double, 3> out{};
   check(wdg_observables(dev->ctx, lo.data(), hi.data(), out.data()));
@@ -272,6 +298,7 @@ std::array<double, 3> DemField::observables(const Vec3& lo, const Vec3& hi) {
 }
 
 std::array<double, 9> DemField::stress(double volume) {
+  guard();
   if (!dev || structureChanged) throw std::runtime_error("no simulation state on the device yet (run a step first)");
   std::array<double, 9> out{};
   check(wdg_stress(dev->ctx, volume, out.data()));
@@ -279,6 +306,7 @@ std::array<double, 9> DemField::stress(double volume) {
 }
 
 double DemField::kineticEnergy() {
+  guard();
   if (!dev || structureChanged) throw std::runtime_error("no simulation state on the device yet (run a step first)");
   double e = 0;
   check(wdg_kinetic_energy(dev->ctx, &e));
@@ -408,21 +436,35 @@ void Scene::run(int64_t steps) {
   stopRequested = false;
   f->scene = this;
   f->push(*this);
+  wdg_ctx* ctx = f->dev->ctx;
+  if (f->devTrackEnergy != trackEnergy) {
+    check(wdg_set_energy_tracking(ctx, trackEnergy ? 1 : 0));
+    f->devTrackEnergy = trackEnergy;
+  }
+  if (pendingEnergy) {
+    check(wdg_energy_set(ctx, pendingEnergy->data()));
+    pendingEnergy.reset();
+  }
   if (std::isnan(dt) || std::isinf(dt) || dt < 0) autoDt(*f);
   bool hostEngines = false;
   const wdg_program prog = program(hostEngines);
-  wdg_ctx* ctx = f->dev->ctx;
   bool hasDyn = false;
   for (int i = 0; i < prog.nEngines; i++) hasDyn = hasDyn || prog.engines[i] == WDG_E_DYNDT;
   f->contactsValid = false;
   if (!hostEngines && !hasDyn) {
-    // one call for all steps; time accumulates step by step as in doOneStep
-    int64_t st = step;
-    double d = dt;
-    f->deviceNewer = true;
-    check(wdg_run(ctx, &prog, steps, &st, &d));
-    for (int64_t s = 0; s < steps; s++) time += dt;
-    step = st;
+    // one call for all steps (a background run: chunks, so that stop() ends it between two);
+    // time accumulates step by step as in doOneStep
+    const int64_t chunk = tlsWorkerOf == this ? 256 : steps;
+    for (int64_t done = 0; done < steps && !(done > 0 && stopRequested);) {
+      const int64_t k = std::min(chunk, steps - done);
+      int64_t st = step;
+      double d = dt;
+      f->deviceNewer = true;
+      check(wdg_run(ctx, &prog, k, &st, &d));
+      for (int64_t s = 0; s < k; s++) time += dt;
+      step = st;
+      done += k;
+    }
     return;
   }
   for (int64_t s = 0; s < steps && !stopRequested; s++) {
@@ -462,6 +504,192 @@ void Scene::run(int64_t steps) {
       nextDt = NaN;
     }
   }
+}
+
+Scene::~Scene() {
+  if (bg.joinable()) {
+    stopRequested = true;
+    // the worker holds the scene; the last reference can be dropped on the worker itself
+    if (bg.get_id() == std::this_thread::get_id()) bg.detach();
+    else bg.join();
+  }
+}
+
+void Scene::runAsync(int64_t steps) {
+  if (bgRunning) throw std::runtime_error("Scene.run: already running");
+  if (bg.joinable()) bg.join();
+  if (!dem()) throw std::runtime_error("Scene has no DemField");
+  bgExc = nullptr;
+  bgRunning = true;
+  // the worker keeps the scene alive until it ends
+  auto self = std::static_pointer_cast<Scene>(shared_from_this());
+  bg = std::thread([self, steps] {
+    tlsWorkerOf = self.get();
+    try {
+      self->run(steps);
+    } catch (...) {
+      self->bgExc = std::current_exception();
+    }
+    tlsWorkerOf = nullptr;
+    self->bgRunning = false;
+  });
+}
+
+void Scene::wait() {
+  if (bg.joinable() && bg.get_id() != std::this_thread::get_id()) bg.join();
+  if (bgExc) {
+    std::exception_ptr e = bgExc;
+    bgExc = nullptr;
+    std::rethrow_exception(e);
+  }
+}
+
+std::array<double, WDG_EN_N> Scene::energy() {
+  std::array<double, WDG_EN_N> e{};
+  if (pendingEnergy) return *pendingEnergy;
+  auto f = dem();
+  if (!f || !f->dev) return e;
+  f->guard();
+  check(wdg_energy(f->dev->ctx, e.data()));
+  return e;
+}
+
+void Scene::energyClear() {
+  pendingEnergy.reset();
+  auto f = dem();
+  if (!f || !f->dev) return;
+  f->guard();
+  check(wdg_energy_clear(f->dev->ctx));
+}
+
+void Scene::energySet(const std::array<double, WDG_EN_N>& v) {
+  auto f = dem();
+  if (!f || !f->dev) {
+    pendingEnergy = std::make_shared<std::array<double, WDG_EN_N>>(v);
+    return;
+  }
+  f->guard();
+  check(wdg_energy_set(f->dev->ctx, v.data()));
+}
+
+// ------------------------------------------------------------------ copies (saveTmp / loadTmp)
+// a copy of the object behind p, of its exact class among T, Ts...
+template <class B, class T, class... Ts>
+static std::shared_ptr<B> copy_as(const std::shared_ptr<B>& p) {
+  if (!p) return nullptr;
+  if (typeid(*p) == typeid(T)) return std::make_shared<T>(static_cast<const T&>(*p));
+  if constexpr (sizeof...(Ts) > 0) return copy_as<B, Ts...>(p);
+  else throw std::runtime_error(std::string("Scene.saveTmp: cannot copy a ") + p->className());
+}
+template <class B>
+static std::vector<std::shared_ptr<B>> copy_functors(const std::vector<std::shared_ptr<B>>& v) {
+  std::vector<std::shared_ptr<B>> out;
+  for (const auto& f : v) {
+    std::shared_ptr<Functor> g =
+        copy_as<Functor, Bo1_Sphere_Aabb, Bo1_Wall_Aabb, Bo1_Facet_Aabb, Cg2_Sphere_Sphere_L6Geom, Cg2_Wall_Sphere_L6Geom,
+                Cg2_Facet_Sphere_L6Geom, Cp2_FrictMat_FrictPhys, Cp2_HertzMat_HertzPhys, Law2_L6Geom_FrictPhys_IdealElPl,
+                Law2_L6Geom_HertzPhys_DMT, In2_Facet>(std::static_pointer_cast<Functor>(f));
+    out.push_back(std::static_pointer_cast<B>(g));
+  }
+  return out;
+}
+static std::shared_ptr<Engine> copy_engine(const std::shared_ptr<Engine>& e) {
+  auto c = copy_as<Engine, InsertionSortCollider, ContactLoop, Leapfrog, ForceResetter, DynDt, PyRunner, IntraForce>(e);
+  if (auto* x = dynamic_cast<InsertionSortCollider*>(c.get())) x->boundDispatcher = copy_functors(x->boundDispatcher);
+  if (auto* x = dynamic_cast<ContactLoop*>(c.get())) {
+    x->geoDisp = copy_functors(x->geoDisp);
+    x->phyDisp = copy_functors(x->phyDisp);
+    x->lawDisp = copy_functors(x->lawDisp);
+  }
+  if (auto* x = dynamic_cast<IntraForce*>(c.get())) x->funcs = copy_functors(x->funcs);
+  return c;
+}
+
+std::shared_ptr<DemField> DemField::copy() {
+  guard();
+  pull();
+  auto f = std::make_shared<DemField>();
+  f->label = label;
+  f->gravity = gravity;
+  f->distFactor = distFactor;
+  f->loneMask = loneMask;
+  std::map<const Material*, std::shared_ptr<Material>> mats;
+  std::map<const Node*, std::shared_ptr<Node>> nodes;
+  for (const auto& p : particles) {
+    auto q = std::make_shared<Particle>(*p);
+    auto& m = mats[p->material.get()];
+    if (!m) m = copy_as<Material, HertzMat, FrictMat, ElastMat, Material>(p->material);
+    q->material = m;
+    q->shape = copy_as<Shape, Sphere, Wall, Facet, Shape>(p->shape);
+    for (auto& n : q->shape->nodes) {
+      auto& c = nodes[n.get()];
+      if (!c) {
+        c = std::make_shared<Node>(*n);
+        c->field = f.get();
+      }
+      n = c;
+    }
+    f->particles.push_back(q);
+  }
+  f->structureChanged = true;
+  if (dev && !structureChanged) {
+    auto sc = std::make_shared<SavedContacts>();
+    int64_t nc = 0;
+    check(wdg_contact_count(dev->ctx, 1, &nc));
+    sc->ids.resize(2 * nc);
+    sc->cellDist.resize(3 * nc);
+    sc->data.resize(16 * nc);
+    if (nc) check(wdg_contact_list(dev->ctx, 1, sc->ids.data(), sc->cellDist.data(), sc->data.data()));
+    wdg_counters st{};
+    check(wdg_stats(dev->ctx, &st));
+    sc->verletDist = st.verletDist;
+    if (nc) f->savedContacts = sc;
+  } else {
+    f->savedContacts = savedContacts;
+  }
+  return f;
+}
+
+std::shared_ptr<Scene> Scene::copy() {
+  auto f = dem();
+  if (f) f->guard();
+  auto S = std::make_shared<Scene>();
+  S->label = label;
+  S->dt = dt; S->nextDt = nextDt; S->dtSafety = dtSafety; S->time = time; S->step = step;
+  S->periodic = periodic;
+  S->device = device;
+  S->trackEnergy = trackEnergy;
+  S->cell = std::make_shared<Cell>(*cell);
+  for (const auto& e : engines) S->engines.push_back(copy_engine(e));
+  for (const auto& g : fields) S->fields.push_back(g ? g->copy() : nullptr);
+  const auto e = energy();
+  bool any = trackEnergy;
+  for (double x : e) any = any || x != 0.;
+  if (any) S->pendingEnergy = std::make_shared<std::array<double, WDG_EN_N>>(e);
+  return S;
+}
+
+static std::mutex g_tmpMu;
+static std::map<std::string, std::shared_ptr<Scene>>& tmp_store() {
+  static std::map<std::string, std::shared_ptr<Scene>> m;
+  return m;
+}
+
+void Scene::saveTmp(const std::string& slot, bool /*quiet*/) {
+  auto c = copy();
+  std::lock_guard<std::mutex> lk(g_tmpMu);
+  tmp_store()[slot] = c;
+}
+
+std::shared_ptr<Scene> Scene::loadTmp(const std::string& slot) {
+  std::shared_ptr<Scene> s;
+  {
+    std::lock_guard<std::mutex> lk(g_tmpMu);
+    auto it = tmp_store().find(slot);
+    if (it == tmp_store().end()) throw std::runtime_error("Scene.loadTmp: no scene saved in slot '" + slot + "'");
+    s = it->second;
+  }
+  return s->copy();
 }
 
 }  // namespace woo

@@ -11,13 +11,17 @@
 // downloaded on the first read after a run.
 #pragma once
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <exception>
 #include <functional>
 #include <limits>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/woo_amd.h"
@@ -264,7 +268,19 @@ struct DemField : Object {
   std::vector<ContactView> contactCache;
   const char* className() const override { return "DemField"; }
 
+  // state loaded with Scene.loadTmp, given to the device at the first upload: the real
+  // contacts (wdg_contact_list layout) and the resolved Verlet distance of the saved run
+  struct SavedContacts {
+    std::vector<int32_t> ids, cellDist;
+    std::vector<double> data;
+    double verletDist = NaN;
+  };
+  std::shared_ptr<SavedContacts> savedContacts;
+  bool devTrackEnergy = false;   // wdg_set_energy_tracking state of the device context
+
   int64_t add(const std::shared_ptr<Particle>& p, int nodes);
+  // device access from another thread while the scene runs in the background is refused
+  void guard() const;
   void pull();                   // device -> host, if the device is newer
   void push(Scene& S);           // host -> device, if the host changed
   void touch() { pull(); hostDirty = true; contactsValid = false; }
@@ -275,25 +291,51 @@ struct DemField : Object {
   std::array<double, 3> observables(const Vec3& lo, const Vec3& hi);
   std::array<double, 9> stress(double volume);  // DemFuncs::stressStiffness, stress part
   double kineticEnergy();
+  // a copy for Scene.saveTmp: particles, shapes, nodes and materials (shared materials stay
+  // shared), with the device's real contacts
+  std::shared_ptr<DemField> copy();
 };
 
-struct Scene : Object {
+struct Scene : Object, std::enable_shared_from_this<Scene> {
   std::vector<std::shared_ptr<DemField>> fields;
   std::vector<std::shared_ptr<Engine>> engines;
   double dt = NaN, nextDt = NaN, dtSafety = .9, time = 0;  // Scene.hpp:191-193
   int64_t step = 0;
   bool periodic = false;
   std::shared_ptr<Cell> cell = std::make_shared<Cell>();
-  bool stopRequested = false;
+  std::atomic<bool> stopRequested{false};
   int device = 0;
+  bool trackEnergy = false;  // Scene.trackEnergy (Scene.hpp:204)
+  // energy terms given to the device at the next run (Scene.loadTmp, EnergyTracker.set)
+  std::shared_ptr<std::array<double, WDG_EN_N>> pendingEnergy;
   const char* className() const override { return "Scene"; }
+  ~Scene() override;
 
   std::shared_ptr<DemField> dem();
   void run(int64_t steps);
   void one() { run(1); }
   void stop() { stopRequested = true; }
+  // Scene.run(steps, wait=False) (Scene.cpp:34-51): the steps run on a worker thread; an
+  // exception there is kept and rethrown by wait() (Scene.cpp:64-83, 94-117)
+  void runAsync(int64_t steps);
+  void wait();
+  bool running() const { return bgRunning; }
+  bool inWorker() const;  // the calling thread is this scene's background worker
+  // Scene.energy (EnergyTracker): the device's terms in WDG_EN_* order
+  std::array<double, WDG_EN_N> energy();
+  void energyClear();
+  void energySet(const std::array<double, WDG_EN_N>& v);
+  // Scene.saveTmp / loadTmp (Scene.cpp:291-294, Master.cpp:280-300): a copy of the scene in a
+  // named in-memory slot, and a fresh copy of it on each load
+  std::shared_ptr<Scene> copy();
+  void saveTmp(const std::string& slot, bool quiet);
+  static std::shared_ptr<Scene> loadTmp(const std::string& slot);
 
  private:
+  std::thread bg;
+  std::atomic<bool> bgRunning{false};
+  std::exception_ptr bgExc;
+
   void autoDt(DemField& f);
   wdg_program program(bool& hostEngines);
 };

@@ -3,6 +3,9 @@
 // WOO_CLASS_BASE_DOC_ATTRS blocks of src/core/Scene.hpp, src/dem/Particle.hpp,
 // src/dem/ParticleContainer.hpp:182-184, src/dem/ContactContainer.hpp): objects take
 // their attributes as keyword arguments, vectors are returned as numpy arrays.
+#include <algorithm>
+#include <cmath>
+
 #include <pybind11/eval.h>
 #include <pybind11/functional.h>
 #include <pybind11/numpy.h>
@@ -83,6 +86,41 @@ py::object contact_or_none(const std::vector<ContactView>& v, int64_t a, int64_t
   for (const auto& c : v)
     if ((c.id1 == a && c.id2 == b) || (c.id1 == b && c.id2 == a)) return py::cast(ContactRef{c});
   return py::none();
+}
+
+// Python code run by an engine: on a background run its Python error becomes a C++ one, kept
+// by the worker and raised again by S.wait() (the Python error object stays on this thread)
+template <class F>
+void in_python(Scene& S, F&& f) {
+  if (!S.inWorker()) {
+    f();
+    return;
+  }
+  try {
+    f();
+  } catch (py::error_already_set& e) {
+    throw std::runtime_error(e.what());
+  }
+}
+
+// Scene.energy: EnergyTracker's Python surface (EnergyTracker.hpp:30-60) over the device's terms
+struct EnergyRef {
+  std::shared_ptr<Scene> S;
+};
+static const char* const EN_NAMES[WDG_EN_N] = {"kinetic", "elast", "grav", "nonviscDamp", "plast", "viscN", "viscT"};
+static int en_index(const std::string& name) {
+  for (int k = 0; k < WDG_EN_N; k++)
+    if (name == EN_NAMES[k]) return k;
+  return -1;
+}
+// the terms created so far (non-zero), in name order as the reference's std::map
+static std::vector<std::pair<std::string, double>> en_items(Scene& S) {
+  const auto e = S.energy();
+  std::vector<std::pair<std::string, double>> v;
+  for (int k = 0; k < WDG_EN_N; k++)
+    if (e[k] != 0.) v.emplace_back(EN_NAMES[k], e[k]);
+  std::sort(v.begin(), v.end());
+  return v;
 }
 
 // S.lab: labelled engines and functors (LabelMapper)
@@ -496,15 +534,21 @@ PYBIND11_MODULE(_woo, m) {
               r.source = cmd.cast<std::string>();
               // a string runs with S and woo in its namespace (PyRunner.command)
               r.command = [keep](Scene& S) {
-                py::dict g;
-                g["S"] = py::cast(&S, py::return_value_policy::reference);
-                g["woo"] = py::module_::import("woo_amd.woo");
-                g["__builtins__"] = py::module_::import("builtins");
-                py::exec(keep->cast<std::string>(), g);
+                py::gil_scoped_acquire gil;
+                in_python(S, [&] {
+                  py::dict g;
+                  g["S"] = py::cast(&S, py::return_value_policy::reference);
+                  g["woo"] = py::module_::import("woo_amd.woo");
+                  g["__builtins__"] = py::module_::import("builtins");
+                  py::exec(keep->cast<std::string>(), g);
+                });
               };
             } else {
               r.source = py::str(py::repr(cmd)).cast<std::string>();
-              r.command = [keep](Scene&) { (*keep)(); };
+              r.command = [keep](Scene& S) {
+                py::gil_scoped_acquire gil;
+                in_python(S, [&] { (*keep)(); });
+              };
             }
           });
 
@@ -527,9 +571,76 @@ PYBIND11_MODULE(_woo, m) {
         return f;
       })
       .def_property_readonly("lab", &labels)
-      .def("run", [](Scene& S, int64_t steps, bool /*wait*/) { S.run(steps); }, py::arg("steps"),
-           py::arg("wait") = true)
+      .def_readwrite("trackEnergy", &Scene::trackEnergy)
+      .def_property_readonly("energy", [](std::shared_ptr<Scene> S) { return EnergyRef{S}; })
+      .def(
+          "run",
+          [](Scene& S, int64_t steps, bool wait) {
+            if (wait) S.run(steps);
+            else S.runAsync(steps);
+          },
+          py::arg("steps"), py::arg("wait") = true)
       .def("one", [](py::object self) { self.attr("run")(1); })
-      .def("wait", [](Scene&) {})
-      .def("stop", &Scene::stop);
+      .def("wait", [](Scene& S) {
+        py::gil_scoped_release nogil;
+        S.wait();
+      })
+      .def_property_readonly("running", &Scene::running)
+      .def("stop", &Scene::stop)
+      .def("saveTmp", &Scene::saveTmp, py::arg("slot") = "", py::arg("quiet") = false)
+      .def_static("loadTmp", &Scene::loadTmp, py::arg("slot") = "");
+
+  py::class_<EnergyRef>(m, "EnergyTracker")
+      .def("__getitem__",
+           [](EnergyRef& r, const std::string& name) {
+             for (const auto& it : en_items(*r.S))
+               if (it.first == name) return it.second;
+             throw py::key_error("Unknown energy name '" + name + "'.");
+           })
+      .def("__setitem__",
+           [](EnergyRef& r, const std::string& name, double v) {
+             const int k = en_index(name);
+             if (k < 0) throw py::key_error("energy term '" + name + "' is not one this engine tracks");
+             auto e = r.S->energy();
+             e[k] = v;
+             r.S->energySet(e);
+           })
+      .def("__contains__",
+           [](EnergyRef& r, const std::string& name) {
+             for (const auto& it : en_items(*r.S))
+               if (it.first == name) return true;
+             return false;
+           })
+      .def("__len__", [](EnergyRef& r) { return en_items(*r.S).size(); })
+      .def("__iter__",
+           [](EnergyRef& r) {
+             py::list l;
+             for (const auto& it : en_items(*r.S)) l.append(it.first);
+             return py::iter(l);
+           })
+      .def("keys",
+           [](EnergyRef& r) {
+             py::list l;
+             for (const auto& it : en_items(*r.S)) l.append(it.first);
+             return l;
+           })
+      .def("items",
+           [](EnergyRef& r) {
+             py::list l;
+             for (const auto& it : en_items(*r.S)) l.append(py::make_tuple(it.first, it.second));
+             return l;
+           })
+      .def("total",
+           [](EnergyRef& r) {
+             double t = 0;
+             for (const auto& it : en_items(*r.S)) t += it.second;
+             return t;
+           })
+      .def("relErr",
+           [](EnergyRef& r) {
+             double a = 0, t = 0;
+             for (const auto& it : en_items(*r.S)) { a += std::abs(it.second); t += it.second; }
+             return a > 0 ? t / a : 0.;
+           })
+      .def("clear", [](EnergyRef& r) { r.S->energyClear(); });
 }
