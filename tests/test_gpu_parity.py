@@ -77,6 +77,29 @@ def test_broadphase_pairs_aperiodic(n, dense, seed):
     assert _sets(o.brute_aabb_pairs()) == _sets(g.d.contacts(data=False)[0])
 
 
+def test_sparse_scene_hashed_cells():
+    # a few spheres far from the rest make the bounding box so large that a dense cell grid
+    # would need far more cells than spheres: the cells are hashed into buckets instead; the
+    # pair set, the contacts and the trajectories stay those of the oracle
+    sc = deposition(n=2000, dense=True, seed=7)
+    far = [(40., 0., 0.), (0., -25., 0.), (-30., 20., 0.), (12., 12., 0.), (0., 3., 0.)]
+    for k, d in enumerate(far):
+        sc["pos"][k] += np.array(d)
+    o = build_oracle(sc)
+    g = GpuRunner(sc)
+    o.run(["Collider"])
+    g.run(["Collider"])
+    compare_contact_sets(o, g, real_only=False)
+    assert _sets(o.brute_aabb_pairs()) == _sets(g.d.contacts(data=False)[0])
+    st = g.d.stats()
+    cells = float(np.prod(st["gridCells"]))
+    print("grid", st["gridCells"], "cells", cells, "spheres", len(sc["radius"]))
+    assert cells > 8 * len(sc["radius"])
+    o, g, worst = _run_both(sc, ENG, 100, check_every=20)
+    print("hashed-cell run: worst position error", worst, "refreshes", g.d.stats()["nRefresh"])
+    assert worst <= TRAJ_TOL
+
+
 @pytest.mark.parametrize("n,spacing,seed", [(300, 1.0, 4), (2000, 1.01, 5)])
 def test_broadphase_pairs_periodic(n, spacing, seed):
     sc = periodic_hertz(n=n, spacing=spacing, seed=seed)

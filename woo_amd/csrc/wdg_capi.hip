@@ -681,6 +681,17 @@ int wdg_upload_particles(wdg_ctx* ctx, const wdg_particles* p) {
     WDG_HIP(hipMemcpy(c.fgeo.p, fgeo.data(), fgeo.size() * sizeof(double), hipMemcpyHostToDevice));
     c.bnd.ensure((size_t)6 * (nW + 1));
     c.fnode.ensure((size_t)18 * (nW + 1));
+    {
+      std::vector<uint32_t> gb;
+      for (int64_t b = 0; b < nW; b++)
+        if (((flags[b] >> F_SHAPE_SHIFT) & 3u) != (uint32_t)WDG_FACET) gb.push_back((uint32_t)b);
+      c.nGBody = (int64_t)gb.size();
+      c.gbodyHost = gb;
+      c.gbody.ensure(gb.size() + 1);
+      if (!gb.empty()) WDG_HIP(hipMemcpy(c.gbody.p, gb.data(), gb.size() * 4, hipMemcpyHostToDevice));
+      c.bndPart.ensure((size_t)64 * 6 * (gb.size() + 1));
+      c.bndParts = 1;
+    }
     WDG_HIP(hipMemset(c.fnode.p, 0, (size_t)18 * (nW + 1) * sizeof(double)));
     c.bodyUid.assign(order.begin(), order.begin() + nW);
     DBuf<double>* d[NPD] = {&c.pos[0], &c.pos[1], &c.pos[2], &c.vel[0], &c.vel[1], &c.vel[2], &c.angVel[0],

@@ -7,6 +7,7 @@
 // Each particle walks its adjacency list, which holds its slots in increasing
 // order of the other particle's user id: the order of Particle::contacts (a
 // std::map keyed by the other id), so sums are reproducible bit for bit.
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 
@@ -115,37 +116,46 @@ __device__ __forceinline__ void gather_all(const GatherArgs& g, int64_t i, d3 po
   for (uint32_t k = k0; k < k1; k++) gather_one<PERIODIC, F32>(g, g.adj[k], pos, F, T);
 }
 
-// Contact forces on the boundary bodies (storage [0, nW)), which may touch thousands of
-// spheres: one workgroup per body, each thread summing every 256th entry of the body's list
-// in order, then a fixed-order tree in LDS, added to the value in the force arrays (which
-// always hold the current force: the reset value, or the gathered one). The spheres'
-// gathers (k_gather, k_leapfrog) skip these bodies. Deterministic; the summation order
-// differs from the sequential one, by rounding.
+// Contact forces on the boundary bodies other than facets (storage [0, nW), listed in
+// gbody), which may touch up to every sphere (a floor under a spread pile): each body's list is
+// cut into `parts` equal parts (set at each refresh from the largest such list), one workgroup
+// each, whose threads sum every 256th entry of their part in order, then a fixed-order tree;
+// k_gather_bnd_sum adds the parts in order to the value in the force arrays (which always hold
+// the current force: the reset value, or the gathered one). The spheres' gathers (k_gather,
+// k_leapfrog) skip these bodies. Deterministic; the summation order differs from the
+// sequential one, by rounding.
+constexpr int BND_PARTS_MAX = 64;
 template <bool PERIODIC, bool F32>
-__global__ __launch_bounds__(256) void k_gather_bnd(GatherArgs g, double* fx, double* fy, double* fz, double* tx,
-                                                     double* ty, double* tz, const double* __restrict__ px,
+__global__ __launch_bounds__(256) void k_gather_bnd(GatherArgs g, const uint32_t* __restrict__ gbody, int parts,
+                                                     double* part, const double* __restrict__ px,
                                                      const double* __restrict__ py, const double* __restrict__ pz) {
-  __shared__ double sh[6][256];
   if (g.gstop && *g.gstop) return;
-  const int64_t i = blockIdx.x;
+  const int64_t b = blockIdx.x / parts;
+  const int p = (int)(blockIdx.x % parts);
+  const int64_t i = gbody[b];
   const uint32_t k0 = g.adjOff[i], k1 = g.adjOff[i + 1];
-  if (k0 == k1 || flag_shape(g.flags[i]) == WDG_FACET) return;
+  const uint32_t chunk = (k1 - k0 + parts - 1) / parts;
+  const uint32_t a0 = k0 + (uint32_t)p * chunk, a1 = a0 + chunk < k1 ? a0 + chunk : k1;
   const d3 pos = mk3(px[i], py[i], pz[i]);
   d3 F = mk3(0., 0., 0.), T = mk3(0., 0., 0.);
-  for (uint32_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) gather_one<PERIODIC, F32>(g, g.adj[k], pos, F, T);
-  const int t = threadIdx.x;
-  sh[0][t] = F.x; sh[1][t] = F.y; sh[2][t] = F.z;
-  sh[3][t] = T.x; sh[4][t] = T.y; sh[5][t] = T.z;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (t < st)
-      for (int q = 0; q < 6; q++) sh[q][t] += sh[q][t + st];
-    __syncthreads();
-  }
-  if (t == 0) {
-    fx[i] = fx[i] + sh[0][0]; fy[i] = fy[i] + sh[1][0]; fz[i] = fz[i] + sh[2][0];
-    tx[i] = tx[i] + sh[3][0]; ty[i] = ty[i] + sh[4][0]; tz[i] = tz[i] + sh[5][0];
-  }
+  for (uint32_t k = a0 + threadIdx.x; k < a1; k += blockDim.x) gather_one<PERIODIC, F32>(g, g.adj[k], pos, F, T);
+  double v[6] = {F.x, F.y, F.z, T.x, T.y, T.z};
+  block_sum_store<6, 256>(v, part + ((int64_t)b * parts + p) * 6);
+}
+__global__ void k_gather_bnd_sum(int64_t nb, const uint32_t* __restrict__ gbody, int parts,
+                                 const uint32_t* __restrict__ adjOff, const uint32_t* gstop,
+                                 const double* __restrict__ part, double* fx, double* fy, double* fz, double* tx,
+                                 double* ty, double* tz) {
+  if (gstop && *gstop) return;
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int64_t i = gbody[b];
+  if (adjOff[i] == adjOff[i + 1]) return;
+  double v[6] = {0., 0., 0., 0., 0., 0.};
+  for (int p = 0; p < parts; p++)
+    for (int q = 0; q < 6; q++) v[q] += part[((int64_t)b * parts + p) * 6 + q];
+  fx[i] = fx[i] + v[0]; fy[i] = fy[i] + v[1]; fz[i] = fz[i] + v[2];
+  tx[i] = tx[i] + v[3]; ty[i] = ty[i] + v[4]; tz[i] = tz[i] + v[5];
 }
 
 template <bool PERIODIC, bool F32>
@@ -454,11 +464,17 @@ static GatherArgs gather_args(Ctx& c) {
 }
 
 static void gather_boundary(Ctx& c, const GatherArgs& g) {
-  if (c.nW == 0) return;
+  const int64_t nb = c.nGBody;
+  if (nb == 0) return;
+  const int parts = std::min(std::max(c.bndParts, 1), BND_PARTS_MAX);
   auto* kern = c.precision == 32 ? k_gather_bnd<false, true>
                                  : (c.field.periodic ? k_gather_bnd<true, false> : k_gather_bnd<false, false>);
-  hipLaunchKernelGGL(kern, dim3((unsigned)c.nW), dim3(256), 0, c.stream, g, c.force[0].p, c.force[1].p, c.force[2].p,
-                     c.torque[0].p, c.torque[1].p, c.torque[2].p, c.pos[0].p, c.pos[1].p, c.pos[2].p);
+  hipLaunchKernelGGL(kern, dim3((unsigned)(nb * parts)), dim3(256), 0, c.stream, g, (const uint32_t*)c.gbody.p, parts,
+                     c.bndPart.p, (const double*)c.pos[0].p, (const double*)c.pos[1].p, (const double*)c.pos[2].p);
+  WDG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_gather_bnd_sum, dim3(nblocks(nb, 64)), dim3(64), 0, c.stream, nb, (const uint32_t*)c.gbody.p,
+                     parts, g.adjOff, g.gstop, (const double*)c.bndPart.p, c.force[0].p, c.force[1].p, c.force[2].p,
+                     c.torque[0].p, c.torque[1].p, c.torque[2].p);
   WDG_LAUNCH_CHECK();
 }
 

@@ -237,6 +237,14 @@ struct Ctx {
   // per boundary body: force on its 3 nodes (9), then torque (9), from the last IntraForce
   // (facets only)
   DBuf<double> fnode;
+  // boundary bodies whose contact forces are gathered (walls: storage indices, facets left
+  // out), the parts each one's list is cut into (from the largest list at the last refresh),
+  // and their partial sums (up to 64 parts of 6 doubles per body)
+  DBuf<uint32_t> gbody;
+  std::vector<uint32_t> gbodyHost;
+  int64_t nGBody = 0;
+  int bndParts = 1;
+  DBuf<double> bndPart;
   std::vector<uint32_t> bodyUid;  // user id of each boundary body
   DBuf<double> bnd;     // enlarged AABB of each boundary body at the last refresh (min3, max3)
   double rmin = INFINITY, rmax = 0;
@@ -332,6 +340,8 @@ struct Ctx {
     double org[3], h[3], L[3];
     int nc[3];
     int periodic;
+    int hashed;       // cell keys are hashed into hmask + 1 buckets (sparse aperiodic scenes)
+    uint32_t hmask;
     bool brute;
     bool valid = false;
   } grid;
