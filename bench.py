@@ -590,11 +590,13 @@ def batched_form(args):
 
 def surrogate_form(args):
     """Learned coarse step on the granular column collapse: the network is trained on the GPU on
-    trajectories of seeded narrow columns (DEM, this engine; graphs and targets stay on the
-    device), then advances the 200,000-sphere column by coarse steps of 10 DEM steps with a DEM
-    re-simulation check every 50 coarse steps (fallback to DEM when the position error exceeds
-    0.1 r_min); against plain DEM over the same simulated time: throughput, run-out and height
-    profile."""
+    DEM trajectories (this engine) of seeded tall narrow columns and of squat columns of the
+    collapse's aspect ratio (graphs and targets stay on the device); then plain DEM runs the
+    200,000-sphere column until the pile is at rest, and the learned step runs the same simulated
+    time by coarse steps of 10 DEM steps, with a DEM re-simulation check every 50 coarse steps
+    (fallback to DEM when the position error exceeds 0.1 r_min). Reported: throughput of both,
+    fallbacks, and the deposit (edge, 99 % radius, height profile) of each against the initial
+    column."""
     import woo_amd.surrogate as S
     from woo_amd._native import Device
 
@@ -603,57 +605,78 @@ def surrogate_form(args):
     hist, ball, n_train, n_graphs = S.train_on_columns(net, n_columns=args.sur_columns, epochs=args.sur_epochs,
                                                        rounds=args.sur_rounds)
     t_train = time.perf_counter() - t0
-    held, _, _ = S.column_training_set(net, 10000, n_columns=32, snapshots=20, every=1500, seed=5000)
+    held, _, _ = S.column_training_set(net, 20000, n_columns=32, snapshots=20, every=1500, seed=5000)
     err_trained, err_ballistic = S.held_out_rmse(net, held)
     dev = Device(0)
     arr, rad = S.collapse_arrays()
     dt = S.dem_dt()
-    coarse = args.sur_steps // S.COARSE
     center = (0.2706 / 2, 0.2706 / 2)
-    # plain DEM: one step (collider, contacts), then coarse * COARSE timed steps
+    prof0, edge0, p99_0 = S.deposit(arr["pos"][1:], rad, center)
+    # plain DEM: one step (collider, contacts), then until the pile is at rest
     S.upload(dev, arr)
     step = S.dem_run(dev, 1, 0, dt)
     dev.sync()
     t0 = time.perf_counter()
-    S.dem_run(dev, coarse * S.COARSE, step, dt)
+    step_rest, ek_hist = S.collapse_to_rest(dev, dt, step, max_steps=args.sur_max_steps, log=log)
     dev.sync()
     t_dem = time.perf_counter() - t0
     p_dem = dev.get("pos")[1:]
-    # learned: the same first step, then coarse steps
+    ek_dem = dev.kinetic_energy()
+    coarse = (step_rest - 1) // S.COARSE
+    # learned: the same first step, then the same simulated time in coarse steps
     S.upload(dev, arr)
     step = S.dem_run(dev, 1, 0, dt)
     dev.sync()
+    # guards beside the position check: the total mechanical energy may not rise over a segment
+    # by more than 2e-4 of the initial one, and a fallback makes the next segments plain DEM
+    energy = lambda d: S.mechanical_energy(d, arr["mass"])
+    slack = 2e-4 * energy(dev)
     t0 = time.perf_counter()
-    st = S.run_learned(dev, net, rad, coarse, dt, step=step)
+    st = S.run_learned(dev, net, rad, coarse, dt, step=step, energy=energy, slack=slack, backoff=True)
     dev.sync()
     t_sur = time.perf_counter() - t0
     p_sur = dev.get("pos")[1:]
+    ek_sur = dev.kinetic_energy()
     dev.close()
     net.close()
-    ro_dem, pr_dem = S.runout_profile(p_dem, rad, center)
-    ro_sur, pr_sur = S.runout_profile(p_sur, rad, center)
-    # the deposit's front without its few farthest spheres: 99.9th percentile of the distance
-    front = lambda p: float(np.percentile(np.hypot(p[:, 0] - center[0], p[:, 1] - center[1]) + rad, 99.9))
-    fr_dem, fr_sur = front(p_dem), front(p_sur)
+    prof_d, edge_d, p99_d = S.deposit(p_dem, rad, center)
+    prof_s, edge_s, p99_s = S.deposit(p_sur, rad, center)
     n = len(rad)
     v_dem = n * (coarse * S.COARSE) / t_dem
     v_sur = n * (coarse * S.COARSE) / t_sur
-    occupied = pr_dem > 0
+    pile = (prof_d > 0) | (prof_s > 0)
+    pile[int(max(edge_d, edge_s) / 0.005) + 1:] = False
+    run_d, run_s = edge_d - edge0, edge_s - edge0
     return dict(value=v_sur, unit="particle-steps/s (DEM-step equivalents)", particles=n,
                 simulated_time_s=(1 + coarse * S.COARSE) * dt, dem_steps=1 + coarse * S.COARSE,
+                rest=dict(rule="kinetic energy below 5 % of its peak and falling by less than 5 % over 40,000 "
+                               "steps (spheres rolling away on the floor keep some)",
+                          kinetic_energy_peak_J=max(e for _, e in ek_hist), kinetic_energy_dem_J=ek_dem,
+                          kinetic_energy_learned_J=ek_sur),
                 plain_dem=dict(value=v_dem, unit="particle-steps/s", seconds=t_dem),
                 speedup_vs_plain_dem=v_sur / v_dem, seconds=t_sur, checks=st["checks"], fallbacks=st["fallbacks"],
+                energy_fallbacks=st["energy_fallbacks"], dem_segments_after_fallbacks=st["dem_segments"],
+                guards=dict(position_bound_m=0.2e-3, energy_slack_J=slack,
+                            rule="a segment falls back to DEM when the check's position error exceeds 0.1 r_min or "
+                                 "the total mechanical energy rose by more than the slack; after a fallback the "
+                                 "next 1, 2, 4, ... 64 segments are plain DEM"),
+                learned_fraction_of_steps=st["learned_coarse_steps"] * S.COARSE / (coarse * S.COARSE),
                 learned_coarse_steps=st["learned_coarse_steps"], dem_steps_in_checks_and_fallbacks=st["dem_steps"],
                 check_error_mm=dict(max=1e3 * max(st["errors"]), mean=1e3 * float(np.mean(st["errors"]))),
-                runout_m=dict(dem=ro_dem, learned=ro_sur, rel_err=abs(ro_sur - ro_dem) / ro_dem,
-                              definition="largest horizontal distance of a sphere's edge from the column axis"),
-                front_p999_m=dict(dem=fr_dem, learned=fr_sur, rel_err=abs(fr_sur - fr_dem) / fr_dem),
-                profile=dict(rms_diff_m=float(np.sqrt(np.mean((pr_sur - pr_dem) ** 2))),
-                             max_rel_err=float(np.max(np.abs(pr_sur - pr_dem)[occupied] / pr_dem[occupied]))),
-                position_diff_max_mm=1e3 * float(np.max(np.linalg.norm(p_sur - p_dem, axis=1))),
-                training=dict(columns=args.sur_columns, spheres=n_train, graphs=n_graphs, epochs=args.sur_epochs,
-                              rollout_rounds=args.sur_rounds, loss_first=hist[0], loss_last=hist[-1],
-                              data_and_train_s=t_train, heldout_rmse_learned=err_trained,
+                deposit=dict(
+                    definition="edge: outer radius of the contiguous pile at least 8 r_max high, from the column "
+                               "axis, rings of 5 mm; run-out: edge at rest minus the initial column's edge",
+                    initial_edge_m=edge0, edge_m=dict(dem=edge_d, learned=edge_s),
+                    runout_m=dict(dem=run_d, learned=run_s,
+                                  rel_err=abs(run_s - run_d) / run_d if run_d > 0 else None),
+                    radius_99pct_m=dict(initial=p99_0, dem=p99_d, learned=p99_s,
+                                        rel_err_of_spread=abs((p99_s - p99_0) - (p99_d - p99_0)) / (p99_d - p99_0)),
+                    height_m=dict(dem=float(prof_d[0]), learned=float(prof_s[0]), initial=float(prof0[0])),
+                    profile_rms_diff_m=float(np.sqrt(np.mean((prof_s[pile] - prof_d[pile]) ** 2))),
+                    profile_max_abs_diff_m=float(np.max(np.abs(prof_s[pile] - prof_d[pile])))),
+                training=dict(columns=args.sur_columns, squat_columns=S.SQUAT[0], spheres=n_train, graphs=n_graphs,
+                              epochs=args.sur_epochs, rollout_rounds=args.sur_rounds, loss_first=hist[0],
+                              loss_last=hist[-1], data_and_train_s=t_train, heldout_rmse_learned=err_trained,
                               heldout_rmse_ballistic=err_ballistic, unit="m/s^2"))
 
 
@@ -718,7 +741,8 @@ def main():
     ap.add_argument("--sur-columns", type=int, default=500)
     ap.add_argument("--sur-epochs", type=int, default=20)
     ap.add_argument("--sur-rounds", type=int, default=5, help="rounds of training on the network's own rollouts")
-    ap.add_argument("--sur-steps", type=int, default=20000, help="DEM steps of simulated time")
+    ap.add_argument("--sur-max-steps", type=int, default=500000,
+                    help="DEM steps at most while waiting for the collapsed pile to come to rest")
     ap.add_argument("--pmc-probe", type=int, default=0,
                     help="1: for rocprofv3 --pmc runs, only the kernel-timing launches (at their fixed state) "
                          "and a device-to-device hipMemcpy of 512 MiB whose FETCH_SIZE/WRITE_SIZE calibrate "

@@ -504,14 +504,26 @@ def held_out_rmse(net, slots):
     return math.sqrt(trained) * A0, math.sqrt(ballistic) * A0
 
 
+# squat columns of the collapse's aspect ratio (height / width = 0.6 / 0.2706), a fifth of its
+# size: (columns, width, height, snapshots, steps between snapshots)
+SQUAT = (64, 0.054, 0.12, 20, 2000)
+
+
 def train_on_columns(net, n_columns=500, epochs=30, rounds=1, rollout=50, snapshots=20, every=1500, seed=100,
-                     log=None):
-    """Train the network on DEM trajectories of seeded columns (column_training_set), then for
-    each of ``rounds`` rounds add the states its own coarse steps reach from the same snapshots
-    (rollout) with their DEM targets, and train on the union. Returns the loss history and the
-    ballistic loss of the DEM states."""
+                     squat=SQUAT, log=None):
+    """Train the network on DEM trajectories of seeded columns (column_training_set): tall narrow
+    ones (3 cm x 60 cm, which reach the collapse's speeds) and, with ``squat``, columns of the
+    collapse's own aspect ratio (whose spreading front and settling pile the tall ones do not
+    show); then for each of ``rounds`` rounds add the states its own coarse steps reach from
+    the tall columns' snapshots (rollout) with their DEM targets, and train on the union.
+    Returns the loss history, the ballistic loss of the DEM states, the spheres and the graphs."""
     slots, ball, n = column_training_set(net, 0, n_columns=n_columns, snapshots=snapshots, every=every, seed=seed,
                                          log=log)
+    if squat:
+        nc, w, h, sn, ev = squat
+        more, b2, n2 = column_training_set(net, 10000, n_columns=nc, width=w, height=h, snapshots=sn, every=ev,
+                                           seed=seed + 50000, log=log)
+        slots, ball, n = slots + more, ball + b2, n + n2
     hist = net.train_kept(slots, epochs=epochs, lr=1e-3, lr_end=1e-4, log=log)
     for r in range(rounds):
         more, _, _ = column_training_set(net, 100 * (r + 1), n_columns=n_columns, snapshots=snapshots, every=every,
@@ -543,22 +555,41 @@ def snapshot(dev):
     return dev.get("pos"), dev.get("vel"), dev.get("angVel"), dev.get("ori")
 
 
-def run_learned(dev, net, radius, coarse_steps, dt, check_every=50, bound=0.2e-3, step=0, log=None):
+def run_learned(dev, net, radius, coarse_steps, dt, check_every=50, bound=0.2e-3, step=0, energy=None, slack=0.,
+                backoff=False, log=None):
     """Coarse steps of COARSE DEM steps by the network, on the context dev (its spheres have the
     given radii). Every check_every coarse steps the last coarse step is redone by DEM from its
     start; when the largest position difference exceeds ``bound``, the segment since the
     previous check is redone by DEM (fallback). The state after a check is the DEM one. The
-    states are kept as device snapshots (slots 0-2). Returns statistics."""
+    states are kept as device snapshots (slots 0-2).
+
+    The check sees one coarse step only, so errors that build up over many segments escape it.
+    Two optional guards: with ``energy`` (a function of dev giving the total mechanical energy,
+    kinetic plus gravitational), a segment whose total energy rose by more than ``slack`` also
+    falls back, since contacts only dissipate; with ``backoff``, a fallback makes the next
+    segment plain DEM, the next two after a second fallback in a row, and so on up to 64, the
+    count starting again at one after a segment that passed. Returns statistics."""
     # room for motion between two slot-list refreshes of the graph builder
     skin = 0.25 * float(np.min(radius))
-    stats = dict(checks=0, fallbacks=0, errors=[], learned_coarse_steps=0, dem_steps=0)
+    stats = dict(checks=0, fallbacks=0, energy_fallbacks=0, dem_segments=0, errors=[], learned_coarse_steps=0,
+                 dem_steps=0)
     k = 0
+    dem_left, wait = 0, 1
     while k < coarse_steps:
         seg = min(check_every, coarse_steps - k)
         # the scene's step number at the segment's start: each coarse step stands for COARSE
         # DEM steps, so the check's DEM redo starts at step0 + (seg - 1) COARSE and a fallback
         # at step0 (the trsfRenorm phase of dem_run follows the true step)
         step0 = step
+        if dem_left > 0:
+            dem_run(dev, seg * COARSE, step0, dt)
+            stats["dem_steps"] += seg * COARSE
+            stats["dem_segments"] += 1
+            dem_left -= 1
+            step = step0 + seg * COARSE
+            k += seg
+            continue
+        e0 = energy(dev) if energy else 0.
         dev.snapshot(0)
         for j in range(seg):
             if j == seg - 1:
@@ -574,18 +605,69 @@ def run_learned(dev, net, radius, coarse_steps, dt, check_every=50, bound=0.2e-3
         err = dev.snapshot_distance(2)
         stats["checks"] += 1
         stats["errors"].append(err)
-        if err > bound:
+        fail = err > bound
+        if not fail and energy and energy(dev) > e0 + slack:
+            fail = True
+            stats["energy_fallbacks"] += 1
+        if fail:
             stats["fallbacks"] += 1
             dev.restore(0)
             dem_run(dev, seg * COARSE, step0, dt)
             stats["dem_steps"] += seg * COARSE
+            if backoff:
+                dem_left, wait = wait, min(2 * wait, 64)
+        else:
+            wait = 1
         step = step0 + seg * COARSE
         if log:
             log(f"coarse {k + seg}/{coarse_steps}: check error {err * 1e3:.4f} mm"
-                f"{' -> DEM fallback' if err > bound else ''}")
+                f"{' -> DEM fallback' if fail else ''}")
         k += seg
     stats["step"] = step
     return stats
+
+
+def mechanical_energy(dev, mass, g=G):
+    """kinetic energy of the context's integrated nodes plus the gravitational energy m g z of
+    the spheres (mass per particle, zero for walls)"""
+    return dev.kinetic_energy() + g * float(np.dot(mass, dev.get("pos")[:, 2]))
+
+
+def collapse_to_rest(dev, dt, step=0, chunk=10000, max_steps=500000, log=None):
+    """DEM until the collapsed pile is at rest: the kinetic energy, sampled every ``chunk``
+    steps, is below 5 % of its peak and fell by less than 5 % over the last 4 samples. Spheres
+    without rolling resistance keep rolling on the floor, so the energy levels off above zero
+    instead of vanishing. Returns the step reached and the energy history."""
+    hist = [(step, dev.kinetic_energy())]
+    peak = hist[0][1]
+    while step < max_steps:
+        step = dem_run(dev, chunk, step, dt)
+        ek = dev.kinetic_energy()
+        peak = max(peak, ek)
+        hist.append((step, ek))
+        if log:
+            log(f"collapse: step {step} kinetic energy {ek:.4e} J (peak {peak:.4e})")
+        if len(hist) > 4 and ek < 0.05 * peak and hist[-5][1] - ek < 0.05 * ek:
+            break
+    return step, hist
+
+
+def deposit(pos, radius, center, ring=0.005, rmax=2.5):
+    """the pile's height profile (highest sphere top per ring of horizontal distance from the
+    column axis, rings of ``ring`` m), its edge (outer radius of the last ring of the
+    contiguous pile at least 4 sphere diameters high, so that spheres rolling away alone do not
+    count) and the radius holding 99 % of the spheres"""
+    rh = np.hypot(pos[:, 0] - center[0], pos[:, 1] - center[1])
+    top = pos[:, 2] + radius
+    nb = int(rmax / ring)
+    prof = np.zeros(nb)
+    k = np.clip((rh / ring).astype(np.int64), 0, nb - 1)
+    np.maximum.at(prof, k, top)
+    high = prof >= 8 * float(np.max(radius))
+    edge = 0
+    while edge < nb and high[edge]:
+        edge += 1
+    return prof, edge * ring, float(np.percentile(rh, 99))
 
 
 def runout_profile(pos, radius, center, bins=np.linspace(0., 0.6, 61)):
