@@ -518,16 +518,21 @@ def batch_upload(dev, scenes, r, rho, idx=None):
     return arr
 
 
-def bed_fraction(pos, scenes, idx, r):
-    """per scene: sphere volume over the box cross-section times the bed height (top sphere)"""
-    out, o = [], 0
+def bed_fraction(pos, vel, scenes, idx, r):
+    """per scene: the bed's solid fraction, the volume of the spheres whose centres lie below the
+    90th percentile of the centre heights over the box cross-section times that height plus r
+    (a few spheres still bouncing on top do not count); and the scene's largest sphere speed"""
+    phi, vmax, o = [], [], 0
     for i in idx:
         sc = scenes[i]
         p = pos[o:o + sc["n"]] - sc["offset"]
+        v = vel[o:o + sc["n"]]
         o += sc["n"] + 6
-        h = float(np.max(p[:, 2])) + r
-        out.append(sc["n"] * (4 / 3.) * math.pi * r ** 3 / (sc["box"] ** 2 * h))
-    return np.array(out)
+        h = float(np.percentile(p[:, 2], 90))
+        k = int(np.sum(p[:, 2] <= h))
+        phi.append(k * (4 / 3.) * math.pi * r ** 3 / (sc["box"] ** 2 * (h + r)))
+        vmax.append(float(np.max(np.linalg.norm(v, axis=1))))
+    return np.array(phi), np.array(vmax)
 
 
 def batched_form(args):
@@ -557,7 +562,7 @@ def batched_form(args):
     st1 = dev.stats()
     value = ntot * steps / el
     total = taken + steps
-    pos = dev.get("pos")
+    pos, vel = dev.get("pos"), dev.get("vel")
     dev.close()
     # serial sample: scenes spread over the sizes, each alone for the same steps
     order = np.argsort([sc["n"] for sc in scenes])
@@ -577,15 +582,28 @@ def batched_form(args):
         same = same and bool(np.array_equal(d.get("pos"), pos[offs[i]:offs[i + 1]]))
         d.close()
     serial = n_ser * total / t_ser
-    phi_b = bed_fraction(pos, scenes, range(len(scenes)), r)
+    phi_b, vmax = bed_fraction(pos, vel, scenes, range(len(scenes)), r)
+    settled = vmax < 0.05
+    t_ref = st1["refreshSeconds"] - st0["refreshSeconds"]
     return dict(value=value, unit="particle-steps/s", scenes=len(scenes), particles=ntot, steps=steps,
                 ms_per_step=el / steps * 1e3, tuning=tun, tuning_steps=taken, tuning_s=t_tune,
-                refreshes=st1["nRefresh"] - st0["nRefresh"], serial_plain_engine=dict(
+                refreshes=st1["nRefresh"] - st0["nRefresh"], refresh_seconds=t_ref, refresh_share=t_ref / el,
+                serial_plain_engine=dict(
                     value=serial, unit="particle-steps/s", scenes=len(sample), particles=n_ser,
                     note="each sampled scene alone on the plain engine loop (wdg_run), the same steps "
                          "(tuning steps included)"),
                 speedup_vs_serial=value / serial, sampled_scenes_identical=same,
-                bed_solid_fraction=dict(mean=float(phi_b.mean()), min=float(phi_b.min()), max=float(phi_b.max())))
+                per_scene=dict(
+                    definition="bed solid fraction: spheres whose centres lie below the 90th percentile of the "
+                               "centre heights, over the box cross-section times that height plus r; settled: "
+                               "largest sphere speed below 0.05 m/s. The sampled scenes run alone end in the same "
+                               "positions bit for bit, so their fractions are the same",
+                    solid_fraction=[round(float(x), 4) for x in phi_b],
+                    settled=int(np.sum(settled)),
+                    solid_fraction_settled=dict(mean=float(phi_b[settled].mean()) if settled.any() else None,
+                                                min=float(phi_b[settled].min()) if settled.any() else None,
+                                                max=float(phi_b[settled].max()) if settled.any() else None),
+                    sample_scenes=sample))
 
 
 def surrogate_form(args):
@@ -733,7 +751,7 @@ def main():
     ap.add_argument("--drum-n", type=int, default=4_000_000)
     ap.add_argument("--drum-steps", type=int, default=500)
     ap.add_argument("--batch", type=int, default=1, help="also time the batched 256-scene study")
-    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--batch-steps", type=int, default=10000)
     ap.add_argument("--batch-sample", type=int, default=8, help="scenes also run one at a time")
     ap.add_argument("--batch-graph", type=int, default=1,
                     help="0: step the batch with the plain engine loop (no tuning, no graph), e.g. under a tracer")

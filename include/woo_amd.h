@@ -155,6 +155,7 @@ typedef struct {
   double verletDist;     /* resolved absolute Verlet distance */
   int32_t brute;         /* last refresh used the all-pairs path */
   int32_t gridCells[3];  /* cell grid of the last refresh */
+  double refreshSeconds; /* host time spent in broadphase refreshes so far */
 } wdg_counters;
 
 /* one Scene::doOneStep engine list, for wdg_run */
@@ -275,11 +276,16 @@ int wdg_sync(wdg_ctx* ctx);
  * Only the engine list [Collider, ContactLoop, Leapfrog] is run in this mode (no DynDt). */
 int wdg_set_scenes(wdg_ctx* ctx, int32_t nScenes, const double* dt);
 /* tuning of the engine: workgroups of the force kernel per group of 2048 contact slots (also
- * used by wdg_run and wdg_contact_loop), steps per captured graph (even); {2, 8} until
- * wdg_tune or wdg_set_tuning changes it */
+ * used by wdg_run and wdg_contact_loop), steps per captured graph (even), and for batched
+ * scenes the per-scene Verlet skin: 0 = every scene uses the collider's Verlet distance;
+ * k > 0 = at each refresh each scene's distance is set to cover 1.25 times what its fastest
+ * sphere moves in k of its steps (its speed, plus gravity over that time), between the
+ * collider's Verlet distance and the smallest radius. The skin changes only which pairs are
+ * candidates, never a result. {2, 8, 0} until wdg_tune or wdg_set_tuning changes it */
 typedef struct {
   int32_t cfBlocks;
   int32_t graphSteps;
+  int32_t skinSteps;
 } wdg_tuning;
 /* wdg_run for the engine list [Collider, ContactLoop, Leapfrog(reset)] with the contact loop
  * and leapfrog of graphSteps steps captured once as a hipGraph and replayed; a step whose

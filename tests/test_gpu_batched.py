@@ -103,7 +103,7 @@ def test_autotuner_picks_a_candidate_and_keeps_results_exact():
     d1, _ = build_batch(scenes)
     t, step, taken = d1.tune(ENG, 0, sc["dt"], **kw)
     print("tuned", t, "steps taken", taken)
-    assert t["cfBlocks"] in (1, 2, 4) and t["graphSteps"] in (4, 8, 16)
+    assert t["cfBlocks"] in (1, 2, 4) and t["graphSteps"] in (4, 8, 16) and t["skinSteps"] in (0, 8, 16, 32)
     assert step == taken
     step, _ = d1.run(ENG, 100, step, sc["dt"], graph=True, **kw)
     d0, _ = build_batch(scenes)
@@ -132,16 +132,18 @@ def scenes_c4_sample(k=16, seed=71, nmin=500, nmax=5000):
     return out
 
 
-def test_c4_batch_against_oracle_per_scene():
+@pytest.mark.parametrize("skin", [0, 16])
+def test_c4_batch_against_oracle_per_scene(skin):
     """16 C4-shaped scenes (500-5000 spheres, own E, tanPhi and dt) stepped together, graph
     replayed, through several refreshes; each scene against the oracle running it alone:
     real-contact sets equal, positions within 1e-9 r (Scene::doOneStep per scene,
-    Scene.cpp:366-467)."""
+    Scene.cpp:366-467). With per-scene skins (skinSteps 16) the candidate pairs differ from
+    the oracle's, the real contacts and the trajectories do not."""
     from tests.scenes import build_oracle
 
     scenes = scenes_c4_sample()
     steps = 240
-    d, _ = batch_run(scenes, steps, graph=True, tuning=dict(cfBlocks=2, graphSteps=8))
+    d, _ = batch_run(scenes, steps, graph=True, tuning=dict(cfBlocks=2, graphSteps=8, skinSteps=skin))
     st = d.stats()
     print("batch: particles", st["nParticles"], "real", st["nReal"], "refreshes", st["nRefresh"], flush=True)
     assert st["nRefresh"] >= 3, "the window must hold at least two refreshes after the first"

@@ -61,7 +61,7 @@ class Particles(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("cfBlocks", C.c_int32), ("graphSteps", C.c_int32)]
+    _fields_ = [("cfBlocks", C.c_int32), ("graphSteps", C.c_int32), ("skinSteps", C.c_int32)]
 
 
 class CollideParams(C.Structure):
@@ -81,7 +81,7 @@ class LeapfrogParams(C.Structure):
 class Counters(C.Structure):
     _fields_ = [("nParticles", C.c_int64), ("nCandidates", C.c_int64), ("nReal", C.c_int64),
                 ("nRefresh", C.c_int64), ("nSteps", C.c_int64), ("verletDist", C.c_double),
-                ("brute", C.c_int32), ("gridCells", C.c_int32 * 3)]
+                ("brute", C.c_int32), ("gridCells", C.c_int32 * 3), ("refreshSeconds", C.c_double)]
 
 
 class Program(C.Structure):
@@ -449,15 +449,15 @@ class Device:
         t = Tuning()
         n = C.c_int64()
         check(self.L.wdg_tune(self.h, C.byref(prog), C.byref(st), C.byref(d), C.byref(t), C.byref(n)))
-        return dict(cfBlocks=t.cfBlocks, graphSteps=t.graphSteps), st.value, n.value
+        return dict(cfBlocks=t.cfBlocks, graphSteps=t.graphSteps, skinSteps=t.skinSteps), st.value, n.value
 
     def tuning(self):
         t = Tuning()
         check(self.L.wdg_get_tuning(self.h, C.byref(t)))
-        return dict(cfBlocks=t.cfBlocks, graphSteps=t.graphSteps)
+        return dict(cfBlocks=t.cfBlocks, graphSteps=t.graphSteps, skinSteps=t.skinSteps)
 
-    def set_tuning(self, cfBlocks, graphSteps):
-        t = Tuning(cfBlocks, graphSteps)
+    def set_tuning(self, cfBlocks, graphSteps, skinSteps=0):
+        t = Tuning(cfBlocks, graphSteps, skinSteps)
         check(self.L.wdg_set_tuning(self.h, C.byref(t)))
 
     def relax(self, maxIter=2000, inflate=0.01, tol=0.):

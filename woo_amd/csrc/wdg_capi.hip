@@ -911,10 +911,12 @@ int wdg_get_tuning(wdg_ctx* ctx, wdg_tuning* out) {
 
 int wdg_set_tuning(wdg_ctx* ctx, const wdg_tuning* t) {
   if (!ctx || !t || !(t->cfBlocks == 1 || t->cfBlocks == 2 || t->cfBlocks == 4 || t->cfBlocks == 8) ||
-      t->graphSteps < 2 || (t->graphSteps & 1) || t->graphSteps > 256) {
-    g_err = "tuning: cfBlocks in {1,2,4,8}, graphSteps even in [2,256]";
+      t->graphSteps < 2 || (t->graphSteps & 1) || t->graphSteps > 256 || t->skinSteps < 0 || t->skinSteps > 1024) {
+    g_err = "tuning: cfBlocks in {1,2,4,8}, graphSteps even in [2,256], skinSteps in [0,1024]";
     return WDG_E_ARG;
   }
+  // a new skin takes effect at a refresh: the next collider call makes one
+  if (t->skinSteps != ctx->c.tune.skinSteps) ctx->c.boundsValid = false;
   ctx->c.tune = *t;
   return WDG_OK;
 }
@@ -972,6 +974,7 @@ int wdg_stats(wdg_ctx* ctx, wdg_counters* out) {
     out->verletDist = c.verletDist;
     out->brute = c.grid.brute;
     for (int k = 0; k < 3; k++) out->gridCells[k] = c.grid.brute ? 0 : c.grid.nc[k];
+    out->refreshSeconds = c.refreshSec;
   });
 }
 

@@ -185,6 +185,7 @@ struct LeapArgs {
   int hasGravity;
   int gather;
   double maxD2;
+  const double* vdS;  // per-scene Verlet distances (else null: maxD2)
   int checkDirty;
   int clean;  // force/torque of integrated particles hold the reset value
   uint32_t* dirty;      // set when a particle left its Verlet skin
@@ -313,7 +314,12 @@ __global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
         d3 d = sub3(mk3(a.lx[i], a.ly[i], a.lz[i]), pos);
         double d2 = 0;
         d2 = fmax(d2, sqn3(d));
-        dirty = d2 + 0. > a.maxD2;
+        double m2 = a.maxD2;
+        if (a.vdS) {
+          const double v = a.vdS[a.scene[i]];
+          m2 = v * v;
+        }
+        dirty = d2 + 0. > m2;
       }
       // leapfrogSphericalRotate (Leapfrog.cpp:280-297)
       q4 ori = q4{a.qw[i], a.qx[i], a.qy[i], a.qz[i]};
@@ -522,6 +528,7 @@ void leapfrog(Ctx& c, const wdg_leapfrog_params* p) {
   a.gather = (c.forcesPending && c.con().n > 0) ? 1 : 0;
   a.checkDirty = c.boundsValid && c.verletDist > 0 ? 1 : 0;
   a.maxD2 = c.verletDist * c.verletDist;
+  a.vdS = c.vdSceneActive ? c.vdScene.p : nullptr;
   a.clean = c.forceClean ? 1 : 0;
   a.dtScene = c.nScenes > 0 ? c.dtScene.p : nullptr;
   a.scene = c.scene.p;

@@ -257,10 +257,18 @@ struct Ctx {
   std::vector<uint32_t> bodyScene;        // scene of each boundary body (host)
   DBuf<uint32_t> sceneBodyOff, sceneBody;  // boundary bodies by scene (batched scenes)
   DBuf<double> dtScene;
+  // per-scene Verlet distances of the last refresh (tuning.skinSteps > 0), and scratch for the
+  // scenes' largest speeds
+  DBuf<double> vdScene;
+  DBuf<unsigned long long> sceneVmax;
+  bool vdSceneActive = false;
+  double refreshSec = 0;  // host time spent in refreshes so far
+  int pairStage = 24;     // pairs staged per sphere by the refresh's pair pass
+  int nbStage = 48;       // neighbours staged per sphere by it
   // graph runner: device words [0] stop, [1] steps executed, [2] renorm flag of the step,
   // [3..4] first step number (int64), [5] trsfRenorm
   DBuf<uint32_t> gstate;
-  wdg_tuning tune{2, 8};  // until wdg_tune / wdg_set_tuning: 2 force workgroups per group of slots
+  wdg_tuning tune{2, 8, 0};  // until wdg_tune / wdg_set_tuning: 2 force workgroups per group of slots
   int64_t gen = 0;  // bumped whenever buffers a captured graph uses may change
   bool graphCapture = false;  // kernels being launched are captured for replay
   struct GraphCache {
