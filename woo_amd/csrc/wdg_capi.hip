@@ -523,6 +523,8 @@ int wdg_ctx_destroy(wdg_ctx* ctx) {
     c.adjEnt.release(); c.adjDeg.release(); c.stNb.release(); c.sdeg.release(); c.posf.release();
     c.fgeo.release(); c.bnd.release(); c.pairMax.release(); c.scene.release(); c.dtScene.release();
     c.sceneBodyOff.release(); c.sceneBody.release();
+    c.fnode.release(); c.gbody.release(); c.bndPart.release(); c.vdScene.release(); c.sceneVmax.release();
+    c.sceneBox.release(); c.sgrid.release();
     c.gstate.release();
     if (c.graph.exec) (void)hipGraphExecDestroy(c.graph.exec);
     c.graph.exec = nullptr; c.bigOff.release(); c.bigRank.release(); c.bigVal.release();

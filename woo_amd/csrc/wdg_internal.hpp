@@ -262,6 +262,8 @@ struct Ctx {
   DBuf<double> vdScene;
   DBuf<unsigned long long> sceneVmax;
   bool vdSceneActive = false;
+  DBuf<unsigned long long> sceneBox;  // per scene: bounds (ordered bits) and sphere count
+  DBuf<unsigned char> sgrid;          // per scene: its cell grid (batched aperiodic scenes)
   double refreshSec = 0;  // host time spent in refreshes so far
   int pairStage = 24;     // pairs staged per sphere by the refresh's pair pass
   int nbStage = 48;       // neighbours staged per sphere by it
@@ -350,6 +352,7 @@ struct Ctx {
     int periodic;
     int hashed;       // cell keys are hashed into hmask + 1 buckets (sparse aperiodic scenes)
     uint32_t hmask;
+    const void* sg;   // batched aperiodic scenes: the grid of every scene (in sgrid), else null
     bool brute;
     bool valid = false;
   } grid;
