@@ -735,6 +735,30 @@ def pmc_probe(wl):
     dev.close()
 
 
+def history_probe(wl):
+    """The contact-history layout against a device hash keyed by the particle ids: the real
+    contacts of the fixed kernel-timing state, their 11 history doubles read and 10 written per
+    contact, found by index in the engine's packed layout or by linear probing in a hash of load
+    factor 1/2 (woo_amd/probe/history_probe.hip); HIP events, 20 launches each."""
+    dev, _ = fixed_state(wl)
+    ids = dev.contacts(real_only=True, data=False)[0]
+    dev.close()
+    keys = np.ascontiguousarray((ids[:, 0].astype(np.uint64) << np.uint64(32)) | ids[:, 1].astype(np.uint64))
+    L = C.CDLL(os.path.join(ROOT, "woo_amd", "libwdg_probe.so"))
+    L.hp_run.argtypes = [C.POINTER(C.c_uint64), C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_double),
+                         C.POINTER(C.c_uint32)]
+    out = dict(history_probe="contact history: packed stream vs hash", contacts=int(len(keys)))
+    for mode, name in ((0, "packed_stream"), (1, "hash")):
+        ms, miss = C.c_double(), C.c_uint32()
+        rc = L.hp_run(keys.ctypes.data_as(C.POINTER(C.c_uint64)), len(keys), mode, 20, C.byref(ms), C.byref(miss))
+        if rc != 0:
+            raise RuntimeError("history probe failed")
+        us = ms.value * 1e3
+        out[name] = dict(us_per_launch=us, state_GBs=len(keys) * (88 + 80) / (us * 1e-6) / 1e9, missed=miss.value)
+    out["hash_over_stream"] = out["hash"]["us_per_launch"] / out["packed_stream"]["us_per_launch"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -761,6 +785,9 @@ def main():
     ap.add_argument("--sur-rounds", type=int, default=5, help="rounds of training on the network's own rollouts")
     ap.add_argument("--sur-max-steps", type=int, default=500000,
                     help="DEM steps at most while waiting for the collapsed pile to come to rest")
+    ap.add_argument("--history-probe", type=int, default=0,
+                    help="1: only the contact-history layout probe (packed stream against a hash); prints its "
+                         "line, no benchmark line")
     ap.add_argument("--pmc-probe", type=int, default=0,
                     help="1: for rocprofv3 --pmc runs, only the kernel-timing launches (at their fixed state) "
                          "and a device-to-device hipMemcpy of 512 MiB whose FETCH_SIZE/WRITE_SIZE calibrate "
@@ -781,6 +808,10 @@ def main():
     if args.pmc_probe:
         dev.close()
         pmc_probe(wl)
+        return
+    if args.history_probe:
+        dev.close()
+        history_probe(wl)
         return
     log(f"dt={dt:.4e} s, warmup {args.warmup} steps")
     step, _ = dev.run(ENGINES, args.warmup, step, dt, contact=cp, leapfrog=dict(damping=0.), dynDtPeriod=0)

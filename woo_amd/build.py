@@ -83,6 +83,24 @@ def build_host(force=False, verbose=True):
     return out
 
 
+PROBE_SRC = os.path.join(HERE, "probe", "history_probe.hip")
+PROBE_LIB = os.path.join(HERE, "libwdg_probe.so")
+
+
+def build_probe(force=False, verbose=True):
+    """The history-access probe bench.py --history-probe uses (a measurement, not the engine)."""
+    if not force and os.path.exists(PROBE_LIB) and os.path.getmtime(PROBE_LIB) >= os.path.getmtime(PROBE_SRC):
+        return PROBE_LIB
+    cmd = [HIPCC] + FLAGS + ["-shared", PROBE_SRC, "-o", PROBE_LIB + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for the history probe:\n{r.stderr[-6000:]}")
+    os.replace(PROBE_LIB + ".tmp", PROBE_LIB)
+    if verbose:
+        print(f"built {PROBE_LIB}")
+    return PROBE_LIB
+
+
 def build(force=False, verbose=True):
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
@@ -96,6 +114,7 @@ def build(force=False, verbose=True):
     if verbose:
         print(f"built {LIB}")
     build_host(force=force, verbose=verbose)
+    build_probe(force=force, verbose=verbose)
     return LIB
 
 
