@@ -253,6 +253,44 @@ def observables(dev, wl):
                 solid_fraction=ob["solidFraction"], stress=dev.stress(-1.))
 
 
+def fp32_drift(wl, steps=5000, every=1000):
+    """The FP32 contact form against FP64 over the 1M workload's 5,000 steps (its configured
+    length): both from the initial state, graph-replayed side by side, each observable's
+    relative error at every `every` steps."""
+    from woo_amd._native import Device, SPHERE
+
+    n = len(wl["radius"])
+    cp = contact_params()
+    devs = {}
+    for prec in (64, 32):
+        d = Device(0)
+        d.set_field(periodic=True, cell=(wl["L"],) * 3)
+        d.set_materials([wl["mat"]])
+        d.upload_particles(shape=np.full(n, SPHERE, np.int32), pos=wl["pos"], radius=wl["radius"], mass=wl["mass"],
+                           inertia=wl["inertia"], vel=wl["vel"])
+        d.set_precision(prec)
+        devs[prec] = [d, 0]
+    out = []
+    for k in range(steps // every):
+        obs = {}
+        for prec, dv in devs.items():
+            dv[1], _ = dv[0].run(ENGINES, every, dv[1], wl["dt"], graph=True, contact=cp, leapfrog=dict(damping=0.),
+                                 dynDtPeriod=0)
+            obs[prec] = observables(dv[0], wl)
+        a, b = obs[64], obs[32]
+        row = dict(step=devs[64][1])
+        for q in ("kinetic_energy", "coord_number", "solid_fraction"):
+            row[q] = abs(b[q] - a[q]) / abs(a[q])
+        p64, p32 = np.trace(a["stress"]) / 3, np.trace(b["stress"]) / 3
+        row["pressure"] = abs(p32 - p64) / abs(p64)
+        out.append(row)
+    for dv in devs.values():
+        dv[0].close()
+    worst = {q: max(r[q] for r in out) for q in ("kinetic_energy", "coord_number", "solid_fraction", "pressure")}
+    return dict(steps=steps, rel_err_by_step=out, worst_rel_err=worst, bar=0.01,
+                within_bar=bool(all(v <= 0.01 for v in worst.values())))
+
+
 def fp32_form(wl, args, value64, obs64, tuning):
     """The FP32 contact form on the same initial state and step count: throughput and its
     speedup over FP64, and each observable's relative error against the FP64 run (which
@@ -290,10 +328,11 @@ def fp32_form(wl, args, value64, obs64, tuning):
     err["pressure"] = dict(fp64=p64, fp32=p32, rel_err=abs(p32 - p64) / abs(p64))
     err["stress_max_abs_diff_over_max"] = float(np.max(np.abs(o32["stress"] - obs64["stress"])) /
                                                 np.max(np.abs(obs64["stress"])))
+    drift = fp32_drift(wl) if args.fp32_drift else None
     return dict(value=v32, unit="particle-steps/s", ms_per_step=el / args.steps * 1e3,
                 speedup_vs_fp64=v32 / value64, steps=args.steps, warmup=args.warmup,
                 dtype="f32 contact geometry/law, f64 positions and sums",
-                observable_errors_vs_fp64=err)
+                observable_errors_vs_fp64=err, drift_vs_fp64=drift)
 
 
 def measured_traffic(kernel, slots, real):
@@ -551,6 +590,7 @@ def batched_form(args):
     if args.batch_graph:
         tun, step, taken = dev.tune(ENGINES, 0, scenes[0]["dt"], **kw)
     else:
+        dev.set_tuning(cfBlocks=2, graphSteps=8, skinSteps=args.batch_skin)
         tun, step, taken = dev.tuning(), 0, 0
     t_tune = time.perf_counter() - t0
     st0 = dev.stats()
@@ -771,6 +811,8 @@ def main():
                          "hipGraph, and the timed steps are graph replays (bit-identical to the plain loop); "
                          "0: the plain engine loop")
     ap.add_argument("--fp32", type=int, default=1, help="also time the FP32 contact form")
+    ap.add_argument("--fp32-drift", type=int, default=1,
+                    help="also run FP32 and FP64 side by side for the workload's 5,000 steps, observables compared")
     ap.add_argument("--drum", type=int, default=1, help="also time the rotating-drum workload (facets)")
     ap.add_argument("--drum-n", type=int, default=4_000_000)
     ap.add_argument("--drum-steps", type=int, default=500)
@@ -779,6 +821,8 @@ def main():
     ap.add_argument("--batch-sample", type=int, default=8, help="scenes also run one at a time")
     ap.add_argument("--batch-graph", type=int, default=1,
                     help="0: step the batch with the plain engine loop (no tuning, no graph), e.g. under a tracer")
+    ap.add_argument("--batch-skin", type=int, default=16,
+                    help="per-scene skin (steps) when the batch is not tuned (--batch-graph 0)")
     ap.add_argument("--surrogate", type=int, default=1, help="also run the learned coarse step (column collapse)")
     ap.add_argument("--sur-columns", type=int, default=500)
     ap.add_argument("--sur-epochs", type=int, default=20)
