@@ -181,6 +181,16 @@ int wdg_download(wdg_ctx* ctx, int32_t field, double* host);
 
 int wdg_collide(wdg_ctx* ctx, const wdg_collide_params* p);
 int wdg_contact_loop(wdg_ctx* ctx, const wdg_contact_params* p);
+/* ContactLoop.updatePhys (src/dem/ContactLoop.hpp:117, ContactLoop.cpp:180 and 265-266):
+ * WDG_UPDATE_PHYS_NEVER keeps each contact's phys from the materials it was created with (a
+ * material table changed by wdg_set_materials applies to new contacts only), ALWAYS makes
+ * every contact loop take the current materials, ONCE does so in the next contact loop and
+ * then reads NEVER. A checkpoint's restored contacts take the current materials. */
+#define WDG_UPDATE_PHYS_NEVER 0
+#define WDG_UPDATE_PHYS_ALWAYS 1
+#define WDG_UPDATE_PHYS_ONCE 2
+int wdg_set_update_phys(wdg_ctx* ctx, int32_t mode);
+int wdg_get_update_phys(wdg_ctx* ctx, int32_t* mode);
 int wdg_leapfrog(wdg_ctx* ctx, const wdg_leapfrog_params* p);
 int wdg_force_reset(wdg_ctx* ctx);
 /* IntraForce::run (src/dem/IntraForce.cpp:26-42) with In2_Facet (src/dem/Facet.cpp:301-328,

@@ -1003,8 +1003,9 @@ struct orc_scene {
         else if (par[C.pA].shape == ORC_FACET) geomOk = geomFacetSphere(C, shift2);
         else geomOk = geomSphereSphere(C, shift2);
         if (!geomOk) continue;
-        if (!C.phys) {
-          C.stepCreated = step;
+        // ContactLoop.cpp:179-180: phys at creation, and on existing contacts by updatePhys
+        if (!C.phys) C.stepCreated = step;
+        if (!C.phys || prm.updatePhys > 0) {
           if (prm.physKind == ORC_PHYS_HERTZ) physHertz(C);
           else physFrict(C);
         }
@@ -1032,6 +1033,8 @@ struct orc_scene {
     }
     for (auto& m : errMsg)
       if (!m.empty()) throw std::runtime_error(m);
+    // ContactLoop.cpp:265-266
+    if (prm.updatePhys == 2) prm.updatePhys = 0;
     for (auto& v : pendT)
       for (auto& pr : v) pending.push_back(pr);
     if (prm.trackEnergy) {
@@ -1275,6 +1278,18 @@ int orc_add_material(orc_scene* s, int kind, double density, double young, doubl
                      double surfEnergy, double alpha) {
   s->mats.push_back(Material{kind, density, young, ktDivKn, tanPhi, surfEnergy, alpha});
   return (int)s->mats.size() - 1;
+}
+
+int orc_set_material(orc_scene* s, int m, int kind, double density, double young, double ktDivKn, double tanPhi,
+                     double surfEnergy, double alpha) {
+  if (m < 0 || m >= (int)s->mats.size()) return -1;
+  s->mats[m] = Material{kind, density, young, ktDivKn, tanPhi, surfEnergy, alpha};
+  return 0;
+}
+
+int orc_get_params(const orc_scene* s, orc_params* out) {
+  *out = s->prm;
+  return 0;
 }
 
 // woo.utils.sphere (utils.py:153) + Sphere::lumpMassInertia (Sphere.cpp:26-33)

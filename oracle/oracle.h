@@ -82,6 +82,8 @@ typedef struct {
   double maxRelInc;
   /* Scene.trackEnergy */
   int trackEnergy;
+  /* ContactLoop.updatePhys (ContactLoop.hpp:119): 0 never, 1 always, 2 once (then never) */
+  int updatePhys;
 } orc_params;
 
 /* Scene.energy (EnergyTracker, src/core/EnergyTracker.hpp:30-61): the terms the path's engines
@@ -98,6 +100,11 @@ int orc_set_params(orc_scene* s, const orc_params* p);
 
 int orc_add_material(orc_scene* s, int kind, double density, double young, double ktDivKn, double tanPhi,
                      double surfEnergy, double alpha);
+/* material m edited in place (the Material object's attributes assigned from Python) */
+int orc_set_material(orc_scene* s, int m, int kind, double density, double young, double ktDivKn, double tanPhi,
+                     double surfEnergy, double alpha);
+/* the scene's parameters as they stand (ContactLoop.updatePhys 'once' turns into 'never') */
+int orc_get_params(const orc_scene* s, orc_params* out);
 /* returns particle id */
 int orc_add_sphere(orc_scene* s, const double pos[3], double radius, int mat, int fixed, uint32_t mask);
 int orc_add_wall(orc_scene* s, const double pos[3], int axis, int sense, int mat, uint32_t mask);

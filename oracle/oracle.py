@@ -26,6 +26,7 @@ class OrcParams(C.Structure):
         ("noSlip", C.c_int), ("noBreak", C.c_int), ("noFrict", C.c_int), ("noAttraction", C.c_int),
         ("damping", C.c_double), ("reset", C.c_int),
         ("dynDtPeriod", C.c_int), ("maxRelInc", C.c_double), ("trackEnergy", C.c_int),
+        ("updatePhys", C.c_int),
     ]
 
 # Scene.energy terms, in the order orc_energy and wdg_energy give them
@@ -74,6 +75,8 @@ def _load(path=None):
     L.orc_destroy.argtypes = [P]
     L.orc_set_params.argtypes = [P, C.POINTER(OrcParams)]
     L.orc_add_material.argtypes = [P, C.c_int] + [C.c_double] * 6; L.orc_add_material.restype = C.c_int
+    L.orc_set_material.argtypes = [P, C.c_int, C.c_int] + [C.c_double] * 6; L.orc_set_material.restype = C.c_int
+    L.orc_get_params.argtypes = [P, C.POINTER(OrcParams)]; L.orc_get_params.restype = C.c_int
     L.orc_add_sphere.argtypes = [P, D, C.c_double, C.c_int, C.c_int, C.c_uint32]; L.orc_add_sphere.restype = C.c_int
     L.orc_add_wall.argtypes = [P, D, C.c_int, C.c_int, C.c_int, C.c_uint32]; L.orc_add_wall.restype = C.c_int
     L.orc_add_facet.argtypes = [P, D, D, C.c_double, C.c_int, C.c_uint32]; L.orc_add_facet.restype = C.c_int
@@ -124,6 +127,9 @@ class Oracle:
         self.L = L
 
     def set_params(self, _apply=True, **kw):
+        if _apply:
+            # from the scene's own values (DynDt changes dt, 'once' turns into 'never')
+            self.L.orc_get_params(self.h, C.byref(self.p))
         for k, v in kw.items():
             if k in ("gravity", "cell"):
                 getattr(self.p, k)[:] = list(v)
@@ -140,6 +146,25 @@ class Oracle:
 
     def material(self, kind=0, density=1e3, young=1e7, ktDivKn=.2, tanPhi=.5, surfEnergy=0., alpha=0.):
         return self.L.orc_add_material(self.h, kind, density, young, ktDivKn, tanPhi, surfEnergy, alpha)
+
+    def set_material(self, m, kind=0, density=1e3, young=1e7, ktDivKn=.2, tanPhi=.5, surfEnergy=0., alpha=0.):
+        """Material m edited in place (existing contacts keep their phys unless updatePhys)."""
+        if self.L.orc_set_material(self.h, m, kind, density, young, ktDivKn, tanPhi, surfEnergy, alpha) != 0:
+            raise IndexError(f"no material {m}")
+
+    UPDATE_PHYS = ("never", "always", "once")
+
+    @property
+    def update_phys(self):
+        """ContactLoop.updatePhys: 'never', 'always' or 'once' ('once' reads 'never' after a loop)."""
+        p = OrcParams()
+        self.L.orc_get_params(self.h, C.byref(p))
+        self.p.updatePhys = p.updatePhys
+        return self.UPDATE_PHYS[p.updatePhys]
+
+    @update_phys.setter
+    def update_phys(self, mode):
+        self.set_params(updatePhys=self.UPDATE_PHYS.index(mode) if isinstance(mode, str) else int(mode))
 
     def sphere(self, pos, radius, mat=0, fixed=False, mask=0b0101):
         a = np.ascontiguousarray(pos, dtype=np.float64)

@@ -322,3 +322,35 @@ def test_background_run_matches_synchronous_and_stops():
     with pytest.raises(RuntimeError, match="stop here"):
         S.wait()
     assert not S.running
+
+
+@pytest.mark.parametrize("up", ["never", "always", "once"])
+def test_contact_loop_update_phys(up):
+    # the reference's case (py/tests/test_demfield.py:24-50) through the scripting surface: the
+    # contact handle taken after the first step reads the current phys at every later step
+    import woo_amd.woo as woo
+    from woo_amd.woo.dem import DemField, FrictMat, Sphere
+
+    E0, E1 = 1e3, 1e6
+    m = FrictMat(young=E0)
+    S = woo.core.Scene(fields=[DemField(par=[Sphere.make((0, 0, 0), .6, mat=m), Sphere.make((0, 0, 1), .6, mat=m)])],
+                       engines=DemField.minimalEngines(), dtSafety=1e-8, dt=1e-8)
+    S.one()
+    c = S.dem.con[0]
+    kn0 = c.phys.kn
+    S.lab.contactLoop.updatePhys = up
+    m.young = E1
+    S.one()
+    kn1 = c.phys.kn
+    if up in ("always", "once"):
+        assert kn1 == pytest.approx(kn0 * (E1 / E0), rel=1e-12)
+    else:
+        assert kn0 == c.phys.kn
+    m.young = E0
+    S.one()
+    if up == "always":
+        assert S.lab.contactLoop.updatePhys == "always"
+        assert kn0 == pytest.approx(c.phys.kn, rel=1e-12)
+    else:
+        assert S.lab.contactLoop.updatePhys == "never"
+        assert kn1 == c.phys.kn

@@ -207,3 +207,34 @@ def test_energy_tracker_surface_without_device(woo):
         S.energy['gravV'] = 1.
     S.energy.clear()
     assert len(S.energy) == 0
+
+
+def test_contact_loop_update_phys_attribute(woo):
+    # ContactLoop.updatePhys is a named enum (ContactLoop.hpp:119): read as its name, assigned by
+    # name or number; dist00 and reorderEvery are accepted (ContactLoop.hpp:121,123); a copy keeps
+    # them (Scene.saveTmp / loadTmp)
+    from woo_amd.woo.dem import ContactLoop, DemField, FrictMat, Sphere
+
+    m = FrictMat(young=1e3)
+    S = woo.core.Scene(fields=[DemField(par=[Sphere.make((0, 0, 0), .6, mat=m), Sphere.make((0, 0, 1), .6, mat=m)])],
+                       engines=DemField.minimalEngines(), dtSafety=1e-8, dt=1e-8)
+    assert len(S.dem.par) == 2 and S.dem.par[1].pos[2] == 1
+    cl = S.lab.contactLoop
+    assert isinstance(cl, ContactLoop) and cl.updatePhys == "never"
+    for v in ("always", "once", "never"):
+        cl.updatePhys = v
+        assert cl.updatePhys == v
+    cl.updatePhys = 1
+    assert cl.updatePhys == "always"
+    with pytest.raises(ValueError):
+        cl.updatePhys = "sometimes"
+    with pytest.raises(ValueError):
+        cl.updatePhys = 3
+    assert cl.dist00 is True and cl.reorderEvery == 1000
+    cl.reorderEvery = 50
+    S.saveTmp("up", quiet=True)
+    S2 = woo.core.Scene.loadTmp("up")
+    assert S2.lab.contactLoop.updatePhys == "always" and S2.lab.contactLoop.reorderEvery == 50
+    # DemField(par=...) fills an empty field only
+    with pytest.raises(ValueError):
+        S.dem.par = [Sphere.make((0, 0, 5), .1, mat=m)]

@@ -91,6 +91,43 @@ def test_linear_contact_period_pins_kn():
         assert o.get("vel")[1, 2] == pytest.approx(0.1, rel=1e-6)
 
 
+def _kn_of_contact(o):
+    ids, _, data = o.contacts(real_only=True)
+    assert len(ids) == 1
+    return float(data[0, 13])
+
+
+@pytest.mark.parametrize("up", ["never", "always", "once"])
+def test_update_phys_never_always_once(up):
+    # ContactLoop.updatePhys, the reference's case (py/tests/test_demfield.py:24-50): two
+    # overlapping spheres, FrictMat young E0 -> E1 -> E0 between single steps; FrictPhys.kn is
+    # proportional to young (FrictMat.cpp:37)
+    E0, E1 = 1e3, 1e6
+    mat = dict(kind=0, density=1e3, young=E0, ktDivKn=.2, tanPhi=.5)
+    o = Oracle(dt=1e-8, damping=0., gravity=(0, 0, 0))
+    m = o.material(**mat)
+    o.sphere((0, 0, 0), .6, m)
+    o.sphere((0, 0, 1), .6, m)
+    o.run(ENG, 1)
+    kn0 = _kn_of_contact(o)
+    o.update_phys = up
+    o.set_material(m, **dict(mat, young=E1))
+    o.run(ENG, 1)
+    kn1 = _kn_of_contact(o)
+    if up in ("always", "once"):
+        assert kn1 == pytest.approx(kn0 * E1 / E0, rel=1e-12)
+    else:
+        assert kn1 == kn0
+    o.set_material(m, **mat)
+    o.run(ENG, 1)
+    if up == "always":
+        assert o.update_phys == "always"
+        assert _kn_of_contact(o) == pytest.approx(kn0, rel=1e-12)
+    else:
+        assert o.update_phys == "never"
+        assert _kn_of_contact(o) == kn1
+
+
 def test_sliding_sphere_rolls_at_five_sevenths():
     # a sphere sliding on a wall ends up rolling at 5/7 of its initial speed (solid sphere),
     # independent of the friction coefficient; checks IdealElPl slip, torque branch and rotation

@@ -30,7 +30,8 @@ EXPORTS = [
     "wdg_mpnn_graph_from_ctx", "wdg_mpnn_advance", "wdg_snapshot", "wdg_restore", "wdg_snapshot_distance",
     "wdg_mpnn_set_targets", "wdg_mpnn_keep_graph", "wdg_mpnn_use_graph", "wdg_mpnn_get_graph",
     "wdg_set_energy_tracking", "wdg_energy", "wdg_energy_clear", "wdg_restore_contacts", "wdg_energy_set",
-    "wdg_kinetic_energy_noalloc", "wdg_intra_force", "wdg_facet_nodes",
+    "wdg_kinetic_energy_noalloc", "wdg_intra_force", "wdg_facet_nodes", "wdg_set_update_phys",
+    "wdg_get_update_phys",
 ]
 
 # Scene.energy terms, in WDG_EN_* order
@@ -120,6 +121,8 @@ def lib():
     L.wdg_leapfrog.argtypes = [V, _P(LeapfrogParams)]
     L.wdg_force_reset.argtypes = [V]
     L.wdg_intra_force.argtypes = [V]
+    L.wdg_set_update_phys.argtypes = [V, C.c_int32]
+    L.wdg_get_update_phys.argtypes = [V, _P(C.c_int32)]
     L.wdg_facet_nodes.argtypes = [V, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_double)]
     L.wdg_crit_dt.argtypes = [V, _P(C.c_double)]
     L.wdg_contact_count.argtypes = [V, C.c_int32, _P(C.c_int64)]
@@ -455,6 +458,19 @@ class Device:
         t = Tuning()
         check(self.L.wdg_get_tuning(self.h, C.byref(t)))
         return dict(cfBlocks=t.cfBlocks, graphSteps=t.graphSteps, skinSteps=t.skinSteps)
+
+    UPDATE_PHYS = ("never", "always", "once")
+
+    @property
+    def update_phys(self):
+        """ContactLoop.updatePhys (ContactLoop.hpp:119): 'never', 'always' or 'once'."""
+        m = C.c_int32()
+        check(self.L.wdg_get_update_phys(self.h, C.byref(m)))
+        return self.UPDATE_PHYS[m.value]
+
+    @update_phys.setter
+    def update_phys(self, mode):
+        check(self.L.wdg_set_update_phys(self.h, self.UPDATE_PHYS.index(mode) if isinstance(mode, str) else int(mode)))
 
     def set_tuning(self, cfBlocks, graphSteps, skinSteps=0):
         t = Tuning(cfBlocks, graphSteps, skinSteps)

@@ -39,10 +39,13 @@ struct ExportArgs {
   const double *px, *py, *pz;
   const int32_t* mat;
   const wdg_material* mats;
+  const uint8_t* hE;  // each real contact's material epoch (null: one table)
+  int nMats;
   const double* fgeo;
   int hertz;
   double poisson, en, viscAlpha;
   double* out;
+  uint8_t* outE;  // the epochs per slot (snapshots), or null
 };
 
 template <int PREC>
@@ -63,7 +66,10 @@ __global__ void k_export(ExportArgs a) {
   contact_radii(inf, pA, pB, a.rad, a.fgeo, r1, r2);
   double l0, l1, A;
   lens_area(r1, r2, l0, l1, A);
-  const wdg_material mA = a.mats[a.mat[pA]], mB = a.mats[a.mat[pB]];
+  const uint32_t ep = a.hE ? (uint32_t)a.hE[r] : 0u;
+  if (a.outE) a.outE[s] = (uint8_t)ep;
+  const uint32_t eb = ep * (uint32_t)a.nMats;
+  const wdg_material mA = a.mats[eb + a.mat[pA]], mB = a.mats[eb + a.mat[pB]];
   PhysVals ph;
   double kn, kt;
   const double uN = ldr<PREC>(a.uN, r);
@@ -101,6 +107,8 @@ static ExportArgs export_args(Ctx& c, double* out) {
   a.Y0 = RS.hY[0].p; a.Y1 = RS.hY[1].p; a.Y2 = RS.hY[2].p;
   a.F0 = RS.hF[0].p; a.F1 = RS.hF[1].p; a.F2 = RS.hF[2].p; a.uN = RS.huN.p; a.gp = RS.gp.p;
   a.rad = c.radius.p; a.mass = c.mass.p; a.mat = c.mat.p; a.mats = c.dMats.p; a.fgeo = c.fgeo.p;
+  a.nMats = (int)c.mats.size();
+  if (c.matEpochs > 1) a.hE = RS.hE.p;
   a.px = c.pos[0].p; a.py = c.pos[1].p; a.pz = c.pos[2].p;
   a.hertz = c.lastPhysKind == WDG_PHYS_HERTZ;
   a.poisson = c.lastPoisson; a.en = c.lastEn; a.viscAlpha = c.lastViscAlpha;
@@ -117,6 +125,10 @@ struct RestoreArgs {
   RealView rv;
   const uint64_t* dist;  // packed cellDist per slot (periodic cells; else null)
   double *N0, *N1, *N2, *Y0, *Y1, *Y2, *F0, *F1, *F2, *uN, *gp;
+  // material epochs (while more than one table is kept): each record's (else all take epoch)
+  uint8_t* hE;
+  const uint8_t* recE;
+  uint32_t epoch;
 };
 
 __global__ void k_restore(RestoreArgs a) {
@@ -126,6 +138,7 @@ __global__ void k_restore(RestoreArgs a) {
   if (q == 0u) return;
   const double* d = a.data + (int64_t)16 * (q - 1u);
   const uint32_t k = rv_index(a.rv, (uint32_t)s);
+  if (a.hE) a.hE[k] = a.recE ? a.recE[q - 1u] : (uint8_t)a.epoch;
   const d3 T0 = mk3(d[0], d[1], d[2]), T1 = mk3(d[3], d[4], d[5]);
   a.N0[k] = T0.x; a.N1[k] = T0.y; a.N2[k] = T0.z;
   a.Y0[k] = T1.x; a.Y1[k] = T1.y; a.Y2[k] = T1.z;
@@ -208,6 +221,8 @@ void restore_contacts(Ctx& c, int64_t n, const int32_t* ids, const int32_t* cell
   a.N0 = R.hN[0].p; a.N1 = R.hN[1].p; a.N2 = R.hN[2].p;
   a.Y0 = R.hY[0].p; a.Y1 = R.hY[1].p; a.Y2 = R.hY[2].p;
   a.F0 = R.hF[0].p; a.F1 = R.hF[1].p; a.F2 = R.hF[2].p; a.uN = R.huN.p; a.gp = R.gp.p;
+  // a checkpoint's contacts take the current materials
+  if (c.matEpochs > 1) { a.hE = R.hE.p; a.epoch = (uint32_t)(c.matEpochs - 1); }
   hipLaunchKernelGGL(k_restore, dim3(nblocks(nc, 256)), dim3(256), 0, c.stream, a);
   WDG_LAUNCH_CHECK();
   WDG_HIP(hipStreamSynchronize(c.stream));
@@ -282,7 +297,14 @@ void snapshot_contacts_save(Ctx& c, int slot) {
   WDG_LAUNCH_CHECK();
   radix_sort_pairs<uint64_t>(c, c.snapKey[slot].p, c.snapSlot[slot].p, c.bigKeyTmp.p, c.bigValTmp.p, nc,
                              32 + bits_for((uint64_t)c.n));
-  hipLaunchKernelGGL(k_export<64>, dim3(nblocks(nc, 256)), dim3(256), 0, c.stream, export_args(c, c.snapRec[slot].p));
+  ExportArgs ea = export_args(c, c.snapRec[slot].p);
+  c.snapMatGen[slot] = -1;
+  if (c.matEpochs > 1) {
+    c.snapEp[slot].ensure(nc);
+    ea.outE = c.snapEp[slot].p;
+    c.snapMatGen[slot] = c.matGen;
+  }
+  hipLaunchKernelGGL(k_export<64>, dim3(nblocks(nc, 256)), dim3(256), 0, c.stream, ea);
   WDG_LAUNCH_CHECK();
   c.snapReal[slot] = nReal;
   c.snapVerlet[slot] = c.verletDist;
@@ -315,6 +337,12 @@ void snapshot_contacts_load(Ctx& c, int slot) {
     a.N0 = R.hN[0].p; a.N1 = R.hN[1].p; a.N2 = R.hN[2].p;
     a.Y0 = R.hY[0].p; a.Y1 = R.hY[1].p; a.Y2 = R.hY[2].p;
     a.F0 = R.hF[0].p; a.F1 = R.hF[1].p; a.F2 = R.hF[2].p; a.uN = R.huN.p; a.gp = R.gp.p;
+    // the saved epochs while the tables are numbered as at the save, else the current one
+    if (c.matEpochs > 1) {
+      a.hE = R.hE.p;
+      a.epoch = (uint32_t)(c.matEpochs - 1);
+      if (c.snapMatGen[slot] == c.matGen) a.recE = c.snapEp[slot].p;
+    }
     hipLaunchKernelGGL(k_restore, dim3(nblocks(nc, 256)), dim3(256), 0, c.stream, a);
     WDG_LAUNCH_CHECK();
   }
@@ -532,6 +560,7 @@ int wdg_ctx_destroy(wdg_ctx* ctx) {
     for (auto& b : c.snap) b.release();
     for (int k = 0; k < Ctx::NSNAP; k++) {
       c.snapKey[k].release(); c.snapCd[k].release(); c.snapSlot[k].release(); c.snapRec[k].release();
+      c.snapEp[k].release();
     }
     c.snapMatch.release();
     c.sortHist.release(); c.scanAux.release(); c.red.release(); c.xfer.release(); c.dflags.release();
@@ -571,15 +600,7 @@ int wdg_set_field(wdg_ctx* ctx, const wdg_field* f) {
 int wdg_set_materials(wdg_ctx* ctx, const wdg_material* mats, int32_t n) {
   if (!ctx || (n > 0 && !mats) || n < 0) { g_err = "bad materials"; return WDG_E_ARG; }
   return guard(ctx, [&] {
-    Ctx& c = ctx->c;
-    // kernels still queued on the engine's stream may read the table
-    WDG_HIP(hipStreamSynchronize(c.stream));
-    c.mats.assign(mats, mats + n);
-    const wdg_material* before = c.dMats.p;
-    c.dMats.ensure((size_t)std::max(n, 1));
-    // a captured step graph holds the table's address
-    if (c.dMats.p != before) c.gen++;
-    if (n) WDG_HIP(hipMemcpy(c.dMats.p, mats, n * sizeof(wdg_material), hipMemcpyHostToDevice));
+    set_materials(ctx->c, mats, n);
   });
 }
 
@@ -908,6 +929,21 @@ int wdg_tune(wdg_ctx* ctx, const wdg_program* prog, int64_t* step, double* dt, w
 int wdg_get_tuning(wdg_ctx* ctx, wdg_tuning* out) {
   if (!ctx || !out) { g_err = "NULL argument"; return WDG_E_ARG; }
   *out = ctx->c.tune;
+  return WDG_OK;
+}
+
+int wdg_set_update_phys(wdg_ctx* ctx, int32_t mode) {
+  if (!ctx || mode < WDG_UPDATE_PHYS_NEVER || mode > WDG_UPDATE_PHYS_ONCE) {
+    g_err = "updatePhys: 0 (never), 1 (always) or 2 (once)";
+    return WDG_E_ARG;
+  }
+  ctx->c.updatePhys = mode;
+  return WDG_OK;
+}
+
+int wdg_get_update_phys(wdg_ctx* ctx, int32_t* mode) {
+  if (!ctx || !mode) { g_err = "null argument"; return WDG_E_ARG; }
+  *mode = ctx->c.updatePhys;
   return WDG_OK;
 }
 

@@ -43,6 +43,12 @@ struct ContactArgs {
   const int32_t* mat;
   const wdg_material* mats;
   int nMats;
+  // material epochs (more than one table kept): each real contact's epoch in the old and new
+  // state, the current epoch, and whether every contact takes the current one (updatePhys)
+  const uint8_t* oE;
+  uint8_t* nE;
+  uint32_t epoch;
+  int updAll;
   const double* fgeo;  // facet geometry per boundary body
   // batched scenes: dt of each particle's scene (else null, dt above)
   const double* dtScene;
@@ -285,7 +291,8 @@ constexpr int CF_BS = 256;
 // materials staged in LDS by the force kernels (a larger table is read from HBM beyond this)
 constexpr int MAT_LDS = 32;
 __device__ __forceinline__ void stage_materials(const ContactArgs& a, wdg_material* smat) {
-  const int nm = a.nMats < MAT_LDS ? a.nMats : MAT_LDS;
+  const int nt = a.nMats * (int)(a.epoch + 1);
+  const int nm = nt < MAT_LDS ? nt : MAT_LDS;
   for (int t = threadIdx.x; t < nm; t += blockDim.x) smat[t] = a.mats[t];
   __syncthreads();
 }
@@ -445,7 +452,14 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
     // Cp2 (recomputed; constant over the life of the contact)
     double l0, l1, A;
     lens_area(r1, r2, l0, l1, A);
-    const wdg_material mA = material(a, smat, matA), mB = material(a, smat, matB);
+    // the contact's material epoch: the one it was created in, or the current one
+    uint32_t eb = 0u;
+    if (a.nE) {
+      const uint32_t ep = (real && !a.updAll) ? (uint32_t)a.oE[ko] : a.epoch;
+      a.nE[k] = (uint8_t)ep;
+      eb = ep * (uint32_t)a.nMats;
+    }
+    const wdg_material mA = material(a, smat, eb + matA), mB = material(a, smat, eb + matB);
     double Fn, Ft0 = real ? oFt0 : 0., Ft1 = real ? oFt1 : 0.;
     if (LAW == WDG_LAW_IDEALELPL) {
       // Law2_L6Geom_FrictPhys_IdealElPl::go (IdealElPl.cpp:15-125); breaking decided by k_cfilter
@@ -763,7 +777,13 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
     // Cp2 parameters in FP64 (functions of radii, materials and masses), the law in float
     double l0, l1, A;
     lens_area((double)r1, (double)r2, l0, l1, A);
-    const wdg_material mA = a.mats[a.mat[pA]], mB = a.mats[a.mat[pB]];
+    uint32_t eb = 0u;
+    if (a.nE) {
+      const uint32_t ep = (real && !a.updAll) ? (uint32_t)a.oE[ko] : a.epoch;
+      a.nE[k] = (uint8_t)ep;
+      eb = ep * (uint32_t)a.nMats;
+    }
+    const wdg_material mA = a.mats[eb + a.mat[pA]], mB = a.mats[eb + a.mat[pB]];
     float Fn, Ft0 = real ? oF1[ko] : 0.f, Ft1 = real ? oF2[ko] : 0.f;
     if (LAW == WDG_LAW_IDEALELPL) {
       PhysVals ph;
@@ -873,6 +893,12 @@ static ContactArgs contact_args(Ctx& c, int from, int to) {
   a.wx = c.angVel[0].p; a.wy = c.angVel[1].p; a.wz = c.angVel[2].p;
   a.rad = c.radius.p; a.mass = c.mass.p; a.flags = c.flags.p; a.mat = c.mat.p; a.mats = c.dMats.p;
   a.nMats = (int)c.mats.size();
+  a.epoch = (uint32_t)(c.matEpochs - 1);
+  a.updAll = c.updatePhys != 0;
+  if (c.matEpochs > 1) {
+    a.oE = O.hE.p;
+    a.nE = N.hE.p;
+  }
   a.fgeo = c.fgeo.p;
   a.dtScene = c.nScenes > 0 ? c.dtScene.p : nullptr;
   a.scene = c.scene.p;
@@ -890,6 +916,96 @@ static ContactArgs contact_args(Ctx& c, int from, int to) {
 __global__ void k_fill_u32(int64_t n, uint32_t* p, uint32_t v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------ material epochs
+// which epochs the real contacts refer to (256 bits); a workgroup per group of slots, whose
+// real contacts hold the state indices state_index(g, 0 .. gcnt[g]-1)
+__global__ void k_epoch_used(int64_t ng, const uint32_t* __restrict__ gcnt, const uint8_t* __restrict__ hE,
+                             uint32_t* used) {
+  __shared__ uint32_t sm[8];
+  if (threadIdx.x < 8) sm[threadIdx.x] = 0u;
+  __syncthreads();
+  for (int64_t g = blockIdx.x; g < ng; g += gridDim.x)
+    for (uint32_t loc = threadIdx.x; loc < gcnt[g]; loc += blockDim.x) {
+      const uint32_t e = hE[state_index((uint32_t)g, loc)];
+      atomicOr(&sm[e >> 5], 1u << (e & 31u));
+    }
+  __syncthreads();
+  if (threadIdx.x < 8 && sm[threadIdx.x]) atomicOr(&used[threadIdx.x], sm[threadIdx.x]);
+}
+struct EpochMap {
+  uint8_t m[256];
+};
+// the real contacts' epochs renumbered
+__global__ void k_epoch_map(int64_t ng, const uint32_t* __restrict__ gcnt, uint8_t* hE, EpochMap map) {
+  for (int64_t g = blockIdx.x; g < ng; g += gridDim.x)
+    for (uint32_t loc = threadIdx.x; loc < gcnt[g]; loc += blockDim.x) {
+      const uint32_t k = state_index((uint32_t)g, loc);
+      hE[k] = map.m[hE[k]];
+    }
+}
+
+// ContactLoop.updatePhys (ContactLoop.hpp:117, ContactLoop.cpp:180): a contact keeps the phys
+// its creation computed unless the loop updates it. Phys here are recomputed from the materials
+// at every step, so 'never' keeps, for each real contact, the material table it was created
+// with: a changed table while contacts exist becomes a new epoch after the tables (of the same
+// length) the real contacts still refer to, and the contact loop reads each contact's epoch.
+static bool same_material(const wdg_material& a, const wdg_material& b) {
+  return a.kind == b.kind && a.density == b.density && a.young == b.young && a.ktDivKn == b.ktDivKn &&
+         a.tanPhi == b.tanPhi && a.surfEnergy == b.surfEnergy && a.alpha == b.alpha;
+}
+
+void set_materials(Ctx& c, const wdg_material* mats, int n) {
+  const int nOld = (int)c.mats.size();
+  bool same = n == nOld && n > 0;
+  for (int m = 0; same && m < n; m++) same = same_material(c.mats[m], mats[m]);
+  if (same) return;
+  const int64_t nReal = (c.n > 0 && c.con().n > 0) ? count_real(c) : 0;
+  std::vector<wdg_material> tab;
+  int epochs = 1;
+  if (nReal > 0 && nOld > 0 && n >= nOld) {
+    Reals& R = c.reals();
+    const int64_t ng = ngroups(c.con().n);
+    const unsigned nb = (unsigned)std::min<int64_t>(ng, 1024);
+    // the state arrays span every group's range (state_index)
+    if (c.matEpochs == 1) WDG_HIP(hipMemsetAsync(R.hE.p, 0, (size_t)ng * GROUP, c.stream));
+    // the epochs still referred to, renumbered in order; the new table is the last one
+    DBuf<uint32_t> used;
+    used.ensure(8);
+    WDG_HIP(hipMemsetAsync(used.p, 0, 8 * sizeof(uint32_t), c.stream));
+    hipLaunchKernelGGL(k_epoch_used, dim3(nb), dim3(256), 0, c.stream, ng, (const uint32_t*)R.gcnt.p,
+                       (const uint8_t*)R.hE.p, used.p);
+    WDG_LAUNCH_CHECK();
+    uint32_t hu[8];
+    WDG_HIP(hipMemcpyAsync(hu, used.p, sizeof(hu), hipMemcpyDeviceToHost, c.stream));
+    WDG_HIP(hipStreamSynchronize(c.stream));
+    used.release();
+    EpochMap map{};
+    int kept = 0;
+    for (int e = 0; e < c.matEpochs; e++)
+      if (hu[e >> 5] >> (e & 31) & 1u) {
+        if (kept == 255) throw Error(WDG_E_CAPACITY, "material tables: more than 255 epochs referred to by contacts");
+        map.m[e] = (uint8_t)kept++;
+        // an older table extended by the materials added since (no contact of it refers to them)
+        for (int m = 0; m < n; m++) tab.push_back(m < nOld ? c.epTab[(size_t)e * nOld + m] : mats[m]);
+      }
+    hipLaunchKernelGGL(k_epoch_map, dim3(nb), dim3(256), 0, c.stream, ng, (const uint32_t*)R.gcnt.p, R.hE.p, map);
+    WDG_LAUNCH_CHECK();
+    epochs = kept + 1;
+  }
+  tab.insert(tab.end(), mats, mats + n);
+  // kernels still queued on the engine's stream may read the table
+  WDG_HIP(hipStreamSynchronize(c.stream));
+  c.mats.assign(mats, mats + n);
+  c.epTab = tab;
+  const wdg_material* before = c.dMats.p;
+  c.dMats.ensure(std::max<size_t>(tab.size(), 1));
+  if (!tab.empty()) WDG_HIP(hipMemcpy(c.dMats.p, tab.data(), tab.size() * sizeof(wdg_material), hipMemcpyHostToDevice));
+  // a captured step graph holds the table's address and the epoch arguments
+  if (c.dMats.p != before || epochs != c.matEpochs) c.gen++;
+  c.matEpochs = epochs;
+  c.matGen++;
 }
 
 void eidx_clear(Ctx& c) {
@@ -1014,6 +1130,8 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
   c.eidxClean = false;
   c.rcur = to;
   c.forcesPending = p->applyForces != 0;
+  // ContactLoop::run's epilogue (ContactLoop.cpp:265-266): 'once' updates one step only
+  if (c.updatePhys == 2 && !c.graphCapture) c.updatePhys = 0;
 }
 
 // ------------------------------------------------------------------ real set from per-slot flags

@@ -135,6 +135,8 @@ static void capture(Ctx& c, const wdg_program* prog, int K, int64_t step, double
   G.damping = prog->leapfrog.damping;
   G.cp = prog->contact;
   G.intra = has_intra(prog);
+  G.epochs = c.matEpochs;
+  G.updAll = c.updatePhys != 0;
 }
 
 static bool graph_valid(const Ctx& c, const wdg_program* prog, int K, double dt) {
@@ -142,7 +144,7 @@ static bool graph_valid(const Ctx& c, const wdg_program* prog, int K, double dt)
   const wdg_contact_params& a = G.cp;
   const wdg_contact_params& b = prog->contact;
   return G.exec && G.gen == c.gen && G.K == K && G.cfBlocks == c.tune.cfBlocks && G.track == c.trackEnergy &&
-         G.intra == has_intra(prog) &&
+         G.intra == has_intra(prog) && G.epochs == c.matEpochs && G.updAll == (c.updatePhys != 0) &&
          G.rcur == c.rcur &&
          G.parity == c.lfParity && G.dt == dt && G.damping == prog->leapfrog.damping && a.physKind == b.physKind &&
          a.lawKind == b.lawKind && a.applyForces == b.applyForces && a.trsfRenorm == b.trsfRenorm &&
@@ -211,7 +213,9 @@ static void run_graph_steps(Ctx& c, const wdg_program* prog, int64_t nSteps, int
     const int64_t left = nSteps - done;
     // (a periodic scene's step 0 is a plain one when tracking energy: its leapfrog adds no terms,
     // and a graph captured there would skip them at every replay)
-    if (!c.forceClean || c.forcesPending || left < K || c.n == 0 || (c.trackEnergy && c.field.periodic && *step == 0)) {
+    // (updatePhys 'once' is one plain step, after which the loop reads 'never')
+    if (!c.forceClean || c.forcesPending || left < K || c.n == 0 || c.updatePhys == 2 ||
+        (c.trackEnergy && c.field.periodic && *step == 0)) {
       plain_step(c, prog, *step, *dt);
       *step += 1;
       done += 1;

@@ -587,6 +587,8 @@ struct DynArgs {
   const uint32_t* flags;
   const int32_t* mat;
   const wdg_material* mats;
+  const uint8_t* hE;  // each real contact's material epoch (null: one table)
+  int nMats;
   const double* fgeo;
   int hertz;
   double poisson, en, viscAlpha;
@@ -606,7 +608,8 @@ __device__ __forceinline__ void dyn_one(const DynArgs& a, uint32_t e, d3& kt, d3
   contact_radii(inf, pA, pB, a.rad, a.fgeo, r1, r2);
   double l0, l1, A;
   lens_area(r1, r2, l0, l1, A);
-  const wdg_material mA = a.mats[a.mat[pA]], mB = a.mats[a.mat[pB]];
+  const uint32_t eb = a.hE ? (uint32_t)a.hE[r] * (uint32_t)a.nMats : 0u;
+  const wdg_material mA = a.mats[eb + a.mat[pA]], mB = a.mats[eb + a.mat[pB]];
   PhysVals ph;
   double kn, ktt;
   if (a.hertz) {
@@ -672,6 +675,8 @@ double crit_dt(Ctx& c) {
   a.Y0 = R.hY[0].p; a.Y1 = R.hY[1].p; a.Y2 = R.hY[2].p; a.uN = R.huN.p;
   a.rad = c.radius.p; a.mass = c.mass.p; a.ix = c.inertia[0].p; a.iy = c.inertia[1].p; a.iz = c.inertia[2].p;
   a.flags = c.flags.p; a.mat = c.mat.p; a.mats = c.dMats.p; a.fgeo = c.fgeo.p;
+  a.nMats = (int)c.mats.size();
+  if (c.matEpochs > 1) a.hE = R.hE.p;
   a.hertz = c.lastPhysKind == WDG_PHYS_HERTZ;
   a.poisson = c.lastPoisson; a.en = c.lastEn; a.viscAlpha = c.lastViscAlpha;
   unsigned long long* out = reinterpret_cast<unsigned long long*>(c.red.p);

@@ -161,6 +161,8 @@ struct Reals {
   DBuf<double> huN;               // L6Geom.uN
   // gather record per contact: global force on pA (3), contact point (3), cellDist bits, unused
   DBuf<double> gp;
+  // material table (epoch) each real contact takes its phys from, while more than one is kept
+  DBuf<uint8_t> hE;
   // state index of every real slot, written by the contact loop's filter (stale for the other
   // slots); valid only for a set that filter produced
   DBuf<uint32_t> sidx;
@@ -175,12 +177,13 @@ struct Reals {
     for (int k = 0; k < 3; k++) { hN[k].ensure(c); hY[k].ensure(c); hF[k].ensure(c); }
     huN.ensure(c);
     gp.ensure(GP * c);
+    hE.ensure(c);
     sidx.ensure(c);
   }
   void release() {
     bits.release(); tpre.release(); gcnt.release();
     for (int k = 0; k < 3; k++) { hN[k].release(); hY[k].release(); hF[k].release(); }
-    huN.release(); gp.release(); sidx.release();
+    huN.release(); gp.release(); sidx.release(); hE.release();
     sidxValid = false;
   }
 };
@@ -226,8 +229,15 @@ struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   wdg_field field{};
-  std::vector<wdg_material> mats;
+  std::vector<wdg_material> mats;  // the current material table
+  // ContactLoop.updatePhys 'never' keeps each contact's phys from the materials it was created
+  // with: a change of materials while contacts exist appends the new table (an epoch) after the
+  // old ones in dMats, and every real contact carries the epoch it takes its phys from
   DBuf<wdg_material> dMats;
+  std::vector<wdg_material> epTab;  // host copy of dMats: matEpochs tables of mats.size() each
+  int matEpochs = 1;
+  int64_t matGen = 0;  // counts the changes of the epochs' numbering
+  int updatePhys = 0;  // 0 never, 1 always, 2 once (then never)
 
   // particles, storage order
   int64_t n = 0;
@@ -279,6 +289,7 @@ struct Ctx {
     int K = 0, cfBlocks = 0, rcur = -1, parity = -1;
     bool track = false;  // captured with energy tracking
     bool intra = false;  // captured with IntraForce
+    int epochs = 1, updAll = 0;  // material epochs and updatePhys of the capture
     double dt = 0, damping = 0;
     wdg_contact_params cp{};
   } graph;
@@ -330,6 +341,9 @@ struct Ctx {
   DBuf<double> snapRec[NSNAP];
   int64_t snapReal[NSNAP] = {0, 0, 0, 0};
   double snapVerlet[NSNAP] = {NAN, NAN, NAN, NAN};
+  // each saved contact's material epoch, valid while matGen is the saved one
+  DBuf<uint8_t> snapEp[NSNAP];
+  int64_t snapMatGen[NSNAP] = {-1, -1, -1, -1};
   DBuf<uint32_t> snapMatch;  // per slot of a restored list: 1 + its slot in the snapshot, or 0
   uint64_t uploads = 0;   // particle uploads so far
   uint64_t serial = 0;    // unique per context of the process
@@ -414,6 +428,9 @@ void reals_from_flags(Ctx& c, const uint32_t* flag, int64_t nslots, int dst);
 void check_errors(Ctx& c);
 void relax(Ctx& c, int maxIter, double inflate, double tol, double* maxOverlap, int* iters);
 void drop_contacts(Ctx& c);
+// a new material table (ContactLoop.updatePhys: the old tables the real contacts still refer to
+// are kept as earlier epochs)
+void set_materials(Ctx& c, const wdg_material* mats, int n);
 void snapshot_save(Ctx& c, int slot);
 // the contact history part of a snapshot: kept on the device (FP64 form), and restored after a
 // full refresh at the snapshot's positions

@@ -129,6 +129,21 @@ void DemField::pull() {
   }
 }
 
+static wdg_material device_material(const Material* mat) {
+  const auto* fm = dynamic_cast<const FrictMat*>(mat);
+  if (!fm) throw std::runtime_error(std::string("material ") + mat->className() + " is not supported (FrictMat or HertzMat)");
+  const auto* hm = dynamic_cast<const HertzMat*>(fm);
+  wdg_material m{};
+  m.kind = hm ? WDG_HERTZMAT : WDG_FRICTMAT;
+  m.density = fm->density;
+  m.young = fm->young;
+  m.ktDivKn = fm->ktDivKn;
+  m.tanPhi = fm->tanPhi;
+  m.surfEnergy = hm ? hm->surfEnergy : 0.;
+  m.alpha = hm ? hm->alpha : 0.;
+  return m;
+}
+
 void DemField::push(Scene& S) {
   guard();
   if (!dev) {
@@ -148,6 +163,7 @@ void DemField::push(Scene& S) {
     // materials in first-use order
     std::map<const Material*, int> midx;
     std::vector<wdg_material> mats;
+    matOrder.clear();
     std::vector<int32_t> shape(n), axis(n), sense(n), mat(n);
     std::vector<uint32_t> mask(n), flags(n);
     std::vector<uint8_t> integ(n);
@@ -156,20 +172,10 @@ void DemField::push(Scene& S) {
       const Particle& p = *particles[i];
       auto it = midx.find(p.material.get());
       if (it == midx.end()) {
-        const auto* fm = dynamic_cast<const FrictMat*>(p.material.get());
-        if (!fm) throw std::runtime_error(std::string("material ") + p.material->className() +
-                                          " is not supported (FrictMat or HertzMat)");
-        const auto* hm = dynamic_cast<const HertzMat*>(fm);
-        wdg_material m{};
-        m.kind = hm ? WDG_HERTZMAT : WDG_FRICTMAT;
-        m.density = fm->density;
-        m.young = fm->young;
-        m.ktDivKn = fm->ktDivKn;
-        m.tanPhi = fm->tanPhi;
-        m.surfEnergy = hm ? hm->surfEnergy : 0.;
-        m.alpha = hm ? hm->alpha : 0.;
+        const wdg_material m = device_material(p.material.get());
         it = midx.emplace(p.material.get(), (int)mats.size()).first;
         mats.push_back(m);
+        matOrder.push_back(p.material);
       }
       mat[i] = it->second;
       if (const auto* s = dynamic_cast<const Sphere*>(p.shape.get())) {
@@ -217,7 +223,13 @@ void DemField::push(Scene& S) {
                                  sc->cellDist.empty() ? nullptr : sc->cellDist.data(), sc->data.data(),
                                  sc->verletDist));
     }
-  } else if (hostDirty) {
+  } else {
+    // materials edited in place (the device ignores an unchanged table)
+    std::vector<wdg_material> mats;
+    for (auto& m : matOrder) mats.push_back(device_material(m.get()));
+    check(wdg_set_materials(dev->ctx, mats.data(), (int32_t)mats.size()));
+  }
+  if (!structureChanged && hostDirty) {
     std::vector<double> buf;
     for (int f : FIELDS) {
       const int w = field_width(f);
@@ -269,7 +281,20 @@ const std::vector<ContactView>& DemField::contacts() {
     }
   }
   contactsValid = true;
+  contactGen++;
   return contactCache;
+}
+
+const ContactView* DemField::findContact(int64_t a, int64_t b) {
+  const auto& v = contacts();
+  if (contactIndexGen != contactGen) {
+    contactIndex.clear();
+    for (size_t i = 0; i < v.size(); i++)
+      contactIndex[(uint64_t)std::min(v[i].id1, v[i].id2) << 32 | (uint64_t)std::max(v[i].id1, v[i].id2)] = i;
+    contactIndexGen = contactGen;
+  }
+  auto it = contactIndex.find((uint64_t)std::min(a, b) << 32 | (uint64_t)std::max(a, b));
+  return it == contactIndex.end() ? nullptr : &v[it->second];
 }
 
 int64_t DemField::countReal() {
@@ -451,6 +476,20 @@ void Scene::run(int64_t steps) {
   bool hasDyn = false;
   for (int i = 0; i < prog.nEngines; i++) hasDyn = hasDyn || prog.engines[i] == WDG_E_DYNDT;
   f->contactsValid = false;
+  // ContactLoop.updatePhys to the device before each device call, and back ('once' turns into
+  // 'never' after one contact loop)
+  ContactLoop* cl = nullptr;
+  for (auto& e : engines)
+    if (e && !e->dead && !cl) cl = dynamic_cast<ContactLoop*>(e.get());
+  auto device_run = [&](const wdg_program* pr, int64_t k, int64_t* st, double* d) {
+    if (cl) check(wdg_set_update_phys(ctx, cl->updatePhys));
+    check(wdg_run(ctx, pr, k, st, d));
+    if (cl) {
+      int32_t m = 0;
+      check(wdg_get_update_phys(ctx, &m));
+      cl->updatePhys = m;
+    }
+  };
   if (!hostEngines && !hasDyn) {
     // one call for all steps (a background run: chunks, so that stop() ends it between two);
     // time accumulates step by step as in doOneStep
@@ -460,7 +499,7 @@ void Scene::run(int64_t steps) {
       int64_t st = step;
       double d = dt;
       f->deviceNewer = true;
-      check(wdg_run(ctx, &prog, k, &st, &d));
+      device_run(&prog, k, &st, &d);
       for (int64_t s = 0; s < k; s++) time += dt;
       step = st;
       done += k;
@@ -471,7 +510,7 @@ void Scene::run(int64_t steps) {
     if (!hostEngines) {
       const double d0 = dt;
       f->deviceNewer = true;
-      check(wdg_run(ctx, &prog, 1, &step, &dt));  // applies DynDt's nextDt at the end of the step
+      device_run(&prog, 1, &step, &dt);  // applies DynDt's nextDt at the end of the step
       time += d0;
       continue;
     }
@@ -494,7 +533,7 @@ void Scene::run(int64_t steps) {
       int64_t st = step;
       double d = dt;
       f->deviceNewer = true;
-      check(wdg_run(ctx, &one, 1, &st, &d));
+      device_run(&one, 1, &st, &d);
       if (one.engines[0] == WDG_E_DYNDT && d != dt) nextDt = d;
     }
     step++;

@@ -22,6 +22,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/woo_amd.h"
@@ -207,6 +208,12 @@ struct ContactLoop : Engine {
   std::vector<std::shared_ptr<CPhysFunctor>> phyDisp;
   std::vector<std::shared_ptr<LawFunctor>> lawDisp;
   bool applyForces = true;  // ContactLoop.hpp:118
+  // ContactLoop.hpp:119: 0 'never', 1 'always', 2 'once' (back to 'never' after one loop)
+  int updatePhys = 0;
+  // ContactLoop.hpp:121,123: the minDist00Sq shortcut switch and the OpenMP reordering period;
+  // the device keeps its real contacts packed in slot order at every step, so neither applies
+  bool dist00 = true;
+  int reorderEvery = 1000;
   const char* className() const override { return "ContactLoop"; }
 };
 struct Leapfrog : Engine {
@@ -266,6 +273,11 @@ struct DemField : Object {
   bool deviceNewer = false;      // device ran steps since the last download
   bool contactsValid = false;
   std::vector<ContactView> contactCache;
+  int64_t contactGen = 0;  // counts the rebuilds of contactCache
+  // the contact of particles a and b in contactCache (either order), or null
+  const ContactView* findContact(int64_t a, int64_t b);
+  std::unordered_map<uint64_t, size_t> contactIndex;
+  int64_t contactIndexGen = -1;
   const char* className() const override { return "DemField"; }
 
   // state loaded with Scene.loadTmp, given to the device at the first upload: the real
@@ -277,6 +289,9 @@ struct DemField : Object {
   };
   std::shared_ptr<SavedContacts> savedContacts;
   bool devTrackEnergy = false;   // wdg_set_energy_tracking state of the device context
+  // the device's material table, in first-use order; pushed again before every run, so that a
+  // material edited in place applies as ContactLoop.updatePhys says
+  std::vector<std::shared_ptr<Material>> matOrder;
 
   int64_t add(const std::shared_ptr<Particle>& p, int nodes);
   // device access from another thread while the scene runs in the background is refused

@@ -72,20 +72,50 @@ struct ParRef {
 struct ConRef {
   std::shared_ptr<DemField> f;
 };
-struct ContactRef {
+// A contact as Python holds it. The reference's Contact objects are shared and updated in
+// place by the engines (ContactContainer holds shared_ptr<Contact>), so a handle kept across
+// steps reads the current state: each read after the field's contacts changed looks the pair
+// up again; a pair no longer in the container reads as not real (a removed Contact is reset).
+struct LiveContact {
+  std::shared_ptr<DemField> f;
   ContactView c;
+  int64_t gen = -1;
+  const ContactView& view() {
+    if (f && (!f->contactsValid || f->contactGen != gen)) {
+      const ContactView* x = f->findContact(c.id1, c.id2);
+      gen = f->contactGen;
+      if (x) {
+        c = *x;
+      } else {
+        const int64_t a = c.id1, b = c.id2;
+        c = ContactView{};
+        c.id1 = a;
+        c.id2 = b;
+      }
+    }
+    return c;
+  }
+};
+struct ContactRef {
+  std::shared_ptr<LiveContact> L;
+  const ContactView& c() const { return L->view(); }
 };
 struct L6GeomRef {
-  ContactView c;
+  std::shared_ptr<LiveContact> L;
+  const ContactView& c() const { return L->view(); }
 };
 struct PhysRef {
-  ContactView c;
+  std::shared_ptr<LiveContact> L;
+  const ContactView& c() const { return L->view(); }
 };
 
-py::object contact_or_none(const std::vector<ContactView>& v, int64_t a, int64_t b) {
-  for (const auto& c : v)
-    if ((c.id1 == a && c.id2 == b) || (c.id1 == b && c.id2 == a)) return py::cast(ContactRef{c});
-  return py::none();
+ContactRef contact_ref(const std::shared_ptr<DemField>& f, const ContactView& c) {
+  return ContactRef{std::make_shared<LiveContact>(LiveContact{f, c, f->contactGen})};
+}
+
+py::object contact_or_none(const std::shared_ptr<DemField>& f, int64_t a, int64_t b) {
+  const ContactView* c = f->findContact(a, b);
+  return c ? py::cast(contact_ref(f, *c)) : py::none();
 }
 
 // Python code run by an engine: on a background run its Python error becomes a C++ one, kept
@@ -296,35 +326,36 @@ PYBIND11_MODULE(_woo, m) {
 
   // ---------------------------------------------------------------- contacts
   py::class_<L6GeomRef>(m, "L6Geom")
-      .def_property_readonly("uN", [](L6GeomRef& g) { return g.c.uN; })
+      .def_property_readonly("uN", [](L6GeomRef& g) { return g.c().uN; })
       .def_property_readonly("trsf", [](L6GeomRef& g) {
         py::array_t<double> a({3, 3});
-        std::copy(g.c.trsf.begin(), g.c.trsf.end(), a.mutable_data());
+        std::copy(g.c().trsf.begin(), g.c().trsf.end(), a.mutable_data());
         return a;
       })
-      .def_property_readonly("contPt", [](L6GeomRef& g) { return arr(g.c.contPt.data(), 3); })
+      .def_property_readonly("contPt", [](L6GeomRef& g) { return arr(g.c().contPt.data(), 3); })
       .def_property_readonly("node", [](L6GeomRef& g) {
         auto n = std::make_shared<Node>();
-        n->pos = g.c.contPt;
+        n->pos = g.c().contPt;
         return n;
       });
   py::class_<PhysRef>(m, "FrictPhys")
-      .def_property_readonly("force", [](PhysRef& p) { return arr(p.c.force.data(), 3); })
-      .def_property_readonly("kn", [](PhysRef& p) { return p.c.kn; })
-      .def_property_readonly("kt", [](PhysRef& p) { return p.c.kt; });
+      .def_property_readonly("force", [](PhysRef& p) { return arr(p.c().force.data(), 3); })
+      .def_property_readonly("kn", [](PhysRef& p) { return p.c().kn; })
+      .def_property_readonly("kt", [](PhysRef& p) { return p.c().kt; });
   py::class_<ContactRef>(m, "Contact")
-      .def_property_readonly("id1", [](ContactRef& c) { return c.c.id1; })
-      .def_property_readonly("id2", [](ContactRef& c) { return c.c.id2; })
-      .def_property_readonly("real", [](ContactRef& c) { return c.c.real; })
-      .def_property_readonly("cellDist", [](ContactRef& c) { return c.c.cellDist; })
+      .def_property_readonly("id1", [](ContactRef& c) { return c.c().id1; })
+      .def_property_readonly("id2", [](ContactRef& c) { return c.c().id2; })
+      .def_property_readonly("real", [](ContactRef& c) { return c.c().real; })
+      .def_property_readonly("cellDist", [](ContactRef& c) { return c.c().cellDist; })
       .def_property_readonly("geom", [](ContactRef& c) -> py::object {
-        return c.c.real ? py::cast(L6GeomRef{c.c}) : py::none();
+        return c.c().real ? py::cast(L6GeomRef{c.L}) : py::none();
       })
       .def_property_readonly("phys", [](ContactRef& c) -> py::object {
-        return c.c.real ? py::cast(PhysRef{c.c}) : py::none();
+        return c.c().real ? py::cast(PhysRef{c.L}) : py::none();
       })
       .def("__repr__", [](ContactRef& c) {
-        return "<Contact ##" + std::to_string(c.c.id1) + "+" + std::to_string(c.c.id2) + (c.c.real ? "" : " (potential)") + ">";
+        return "<Contact ##" + std::to_string(c.c().id1) + "+" + std::to_string(c.c().id2) +
+               (c.c().real ? "" : " (potential)") + ">";
       });
 
   py::class_<ConRef>(m, "ContactContainer")
@@ -334,21 +365,21 @@ PYBIND11_MODULE(_woo, m) {
         const auto& v = r.f->contacts();
         if (i < 0) i += (int64_t)v.size();
         if (i < 0 || i >= (int64_t)v.size()) throw py::index_error("contact index out of range");
-        return ContactRef{v[(size_t)i]};
+        return contact_ref(r.f, v[(size_t)i]);
       })
       .def("__getitem__", [](ConRef& r, std::pair<int64_t, int64_t> ij) {
-        py::object c = contact_or_none(r.f->contacts(), ij.first, ij.second);
+        py::object c = contact_or_none(r.f, ij.first, ij.second);
         if (c.is_none()) throw py::index_error("no contact between the particles");
         return c;
       })
-      .def("exists", [](ConRef& r, int64_t a, int64_t b) { return !contact_or_none(r.f->contacts(), a, b).is_none(); })
+      .def("exists", [](ConRef& r, int64_t a, int64_t b) { return r.f->findContact(a, b) != nullptr; })
       .def("existsReal", [](ConRef& r, int64_t a, int64_t b) {
-        py::object c = contact_or_none(r.f->contacts(), a, b);
-        return !c.is_none() && c.cast<ContactRef>().c.real;
+        const ContactView* c = r.f->findContact(a, b);
+        return c && c->real;
       })
       .def("__iter__", [](ConRef& r) {
         py::list l;
-        for (const auto& c : r.f->contacts()) l.append(ContactRef{c});
+        for (const auto& c : r.f->contacts()) l.append(contact_ref(r.f, c));
         return py::iter(l);
       });
 
@@ -387,7 +418,14 @@ PYBIND11_MODULE(_woo, m) {
       .def_property(
           "loneMask", [](DemField& f) { return f.loneMask; },
           [](DemField& f, uint32_t v) { f.restructure(); f.loneMask = v; })
-      .def_property_readonly("par", [](std::shared_ptr<DemField> f) { return ParRef{f}; })
+      // DemField(par=[...]): the list added to an empty field, as the reference's list to
+      // ParticleContainer conversion does
+      .def_property(
+          "par", [](std::shared_ptr<DemField> f) { return ParRef{f}; },
+          [](std::shared_ptr<DemField> f, std::vector<std::shared_ptr<Particle>> pp) {
+            if (!f->particles.empty()) throw py::value_error("DemField.par is assigned only while empty (use par.add)");
+            for (auto& p : pp) f->add(p, -1);
+          })
       .def_property_readonly("con", [](std::shared_ptr<DemField> f) { return ConRef{f}; })
       .def_property_readonly("nodes", [](DemField& f) {
         py::list l;
@@ -490,7 +528,25 @@ PYBIND11_MODULE(_woo, m) {
       .def_readwrite("geoDisp", &ContactLoop::geoDisp)
       .def_readwrite("phyDisp", &ContactLoop::phyDisp)
       .def_readwrite("lawDisp", &ContactLoop::lawDisp)
-      .def_readwrite("applyForces", &ContactLoop::applyForces);
+      .def_readwrite("applyForces", &ContactLoop::applyForces)
+      // a named enum (ContactLoop.hpp:119): reads as its name, takes the name or the number
+      .def_property(
+          "updatePhys",
+          [](const ContactLoop& c) { return std::string(c.updatePhys == 1 ? "always" : c.updatePhys == 2 ? "once" : "never"); },
+          [](ContactLoop& c, py::object v) {
+            static const char* names[] = {"never", "always", "once"};
+            if (py::isinstance<py::str>(v)) {
+              const std::string s = v.cast<std::string>();
+              for (int k = 0; k < 3; k++)
+                if (s == names[k]) { c.updatePhys = k; return; }
+              throw py::value_error("ContactLoop.updatePhys: '" + s + "' is not one of never, always, once");
+            }
+            const int k = v.cast<int>();
+            if (k < 0 || k > 2) throw py::value_error("ContactLoop.updatePhys: 0 (never), 1 (always) or 2 (once)");
+            c.updatePhys = k;
+          })
+      .def_readwrite("dist00", &ContactLoop::dist00)
+      .def_readwrite("reorderEvery", &ContactLoop::reorderEvery);
   py::class_<Leapfrog, Engine, std::shared_ptr<Leapfrog>>(m, "Leapfrog", py::dynamic_attr())
       .def(py::init(&make_kw<Leapfrog>))
       .def_readwrite("damping", &Leapfrog::damping)

@@ -162,3 +162,7 @@ def stress(S, volume=-1.):
 def kineticEnergy(S):
     """Kinetic energy of the integrated nodes (DemData::getEk_any, spherical inertia)."""
     return S.dem.kineticEnergy()
+
+
+# DemField.minimalEngines is defaultEngines (py/_monkey/aliases.py:102)
+DemField.minimalEngines = staticmethod(defaultEngines)
