@@ -137,6 +137,10 @@ typedef struct {
   double poisson, en;  /* Cp2_HertzMat_HertzPhys.poisson, .en */
   int32_t noSlip, noBreak, noFrict; /* Law2_L6Geom_FrictPhys_IdealElPl */
   int32_t noAttraction;             /* Law2_L6Geom_HertzPhys_DMT */
+  /* Law2_L6Geom_FrictPhys_IdealElPl rolling and twisting resistance (IdealElPl.hpp:18-20,
+   * IdealElPl.cpp:93-123): on when relRollStiff > 0 and rollTanPhi > 0 (FP64 contact form
+   * only); 0 = off */
+  double relRollStiff, relTwistStiff, rollTanPhi;
 } wdg_contact_params;
 
 typedef struct {

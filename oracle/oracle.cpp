@@ -864,6 +864,31 @@ struct orc_scene {
       if (std::isnan(elast)) elast = 0.;
       c.en[0] = elast;
     }
+    // IdealElPl.cpp:93-123: rolling and twisting resistance; the torque was zeroed above, so it
+    // is this step's increment, capped (no history)
+    if (prm.relRollStiff > 0. && prm.rollTanPhi > 0.) {
+      const double charLen = c.lens[0] + c.lens[1];
+      if (charLen <= 0) throw std::runtime_error("charLen<=0");
+      const double kr = c.kn * charLen;
+      if (prm.relTwistStiff > 0) {
+        c.torque[0] += prm.dt * prm.relTwistStiff * kr * c.angVel[0];
+        const double maxTt = std::fabs(c.force[0] * prm.rollTanPhi * charLen);
+        if (std::fabs(c.torque[0]) > maxTt) c.torque[0] = std::copysign(maxTt, c.torque[0]);
+      }
+      const double dtkr = prm.dt * kr;
+      c.torque[1] += dtkr * c.angVel[1];
+      c.torque[2] += dtkr * c.angVel[2];
+      const double maxTr = std::max(0., std::fabs(c.force[0]) * prm.rollTanPhi * charLen);
+      if (c.torque[1] * c.torque[1] + c.torque[2] * c.torque[2] > maxTr * maxTr) {
+        const double TrNorm = std::sqrt(c.torque[1] * c.torque[1] + c.torque[2] * c.torque[2]);
+        double ratio = maxTr / TrNorm;
+        if (TrNorm == 0.) ratio = 0.;
+        c.torque[1] *= ratio;
+        c.torque[2] *= ratio;
+      }
+    } else {
+      c.torque = zero3();
+    }
     return true;
   }
   // Law2_L6Geom_HertzPhys_DMT::go (Hertz.cpp:85-210), DMT/Hertz branch (alpha==0 or gamma==0)
@@ -914,7 +939,9 @@ struct orc_scene {
     F = mul(quatRot(c.nodeOri, c.force), sign);
     V3 shift = (!isPA && prm.periodic) ? shiftPos(c.cellDist) : zero3();
     V3 xc = sub(c.nodePos, add(ld3(par[pid].pos), shift));
-    T = cross(xc, F);
+    // the phys torque (rolling resistance) is added when it is not zero
+    if (c.torque[0] == 0 && c.torque[1] == 0 && c.torque[2] == 0) T = cross(xc, F);
+    else T = add(cross(xc, F), mul(quatRot(c.nodeOri, c.torque), sign));
   }
   // CompUtils::triangleBarycentrics (CompUtils.cpp:94-112): x - A = u (B - A) + v (C - A), dotted
   // with B - A and C - A gives a 2x2 system, solved with the inverse of its matrix (adjugate over
@@ -932,8 +959,8 @@ struct orc_scene {
   // each real contact's force over its 3 nodes, weighted by the barycentric coordinates of the
   // contact point projected onto the facet plane; Contact::getForceTorqueBranch (Contact.cpp:
   // 24-34) per node gives F = ori * force * sign and xc = contact point - node position (the
-  // facet is always pA, so no periodic shift); the phys torque is zero for these laws. Node
-  // i gets w_i F and xc_i x (w_i F). Contacts are visited in Particle::contacts order (the
+  // facet is always pA, so no periodic shift) and T = ori * torque * sign (rolling
+  // resistance; zero otherwise). Node i gets w_i F and xc_i x (w_i F) + w_i T. Contacts are visited in Particle::contacts order (the
   // other id). The facet nodes are massless and outside gravity: the reset at the start of the
   // step (ForceResetter, or Leapfrog.reset of the step before) sets them to zero, so the values
   // kept are this step's sums, which ReadForce reads before Leapfrog resets them.
@@ -956,10 +983,13 @@ struct orc_scene {
         const bool isPA = (int)pid == C.pA;
         const V3 Fc = mul(quatRot(C.nodeOri, C.force), isPA ? 1. : -1.);
         const V3 shift = (!isPA && prm.periodic) ? shiftPos(C.cellDist) : zero3();
+        // the phys torque (rolling resistance), zero for the other laws
+        const bool hasT = !(C.torque[0] == 0 && C.torque[1] == 0 && C.torque[2] == 0);
+        const V3 Tc = hasT ? mul(quatRot(C.nodeOri, C.torque), isPA ? 1. : -1.) : zero3();
         for (int i = 0; i < 3; i++) {
           const V3 xc = sub(c, add(v[i], shift));
           const V3 F = mul(Fc, w[i]);
-          const V3 T = cross(xc, F);
+          const V3 T = hasT ? add(cross(xc, F), mul(Tc, w[i])) : cross(xc, F);
           for (int k = 0; k < 3; k++) { f.nodeF[3 * i + k] += F[k]; f.nodeT[3 * i + k] += T[k]; }
         }
       }

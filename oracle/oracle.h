@@ -19,7 +19,7 @@
  *   src/supp/base/CompUtils.cpp:94-112    triangleBarycentrics
  *   src/dem/L6Geom.cpp:25-216             handleSpheresLikeContact
  *   src/dem/FrictMat.cpp:20-44            Cp2_FrictMat_FrictPhys
- *   src/dem/IdealElPl.cpp:15-125          Law2_L6Geom_FrictPhys_IdealElPl
+ *   src/dem/IdealElPl.cpp:15-125          Law2_L6Geom_FrictPhys_IdealElPl (with rolling/twisting resistance)
  *   src/dem/Hertz.cpp:34-210              Cp2_HertzMat_HertzPhys, Law2_L6Geom_HertzPhys_DMT (DMT branch)
  *   src/dem/ContactLoop.cpp:93-276        ContactLoop::run, applyForceUninodal
  *   src/dem/Contact.cpp:24-34             getForceTorqueBranch
@@ -84,6 +84,9 @@ typedef struct {
   int trackEnergy;
   /* ContactLoop.updatePhys (ContactLoop.hpp:119): 0 never, 1 always, 2 once (then never) */
   int updatePhys;
+  /* Law2_L6Geom_FrictPhys_IdealElPl rolling and twisting resistance (IdealElPl.hpp:18-20);
+   * off unless relRollStiff > 0 and rollTanPhi > 0 */
+  double relRollStiff, relTwistStiff, rollTanPhi;
 } orc_params;
 
 /* Scene.energy (EnergyTracker, src/core/EnergyTracker.hpp:30-61): the terms the path's engines

@@ -27,6 +27,7 @@ class OrcParams(C.Structure):
         ("damping", C.c_double), ("reset", C.c_int),
         ("dynDtPeriod", C.c_int), ("maxRelInc", C.c_double), ("trackEnergy", C.c_int),
         ("updatePhys", C.c_int),
+        ("relRollStiff", C.c_double), ("relTwistStiff", C.c_double), ("rollTanPhi", C.c_double),
     ]
 
 # Scene.energy terms, in the order orc_energy and wdg_energy give them

@@ -238,3 +238,14 @@ def test_contact_loop_update_phys_attribute(woo):
     # DemField(par=...) fills an empty field only
     with pytest.raises(ValueError):
         S.dem.par = [Sphere.make((0, 0, 5), .1, mat=m)]
+
+
+def test_idealelpl_rolling_attributes(woo):
+    # Law2_L6Geom_FrictPhys_IdealElPl.relRollStiff / relTwistStiff / rollTanPhi (IdealElPl.hpp:18-20),
+    # off by default, settable as keyword arguments
+    from woo_amd.woo.dem import Law2_L6Geom_FrictPhys_IdealElPl
+
+    law = Law2_L6Geom_FrictPhys_IdealElPl()
+    assert (law.relRollStiff, law.relTwistStiff, law.rollTanPhi) == (0., 0., 0.)
+    law = Law2_L6Geom_FrictPhys_IdealElPl(relRollStiff=1., relTwistStiff=.5, rollTanPhi=.1, noSlip=True)
+    assert (law.relRollStiff, law.relTwistStiff, law.rollTanPhi, law.noSlip) == (1., .5, .1, True)

@@ -128,6 +128,59 @@ def test_update_phys_never_always_once(up):
         assert _kn_of_contact(o) == kn1
 
 
+def _rolling_on_floor(relRollStiff, rollTanPhi, v0=.1, steps=3000, r=.01, rho=2000., E=1e7, dtf=.1):
+    """A sphere rolling without slip on a floor wall under gravity (friction angle large enough),
+    with rolling resistance; returns the horizontal speed sampled every step after a settling
+    period, the time step and the contact's lens sum."""
+    o = Oracle(damping=0., gravity=(0, 0, -9.81), relRollStiff=relRollStiff, rollTanPhi=rollTanPhi)
+    m = o.material(density=rho, young=E, ktDivKn=.2, tanPhi=.8)
+    o.wall((0, 0, 0), 2, 1, m)
+    mass = 4 / 3. * math.pi * r ** 3 * rho
+    kn = E * math.pi * r / 2
+    o.sphere((0, 0, r - mass * 9.81 / kn), r, m)
+    o.set("vel", np.array([[0, 0, 0], [v0, 0, 0]]))
+    o.set("angVel", np.array([[0, 0, 0], [0, v0 / r, 0]]))
+    o.set_params(dt=dtf * o.pwave_dt())
+    o.run(ENG, 200)
+    v = []
+    for _ in range(steps):
+        o.run(ENG, 1)
+        v.append(o.get("vel")[1, 0])
+    _, _, data = o.contacts(real_only=True)
+    return np.array(v), o.dt, mass, kn
+
+
+def test_rolling_resistance_at_its_cap_decelerates_at_five_sevenths_of_the_cap():
+    # IdealElPl.cpp:93-123: a saturated rolling torque M = |Fn| rollTanPhi charLen opposes the
+    # rotation; rolling without slip, (I + m r^2) alpha = -M with I = 2/5 m r^2 and Fn = m g, so
+    # the speed falls at a = (5/7) g rollTanPhi charLen / r, charLen = 2 r (wall contact lens
+    # r + uN/2 each, the overlap is 1e-5 r here)
+    r, tp = .01, .05
+    v, dt, mass, kn = _rolling_on_floor(relRollStiff=1., rollTanPhi=tp, r=r)
+    a = np.polyfit(np.arange(len(v)) * dt, v, 1)[0]
+    assert a == pytest.approx(-(5 / 7.) * 9.81 * tp * 2 * r / r, rel=5e-3)
+    # without resistance it rolls on
+    v0, _, _, _ = _rolling_on_floor(relRollStiff=0., rollTanPhi=tp, r=r, steps=500)
+    assert abs(v0[-1] - v0[0]) < 1e-4 * v0[0]
+
+
+def test_rolling_resistance_below_its_cap_damps_exponentially():
+    # below the cap the torque is dt kr omega with kr = kn charLen (IdealElPl.cpp:97,109;
+    # relRollStiff only switches it on); kn charLen = E A (FrictMat.cpp:37, lens l0 + l1 =
+    # charLen) with A = pi r^2. Rolling without slip, (7/5) m r^2 domega/dt = -dt E pi r^2 omega,
+    # so v decays at the rate dt E pi / (7/5 m). A large sphere and a small step keep that rate
+    # times dt small (3e-4), where the discrete steps follow the exponential (the fit over one
+    # second averages the tangential spring's oscillation; it comes within 1.4 % of the rate, 4 %
+    # at four times the step)
+    r, E = 1., 1e7
+    v, dt, mass, kn = _rolling_on_floor(relRollStiff=1., rollTanPhi=1.5, v0=.1, r=r, E=E, steps=2830, dtf=.025)
+    t = np.arange(len(v)) * dt
+    rate = -np.polyfit(t, np.log(v), 1)[0]
+    expect = dt * E * math.pi / (1.4 * mass)
+    assert rate * dt < 1e-3
+    assert rate == pytest.approx(expect, rel=2e-2)
+
+
 def test_sliding_sphere_rolls_at_five_sevenths():
     # a sphere sliding on a wall ends up rolling at 5/7 of its initial speed (solid sphere),
     # independent of the friction coefficient; checks IdealElPl slip, torque branch and rotation

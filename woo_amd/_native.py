@@ -72,7 +72,8 @@ class CollideParams(C.Structure):
 class ContactParams(C.Structure):
     _fields_ = [("step", C.c_int64), ("dt", C.c_double), ("physKind", C.c_int32), ("lawKind", C.c_int32),
                 ("applyForces", C.c_int32), ("trsfRenorm", C.c_int32), ("poisson", C.c_double), ("en", C.c_double),
-                ("noSlip", C.c_int32), ("noBreak", C.c_int32), ("noFrict", C.c_int32), ("noAttraction", C.c_int32)]
+                ("noSlip", C.c_int32), ("noBreak", C.c_int32), ("noFrict", C.c_int32), ("noAttraction", C.c_int32),
+                ("relRollStiff", C.c_double), ("relTwistStiff", C.c_double), ("rollTanPhi", C.c_double)]
 
 
 class LeapfrogParams(C.Structure):
@@ -267,10 +268,12 @@ class Device:
         check(self.L.wdg_collide(self.h, C.byref(p)))
 
     def contact_loop(self, step, dt, physKind=PHYS_FRICT, lawKind=LAW_IDEALELPL, applyForces=True, trsfRenorm=100,
-                     poisson=.2, en=float("nan"), noSlip=False, noBreak=False, noFrict=False, noAttraction=True):
+                     poisson=.2, en=float("nan"), noSlip=False, noBreak=False, noFrict=False, noAttraction=True,
+                     relRollStiff=0., relTwistStiff=0., rollTanPhi=0.):
         p = ContactParams(step=step, dt=dt, physKind=physKind, lawKind=lawKind, applyForces=int(applyForces),
                           trsfRenorm=trsfRenorm, poisson=poisson, en=en, noSlip=int(noSlip), noBreak=int(noBreak),
-                          noFrict=int(noFrict), noAttraction=int(noAttraction))
+                          noFrict=int(noFrict), noAttraction=int(noAttraction), relRollStiff=relRollStiff,
+                          relTwistStiff=relTwistStiff, rollTanPhi=rollTanPhi)
         check(self.L.wdg_contact_loop(self.h, C.byref(p)))
 
     def leapfrog(self, dt, damping=.2, reset=True):
@@ -418,7 +421,8 @@ class Device:
             prog.engines[i] = ENG[e] if isinstance(e, str) else e
         prog.collide = CollideParams(**(collide or dict(verletDist=-.05, forceInit=0)))
         cp = dict(step=0, dt=dt, physKind=PHYS_FRICT, lawKind=LAW_IDEALELPL, applyForces=1, trsfRenorm=100,
-                  poisson=.2, en=float("nan"), noSlip=0, noBreak=0, noFrict=0, noAttraction=1)
+                  poisson=.2, en=float("nan"), noSlip=0, noBreak=0, noFrict=0, noAttraction=1, relRollStiff=0.,
+                  relTwistStiff=0., rollTanPhi=0.)
         cp.update(contact or {})
         prog.contact = ContactParams(**cp)
         lp = dict(dt=dt, damping=.2, reset=1)

@@ -229,6 +229,7 @@ void restore_contacts(Ctx& c, int64_t n, const int32_t* ids, const int32_t* cell
   drec.release();
   ddata.release();
   c.forcesPending = false;
+  c.rollActive = false;  // the records hold no phys torque; the next contact loop writes it
   c.gen++;
 }
 
@@ -352,6 +353,7 @@ void snapshot_contacts_load(Ctx& c, int slot) {
     throw Error(WDG_E_STATE, "snapshot: " + std::to_string(nReal - got) +
                                  " saved contacts are not in the slot list rebuilt at the snapshot's positions");
   c.forcesPending = false;
+  c.rollActive = false;
   c.gen++;
 }
 

@@ -64,6 +64,10 @@ struct ContactArgs {
   int renorm;  // 1 if trsf is renormalised this step
   double poisson, en, viscAlpha;
   int noSlip, noBreak, noFrict, noAttraction;
+  // IdealElPl rolling and twisting resistance (the LAW_IEP_ROLL instantiation): the parameters
+  // and the global phys torque of each real contact (3 doubles per state index)
+  double relTwistStiff, rollTanPhi;
+  double* gtq;
   uint32_t* dflags;
   double* enPart;  // energy tracking: 4 partial sums per force workgroup
   uint32_t* eidx;          // per adjacency entry: its slot when real, else NO_STATE (null: not kept)
@@ -300,6 +304,34 @@ __device__ __forceinline__ wdg_material material(const ContactArgs& a, const wdg
   return m < (uint32_t)MAT_LDS ? smat[m] : a.mats[m];
 }
 
+// Law2_L6Geom_FrictPhys_IdealElPl with rolling and twisting resistance (an internal law code)
+constexpr int LAW_IEP_ROLL = 2;
+// IdealElPl.cpp:93-123: the phys torque is zeroed at every step (IdealElPl.cpp:44), so it is this
+// step's dt kr angVel, kr = kn charLen, capped at |Fn| rollTanPhi charLen (twist scaled by
+// relTwistStiff, capped alone); the operations in the reference's order
+__device__ __forceinline__ d3 rolling_torque(const ContactArgs& a, double kn, double Fn, double charLen, double dt,
+                                             d3 lav) {
+  const double kr = kn * charLen;
+  double t0 = 0., t1 = 0., t2 = 0.;
+  if (a.relTwistStiff > 0) {
+    t0 = t0 + dt * a.relTwistStiff * kr * lav.x;
+    const double maxTt = fabs(Fn * a.rollTanPhi * charLen);
+    if (fabs(t0) > maxTt) t0 = copysign(maxTt, t0);
+  }
+  const double dtkr = dt * kr;
+  t1 = t1 + dtkr * lav.y;
+  t2 = t2 + dtkr * lav.z;
+  const double maxTr = fmax(0., fabs(Fn) * a.rollTanPhi * charLen);
+  if (t1 * t1 + t2 * t2 > maxTr * maxTr) {
+    const double TrNorm = sqrt(t1 * t1 + t2 * t2);
+    double ratio = maxTr / TrNorm;
+    if (TrNorm == 0.) ratio = 0.;
+    t1 = t1 * ratio;
+    t2 = t2 * ratio;
+  }
+  return mk3(t0, t1, t2);
+}
+
 template <bool PERIODIC, int LAW, bool FACETS, bool ENERGY>
 __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
   __shared__ wdg_material smat[MAT_LDS];
@@ -406,6 +438,8 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
 
     // Cg2_Any_Any_L6Geom__Base::handleSpheresLikeContact (L6Geom.cpp:49-216)
     d3 T0, T1, T2, gvel;
+    // L6Geom.angVel (L6Geom.cpp:211), read by the rolling resistance only; zero when fresh
+    d3 lav = mk3(0., 0., 0.);
     if (!real) {
       // setInitialLocalCoords (L6Geom.cpp:25-36)
       const d3 locX = normal;
@@ -447,6 +481,10 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
       const d3 c2x = sub3(midContPt, midPos2);
       const d3 relVel = sub3(add3(vel2, cross3(av2, c2x)), add3(vel1, cross3(av1, c1x)));
       gvel = mk3(dot3(M0, relVel), dot3(M1, relVel), dot3(M2, relVel));
+      if (LAW == LAW_IEP_ROLL) {
+        const d3 dw = sub3(av2, av1);
+        lav = mk3(dot3(M0, dw), dot3(M1, dw), dot3(M2, dw));
+      }
     }
 
     // Cp2 (recomputed; constant over the life of the contact)
@@ -461,7 +499,8 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
     }
     const wdg_material mA = material(a, smat, eb + matA), mB = material(a, smat, eb + matB);
     double Fn, Ft0 = real ? oFt0 : 0., Ft1 = real ? oFt1 : 0.;
-    if (LAW == WDG_LAW_IDEALELPL) {
+    d3 tq = mk3(0., 0., 0.);  // CPhys.torque (local), rolling resistance only
+    if (LAW != WDG_LAW_HERTZDMT) {
       // Law2_L6Geom_FrictPhys_IdealElPl::go (IdealElPl.cpp:15-125); breaking decided by k_cfilter
       PhysVals ph;
       phys_frict(mA, mB, l0, l1, A, ph);
@@ -491,6 +530,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
         if (isnan(el)) el = 0.;
         en[0] += el;
       }
+      if (LAW == LAW_IEP_ROLL) tq = rolling_torque(a, ph.kn, Fn, l0 + l1, dt, lav);
     } else {
       // Law2_L6Geom_HertzPhys_DMT::go (Hertz.cpp:85-210), DMT branch; breaking decided by k_cfilter
       PhysVals ph;
@@ -539,6 +579,13 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
     g2[1] = make_double2(G.z, contPt.x);
     g2[2] = make_double2(contPt.y, contPt.z);
     g2[3] = make_double2(__longlong_as_double((long long)cd), 0.);
+    if (LAW == LAW_IEP_ROLL) {
+      // Contact::getForceTorqueBranch: T = node.ori * torque, zero when the torque is zero
+      const bool tz = tq.x == 0. && tq.y == 0. && tq.z == 0.;
+      const d3 TG = tz ? mk3(0., 0., 0.) : quat_rot(q, tq);
+      double* gt = a.gtq + (int64_t)3 * k;
+      gt[0] = TG.x; gt[1] = TG.y; gt[2] = TG.z;
+    }
   }
   if (ENERGY) block_sum_store<4, CF_BS>(en, a.enPart + (int64_t)4 * blockIdx.x);
 }
@@ -1018,7 +1065,8 @@ void eidx_clear(Ctx& c) {
 }
 
 template <bool E>
-static void launch_cforce(Ctx& c, const ContactArgs& a, dim3 grid, bool f32, bool hertz, bool per, bool fac) {
+static void launch_cforce(Ctx& c, const ContactArgs& a, dim3 grid, bool f32, bool hertz, bool roll, bool per,
+                          bool fac) {
   const dim3 block(CF_BS);
   if (f32) {
     if (hertz) {
@@ -1035,6 +1083,10 @@ static void launch_cforce(Ctx& c, const ContactArgs& a, dim3 grid, bool f32, boo
       if (per) hipLaunchKernelGGL((k_cforce<true, WDG_LAW_HERTZDMT, false, E>), grid, block, 0, c.stream, a);
       else if (fac) hipLaunchKernelGGL((k_cforce<false, WDG_LAW_HERTZDMT, true, E>), grid, block, 0, c.stream, a);
       else hipLaunchKernelGGL((k_cforce<false, WDG_LAW_HERTZDMT, false, E>), grid, block, 0, c.stream, a);
+    } else if (roll) {
+      if (per) hipLaunchKernelGGL((k_cforce<true, LAW_IEP_ROLL, false, E>), grid, block, 0, c.stream, a);
+      else if (fac) hipLaunchKernelGGL((k_cforce<false, LAW_IEP_ROLL, true, E>), grid, block, 0, c.stream, a);
+      else hipLaunchKernelGGL((k_cforce<false, LAW_IEP_ROLL, false, E>), grid, block, 0, c.stream, a);
     } else {
       if (per) hipLaunchKernelGGL((k_cforce<true, WDG_LAW_IDEALELPL, false, E>), grid, block, 0, c.stream, a);
       else if (fac) hipLaunchKernelGGL((k_cforce<false, WDG_LAW_IDEALELPL, true, E>), grid, block, 0, c.stream, a);
@@ -1055,6 +1107,9 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
     for (const auto& m : c.mats)
       if (!(m.alpha == 0. || m.surfEnergy == 0.))
         throw Error(WDG_E_ARG, "Schwarz adhesion (HertzMat.alpha and surfEnergy both non-zero) is not supported");
+  const bool roll = rolling(*p);
+  if (roll && c.precision == 32)
+    throw Error(WDG_E_ARG, "rolling resistance (IdealElPl.relRollStiff, rollTanPhi) needs the FP64 contact form");
   if (c.forcesPending) flush_forces(c);
   const double viscAlpha = (p->en > 0 && p->en < 1.)
                                ? -std::sqrt(5) * std::log(p->en) /
@@ -1069,10 +1124,12 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
     // no contact: this step's elast is zero (EnergyTracker resets it every step)
     if (c.trackEnergy) energy_zero(c, WDG_EN_ELAST);
     c.forcesPending = false;
+    c.rollActive = false;
     return;
   }
   const int from = c.rcur, to = 1 - c.rcur;
   c.R[to].ensure(C.n);
+  if (roll) c.R[to].ensureRoll(C.n);
   const int64_t ng = ngroups(C.n);
   c.act.ensure(ng * GROUP);
   c.desc.ensure(ng * GROUP);
@@ -1092,6 +1149,11 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
   a.en = p->en;
   a.viscAlpha = viscAlpha;
   a.noSlip = p->noSlip; a.noBreak = p->noBreak; a.noFrict = p->noFrict; a.noAttraction = p->noAttraction;
+  if (roll) {
+    a.relTwistStiff = p->relTwistStiff;
+    a.rollTanPhi = p->rollTanPhi;
+    a.gtq = c.R[to].gtq.p;
+  }
   const bool per = c.field.periodic != 0;
   const bool fac = c.nFacet > 0;
   const bool hertz = p->lawKind == WDG_LAW_HERTZDMT;
@@ -1118,17 +1180,18 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
   if (c.trackEnergy) {
     c.enPart.ensure((size_t)4 * gforce.x);
     a.enPart = c.enPart.p;
-    launch_cforce<true>(c, a, gforce, c.precision == 32, hertz, per, fac);
+    launch_cforce<true>(c, a, gforce, c.precision == 32, hertz, roll, per, fac);
     // EnergyTracker: elast is resettable (this step's sum replaces it), the dissipations accumulate
     static const int tgt[4] = {WDG_EN_ELAST, WDG_EN_PLAST, WDG_EN_VISCN, WDG_EN_VISCT};
     energy_add(c, gforce.x, 4, tgt, 1);
   } else {
-    launch_cforce<false>(c, a, gforce, c.precision == 32, hertz, per, fac);
+    launch_cforce<false>(c, a, gforce, c.precision == 32, hertz, roll, per, fac);
   }
   c.R[to].sidxValid = true;
   c.eidxValid = a.eidx != nullptr;
   c.eidxClean = false;
   c.rcur = to;
+  c.rollActive = roll;
   c.forcesPending = p->applyForces != 0;
   // ContactLoop::run's epilogue (ContactLoop.cpp:265-266): 'once' updates one step only
   if (c.updatePhys == 2 && !c.graphCapture) c.updatePhys = 0;

@@ -96,6 +96,10 @@ static void capture(Ctx& c, const wdg_program* prog, int K, int64_t step, double
   const int64_t nc = c.con().n;
   c.R[0].ensure(nc);
   c.R[1].ensure(nc);
+  if (rolling(prog->contact)) {
+    c.R[0].ensureRoll(nc);
+    c.R[1].ensureRoll(nc);
+  }
   c.act.ensure(ngroups(nc) * GROUP);
   c.desc.ensure(ngroups(nc) * GROUP);
   if (c.field.periodic) c.descDist.ensure(ngroups(nc) * GROUP);
@@ -149,7 +153,8 @@ static bool graph_valid(const Ctx& c, const wdg_program* prog, int K, double dt)
          G.parity == c.lfParity && G.dt == dt && G.damping == prog->leapfrog.damping && a.physKind == b.physKind &&
          a.lawKind == b.lawKind && a.applyForces == b.applyForces && a.trsfRenorm == b.trsfRenorm &&
          a.poisson == b.poisson && (a.en == b.en || (a.en != a.en && b.en != b.en)) && a.noSlip == b.noSlip &&
-         a.noBreak == b.noBreak && a.noFrict == b.noFrict && a.noAttraction == b.noAttraction;
+         a.noBreak == b.noBreak && a.noFrict == b.noFrict && a.noAttraction == b.noAttraction &&
+         a.relRollStiff == b.relRollStiff && a.relTwistStiff == b.relTwistStiff && a.rollTanPhi == b.rollTanPhi;
 }
 
 // one batch of R replays of the K-step graph (up to R*K steps); returns the steps executed. The

@@ -26,6 +26,7 @@ struct GatherArgs {
   const uint32_t* gstop;  // graph replay: a set stop word skips the kernel
   RealView rv;
   const double* gp;
+  const double* gtq;  // global phys torque per real contact (rolling resistance), else null
   double L[3];
 };
 
@@ -60,7 +61,13 @@ __device__ __forceinline__ void gather_one(const GatherArgs& g, uint32_t e, d3 p
     // Contact::getForceTorqueBranch: xc = contact point - (node pos + periodic shift of pB)
     const d3 xc = sub3(mk3(r1.y, r2.x, r2.y), add3(pos, shift));
     F = add3(F, Fc);
-    T = add3(T, cross3(xc, Fc));
+    if (g.gtq) {
+      // applyForceUninodal: xc x F + T, T the phys torque (rolling resistance) times the sign
+      const double* t = g.gtq + (int64_t)3 * rv_index(g.rv, slot);
+      T = add3(T, add3(cross3(xc, Fc), mul3(mk3(t[0], t[1], t[2]), sign)));
+    } else {
+      T = add3(T, cross3(xc, Fc));
+    }
   }
 }
 
@@ -87,7 +94,13 @@ __device__ __forceinline__ void gather_record(const GatherArgs& g, uint32_t q, b
     // Contact::getForceTorqueBranch: xc = contact point - (node pos + periodic shift of pB)
     const d3 xc = sub3(mk3(r1.y, r2.x, r2.y), add3(pos, shift));
     F = add3(F, Fc);
-    T = add3(T, cross3(xc, Fc));
+    if (g.gtq) {
+      // applyForceUninodal: xc x F + T, T the phys torque (rolling resistance) times the sign
+      const double* t = g.gtq + (int64_t)3 * q;
+      T = add3(T, add3(cross3(xc, Fc), mul3(mk3(t[0], t[1], t[2]), isPA ? 1. : -1.)));
+    } else {
+      T = add3(T, cross3(xc, Fc));
+    }
   }
 }
 
@@ -412,7 +425,8 @@ __global__ __launch_bounds__(FN_BS) void k_facet_nodes(GatherArgs g, const doubl
       }
     }
     if (real) {
-      d3 Fc, cp, shift = mk3(0., 0., 0.);
+      d3 Fc, cp, shift = mk3(0., 0., 0.), Tc = mk3(0., 0., 0.);
+      bool hasT = false;
       if (F32) {
         // the FP32 record: force on pA, and the branches from pA and from pB's image
         const float4* r = reinterpret_cast<const float4*>(reinterpret_cast<const float*>(g.gp) + (int64_t)GP32 * q);
@@ -429,6 +443,12 @@ __global__ __launch_bounds__(FN_BS) void k_facet_nodes(GatherArgs g, const doubl
           const uint64_t cd = (uint64_t)__double_as_longlong(r[3].x);
           shift = mk3(g.L[0] * unpack_cd(cd, 0), g.L[1] * unpack_cd(cd, 1), g.L[2] * unpack_cd(cd, 2));
         }
+        if (g.gtq) {
+          // the phys torque (rolling resistance): node i gets w_i T besides xc x (w_i F)
+          const double* tq = g.gtq + (int64_t)3 * q;
+          hasT = !(tq[0] == 0. && tq[1] == 0. && tq[2] == 0.);
+          Tc = mul3(mk3(tq[0], tq[1], tq[2]), isPA ? 1. : -1.);
+        }
       }
       const d3 p = sub3(cp, mul3(normal, dot3(sub3(cp, v[0]), normal)));
       const d3 vX = sub3(p, v[0]);
@@ -439,7 +459,7 @@ __global__ __launch_bounds__(FN_BS) void k_facet_nodes(GatherArgs g, const doubl
       for (int n = 0; n < 3; n++) {
         const d3 xc = sub3(cp, add3(v[n], shift));
         const d3 F = mul3(Fc, w[n]);
-        const d3 T = cross3(xc, F);
+        const d3 T = hasT ? add3(cross3(xc, F), mul3(Tc, w[n])) : cross3(xc, F);
         cv[3 * n] = F.x; cv[3 * n + 1] = F.y; cv[3 * n + 2] = F.z;
         cv[9 + 3 * n] = T.x; cv[9 + 3 * n + 1] = T.y; cv[9 + 3 * n + 2] = T.z;
       }
@@ -461,6 +481,7 @@ static GatherArgs gather_args(Ctx& c) {
   GatherArgs g{};
   g.n = c.n;
   g.adj = C.adj.p; g.adjOff = C.adjOff.p; g.rv = c.rview(); g.gp = c.reals().gp.p; g.flags = c.flags.p;
+  if (c.rollActive) g.gtq = c.reals().gtq.p;
   g.sidx = c.reals().sidxValid ? c.reals().sidx.p : nullptr;
   g.eidx = c.eidxValid ? C.eidx.p : nullptr;
   g.nW = c.nW;

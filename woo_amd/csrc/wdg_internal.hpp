@@ -163,6 +163,8 @@ struct Reals {
   DBuf<double> gp;
   // material table (epoch) each real contact takes its phys from, while more than one is kept
   DBuf<uint8_t> hE;
+  // global phys torque per real contact (rolling resistance; allocated when it is used)
+  DBuf<double> gtq;
   // state index of every real slot, written by the contact loop's filter (stale for the other
   // slots); valid only for a set that filter produced
   DBuf<uint32_t> sidx;
@@ -180,10 +182,14 @@ struct Reals {
     hE.ensure(c);
     sidx.ensure(c);
   }
+  void ensureRoll(int64_t nslots) {
+    const size_t g = (size_t)ngroups(nslots);
+    gtq.ensure(3 * (g > 0 ? g * GROUP : 1));
+  }
   void release() {
     bits.release(); tpre.release(); gcnt.release();
     for (int k = 0; k < 3; k++) { hN[k].release(); hY[k].release(); hF[k].release(); }
-    huN.release(); gp.release(); sidx.release(); hE.release();
+    huN.release(); gp.release(); sidx.release(); hE.release(); gtq.release();
     sidxValid = false;
   }
 };
@@ -236,7 +242,9 @@ struct Ctx {
   DBuf<wdg_material> dMats;
   std::vector<wdg_material> epTab;  // host copy of dMats: matEpochs tables of mats.size() each
   int matEpochs = 1;
-  int64_t matGen = 0;  // counts the changes of the epochs' numbering
+  int64_t matGen = 0;
+  // the last contact loop had rolling resistance: the real contacts' records have a torque
+  bool rollActive = false;  // counts the changes of the epochs' numbering
   int updatePhys = 0;  // 0 never, 1 always, 2 once (then never)
 
   // particles, storage order
@@ -408,6 +416,10 @@ void collide(Ctx& c, const wdg_collide_params* p);
 // potential pairs (enlarged AABBs of the last refresh overlapping) as user-id pairs (pA,pB) + cellDist
 void potential_pairs(Ctx& c, std::vector<uint32_t>& ida, std::vector<uint32_t>& idb, std::vector<uint64_t>& cd);
 void contact_loop(Ctx& c, const wdg_contact_params* p);
+// Law2_L6Geom_FrictPhys_IdealElPl with rolling and twisting resistance (IdealElPl.cpp:94)
+inline bool rolling(const wdg_contact_params& p) {
+  return p.lawKind == WDG_LAW_IDEALELPL && p.relRollStiff > 0. && p.rollTanPhi > 0.;
+}
 void leapfrog(Ctx& c, const wdg_leapfrog_params* p);
 void force_reset(Ctx& c);
 void flush_forces(Ctx& c);

@@ -425,6 +425,9 @@ wdg_program Scene::program(bool& hostEngines) {
         p.contact.noSlip = iep->noSlip;
         p.contact.noBreak = iep->noBreak;
         p.contact.noFrict = iep->noFrict;
+        p.contact.relRollStiff = iep->relRollStiff;
+        p.contact.relTwistStiff = iep->relTwistStiff;
+        p.contact.rollTanPhi = iep->rollTanPhi;
       } else {
         throw std::runtime_error("ContactLoop: unsupported functor combination (supported: Cp2_FrictMat_FrictPhys + "
                                  "Law2_L6Geom_FrictPhys_IdealElPl, Cp2_HertzMat_HertzPhys + Law2_L6Geom_HertzPhys_DMT "

@@ -163,6 +163,7 @@ struct Cp2_HertzMat_HertzPhys : CPhysFunctor {
 struct LawFunctor : Functor {};
 struct Law2_L6Geom_FrictPhys_IdealElPl : LawFunctor {
   bool noSlip = false, noBreak = false, noFrict = false;  // IdealElPl.hpp:22-24
+  double relRollStiff = 0., relTwistStiff = 0., rollTanPhi = 0.;  // IdealElPl.hpp:18-20
   const char* className() const override { return "Law2_L6Geom_FrictPhys_IdealElPl"; }
 };
 struct Law2_L6Geom_HertzPhys_DMT : LawFunctor {

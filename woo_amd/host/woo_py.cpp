@@ -483,7 +483,10 @@ PYBIND11_MODULE(_woo, m) {
       .def(py::init(&make_kw<Law2_L6Geom_FrictPhys_IdealElPl>))
       .def_readwrite("noSlip", &Law2_L6Geom_FrictPhys_IdealElPl::noSlip)
       .def_readwrite("noBreak", &Law2_L6Geom_FrictPhys_IdealElPl::noBreak)
-      .def_readwrite("noFrict", &Law2_L6Geom_FrictPhys_IdealElPl::noFrict);
+      .def_readwrite("noFrict", &Law2_L6Geom_FrictPhys_IdealElPl::noFrict)
+      .def_readwrite("relRollStiff", &Law2_L6Geom_FrictPhys_IdealElPl::relRollStiff)
+      .def_readwrite("relTwistStiff", &Law2_L6Geom_FrictPhys_IdealElPl::relTwistStiff)
+      .def_readwrite("rollTanPhi", &Law2_L6Geom_FrictPhys_IdealElPl::rollTanPhi);
   py::class_<Law2_L6Geom_HertzPhys_DMT, LawFunctor, std::shared_ptr<Law2_L6Geom_HertzPhys_DMT>>(
       m, "Law2_L6Geom_HertzPhys_DMT", py::dynamic_attr())
       .def(py::init(&make_kw<Law2_L6Geom_HertzPhys_DMT>))
