@@ -287,8 +287,13 @@ int wdg_sync(wdg_ctx* ctx);
  * wdg_set_scenes: nScenes independent scenes sharing this context, each with its own
  * Scene.dt (dt[nScenes]); the contact loop and leapfrog then take dt per particle from its
  * scene (the dt passed with the engine parameters is ignored). nScenes = 0 switches back.
- * Only the engine list [Collider, ContactLoop, Leapfrog] is run in this mode (no DynDt). */
+ * The graph replay runs [Collider, ContactLoop, Leapfrog] in this mode; wdg_run also runs
+ * DynDt, each scene's DynDt::run (DynDt.cpp:90-113) on its own critical timestep and dt. */
 int wdg_set_scenes(wdg_ctx* ctx, int32_t nScenes, const double* dt);
+/* the current dt of each scene (out[nScenes]; DynDt changes them) */
+int wdg_get_scene_dt(wdg_ctx* ctx, double* out);
+/* DynDt::critDt_stiffness (DynDt.cpp:16-71) of each scene (out[nScenes]; inf: none) */
+int wdg_crit_dt_scenes(wdg_ctx* ctx, double* out);
 /* tuning of the engine: workgroups of the force kernel per group of 2048 contact slots (also
  * used by wdg_run and wdg_contact_loop), steps per captured graph (even), and for batched
  * scenes the per-scene Verlet skin: 0 = every scene uses the collider's Verlet distance;

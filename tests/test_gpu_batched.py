@@ -185,3 +185,28 @@ def test_graph_replay_follows_field_and_material_changes():
         out.append({f: g.d.get(f) for f in ("pos", "vel", "angVel")})
     for f in out[0]:
         np.testing.assert_array_equal(out[1][f], out[0][f], err_msg=f)
+
+
+def test_batched_dyndt_per_scene_equals_serial_runs():
+    # DynDt in a batched context: each scene's DynDt::run (DynDt.cpp:90-113) on its own critical
+    # timestep and dt, exactly as the scene run alone (the scenes' stiffnesses differ, so do their
+    # timesteps)
+    eng = ENG + ["DynDt"]
+    scenes = scenes_c4_like(k=4, seed=66)
+    for sc in scenes:
+        sc["dt"] *= 0.3  # below the critical one, so that DynDt raises it step by step
+    steps = 240
+    kw = dict(leapfrog=dict(damping=scenes[0]["damping"]), dynDtPeriod=20, maxRelInc=1e-2, dtSafety=.7)
+    d, _ = build_batch(scenes)
+    d.run(eng, steps, 0, scenes[0]["dt"], contact=contact_kw(scenes[0]), **kw)
+    dts = d.scene_dt()
+    crit = d.crit_dt_scenes()
+    P = d.get("pos")
+    print("scene dt", dts, "critical", crit)
+    for q, (sc, sl) in enumerate(zip(scenes, slices(scenes))):
+        g = GpuRunner(sc)
+        _, dt1 = g.d.run(eng, steps, 0, sc["dt"], contact=contact_kw(sc), **kw)
+        assert dts[q] == dt1 and dt1 != sc["dt"]
+        assert crit[q] == g.d.crit_dt()
+        np.testing.assert_array_equal(P[sl], g.d.get("pos"))
+    assert len(set(dts.tolist())) == len(scenes)

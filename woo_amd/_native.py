@@ -31,7 +31,7 @@ EXPORTS = [
     "wdg_mpnn_set_targets", "wdg_mpnn_keep_graph", "wdg_mpnn_use_graph", "wdg_mpnn_get_graph",
     "wdg_set_energy_tracking", "wdg_energy", "wdg_energy_clear", "wdg_restore_contacts", "wdg_energy_set",
     "wdg_kinetic_energy_noalloc", "wdg_intra_force", "wdg_facet_nodes", "wdg_set_update_phys",
-    "wdg_get_update_phys",
+    "wdg_get_update_phys", "wdg_get_scene_dt", "wdg_crit_dt_scenes",
 ]
 
 # Scene.energy terms, in WDG_EN_* order
@@ -123,6 +123,8 @@ def lib():
     L.wdg_force_reset.argtypes = [V]
     L.wdg_intra_force.argtypes = [V]
     L.wdg_set_update_phys.argtypes = [V, C.c_int32]
+    L.wdg_get_scene_dt.argtypes = [V, _P(C.c_double)]
+    L.wdg_crit_dt_scenes.argtypes = [V, _P(C.c_double)]
     L.wdg_get_update_phys.argtypes = [V, _P(C.c_int32)]
     L.wdg_facet_nodes.argtypes = [V, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_double)]
     L.wdg_crit_dt.argtypes = [V, _P(C.c_double)]
@@ -447,6 +449,18 @@ class Device:
         a = np.ascontiguousarray(dt, dtype=np.float64)
         check(self.L.wdg_set_scenes(self.h, len(a), dptr(a) if len(a) else None))
         self._scenes = a.copy()
+
+    def scene_dt(self):
+        """current dt of each batched scene"""
+        out = np.empty(0 if self._scenes is None else len(self._scenes), dtype=np.float64)
+        check(self.L.wdg_get_scene_dt(self.h, dptr(out) if len(out) else None))
+        return out
+
+    def crit_dt_scenes(self):
+        """DynDt's critical timestep of each batched scene"""
+        out = np.empty(0 if self._scenes is None else len(self._scenes), dtype=np.float64)
+        check(self.L.wdg_crit_dt_scenes(self.h, dptr(out) if len(out) else None))
+        return out
 
     def tune(self, engines, step, dt, **kw):
         """autotuner; returns (tuning dict, new step, steps taken)"""

@@ -904,6 +904,7 @@ int wdg_set_scenes(wdg_ctx* ctx, int32_t nScenes, const double* dt) {
     scene_bodies(c);
     c.dtScene.ensure((size_t)std::max(nScenes, 1));
     if (nScenes) WDG_HIP(hipMemcpy(c.dtScene.p, dt, nScenes * sizeof(double), hipMemcpyHostToDevice));
+    c.dtSceneHost.assign(dt, dt + nScenes);
     c.gen++;
   });
 }
@@ -1035,8 +1036,10 @@ int wdg_run(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t* step
   return guard(ctx, [&] {
     Ctx& c = ctx->c;
     if (c.trackEnergy && c.nScenes > 1) throw Error(WDG_E_STATE, "energy tracking needs a context of one scene");
+    std::vector<double> crs, nextScene;
     for (int64_t s = 0; s < nSteps; s++) {
       double nextDt = NAN;
+      bool sceneDt = false;
       c.stepNow = *step;
       if (c.trackEnergy) energy_step_begin(c);
       for (int e = 0; e < prog->nEngines; e++) {
@@ -1063,6 +1066,23 @@ int wdg_run(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t* step
             if ((*step - c.dynStepLast >= per) || (*step % per == 0)) {
               c.dynStepPrev = c.dynStepLast;
               c.dynStepLast = *step;
+              if (c.nScenes > 0) {
+                // batched scenes: DynDt::run of each scene with its own dt (every scene is
+                // stepped by this loop, so all share the activation steps)
+                crit_dt_scenes(c, crs);
+                const int64_t ns = *step - c.dynStepPrev;
+                nextScene.assign(crs.size(), NAN);
+                for (size_t q = 0; q < crs.size(); q++) {
+                  if (std::isinf(crs[q])) continue;
+                  const double d0 = c.dtSceneHost[q];
+                  const double nd = std::min(crs[q] * prog->dtSafety, d0 * pown(1. + prog->maxRelInc, (unsigned)ns));
+                  if (nd != d0) {
+                    nextScene[q] = nd;
+                    sceneDt = true;
+                  }
+                }
+                break;
+              }
               // DynDt::run (DynDt.cpp:90-113)
               double cr = crit_dt(c);
               if (std::isinf(cr)) break;
@@ -1077,8 +1097,31 @@ int wdg_run(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t* step
       }
       *step += 1;
       if (!std::isnan(nextDt)) *dt = nextDt;
+      if (sceneDt) {
+        for (size_t q = 0; q < nextScene.size(); q++)
+          if (!std::isnan(nextScene[q])) c.dtSceneHost[q] = nextScene[q];
+        WDG_HIP(hipMemcpyAsync(c.dtScene.p, c.dtSceneHost.data(), c.dtSceneHost.size() * sizeof(double),
+                               hipMemcpyHostToDevice, c.stream));
+        WDG_HIP(hipStreamSynchronize(c.stream));
+      }
     }
   });
+}
+
+int wdg_crit_dt_scenes(wdg_ctx* ctx, double* out) {
+  if (!ctx || !out) { g_err = "NULL argument"; return WDG_E_ARG; }
+  return guard(ctx, [&] {
+    std::vector<double> v;
+    crit_dt_scenes(ctx->c, v);
+    std::copy(v.begin(), v.end(), out);
+  });
+}
+
+int wdg_get_scene_dt(wdg_ctx* ctx, double* out) {
+  if (!ctx || !out) { g_err = "NULL argument"; return WDG_E_ARG; }
+  if (ctx->c.nScenes <= 0) { g_err = "not a batched context (wdg_set_scenes)"; return WDG_E_STATE; }
+  std::copy(ctx->c.dtSceneHost.begin(), ctx->c.dtSceneHost.end(), out);
+  return WDG_OK;
 }
 
 int wdg_relax(wdg_ctx* ctx, int32_t maxIter, double inflate, double tol, double* maxOverlap, int32_t* iters) {

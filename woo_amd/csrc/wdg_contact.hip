@@ -333,7 +333,7 @@ __device__ __forceinline__ d3 rolling_torque(const ContactArgs& a, double kn, do
 }
 
 template <bool PERIODIC, int LAW, bool FACETS, bool ENERGY>
-__global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
+__global__ __launch_bounds__(CF_BS, FACETS ? 3 : 4) void k_cforce(ContactArgs a) {
   __shared__ wdg_material smat[MAT_LDS];
   if (a.gstop && *a.gstop) return;
   stage_materials(a, smat);
@@ -362,7 +362,15 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
     const d3 vel2 = mk3(a.vx[pB], a.vy[pB], a.vz[pB]);
     const d3 av2 = mk3(a.wx[pB], a.wy[pB], a.wz[pB]);
     const double massA = a.mass[pA], massB = a.mass[pB];
-    const uint32_t matA = (uint32_t)a.mat[pA], matB = (uint32_t)a.mat[pB];
+    // material rows of the contact's epoch (the one it was created in, or the current one):
+    // folded into the indices here, so that no extra register lives through the geometry
+    uint32_t matA = (uint32_t)a.mat[pA], matB = (uint32_t)a.mat[pB];
+    if (a.nE) {
+      const uint32_t ep = (real && !a.updAll) ? (uint32_t)a.oE[ko] : a.epoch;
+      a.nE[k] = (uint8_t)ep;
+      matA += ep * (uint32_t)a.nMats;
+      matB += ep * (uint32_t)a.nMats;
+    }
     const d3 prevNormalL = mk3(a.oN0[ko], a.oN1[ko], a.oN2[ko]);
     const d3 prevYL = mk3(a.oY0[ko], a.oY1[ko], a.oY2[ko]);
     const double oFt0 = a.oF1[ko], oFt1 = a.oF2[ko];
@@ -490,14 +498,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce(ContactArgs a) {
     // Cp2 (recomputed; constant over the life of the contact)
     double l0, l1, A;
     lens_area(r1, r2, l0, l1, A);
-    // the contact's material epoch: the one it was created in, or the current one
-    uint32_t eb = 0u;
-    if (a.nE) {
-      const uint32_t ep = (real && !a.updAll) ? (uint32_t)a.oE[ko] : a.epoch;
-      a.nE[k] = (uint8_t)ep;
-      eb = ep * (uint32_t)a.nMats;
-    }
-    const wdg_material mA = material(a, smat, eb + matA), mB = material(a, smat, eb + matB);
+    const wdg_material mA = material(a, smat, matA), mB = material(a, smat, matB);
     double Fn, Ft0 = real ? oFt0 : 0., Ft1 = real ? oFt1 : 0.;
     d3 tq = mk3(0., 0., 0.);  // CPhys.torque (local), rolling resistance only
     if (LAW != WDG_LAW_HERTZDMT) {
@@ -715,6 +716,13 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
     const uint32_t info = dk.x, pA = dk.y, pB = dk.z;
     const bool real = dk.w != NO_STATE;
     const uint32_t ko = real ? dk.w : 0u;
+    // material rows of the contact's epoch (folded into the particles' material indices)
+    uint32_t eb = 0u;
+    if (a.nE) {
+      const uint32_t ep = (real && !a.updAll) ? (uint32_t)a.oE[ko] : a.epoch;
+      a.nE[k] = (uint8_t)ep;
+      eb = ep * (uint32_t)a.nMats;
+    }
     const float dt = (float)(a.dtScene ? a.dtScene[a.scene[pA]] : a.dt);
     uint64_t cd = 0;
     d3 shift2 = mk3(0., 0., 0.);
@@ -824,12 +832,6 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
     // Cp2 parameters in FP64 (functions of radii, materials and masses), the law in float
     double l0, l1, A;
     lens_area((double)r1, (double)r2, l0, l1, A);
-    uint32_t eb = 0u;
-    if (a.nE) {
-      const uint32_t ep = (real && !a.updAll) ? (uint32_t)a.oE[ko] : a.epoch;
-      a.nE[k] = (uint8_t)ep;
-      eb = ep * (uint32_t)a.nMats;
-    }
     const wdg_material mA = a.mats[eb + a.mat[pA]], mB = a.mats[eb + a.mat[pB]];
     float Fn, Ft0 = real ? oF1[ko] : 0.f, Ft1 = real ? oF2[ko] : 0.f;
     if (LAW == WDG_LAW_IDEALELPL) {

@@ -614,6 +614,7 @@ struct DynArgs {
   int hertz;
   double poisson, en, viscAlpha;
   unsigned long long* out;
+  const uint32_t* scene;  // batched scenes: the minimum per scene, out[scene] (else one, out[0])
 };
 
 template <int PREC>
@@ -676,18 +677,27 @@ __global__ __launch_bounds__(256) void k_dyndt(DynArgs a) {
   }
   // non-negative doubles order like their bit patterns
   unsigned long long b = (unsigned long long)__double_as_longlong(ret);
+  uint32_t sc = 0u;
+  if (a.scene) {
+    // scenes lie in runs of storage order: a wavefront of one scene reduces first, a mixed one
+    // (or the tail past n) takes one atomic per particle
+    sc = i < a.n ? a.scene[i] : 0xFFFFFFFFu;
+    const uint32_t s0 = __shfl(sc, 0);
+    if (!__all(sc == s0)) {
+      if (i < a.n && ret != INFINITY) atomicMin(&a.out[sc], b);
+      return;
+    }
+    if (s0 == 0xFFFFFFFFu) return;
+  }
   for (int off = 32; off > 0; off >>= 1) {
     unsigned long long o = __shfl_xor(b, off);
     b = o < b ? o : b;
   }
-  if ((threadIdx.x & 63) == 0) atomicMin(a.out, b);
+  if ((threadIdx.x & 63) == 0) atomicMin(&a.out[sc], b);
 }
 
-double crit_dt(Ctx& c) {
-  if (c.nScenes > 0) throw Error(WDG_E_STATE, "DynDt is not available for batched scenes");
-  if (c.n == 0) return INFINITY;
+static DynArgs dyn_args(Ctx& c) {
   Contacts& C = c.con();
-  if (C.n == 0) return INFINITY;
   DynArgs a{};
   a.n = c.n;
   Reals& R = c.reals();
@@ -700,6 +710,14 @@ double crit_dt(Ctx& c) {
   if (c.matEpochs > 1) a.hE = R.hE.p;
   a.hertz = c.lastPhysKind == WDG_PHYS_HERTZ;
   a.poisson = c.lastPoisson; a.en = c.lastEn; a.viscAlpha = c.lastViscAlpha;
+  return a;
+}
+
+double crit_dt(Ctx& c) {
+  if (c.nScenes > 0) throw Error(WDG_E_STATE, "batched scenes: DynDt's timestep is per scene (wdg_crit_dt_scenes)");
+  if (c.n == 0) return INFINITY;
+  if (c.con().n == 0) return INFINITY;
+  DynArgs a = dyn_args(c);
   unsigned long long* out = reinterpret_cast<unsigned long long*>(c.red.p);
   unsigned long long init = (unsigned long long)0x7FF0000000000000ull;  // +inf
   WDG_HIP(hipMemcpyAsync(out, &init, sizeof(init), hipMemcpyHostToDevice, c.stream));
@@ -713,6 +731,32 @@ double crit_dt(Ctx& c) {
   double r;
   std::memcpy(&r, &hb, 8);
   return std::sqrt(r);
+}
+
+// DynDt::critDt_stiffness of each scene of a batched context: the same per-particle values,
+// their minimum taken per scene (a minimum does not depend on the order)
+void crit_dt_scenes(Ctx& c, std::vector<double>& out) {
+  if (c.nScenes <= 0) throw Error(WDG_E_STATE, "not a batched context (wdg_set_scenes)");
+  out.assign((size_t)c.nScenes, INFINITY);
+  if (c.n == 0 || c.con().n == 0) return;
+  DynArgs a = dyn_args(c);
+  DBuf<unsigned long long> d;
+  d.ensure((size_t)c.nScenes);
+  std::vector<unsigned long long> h((size_t)c.nScenes, 0x7FF0000000000000ull);  // +inf
+  WDG_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * 8, hipMemcpyHostToDevice, c.stream));
+  a.out = d.p;
+  a.scene = c.scene.p;
+  hipLaunchKernelGGL(c.precision == 32 ? k_dyndt<32> : k_dyndt<64>, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream,
+                     a);
+  WDG_LAUNCH_CHECK();
+  WDG_HIP(hipMemcpyAsync(h.data(), d.p, h.size() * 8, hipMemcpyDeviceToHost, c.stream));
+  WDG_HIP(hipStreamSynchronize(c.stream));
+  d.release();
+  for (size_t s = 0; s < h.size(); s++) {
+    double r;
+    std::memcpy(&r, &h[s], 8);
+    out[s] = std::sqrt(r);
+  }
 }
 
 // ------------------------------------------------------------------ observables

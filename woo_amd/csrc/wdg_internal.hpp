@@ -275,6 +275,7 @@ struct Ctx {
   std::vector<uint32_t> bodyScene;        // scene of each boundary body (host)
   DBuf<uint32_t> sceneBodyOff, sceneBody;  // boundary bodies by scene (batched scenes)
   DBuf<double> dtScene;
+  std::vector<double> dtSceneHost;  // the same, on the host (DynDt updates both)
   // per-scene Verlet distances of the last refresh (tuning.skinSteps > 0), and scratch for the
   // scenes' largest speeds
   DBuf<double> vdScene;
@@ -429,6 +430,8 @@ void run_graph(Ctx& c, const wdg_program* prog, int64_t nSteps, int64_t* step, d
 uint32_t read_u32(Ctx& c, const uint32_t* d);
 int64_t tune(Ctx& c, const wdg_program* prog, int64_t* step, double* dt);
 double crit_dt(Ctx& c);
+// DynDt's critical timestep of every scene of a batched context (inf: no stiffness)
+void crit_dt_scenes(Ctx& c, std::vector<double>& out);
 void observables(Ctx& c, const double lo[3], const double hi[3], double out[3]);
 int64_t count_real(Ctx& c);
 void stress(Ctx& c, double volume, double out[9]);
