@@ -195,6 +195,13 @@ int wdg_contact_loop(wdg_ctx* ctx, const wdg_contact_params* p);
 #define WDG_UPDATE_PHYS_ONCE 2
 int wdg_set_update_phys(wdg_ctx* ctx, int32_t mode);
 int wdg_get_update_phys(wdg_ctx* ctx, int32_t* mode);
+/* ContactLoop.evalStress (src/dem/ContactLoop.hpp:117, ContactLoop.cpp:117,123,216-229,243-254):
+ * in a periodic cell each contact loop also sums F branch^T over its real contacts (F the
+ * global contact force, branch = Contact::dPos) and divides by the cell volume; read with
+ * wdg_contact_loop_stress (ContactLoop.stress, row-major; zero when not evaluated). Cell.gradV
+ * is zero here, so the gradV work it feeds is zero. */
+int wdg_set_eval_stress(wdg_ctx* ctx, int32_t on);
+int wdg_contact_loop_stress(wdg_ctx* ctx, double out[9]);
 int wdg_leapfrog(wdg_ctx* ctx, const wdg_leapfrog_params* p);
 int wdg_force_reset(wdg_ctx* ctx);
 /* IntraForce::run (src/dem/IntraForce.cpp:26-42) with In2_Facet (src/dem/Facet.cpp:301-328,

@@ -227,6 +227,7 @@ struct orc_scene {
   std::vector<std::mutex> locks;
   // Scene.energy, ORC_E_* order
   double energy[ORC_E_N] = {0, 0, 0, 0, 0, 0, 0};
+  double clStress[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // ContactLoop.stress of the last contact loop
 
   // ---------------------------------------------------------- contacts
   int find(int a, int b) const {
@@ -1065,6 +1066,22 @@ struct orc_scene {
       if (!m.empty()) throw std::runtime_error(m);
     // ContactLoop.cpp:265-266
     if (prm.updatePhys == 2) prm.updatePhys = 0;
+    // ContactLoop.cpp:117,123,216-229,243-244: stress = sum of F branch^T over the real contacts,
+    // over the cell volume (periodic cells only); the gradV work is zero (gradV = 0)
+    for (double& x : clStress) x = 0.;
+    if (prm.evalStress && prm.periodic) {
+      for (const auto& C : con) {
+        if (!C.isReal()) continue;
+        if (par[C.pA].shape == ORC_FACET || par[C.pB].shape == ORC_FACET)
+          throw std::runtime_error("ContactLoop.evalStress: multi-nodal particles in contact");
+        const V3 b = sub(add(ld3(par[C.pB].pos), shiftPos(C.cellDist)), ld3(par[C.pA].pos));
+        const V3 F = quatRot(C.nodeOri, C.force);
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) clStress[3 * i + j] += F[i] * b[j];
+      }
+      const double V = prm.cell[0] * prm.cell[1] * prm.cell[2];
+      for (double& x : clStress) x /= V;
+    }
     for (auto& v : pendT)
       for (auto& pr : v) pending.push_back(pr);
     if (prm.trackEnergy) {
@@ -1546,6 +1563,11 @@ double orc_kinetic_energy(const orc_scene* s) {
 // ContactContainer order, d0 = |posA - shift - posB| with the node positions,
 // n = ori*e_x, F = ori*force, sigma_ij += d0 (fN n_i n_j + (fT_i n_j + fT_j n_i)/2),
 // divided by the volume (the periodic cell's when volume <= 0)
+int orc_cl_stress(const orc_scene* s, double out[9]) {
+  for (int k = 0; k < 9; k++) out[k] = s->clStress[k];
+  return 0;
+}
+
 int orc_stress(const orc_scene* s, double volume, double out[9]) {
   double S[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   for (const auto& C : s->con) {

@@ -87,6 +87,8 @@ typedef struct {
   /* Law2_L6Geom_FrictPhys_IdealElPl rolling and twisting resistance (IdealElPl.hpp:18-20);
    * off unless relRollStiff > 0 and rollTanPhi > 0 */
   double relRollStiff, relTwistStiff, rollTanPhi;
+  /* ContactLoop.evalStress (ContactLoop.hpp:117): the contact loop's stress in periodic cells */
+  int evalStress;
 } orc_params;
 
 /* Scene.energy (EnergyTracker, src/core/EnergyTracker.hpp:30-61): the terms the path's engines
@@ -147,6 +149,10 @@ double orc_coord_number(const orc_scene* s, int skipFree);
 double orc_solid_fraction(const orc_scene* s, const double boxMin[3], const double boxMax[3]);
 double orc_kinetic_energy(const orc_scene* s);
 int orc_stress(const orc_scene* s, double volume, double out[9]);
+/* ContactLoop.stress (ContactLoop.cpp:117,216-229,243-244): sum over the real contacts of the
+ * last contact loop of F branch^T (F = node.ori * force, branch = Contact::dPos) over the cell
+ * volume, row-major; zero unless evalStress in a periodic cell */
+int orc_cl_stress(const orc_scene* s, double out[9]);
 double orc_pwave_dt(const orc_scene* s);
 /* DynDt::critDt_stiffness on the current contact state (no side effects) */
 double orc_crit_dt(const orc_scene* s);

@@ -28,6 +28,7 @@ class OrcParams(C.Structure):
         ("dynDtPeriod", C.c_int), ("maxRelInc", C.c_double), ("trackEnergy", C.c_int),
         ("updatePhys", C.c_int),
         ("relRollStiff", C.c_double), ("relTwistStiff", C.c_double), ("rollTanPhi", C.c_double),
+        ("evalStress", C.c_int),
     ]
 
 # Scene.energy terms, in the order orc_energy and wdg_energy give them
@@ -98,6 +99,7 @@ def _load(path=None):
     L.orc_energy.argtypes = [P, C.POINTER(C.c_double)]
     L.orc_set_flags.argtypes = [P, C.c_int64, C.c_uint32]
     L.orc_stress.argtypes = [P, C.c_double, D]; L.orc_stress.restype = C.c_int
+    L.orc_cl_stress.argtypes = [P, D]; L.orc_cl_stress.restype = C.c_int
     L.orc_pwave_dt.argtypes = [P]; L.orc_pwave_dt.restype = C.c_double
     L.orc_crit_dt.argtypes = [P]; L.orc_crit_dt.restype = C.c_double
     L.orc_facet_nodes.argtypes = [P, C.c_int64, D]; L.orc_facet_nodes.restype = C.c_int
@@ -269,6 +271,12 @@ class Oracle:
         out = np.zeros(9)
         if self.L.orc_stress(self.h, volume, _dp(out)) != 0:
             raise RuntimeError(self.L.orc_last_error().decode())
+        return out.reshape(3, 3)
+
+    def cl_stress(self):
+        """ContactLoop.stress (3x3) of the last contact loop (evalStress in a periodic cell)."""
+        out = np.zeros(9)
+        self.L.orc_cl_stress(self.h, _dp(out))
         return out.reshape(3, 3)
 
     def pwave_dt(self):

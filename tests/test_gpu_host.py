@@ -354,3 +354,15 @@ def test_contact_loop_update_phys(up):
     else:
         assert S.lab.contactLoop.updatePhys == "never"
         assert kn1 == c.phys.kn
+
+
+def test_scripted_contact_loop_eval_stress():
+    # ContactLoop(evalStress=True).stress through the scripting surface, against the oracle
+    sc = periodic_hertz(n=800, spacing=0.92, seed=36)
+    S = host_scene(sc)
+    S.lab.contactLoop.evalStress = True
+    o = build_oracle(sc, evalStress=1)
+    S.run(40)
+    o.run(MINIMAL, 40)
+    so = o.cl_stress()
+    np.testing.assert_allclose(S.lab.contactLoop.stress, so, rtol=0, atol=1e-10 * np.max(np.abs(so)))

@@ -231,10 +231,13 @@ def test_contact_loop_update_phys_attribute(woo):
     with pytest.raises(ValueError):
         cl.updatePhys = 3
     assert cl.dist00 is True and cl.reorderEvery == 1000
+    assert cl.evalStress is False and cl.stress.shape == (3, 3) and not cl.stress.any()
+    cl.evalStress = True
     cl.reorderEvery = 50
     S.saveTmp("up", quiet=True)
     S2 = woo.core.Scene.loadTmp("up")
     assert S2.lab.contactLoop.updatePhys == "always" and S2.lab.contactLoop.reorderEvery == 50
+    assert S2.lab.contactLoop.evalStress is True
     # DemField(par=...) fills an empty field only
     with pytest.raises(ValueError):
         S.dem.par = [Sphere.make((0, 0, 5), .1, mat=m)]

@@ -340,6 +340,36 @@ def test_stress_tangential_term_symmetric():
     np.testing.assert_allclose(sig, sig.T, rtol=0, atol=1e-15 * np.max(np.abs(sig)))
 
 
+def test_contact_loop_stress_parts():
+    # ContactLoop.stress (evalStress, ContactLoop.cpp:216-229,243-244) is sum F branch^T / V at the
+    # contact loop's positions. Its symmetric part is DemFuncs' stress (branch = d0 n for spheres);
+    # its antisymmetric part is the torque the contacts put on the particles: sum F x b =
+    # -sum b x F, and b x F = d0 n x F = (l0 + l1) n x F is the torque on pA and pB together
+    # (contact point at l0 from A, l1 from B). After a reset and a contact loop alone, the
+    # particles' torques are those contact torques and their positions the loop's. DemFuncs takes
+    # the normal from node.ori, the quaternion of the contact's frame, whose axes are kept
+    # orthonormal only every trsfRenorm steps: the two agree to about 1e-9, not to roundoff.
+    sc = periodic_hertz(n=600, spacing=0.92, seed=34, vel_sigma=1.)
+    o = build_oracle(sc, evalStress=1)
+    o.run(ENG, 30)
+    o.run(["ForceResetter", "Collider", "ContactLoop"], 1)
+    S = o.cl_stress()
+    V = float(np.prod(sc["cell"]))
+    sym = .5 * (S + S.T)
+    sig = o.stress()
+    scale = float(np.max(np.abs(sig)))
+    assert scale > 0
+    np.testing.assert_allclose(sym, sig, rtol=0, atol=1e-7 * scale)
+    w = V * np.array([S[1, 2] - S[2, 1], S[2, 0] - S[0, 2], S[0, 1] - S[1, 0]])
+    tsum = o.get("torque").sum(axis=0)
+    assert np.max(np.abs(tsum)) > 1e-6 * scale * V / sc["cell"][0]
+    np.testing.assert_allclose(w, -tsum, rtol=0, atol=1e-10 * np.max(np.abs(tsum)))
+    # off (the default) or in an aperiodic scene it is zero
+    o2 = build_oracle(sc)
+    o2.run(ENG, 2)
+    assert not o2.cl_stress().any()
+
+
 @pytest.mark.parametrize("seed", [1, 2])
 def test_sap_incremental_equals_full_rebuild_and_bruteforce(seed):
     sc = deposition(n=1200, dense=True, seed=seed, vel_sigma=0.3)

@@ -31,7 +31,8 @@ EXPORTS = [
     "wdg_mpnn_set_targets", "wdg_mpnn_keep_graph", "wdg_mpnn_use_graph", "wdg_mpnn_get_graph",
     "wdg_set_energy_tracking", "wdg_energy", "wdg_energy_clear", "wdg_restore_contacts", "wdg_energy_set",
     "wdg_kinetic_energy_noalloc", "wdg_intra_force", "wdg_facet_nodes", "wdg_set_update_phys",
-    "wdg_get_update_phys", "wdg_get_scene_dt", "wdg_crit_dt_scenes",
+    "wdg_get_update_phys", "wdg_get_scene_dt", "wdg_crit_dt_scenes", "wdg_set_eval_stress",
+    "wdg_contact_loop_stress",
 ]
 
 # Scene.energy terms, in WDG_EN_* order
@@ -124,6 +125,8 @@ def lib():
     L.wdg_intra_force.argtypes = [V]
     L.wdg_set_update_phys.argtypes = [V, C.c_int32]
     L.wdg_get_scene_dt.argtypes = [V, _P(C.c_double)]
+    L.wdg_set_eval_stress.argtypes = [V, C.c_int32]
+    L.wdg_contact_loop_stress.argtypes = [V, _P(C.c_double)]
     L.wdg_crit_dt_scenes.argtypes = [V, _P(C.c_double)]
     L.wdg_get_update_phys.argtypes = [V, _P(C.c_int32)]
     L.wdg_facet_nodes.argtypes = [V, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_double)]
@@ -449,6 +452,16 @@ class Device:
         a = np.ascontiguousarray(dt, dtype=np.float64)
         check(self.L.wdg_set_scenes(self.h, len(a), dptr(a) if len(a) else None))
         self._scenes = a.copy()
+
+    def set_eval_stress(self, on):
+        """ContactLoop.evalStress: the contact loop's own stress in periodic cells"""
+        check(self.L.wdg_set_eval_stress(self.h, int(bool(on))))
+
+    def contact_loop_stress(self):
+        """ContactLoop.stress of the last contact loop (3x3; zero when not evaluated)"""
+        out = np.zeros(9)
+        check(self.L.wdg_contact_loop_stress(self.h, dptr(out)))
+        return out.reshape(3, 3)
 
     def scene_dt(self):
         """current dt of each batched scene"""

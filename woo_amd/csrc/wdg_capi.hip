@@ -944,6 +944,23 @@ int wdg_set_update_phys(wdg_ctx* ctx, int32_t mode) {
   return WDG_OK;
 }
 
+int wdg_set_eval_stress(wdg_ctx* ctx, int32_t on) {
+  if (!ctx) { g_err = "null argument"; return WDG_E_ARG; }
+  ctx->c.evalStress = on != 0;
+  return WDG_OK;
+}
+
+int wdg_contact_loop_stress(wdg_ctx* ctx, double out[9]) {
+  if (!ctx || !out) { g_err = "null argument"; return WDG_E_ARG; }
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    for (int k = 0; k < 9; k++) out[k] = 0.;
+    if (!c.clStressOn) return;
+    WDG_HIP(hipMemcpyAsync(out, c.clStress.p, 9 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    WDG_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
 int wdg_get_update_phys(wdg_ctx* ctx, int32_t* mode) {
   if (!ctx || !mode) { g_err = "null argument"; return WDG_E_ARG; }
   *mode = ctx->c.updatePhys;

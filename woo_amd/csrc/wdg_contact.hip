@@ -967,6 +967,48 @@ __global__ void k_fill_u32(int64_t n, uint32_t* p, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
+// ------------------------------------------------------------------ ContactLoop.stress
+// ContactLoop.cpp:216-229: per real contact F branch^T, F = node.ori * force (the gather
+// record's global force), branch = Contact::dPos = pB + periodic shift - pA, at the contact
+// loop's positions. One workgroup per group of slots (its real contacts are state_index(g, 0 ..
+// gcnt[g]-1)), a fixed-order workgroup sum per group; k_cl_stress_sum adds the groups in order.
+template <bool PERIODIC>
+__global__ __launch_bounds__(256) void k_cl_stress(ContactArgs a, int64_t ng, double* part) {
+  if (a.gstop && *a.gstop) return;
+  const int64_t g = blockIdx.x;
+  const uint32_t cnt = a.ngcnt[g];
+  double v[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+  for (uint32_t loc = threadIdx.x; loc < cnt; loc += blockDim.x) {
+    const int64_t k = state_index((uint32_t)g, loc);
+    const uint4 dk = a.desc[k];
+    const uint32_t pA = dk.y, pB = dk.z;
+    d3 b = sub3(mk3(a.px[pB], a.py[pB], a.pz[pB]), mk3(a.px[pA], a.py[pA], a.pz[pA]));
+    if (PERIODIC) {
+      const uint64_t cd = a.descDist[k];
+      b = add3(b, mk3(a.L[0] * unpack_cd(cd, 0), a.L[1] * unpack_cd(cd, 1), a.L[2] * unpack_cd(cd, 2)));
+    }
+    const double2* r = reinterpret_cast<const double2*>(a.gp + (int64_t)GP * k);
+    const double2 r0 = r[0], r1 = r[1];
+    const double F[3] = {r0.x, r0.y, r1.x}, B[3] = {b.x, b.y, b.z};
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) v[3 * i + j] += F[i] * B[j];
+  }
+  block_sum_store<9, 256>(v, part + 9 * g);
+}
+__global__ __launch_bounds__(256) void k_cl_stress_sum(int64_t ng, const double* __restrict__ part, double vol,
+                                                       const uint32_t* gstop, double* out) {
+  if (gstop && *gstop) return;
+  double v[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+  for (int64_t g = threadIdx.x; g < ng; g += blockDim.x)
+#pragma unroll
+    for (int q = 0; q < 9; q++) v[q] += part[9 * g + q];
+  block_sum_store<9, 256>(v, out);
+  __syncthreads();
+  if (threadIdx.x < 9) out[threadIdx.x] = out[threadIdx.x] / vol;
+}
+
 // ------------------------------------------------------------------ material epochs
 // which epochs the real contacts refer to (256 bits); a workgroup per group of slots, whose
 // real contacts hold the state indices state_index(g, 0 .. gcnt[g]-1)
@@ -1127,6 +1169,7 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
     if (c.trackEnergy) energy_zero(c, WDG_EN_ELAST);
     c.forcesPending = false;
     c.rollActive = false;
+    c.clStressOn = false;  // no contact: the stress is zero
     return;
   }
   const int from = c.rcur, to = 1 - c.rcur;
@@ -1188,6 +1231,20 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
     energy_add(c, gforce.x, 4, tgt, 1);
   } else {
     launch_cforce<false>(c, a, gforce, c.precision == 32, hertz, roll, per, fac);
+  }
+  // ContactLoop.evalStress (ContactLoop.cpp:123): the stress of this loop's real contacts, in
+  // periodic cells (gradV is zero, so there is no gradV work to add)
+  c.clStressOn = c.evalStress && c.field.periodic;
+  if (c.clStressOn) {
+    if (c.nFacet > 0) throw Error(WDG_E_ARG, "ContactLoop.evalStress with facets (multi-nodal particles)");
+    c.clPart.ensure((size_t)9 * ng);
+    c.clStress.ensure(9);
+    hipLaunchKernelGGL(k_cl_stress<true>, dim3((unsigned)ng), dim3(256), 0, c.stream, a, ng, c.clPart.p);
+    WDG_LAUNCH_CHECK();
+    const double vol = c.field.cell[0] * c.field.cell[1] * c.field.cell[2];
+    hipLaunchKernelGGL(k_cl_stress_sum, dim3(1), dim3(256), 0, c.stream, ng, (const double*)c.clPart.p, vol,
+                       c.graphCapture ? (const uint32_t*)c.gstate.p : nullptr, c.clStress.p);
+    WDG_LAUNCH_CHECK();
   }
   c.R[to].sidxValid = true;
   c.eidxValid = a.eidx != nullptr;

@@ -141,6 +141,7 @@ static void capture(Ctx& c, const wdg_program* prog, int K, int64_t step, double
   G.intra = has_intra(prog);
   G.epochs = c.matEpochs;
   G.updAll = c.updatePhys != 0;
+  G.evalStress = c.evalStress;
 }
 
 static bool graph_valid(const Ctx& c, const wdg_program* prog, int K, double dt) {
@@ -149,6 +150,7 @@ static bool graph_valid(const Ctx& c, const wdg_program* prog, int K, double dt)
   const wdg_contact_params& b = prog->contact;
   return G.exec && G.gen == c.gen && G.K == K && G.cfBlocks == c.tune.cfBlocks && G.track == c.trackEnergy &&
          G.intra == has_intra(prog) && G.epochs == c.matEpochs && G.updAll == (c.updatePhys != 0) &&
+         G.evalStress == c.evalStress &&
          G.rcur == c.rcur &&
          G.parity == c.lfParity && G.dt == dt && G.damping == prog->leapfrog.damping && a.physKind == b.physKind &&
          a.lawKind == b.lawKind && a.applyForces == b.applyForces && a.trsfRenorm == b.trsfRenorm &&

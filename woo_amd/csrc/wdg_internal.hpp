@@ -244,7 +244,11 @@ struct Ctx {
   int matEpochs = 1;
   int64_t matGen = 0;
   // the last contact loop had rolling resistance: the real contacts' records have a torque
-  bool rollActive = false;  // counts the changes of the epochs' numbering
+  bool rollActive = false;
+  // ContactLoop.evalStress; clStressOn: the last contact loop evaluated it (clStress holds
+  // the 9 components, else the stress is zero); clPart: its per-group partial sums
+  bool evalStress = false, clStressOn = false;
+  DBuf<double> clStress, clPart;  // counts the changes of the epochs' numbering
   int updatePhys = 0;  // 0 never, 1 always, 2 once (then never)
 
   // particles, storage order
@@ -299,6 +303,7 @@ struct Ctx {
     bool track = false;  // captured with energy tracking
     bool intra = false;  // captured with IntraForce
     int epochs = 1, updAll = 0;  // material epochs and updatePhys of the capture
+    bool evalStress = false;
     double dt = 0, damping = 0;
     wdg_contact_params cp{};
   } graph;

@@ -485,12 +485,18 @@ void Scene::run(int64_t steps) {
   for (auto& e : engines)
     if (e && !e->dead && !cl) cl = dynamic_cast<ContactLoop*>(e.get());
   auto device_run = [&](const wdg_program* pr, int64_t k, int64_t* st, double* d) {
-    if (cl) check(wdg_set_update_phys(ctx, cl->updatePhys));
+    if (cl) {
+      check(wdg_set_update_phys(ctx, cl->updatePhys));
+      check(wdg_set_eval_stress(ctx, cl->evalStress ? 1 : 0));
+    }
     check(wdg_run(ctx, pr, k, st, d));
     if (cl) {
       int32_t m = 0;
       check(wdg_get_update_phys(ctx, &m));
       cl->updatePhys = m;
+      bool ran = false;
+      for (int e = 0; e < pr->nEngines; e++) ran = ran || pr->engines[e] == WDG_E_CONTACTLOOP;
+      if (ran) check(wdg_contact_loop_stress(ctx, cl->stress.data()));
     }
   };
   if (!hostEngines && !hasDyn) {

@@ -215,6 +215,10 @@ struct ContactLoop : Engine {
   // the device keeps its real contacts packed in slot order at every step, so neither applies
   bool dist00 = true;
   int reorderEvery = 1000;
+  // ContactLoop.hpp:117,122: the contact loop's stress in periodic cells, read back after each
+  // device call while evalStress is set (zero otherwise)
+  bool evalStress = false;
+  std::array<double, 9> stress{};
   const char* className() const override { return "ContactLoop"; }
 };
 struct Leapfrog : Engine {

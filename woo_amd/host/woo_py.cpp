@@ -549,7 +549,13 @@ PYBIND11_MODULE(_woo, m) {
             c.updatePhys = k;
           })
       .def_readwrite("dist00", &ContactLoop::dist00)
-      .def_readwrite("reorderEvery", &ContactLoop::reorderEvery);
+      .def_readwrite("reorderEvery", &ContactLoop::reorderEvery)
+      .def_readwrite("evalStress", &ContactLoop::evalStress)
+      .def_property_readonly("stress", [](ContactLoop& c) {
+        py::array_t<double> a({3, 3});
+        std::copy(c.stress.begin(), c.stress.end(), a.mutable_data());
+        return a;
+      });
   py::class_<Leapfrog, Engine, std::shared_ptr<Leapfrog>>(m, "Leapfrog", py::dynamic_attr())
       .def(py::init(&make_kw<Leapfrog>))
       .def_readwrite("damping", &Leapfrog::damping)
