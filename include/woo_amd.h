@@ -120,6 +120,11 @@ typedef struct {
   /* Batched independent scenes: the scene of each particle (0..nScenes-1, see
    * wdg_set_scenes); particles of different scenes never collide. May be NULL (all 0). */
   const int32_t* scene;
+  /* x3, Facet.n21lim (src/dem/Facet.hpp:45; Facet::computeNeighborAngles, Facet.cpp:180-216):
+   * per edge the dot product of the facet's normal with its edge neighbour's, NaN for none; an
+   * edge or vertex contact's normal is then held within the neighbour's
+   * (Facet::adjustBoundaryContactGeom, Facet.cpp:220-265). May be NULL (all NaN). */
+  const double* facetN21lim;
 } wdg_particles;
 
 typedef struct {

@@ -151,6 +151,8 @@ struct Particle {
   double fv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   double fakeVel[3] = {0, 0, 0};
   double halfThick = 0;
+  // Facet.n21lim: per edge, the facet normal's dot product with the edge-neighbour's (NaN: none)
+  double n21lim[3] = {NaN, NaN, NaN};
   // force and torque on the facet's 3 nodes from the last IntraForce (In2_Facet)
   double nodeF[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, nodeT[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   // Aabb
@@ -758,6 +760,45 @@ struct orc_scene {
   // zero node velocities (linear velocity = fakeVel, or zero when fakeVel.x is NaN; angular
   // velocity zero). Facet.n21lim is NaN (no neighbour angles), so adjustBoundaryContactGeom
   // (220-265) returns at once and is not restated.
+  // Facet::adjustBoundaryContactGeom (Facet.cpp:220-237): an edge contact uses the edge's
+  // n21lim and outer vector; a vertex contact interpolates both between its two edges
+  static void adjustBoundaryContactGeom(const Particle& f, const V3 v[3], const int e0e1[2], const V3& fNormal,
+                                        const V3 outVec[3], const V3& otherCenter, V3& normal, V3& contPt) {
+    const int e0 = e0e1[0], e1 = e0e1[1];
+    if (std::isnan(f.n21lim[e0])) return;
+    V3 out0 = outVec[e0];
+    normalize(out0);
+    if (e1 < 0) {
+      adjustBoundaryContactGeom_impl(fNormal, v[e0], out0, otherCenter, f.n21lim[e0], normal, contPt);
+      return;
+    }
+    if (std::isnan(f.n21lim[e1])) return;
+    V3 out1 = outVec[e1];
+    normalize(out1);
+    const double p0 = dot(out0, normal), pMin = dot(out0, out1), pMax = 1.;
+    const double t01 = (p0 - pMin) / (pMax - pMin);
+    const V3 uo = add(mul(out0, t01), mul(out1, 1 - t01));
+    adjustBoundaryContactGeom_impl(fNormal, v[e1], uo, otherCenter, t01 * f.n21lim[e0] + (1 - t01) * f.n21lim[e1],
+                                   normal, contPt);
+  }
+  // Facet::adjustBoundaryContactGeom_impl (Facet.cpp:239-265)
+  static void adjustBoundaryContactGeom_impl(const V3& fN, const V3& edgePt, const V3& unitOutVec,
+                                             const V3& otherCenter, double nnLim, V3& normal, V3& contPt) {
+    if (std::isnan(nnLim)) return;
+    const double nn = dot(normal, fN);
+    if ((nn >= 0 && nnLim > 0 && nn >= nnLim) || (nn < 0 && nnLim < 0 && nn <= nnLim)) return;
+    const double noLim = (std::fabs(nnLim) >= 1. ? 1. : std::sqrt(1. - nnLim * nnLim));
+    double nno0, nno1;
+    if ((nnLim >= 0 && nn <= -noLim) || (nnLim <= 0 && nn >= noLim)) {
+      nno0 = -nnLim;
+      nno1 = 0;
+    } else {
+      nno0 = nnLim;
+      nno1 = noLim;
+    }
+    normal = add(mul(fN, nno0), mul(unitOutVec, nno1));
+    contPt = sub(otherCenter, mul(normal, dot(sub(otherCenter, edgePt), normal)));
+  }
   bool geomFacetSphere(Contact& c, const V3& shift2) const {
     const Particle &f = par[c.pA], &s = par[c.pB];
     const V3 v[3] = {ld3(f.fv), ld3(f.fv + 3), ld3(f.fv + 6)};
@@ -788,6 +829,9 @@ struct orc_scene {
     const double dist = nrm(normal);
     if (dist != 0) normal = divs(normal, dist);
     else normal = c.geom ? quatRot(c.nodeOri, mk(1, 0, 0)) : mk(1, 0, 0);
+    // Facet.cpp:372-376: an edge or vertex contact smoothed towards the neighbouring facet
+    static const int edgeNums[7][2] = {{-1, -1}, {0, -1}, {1, -1}, {0, 1}, {2, -1}, {2, 0}, {1, 2}};
+    if (w > 0) adjustBoundaryContactGeom(f, v, edgeNums[w], fNormal, outVec, sC, normal, contPt);
     if (f.halfThick > 0) contPt = add(contPt, mul(normal, f.halfThick));
     const double uN = (dist - s.radius) - f.halfThick;
     V3 linVel = zero3();
@@ -1402,6 +1446,68 @@ int orc_add_facet(orc_scene* s, const double verts[9], const double fakeVel[3], 
   p.integrated = false;
   s->par.push_back(p);
   return (int)s->par.size() - 1;
+}
+
+int orc_set_facet_n21lim(orc_scene* s, int64_t i, const double n21lim[3]) {
+  if (i < 0 || i >= (int64_t)s->par.size() || s->par[i].shape != ORC_FACET) return -1;
+  for (int k = 0; k < 3; k++) s->par[i].n21lim[k] = n21lim[k];
+  return 0;
+}
+
+// Facet::computeNeighborAngles (Facet.cpp:180-216) for every facet; a shared node is a vertex
+// at the same coordinates. Edge e0 (nodes e0, e0+1) with exactly one neighbouring facet on
+// both nodes gets n1 . n2, n2 turned to our orientation (flipped when the neighbour runs the
+// edge in the same direction); otherwise NaN.
+int orc_facet_neighbor_angles(orc_scene* s) {
+  auto same = [](const double* a, const double* b) { return a[0] == b[0] && a[1] == b[1] && a[2] == b[2]; };
+  auto normalOf = [](const Particle& f) {
+    V3 n = cross(sub(ld3(f.fv + 3), ld3(f.fv)), sub(ld3(f.fv + 6), ld3(f.fv)));
+    normalize(n);
+    return n;
+  };
+  const int64_t np = (int64_t)s->par.size();
+  for (int64_t i = 0; i < np; i++) {
+    Particle& f = s->par[i];
+    if (f.shape != ORC_FACET) continue;
+    for (int e0 = 0; e0 < 3; e0++) {
+      const int e1 = (e0 + 1) % 3;
+      int cnt = 0;
+      int64_t f2 = -1;
+      for (int64_t j = 0; j < np; j++) {
+        const Particle& g = s->par[j];
+        if (j == i || g.shape != ORC_FACET) continue;
+        bool h0 = false, h1 = false;
+        for (int q = 0; q < 3; q++) {
+          h0 = h0 || same(g.fv + 3 * q, f.fv + 3 * e0);
+          h1 = h1 || same(g.fv + 3 * q, f.fv + 3 * e1);
+        }
+        if (h0 && h1) {
+          f2 = j;
+          cnt++;
+        }
+      }
+      if (cnt != 1) {
+        f.n21lim[e0] = NaN;
+        continue;
+      }
+      const Particle& g = s->par[f2];
+      bool inv = false;
+      for (int q = 0; q < 3; q++) {
+        if (!same(g.fv + 3 * q, f.fv + 3 * e0)) continue;
+        inv = same(g.fv + 3 * ((q + 1) % 3), f.fv + 3 * e1);
+        break;
+      }
+      const V3 n2 = mul(normalOf(g), inv ? -1. : 1.);
+      f.n21lim[e0] = dot(normalOf(f), n2);
+    }
+  }
+  return 0;
+}
+
+int orc_facet_n21lim(const orc_scene* s, int64_t i, double out[3]) {
+  if (i < 0 || i >= (int64_t)s->par.size() || s->par[i].shape != ORC_FACET) return -1;
+  for (int k = 0; k < 3; k++) out[k] = s->par[i].n21lim[k];
+  return 0;
 }
 
 int orc_num_particles(const orc_scene* s) { return (int)s->par.size(); }

@@ -15,6 +15,7 @@
  *   src/dem/Facet.cpp:74-118,161-177,275-298,333-390  Facet normal/outer vectors, velocity
  *                                         interpolation, Bo1_Facet_Aabb, Cg2_Facet_Sphere_L6Geom
  *   src/dem/Facet.cpp:301-328             In2_Facet (IntraForce): contact forces spread to facet nodes
+ *   src/dem/Facet.cpp:180-265             computeNeighborAngles, adjustBoundaryContactGeom (n21lim)
  *   src/supp/base/CompUtils.cpp:59-65     closestSegmentPt
  *   src/supp/base/CompUtils.cpp:94-112    triangleBarycentrics
  *   src/dem/L6Geom.cpp:25-216             handleSpheresLikeContact
@@ -117,6 +118,11 @@ int orc_add_wall(orc_scene* s, const double pos[3], int axis, int sense, int mat
  * particle's node position is vertex 0 */
 int orc_add_facet(orc_scene* s, const double verts[9], const double fakeVel[3], double halfThick, int mat,
                   uint32_t mask);
+/* Facet.n21lim of facet i (edge-neighbour normal dot products, NaN: no smoothing) */
+int orc_set_facet_n21lim(orc_scene* s, int64_t i, const double n21lim[3]);
+int orc_facet_n21lim(const orc_scene* s, int64_t i, double out[3]);
+/* Facet::computeNeighborAngles for every facet (a shared node = a vertex at the same place) */
+int orc_facet_neighbor_angles(orc_scene* s);
 /* bulk: n free spheres (pos n*3, radius n), one material and mask; returns the first id */
 int orc_add_spheres(orc_scene* s, int64_t n, const double* pos, const double* radius, int mat, uint32_t mask);
 /* DemData.flags of particle i (blocked DOFs and the *Skip bits, Particle.hpp:196-199) */

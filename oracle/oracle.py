@@ -100,6 +100,9 @@ def _load(path=None):
     L.orc_set_flags.argtypes = [P, C.c_int64, C.c_uint32]
     L.orc_stress.argtypes = [P, C.c_double, D]; L.orc_stress.restype = C.c_int
     L.orc_cl_stress.argtypes = [P, D]; L.orc_cl_stress.restype = C.c_int
+    L.orc_set_facet_n21lim.argtypes = [P, C.c_int64, D]; L.orc_set_facet_n21lim.restype = C.c_int
+    L.orc_facet_n21lim.argtypes = [P, C.c_int64, D]; L.orc_facet_n21lim.restype = C.c_int
+    L.orc_facet_neighbor_angles.argtypes = [P]; L.orc_facet_neighbor_angles.restype = C.c_int
     L.orc_pwave_dt.argtypes = [P]; L.orc_pwave_dt.restype = C.c_double
     L.orc_crit_dt.argtypes = [P]; L.orc_crit_dt.restype = C.c_double
     L.orc_facet_nodes.argtypes = [P, C.c_int64, D]; L.orc_facet_nodes.restype = C.c_int
@@ -272,6 +275,21 @@ class Oracle:
         if self.L.orc_stress(self.h, volume, _dp(out)) != 0:
             raise RuntimeError(self.L.orc_last_error().decode())
         return out.reshape(3, 3)
+
+    def facet_neighbor_angles(self):
+        """Facet::computeNeighborAngles for every facet (shared node = same vertex coordinates)."""
+        self.L.orc_facet_neighbor_angles(self.h)
+
+    def facet_n21lim(self, i):
+        out = np.empty(3)
+        if self.L.orc_facet_n21lim(self.h, int(i), _dp(out)) != 0:
+            raise IndexError(f"particle {i} is not a facet")
+        return out
+
+    def set_facet_n21lim(self, i, v):
+        a = np.ascontiguousarray(v, dtype=np.float64)
+        if self.L.orc_set_facet_n21lim(self.h, int(i), _dp(a)) != 0:
+            raise IndexError(f"particle {i} is not a facet")
 
     def cl_stress(self):
         """ContactLoop.stress (3x3) of the last contact loop (evalStress in a periodic cell)."""

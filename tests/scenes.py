@@ -246,7 +246,9 @@ def build_oracle(sc, **params):
             o.wall(w["pos"], w["axis"], w["sense"], 0, BOUNDARY_MASK)
         else:
             f = sc["facets"][i]
-            o.facet(f["verts"], f.get("fakeVel"), f.get("halfThick", 0.), 0, BOUNDARY_MASK)
+            fid = o.facet(f["verts"], f.get("fakeVel"), f.get("halfThick", 0.), 0, BOUNDARY_MASK)
+            if f.get("n21lim") is not None:
+                o.set_facet_n21lim(fid, f["n21lim"])
     order = _particle_order(sc)
     o.set("vel", _gather(sc, order, "vel", 3))
     o.set("mass", _gather(sc, order, "mass", 1))
@@ -270,6 +272,7 @@ def gpu_arrays(sc):
     fverts = np.zeros((n, 9))
     fvel = np.zeros((n, 3))
     fht = np.zeros(n)
+    fn21 = np.full((n, 3), np.nan)
     radius = np.zeros(n)
     axis = np.zeros(n, dtype=np.int32)
     sense = np.zeros(n, dtype=np.int32)
@@ -296,14 +299,30 @@ def gpu_arrays(sc):
             if f.get("fakeVel") is not None:
                 fvel[k] = f["fakeVel"]
             fht[k] = f.get("halfThick", 0.)
+            if f.get("n21lim") is not None:
+                fn21[k] = f["n21lim"]
             mask[k] = BOUNDARY_MASK
             flags[k] = DOF_ALL
     out = dict(shape=shape, pos=pos, radius=radius, wallAxis=axis, wallSense=sense, material=np.zeros(n, np.int32),
                mask=mask, flags=flags, integrated=integ, vel=_gather(sc, order, "vel", 3),
                mass=_gather(sc, order, "mass", 1), inertia=_gather(sc, order, "inertia", 3))
     if sc.get("facets"):
-        out.update(facetVerts=fverts, fakeVel=fvel, halfThick=fht)
+        out.update(facetVerts=fverts, fakeVel=fvel, halfThick=fht, facetN21lim=fn21)
     return out
+
+
+def smooth_facets(sc):
+    """Facet.n21lim of every facet of the scene, by the oracle's Facet::computeNeighborAngles
+    (facets sharing vertices are edge neighbours)"""
+    o = build_oracle(sc)
+    for k, (kind, i) in enumerate(_particle_order(sc)):
+        if kind == "f":
+            sc["facets"][i]["n21lim"] = None
+    o.facet_neighbor_angles()
+    for k, (kind, i) in enumerate(_particle_order(sc)):
+        if kind == "f":
+            sc["facets"][i]["n21lim"] = o.facet_n21lim(k)
+    return sc
 
 
 def build_gpu(sc, device=0):

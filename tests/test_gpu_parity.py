@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 from tests.scenes import (GpuRunner, build_oracle, deposition, drum, facet_box, n_boundary, periodic_bidisperse,
+                          smooth_facets,
                           periodic_hertz)
 
 pytestmark = pytest.mark.gpu
@@ -385,6 +386,32 @@ def test_trajectory_drum_facets(halfThick):
     # facets take no force from the contact loop
     F = g.d.get("force")
     assert np.all(F[2:nb] == 0.)
+
+
+def test_trajectory_drum_smoothed_facet_edges():
+    # Facet.n21lim from Facet::computeNeighborAngles on the drum's mantle: the edge and vertex
+    # contacts' normals held within the neighbours' (Facet::adjustBoundaryContactGeom), as the
+    # oracle's
+    sc = smooth_facets(drum(n=2000, seed=11))
+    nb = n_boundary(sc)
+    assert sum(int(np.isfinite(f["n21lim"]).sum()) for f in sc["facets"]) > 100
+    o, g, worst = _run_both(sc, ENG, 200, check_every=20)
+    ids = o.contacts(real_only=True, data=False)[0]
+    nf = int(np.sum((ids[:, 0] >= 2) & (ids[:, 0] < nb)))
+    print("smoothed drum: facet contacts", nf, "worst position err", worst)
+    assert nf > 20
+    assert worst <= TRAJ_TOL
+    # the smoothing changes the motion: the same drum without n21lim ends elsewhere
+    o2 = build_oracle(drum(n=2000, seed=11))
+    o2.run(ENG, 200)
+    assert np.max(np.abs(o2.get("pos") - o.get("pos"))) > 1e-12
+    # the FP32 contact form smooths the same way: 10 steps stay close to the FP64 run
+    g64, g32 = GpuRunner(sc), GpuRunner(sc)
+    g32.d.set_precision(32)
+    g64.run(ENG, 10)
+    g32.run(ENG, 10)
+    dev = np.max(np.linalg.norm(g32.d.get("pos") - g64.d.get("pos"), axis=1)) / float(np.min(sc["radius"]))
+    assert dev < 1e-3
 
 
 def _drum_torque(sc, first, F, T):

@@ -252,3 +252,26 @@ def test_idealelpl_rolling_attributes(woo):
     assert (law.relRollStiff, law.relTwistStiff, law.rollTanPhi) == (0., 0., 0.)
     law = Law2_L6Geom_FrictPhys_IdealElPl(relRollStiff=1., relTwistStiff=.5, rollTanPhi=.1, noSlip=True)
     assert (law.relRollStiff, law.relTwistStiff, law.rollTanPhi, law.noSlip) == (1., .5, .1, True)
+
+
+def test_facet_compute_neighbor_angles(woo):
+    # Facet.computeNeighborAngles (Facet.cpp:180-216) over facets that share Node objects: the
+    # shared edge gets cos(fold), the others NaN (Facet.n21lim)
+    import math
+
+    from woo_amd.woo.core import Node
+    from woo_amd.woo.dem import DemField
+
+    t = math.radians(30.)
+    n0, n1 = Node(pos=(0, 0, 0)), Node(pos=(0, 1, 0))
+    a = woo.utils.facet([n0, n1, (-1., 0, 0)])
+    b = woo.utils.facet([n0, (math.cos(t), 0, -math.sin(t)), n1])
+    S = woo.core.Scene(fields=[DemField()])
+    S.dem.par.add([a, b])
+    a.shape.computeNeighborAngles()
+    b.shape.computeNeighborAngles()
+    na, nb = list(a.shape.n21lim), list(b.shape.n21lim)
+    assert na[0] == pytest.approx(math.cos(t), abs=1e-15) and math.isnan(na[1]) and math.isnan(na[2])
+    assert nb[2] == pytest.approx(math.cos(t), abs=1e-15) and math.isnan(nb[0]) and math.isnan(nb[1])
+    with pytest.raises(RuntimeError):
+        woo.utils.facet([(0, 0, 0), (1, 0, 0), (0, 1, 0)]).shape.computeNeighborAngles()

@@ -577,6 +577,90 @@ def _global_force(d):
     return d[:, 9:10] * n + d[:, 10:11] * y + d[:, 11:12] * z
 
 
+def _two_facets(fold_deg, flip=False):
+    """two unit right triangles sharing the edge (0,0,0)-(0,1,0): A in the plane z=0 on x<0 with
+    normal +z, B on x>0 folded down about the edge by fold_deg (0: coplanar), listed so that its
+    normal is A's turned by the fold (sin, 0, cos); flip lists B's vertices the other way round"""
+    t = math.radians(fold_deg)
+    a = [(0., 0., 0.), (0., 1., 0.), (-1., 0., 0.)]
+    b = [(0., 0., 0.), (math.cos(t), 0., -math.sin(t)), (0., 1., 0.)]
+    if flip:
+        b = [b[0], b[2], b[1]]
+    return a, b
+
+
+@pytest.mark.parametrize("fold", [0., 30., 90.])
+@pytest.mark.parametrize("flip", [False, True])
+def test_facet_neighbor_angles(fold, flip):
+    # Facet::computeNeighborAngles (Facet.cpp:180-216): the shared edge gets n1 . n2 with the
+    # neighbour's normal taken in our orientation: cos(fold) whichever way the neighbour lists its
+    # vertices; the unshared edges get NaN
+    o = Oracle(gravity=(0, 0, 0))
+    m = o.material()
+    a, b = _two_facets(fold, flip)
+    ia = o.facet(np.array(a).ravel(), None, 0., m)
+    ib = o.facet(np.array(b).ravel(), None, 0., m)
+    o.facet_neighbor_angles()
+    na, nb = o.facet_n21lim(ia), o.facet_n21lim(ib)
+    # A's edge 0 (v0, v1) is the shared one
+    assert np.isnan(na[1]) and np.isnan(na[2])
+    assert na[0] == pytest.approx(math.cos(math.radians(fold)), abs=1e-15)
+    shared_b = [k for k in range(3) if not np.isnan(nb[k])]
+    assert len(shared_b) == 1 and nb[shared_b[0]] == pytest.approx(math.cos(math.radians(fold)), abs=1e-15)
+
+
+def _edge_contact(fold, smooth, centre, r=.2):
+    o = Oracle(gravity=(0, 0, 0), damping=0.)
+    m = o.material(young=1e7)
+    a, b = _two_facets(fold)
+    ia = o.facet(np.array(a).ravel(), None, 0., m)
+    o.facet(np.array(b).ravel(), None, 0., m)
+    o.sphere(centre, r, m)
+    if smooth:
+        o.facet_neighbor_angles()
+    o.set_params(dt=1e-7)
+    o.run(["Collider", "ContactLoop"], 1)
+    ids, _, d = o.contacts(real_only=True)
+    k = [q for q in range(len(ids)) if ia in ids[q]]
+    assert len(k) == 1
+    return d[k[0], 0:3], d[k[0], 6:9]
+
+
+def test_n21lim_limits_the_edge_normal_to_the_neighbours():
+    # Facet::adjustBoundaryContactGeom_impl (Facet.cpp:239-265): a sphere past A's edge, above a
+    # neighbour folded by 30 degrees, touches A's edge with a normal tilted towards it; n21lim =
+    # cos 30 bounds the tilt: the normal becomes cos30 fN + sin30 out (A's unit outer edge vector,
+    # +x), i.e. the neighbour's normal, and the contact point the centre's projection along it
+    # onto the plane through the edge
+    c = np.array([.15, .4, .1])
+    n0, p0 = _edge_contact(30., False, c)
+    np.testing.assert_allclose(n0, np.array([.15, 0, .1]) / math.hypot(.15, .1), atol=1e-14)
+    n1, p1 = _edge_contact(30., True, c)
+    t = math.radians(30.)
+    nb = np.array([math.sin(t), 0., math.cos(t)])
+    np.testing.assert_allclose(n1, nb, atol=1e-14)
+    np.testing.assert_allclose(p1, c - nb * (c @ nb), atol=1e-14)
+    # inside the bound (a normal steeper than the neighbour's) nothing changes
+    c2 = np.array([.02, .4, .15])
+    n2, _ = _edge_contact(30., True, c2)
+    np.testing.assert_allclose(n2, np.array([.02, 0, .15]) / math.hypot(.02, .15), atol=1e-14)
+
+
+def test_n21lim_folds_past_a_right_angle():
+    # n21lim = cos(fold): up to a right angle (n21lim >= 0; cos 90 is 6e-17 here) a normal between
+    # the two facets' normals is left as it is; past a right angle (n21lim < 0) the same normal is
+    # set to nnLim fN + sqrt(1 - nnLim^2) out, which is the neighbour's normal (sin, 0, cos)
+    c = np.array([.1, .4, .12])
+    n0, _ = _edge_contact(90., False, c)
+    n1, _ = _edge_contact(90., True, c)
+    np.testing.assert_array_equal(n1, n0)
+    t = math.radians(120.)
+    nb = np.array([math.sin(t), 0., math.cos(t)])
+    n2, p2 = _edge_contact(120., True, c)
+    np.testing.assert_allclose(n2, nb, atol=1e-14)
+    np.testing.assert_allclose(p2, c - nb * (c @ nb), atol=1e-14)
+
+
 @pytest.mark.parametrize("centre", [
     (0.2, 0.3, 0.0079),     # face: weights (0.5, 0.2, 0.3)
     (0.5, -0.002, 0.0075),  # edge v0-v1: node 2 gets nothing

@@ -59,7 +59,8 @@ class Particles(C.Structure):
                 ("pos", _P(C.c_double)), ("vel", _P(C.c_double)), ("angVel", _P(C.c_double)),
                 ("ori", _P(C.c_double)), ("force", _P(C.c_double)), ("torque", _P(C.c_double)),
                 ("mass", _P(C.c_double)), ("inertia", _P(C.c_double)), ("facetVerts", _P(C.c_double)),
-                ("fakeVel", _P(C.c_double)), ("halfThick", _P(C.c_double)), ("scene", _P(C.c_int32))]
+                ("fakeVel", _P(C.c_double)), ("halfThick", _P(C.c_double)), ("scene", _P(C.c_int32)),
+                ("facetN21lim", _P(C.c_double))]
 
 
 class Tuning(C.Structure):
@@ -216,7 +217,7 @@ class Device:
 
     def upload_particles(self, shape, pos, mass, inertia, radius=None, wallAxis=None, wallSense=None, material=None,
                          mask=None, flags=None, integrated=None, vel=None, angVel=None, ori=None, force=None,
-                         torque=None, facetVerts=None, fakeVel=None, halfThick=None, scene=None):
+                         torque=None, facetVerts=None, fakeVel=None, halfThick=None, scene=None, facetN21lim=None):
         n = len(shape)
         keep = []
 
@@ -251,11 +252,12 @@ class Device:
         p.fakeVel = dptr(arr(fakeVel, np.float64, 3))
         p.halfThick = dptr(arr(halfThick, np.float64))
         p.scene = iptr(arr(scene, np.int32))
+        p.facetN21lim = dptr(arr(facetN21lim, np.float64, 3))
         check(self.L.wdg_upload_particles(self.h, C.byref(p)))
         self.n = n
         static = dict(shape=shape, radius=radius, wallAxis=wallAxis, wallSense=wallSense, material=material, mask=mask,
                       flags=flags, integrated=integrated, facetVerts=facetVerts, fakeVel=fakeVel, halfThick=halfThick,
-                      scene=scene)
+                      scene=scene, facetN21lim=facetN21lim)
         self._static = {k: np.array(v) for k, v in static.items() if v is not None}
 
     def get(self, field):

@@ -666,6 +666,7 @@ int wdg_upload_particles(wdg_ctx* ctx, const wdg_particles* p) {
         for (int k = 0; k < 9; k++) g[k] = p->facetVerts[9 * i + k];
         for (int k = 0; k < 3; k++) g[9 + k] = p->fakeVel ? p->fakeVel[3 * i + k] : 0.;
         g[12] = ht;
+        for (int k = 0; k < 3; k++) g[13 + k] = p->facetN21lim ? p->facetN21lim[3 * i + k] : NAN;
         nW++;
         nF++;
       } else {

@@ -96,7 +96,8 @@ __device__ __forceinline__ V closest_segment_pt(V P, V A, V B) {
 // normal, the signed plane distance, and the closest point by Voronoi region of the
 // triangle v0 v1 v2. Returns false for the impossible region (outside all three edges).
 template <class V, class T>
-__device__ __forceinline__ bool facet_closest(V v0, V v1, V v2, V sC, V& fNormal, T& planeDist, V& contPt) {
+__device__ __forceinline__ bool facet_closest(V v0, V v1, V v2, V sC, V& fNormal, T& planeDist, V& contPt,
+                                              int* region = nullptr) {
   const V nn = cross3(sub3(v1, v0), sub3(v2, v0));
   fNormal = normalized3(nn);
   planeDist = dot3(sub3(sC, v0), fNormal);
@@ -115,7 +116,56 @@ __device__ __forceinline__ bool facet_closest(V v0, V v1, V v2, V sC, V& fNormal
     case 6: contPt = v2; break;
     default: contPt = fC; return false;
   }
+  if (region) *region = w;
   return true;
+}
+
+// Facet::adjustBoundaryContactGeom (Facet.cpp:220-237) and its _impl (239-265): an edge or
+// vertex contact (region w of facet_closest) of a facet with Facet.n21lim (per edge, the dot
+// product of its normal with the edge neighbour's; NaN: none) has its normal held within the
+// neighbour's, and its contact point moved to the sphere centre's projection along it; a vertex
+// interpolates the outer vector and the limit between its two edges (edgeNums, Facet.cpp:372)
+template <class V, class T>
+__device__ __forceinline__ void facet_adjust(const V v[3], int w, V fN, V sC, const T lim[3], V& normal, V& contPt) {
+  int e0, e1;
+  switch (w) {
+    case 1: e0 = 0; e1 = -1; break;
+    case 2: e0 = 1; e1 = -1; break;
+    case 3: e0 = 0; e1 = 1; break;
+    case 4: e0 = 2; e1 = -1; break;
+    case 5: e0 = 2; e1 = 0; break;
+    case 6: e0 = 1; e1 = 2; break;
+    default: return;
+  }
+  if (isnan(lim[e0])) return;
+  // Facet::getOuterVectors (Facet.cpp:113-118): edge i is (v_i, v_i+1), its outer vector
+  // (v_i+1 - v_i) x nn
+  const V nn = cross3(sub3(v[1], v[0]), sub3(v[2], v[0]));
+  const V out0 = normalized3(cross3(sub3(v[(e0 + 1) % 3], v[e0]), nn));
+  V uo = out0, edgePt = v[e0];
+  T nnLim = lim[e0];
+  if (e1 >= 0) {
+    if (isnan(lim[e1])) return;
+    const V out1 = normalized3(cross3(sub3(v[(e1 + 1) % 3], v[e1]), nn));
+    const T p0 = dot3(out0, normal), pMin = dot3(out0, out1), pMax = T(1);
+    const T t01 = (p0 - pMin) / (pMax - pMin);
+    uo = add3(mul3(out0, t01), mul3(out1, T(1) - t01));
+    nnLim = t01 * lim[e0] + (T(1) - t01) * lim[e1];
+    edgePt = v[e1];
+  }
+  const T nnv = dot3(normal, fN);
+  if ((nnv >= T(0) && nnLim > T(0) && nnv >= nnLim) || (nnv < T(0) && nnLim < T(0) && nnv <= nnLim)) return;
+  const T noLim = (fabs(nnLim) >= T(1) ? T(1) : sqrt(T(1) - nnLim * nnLim));
+  T nno0, nno1;
+  if ((nnLim >= T(0) && nnv <= -noLim) || (nnLim <= T(0) && nnv >= noLim)) {
+    nno0 = -nnLim;
+    nno1 = T(0);
+  } else {
+    nno0 = nnLim;
+    nno1 = noLim;
+  }
+  normal = add3(mul3(fN, nno0), mul3(uo, nno1));
+  contPt = sub3(sC, mul3(normal, dot3(sub3(sC, edgePt), normal)));
 }
 
 // Law2 break test: IdealElPl.cpp:26-33 (unless noBreak), Hertz.cpp:105-108
@@ -407,8 +457,9 @@ __global__ __launch_bounds__(CF_BS, FACETS ? 3 : 4) void k_cforce(ContactArgs a)
       pos2 = add3(posB, shift2);
       d3 fNormal;
       double planeDist;
-      facet_closest<d3, double>(mk3(g[0], g[1], g[2]), mk3(g[3], g[4], g[5]), mk3(g[6], g[7], g[8]), pos2, fNormal,
-                                planeDist, contPt);
+      const d3 fv[3] = {mk3(g[0], g[1], g[2]), mk3(g[3], g[4], g[5]), mk3(g[6], g[7], g[8])};
+      int w = 0;
+      facet_closest<d3, double>(fv[0], fv[1], fv[2], pos2, fNormal, planeDist, contPt, &w);
       const d3 nv = sub3(pos2, contPt);
       const double dist = sqrt(sqn3(nv));
       if (dist != 0) normal = div3(nv, dist);
@@ -417,6 +468,10 @@ __global__ __launch_bounds__(CF_BS, FACETS ? 3 : 4) void k_cforce(ContactArgs a)
         const d3 pY = prevYL;
         normal = quat_rot(quat_from_cols(prevNormal, pY, cross3(prevNormal, pY)), mk3(1., 0., 0.));
       } else normal = mk3(1., 0., 0.);
+      if (w > 0) {
+        const double lim[3] = {g[13], g[14], g[15]};
+        facet_adjust<d3, double>(fv, w, fNormal, pos2, lim, normal, contPt);
+      }
       if (ht > 0) contPt = add3(contPt, mul3(normal, ht));
       uN = (dist - radius) - ht;
       pos1 = contPt;
@@ -751,7 +806,8 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
       const f3 e2 = tof3(mk3(g[6] - g[0], g[7] - g[1], g[8] - g[2]));
       f3 fNormal;
       float planeDist;
-      facet_closest<f3, float>(mkf3(0.f, 0.f, 0.f), e1, e2, r, fNormal, planeDist, cA);
+      int w = 0;
+      facet_closest<f3, float>(mkf3(0.f, 0.f, 0.f), e1, e2, r, fNormal, planeDist, cA, &w);
       const f3 nv = sub3(r, cA);
       const float dist = sqrtf(sqn3(nv));
       if (dist != 0.f) normal = div3(nv, dist);
@@ -759,6 +815,11 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
         const f3 pY = mkf3(oY0[ko], oY1[ko], oY2[ko]);
         normal = quat_rot(quat_from_cols(prevNormal, pY, cross3(prevNormal, pY)), mkf3(1.f, 0.f, 0.f));
       } else normal = mkf3(1.f, 0.f, 0.f);
+      if (w > 0) {
+        const f3 fv[3] = {mkf3(0.f, 0.f, 0.f), e1, e2};
+        const float lim[3] = {(float)g[13], (float)g[14], (float)g[15]};
+        facet_adjust<f3, float>(fv, w, fNormal, r, lim, normal, cA);
+      }
       if (ht > 0.f) cA = add3(cA, mul3(normal, ht));
       uN = (dist - radius) - ht;
       r1 = fmaxf(ht, radius);

@@ -63,7 +63,7 @@ enum : uint32_t {
   CI_NEAR = 8u,        // close enough at the last refresh to touch before the next one
 };
 enum { CT_SS = 0, CT_WS = 1, CT_FS = 2 };
-// per boundary body (storage [0, nW)): facet vertices v0 v1 v2, fakeVel, halfThick, unused
+// per boundary body (storage [0, nW)): facet vertices v0 v1 v2, fakeVel, halfThick, n21lim
 constexpr int FGEO = 16;
 constexpr uint32_t ADJ_PA = 0x80000000u;  // adjacency entry: this particle is pA of the slot
 

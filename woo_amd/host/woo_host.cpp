@@ -167,7 +167,7 @@ void DemField::push(Scene& S) {
     std::vector<int32_t> shape(n), axis(n), sense(n), mat(n);
     std::vector<uint32_t> mask(n), flags(n);
     std::vector<uint8_t> integ(n);
-    std::vector<double> radius(n), bufs[8], fverts(9 * n, 0.), fvel(3 * n, 0.), fht(n, 0.);
+    std::vector<double> radius(n), bufs[8], fverts(9 * n, 0.), fvel(3 * n, 0.), fht(n, 0.), fn21(3 * n, NaN);
     for (size_t i = 0; i < n; i++) {
       const Particle& p = *particles[i];
       auto it = midx.find(p.material.get());
@@ -187,6 +187,7 @@ void DemField::push(Scene& S) {
           for (int k = 0; k < 3; k++) fverts[9 * i + 3 * v + k] = f->nodes[v]->pos[k];
         for (int k = 0; k < 3; k++) fvel[3 * i + k] = f->fakeVel[k];
         fht[i] = f->halfThick;
+        for (int k = 0; k < 3; k++) fn21[3 * i + k] = f->n21lim[k];
       } else {
         const auto* w = dynamic_cast<const Wall*>(p.shape.get());
         shape[i] = WDG_WALL;
@@ -212,7 +213,7 @@ void DemField::push(Scene& S) {
     pd.material = mat.data(); pd.mask = mask.data(); pd.flags = flags.data(); pd.integrated = integ.data();
     pd.pos = bufs[0].data(); pd.vel = bufs[1].data(); pd.angVel = bufs[2].data(); pd.ori = bufs[3].data();
     pd.force = bufs[4].data(); pd.torque = bufs[5].data(); pd.mass = bufs[6].data(); pd.inertia = bufs[7].data();
-    pd.facetVerts = fverts.data(); pd.fakeVel = fvel.data(); pd.halfThick = fht.data();
+    pd.facetVerts = fverts.data(); pd.fakeVel = fvel.data(); pd.halfThick = fht.data(); pd.facetN21lim = fn21.data();
     check(wdg_upload_particles(dev->ctx, &pd));
     structureChanged = hostDirty = false;
     if (savedContacts) {

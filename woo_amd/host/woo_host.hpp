@@ -76,6 +76,8 @@ struct Wall : Shape {
 struct Facet : Shape {
   Vec3 fakeVel{};
   double halfThick = 0;
+  // Facet.hpp:45: per edge the normal's dot product with the edge neighbour's (NaN: none)
+  Vec3 n21lim{NaN, NaN, NaN};
   const char* className() const override { return "Facet"; }
 };
 

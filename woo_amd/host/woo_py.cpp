@@ -265,6 +265,61 @@ PYBIND11_MODULE(_woo, m) {
       .def_property(
           "halfThick", [](Facet& f) { return f.halfThick; },
           [restructure_shape](Facet& f, double h) { restructure_shape(f); f.halfThick = h; })
+      .def_property(
+          "n21lim", [](Facet& f) { return arr(f.n21lim.data(), 3); },
+          [restructure_shape](Facet& f, py::handle v) { restructure_shape(f); f.n21lim = vec3(v); })
+      // Facet::computeNeighborAngles (Facet.cpp:180-216): an edge shared (both nodes) with exactly
+      // one other facet of the field gets n1 . n2, the neighbour's normal taken in this facet's
+      // orientation; the other edges NaN
+      .def("computeNeighborAngles", [restructure_shape](Facet& f) {
+        if (f.nodes.size() != 3) throw std::runtime_error("Facet needs 3 nodes");
+        DemField* fld = f.nodes[0] ? f.nodes[0]->field : nullptr;
+        if (!fld) throw std::runtime_error("Facet.computeNeighborAngles: the facet is not in a DemField");
+        auto normal = [](const Facet& g) {
+          const Vec3 &a = g.nodes[0]->pos, &b = g.nodes[1]->pos, &c = g.nodes[2]->pos;
+          const Vec3 u{b[0] - a[0], b[1] - a[1], b[2] - a[2]}, v{c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+          Vec3 n{u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+          const double z = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+          if (z > 0) {
+            const double l = std::sqrt(z);
+            for (auto& x : n) x = x / l;
+          }
+          return n;
+        };
+        restructure_shape(f);
+        const Vec3 n1 = normal(f);
+        for (int e0 = 0; e0 < 3; e0++) {
+          const int e1 = (e0 + 1) % 3;
+          int cnt = 0;
+          const Facet* f2 = nullptr;
+          for (const auto& p : fld->particles) {
+            const auto* g = dynamic_cast<const Facet*>(p->shape.get());
+            if (!g || g == &f || g->nodes.size() != 3) continue;
+            bool h0 = false, h1 = false;
+            for (const auto& nd : g->nodes) {
+              h0 = h0 || nd == f.nodes[e0];
+              h1 = h1 || nd == f.nodes[e1];
+            }
+            if (h0 && h1) {
+              f2 = g;
+              cnt++;
+            }
+          }
+          if (cnt != 1) {
+            f.n21lim[e0] = NaN;
+            continue;
+          }
+          bool inv = false;
+          for (int i = 0; i < 3; i++) {
+            if (f2->nodes[i] != f.nodes[e0]) continue;
+            inv = f2->nodes[(i + 1) % 3] == f.nodes[e1];
+            break;
+          }
+          const Vec3 n2 = normal(*f2);
+          const double s = inv ? -1. : 1.;
+          f.n21lim[e0] = (n1[0] * (n2[0] * s) + n1[1] * (n2[1] * s)) + n1[2] * (n2[2] * s);
+        }
+      })
       // Facet::getNormal (Facet.cpp:74-77)
       .def("getNormal", [](Facet& f) {
         if (f.nodes.size() != 3) throw std::runtime_error("Facet needs 3 nodes");
