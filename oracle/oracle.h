@@ -15,7 +15,10 @@
  *   src/dem/Facet.cpp:74-118,161-177,275-298,333-390  Facet normal/outer vectors, velocity
  *                                         interpolation, Bo1_Facet_Aabb, Cg2_Facet_Sphere_L6Geom
  *   src/dem/Facet.cpp:301-328             In2_Facet (IntraForce): contact forces spread to facet nodes
- *   src/dem/Facet.cpp:180-265             computeNeighborAngles, adjustBoundaryContactGeom (n21lim)
+ *   src/dem/Facet.cpp:180-265             computeNeighborAngles, adjustBoundaryContactGeom (n21lim);
+ *                                         the vertex branch (interpolation between two edges,
+ *                                         Facet.cpp:228-236): parity unpinned (no closed form;
+ *                                         the edge branch and the neighbour angles are pinned)
  *   src/supp/base/CompUtils.cpp:59-65     closestSegmentPt
  *   src/supp/base/CompUtils.cpp:94-112    triangleBarycentrics
  *   src/dem/L6Geom.cpp:25-216             handleSpheresLikeContact
