@@ -248,6 +248,8 @@ struct Ctx {
   // ContactLoop.evalStress; clStressOn: the last contact loop evaluated it (clStress holds
   // the 9 components, else the stress is zero); clPart: its per-group partial sums
   bool evalStress = false, clStressOn = false;
+  // a single dense cell grid orders the spheres by the Morton code of their cell
+  bool cellMorton = true;
   DBuf<double> clStress, clPart;  // counts the changes of the epochs' numbering
   int updatePhys = 0;  // 0 never, 1 always, 2 once (then never)
 
@@ -380,6 +382,7 @@ struct Ctx {
     int periodic;
     int hashed;       // cell keys are hashed into hmask + 1 buckets (sparse aperiodic scenes)
     uint32_t hmask;
+    int morton;       // dense grid with Morton-coded cell keys
     const void* sg;   // batched aperiodic scenes: the grid of every scene (in sgrid), else null
     bool brute;
     bool valid = false;
