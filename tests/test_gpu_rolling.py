@@ -93,3 +93,23 @@ def test_rolling_resistance_changes_the_motion_and_needs_fp64():
         d.run(["Collider", "ContactLoop", "Leapfrog"], 2, 0, sc["dt"], contact=dict(contact_kw(sc), **ROLL),
               leapfrog=dict(damping=sc["damping"]), dynDtPeriod=0)
     d.close()
+
+
+def test_snapshot_restore_with_rolling_resistance():
+    # the phys torque has no history (it is this step's), so a restored snapshot steps on as the
+    # first continuation did, bit for bit
+    sc = deposition(n=600, seed=10, dense=True, vel_sigma=.05)
+    rng = np.random.default_rng(5)
+    w = np.vstack([rng.normal(0, 20., size=(len(sc["radius"]), 3)), np.zeros((len(sc["walls"]), 3))])
+    d = build_gpu(sc)
+    d.set("angVel", w)
+    kw = dict(contact=dict(contact_kw(sc), **ROLL), leapfrog=dict(damping=sc["damping"]), dynDtPeriod=0)
+    step, _ = d.run(["Collider", "ContactLoop", "Leapfrog"], 30, 0, sc["dt"], **kw)
+    d.snapshot(2)
+    d.run(["Collider", "ContactLoop", "Leapfrog"], 30, step, sc["dt"], **kw)
+    p1, w1 = d.get("pos"), d.get("angVel")
+    d.restore(2)
+    d.run(["Collider", "ContactLoop", "Leapfrog"], 30, step, sc["dt"], **kw)
+    np.testing.assert_array_equal(d.get("pos"), p1)
+    np.testing.assert_array_equal(d.get("angVel"), w1)
+    d.close()

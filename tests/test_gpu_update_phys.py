@@ -85,3 +85,24 @@ def test_update_phys_argument_checks():
     d.update_phys = "once"
     assert d.update_phys == "once"
     d.close()
+
+
+def test_snapshot_keeps_each_contacts_material_epoch():
+    # with 'never' after a material change, a snapshot keeps which contacts have the old table:
+    # stepping on from the restored snapshot repeats the first continuation bit for bit
+    sc = _scene("frict")
+    d = build_gpu(sc)
+    kw = dict(contact=contact_kw(sc), leapfrog=dict(damping=sc["damping"]), dynDtPeriod=0)
+    step, _ = d.run(ENG, 30, 0, sc["dt"], **kw)
+    m0 = dict(sc["materials"][0])
+    d.set_materials([dict(m0, young=4 * m0["young"])])
+    step, _ = d.run(ENG, 20, step, sc["dt"], **kw)
+    d.snapshot(1)
+    s1, _ = d.run(ENG, 40, step, sc["dt"], **kw)
+    p1, k1 = d.get("pos"), d.contacts(real_only=True)[2][:, 13]
+    d.restore(1)
+    s2, _ = d.run(ENG, 40, step, sc["dt"], **kw)
+    np.testing.assert_array_equal(d.get("pos"), p1)
+    np.testing.assert_array_equal(d.contacts(real_only=True)[2][:, 13], k1)
+    assert len(set(np.round(k1 / k1.min(), 6).tolist())) > 1  # both tables are in use
+    d.close()

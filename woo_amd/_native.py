@@ -519,11 +519,13 @@ class Device:
         check(self.L.wdg_sync(self.h))
 
     def snapshot(self, slot):
-        """keep the kinematic state (pos, vel, angVel, ori, force, torque) on the device"""
+        """keep the kinematic state (pos, vel, angVel, ori, force, torque) and, in the FP64 contact
+        form, the contact history on the device (slot 0..3)"""
         check(self.L.wdg_snapshot(self.h, slot))
 
     def restore(self, slot):
-        """load a snapshot back; the contact history restarts"""
+        """load a snapshot back: stepping on gives the trajectory the saved state would have had
+        (the contact history restarts only in the FP32 form or for a snapshot without contacts)"""
         check(self.L.wdg_restore(self.h, slot))
 
     def snapshot_distance(self, slot):
