@@ -280,6 +280,12 @@ int wdg_energy_clear(wdg_ctx* ctx);
 /* EnergyTracker.set / __setitem__ (src/core/EnergyTracker.hpp:46-49,57): the terms from
  * in[WDG_EN_N] in WDG_EN_* order, e.g. those of a checkpoint */
 int wdg_energy_set(wdg_ctx* ctx, const double in[7]);
+/* Scene.energy of each scene of a batched context of several scenes: out[WDG_EN_N * s + term]
+ * (synchronises the stream); wdg_energy gives their sum over the scenes there. The terms of a
+ * scene are summed in another order than when it is stepped alone, so they agree to rounding. */
+int wdg_energy_scenes(wdg_ctx* ctx, double* out);
+/* EnergyTracker.set of each scene (in[WDG_EN_N * nScenes]) */
+int wdg_energy_scenes_set(wdg_ctx* ctx, const double* in);
 int wdg_stats(wdg_ctx* ctx, wdg_counters* out);
 
 /* run nSteps of Scene::doOneStep with a fixed engine list; *step and *dt are the

@@ -210,3 +210,45 @@ def test_batched_dyndt_per_scene_equals_serial_runs():
         assert crit[q] == g.d.crit_dt()
         np.testing.assert_array_equal(P[sl], g.d.get("pos"))
     assert len(set(dts.tolist())) == len(scenes)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batched_energy_per_scene_equals_serial_runs(graph):
+    # EnergyTracker per scene of a batch (Scene.energy, EnergyTracker.hpp): each scene's terms as
+    # when the scene is stepped alone. The additions go in another order (per particle, then per
+    # scene, against per workgroup), so the terms agree to rounding: within 1e-11 of the scene's
+    # largest term
+    from woo_amd._native import ENERGY_TERMS
+    scenes = scenes_c4_like(k=4, seed=67)
+    steps = 160
+    sc0 = scenes[0]
+    kw = dict(contact=contact_kw(sc0), leapfrog=dict(damping=sc0["damping"]), dynDtPeriod=0)
+    d, _ = build_batch(scenes)
+    d.set_energy_tracking(True)
+    d.run(ENG, steps, 0, sc0["dt"], graph=graph, **kw)
+    E = d.energy_scenes()
+    tot = d.energy()
+    assert E.shape == (len(scenes), 7)
+    for t, name in enumerate(ENERGY_TERMS):
+        assert tot[name] == pytest.approx(float(np.sum(E[:, t])), rel=1e-14, abs=1e-300)
+    P = d.get("pos")
+    for q, (sc, sl) in enumerate(zip(scenes, slices(scenes))):
+        g = GpuRunner(sc)
+        g.d.set_energy_tracking(True)
+        g.run(ENG, steps)
+        np.testing.assert_array_equal(P[sl], g.d.get("pos"))
+        e1 = g.d.energy()
+        ref = np.array([e1[k] for k in ENERGY_TERMS])
+        scale = float(np.max(np.abs(ref)))
+        print(f"scene {q}: batched {E[q]} alone {ref}", flush=True)
+        assert scale > 0
+        np.testing.assert_allclose(E[q], ref, rtol=0, atol=1e-11 * scale)
+        # the dissipation terms of the law and the damping are there
+        for k in ("kinetic", "elast", "plast", "nonviscDamp"):
+            assert e1[k] != 0, k
+    # checkpoint of the batch keeps each scene's terms
+    ck = d.checkpoint()
+    d2 = type(d).from_checkpoint(ck)
+    np.testing.assert_array_equal(d2.energy_scenes(), E)
+    d2.close()
+    d.close()

@@ -269,7 +269,10 @@ def test_scripted_energy_tracking_matches_oracle():
 
 def test_savetmp_loadtmp_continues_bit_for_bit():
     # Scene.saveTmp keeps the scene with its contact history and energy; a loaded copy stepped on
-    # gives the trajectory, the contacts and the energy of the original, bit for bit
+    # gives the trajectory and the contacts of the original bit for bit, and its energy to
+    # rounding: the energy terms are summed over the contact slots in storage order, which
+    # follows the cell order of the last refresh, and the loaded copy refreshes at its first step
+    # (the particles' force sums go in the order of the partners' ids and do not depend on it)
     import woo_amd.woo as woo
 
     sc = deposition(n=900, dense=True, seed=61, vel_sigma=0.05)
@@ -286,7 +289,10 @@ def test_savetmp_loadtmp_continues_bit_for_bit():
     np.testing.assert_array_equal(positions(S2), positions(S))
     np.testing.assert_array_equal(np.array([p.vel for p in S2.dem.par]), np.array([p.vel for p in S.dem.par]))
     assert [(c.id1, c.id2) for c in S2.dem.con if c.real] == [(c.id1, c.id2) for c in S.dem.con if c.real]
-    assert S2.energy.items() == S.energy.items()
+    assert S2.energy.keys() == S.energy.keys()
+    scale = max(abs(S.energy[k]) for k in S.energy.keys())
+    for k in S.energy.keys():
+        assert abs(S2.energy[k] - S.energy[k]) <= 1e-12 * scale, k
     assert nreal > 100
 
 

@@ -32,7 +32,7 @@ EXPORTS = [
     "wdg_set_energy_tracking", "wdg_energy", "wdg_energy_clear", "wdg_restore_contacts", "wdg_energy_set",
     "wdg_kinetic_energy_noalloc", "wdg_intra_force", "wdg_facet_nodes", "wdg_set_update_phys",
     "wdg_get_update_phys", "wdg_get_scene_dt", "wdg_crit_dt_scenes", "wdg_set_eval_stress",
-    "wdg_contact_loop_stress",
+    "wdg_contact_loop_stress", "wdg_energy_scenes", "wdg_energy_scenes_set",
 ]
 
 # Scene.energy terms, in WDG_EN_* order
@@ -143,6 +143,8 @@ def lib():
     L.wdg_energy.argtypes = [V, _P(C.c_double)]
     L.wdg_energy_clear.argtypes = [V]
     L.wdg_energy_set.argtypes = [V, _P(C.c_double)]
+    L.wdg_energy_scenes.argtypes = [V, _P(C.c_double)]
+    L.wdg_energy_scenes_set.argtypes = [V, _P(C.c_double)]
     L.wdg_restore_contacts.argtypes = [V, C.c_int64, _P(C.c_int32), _P(C.c_int32), _P(C.c_double), C.c_double]
     L.wdg_set_precision.argtypes = [V, C.c_int32]
     L.wdg_run.argtypes = [V, _P(Program), C.c_int64, _P(C.c_int64), _P(C.c_double)]
@@ -344,7 +346,10 @@ class Device:
         ids, cd, data = self.contacts(real_only=True)
         ck.update(c_ids=ids, c_cellDist=cd, c_data=data)
         # Scene.trackEnergy and Scene.energy
-        ck["energy"] = np.array([self.energy()[k] for k in ENERGY_TERMS])
+        if self._scenes is not None and len(self._scenes) > 1:
+            ck["energy_scenes"] = self.energy_scenes()
+        else:
+            ck["energy"] = np.array([self.energy()[k] for k in ENERGY_TERMS])
         ck["track_energy"] = np.array(self._tracking)
         return ck
 
@@ -362,9 +367,13 @@ class Device:
         if "scenes_dt" in ck:
             d.set_scenes(np.asarray(ck["scenes_dt"]))
         d.restore_contacts(ck["c_ids"], ck["c_cellDist"], ck["c_data"], verletDist)
+        # tracking first: switching it on for the first time clears the terms
+        if "track_energy" in ck:
+            d.set_energy_tracking(bool(ck["track_energy"]))
         if "energy" in ck:
             d.energy_set(np.asarray(ck["energy"]))
-            d.set_energy_tracking(bool(ck["track_energy"]))
+        if "energy_scenes" in ck:
+            d.energy_scenes_set(np.asarray(ck["energy_scenes"]))
         return d
 
     def observables(self, lo, hi):
@@ -404,6 +413,20 @@ class Device:
         out = np.empty(7, dtype=np.float64)
         check(self.L.wdg_energy(self.h, dptr(out)))
         return dict(zip(ENERGY_TERMS, out.tolist()))
+
+    def energy_scenes(self):
+        """Scene.energy of each scene of a batched context of several scenes: an (nScenes, 7)
+        array, columns in ENERGY_TERMS order (energy() is their sum over the scenes)"""
+        ns = len(self._scenes) if self._scenes is not None else 0
+        out = np.empty((max(ns, 1), 7), dtype=np.float64)
+        check(self.L.wdg_energy_scenes(self.h, dptr(out)))
+        return out
+
+    def energy_scenes_set(self, terms):
+        """EnergyTracker.set of each scene from an (nScenes, 7) array"""
+        ns = len(self._scenes) if self._scenes is not None else 0
+        a = np.ascontiguousarray(terms, dtype=np.float64).reshape(ns, 7)
+        check(self.L.wdg_energy_scenes_set(self.h, dptr(a)))
 
     def energy_clear(self):
         check(self.L.wdg_energy_clear(self.h))

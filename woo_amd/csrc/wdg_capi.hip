@@ -540,6 +540,7 @@ int wdg_ctx_destroy(wdg_ctx* ctx) {
     for (auto& s : c.C) s.release();
     for (auto& r : c.R) r.release();
     c.act.release(); c.desc.release(); c.descDist.release(); c.energy.release(); c.enPart.release();
+    c.energyS.release(); c.pe.release(); c.pc.release(); c.enC.release();
     DBuf<double>* arrs[] = {c.pos, c.vel, c.angVel, c.force, c.torque, c.inertia, c.lastPos};
     for (auto* a : arrs) for (int k = 0; k < 3; k++) a[k].release();
     for (int k = 0; k < 4; k++) c.ori[k].release();
@@ -913,8 +914,6 @@ int wdg_set_scenes(wdg_ctx* ctx, int32_t nScenes, const double* dt) {
 int wdg_run_graph(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t* step, double* dt) {
   if (!ctx || !prog || !step || !dt || nSteps < 0) { g_err = "bad arguments"; return WDG_E_ARG; }
   return guard(ctx, [&] {
-    if (ctx->c.trackEnergy && ctx->c.nScenes > 1)
-      throw Error(WDG_E_STATE, "energy tracking needs a context of one scene");
     run_graph(ctx->c, prog, nSteps, step, dt);
   });
 }
@@ -922,8 +921,6 @@ int wdg_run_graph(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t
 int wdg_tune(wdg_ctx* ctx, const wdg_program* prog, int64_t* step, double* dt, wdg_tuning* out, int64_t* stepsTaken) {
   if (!ctx || !prog || !step || !dt) { g_err = "bad arguments"; return WDG_E_ARG; }
   return guard(ctx, [&] {
-    if (ctx->c.trackEnergy && ctx->c.nScenes > 1)
-      throw Error(WDG_E_STATE, "energy tracking needs a context of one scene");
     const int64_t n = tune(ctx->c, prog, step, dt);
     if (out) *out = ctx->c.tune;
     if (stepsTaken) *stepsTaken = n;
@@ -1009,8 +1006,30 @@ int wdg_energy_set(wdg_ctx* ctx, const double in[7]) {
   if (!ctx || !in) { g_err = "NULL argument"; return WDG_E_ARG; }
   return guard(ctx, [&] {
     Ctx& c = ctx->c;
+    if (c.nScenes > 1) throw Error(WDG_E_STATE, "a batched context of several scenes: wdg_energy_scenes_set");
     c.energy.ensure(WDG_EN_N);
     WDG_HIP(hipMemcpyAsync(c.energy.p, in, WDG_EN_N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    WDG_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int wdg_energy_scenes(wdg_ctx* ctx, double* out) {
+  if (!ctx || !out) { g_err = "NULL argument"; return WDG_E_ARG; }
+  return guard(ctx, [&] {
+    std::vector<double> v;
+    energy_scenes_read(ctx->c, v);
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+  });
+}
+
+int wdg_energy_scenes_set(wdg_ctx* ctx, const double* in) {
+  if (!ctx || !in) { g_err = "NULL argument"; return WDG_E_ARG; }
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    if (c.nScenes <= 1) throw Error(WDG_E_STATE, "not a batched context of several scenes (wdg_set_scenes)");
+    c.energyS.ensure((size_t)c.nScenes * WDG_EN_N);
+    WDG_HIP(hipMemcpyAsync(c.energyS.p, in, (size_t)c.nScenes * WDG_EN_N * sizeof(double), hipMemcpyHostToDevice,
+                           c.stream));
     WDG_HIP(hipStreamSynchronize(c.stream));
   });
 }
@@ -1053,7 +1072,6 @@ int wdg_run(wdg_ctx* ctx, const wdg_program* prog, int64_t nSteps, int64_t* step
   }
   return guard(ctx, [&] {
     Ctx& c = ctx->c;
-    if (c.trackEnergy && c.nScenes > 1) throw Error(WDG_E_STATE, "energy tracking needs a context of one scene");
     std::vector<double> crs, nextScene;
     for (int64_t s = 0; s < nSteps; s++) {
       double nextDt = NAN;

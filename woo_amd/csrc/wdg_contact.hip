@@ -70,6 +70,7 @@ struct ContactArgs {
   double* gtq;
   uint32_t* dflags;
   double* enPart;  // energy tracking: 4 partial sums per force workgroup
+  double* enC;     // energy tracking per scene: the 4 terms of each contact (state index), else null
   uint32_t* eidx;          // per adjacency entry: its slot when real, else NO_STATE (null: not kept)
   const uint32_t* entPos;  // per slot: positions of its two entries
 };
@@ -382,6 +383,18 @@ __device__ __forceinline__ d3 rolling_torque(const ContactArgs& a, double kn, do
   return mk3(t0, t1, t2);
 }
 
+// a law's energy term q of the contact at state index k (each term is added at most once per
+// contact, so the sums are those of adding it to en alone); per scene it is kept per contact too
+__device__ __forceinline__ void en_add(const ContactArgs& a, double* en, int64_t k, int q, double x) {
+  en[q] += x;
+  if (a.enC) a.enC[4 * k + q] = x;
+}
+__device__ __forceinline__ void en_clear(const ContactArgs& a, int64_t k) {
+  double2* e2 = reinterpret_cast<double2*>(a.enC + 4 * k);
+  e2[0] = make_double2(0., 0.);
+  e2[1] = make_double2(0., 0.);
+}
+
 template <bool PERIODIC, int LAW, bool FACETS, bool ENERGY>
 __global__ __launch_bounds__(CF_BS, FACETS ? 3 : 4) void k_cforce(ContactArgs a) {
   __shared__ wdg_material smat[MAT_LDS];
@@ -397,6 +410,7 @@ __global__ __launch_bounds__(CF_BS, FACETS ? 3 : 4) void k_cforce(ContactArgs a)
   if (a.eidx && blockIdx.x == 0 && threadIdx.x == 0) a.dflags[EIDX_FULL] = 0u;
   for (uint32_t loc = (lb % nblk) * CF_BS + threadIdx.x; loc < cnt; loc += CF_BS * nblk) {
     const int64_t k = state_index((uint32_t)g, loc);
+    if (ENERGY && a.enC) en_clear(a, k);
     // the filter's record of the slot: type bits, particles in role order, old state index
     const uint4 dk = a.desc[k];
     const uint32_t info = dk.x, pA = dk.y, pB = dk.z;
@@ -574,7 +588,7 @@ __global__ __launch_bounds__(CF_BS, FACETS ? 3 : 4) void k_cforce(ContactArgs a)
           double ratio = maxFt / FtNorm;
           if (FtNorm == 0.) ratio = 0.;
           // IdealElPl.cpp:62-73: slip dissipation, while Ft is the trial value
-          if (ENERGY) en[1] += (ph.kt != 0. ? maxFt * (FtNorm - maxFt) / ph.kt : 0.);
+          if (ENERGY) en_add(a, en, k, 1, (ph.kt != 0. ? maxFt * (FtNorm - maxFt) / ph.kt : 0.));
           Ft0 = Ft0 * ratio;
           Ft1 = Ft1 * ratio;
         }
@@ -584,7 +598,7 @@ __global__ __launch_bounds__(CF_BS, FACETS ? 3 : 4) void k_cforce(ContactArgs a)
         // IdealElPl.cpp:85-92: elastic potential, 0 when it is NaN
         double el = 0.5 * (p2(Fn) / ph.kn + (ph.kt != 0. ? (Ft0 * Ft0 + Ft1 * Ft1) / ph.kt : 0.));
         if (isnan(el)) el = 0.;
-        en[0] += el;
+        en_add(a, en, k, 0, el);
       }
       if (LAW == LAW_IEP_ROLL) tq = rolling_torque(a, ph.kn, Fn, l0 + l1, dt, lav);
     } else {
@@ -600,7 +614,7 @@ __global__ __launch_bounds__(CF_BS, FACETS ? 3 : 4) void k_cforce(ContactArgs a)
       double Fnv = eta * velN;
       if (ph.alpha == 0. && a.noAttraction && Fne + Fnv > 0) Fnv = -Fne;
       Fn = Fne + Fna + Fnv;
-      if (ENERGY) en[2] += Fnv * velN * dt;  // Hertz.cpp:181
+      if (ENERGY) en_add(a, en, k, 2, Fnv * velN * dt);  // Hertz.cpp:181
       const double kt = ph.kt0 * sqrt(uN < 0 ? -uN : 0);
       const double dk = dt * kt;
       Ft0 = Ft0 + dk * gvel.y;
@@ -609,18 +623,18 @@ __global__ __launch_bounds__(CF_BS, FACETS ? 3 : 4) void k_cforce(ContactArgs a)
       if (Ft0 * Ft0 + Ft1 * Ft1 > p2(maxFt)) {
         const double FtNorm = sqrt(Ft0 * Ft0 + Ft1 * Ft1);
         const double ratio = maxFt / FtNorm;
-        if (ENERGY) en[1] += (.5 * (FtNorm - maxFt) + maxFt) * (FtNorm - maxFt) / kt;  // Hertz.cpp:193
+        if (ENERGY) en_add(a, en, k, 1, (.5 * (FtNorm - maxFt) + maxFt) * (FtNorm - maxFt) / kt);  // Hertz.cpp:193
         Ft0 = Ft0 * ratio;
         Ft1 = Ft1 * ratio;
       } else {
         Ft0 = Ft0 + eta * gvel.y;
         Ft1 = Ft1 + eta * gvel.z;
-        if (ENERGY) en[3] += eta * (gvel.y * gvel.y + gvel.z * gvel.z) * dt;  // Hertz.cpp:199
+        if (ENERGY) en_add(a, en, k, 3, eta * (gvel.y * gvel.y + gvel.z * gvel.z) * dt);  // Hertz.cpp:199
       }
       if (ENERGY && uN < 0) {
         // Hertz.cpp:203-206: kn0 (2/5) (-uN)^(5/2) with pown's square-and-multiply
         const double q = s * s;
-        en[0] += kn0 * (2 / 5.) * ((q * q) * s) + 0.5 * (Ft0 * Ft0 + Ft1 * Ft1) / kt;
+        en_add(a, en, k, 0, kn0 * (2 / 5.) * ((q * q) * s) + 0.5 * (Ft0 * Ft0 + Ft1 * Ft1) / kt);
       }
     }
     a.N0[k] = T0.x; a.N1[k] = T0.y; a.N2[k] = T0.z;
@@ -643,7 +657,7 @@ __global__ __launch_bounds__(CF_BS, FACETS ? 3 : 4) void k_cforce(ContactArgs a)
       gt[0] = TG.x; gt[1] = TG.y; gt[2] = TG.z;
     }
   }
-  if (ENERGY) block_sum_store<4, CF_BS>(en, a.enPart + (int64_t)4 * blockIdx.x);
+  if (ENERGY && !a.enC) block_sum_store<4, CF_BS>(en, a.enPart + (int64_t)4 * blockIdx.x);
 }
 
 // ================================================================== FP32 contact form
@@ -766,6 +780,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
   if (a.eidx && blockIdx.x == 0 && threadIdx.x == 0) a.dflags[EIDX_FULL] = 0u;
   for (uint32_t loc = (lb % nblk) * CF_BS + threadIdx.x; loc < cnt; loc += CF_BS * nblk) {
     const int64_t k = state_index((uint32_t)g, loc);
+    if (ENERGY && a.enC) en_clear(a, k);
     // the filter's record of the slot: type bits, particles in role order, old state index
     const uint4 dk = a.desc[k];
     const uint32_t info = dk.x, pA = dk.y, pB = dk.z;
@@ -912,7 +927,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
           const float FtNorm = sqrtf(Ft0 * Ft0 + Ft1 * Ft1);
           float ratio = maxFt / FtNorm;
           if (FtNorm == 0.f) ratio = 0.f;
-          if (ENERGY) en[1] += (kt != 0.f ? (double)maxFt * ((double)FtNorm - maxFt) / kt : 0.);
+          if (ENERGY) en_add(a, en, k, 1, (kt != 0.f ? (double)maxFt * ((double)FtNorm - maxFt) / kt : 0.));
           Ft0 = Ft0 * ratio;
           Ft1 = Ft1 * ratio;
         }
@@ -922,7 +937,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
         const double f0 = Fn, f1 = Ft0, f2 = Ft1;
         double el = 0.5 * (f0 * f0 / kn + (kt != 0.f ? (f1 * f1 + f2 * f2) / kt : 0.));
         if (isnan(el)) el = 0.;
-        en[0] += el;
+        en_add(a, en, k, 0, el);
       }
     } else {
       PhysVals ph;
@@ -937,7 +952,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
       float Fnv = eta * velN;
       if (ph.alpha == 0. && a.noAttraction && Fne + Fnv > 0.f) Fnv = -Fne;
       Fn = Fne + Fna + Fnv;
-      if (ENERGY) en[2] += (double)Fnv * velN * dt;
+      if (ENERGY) en_add(a, en, k, 2, (double)Fnv * velN * dt);
       const float kt = (float)ph.kt0 * sqrtf(uN < 0.f ? -uN : 0.f);
       const float dk = dt * kt;
       Ft0 = Ft0 + dk * gvel.y;
@@ -948,18 +963,18 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
         const float ratio = maxFt / FtNorm;
         if (ENERGY) {
           const double d = (double)FtNorm - maxFt;
-          en[1] += (.5 * d + maxFt) * d / kt;
+          en_add(a, en, k, 1, (.5 * d + maxFt) * d / kt);
         }
         Ft0 = Ft0 * ratio;
         Ft1 = Ft1 * ratio;
       } else {
         Ft0 = Ft0 + eta * gvel.y;
         Ft1 = Ft1 + eta * gvel.z;
-        if (ENERGY) en[3] += (double)eta * ((double)gvel.y * gvel.y + (double)gvel.z * gvel.z) * dt;
+        if (ENERGY) en_add(a, en, k, 3, (double)eta * ((double)gvel.y * gvel.y + (double)gvel.z * gvel.z) * dt);
       }
       if (ENERGY && uN < 0.f) {
         const double sd = sqrt(-(double)uN), q = sd * sd, f1 = Ft0, f2 = Ft1;
-        en[0] += ph.K * sqrt(ph.R) * (2 / 5.) * ((q * q) * sd) + 0.5 * (f1 * f1 + f2 * f2) / kt;
+        en_add(a, en, k, 0, ph.K * sqrt(ph.R) * (2 / 5.) * ((q * q) * sd) + 0.5 * (f1 * f1 + f2 * f2) / kt);
       }
     }
     N0[k] = T0.x; N1[k] = T0.y; N2[k] = T0.z;
@@ -972,7 +987,7 @@ __global__ __launch_bounds__(CF_BS, 4) void k_cforce32(ContactArgs a) {
     g4[1] = make_float4(cA.y, cA.z, cB.x, cB.y);
     g4[2] = make_float4(cB.z, 0.f, 0.f, 0.f);
   }
-  if (ENERGY) block_sum_store<4, CF_BS>(en, a.enPart + (int64_t)4 * blockIdx.x);
+  if (ENERGY && !a.enC) block_sum_store<4, CF_BS>(en, a.enPart + (int64_t)4 * blockIdx.x);
 }
 
 static ContactArgs contact_args(Ctx& c, int from, int to) {
@@ -1283,7 +1298,12 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
   }
   WDG_LAUNCH_CHECK();
   const dim3 gforce((unsigned)(ng * a.cfBlocks));
-  if (c.trackEnergy) {
+  if (energy_per_scene(c)) {
+    // per contact, summed per scene once the new real set is current (below)
+    energy_scenes_prepare(c, C.n);
+    a.enC = c.enC.p;
+    launch_cforce<true>(c, a, gforce, c.precision == 32, hertz, roll, per, fac);
+  } else if (c.trackEnergy) {
     c.enPart.ensure((size_t)4 * gforce.x);
     a.enPart = c.enPart.p;
     launch_cforce<true>(c, a, gforce, c.precision == 32, hertz, roll, per, fac);
@@ -1312,6 +1332,7 @@ void contact_loop(Ctx& c, const wdg_contact_params* p) {
   c.eidxClean = false;
   c.rcur = to;
   c.rollActive = roll;
+  if (energy_per_scene(c)) energy_scenes_contacts(c);
   c.forcesPending = p->applyForces != 0;
   // ContactLoop::run's epilogue (ContactLoop.cpp:265-266): 'once' updates one step only
   if (c.updatePhys == 2 && !c.graphCapture) c.updatePhys = 0;

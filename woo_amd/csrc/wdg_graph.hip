@@ -105,6 +105,7 @@ static void capture(Ctx& c, const wdg_program* prog, int K, int64_t step, double
   if (c.field.periodic) c.descDist.ensure(ngroups(nc) * GROUP);
   if (c.trackEnergy)
     c.enPart.ensure(std::max<size_t>((size_t)4 * ngroups(nc) * c.tune.cfBlocks, (size_t)3 * nblocks(c.n, 256)));
+  if (energy_per_scene(c)) energy_scenes_prepare(c, nc);
   const HostState h = save(c);
   hipGraph_t graph = nullptr;
   c.graphCapture = true;

@@ -207,6 +207,7 @@ struct LeapArgs {
   const uint32_t* scene;
   const uint32_t* gstop;  // graph replay: a set stop word skips the kernel
   double* enPart;         // energy tracking: kinetic, grav, nonviscDamp partials per workgroup
+  double* enP;            // energy tracking per scene: the three terms of each particle, else null
 };
 
 // std::max's comparison, so that a NaN propagates as in the reference's isinf(maxCoeff)
@@ -351,7 +352,14 @@ __global__ __launch_bounds__(256) void k_leapfrog(LeapArgs a, GatherArgs g) {
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) *a.dirtyNext = 0u;
   if (__ballot(dirty) != 0ull && (threadIdx.x & 63) == 0) atomicOr(a.dirty, 1u);
-  if (ENERGY) block_sum_store<3, 256>(en, a.enPart + (int64_t)3 * blockIdx.x);
+  if (ENERGY && a.enP) {
+    if (i < a.n) {
+      double* e = a.enP + 3 * i;
+      e[0] = en[0]; e[1] = en[1]; e[2] = en[2];
+    }
+  } else if (ENERGY) {
+    block_sum_store<3, 256>(en, a.enPart + (int64_t)3 * blockIdx.x);
+  }
 }
 
 __global__ void k_force_reset(int64_t n, const uint32_t* __restrict__ flags, const double* __restrict__ mass,
@@ -563,7 +571,15 @@ void leapfrog(Ctx& c, const wdg_leapfrog_params* p) {
   dim3 grid(nblocks(c.n, 256)), block(256);
   // Leapfrog.cpp:132: a periodic scene is not tracked at step 0 (gradV is zero here)
   const bool track = c.trackEnergy && (!c.field.periodic || c.stepNow > 0);
-  if (track) {
+  if (track && energy_per_scene(c)) {
+    energy_scenes_prepare(c, c.con().n);
+    a.enP = c.pe.p;
+    auto* kern = c.precision == 32 ? k_leapfrog<false, true, true>
+                                   : (c.field.periodic ? k_leapfrog<true, false, true> : k_leapfrog<false, false, true>);
+    hipLaunchKernelGGL(kern, grid, block, 0, c.stream, a, g);
+    WDG_LAUNCH_CHECK();
+    energy_scenes_particles(c);
+  } else if (track) {
     c.enPart.ensure((size_t)3 * grid.x);
     a.enPart = c.enPart.p;
     auto* kern = c.precision == 32 ? k_leapfrog<false, true, true>
@@ -1189,6 +1205,21 @@ __global__ void k_energy_reset(double* energy, int clearAll) {
   if (t < WDG_EN_N && (clearAll || t == WDG_EN_KINETIC || t == WDG_EN_ELAST)) energy[t] = 0.;
 }
 
+// the same for every scene's terms (energy[WDG_EN_N * s + term])
+__global__ void k_energy_reset_scenes(double* energy, int64_t ns, int clearAll) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns * WDG_EN_N) return;
+  const int t = (int)(i % WDG_EN_N);
+  if (clearAll || t == WDG_EN_KINETIC || t == WDG_EN_ELAST) energy[i] = 0.;
+}
+
+static void energy_scenes_reset(Ctx& c, int clearAll) {
+  const int64_t m = (int64_t)c.nScenes * WDG_EN_N;
+  hipLaunchKernelGGL(k_energy_reset_scenes, dim3(nblocks(m, 256)), dim3(256), 0, c.stream, c.energyS.p,
+                     (int64_t)c.nScenes, clearAll);
+  WDG_LAUNCH_CHECK();
+}
+
 void energy_enable(Ctx& c, bool on) {
   if (on && !c.energy.p) {
     c.energy.ensure(WDG_EN_N);
@@ -1201,6 +1232,10 @@ void energy_clear(Ctx& c) {
   if (!c.energy.p) return;
   hipLaunchKernelGGL(k_energy_reset, dim3(1), dim3(64), 0, c.stream, c.energy.p, 1);
   WDG_LAUNCH_CHECK();
+  if (c.nScenes > 1) {
+    c.energyS.ensure((size_t)c.nScenes * WDG_EN_N);
+    energy_scenes_reset(c, 1);
+  }
 }
 
 // one term to zero, skipped by a stopped graph replay (a step of a contact loop without slots:
@@ -1210,13 +1245,32 @@ __global__ void k_energy_zero(double* energy, int term, const uint32_t* gstop) {
   if (threadIdx.x == 0) energy[term] = 0.;
 }
 
+// the term of every scene
+__global__ void k_energy_zero_scenes(double* energy, int64_t ns, int term, const uint32_t* gstop) {
+  if (gstop && *gstop) return;
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < ns) energy[WDG_EN_N * s + term] = 0.;
+}
+
 void energy_zero(Ctx& c, int term) {
+  if (energy_per_scene(c)) {
+    energy_scenes_prepare(c, c.con().n);
+    hipLaunchKernelGGL(k_energy_zero_scenes, dim3(nblocks(c.nScenes, 256)), dim3(256), 0, c.stream, c.energyS.p,
+                       (int64_t)c.nScenes, term, c.graphCapture ? (const uint32_t*)c.gstate.p : nullptr);
+    WDG_LAUNCH_CHECK();
+    return;
+  }
   hipLaunchKernelGGL(k_energy_zero, dim3(1), dim3(64), 0, c.stream, c.energy.p, term,
                      c.graphCapture ? (const uint32_t*)c.gstate.p : nullptr);
   WDG_LAUNCH_CHECK();
 }
 
 void energy_step_begin(Ctx& c) {
+  if (energy_per_scene(c)) {
+    energy_scenes_prepare(c, c.con().n);
+    energy_scenes_reset(c, 0);
+    return;
+  }
   hipLaunchKernelGGL(k_energy_reset, dim3(1), dim3(64), 0, c.stream, c.energy.p, 0);
   WDG_LAUNCH_CHECK();
 }
@@ -1230,11 +1284,164 @@ void energy_add(Ctx& c, int64_t nBlocks, int nTerms, const int* target, int setM
 }
 
 void energy_read(Ctx& c, double out[7]) {
+  if (c.nScenes > 1 && c.energyS.p) {
+    // a batched context: the sum over its scenes, in scene order
+    std::vector<double> s;
+    energy_scenes_read(c, s);
+    for (int q = 0; q < WDG_EN_N; q++) out[q] = 0.;
+    for (int64_t k = 0; k < c.nScenes; k++)
+      for (int q = 0; q < WDG_EN_N; q++) out[q] += s[(size_t)k * WDG_EN_N + q];
+    return;
+  }
   if (!c.energy.p) {
     for (int q = 0; q < WDG_EN_N; q++) out[q] = 0.;
     return;
   }
   WDG_HIP(hipMemcpyAsync(out, c.energy.p, WDG_EN_N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  WDG_HIP(hipStreamSynchronize(c.stream));
+}
+
+// ------------------------------------------------------------------ energy of batched scenes
+// EnergyTracker per scene (Scene.energy of each scene of the batch). The leapfrog writes its
+// three terms per particle (pe), the contact loop its four per contact (enC); a contact's terms
+// are counted at its pB, a sphere (boundary bodies are always pA), by walking the sphere's
+// adjacency (pc); then one workgroup per scene sums its particles' terms in storage order with
+// a fixed tree. Deterministic; the order of the additions differs from that of the scene
+// stepped alone, so the sums agree to rounding.
+
+__global__ void k_scene_keys(int64_t n, const uint32_t* __restrict__ scene, uint32_t* key, uint32_t* idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    key[i] = scene[i];
+    idx[i] = (uint32_t)i;
+  }
+}
+
+// off[s] = the first position of the sorted keys at or above s (off[ns] = n)
+__global__ void k_scene_off(int64_t n, int64_t ns, const uint32_t* __restrict__ key, uint32_t* off) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > ns) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)key[mid] < s) lo = mid + 1;
+    else hi = mid;
+  }
+  off[s] = (uint32_t)lo;
+}
+
+void energy_scenes_prepare(Ctx& c, int64_t nc) {
+  const int64_t n = c.n, ns = c.nScenes;
+  c.energyS.ensure((size_t)ns * WDG_EN_N);
+  c.pe.ensure((size_t)3 * std::max<int64_t>(n, 1));
+  c.pc.ensure((size_t)4 * std::max<int64_t>(n, 1));
+  c.enC.ensure((size_t)4 * std::max<int64_t>(ngroups(nc) * GROUP, 1));
+  c.sceneOff.ensure((size_t)ns + 1);
+  if (c.sceneListGen == c.gen) return;
+  if (c.graphCapture) throw Error(WDG_E_STATE, "the scenes' particle lists are stale while capturing");
+  c.sceneKey.ensure((size_t)std::max<int64_t>(n, 1));
+  c.sceneKeyTmp.ensure((size_t)std::max<int64_t>(n, 1));
+  c.sceneList.ensure((size_t)std::max<int64_t>(n, 1));
+  c.sceneListTmp.ensure((size_t)std::max<int64_t>(n, 1));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_scene_keys, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, n, (const uint32_t*)c.scene.p,
+                       c.sceneKey.p, c.sceneList.p);
+    WDG_LAUNCH_CHECK();
+  }
+  int bits = 1;
+  while (bits < 32 && ((int64_t)1 << bits) < ns) bits++;
+  // stable: within a scene the particles stay in storage order
+  radix_sort_pairs<uint32_t>(c, c.sceneKey.p, c.sceneList.p, c.sceneKeyTmp.p, c.sceneListTmp.p, n, bits);
+  hipLaunchKernelGGL(k_scene_off, dim3(nblocks(ns + 1, 256)), dim3(256), 0, c.stream, n, ns,
+                     (const uint32_t*)c.sceneKey.p, c.sceneOff.p);
+  WDG_LAUNCH_CHECK();
+  c.sceneListGen = c.gen;
+}
+
+// pc[4 i + q] = the terms of the real contacts whose pB is particle i, in adjacency order
+__global__ __launch_bounds__(256) void k_energy_at_pb(int64_t n, const uint32_t* __restrict__ adj,
+                                                      const uint32_t* __restrict__ adjOff, RealView rv,
+                                                      const double* __restrict__ enC, double* pc,
+                                                      const uint32_t* gstop) {
+  if (gstop && *gstop) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double e[4] = {0., 0., 0., 0.};
+  for (uint32_t k = adjOff[i]; k < adjOff[i + 1]; k++) {
+    const uint32_t v = adj[k];
+    if (v & ADJ_PA) continue;
+    const uint32_t slot = v & ~ADJ_PA;
+    if (!rv_real(rv, slot)) continue;
+    const double* t = enC + 4 * (int64_t)rv_index(rv, slot);
+#pragma unroll
+    for (int q = 0; q < 4; q++) e[q] += t[q];
+  }
+  double* o = pc + 4 * i;
+#pragma unroll
+  for (int q = 0; q < 4; q++) o[q] = e[q];
+}
+
+// one workgroup per scene: nTerms terms of stride `stride` per particle, thread t summing the
+// scene's particles t, t+256, ... in list order, then a fixed tree; bit q of setMask: the sum
+// replaces energy[WDG_EN_N s + target q] instead of being added to it
+__global__ __launch_bounds__(256) void k_energy_scenes(const double* __restrict__ part, int stride, int nTerms,
+                                                       int4 target, int setMask, const uint32_t* __restrict__ list,
+                                                       const uint32_t* __restrict__ off, double* energy,
+                                                       const uint32_t* gstop) {
+  if (gstop && *gstop) return;
+  __shared__ double sh[4][256];
+  const int64_t s = blockIdx.x;
+  const int t = threadIdx.x;
+  double acc[4] = {0., 0., 0., 0.};
+  for (uint32_t k = off[s] + t; k < off[s + 1]; k += 256) {
+    const double* p = part + (int64_t)stride * list[k];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (q < nTerms) acc[q] += p[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++) sh[q][t] = acc[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w)
+#pragma unroll
+      for (int q = 0; q < 4; q++) sh[q][t] += sh[q][t + w];
+    __syncthreads();
+  }
+  if (t < nTerms) {
+    const int e = t == 0 ? target.x : t == 1 ? target.y : t == 2 ? target.z : target.w;
+    double* d = energy + (int64_t)WDG_EN_N * s + e;
+    *d = ((setMask >> t) & 1) ? sh[t][0] : *d + sh[t][0];
+  }
+}
+
+void energy_scenes_contacts(Ctx& c) {
+  Contacts& C = c.con();
+  const uint32_t* gstop = c.graphCapture ? (const uint32_t*)c.gstate.p : nullptr;
+  hipLaunchKernelGGL(k_energy_at_pb, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, c.n, (const uint32_t*)C.adj.p,
+                     (const uint32_t*)C.adjOff.p, c.rview(), (const double*)c.enC.p, c.pc.p, gstop);
+  WDG_LAUNCH_CHECK();
+  // EnergyTracker: elast is resettable (this step's sum replaces it), the dissipations accumulate
+  const int4 tgt = make_int4(WDG_EN_ELAST, WDG_EN_PLAST, WDG_EN_VISCN, WDG_EN_VISCT);
+  hipLaunchKernelGGL(k_energy_scenes, dim3((unsigned)c.nScenes), dim3(256), 0, c.stream, (const double*)c.pc.p, 4, 4,
+                     tgt, 1, (const uint32_t*)c.sceneList.p, (const uint32_t*)c.sceneOff.p, c.energyS.p, gstop);
+  WDG_LAUNCH_CHECK();
+}
+
+void energy_scenes_particles(Ctx& c) {
+  const uint32_t* gstop = c.graphCapture ? (const uint32_t*)c.gstate.p : nullptr;
+  // kinetic is resettable; grav and nonviscDamp accumulate
+  const int4 tgt = make_int4(WDG_EN_KINETIC, WDG_EN_GRAV, WDG_EN_NONVISCDAMP, 0);
+  hipLaunchKernelGGL(k_energy_scenes, dim3((unsigned)c.nScenes), dim3(256), 0, c.stream, (const double*)c.pe.p, 3, 3,
+                     tgt, 1, (const uint32_t*)c.sceneList.p, (const uint32_t*)c.sceneOff.p, c.energyS.p, gstop);
+  WDG_LAUNCH_CHECK();
+}
+
+void energy_scenes_read(Ctx& c, std::vector<double>& out) {
+  if (c.nScenes <= 1) throw Error(WDG_E_STATE, "not a batched context of several scenes (wdg_set_scenes)");
+  out.assign((size_t)c.nScenes * WDG_EN_N, 0.);
+  if (!c.energyS.p) return;
+  WDG_HIP(hipMemcpyAsync(out.data(), c.energyS.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   WDG_HIP(hipStreamSynchronize(c.stream));
 }
 

@@ -250,6 +250,12 @@ struct Ctx {
   bool evalStress = false, clStressOn = false;
   // a single dense cell grid orders the spheres by the Morton code of their cell
   bool cellMorton = true;
+  // energy tracking of batched scenes: the terms per particle (pe: kinetic, grav, nonviscDamp
+  // of the last leapfrog; pc: elast, plast, viscN, viscT of the last contact loop, each contact
+  // counted at its pB), per contact (enC), and the particles of each scene in storage order
+  DBuf<double> pe, pc, enC, energyS;  // energyS: WDG_EN_N terms per scene
+  DBuf<uint32_t> sceneList, sceneListTmp, sceneKey, sceneKeyTmp, sceneOff;
+  int64_t sceneListGen = -1;
   DBuf<double> clStress, clPart;  // counts the changes of the epochs' numbering
   int updatePhys = 0;  // 0 never, 1 always, 2 once (then never)
 
@@ -474,6 +480,16 @@ void energy_zero(Ctx& c, int term);
 // setMask: stores it instead)
 void energy_add(Ctx& c, int64_t nBlocks, int nTerms, const int* target, int setMask);
 void energy_read(Ctx& c, double out[7]);
+// energy tracking per scene (a batched context of several scenes)
+inline bool energy_per_scene(const Ctx& c) { return c.trackEnergy && c.nScenes > 1; }
+// sizes the per-particle and per-contact buffers for nc slots and lists each scene's particles
+// in storage order (again after a change of order); no allocation when called again unchanged
+void energy_scenes_prepare(Ctx& c, int64_t nc);
+// after the contact loop: each contact's four terms at its pB, then summed per scene
+void energy_scenes_contacts(Ctx& c);
+// after the leapfrog: its three terms per particle summed per scene
+void energy_scenes_particles(Ctx& c);
+void energy_scenes_read(Ctx& c, std::vector<double>& out);
 
 // the sum over a workgroup of BS threads of NT values per thread, stored by thread 0..NT-1 at
 // out[0..NT-1]; a fixed order (butterfly in the wavefront, then wavefronts in order), so the
