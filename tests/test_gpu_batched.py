@@ -24,8 +24,10 @@ def scenes_c4_like(k=5, seed=61):
     return out
 
 
-def batch_run(scenes, steps, graph=False, tuning=None):
+def batch_run(scenes, steps, graph=False, tuning=None, energy=False):
     d, arr = build_batch(scenes)
+    if energy:
+        d.set_energy_tracking(True)
     if tuning:
         d.set_tuning(**tuning)
     sc = scenes[0]
@@ -138,12 +140,15 @@ def test_c4_batch_against_oracle_per_scene(skin):
     replayed, through several refreshes; each scene against the oracle running it alone:
     real-contact sets equal, positions within 1e-9 r (Scene::doOneStep per scene,
     Scene.cpp:366-467). With per-scene skins (skinSteps 16) the candidate pairs differ from
-    the oracle's, the real contacts and the trajectories do not."""
+    the oracle's, the real contacts and the trajectories do not. Energy is tracked: each
+    scene's Scene.energy terms within 1e-9 of its largest term."""
     from tests.scenes import build_oracle
+    from woo_amd._native import ENERGY_TERMS
 
     scenes = scenes_c4_sample()
     steps = 240
-    d, _ = batch_run(scenes, steps, graph=True, tuning=dict(cfBlocks=2, graphSteps=8, skinSteps=skin))
+    d, _ = batch_run(scenes, steps, graph=True, tuning=dict(cfBlocks=2, graphSteps=8, skinSteps=skin), energy=True)
+    E = d.energy_scenes()
     st = d.stats()
     print("batch: particles", st["nParticles"], "real", st["nReal"], "refreshes", st["nRefresh"], flush=True)
     assert st["nRefresh"] >= 3, "the window must hold at least two refreshes after the first"
@@ -151,8 +156,12 @@ def test_c4_batch_against_oracle_per_scene(skin):
     ids = d.contacts(real_only=True, data=False)[0]
     total_real = 0
     for i, (sc, sl) in enumerate(zip(scenes, slices(scenes))):
-        o = build_oracle(sc)
+        o = build_oracle(sc, trackEnergy=1)
         o.run(ENG, steps)
+        eo = o.energy()
+        ref = np.array([eo.get(k, 0.) for k in ENERGY_TERMS])
+        np.testing.assert_allclose(E[i], ref, rtol=0, atol=1e-9 * float(np.max(np.abs(ref))),
+                                   err_msg=f"scene {i}: energy")
         r = sc["radius"][0]
         err = float(np.max(np.abs(P[sl] - o.get("pos")))) / r
         mine = ids[(ids[:, 0] >= sl.start) & (ids[:, 0] < sl.stop)] - sl.start
