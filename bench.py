@@ -603,6 +603,21 @@ def batched_form(args):
     value = ntot * steps / el
     total = taken + steps
     pos, vel = dev.get("pos"), dev.get("vel")
+    # the cost of energy tracking per scene (Scene.trackEnergy of every scene), 200 steps each,
+    # after the timed window (the sampled scenes below compare the state at its end)
+    energy = {}
+    st = step
+    for on in (False, True):
+        dev.set_energy_tracking(on)
+        st, _ = dev.run(ENGINES, 8, st, scenes[0]["dt"], graph=bool(args.batch_graph), **kw)
+        dev.sync()
+        t0 = time.perf_counter()
+        st, _ = dev.run(ENGINES, 200, st, scenes[0]["dt"], graph=bool(args.batch_graph), **kw)
+        dev.sync()
+        energy["ms_per_step_" + ("tracking" if on else "plain")] = (time.perf_counter() - t0) / 200 * 1e3
+    energy["overhead"] = energy["ms_per_step_tracking"] / energy["ms_per_step_plain"] - 1.
+    es = dev.energy_scenes()
+    energy["kinetic_per_scene_J"] = dict(min=float(es[:, 0].min()), max=float(es[:, 0].max()))
     dev.close()
     # serial sample: scenes spread over the sizes, each alone for the same steps
     order = np.argsort([sc["n"] for sc in scenes])
@@ -632,7 +647,7 @@ def batched_form(args):
                     value=serial, unit="particle-steps/s", scenes=len(sample), particles=n_ser,
                     note="each sampled scene alone on the plain engine loop (wdg_run), the same steps "
                          "(tuning steps included)"),
-                speedup_vs_serial=value / serial, sampled_scenes_identical=same,
+                speedup_vs_serial=value / serial, sampled_scenes_identical=same, energy_tracking=energy,
                 per_scene=dict(
                     definition="bed solid fraction: spheres whose centres lie below the 90th percentile of the "
                                "centre heights, over the box cross-section times that height plus r; settled: "

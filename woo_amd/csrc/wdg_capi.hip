@@ -540,7 +540,7 @@ int wdg_ctx_destroy(wdg_ctx* ctx) {
     for (auto& s : c.C) s.release();
     for (auto& r : c.R) r.release();
     c.act.release(); c.desc.release(); c.descDist.release(); c.energy.release(); c.enPart.release();
-    c.energyS.release(); c.pe.release(); c.pc.release(); c.enC.release();
+    c.energyS.release(); c.pe.release(); c.pc.release(); c.enC.release(); c.enSPart.release();
     DBuf<double>* arrs[] = {c.pos, c.vel, c.angVel, c.force, c.torque, c.inertia, c.lastPos};
     for (auto* a : arrs) for (int k = 0; k < 3; k++) a[k].release();
     for (int k = 0; k < 4; k++) c.ori[k].release();

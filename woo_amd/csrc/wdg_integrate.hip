@@ -1305,9 +1305,11 @@ void energy_read(Ctx& c, double out[7]) {
 // EnergyTracker per scene (Scene.energy of each scene of the batch). The leapfrog writes its
 // three terms per particle (pe), the contact loop its four per contact (enC); a contact's terms
 // are counted at its pB, a sphere (boundary bodies are always pA), by walking the sphere's
-// adjacency (pc); then one workgroup per scene sums its particles' terms in storage order with
-// a fixed tree. Deterministic; the order of the additions differs from that of the scene
+// adjacency (pc); then the workgroups of a scene sum equal parts of its particles' terms in
+// storage order with a fixed tree, and the parts are added in order. Deterministic; the order of the additions differs from that of the scene
 // stepped alone, so the sums agree to rounding.
+
+constexpr int ENS_PARTS_MAX = 64;
 
 __global__ void k_scene_keys(int64_t n, const uint32_t* __restrict__ scene, uint32_t* key, uint32_t* idx) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1337,6 +1339,7 @@ void energy_scenes_prepare(Ctx& c, int64_t nc) {
   c.pc.ensure((size_t)4 * std::max<int64_t>(n, 1));
   c.enC.ensure((size_t)4 * std::max<int64_t>(ngroups(nc) * GROUP, 1));
   c.sceneOff.ensure((size_t)ns + 1);
+  c.enSPart.ensure((size_t)4 * ns * ENS_PARTS_MAX);
   if (c.sceneListGen == c.gen) return;
   if (c.graphCapture) throw Error(WDG_E_STATE, "the scenes' particle lists are stale while capturing");
   c.sceneKey.ensure((size_t)std::max<int64_t>(n, 1));
@@ -1358,8 +1361,9 @@ void energy_scenes_prepare(Ctx& c, int64_t nc) {
   c.sceneListGen = c.gen;
 }
 
-// pc[4 i + q] = the terms of the real contacts whose pB is particle i, in adjacency order
-__global__ __launch_bounds__(256) void k_energy_at_pb(int64_t n, const uint32_t* __restrict__ adj,
+// pc[4 i + q] = the terms of the real contacts whose pB is particle i, in adjacency order (zero
+// for the boundary bodies, storage [0, nW): they are never pB, and their long lists are not walked)
+__global__ __launch_bounds__(256) void k_energy_at_pb(int64_t n, int64_t nW, const uint32_t* __restrict__ adj,
                                                       const uint32_t* __restrict__ adjOff, RealView rv,
                                                       const double* __restrict__ enC, double* pc,
                                                       const uint32_t* gstop) {
@@ -1367,7 +1371,8 @@ __global__ __launch_bounds__(256) void k_energy_at_pb(int64_t n, const uint32_t*
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double e[4] = {0., 0., 0., 0.};
-  for (uint32_t k = adjOff[i]; k < adjOff[i + 1]; k++) {
+  const uint32_t k1 = i < nW ? adjOff[i] : adjOff[i + 1];
+  for (uint32_t k = adjOff[i]; k < k1; k++) {
     const uint32_t v = adj[k];
     if (v & ADJ_PA) continue;
     const uint32_t slot = v & ~ADJ_PA;
@@ -1381,19 +1386,24 @@ __global__ __launch_bounds__(256) void k_energy_at_pb(int64_t n, const uint32_t*
   for (int q = 0; q < 4; q++) o[q] = e[q];
 }
 
-// one workgroup per scene: nTerms terms of stride `stride` per particle, thread t summing the
-// scene's particles t, t+256, ... in list order, then a fixed tree; bit q of setMask: the sum
-// replaces energy[WDG_EN_N s + target q] instead of being added to it
+// `parts` workgroups per scene, each over an equal contiguous part of the scene's list: nTerms
+// terms of stride `stride` per particle, thread t summing the part's particles t, t+256, ... in
+// list order, then a fixed tree, into sp[4 (s parts + part) + q]
 __global__ __launch_bounds__(256) void k_energy_scenes(const double* __restrict__ part, int stride, int nTerms,
-                                                       int4 target, int setMask, const uint32_t* __restrict__ list,
-                                                       const uint32_t* __restrict__ off, double* energy,
+                                                       int parts, const uint32_t* __restrict__ list,
+                                                       const uint32_t* __restrict__ off, double* sp,
                                                        const uint32_t* gstop) {
   if (gstop && *gstop) return;
   __shared__ double sh[4][256];
-  const int64_t s = blockIdx.x;
+  const int64_t s = blockIdx.x / parts;
+  const int pi = (int)(blockIdx.x % parts);
   const int t = threadIdx.x;
+  const uint32_t o0 = off[s], o1 = off[s + 1];
+  const uint32_t chunk = (o1 - o0 + parts - 1) / parts;
+  const uint32_t a0 = o0 + (uint32_t)pi * chunk < o1 ? o0 + (uint32_t)pi * chunk : o1;
+  const uint32_t a1 = a0 + chunk < o1 ? a0 + chunk : o1;
   double acc[4] = {0., 0., 0., 0.};
-  for (uint32_t k = off[s] + t; k < off[s + 1]; k += 256) {
+  for (uint32_t k = a0 + t; k < a1; k += 256) {
     const double* p = part + (int64_t)stride * list[k];
 #pragma unroll
     for (int q = 0; q < 4; q++)
@@ -1408,33 +1418,51 @@ __global__ __launch_bounds__(256) void k_energy_scenes(const double* __restrict_
       for (int q = 0; q < 4; q++) sh[q][t] += sh[q][t + w];
     __syncthreads();
   }
-  if (t < nTerms) {
-    const int e = t == 0 ? target.x : t == 1 ? target.y : t == 2 ? target.z : target.w;
+  if (t < nTerms) sp[4 * blockIdx.x + t] = sh[t][0];
+}
+
+// per scene and term: the parts' sums in order; bit q of setMask: the sum replaces
+// energy[WDG_EN_N s + target q] instead of being added to it
+__global__ void k_energy_scenes_sum(const double* __restrict__ sp, int64_t ns, int parts, int nTerms, int4 target,
+                                    int setMask, double* energy, const uint32_t* gstop) {
+  if (gstop && *gstop) return;
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  for (int q = 0; q < nTerms; q++) {
+    double v = 0.;
+    for (int p = 0; p < parts; p++) v += sp[4 * (s * parts + p) + q];
+    const int e = q == 0 ? target.x : q == 1 ? target.y : q == 2 ? target.z : target.w;
     double* d = energy + (int64_t)WDG_EN_N * s + e;
-    *d = ((setMask >> t) & 1) ? sh[t][0] : *d + sh[t][0];
+    *d = ((setMask >> q) & 1) ? v : *d + v;
   }
+}
+
+static void energy_scenes_sum(Ctx& c, const double* part, int stride, int nTerms, int4 tgt) {
+  const uint32_t* gstop = c.graphCapture ? (const uint32_t*)c.gstate.p : nullptr;
+  const int64_t ns = c.nScenes;
+  // about 2048 workgroups over the scenes, so that a batch of few scenes still fills the chip
+  const int parts = (int)std::min<int64_t>(ENS_PARTS_MAX, std::max<int64_t>(1, (2048 + ns - 1) / ns));
+  hipLaunchKernelGGL(k_energy_scenes, dim3((unsigned)(ns * parts)), dim3(256), 0, c.stream, part, stride, nTerms,
+                     parts, (const uint32_t*)c.sceneList.p, (const uint32_t*)c.sceneOff.p, c.enSPart.p, gstop);
+  WDG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_energy_scenes_sum, dim3(nblocks(ns, 256)), dim3(256), 0, c.stream, (const double*)c.enSPart.p,
+                     ns, parts, nTerms, tgt, 1, c.energyS.p, gstop);
+  WDG_LAUNCH_CHECK();
 }
 
 void energy_scenes_contacts(Ctx& c) {
   Contacts& C = c.con();
   const uint32_t* gstop = c.graphCapture ? (const uint32_t*)c.gstate.p : nullptr;
-  hipLaunchKernelGGL(k_energy_at_pb, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, c.n, (const uint32_t*)C.adj.p,
+  hipLaunchKernelGGL(k_energy_at_pb, dim3(nblocks(c.n, 256)), dim3(256), 0, c.stream, c.n, c.nW, (const uint32_t*)C.adj.p,
                      (const uint32_t*)C.adjOff.p, c.rview(), (const double*)c.enC.p, c.pc.p, gstop);
   WDG_LAUNCH_CHECK();
   // EnergyTracker: elast is resettable (this step's sum replaces it), the dissipations accumulate
-  const int4 tgt = make_int4(WDG_EN_ELAST, WDG_EN_PLAST, WDG_EN_VISCN, WDG_EN_VISCT);
-  hipLaunchKernelGGL(k_energy_scenes, dim3((unsigned)c.nScenes), dim3(256), 0, c.stream, (const double*)c.pc.p, 4, 4,
-                     tgt, 1, (const uint32_t*)c.sceneList.p, (const uint32_t*)c.sceneOff.p, c.energyS.p, gstop);
-  WDG_LAUNCH_CHECK();
+  energy_scenes_sum(c, c.pc.p, 4, 4, make_int4(WDG_EN_ELAST, WDG_EN_PLAST, WDG_EN_VISCN, WDG_EN_VISCT));
 }
 
 void energy_scenes_particles(Ctx& c) {
-  const uint32_t* gstop = c.graphCapture ? (const uint32_t*)c.gstate.p : nullptr;
   // kinetic is resettable; grav and nonviscDamp accumulate
-  const int4 tgt = make_int4(WDG_EN_KINETIC, WDG_EN_GRAV, WDG_EN_NONVISCDAMP, 0);
-  hipLaunchKernelGGL(k_energy_scenes, dim3((unsigned)c.nScenes), dim3(256), 0, c.stream, (const double*)c.pe.p, 3, 3,
-                     tgt, 1, (const uint32_t*)c.sceneList.p, (const uint32_t*)c.sceneOff.p, c.energyS.p, gstop);
-  WDG_LAUNCH_CHECK();
+  energy_scenes_sum(c, c.pe.p, 3, 3, make_int4(WDG_EN_KINETIC, WDG_EN_GRAV, WDG_EN_NONVISCDAMP, 0));
 }
 
 void energy_scenes_read(Ctx& c, std::vector<double>& out) {
