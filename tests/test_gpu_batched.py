@@ -261,3 +261,23 @@ def test_batched_energy_per_scene_equals_serial_runs(graph):
     np.testing.assert_array_equal(d2.energy_scenes(), E)
     d2.close()
     d.close()
+
+
+def test_batched_energy_set_clear_and_refusals():
+    # the per-scene terms start at zero, can be set (a checkpoint) and cleared; the one-scene
+    # setter is refused in a batched context
+    scenes = scenes_c4_like(k=3, seed=68)
+    d, _ = build_batch(scenes)
+    d.set_energy_tracking(True)
+    assert np.all(d.energy_scenes() == 0)
+    with pytest.raises(Exception):
+        d.energy_set([0.] * 7)
+    d.energy_scenes_set(np.ones((3, 7)))
+    np.testing.assert_array_equal(d.energy_scenes(), np.ones((3, 7)))
+    sc = scenes[0]
+    d.run(ENG, 5, 0, sc["dt"], contact=contact_kw(sc), leapfrog=dict(damping=sc["damping"]), dynDtPeriod=0)
+    E = d.energy_scenes()
+    assert np.all(E[:, 4] >= 1.)  # plast accumulates on the value set
+    d.energy_clear()
+    assert np.all(d.energy_scenes() == 0)
+    d.close()

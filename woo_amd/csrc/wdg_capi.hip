@@ -541,6 +541,7 @@ int wdg_ctx_destroy(wdg_ctx* ctx) {
     for (auto& r : c.R) r.release();
     c.act.release(); c.desc.release(); c.descDist.release(); c.energy.release(); c.enPart.release();
     c.energyS.release(); c.pe.release(); c.pc.release(); c.enC.release(); c.enSPart.release();
+    c.energySScenes = -1;
     DBuf<double>* arrs[] = {c.pos, c.vel, c.angVel, c.force, c.torque, c.inertia, c.lastPos};
     for (auto* a : arrs) for (int k = 0; k < 3; k++) a[k].release();
     for (int k = 0; k < 4; k++) c.ori[k].release();
@@ -1031,6 +1032,7 @@ int wdg_energy_scenes_set(wdg_ctx* ctx, const double* in) {
     WDG_HIP(hipMemcpyAsync(c.energyS.p, in, (size_t)c.nScenes * WDG_EN_N * sizeof(double), hipMemcpyHostToDevice,
                            c.stream));
     WDG_HIP(hipStreamSynchronize(c.stream));
+    c.energySScenes = c.nScenes;
   });
 }
 

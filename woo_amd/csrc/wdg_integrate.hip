@@ -1235,6 +1235,7 @@ void energy_clear(Ctx& c) {
   if (c.nScenes > 1) {
     c.energyS.ensure((size_t)c.nScenes * WDG_EN_N);
     energy_scenes_reset(c, 1);
+    c.energySScenes = c.nScenes;
   }
 }
 
@@ -1335,6 +1336,12 @@ __global__ void k_scene_off(int64_t n, int64_t ns, const uint32_t* __restrict__ 
 void energy_scenes_prepare(Ctx& c, int64_t nc) {
   const int64_t n = c.n, ns = c.nScenes;
   c.energyS.ensure((size_t)ns * WDG_EN_N);
+  if (c.energySScenes != ns) {
+    // another set of scenes (wdg_set_scenes): their terms start at zero
+    if (c.graphCapture) throw Error(WDG_E_STATE, "the scenes' energy terms are not set up while capturing");
+    energy_scenes_reset(c, 1);
+    c.energySScenes = ns;
+  }
   c.pe.ensure((size_t)3 * std::max<int64_t>(n, 1));
   c.pc.ensure((size_t)4 * std::max<int64_t>(n, 1));
   c.enC.ensure((size_t)4 * std::max<int64_t>(ngroups(nc) * GROUP, 1));
@@ -1468,7 +1475,8 @@ void energy_scenes_particles(Ctx& c) {
 void energy_scenes_read(Ctx& c, std::vector<double>& out) {
   if (c.nScenes <= 1) throw Error(WDG_E_STATE, "not a batched context of several scenes (wdg_set_scenes)");
   out.assign((size_t)c.nScenes * WDG_EN_N, 0.);
-  if (!c.energyS.p) return;
+  // none tracked yet for this set of scenes: zero
+  if (!c.energyS.p || c.energySScenes != c.nScenes) return;
   WDG_HIP(hipMemcpyAsync(out.data(), c.energyS.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   WDG_HIP(hipStreamSynchronize(c.stream));
 }

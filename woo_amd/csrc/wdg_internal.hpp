@@ -255,6 +255,7 @@ struct Ctx {
   // counted at its pB), per contact (enC), and the particles of each scene in storage order
   DBuf<double> pe, pc, enC, energyS;  // energyS: WDG_EN_N terms per scene
   DBuf<double> enSPart;                // partial sums of the parts of each scene
+  int64_t energySScenes = -1;          // the scene count energyS holds terms for
   DBuf<uint32_t> sceneList, sceneListTmp, sceneKey, sceneKeyTmp, sceneOff;
   int64_t sceneListGen = -1;
   DBuf<double> clStress, clPart;  // counts the changes of the epochs' numbering
