@@ -32,6 +32,8 @@
  *   wdg_set_energy_tracking / wdg_energy / wdg_energy_clear / wdg_energy_set
  *                      Scene.trackEnergy, Scene.energy (EnergyTracker)
  *                                                        src/core/Scene.hpp:204,223, src/core/EnergyTracker.hpp:30-61
+ *   wdg_energy_scenes / wdg_energy_scenes_set
+ *                      Scene.energy of each scene of a batch (one Scene per scene in the reference)
  *   wdg_run            Scene::doOneStep (engine loop)    src/core/Scene.cpp:366-467
  */
 #ifndef WOO_AMD_H
