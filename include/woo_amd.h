@@ -308,7 +308,8 @@ int wdg_sync(wdg_ctx* ctx);
  * Scene.dt (dt[nScenes]); the contact loop and leapfrog then take dt per particle from its
  * scene (the dt passed with the engine parameters is ignored). nScenes = 0 switches back.
  * The graph replay runs [Collider, ContactLoop, Leapfrog] in this mode; wdg_run also runs
- * DynDt, each scene's DynDt::run (DynDt.cpp:90-113) on its own critical timestep and dt. */
+ * DynDt, each scene's DynDt::run (DynDt.cpp:90-113) on its own critical timestep and dt.
+ * With energy tracking on, each scene keeps its own Scene.energy (wdg_energy_scenes). */
 int wdg_set_scenes(wdg_ctx* ctx, int32_t nScenes, const double* dt);
 /* the current dt of each scene (out[nScenes]; DynDt changes them) */
 int wdg_get_scene_dt(wdg_ctx* ctx, double* out);
