@@ -281,3 +281,22 @@ def test_batched_energy_set_clear_and_refusals():
     d.energy_clear()
     assert np.all(d.energy_scenes() == 0)
     d.close()
+
+
+def test_batched_energy_with_the_autotuner():
+    # the autotuner's trial replays capture the per-scene energy kernels as well; the terms stay
+    # finite, every scene moves, and the total is the sum over the scenes
+    from woo_amd._native import ENERGY_TERMS
+    scenes = scenes_c4_like(k=3, seed=69)
+    d, _ = build_batch(scenes)
+    d.set_energy_tracking(True)
+    sc = scenes[0]
+    kw = dict(contact=contact_kw(sc), leapfrog=dict(damping=sc["damping"]), dynDtPeriod=0)
+    tun, step, taken = d.tune(ENG, 0, sc["dt"], **kw)
+    d.run(ENG, 24, step, sc["dt"], graph=True, **kw)
+    E = d.energy_scenes()
+    assert np.all(np.isfinite(E)) and np.all(E[:, 0] > 0)
+    tot = d.energy()
+    for t, name in enumerate(ENERGY_TERMS):
+        assert tot[name] == pytest.approx(float(np.sum(E[:, t])), rel=1e-14, abs=1e-300)
+    d.close()
